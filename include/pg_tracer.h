@@ -1,0 +1,166 @@
+/*
+ * pg_tracer.h — C ABI of the MI355X path-tracing hot path.
+ *
+ * The reference (IsaGeriler/PathGuidingDissertation) has no FFI: its hot path is a
+ * set of C++ classes called directly (Scene, BVHNode, BSDF, Light, RayTracer). Each
+ * entry point below names the reference interface it replaces (file:line, relative
+ * to PathGuidingDissertation/). Host C++ classes mirroring those interfaces
+ * (pathguidingdissertation_amd/include/pgd/*.hpp) flatten into these calls.
+ *
+ * Conventions: every function returns pg_status (0 = OK, negative = error) and
+ * never throws; pg_last_error() gives the message. Caller-owned host arrays are
+ * copied in/out; the context owns device memory. One context per device and
+ * host thread; calls are ordered on the context's HIP stream and return after
+ * the work completes. Results are deterministic for a given (seed, params).
+ */
+#ifndef PG_TRACER_H
+#define PG_TRACER_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t pg_status;
+#define PG_OK 0
+#define PG_ERR_INVALID (-1)   /* bad argument or size                          */
+#define PG_ERR_NOMEM (-2)     /* device or host allocation failed              */
+#define PG_ERR_DEVICE (-3)    /* HIP runtime error / no device                 */
+#define PG_ERR_OVERFLOW (-4)  /* queue, stack or capacity exceeded             */
+#define PG_ERR_STATE (-5)     /* call out of order (e.g. trace before build)   */
+
+typedef struct pg_ctx pg_ctx;
+
+/* ------------------------------------------------------------------ data formats */
+
+/* Material (Materials.h:121-897; parameters as SceneLoader.h:104-181 reads them).
+ * albedo is a constant colour (procedural scenes carry no textures).            */
+enum { PG_BSDF_DIFFUSE = 0, PG_BSDF_MIRROR = 1, PG_BSDF_CONDUCTOR = 2, PG_BSDF_GLASS = 3,
+       PG_BSDF_DIELECTRIC = 4 };
+typedef struct {
+    float type;          /* PG_BSDF_* stored as float so the record is 16 floats   */
+    float albedo[3];
+    float emission[3];   /* > 0 luminance makes every triangle an AreaLight        */
+    float eta[3], k[3];  /* conductor                                              */
+    float int_ior, ext_ior;
+    float roughness;     /* alpha = 1.62142 * sqrt(roughness) (Materials.h:249)    */
+} pg_material;
+
+/* Pinhole camera as the scene loader builds it (SceneLoader.h:226-244). */
+typedef struct {
+    float from[3], to[3], up[3];
+    float fov_deg;
+    float width, height;
+} pg_camera_desc;
+
+/* Ray for the trace entry points: origin, direction (Geometry.h:22-39). */
+typedef struct { float o[3]; float d[3]; } pg_ray;
+
+/* Closest hit (Geometry.h:247-253): ID = UINT32_MAX, t = FLT_MAX on a miss. */
+typedef struct { uint32_t id; float t, alpha, beta, gamma; } pg_hit;
+
+/* Path-vertex record (Renderer.h:42-47), 48 bytes in HBM. */
+typedef struct { float pos[3]; float normal[3]; float wi[3]; float li[3]; } pg_path_vertex;
+
+enum { PG_INTEGRATOR_PATH = 0, PG_INTEGRATOR_RECORD = 1, PG_INTEGRATOR_GUIDED_QT = 2,
+       PG_INTEGRATOR_GUIDED_MLP = 3 };
+
+/* Render parameters: the reference's hard-coded constants become fields whose
+ * defaults (pg_render_params_default) equal the reference.                    */
+typedef struct {
+    uint32_t seed;
+    uint32_t max_depth;     /* Renderer.h:568 -> 8                              */
+    uint32_t rr_depth;      /* Renderer.h:561 -> Russian roulette when depth > 3 */
+    int32_t integrator;     /* PG_INTEGRATOR_*                                  */
+    uint32_t record;        /* RECORD integrator: keep PathVertex records       */
+    float guide_prob;       /* guided integrators: P(sample guide), 0.5         */
+    uint32_t knn_k;         /* guided QT: neighbours per lookup (16)            */
+    uint32_t reserved[9];
+} pg_render_params;
+
+typedef struct {
+    uint64_t closest_rays, shadow_rays, paths, records;
+    double kernel_ms_extend, kernel_ms_shade, kernel_ms_shadow, kernel_ms_total;
+    uint64_t extend_launches, shadow_launches;
+    uint64_t nodes_visited, tris_tested;       /* filled when counting is enabled */
+    uint64_t shadow_nodes_visited, shadow_tris_tested;
+} pg_stats;
+
+/* ------------------------------------------------------------------ context */
+pg_status pg_create(pg_ctx** ctx, int device);          /* RayTracer::init Renderer.h:251-263 */
+pg_status pg_destroy(pg_ctx* ctx);
+const char* pg_last_error(const pg_ctx* ctx);            /* NULL ctx: last global error        */
+pg_status pg_device_info(pg_ctx* ctx, char* name, size_t name_len, int32_t* cu_count,
+                         double* peak_hbm_gbs);
+
+/* ------------------------------------------------------------------ scene */
+/* Scene::init + Scene::build (Scene.h:84-137): copies triangles (v0,v1,v2 positions
+ * and per-vertex shading normals, 9 floats each), per-triangle material index,
+ * the material table and an optional lat-long RGB32F environment map (y-up,
+ * Lights.h:128-228; NULL => black BackgroundColour, Lights.h:75-126). The host
+ * builds the binned-SAH BVH in the reference's topology (Geometry.h:293-446) and
+ * uploads 32-byte nodes, leaf-ordered triangles, shading attributes, the light
+ * list and the environment alias table.                                        */
+pg_status pg_scene_upload(pg_ctx* ctx, const float* positions, const float* normals,
+                          const uint32_t* material_index, uint32_t n_tris,
+                          const pg_material* materials, uint32_t n_materials,
+                          const float* env_rgb, int32_t env_width, int32_t env_height,
+                          const pg_camera_desc* camera);
+/* BVH statistics: node count, maximum leaf depth, build milliseconds. */
+pg_status pg_bvh_info(pg_ctx* ctx, uint32_t* n_nodes, uint32_t* max_depth, double* build_ms);
+/* Export the topology as built (8 floats per node: min.xyz, leftFirst bits,
+ * max.xyz, used bits) and the triangle order (BVHNode::triangleIndexes).        */
+pg_status pg_bvh_export(pg_ctx* ctx, float* nodes, int32_t* tri_index);
+
+/* ------------------------------------------------------------------ queries (tests) */
+/* BVHNode::traverse (Geometry.h:448-512) for a batch of rays. */
+pg_status pg_trace_closest(pg_ctx* ctx, const pg_ray* rays, uint32_t n, pg_hit* hits);
+/* BVHNode::traverseVisible (Geometry.h:514-567): visible[i] = 1 when no hit in
+ * (1e-4, max_t[i]).                                                            */
+pg_status pg_trace_shadow(pg_ctx* ctx, const pg_ray* rays, const float* max_t, uint32_t n,
+                          uint8_t* visible);
+/* Camera::generateRay (Scene.h:45-57) for film positions (x, y). */
+pg_status pg_generate_rays(pg_ctx* ctx, const float* pxy, uint32_t n, pg_ray* rays);
+/* Counter RNG values for (seed, sample, pixel, dim) — parity aid. */
+pg_status pg_rng_values(pg_ctx* ctx, uint32_t seed, uint32_t sample, const uint32_t* pixel,
+                        const uint32_t* dim, uint32_t n, float* out);
+
+/* ------------------------------------------------------------------ rendering */
+pg_render_params pg_render_params_default(void);
+/* RayTracer::clear (Renderer.h:265-267): zero the film and SPP count. */
+pg_status pg_film_clear(pg_ctx* ctx);
+/* RayTracer::render (Renderer.h:648-747) n_spp times: each pass traces one sample
+ * per pixel through the wavefront pipeline (ray-gen, extend, shade, shadow) and
+ * splats it with the reference's box filter (Imaging.h:186-217).               */
+pg_status pg_render_spp(pg_ctx* ctx, const pg_render_params* params, uint32_t n_spp);
+/* Film::film / Film::SPP (Imaging.h:178-272): accumulated RGB sums (W*H*3) and SPP. */
+pg_status pg_film_read(pg_ctx* ctx, float* rgb_sum, uint32_t* spp);
+pg_status pg_stats_read(pg_ctx* ctx, pg_stats* stats);
+pg_status pg_stats_reset(pg_ctx* ctx);
+/* Instrumented traversal: count node visits / triangle tests per ray (slower). */
+pg_status pg_set_counting(pg_ctx* ctx, int32_t enable);
+
+/* ------------------------------------------------------------------ host-side helpers */
+/* Procedural scenes of the benchmark configurations (no reference counterpart:
+ * the reference loads scene.json + .gem files, SceneLoader.h:222-269).
+ * config 1: Cornell box; 2: open interior (~1.09M tris, env map);
+ * 3: interior closed except a 2x2 window; 5: config-2 family for batched views. */
+typedef struct pg_host_scene pg_host_scene;
+pg_status pg_scene_generate(int32_t config, uint32_t seed, pg_host_scene** out);
+pg_status pg_scene_sizes(const pg_host_scene* s, uint32_t* n_tris, uint32_t* n_materials,
+                         int32_t* env_width, int32_t* env_height);
+pg_status pg_scene_copy(const pg_host_scene* s, float* positions, float* normals,
+                        uint32_t* material_index, pg_material* materials, float* env_rgb,
+                        pg_camera_desc* camera);
+pg_status pg_scene_free(pg_host_scene* s);
+
+/* Host BVH build alone (Geometry.h:419-446) — for topology parity checks. */
+pg_status pg_bvh_build_host(const float* positions, uint32_t n_tris, uint32_t* n_nodes,
+                            float* nodes_out /* nullable: 8*(2n-1) floats */,
+                            int32_t* tri_index_out /* nullable: n */);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PG_TRACER_H */

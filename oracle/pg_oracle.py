@@ -1,0 +1,241 @@
+"""ctypes wrapper of the CPU oracle (oracle/pg_oracle.cpp) and, where it was built,
+of the reference golden-vector driver (oracle/_ref/libpgref.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the CPU
+baseline leg of bench.py.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+ORACLE_SO = HERE / "build" / "libpgoracle.so"
+REF_SO = HERE / "_ref" / "libpgref.so"
+
+_F = C.POINTER(C.c_float)
+_D = C.POINTER(C.c_double)
+_U32 = C.POINTER(C.c_uint32)
+_I32 = C.POINTER(C.c_int32)
+_U8 = C.POINTER(C.c_uint8)
+
+
+def fp(a):
+    return None if a is None else a.ctypes.data_as(_F)
+
+
+def up(a):
+    return None if a is None else a.ctypes.data_as(_U32)
+
+
+class RenderParams(C.Structure):
+    _fields_ = [("spp", C.c_uint32), ("spp_offset", C.c_uint32), ("seed", C.c_uint32), ("max_depth", C.c_uint32),
+                ("rr_depth", C.c_uint32), ("integrator", C.c_int32), ("sampler", C.c_int32),
+                ("accum_f64", C.c_int32), ("n_threads", C.c_int32), ("record", C.c_int32),
+                ("env_mode", C.c_int32)]
+
+
+class RenderStats(C.Structure):
+    _fields_ = [("closest_rays", C.c_uint64), ("shadow_rays", C.c_uint64), ("paths", C.c_uint64),
+                ("seconds", C.c_double), ("n_records", C.c_uint64)]
+
+
+def build(force: bool = False) -> None:
+    """Compile the oracle (and the reference driver when the reference is present)."""
+    cmd = ["make", "-s", "-C", str(HERE), "all"]
+    if force:
+        cmd.insert(2, "-B")
+    subprocess.run(cmd, check=True)
+    if Path("/root/reference/PathGuidingDissertation/Geometry.h").exists():
+        cmd = ["make", "-s", "-C", str(HERE), "ref"]
+        if force:
+            cmd.insert(2, "-B")
+        subprocess.run(cmd, check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not ORACLE_SO.exists():
+            build()
+        L = C.CDLL(str(ORACLE_SO))
+        L.orc_scene_create.restype = C.c_void_p
+        L.orc_scene_create.argtypes = [_F, _F, _U32, C.c_uint32, _F, C.c_uint32, _F, C.c_int32, C.c_int32, _F]
+        L.orc_scene_destroy.argtypes = [C.c_void_p]
+        L.orc_bvh_node_count.restype = C.c_uint32
+        L.orc_bvh_node_count.argtypes = [C.c_void_p]
+        L.orc_bvh_max_depth.restype = C.c_int32
+        L.orc_bvh_max_depth.argtypes = [C.c_void_p]
+        L.orc_bvh_export.argtypes = [C.c_void_p, _F, _I32]
+        L.orc_camera_export.argtypes = [C.c_void_p, _F, _F, _F]
+        L.orc_generate_rays.argtypes = [C.c_void_p, _F, C.c_uint32, _F]
+        L.orc_trace_closest.argtypes = [C.c_void_p, _F, C.c_uint32, _U32, _F, _F, _F, _F, _U32, _U32, C.c_int32]
+        L.orc_trace_shadow.argtypes = [C.c_void_p, _F, _F, C.c_uint32, _U8, _U32, _U32, C.c_int32]
+        L.orc_trace_brute.argtypes = [C.c_void_p, _F, C.c_uint32, _U32, _F]
+        L.orc_render.restype = C.c_int
+        L.orc_render.argtypes = [C.c_void_p, C.POINTER(RenderParams), _D, C.POINTER(RenderStats)]
+        L.orc_records_count.restype = C.c_uint64
+        L.orc_records_count.argtypes = [C.c_void_p]
+        L.orc_records_export.argtypes = [C.c_void_p, _F]
+        L.orc_rng.restype = C.c_float
+        L.orc_rng.argtypes = [C.c_uint32] * 4
+        L.orc_bsdf_sample.argtypes = [_F, _F, _F, _F, _F, _F]
+        L.orc_bsdf_eval.argtypes = [_F, _F, _F, _F, _F]
+        L.orc_bsdf_invert.argtypes = [_F, _F, _F, _F]
+        L.orc_fresnel_dielectric.restype = C.c_float
+        L.orc_fresnel_dielectric.argtypes = [C.c_float] * 3
+        L.orc_fresnel_conductor.argtypes = [C.c_float, _F, _F, _F]
+        L.orc_ggx_d.restype = C.c_float
+        L.orc_ggx_d.argtypes = [_F, C.c_float]
+        L.orc_ggx_g.restype = C.c_float
+        L.orc_ggx_g.argtypes = [_F, _F, C.c_float]
+        L.orc_env_sample.argtypes = [C.c_void_p, C.c_float, C.c_float, _F, _F, _F]
+        L.orc_env_eval.argtypes = [C.c_void_p, _F, _F, _F]
+        L.orc_env_marginal_integral.restype = C.c_float
+        L.orc_env_marginal_integral.argtypes = [C.c_void_p]
+        L.orc_scene_bounds.argtypes = [C.c_void_p, _F, _F]
+        L.orc_point_bvh_stats.restype = C.c_int32
+        L.orc_point_bvh_stats.argtypes = [_F, C.c_uint32, _I32]
+        L.orc_knn_brute.argtypes = [_F, C.c_uint32, _F, C.c_uint32, C.c_uint32, _U32, _F, C.c_int32]
+        _lib = L
+    return _lib
+
+
+def ref_available() -> bool:
+    return REF_SO.exists()
+
+
+_ref = None
+
+
+def ref():
+    """The reference's own classes behind C entry points (built only where the
+    reference sources exist)."""
+    global _ref
+    if _ref is None:
+        R = C.CDLL(str(REF_SO))
+        R.ref_scene_create.restype = C.c_void_p
+        R.ref_scene_create.argtypes = [_F, _F, C.c_uint32]
+        R.ref_scene_destroy.argtypes = [C.c_void_p]
+        R.ref_bvh_node_count.restype = C.c_uint32
+        R.ref_bvh_node_count.argtypes = [C.c_void_p]
+        R.ref_bvh_export.argtypes = [C.c_void_p, _F, _I32]
+        R.ref_trace_closest.argtypes = [C.c_void_p, _F, C.c_uint32, _U32, _F, _F, _F, _F]
+        R.ref_trace_shadow.argtypes = [C.c_void_p, _F, _F, C.c_uint32, _U8]
+        R.ref_visible.argtypes = [C.c_void_p, _F, _F, C.c_uint32, _U8]
+        R.ref_camera.argtypes = [_F, _F, C.c_uint32, _F, _F, _F]
+        R.ref_bsdf.argtypes = [_F, _F, _F, _F, _F]
+        R.ref_fresnel_dielectric.restype = C.c_float
+        R.ref_fresnel_dielectric.argtypes = [C.c_float] * 3
+        R.ref_fresnel_conductor.argtypes = [C.c_float, _F, _F, _F]
+        R.ref_ggx_d.restype = C.c_float
+        R.ref_ggx_d.argtypes = [_F, C.c_float]
+        R.ref_ggx_g.restype = C.c_float
+        R.ref_ggx_g.argtypes = [_F, _F, C.c_float]
+        R.ref_env_create.restype = C.c_void_p
+        R.ref_env_create.argtypes = [_F, C.c_int, C.c_int]
+        R.ref_env_destroy.argtypes = [C.c_void_p]
+        R.ref_env_sample.argtypes = [C.c_void_p, C.c_float, C.c_float, _F]
+        R.ref_env_eval.argtypes = [C.c_void_p, _F, _F]
+        R.ref_env_power.restype = C.c_float
+        R.ref_env_power.argtypes = [C.c_void_p]
+        _ref = R
+    return _ref
+
+
+class OracleScene:
+    """A scene in the CPU oracle (BVH built on construction)."""
+
+    def __init__(self, data):
+        L = lib()
+        self._keep = [np.ascontiguousarray(data.positions, np.float32), np.ascontiguousarray(data.normals, np.float32),
+                      np.ascontiguousarray(data.material_index, np.uint32),
+                      np.ascontiguousarray(data.materials_f32(), np.float32),
+                      None if data.env is None else np.ascontiguousarray(data.env, np.float32),
+                      data.camera_array()]
+        pos, nrm, mi, mats, env, cam = self._keep
+        self.width, self.height = data.width, data.height
+        self.n_tris = pos.shape[0]
+        self.h = L.orc_scene_create(fp(pos), fp(nrm), up(mi), pos.shape[0], fp(mats), mats.shape[0], fp(env),
+                                    0 if env is None else env.shape[1], 0 if env is None else env.shape[0], fp(cam))
+
+    def __del__(self):
+        try:
+            lib().orc_scene_destroy(self.h)
+        except Exception:
+            pass
+
+    def bvh(self):
+        L = lib()
+        n = L.orc_bvh_node_count(self.h)
+        nodes = np.empty((n, 8), np.float32)
+        idx = np.empty(self.n_tris, np.int32)
+        L.orc_bvh_export(self.h, fp(nodes), idx.ctypes.data_as(_I32))
+        return nodes, idx
+
+    def max_depth(self) -> int:
+        return lib().orc_bvh_max_depth(self.h)
+
+    def camera(self):
+        ip, cm, o = np.empty(16, np.float32), np.empty(16, np.float32), np.empty(3, np.float32)
+        lib().orc_camera_export(self.h, fp(ip), fp(cm), fp(o))
+        return ip, cm, o
+
+    def generate_rays(self, pxy):
+        pxy = np.ascontiguousarray(pxy, np.float32).reshape(-1, 2)
+        out = np.empty((pxy.shape[0], 6), np.float32)
+        lib().orc_generate_rays(self.h, fp(pxy), pxy.shape[0], fp(out))
+        return out
+
+    def trace(self, rays, threads=0, counts=False):
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 6)
+        n = rays.shape[0]
+        ids = np.empty(n, np.uint32)
+        t, a, b, g = (np.empty(n, np.float32) for _ in range(4))
+        cn = np.empty(n, np.uint32) if counts else None
+        ct = np.empty(n, np.uint32) if counts else None
+        lib().orc_trace_closest(self.h, fp(rays), n, up(ids), fp(t), fp(a), fp(b), fp(g), up(cn), up(ct), threads)
+        if counts:
+            return ids, t, a, b, g, cn, ct
+        return ids, t, a, b, g
+
+    def shadow(self, rays, max_t, threads=0):
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 6)
+        max_t = np.ascontiguousarray(max_t, np.float32)
+        vis = np.empty(rays.shape[0], np.uint8)
+        lib().orc_trace_shadow(self.h, fp(rays), fp(max_t), rays.shape[0], vis.ctypes.data_as(_U8), None, None, threads)
+        return vis
+
+    def brute(self, rays):
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 6)
+        ids = np.empty(rays.shape[0], np.uint32)
+        t = np.empty(rays.shape[0], np.float32)
+        lib().orc_trace_brute(self.h, fp(rays), rays.shape[0], up(ids), fp(t))
+        return ids, t
+
+    def render(self, spp, seed=1, spp_offset=0, max_depth=8, rr_depth=3, integrator=0, sampler=0, accum_f64=True,
+               threads=0, record=False, env_mode=1):
+        p = RenderParams(spp, spp_offset, seed, max_depth, rr_depth, integrator, sampler, 1 if accum_f64 else 0,
+                         threads, 1 if record else 0, env_mode)
+        film = np.zeros((self.height, self.width, 3), np.float64)
+        st = RenderStats()
+        lib().orc_render(self.h, C.byref(p), film.ctypes.data_as(_D), C.byref(st))
+        return film, {f: getattr(st, f) for f, _ in RenderStats._fields_}
+
+    def records(self):
+        L = lib()
+        n = L.orc_records_count(self.h)
+        out = np.empty((n, 15), np.float32)
+        if n:
+            L.orc_records_export(self.h, fp(out))
+        return out
+
+
+def rng(seed, sample, pixel, dim):
+    return lib().orc_rng(seed, sample, pixel, dim)

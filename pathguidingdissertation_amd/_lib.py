@@ -1,0 +1,116 @@
+"""ctypes binding of the C ABI declared in include/pg_tracer.h.
+
+The product path is the gfx950 library ``libpgtrace.so``; there is no CPU fallback.
+Loading fails loudly when the library has not been built.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+_PKG = Path(__file__).resolve().parent
+LIB_PATH = _PKG / "libpgtrace.so"
+
+PG_OK = 0
+PG_ERR_INVALID = -1
+PG_ERR_NOMEM = -2
+PG_ERR_DEVICE = -3
+PG_ERR_OVERFLOW = -4
+PG_ERR_STATE = -5
+
+PG_INTEGRATOR_PATH = 0
+PG_INTEGRATOR_RECORD = 1
+PG_INTEGRATOR_GUIDED_QT = 2
+PG_INTEGRATOR_GUIDED_MLP = 3
+
+
+class PgMaterial(C.Structure):
+    _fields_ = [("type", C.c_float), ("albedo", C.c_float * 3), ("emission", C.c_float * 3),
+                ("eta", C.c_float * 3), ("k", C.c_float * 3), ("int_ior", C.c_float),
+                ("ext_ior", C.c_float), ("roughness", C.c_float)]
+
+
+class PgCameraDesc(C.Structure):
+    _fields_ = [("from_", C.c_float * 3), ("to", C.c_float * 3), ("up", C.c_float * 3),
+                ("fov_deg", C.c_float), ("width", C.c_float), ("height", C.c_float)]
+
+
+class PgRenderParams(C.Structure):
+    _fields_ = [("seed", C.c_uint32), ("max_depth", C.c_uint32), ("rr_depth", C.c_uint32),
+                ("integrator", C.c_int32), ("record", C.c_uint32), ("guide_prob", C.c_float),
+                ("knn_k", C.c_uint32), ("reserved", C.c_uint32 * 9)]
+
+
+class PgStats(C.Structure):
+    _fields_ = [("closest_rays", C.c_uint64), ("shadow_rays", C.c_uint64), ("paths", C.c_uint64),
+                ("records", C.c_uint64), ("kernel_ms_extend", C.c_double), ("kernel_ms_shade", C.c_double),
+                ("kernel_ms_shadow", C.c_double), ("kernel_ms_total", C.c_double),
+                ("extend_launches", C.c_uint64), ("shadow_launches", C.c_uint64),
+                ("nodes_visited", C.c_uint64), ("tris_tested", C.c_uint64),
+                ("shadow_nodes_visited", C.c_uint64), ("shadow_tris_tested", C.c_uint64)]
+
+
+_P = C.c_void_p
+_F = C.POINTER(C.c_float)
+_U32 = C.POINTER(C.c_uint32)
+_I32 = C.POINTER(C.c_int32)
+_U8 = C.POINTER(C.c_uint8)
+
+# name -> (restype, argtypes); every symbol the header declares
+SIGNATURES = {
+    "pg_create": (C.c_int32, [C.POINTER(_P), C.c_int]),
+    "pg_destroy": (C.c_int32, [_P]),
+    "pg_last_error": (C.c_char_p, [_P]),
+    "pg_device_info": (C.c_int32, [_P, C.c_char_p, C.c_size_t, _I32, C.POINTER(C.c_double)]),
+    "pg_scene_upload": (C.c_int32, [_P, _F, _F, _U32, C.c_uint32, C.POINTER(PgMaterial), C.c_uint32, _F,
+                                    C.c_int32, C.c_int32, C.POINTER(PgCameraDesc)]),
+    "pg_bvh_info": (C.c_int32, [_P, _U32, _U32, C.POINTER(C.c_double)]),
+    "pg_bvh_export": (C.c_int32, [_P, _F, _I32]),
+    "pg_trace_closest": (C.c_int32, [_P, _F, C.c_uint32, _P]),
+    "pg_trace_shadow": (C.c_int32, [_P, _F, _F, C.c_uint32, _U8]),
+    "pg_generate_rays": (C.c_int32, [_P, _F, C.c_uint32, _F]),
+    "pg_rng_values": (C.c_int32, [_P, C.c_uint32, C.c_uint32, _U32, _U32, C.c_uint32, _F]),
+    "pg_render_params_default": (PgRenderParams, []),
+    "pg_film_clear": (C.c_int32, [_P]),
+    "pg_render_spp": (C.c_int32, [_P, C.POINTER(PgRenderParams), C.c_uint32]),
+    "pg_film_read": (C.c_int32, [_P, _F, _U32]),
+    "pg_stats_read": (C.c_int32, [_P, C.POINTER(PgStats)]),
+    "pg_stats_reset": (C.c_int32, [_P]),
+    "pg_set_counting": (C.c_int32, [_P, C.c_int32]),
+    "pg_scene_generate": (C.c_int32, [C.c_int32, C.c_uint32, C.POINTER(_P)]),
+    "pg_scene_sizes": (C.c_int32, [_P, _U32, _U32, _I32, _I32]),
+    "pg_scene_copy": (C.c_int32, [_P, _F, _F, _U32, C.POINTER(PgMaterial), _F, C.POINTER(PgCameraDesc)]),
+    "pg_scene_free": (C.c_int32, [_P]),
+    "pg_bvh_build_host": (C.c_int32, [_F, C.c_uint32, _U32, _F, _I32]),
+}
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    """Load libpgtrace.so (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise RuntimeError(f"{LIB_PATH} is missing: run __graft_entry__.build() (the HIP path has no fallback)")
+    lib = C.CDLL(str(LIB_PATH))
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+class PgError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"pg error {code}: {msg}")
+        self.code = code
+
+
+def check(code: int, ctx=None) -> None:
+    if code != PG_OK:
+        lib = load()
+        msg = lib.pg_last_error(ctx)
+        raise PgError(code, msg.decode() if msg else "")

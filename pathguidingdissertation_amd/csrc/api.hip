@@ -1,0 +1,518 @@
+// api.hip — C ABI (include/pg_tracer.h): device context, scene upload, the
+// wavefront render loop (RayTracer::render drop-in) and test queries.
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/pg_tracer.h"
+#include "host_prep.hpp"
+#include "kernels.h"
+
+using namespace pg;
+
+namespace {
+thread_local std::string g_last_error;
+
+template <class T>
+hipError_t dalloc(T** p, size_t count) {
+    *p = nullptr;
+    if (count == 0) return hipSuccess;
+    return hipMalloc((void**)p, count * sizeof(T));
+}
+template <class T>
+void dfree(T*& p) {
+    if (p) (void)hipFree((void*)p);
+    p = nullptr;
+}
+
+struct TimedLaunch {
+    int kind;  // 0 extend, 1 shade, 2 shadow, 3 other
+    hipEvent_t a, b;
+};
+}  // namespace
+
+struct pg_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    int cu_count = 0;
+    std::string name;
+    // scene
+    PreparedScene host;
+    bool has_scene = false;
+    BvhNode* d_nodes = nullptr;
+    TriIsect* d_tris = nullptr;
+    TriShade* d_shade = nullptr;
+    MaterialRec* d_mats = nullptr;
+    LightRec* d_lights = nullptr;
+    float4* d_env_tex = nullptr;
+    float* d_env_pdf = nullptr;
+    float* d_env_prob = nullptr;
+    uint32_t* d_env_alias = nullptr;
+    DevScene dev{};
+    // path state + film
+    PathState ps{};
+    uint32_t ps_capacity = 0;
+    float4* film = nullptr;
+    uint32_t film_n = 0;
+    uint32_t spp = 0;
+    unsigned long long* d_stats = nullptr;
+    uint32_t* d_counters = nullptr;
+    bool counting = false;
+    bool timing = true;
+    // host-side stats
+    double ms[4] = {0, 0, 0, 0};
+    double ms_total = 0;
+    uint64_t n_extend = 0, n_shadow = 0, paths = 0;
+    std::vector<hipEvent_t> pool;
+    size_t pool_used = 0;
+    std::vector<TimedLaunch> launches;
+
+    pg_status fail(pg_status code, const std::string& m) {
+        err = m;
+        g_last_error = m;
+        return code;
+    }
+    pg_status hip(hipError_t e, const char* what) {
+        if (e == hipSuccess) return PG_OK;
+        return fail(e == hipErrorOutOfMemory ? PG_ERR_NOMEM : PG_ERR_DEVICE,
+                    std::string(what) + ": " + hipGetErrorString(e));
+    }
+    hipEvent_t ev() {
+        if (pool_used == pool.size()) {
+            hipEvent_t e;
+            if (hipEventCreate(&e) != hipSuccess) return nullptr;
+            pool.push_back(e);
+        }
+        return pool[pool_used++];
+    }
+    void free_scene() {
+        dfree(d_nodes); dfree(d_tris); dfree(d_shade); dfree(d_mats); dfree(d_lights);
+        dfree(d_env_tex); dfree(d_env_pdf); dfree(d_env_prob); dfree(d_env_alias);
+        has_scene = false;
+    }
+    void free_paths() {
+        dfree(ps.beta); dfree(ps.radiance); dfree(ps.film_pos); dfree(ps.info);
+        for (int k = 0; k < 2; ++k) { dfree(ps.queue[k]); dfree(ps.ray_o[k]); dfree(ps.ray_d[k]); }
+        dfree(ps.hit); dfree(ps.sh_o); dfree(ps.sh_d); dfree(ps.sh_c);
+        ps_capacity = 0;
+    }
+};
+
+#define PG_CHECK_CTX(c)                                  \
+    do {                                                 \
+        if (!(c)) {                                      \
+            g_last_error = "null context";               \
+            return PG_ERR_INVALID;                       \
+        }                                                \
+        if (hipSetDevice((c)->device) != hipSuccess)     \
+            return (c)->fail(PG_ERR_DEVICE, "hipSetDevice failed"); \
+    } while (0)
+
+#define PG_TRY(c, expr, what)                          \
+    do {                                               \
+        pg_status _s = (c)->hip((expr), (what));       \
+        if (_s != PG_OK) return _s;                    \
+    } while (0)
+
+extern "C" {
+
+const char* pg_last_error(const pg_ctx* ctx) { return ctx ? ctx->err.c_str() : g_last_error.c_str(); }
+
+pg_status pg_create(pg_ctx** out, int device) {
+    if (!out) { g_last_error = "null output"; return PG_ERR_INVALID; }
+    *out = nullptr;
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n == 0) {
+        g_last_error = "no HIP device available";
+        return PG_ERR_DEVICE;
+    }
+    if (device < 0 || device >= n) { g_last_error = "device index out of range"; return PG_ERR_INVALID; }
+    pg_ctx* c = new (std::nothrow) pg_ctx();
+    if (!c) return PG_ERR_NOMEM;
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess) { delete c; g_last_error = "hipSetDevice failed"; return PG_ERR_DEVICE; }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) { delete c; g_last_error = "hipGetDeviceProperties failed"; return PG_ERR_DEVICE; }
+    c->cu_count = prop.multiProcessorCount;
+    c->name = prop.gcnArchName;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c; g_last_error = "hipStreamCreate failed"; return PG_ERR_DEVICE;
+    }
+    if (dalloc(&c->d_stats, 8) != hipSuccess || dalloc(&c->d_counters, 4) != hipSuccess) {
+        pg_destroy(c); g_last_error = "device allocation failed"; return PG_ERR_NOMEM;
+    }
+    (void)hipMemset(c->d_stats, 0, 8 * sizeof(unsigned long long));
+    (void)hipMemset(c->d_counters, 0, 4 * sizeof(uint32_t));
+    *out = c;
+    return PG_OK;
+}
+
+pg_status pg_destroy(pg_ctx* c) {
+    if (!c) return PG_OK;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    c->free_scene();
+    c->free_paths();
+    dfree(c->film);
+    dfree(c->d_stats);
+    dfree(c->d_counters);
+    for (hipEvent_t e : c->pool) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return PG_OK;
+}
+
+pg_status pg_device_info(pg_ctx* c, char* name, size_t name_len, int32_t* cu_count, double* peak_hbm_gbs) {
+    PG_CHECK_CTX(c);
+    if (name && name_len) {
+        std::strncpy(name, c->name.c_str(), name_len - 1);
+        name[name_len - 1] = 0;
+    }
+    if (cu_count) *cu_count = c->cu_count;
+    if (peak_hbm_gbs) *peak_hbm_gbs = 8000.0;  // MI355X HBM3E published peak
+    return PG_OK;
+}
+
+pg_status pg_scene_upload(pg_ctx* c, const float* positions, const float* normals, const uint32_t* material_index,
+                          uint32_t n_tris, const pg_material* materials, uint32_t n_materials, const float* env_rgb,
+                          int32_t env_width, int32_t env_height, const pg_camera_desc* camera) {
+    PG_CHECK_CTX(c);
+    if (!camera) return c->fail(PG_ERR_INVALID, "null camera");
+    PG_TRY(c, hipStreamSynchronize(c->stream), "sync");
+    c->free_scene();
+    c->host = PreparedScene();
+    std::string e = prepare_scene(positions, normals, material_index, n_tris, materials, n_materials, env_rgb,
+                                  env_width, env_height, *camera, c->host);
+    if (!e.empty()) return c->fail(PG_ERR_INVALID, e);
+    if (c->host.bvh.max_depth > 64)
+        return c->fail(PG_ERR_OVERFLOW, "BVH deeper than the 64-entry traversal stack (Geometry.h:461)");
+    PreparedScene& h = c->host;
+    PG_TRY(c, dalloc(&c->d_nodes, h.node_slots.size()), "alloc nodes");
+    PG_TRY(c, dalloc(&c->d_tris, h.tris.size()), "alloc tris");
+    PG_TRY(c, dalloc(&c->d_shade, h.shade.size()), "alloc shade");
+    PG_TRY(c, dalloc(&c->d_mats, h.mats.size()), "alloc mats");
+    PG_TRY(c, dalloc(&c->d_lights, h.lights.size() ? h.lights.size() : 1), "alloc lights");
+    PG_TRY(c, hipMemcpy(c->d_nodes, h.node_slots.data(), h.node_slots.size() * sizeof(BvhNode), hipMemcpyHostToDevice), "copy nodes");
+    PG_TRY(c, hipMemcpy(c->d_tris, h.tris.data(), h.tris.size() * sizeof(TriIsect), hipMemcpyHostToDevice), "copy tris");
+    PG_TRY(c, hipMemcpy(c->d_shade, h.shade.data(), h.shade.size() * sizeof(TriShade), hipMemcpyHostToDevice), "copy shade");
+    PG_TRY(c, hipMemcpy(c->d_mats, h.mats.data(), h.mats.size() * sizeof(MaterialRec), hipMemcpyHostToDevice), "copy mats");
+    if (!h.lights.empty())
+        PG_TRY(c, hipMemcpy(c->d_lights, h.lights.data(), h.lights.size() * sizeof(LightRec), hipMemcpyHostToDevice), "copy lights");
+    if (h.has_env) {
+        size_t n = (size_t)h.env.w * h.env.h;
+        PG_TRY(c, dalloc(&c->d_env_tex, n), "alloc env");
+        PG_TRY(c, dalloc(&c->d_env_pdf, n), "alloc env pdf");
+        PG_TRY(c, dalloc(&c->d_env_prob, n), "alloc env prob");
+        PG_TRY(c, dalloc(&c->d_env_alias, n), "alloc env alias");
+        PG_TRY(c, hipMemcpy(c->d_env_tex, h.env.texels.data(), n * 16, hipMemcpyHostToDevice), "copy env");
+        PG_TRY(c, hipMemcpy(c->d_env_pdf, h.env.pdf.data(), n * 4, hipMemcpyHostToDevice), "copy env pdf");
+        PG_TRY(c, hipMemcpy(c->d_env_prob, h.env.prob.data(), n * 4, hipMemcpyHostToDevice), "copy env prob");
+        PG_TRY(c, hipMemcpy(c->d_env_alias, h.env.alias.data(), n * 4, hipMemcpyHostToDevice), "copy env alias");
+    }
+    DevScene& d = c->dev;
+    d.nodes = c->d_nodes; d.tris = c->d_tris; d.shade = c->d_shade; d.mats = c->d_mats; d.lights = c->d_lights;
+    d.env_tex = c->d_env_tex; d.env_pdf = c->d_env_pdf; d.env_prob = c->d_env_prob; d.env_alias = c->d_env_alias;
+    d.n_lights = (int32_t)h.lights.size();
+    d.env_w = h.has_env ? h.env.w : 0;
+    d.env_h = h.has_env ? h.env.h : 0;
+    d.has_env = h.has_env ? 1 : 0;
+    d.centre[0] = h.centre[0]; d.centre[1] = h.centre[1]; d.centre[2] = h.centre[2];
+    d.radius = h.radius;
+    d.cam = h.camera;
+    c->has_scene = true;
+    // film for the camera resolution
+    uint32_t W = (uint32_t)h.camera.width, H = (uint32_t)h.camera.height;
+    uint32_t fn = W * H;
+    if (fn != c->film_n) {
+        dfree(c->film);
+        PG_TRY(c, dalloc(&c->film, fn), "alloc film");
+        c->film_n = fn;
+    }
+    PG_TRY(c, launch_clear_film(c->film, fn, c->stream), "clear film");
+    c->spp = 0;
+    PG_TRY(c, hipStreamSynchronize(c->stream), "sync");
+    return PG_OK;
+}
+
+pg_status pg_bvh_info(pg_ctx* c, uint32_t* n_nodes, uint32_t* max_depth, double* build_ms) {
+    PG_CHECK_CTX(c);
+    if (!c->has_scene) return c->fail(PG_ERR_STATE, "no scene uploaded");
+    if (n_nodes) *n_nodes = (uint32_t)c->host.bvh.nodes.size();
+    if (max_depth) *max_depth = c->host.bvh.max_depth;
+    if (build_ms) *build_ms = c->host.bvh.build_ms;
+    return PG_OK;
+}
+
+pg_status pg_bvh_export(pg_ctx* c, float* nodes, int32_t* tri_index) {
+    PG_CHECK_CTX(c);
+    if (!c->has_scene) return c->fail(PG_ERR_STATE, "no scene uploaded");
+    if (nodes) std::memcpy(nodes, c->host.bvh.nodes.data(), c->host.bvh.nodes.size() * sizeof(BvhNode));
+    if (tri_index) std::memcpy(tri_index, c->host.bvh.tri_index.data(), c->host.bvh.tri_index.size() * 4);
+    return PG_OK;
+}
+
+pg_status pg_bvh_build_host(const float* positions, uint32_t n_tris, uint32_t* n_nodes, float* nodes_out,
+                            int32_t* tri_index_out) {
+    if (!positions || n_tris == 0) { g_last_error = "empty triangle set"; return PG_ERR_INVALID; }
+    HostBvh b;
+    build_bvh(positions, n_tris, b);
+    if (n_nodes) *n_nodes = (uint32_t)b.nodes.size();
+    if (nodes_out) std::memcpy(nodes_out, b.nodes.data(), b.nodes.size() * sizeof(BvhNode));
+    if (tri_index_out) std::memcpy(tri_index_out, b.tri_index.data(), b.tri_index.size() * 4);
+    return PG_OK;
+}
+
+pg_status pg_trace_closest(pg_ctx* c, const pg_ray* rays, uint32_t n, pg_hit* hits) {
+    PG_CHECK_CTX(c);
+    if (!c->has_scene) return c->fail(PG_ERR_STATE, "no scene uploaded");
+    if (n == 0) return PG_OK;
+    if (!rays || !hits) return c->fail(PG_ERR_INVALID, "null array");
+    float *dr = nullptr, *dt = nullptr, *da = nullptr, *db = nullptr, *dg = nullptr;
+    uint32_t* did = nullptr;
+    pg_status st = PG_OK;
+    do {
+        if ((st = c->hip(dalloc(&dr, (size_t)n * 6), "alloc")) != PG_OK) break;
+        if ((st = c->hip(dalloc(&did, n), "alloc")) != PG_OK) break;
+        if ((st = c->hip(dalloc(&dt, n), "alloc")) != PG_OK) break;
+        if ((st = c->hip(dalloc(&da, n), "alloc")) != PG_OK) break;
+        if ((st = c->hip(dalloc(&db, n), "alloc")) != PG_OK) break;
+        if ((st = c->hip(dalloc(&dg, n), "alloc")) != PG_OK) break;
+        if ((st = c->hip(hipMemcpy(dr, rays, (size_t)n * 24, hipMemcpyHostToDevice), "copy rays")) != PG_OK) break;
+        if ((st = c->hip(launch_trace_closest(c->dev, dr, n, did, dt, da, db, dg, 64, c->counting, c->d_stats, c->stream),
+                         "trace_closest")) != PG_OK) break;
+        if ((st = c->hip(hipStreamSynchronize(c->stream), "sync")) != PG_OK) break;
+        std::vector<uint32_t> id(n);
+        std::vector<float> t(n), a(n), b(n), g(n);
+        hipMemcpy(id.data(), did, n * 4, hipMemcpyDeviceToHost);
+        hipMemcpy(t.data(), dt, n * 4, hipMemcpyDeviceToHost);
+        hipMemcpy(a.data(), da, n * 4, hipMemcpyDeviceToHost);
+        hipMemcpy(b.data(), db, n * 4, hipMemcpyDeviceToHost);
+        if ((st = c->hip(hipMemcpy(g.data(), dg, n * 4, hipMemcpyDeviceToHost), "copy hits")) != PG_OK) break;
+        for (uint32_t i = 0; i < n; ++i) hits[i] = pg_hit{id[i], t[i], a[i], b[i], g[i]};
+    } while (0);
+    dfree(dr); dfree(did); dfree(dt); dfree(da); dfree(db); dfree(dg);
+    return st;
+}
+
+pg_status pg_trace_shadow(pg_ctx* c, const pg_ray* rays, const float* max_t, uint32_t n, uint8_t* visible) {
+    PG_CHECK_CTX(c);
+    if (!c->has_scene) return c->fail(PG_ERR_STATE, "no scene uploaded");
+    if (n == 0) return PG_OK;
+    if (!rays || !max_t || !visible) return c->fail(PG_ERR_INVALID, "null array");
+    float *dr = nullptr, *dm = nullptr;
+    uint8_t* dv = nullptr;
+    pg_status st = PG_OK;
+    do {
+        if ((st = c->hip(dalloc(&dr, (size_t)n * 6), "alloc")) != PG_OK) break;
+        if ((st = c->hip(dalloc(&dm, n), "alloc")) != PG_OK) break;
+        if ((st = c->hip(dalloc(&dv, n), "alloc")) != PG_OK) break;
+        if ((st = c->hip(hipMemcpy(dr, rays, (size_t)n * 24, hipMemcpyHostToDevice), "copy")) != PG_OK) break;
+        if ((st = c->hip(hipMemcpy(dm, max_t, (size_t)n * 4, hipMemcpyHostToDevice), "copy")) != PG_OK) break;
+        if ((st = c->hip(launch_trace_shadow(c->dev, dr, dm, n, dv, 64, c->stream), "trace_shadow")) != PG_OK) break;
+        if ((st = c->hip(hipStreamSynchronize(c->stream), "sync")) != PG_OK) break;
+        st = c->hip(hipMemcpy(visible, dv, n, hipMemcpyDeviceToHost), "copy");
+    } while (0);
+    dfree(dr); dfree(dm); dfree(dv);
+    return st;
+}
+
+pg_status pg_generate_rays(pg_ctx* c, const float* pxy, uint32_t n, pg_ray* rays) {
+    PG_CHECK_CTX(c);
+    if (!c->has_scene) return c->fail(PG_ERR_STATE, "no scene uploaded");
+    if (n == 0) return PG_OK;
+    float *dp = nullptr, *dr = nullptr;
+    pg_status st = PG_OK;
+    do {
+        if ((st = c->hip(dalloc(&dp, (size_t)n * 2), "alloc")) != PG_OK) break;
+        if ((st = c->hip(dalloc(&dr, (size_t)n * 6), "alloc")) != PG_OK) break;
+        if ((st = c->hip(hipMemcpy(dp, pxy, (size_t)n * 8, hipMemcpyHostToDevice), "copy")) != PG_OK) break;
+        if ((st = c->hip(launch_generate_rays(c->dev, dp, n, dr, c->stream), "generate_rays")) != PG_OK) break;
+        if ((st = c->hip(hipStreamSynchronize(c->stream), "sync")) != PG_OK) break;
+        st = c->hip(hipMemcpy(rays, dr, (size_t)n * 24, hipMemcpyDeviceToHost), "copy");
+    } while (0);
+    dfree(dp); dfree(dr);
+    return st;
+}
+
+pg_status pg_rng_values(pg_ctx* c, uint32_t seed, uint32_t sample, const uint32_t* pixel, const uint32_t* dim,
+                        uint32_t n, float* out) {
+    PG_CHECK_CTX(c);
+    if (n == 0) return PG_OK;
+    uint32_t *dp = nullptr, *dd = nullptr;
+    float* dout = nullptr;
+    pg_status st = PG_OK;
+    do {
+        if ((st = c->hip(dalloc(&dp, n), "alloc")) != PG_OK) break;
+        if ((st = c->hip(dalloc(&dd, n), "alloc")) != PG_OK) break;
+        if ((st = c->hip(dalloc(&dout, n), "alloc")) != PG_OK) break;
+        hipMemcpy(dp, pixel, n * 4, hipMemcpyHostToDevice);
+        hipMemcpy(dd, dim, n * 4, hipMemcpyHostToDevice);
+        if ((st = c->hip(launch_rng(seed, sample, dp, dd, n, dout, c->stream), "rng")) != PG_OK) break;
+        if ((st = c->hip(hipStreamSynchronize(c->stream), "sync")) != PG_OK) break;
+        st = c->hip(hipMemcpy(out, dout, n * 4, hipMemcpyDeviceToHost), "copy");
+    } while (0);
+    dfree(dp); dfree(dd); dfree(dout);
+    return st;
+}
+
+pg_render_params pg_render_params_default(void) {
+    pg_render_params p;
+    std::memset(&p, 0, sizeof(p));
+    p.seed = 1;           // MTRandom default seed (Sampling.h:21)
+    p.max_depth = 8;      // Renderer.h:568
+    p.rr_depth = 3;       // Renderer.h:561
+    p.integrator = PG_INTEGRATOR_PATH;
+    p.record = 0;
+    p.guide_prob = 0.5f;
+    p.knn_k = 16;
+    return p;
+}
+
+pg_status pg_film_clear(pg_ctx* c) {
+    PG_CHECK_CTX(c);
+    if (!c->has_scene) return c->fail(PG_ERR_STATE, "no scene uploaded");
+    PG_TRY(c, launch_clear_film(c->film, c->film_n, c->stream), "clear film");
+    PG_TRY(c, hipStreamSynchronize(c->stream), "sync");
+    c->spp = 0;
+    return PG_OK;
+}
+
+static pg_status ensure_paths(pg_ctx* c, uint32_t n) {
+    if (c->ps_capacity >= n) return PG_OK;
+    c->free_paths();
+    PathState& p = c->ps;
+    PG_TRY(c, dalloc(&p.beta, n), "alloc paths");
+    PG_TRY(c, dalloc(&p.radiance, n), "alloc paths");
+    PG_TRY(c, dalloc(&p.film_pos, n), "alloc paths");
+    PG_TRY(c, dalloc(&p.info, n), "alloc paths");
+    for (int k = 0; k < 2; ++k) {
+        PG_TRY(c, dalloc(&p.queue[k], n), "alloc queues");
+        PG_TRY(c, dalloc(&p.ray_o[k], n), "alloc queues");
+        PG_TRY(c, dalloc(&p.ray_d[k], n), "alloc queues");
+    }
+    PG_TRY(c, dalloc(&p.hit, n), "alloc hits");
+    PG_TRY(c, dalloc(&p.sh_o, n), "alloc shadow");
+    PG_TRY(c, dalloc(&p.sh_d, n), "alloc shadow");
+    PG_TRY(c, dalloc(&p.sh_c, n), "alloc shadow");
+    p.counters = c->d_counters;
+    p.stats = c->d_stats;
+    p.capacity = n;
+    c->ps_capacity = n;
+    return PG_OK;
+}
+
+pg_status pg_render_spp(pg_ctx* c, const pg_render_params* params, uint32_t n_spp) {
+    PG_CHECK_CTX(c);
+    if (!c->has_scene) return c->fail(PG_ERR_STATE, "no scene uploaded");
+    pg_render_params prm = params ? *params : pg_render_params_default();
+    if (prm.integrator != PG_INTEGRATOR_PATH)
+        return c->fail(PG_ERR_INVALID, "integrator not available in this build");
+    if (prm.max_depth > 64) return c->fail(PG_ERR_INVALID, "max_depth > 64");
+    uint32_t W = (uint32_t)c->host.camera.width, H = (uint32_t)c->host.camera.height;
+    uint32_t n = W * H;
+    pg_status st = ensure_paths(c, n);
+    if (st != PG_OK) return st;
+    const uint32_t grid_trace = (uint32_t)c->cu_count * 5;
+    const uint32_t grid_shade = (uint32_t)c->cu_count * 8;
+    c->pool_used = 0;
+    c->launches.clear();
+    auto timed = [&](int kind, auto&& fn) -> hipError_t {
+        hipEvent_t a = nullptr, b = nullptr;
+        if (c->timing) {
+            a = c->ev();
+            b = c->ev();
+            if (a && b) (void)hipEventRecord(a, c->stream);
+        }
+        hipError_t e = fn();
+        if (c->timing && a && b) {
+            (void)hipEventRecord(b, c->stream);
+            c->launches.push_back(TimedLaunch{kind, a, b});
+        }
+        return e;
+    };
+    hipEvent_t t0 = c->ev(), t1 = c->ev();
+    (void)hipEventRecord(t0, c->stream);
+    for (uint32_t s = 0; s < n_spp; ++s) {
+        PassParams pp{prm.seed, c->spp, W, H, prm.max_depth, prm.rr_depth, prm.integrator};
+        PG_TRY(c, timed(3, [&] { return launch_raygen(c->dev, c->ps, pp, c->stream); }), "raygen");
+        for (uint32_t depth = 0; depth <= prm.max_depth; ++depth) {
+            int parity = (int)(depth & 1u);
+            PG_TRY(c, timed(0, [&] { return launch_extend(c->dev, c->ps, parity, 64, c->counting, grid_trace, c->stream); }), "extend");
+            PG_TRY(c, timed(1, [&] { return launch_shade(c->dev, c->ps, pp, parity, (int)depth, grid_shade, c->stream); }), "shade");
+            PG_TRY(c, timed(2, [&] { return launch_shadow(c->dev, c->ps, 64, c->counting, grid_trace, c->stream); }), "shadow");
+            c->n_extend++;
+            c->n_shadow++;
+        }
+        PG_TRY(c, timed(3, [&] { return launch_resolve(c->ps, pp, c->film, c->stream); }), "resolve");
+        c->spp++;
+        c->paths += n;
+    }
+    (void)hipEventRecord(t1, c->stream);
+    PG_TRY(c, hipStreamSynchronize(c->stream), "render");
+    float tot = 0.f;
+    (void)hipEventElapsedTime(&tot, t0, t1);
+    c->ms_total += tot;
+    for (const TimedLaunch& l : c->launches) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, l.a, l.b) == hipSuccess) c->ms[l.kind] += ms;
+    }
+    return PG_OK;
+}
+
+pg_status pg_film_read(pg_ctx* c, float* rgb_sum, uint32_t* spp) {
+    PG_CHECK_CTX(c);
+    if (!c->has_scene) return c->fail(PG_ERR_STATE, "no scene uploaded");
+    if (rgb_sum) {
+        std::vector<float4> tmp(c->film_n);
+        PG_TRY(c, hipMemcpy(tmp.data(), c->film, (size_t)c->film_n * 16, hipMemcpyDeviceToHost), "read film");
+        for (uint32_t i = 0; i < c->film_n; ++i) {
+            rgb_sum[3 * i] = tmp[i].x; rgb_sum[3 * i + 1] = tmp[i].y; rgb_sum[3 * i + 2] = tmp[i].z;
+        }
+    }
+    if (spp) *spp = c->spp;
+    return PG_OK;
+}
+
+pg_status pg_stats_read(pg_ctx* c, pg_stats* s) {
+    PG_CHECK_CTX(c);
+    if (!s) return c->fail(PG_ERR_INVALID, "null stats");
+    unsigned long long h[8];
+    PG_TRY(c, hipMemcpy(h, c->d_stats, sizeof(h), hipMemcpyDeviceToHost), "read stats");
+    std::memset(s, 0, sizeof(*s));
+    s->closest_rays = h[0];
+    s->shadow_rays = h[1];
+    s->nodes_visited = h[2];
+    s->tris_tested = h[3];
+    s->shadow_nodes_visited = h[4];
+    s->shadow_tris_tested = h[5];
+    s->paths = c->paths;
+    s->kernel_ms_extend = c->ms[0];
+    s->kernel_ms_shade = c->ms[1];
+    s->kernel_ms_shadow = c->ms[2];
+    s->kernel_ms_total = c->ms_total;
+    s->extend_launches = c->n_extend;
+    s->shadow_launches = c->n_shadow;
+    return PG_OK;
+}
+
+pg_status pg_stats_reset(pg_ctx* c) {
+    PG_CHECK_CTX(c);
+    PG_TRY(c, hipMemset(c->d_stats, 0, 8 * sizeof(unsigned long long)), "reset stats");
+    for (double& m : c->ms) m = 0;
+    c->ms_total = 0;
+    c->n_extend = c->n_shadow = c->paths = 0;
+    return PG_OK;
+}
+
+pg_status pg_set_counting(pg_ctx* c, int32_t enable) {
+    PG_CHECK_CTX(c);
+    c->counting = enable != 0;
+    return PG_OK;
+}
+
+}  // extern "C"

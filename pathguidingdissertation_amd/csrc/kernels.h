@@ -1,0 +1,77 @@
+// kernels.h — device scene view and kernel launch declarations shared by the
+// kernel translation unit and the C-ABI glue.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "../../include/pg_tracer.h"
+#include "pg_common.h"
+
+namespace pg {
+
+// Read-only device scene, passed by value to every kernel.
+struct DevScene {
+    const BvhNode* nodes;       // slot layout: node i at nodes[i + 1]
+    const TriIsect* tris;       // leaf order
+    const TriShade* shade;      // by triangle ID
+    const MaterialRec* mats;
+    const LightRec* lights;
+    const float4* env_tex;      // RGBA32F texels
+    const float* env_pdf;       // uv-space density per texel
+    const float* env_prob;      // alias acceptance
+    const uint32_t* env_alias;  // alias target
+    int32_t n_lights;
+    int32_t env_w, env_h, has_env;
+    float centre[3];
+    float radius;
+    CameraRec cam;
+};
+
+// Structure-of-arrays path state for one pass (one sample per pixel).
+struct PathState {
+    float4* beta;      // throughput rgb, previous BSDF pdf
+    float4* radiance;  // accumulated L rgb
+    float2* film_pos;  // film sample position (px, py)
+    uint4* info;       // rng key, rng dim, previous-specular flag, pixel
+    // ray queues (double-buffered), indexed by queue slot
+    uint32_t* queue[2];
+    float4* ray_o[2];   // origin.xyz, unused
+    float4* ray_d[2];   // dir.xyz, unused
+    float4* hit;        // id bits, t, beta(u), gamma(v) by queue slot
+    // shadow queue
+    float4* sh_o;       // origin.xyz, maxT
+    float4* sh_d;       // dir.xyz, path id bits
+    float4* sh_c;       // contribution rgb
+    uint32_t* counters; // [0],[1] ray queue sizes, [2] shadow queue size
+    unsigned long long* stats;  // [0] closest rays, [1] shadow rays, [2] nodes, [3] tris, [4] sh nodes, [5] sh tris
+    uint32_t capacity;
+};
+
+struct PassParams {
+    uint32_t seed, sample;
+    uint32_t width, height;
+    uint32_t max_depth, rr_depth;
+    int32_t integrator;
+};
+
+// Launch helpers (defined in kernels.hip); all enqueue on `stream`.
+hipError_t launch_trace_closest(const DevScene& s, const float* rays6, uint32_t n, uint32_t* id, float* t,
+                                float* a, float* b, float* g, int stack, bool count,
+                                unsigned long long* stats, hipStream_t stream);
+hipError_t launch_trace_shadow(const DevScene& s, const float* rays6, const float* max_t, uint32_t n,
+                               uint8_t* vis, int stack, hipStream_t stream);
+hipError_t launch_generate_rays(const DevScene& s, const float* pxy, uint32_t n, float* rays6,
+                                hipStream_t stream);
+hipError_t launch_rng(uint32_t seed, uint32_t sample, const uint32_t* pixel, const uint32_t* dim, uint32_t n,
+                      float* out, hipStream_t stream);
+
+hipError_t launch_raygen(const DevScene& s, const PathState& ps, const PassParams& p, hipStream_t stream);
+hipError_t launch_extend(const DevScene& s, const PathState& ps, int parity, int stack, bool count,
+                         uint32_t grid, hipStream_t stream);
+hipError_t launch_shade(const DevScene& s, const PathState& ps, const PassParams& p, int parity, int depth,
+                        uint32_t grid, hipStream_t stream);
+hipError_t launch_shadow(const DevScene& s, const PathState& ps, int stack, bool count, uint32_t grid,
+                         hipStream_t stream);
+hipError_t launch_resolve(const PathState& ps, const PassParams& p, float4* film, hipStream_t stream);
+hipError_t launch_clear_film(float4* film, uint32_t n, hipStream_t stream);
+
+}  // namespace pg

@@ -1,0 +1,224 @@
+"""Host-side mirror of the reference's Scene / Camera / RayTracer interfaces.
+
+The classes keep the reference's method names (Scene.h:74-176, Renderer.h:236-751)
+and flatten into the C ABI of ``libpgtrace.so``; every render runs on the GPU.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+from ._lib import PgCameraDesc, PgMaterial, PgRenderParams, PgStats, check, load
+
+BSDF_DIFFUSE, BSDF_MIRROR, BSDF_CONDUCTOR, BSDF_GLASS, BSDF_DIELECTRIC = range(5)
+
+MATERIAL_DTYPE = np.dtype([("type", "f4"), ("albedo", "f4", 3), ("emission", "f4", 3), ("eta", "f4", 3),
+                           ("k", "f4", 3), ("int_ior", "f4"), ("ext_ior", "f4"), ("roughness", "f4")])
+
+
+def _fp(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _up(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_uint32))
+
+
+@dataclass
+class SceneData:
+    """Triangle soup + materials + optional environment map + camera."""
+
+    positions: np.ndarray          # (n, 3, 3) float32: v0, v1, v2
+    normals: np.ndarray            # (n, 3, 3) float32: per-vertex shading normals
+    material_index: np.ndarray     # (n,) uint32
+    materials: np.ndarray          # (m,) MATERIAL_DTYPE
+    env: np.ndarray | None         # (h, w, 3) float32 lat-long, y-up
+    camera: dict = field(default_factory=dict)  # from, to, up, fov, width, height
+
+    @property
+    def n_tris(self) -> int:
+        return int(self.positions.shape[0])
+
+    @property
+    def width(self) -> int:
+        return int(self.camera["width"])
+
+    @property
+    def height(self) -> int:
+        return int(self.camera["height"])
+
+    def camera_array(self) -> np.ndarray:
+        c = self.camera
+        return np.array([*c["from"], *c["to"], *c["up"], c["fov"], c["width"], c["height"]], dtype=np.float32)
+
+    def materials_f32(self) -> np.ndarray:
+        return np.ascontiguousarray(self.materials).view(np.float32).reshape(-1, 16)
+
+
+def generate_scene(config: int, seed: int = 0) -> SceneData:
+    """Procedural scene of a benchmark configuration (1 Cornell box, 2 open interior,
+    3 interior with a window, 5 batched-view family, 100/101 small test interiors)."""
+    lib = load()
+    h = C.c_void_p()
+    check(lib.pg_scene_generate(int(config), int(seed), C.byref(h)))
+    try:
+        nt, nm = C.c_uint32(), C.c_uint32()
+        ew, eh = C.c_int32(), C.c_int32()
+        check(lib.pg_scene_sizes(h, C.byref(nt), C.byref(nm), C.byref(ew), C.byref(eh)))
+        pos = np.empty((nt.value, 3, 3), np.float32)
+        nrm = np.empty((nt.value, 3, 3), np.float32)
+        mi = np.empty(nt.value, np.uint32)
+        mats = np.zeros(nm.value, MATERIAL_DTYPE)
+        env = np.empty((eh.value, ew.value, 3), np.float32) if ew.value > 0 else None
+        cam = PgCameraDesc()
+        check(lib.pg_scene_copy(h, _fp(pos), _fp(nrm), _up(mi), mats.ctypes.data_as(C.POINTER(PgMaterial)),
+                                _fp(env) if env is not None else None, C.byref(cam)))
+    finally:
+        lib.pg_scene_free(h)
+    camera = {"from": list(cam.from_), "to": list(cam.to), "up": list(cam.up), "fov": cam.fov_deg,
+              "width": int(cam.width), "height": int(cam.height)}
+    return SceneData(pos, nrm, mi, mats, env, camera)
+
+
+class Device:
+    """A device context (pg_ctx) bound to one GPU."""
+
+    def __init__(self, device: int = 0):
+        self.lib = load()
+        self.ctx = C.c_void_p()
+        check(self.lib.pg_create(C.byref(self.ctx), int(device)))
+
+    def close(self):
+        if self.ctx:
+            self.lib.pg_destroy(self.ctx)
+            self.ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, code: int):
+        check(code, self.ctx)
+
+    def info(self) -> dict:
+        name = C.create_string_buffer(64)
+        cu = C.c_int32()
+        peak = C.c_double()
+        self._chk(self.lib.pg_device_info(self.ctx, name, 64, C.byref(cu), C.byref(peak)))
+        return {"name": name.value.decode(), "cu_count": cu.value, "peak_hbm_gbs": peak.value}
+
+
+class Scene:
+    """Scene::init + Scene::build (Scene.h:84-137) on the device; traverse / visible as
+    batched queries (Scene.h:103-146)."""
+
+    def __init__(self, device: Device, data: SceneData):
+        self.device = device
+        self.data = data
+        lib = device.lib
+        pos = np.ascontiguousarray(data.positions, np.float32)
+        nrm = np.ascontiguousarray(data.normals, np.float32)
+        mi = np.ascontiguousarray(data.material_index, np.uint32)
+        mats = np.ascontiguousarray(data.materials)
+        env = None if data.env is None else np.ascontiguousarray(data.env, np.float32)
+        c = data.camera
+        cam = PgCameraDesc((C.c_float * 3)(*c["from"]), (C.c_float * 3)(*c["to"]), (C.c_float * 3)(*c["up"]),
+                           c["fov"], float(c["width"]), float(c["height"]))
+        device._chk(lib.pg_scene_upload(device.ctx, _fp(pos), _fp(nrm), _up(mi), pos.shape[0],
+                                        mats.ctypes.data_as(C.POINTER(PgMaterial)), mats.shape[0],
+                                        _fp(env) if env is not None else None,
+                                        0 if env is None else env.shape[1], 0 if env is None else env.shape[0],
+                                        C.byref(cam)))
+
+    def bvh_info(self) -> dict:
+        n, d, ms = C.c_uint32(), C.c_uint32(), C.c_double()
+        self.device._chk(self.device.lib.pg_bvh_info(self.device.ctx, C.byref(n), C.byref(d), C.byref(ms)))
+        return {"nodes": n.value, "max_depth": d.value, "build_ms": ms.value}
+
+    def bvh_export(self):
+        info = self.bvh_info()
+        nodes = np.empty((info["nodes"], 8), np.float32)
+        idx = np.empty(self.data.n_tris, np.int32)
+        self.device._chk(self.device.lib.pg_bvh_export(self.device.ctx, _fp(nodes),
+                                                        idx.ctypes.data_as(C.POINTER(C.c_int32))))
+        return nodes, idx
+
+    def traverse(self, rays: np.ndarray):
+        """Closest hits for rays (n, 6) = origin, direction. Returns id, t, alpha, beta, gamma."""
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 6)
+        n = rays.shape[0]
+        hits = np.empty(n, np.dtype([("id", "u4"), ("t", "f4"), ("alpha", "f4"), ("beta", "f4"), ("gamma", "f4")]))
+        self.device._chk(self.device.lib.pg_trace_closest(self.device.ctx, _fp(rays), n,
+                                                           hits.ctypes.data_as(C.c_void_p)))
+        return hits["id"].copy(), hits["t"].copy(), hits["alpha"].copy(), hits["beta"].copy(), hits["gamma"].copy()
+
+    def visible_rays(self, rays: np.ndarray, max_t: np.ndarray) -> np.ndarray:
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 6)
+        max_t = np.ascontiguousarray(max_t, np.float32)
+        out = np.empty(rays.shape[0], np.uint8)
+        self.device._chk(self.device.lib.pg_trace_shadow(self.device.ctx, _fp(rays), _fp(max_t), rays.shape[0],
+                                                          out.ctypes.data_as(C.POINTER(C.c_uint8))))
+        return out
+
+    def generate_rays(self, pxy: np.ndarray) -> np.ndarray:
+        pxy = np.ascontiguousarray(pxy, np.float32).reshape(-1, 2)
+        out = np.empty((pxy.shape[0], 6), np.float32)
+        self.device._chk(self.device.lib.pg_generate_rays(self.device.ctx, _fp(pxy), pxy.shape[0], _fp(out)))
+        return out
+
+
+class RayTracer:
+    """RayTracer (Renderer.h:236-751): init / clear / render / getSPP / film read-back."""
+
+    def __init__(self, scene: Scene, **params):
+        self.scene = scene
+        self.device = scene.device
+        self.params = self.device.lib.pg_render_params_default()
+        for k, v in params.items():
+            setattr(self.params, k, v)
+
+    def clear(self):
+        self.device._chk(self.device.lib.pg_film_clear(self.device.ctx))
+
+    def render(self, n_spp: int = 1):
+        self.device._chk(self.device.lib.pg_render_spp(self.device.ctx, C.byref(self.params), int(n_spp)))
+
+    def getSPP(self) -> int:  # noqa: N802 - reference name
+        _, spp = self.film()
+        return spp
+
+    def film(self):
+        """(H, W, 3) float32 sum over samples and the SPP count."""
+        W, H = self.scene.data.width, self.scene.data.height
+        buf = np.empty((H, W, 3), np.float32)
+        spp = C.c_uint32()
+        self.device._chk(self.device.lib.pg_film_read(self.device.ctx, _fp(buf), C.byref(spp)))
+        return buf, spp.value
+
+    def image(self) -> np.ndarray:
+        """Film divided by SPP (Film::save, Imaging.h:263-272)."""
+        buf, spp = self.film()
+        return buf * (1.0 / spp if spp > 0 else 1.0)
+
+    def stats(self) -> dict:
+        s = PgStats()
+        self.device._chk(self.device.lib.pg_stats_read(self.device.ctx, C.byref(s)))
+        return {f: getattr(s, f) for f, _ in PgStats._fields_}
+
+    def reset_stats(self):
+        self.device._chk(self.device.lib.pg_stats_reset(self.device.ctx))
+
+    def set_counting(self, on: bool):
+        self.device._chk(self.device.lib.pg_set_counting(self.device.ctx, 1 if on else 0))
+
+
+def relmse(img: np.ndarray, ref: np.ndarray, eps: float = 1e-2) -> float:
+    """Mean over pixels and RGB of (I - R)^2 / (R^2 + eps), in FP64."""
+    i = img.astype(np.float64)
+    r = ref.astype(np.float64)
+    return float(np.mean((i - r) ** 2 / (r * r + eps)))
