@@ -140,6 +140,10 @@ pg_status pg_stats_read(pg_ctx* ctx, pg_stats* stats);
 pg_status pg_stats_reset(pg_ctx* ctx);
 /* Instrumented traversal: count node visits / triangle tests per ray (slower). */
 pg_status pg_set_counting(pg_ctx* ctx, int32_t enable);
+/* Measurement aid (no reference counterpart): a plain device-to-device streaming
+ * copy of `bytes`, timed with HIP events, as the HBM reference point for achieved
+ * bandwidth and for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE readings.          */
+pg_status pg_copy_benchmark(pg_ctx* ctx, uint64_t bytes, uint32_t iters, double* ms_per_copy);
 
 /* ------------------------------------------------------------------ host-side helpers */
 /* Procedural scenes of the benchmark configurations (no reference counterpart:

@@ -97,6 +97,8 @@ def lib():
         L.orc_ggx_g.argtypes = [_F, _F, C.c_float]
         L.orc_env_sample.argtypes = [C.c_void_p, C.c_float, C.c_float, _F, _F, _F]
         L.orc_env_eval.argtypes = [C.c_void_p, _F, _F, _F]
+        L.orc_env_sample_alias.argtypes = [C.c_void_p, _F, _F, _F, _F]
+        L.orc_env_tables.argtypes = [C.c_void_p, _F, _U32, _F]
         L.orc_env_marginal_integral.restype = C.c_float
         L.orc_env_marginal_integral.argtypes = [C.c_void_p]
         L.orc_scene_bounds.argtypes = [C.c_void_p, _F, _F]

@@ -77,6 +77,7 @@ SIGNATURES = {
     "pg_stats_read": (C.c_int32, [_P, C.POINTER(PgStats)]),
     "pg_stats_reset": (C.c_int32, [_P]),
     "pg_set_counting": (C.c_int32, [_P, C.c_int32]),
+    "pg_copy_benchmark": (C.c_int32, [_P, C.c_uint64, C.c_uint32, C.POINTER(C.c_double)]),
     "pg_scene_generate": (C.c_int32, [C.c_int32, C.c_uint32, C.POINTER(_P)]),
     "pg_scene_sizes": (C.c_int32, [_P, _U32, _U32, _I32, _I32]),
     "pg_scene_copy": (C.c_int32, [_P, _F, _F, _U32, C.POINTER(PgMaterial), _F, C.POINTER(PgCameraDesc)]),

@@ -283,7 +283,7 @@ pg_status pg_trace_closest(pg_ctx* c, const pg_ray* rays, uint32_t n, pg_hit* hi
         if ((st = c->hip(dalloc(&db, n), "alloc")) != PG_OK) break;
         if ((st = c->hip(dalloc(&dg, n), "alloc")) != PG_OK) break;
         if ((st = c->hip(hipMemcpy(dr, rays, (size_t)n * 24, hipMemcpyHostToDevice), "copy rays")) != PG_OK) break;
-        if ((st = c->hip(launch_trace_closest(c->dev, dr, n, did, dt, da, db, dg, 64, c->counting, c->d_stats, c->stream),
+        if ((st = c->hip(launch_trace_closest(c->dev, dr, n, did, dt, da, db, dg, (int)c->host.bvh.max_depth, c->counting, c->d_stats, c->stream),
                          "trace_closest")) != PG_OK) break;
         if ((st = c->hip(hipStreamSynchronize(c->stream), "sync")) != PG_OK) break;
         std::vector<uint32_t> id(n);
@@ -313,7 +313,7 @@ pg_status pg_trace_shadow(pg_ctx* c, const pg_ray* rays, const float* max_t, uin
         if ((st = c->hip(dalloc(&dv, n), "alloc")) != PG_OK) break;
         if ((st = c->hip(hipMemcpy(dr, rays, (size_t)n * 24, hipMemcpyHostToDevice), "copy")) != PG_OK) break;
         if ((st = c->hip(hipMemcpy(dm, max_t, (size_t)n * 4, hipMemcpyHostToDevice), "copy")) != PG_OK) break;
-        if ((st = c->hip(launch_trace_shadow(c->dev, dr, dm, n, dv, 64, c->stream), "trace_shadow")) != PG_OK) break;
+        if ((st = c->hip(launch_trace_shadow(c->dev, dr, dm, n, dv, (int)c->host.bvh.max_depth, c->stream), "trace_shadow")) != PG_OK) break;
         if ((st = c->hip(hipStreamSynchronize(c->stream), "sync")) != PG_OK) break;
         st = c->hip(hipMemcpy(visible, dv, n, hipMemcpyDeviceToHost), "copy");
     } while (0);
@@ -417,7 +417,8 @@ pg_status pg_render_spp(pg_ctx* c, const pg_render_params* params, uint32_t n_sp
     uint32_t n = W * H;
     pg_status st = ensure_paths(c, n);
     if (st != PG_OK) return st;
-    const uint32_t grid_trace = (uint32_t)c->cu_count * 5;
+    const int stack = (int)c->host.bvh.max_depth;
+    const uint32_t grid_trace = trace_grid(c->cu_count, stack);
     const uint32_t grid_shade = (uint32_t)c->cu_count * 8;
     c->pool_used = 0;
     c->launches.clear();
@@ -442,9 +443,9 @@ pg_status pg_render_spp(pg_ctx* c, const pg_render_params* params, uint32_t n_sp
         PG_TRY(c, timed(3, [&] { return launch_raygen(c->dev, c->ps, pp, c->stream); }), "raygen");
         for (uint32_t depth = 0; depth <= prm.max_depth; ++depth) {
             int parity = (int)(depth & 1u);
-            PG_TRY(c, timed(0, [&] { return launch_extend(c->dev, c->ps, parity, 64, c->counting, grid_trace, c->stream); }), "extend");
+            PG_TRY(c, timed(0, [&] { return launch_extend(c->dev, c->ps, parity, stack, c->counting, grid_trace, c->stream); }), "extend");
             PG_TRY(c, timed(1, [&] { return launch_shade(c->dev, c->ps, pp, parity, (int)depth, grid_shade, c->stream); }), "shade");
-            PG_TRY(c, timed(2, [&] { return launch_shadow(c->dev, c->ps, 64, c->counting, grid_trace, c->stream); }), "shadow");
+            PG_TRY(c, timed(2, [&] { return launch_shadow(c->dev, c->ps, stack, c->counting, grid_trace, c->stream); }), "shadow");
             c->n_extend++;
             c->n_shadow++;
         }
@@ -507,6 +508,34 @@ pg_status pg_stats_reset(pg_ctx* c) {
     c->ms_total = 0;
     c->n_extend = c->n_shadow = c->paths = 0;
     return PG_OK;
+}
+
+pg_status pg_copy_benchmark(pg_ctx* c, uint64_t bytes, uint32_t iters, double* ms_per_copy) {
+    PG_CHECK_CTX(c);
+    if (bytes < 16 || iters == 0) return c->fail(PG_ERR_INVALID, "copy size / iterations");
+    size_t n = (size_t)(bytes / 16);
+    float4 *a = nullptr, *b = nullptr;
+    pg_status st = PG_OK;
+    hipEvent_t e0 = c->ev(), e1 = c->ev();
+    do {
+        if ((st = c->hip(dalloc(&a, n), "alloc copy src")) != PG_OK) break;
+        if ((st = c->hip(dalloc(&b, n), "alloc copy dst")) != PG_OK) break;
+        if ((st = c->hip(hipMemsetAsync(a, 0, n * 16, c->stream), "init copy src")) != PG_OK) break;
+        if ((st = c->hip(launch_copy(a, b, n, c->cu_count, c->stream), "copy warmup")) != PG_OK) break;
+        (void)hipEventRecord(e0, c->stream);
+        for (uint32_t i = 0; i < iters && st == PG_OK; ++i)
+            st = c->hip(launch_copy(a, b, n, c->cu_count, c->stream), "copy");
+        (void)hipEventRecord(e1, c->stream);
+        if (st != PG_OK) break;
+        if ((st = c->hip(hipStreamSynchronize(c->stream), "sync")) != PG_OK) break;
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        if (ms_per_copy) *ms_per_copy = ms / iters;
+    } while (0);
+    c->pool_used = 0;
+    dfree(a);
+    dfree(b);
+    return st;
 }
 
 pg_status pg_set_counting(pg_ctx* c, int32_t enable) {

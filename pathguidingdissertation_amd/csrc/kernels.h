@@ -73,5 +73,8 @@ hipError_t launch_shadow(const DevScene& s, const PathState& ps, int stack, bool
                          hipStream_t stream);
 hipError_t launch_resolve(const PathState& ps, const PassParams& p, float4* film, hipStream_t stream);
 hipError_t launch_clear_film(float4* film, uint32_t n, hipStream_t stream);
+hipError_t launch_copy(const float4* src, float4* dst, size_t n, int cu_count, hipStream_t stream);
+int stack_entries(int max_depth);
+uint32_t trace_grid(int cu_count, int stack);
 
 }  // namespace pg

@@ -486,11 +486,11 @@ __device__ F3 bsdf_sample(const MaterialRec& m, const SD& sd, uint32_t key, uint
 }
 
 // ------------------------------------------------------------------ test-API kernels
-template <bool COUNT>
+template <bool COUNT, int STACK>
 __global__ __launch_bounds__(kBlock) void k_trace_closest(DevScene s, const float* __restrict__ rays, uint32_t n,
                                                           uint32_t* id, float* t, float* a, float* b, float* g,
                                                           unsigned long long* stats) {
-    __shared__ int lds[64 * kBlock];
+    __shared__ int lds[STACK * kBlock];
     uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     uint32_t cn = 0, ct = 0;
     if (i < n) {
@@ -511,9 +511,10 @@ __global__ __launch_bounds__(kBlock) void k_trace_closest(DevScene s, const floa
     }
 }
 
+template <int STACK>
 __global__ __launch_bounds__(kBlock) void k_trace_shadow(DevScene s, const float* __restrict__ rays,
                                                          const float* __restrict__ max_t, uint32_t n, uint8_t* vis) {
-    __shared__ int lds[64 * kBlock];
+    __shared__ int lds[STACK * kBlock];
     uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
     const float* r = rays + 6 * (size_t)i;
@@ -563,9 +564,9 @@ __global__ void k_raygen(DevScene s, PathState ps, PassParams p) {
     ps.info[i] = make_uint4(key, 2u, 0u, i);
 }
 
-template <bool COUNT>
+template <bool COUNT, int STACK>
 __global__ __launch_bounds__(kBlock) void k_extend(DevScene s, PathState ps, int parity) {
-    __shared__ int lds[64 * kBlock];
+    __shared__ int lds[STACK * kBlock];
     uint32_t n = ps.counters[parity];
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         ps.counters[parity ^ 1] = 0;
@@ -589,9 +590,9 @@ __global__ __launch_bounds__(kBlock) void k_extend(DevScene s, PathState ps, int
     }
 }
 
-template <bool COUNT>
+template <bool COUNT, int STACK>
 __global__ __launch_bounds__(kBlock) void k_shadow(DevScene s, PathState ps) {
-    __shared__ int lds[64 * kBlock];
+    __shared__ int lds[STACK * kBlock];
     uint32_t n = ps.counters[2];
     if (blockIdx.x == 0 && threadIdx.x == 0) ps.stats[1] += n;
     uint32_t cn = 0, ct = 0;
@@ -843,19 +844,48 @@ __global__ void k_clear_film(float4* film, uint32_t n) {
     if (i < n) film[i] = make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
+// Plain streaming copy: the HBM reference point for achieved bandwidth and for
+// scaling the FETCH_SIZE / WRITE_SIZE counters.
+__global__ __launch_bounds__(256) void k_copy(const float4* __restrict__ src, float4* __restrict__ dst, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) dst[i] = src[i];
+}
+
+hipError_t launch_copy(const float4* src, float4* dst, size_t n, int cu_count, hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_copy, dim3((uint32_t)cu_count * 8), dim3(256), 0, stream, src, dst, n);
+    return hipGetLastError();
+}
+
 // ------------------------------------------------------------------ launchers
+// The LDS stack holds at most one entry per tree level, so a stack of the tree's
+// maximum leaf depth never overflows; `stack` is that bound rounded up.
+#define PG_STACK_DISPATCH(stack, CALL) \
+    do {                                 \
+        if ((stack) <= 16) { CALL(16); } \
+        else if ((stack) <= 32) { CALL(32); } \
+        else if ((stack) <= 48) { CALL(48); } \
+        else { CALL(64); }               \
+    } while (0)
+
 hipError_t launch_trace_closest(const DevScene& s, const float* rays6, uint32_t n, uint32_t* id, float* t, float* a,
-                                float* b, float* g, int, bool count, unsigned long long* stats, hipStream_t stream) {
+                                float* b, float* g, int stack, bool count, unsigned long long* stats,
+                                hipStream_t stream) {
     if (n == 0) return hipSuccess;
     dim3 grid((n + kBlock - 1) / kBlock);
-    if (count) hipLaunchKernelGGL(k_trace_closest<true>, grid, dim3(kBlock), 0, stream, s, rays6, n, id, t, a, b, g, stats);
-    else hipLaunchKernelGGL(k_trace_closest<false>, grid, dim3(kBlock), 0, stream, s, rays6, n, id, t, a, b, g, stats);
+#define PG_CALL(S)                                                                                                  \
+    if (count) hipLaunchKernelGGL((k_trace_closest<true, S>), grid, dim3(kBlock), 0, stream, s, rays6, n, id, t, a, b, g, stats); \
+    else hipLaunchKernelGGL((k_trace_closest<false, S>), grid, dim3(kBlock), 0, stream, s, rays6, n, id, t, a, b, g, stats)
+    PG_STACK_DISPATCH(stack, PG_CALL);
+#undef PG_CALL
     return hipGetLastError();
 }
 hipError_t launch_trace_shadow(const DevScene& s, const float* rays6, const float* max_t, uint32_t n, uint8_t* vis,
-                               int, hipStream_t stream) {
+                               int stack, hipStream_t stream) {
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_trace_shadow, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, s, rays6, max_t, n, vis);
+#define PG_CALL(S) \
+    hipLaunchKernelGGL((k_trace_shadow<S>), dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, s, rays6, max_t, n, vis)
+    PG_STACK_DISPATCH(stack, PG_CALL);
+#undef PG_CALL
     return hipGetLastError();
 }
 hipError_t launch_generate_rays(const DevScene& s, const float* pxy, uint32_t n, float* rays6, hipStream_t stream) {
@@ -874,10 +904,13 @@ hipError_t launch_raygen(const DevScene& s, const PathState& ps, const PassParam
     hipLaunchKernelGGL(k_raygen, dim3((n + 255) / 256), dim3(256), 0, stream, s, ps, p);
     return hipGetLastError();
 }
-hipError_t launch_extend(const DevScene& s, const PathState& ps, int parity, int, bool count, uint32_t grid,
+hipError_t launch_extend(const DevScene& s, const PathState& ps, int parity, int stack, bool count, uint32_t grid,
                          hipStream_t stream) {
-    if (count) hipLaunchKernelGGL(k_extend<true>, dim3(grid), dim3(kBlock), 0, stream, s, ps, parity);
-    else hipLaunchKernelGGL(k_extend<false>, dim3(grid), dim3(kBlock), 0, stream, s, ps, parity);
+#define PG_CALL(S)                                                                                        \
+    if (count) hipLaunchKernelGGL((k_extend<true, S>), dim3(grid), dim3(kBlock), 0, stream, s, ps, parity); \
+    else hipLaunchKernelGGL((k_extend<false, S>), dim3(grid), dim3(kBlock), 0, stream, s, ps, parity)
+    PG_STACK_DISPATCH(stack, PG_CALL);
+#undef PG_CALL
     return hipGetLastError();
 }
 hipError_t launch_shade(const DevScene& s, const PathState& ps, const PassParams& p, int parity, int depth,
@@ -885,10 +918,30 @@ hipError_t launch_shade(const DevScene& s, const PathState& ps, const PassParams
     hipLaunchKernelGGL(k_shade, dim3(grid), dim3(kBlock), 0, stream, s, ps, p, parity, depth);
     return hipGetLastError();
 }
-hipError_t launch_shadow(const DevScene& s, const PathState& ps, int, bool count, uint32_t grid, hipStream_t stream) {
-    if (count) hipLaunchKernelGGL(k_shadow<true>, dim3(grid), dim3(kBlock), 0, stream, s, ps);
-    else hipLaunchKernelGGL(k_shadow<false>, dim3(grid), dim3(kBlock), 0, stream, s, ps);
+hipError_t launch_shadow(const DevScene& s, const PathState& ps, int stack, bool count, uint32_t grid,
+                         hipStream_t stream) {
+#define PG_CALL(S)                                                                                  \
+    if (count) hipLaunchKernelGGL((k_shadow<true, S>), dim3(grid), dim3(kBlock), 0, stream, s, ps); \
+    else hipLaunchKernelGGL((k_shadow<false, S>), dim3(grid), dim3(kBlock), 0, stream, s, ps)
+    PG_STACK_DISPATCH(stack, PG_CALL);
+#undef PG_CALL
     return hipGetLastError();
+}
+
+int stack_entries(int max_depth) {
+    if (max_depth <= 16) return 16;
+    if (max_depth <= 32) return 32;
+    if (max_depth <= 48) return 48;
+    return 64;
+}
+
+// Persistent grid for the traversal kernels: as many 128-lane blocks per CU as
+// the LDS stacks (160 KB per CU) allow.
+uint32_t trace_grid(int cu_count, int stack) {
+    int per_cu = (160 * 1024) / (stack_entries(stack) * kBlock * 4);
+    if (per_cu > 16) per_cu = 16;
+    if (per_cu < 1) per_cu = 1;
+    return (uint32_t)(cu_count * per_cu);
 }
 hipError_t launch_resolve(const PathState& ps, const PassParams& p, float4* film, hipStream_t stream) {
     uint32_t n = p.width * p.height;

@@ -1,0 +1,208 @@
+"""Write the golden fixtures under tests/golden/.
+
+Reference vectors (ref_*.npz) come from the reference's own classes, compiled from
+/root/reference by oracle/Makefile into oracle/_ref/libpgref.so (this container only).
+Oracle images (oracle_*.npz, *_band.json) come from the CPU oracle in FP64.
+
+    python tests/golden/make_golden.py [--ref] [--images]
+"""
+from __future__ import annotations
+
+import argparse
+import hashlib
+import json
+import sys
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parents[2]
+sys.path.insert(0, str(ROOT))
+sys.path.insert(0, str(ROOT / "tests"))
+OUT = ROOT / "tests" / "golden"
+
+MAT_TYPES = {"diffuse": 0, "mirror": 1, "conductor": 2, "glass": 3, "dielectric": 4}
+
+
+def material(kind, albedo=(0.8, 0.6, 0.4), rough=0.3, eta=(1.0, 0.6, 1.5), k=(0.65, 0.85, 1.0), iors=(1.5, 1.0)):
+    m = np.zeros(16, np.float32)
+    m[0] = MAT_TYPES[kind]
+    m[1:4] = albedo
+    m[7:10] = eta
+    m[10:13] = k
+    m[13], m[14] = iors
+    m[15] = rough
+    return m
+
+
+def bsdf_cases(rng, n):
+    """Shading frames, wo, random numbers and an evaluation direction per case."""
+    def unit(v):
+        return v / np.linalg.norm(v, axis=-1, keepdims=True)
+    nrm = unit(rng.normal(size=(n, 3)))
+    wo = unit(rng.normal(size=(n, 3)))
+    flip = np.sum(wo * nrm, 1) < 0
+    wo[flip] = -wo[flip]
+    # a fifth of the cases look from below (exercises two-sided / transmission paths)
+    below = rng.random(n) < 0.2
+    wo[below] = -wo[below]
+    sd = np.zeros((n, 12), np.float32)
+    sd[:, 3:6] = wo
+    sd[:, 6:9] = nrm
+    sd[:, 9:12] = nrm
+    u = rng.random((n, 3)).astype(np.float32)
+    we = unit(rng.normal(size=(n, 3))).astype(np.float32)
+    return sd, u, we
+
+
+def make_ref():
+    from oracle import pg_oracle as po
+    import pathguidingdissertation_amd as pg
+    assert po.ref_available(), "build oracle/_ref first (make -C oracle ref)"
+    R = po.ref()
+    import ctypes as C
+
+    # --- BVH topology + hits (Geometry.h:101-130, 175-192, 293-567)
+    for cfg, n_rays in ((1, 8192), (100, 4096)):
+        data = pg.generate_scene(cfg, 0)
+        osc = po.OracleScene(data)
+        from conftest import seeded_rays
+        rays = seeded_rays(osc, data, n_rays, seed=100 + cfg)
+        pos = np.ascontiguousarray(data.positions)
+        h = R.ref_scene_create(po.fp(pos), po.fp(np.ascontiguousarray(data.normals)), data.n_tris)
+        nn = R.ref_bvh_node_count(h)
+        nodes = np.empty((nn, 8), np.float32)
+        idx = np.empty(data.n_tris, np.int32)
+        R.ref_bvh_export(h, po.fp(nodes), idx.ctypes.data_as(C.POINTER(C.c_int32)))
+        ids = np.empty(n_rays, np.uint32)
+        t, a, b, g = (np.empty(n_rays, np.float32) for _ in range(4))
+        R.ref_trace_closest(h, po.fp(rays), n_rays, po.up(ids), po.fp(t), po.fp(a), po.fp(b), po.fp(g))
+        max_t = np.random.default_rng(cfg).uniform(0.01, 20.0, n_rays).astype(np.float32)
+        vis = np.empty(n_rays, np.uint8)
+        R.ref_trace_shadow(h, po.fp(rays), po.fp(max_t), n_rays, vis.ctypes.data_as(C.POINTER(C.c_uint8)))
+        R.ref_scene_destroy(h)
+        extra = {}
+        if cfg == 1:
+            extra = {"nodes": nodes, "tri_index": idx}
+        np.savez_compressed(OUT / f"ref_trace_config{cfg}.npz", rays=rays, id=ids, t=t, alpha=a, beta=b, gamma=g,
+                            max_t=max_t, visible=vis, n_nodes=np.int64(nn),
+                            topology_sha256=np.bytes_(hashlib.sha256(nodes.tobytes() + idx.tobytes()).hexdigest()),
+                            **extra)
+        print("trace config", cfg, "nodes", nn)
+
+    # full-size topology hash for the ~1.09M-triangle interior
+    data = pg.generate_scene(2, 0)
+    pos = np.ascontiguousarray(data.positions)
+    h = R.ref_scene_create(po.fp(pos), po.fp(np.ascontiguousarray(data.normals)), data.n_tris)
+    nn = R.ref_bvh_node_count(h)
+    nodes = np.empty((nn, 8), np.float32)
+    idx = np.empty(data.n_tris, np.int32)
+    R.ref_bvh_export(h, po.fp(nodes), idx.ctypes.data_as(C.POINTER(C.c_int32)))
+    R.ref_scene_destroy(h)
+    scene_sha = hashlib.sha256(pos.tobytes() + np.ascontiguousarray(data.normals).tobytes()).hexdigest()
+    json.dump({"n_tris": int(data.n_tris), "n_nodes": int(nn), "scene_sha256": scene_sha,
+               "topology_sha256": hashlib.sha256(nodes.tobytes() + idx.tobytes()).hexdigest()},
+              open(OUT / "ref_topology_config2.json", "w"), indent=1)
+    print("config 2 nodes", nn)
+
+    # --- camera (Scene.h:25-57, MyMath.h:286-364)
+    cams = []
+    for cfg in (1, 100):
+        data = pg.generate_scene(cfg, 0)
+        rng = np.random.default_rng(cfg)
+        pxy = np.stack([rng.uniform(0, data.width, 2048), rng.uniform(0, data.height, 2048)], 1).astype(np.float32)
+        ip, cm, rays = np.empty(16, np.float32), np.empty(16, np.float32), np.empty((2048, 6), np.float32)
+        R.ref_camera(po.fp(data.camera_array()), po.fp(pxy), 2048, po.fp(ip), po.fp(cm), po.fp(rays))
+        cams.append((data.camera_array(), pxy, ip, cm, rays))
+    np.savez_compressed(OUT / "ref_camera.npz", **{f"{k}{i}": v for i, c in enumerate(cams)
+                                                  for k, v in zip(("cam", "pxy", "inv_proj", "camera", "rays"), c)})
+
+    # --- BSDFs (Materials.h:36-639) with injected random numbers (TestSampler pattern)
+    rng = np.random.default_rng(42)
+    mats, sds, us, wes, outs = [], [], [], [], []
+    for kind, rough in (("diffuse", 0.0), ("mirror", 0.0), ("conductor", 0.3), ("conductor", 0.0),
+                        ("conductor", 0.8), ("glass", 0.0), ("dielectric", 0.2), ("dielectric", 0.0)):
+        m = material(kind, rough=rough)
+        sd, u, we = bsdf_cases(rng, 512)
+        for i in range(sd.shape[0]):
+            out = np.empty(14, np.float32)
+            R.ref_bsdf(po.fp(m), po.fp(sd[i]), po.fp(u[i]), po.fp(we[i]), po.fp(out))
+            mats.append(m); sds.append(sd[i]); us.append(u[i]); wes.append(we[i]); outs.append(out)
+    np.savez_compressed(OUT / "ref_bsdf.npz", mat=np.array(mats), sd=np.array(sds), u=np.array(us),
+                        wi_eval=np.array(wes), out=np.array(outs))
+    # Fresnel / GGX helpers
+    c = np.linspace(-1.0, 1.0, 257).astype(np.float32)
+    fd = np.array([R.ref_fresnel_dielectric(float(x), 1.5, 1.0) for x in c], np.float32)
+    eta, k = np.array([0.2, 0.9, 1.1], np.float32), np.array([3.9, 2.4, 2.1], np.float32)
+    fc = np.empty((c.size, 3), np.float32)
+    for i, x in enumerate(c):
+        R.ref_fresnel_conductor(float(x), po.fp(eta), po.fp(k), po.fp(fc[i]))
+    hs = rng.normal(size=(256, 3)).astype(np.float32)
+    hs /= np.linalg.norm(hs, axis=1, keepdims=True)
+    alphas = rng.uniform(0.05, 1.5, 256).astype(np.float32)
+    dg = np.array([R.ref_ggx_d(po.fp(hs[i]), float(alphas[i])) for i in range(256)], np.float32)
+    wi2 = rng.normal(size=(256, 3)).astype(np.float32)
+    wi2 /= np.linalg.norm(wi2, axis=1, keepdims=True)
+    gg = np.array([R.ref_ggx_g(po.fp(wi2[i]), po.fp(hs[i]), float(alphas[i])) for i in range(256)], np.float32)
+    np.savez_compressed(OUT / "ref_fresnel_ggx.npz", cos=c, fresnel_dielectric=fd, eta=eta, k=k,
+                        fresnel_conductor=fc, h=hs, alpha=alphas, ggx_d=dg, wi=wi2, ggx_g=gg)
+
+    # --- environment map (Lights.h:128-228, Sampling.h:105-209) on a small synthetic map
+    env = synthetic_env(64, 32)
+    he = R.ref_env_create(po.fp(env), 64, 32)
+    u2 = rng.random((4096, 2)).astype(np.float32)
+    samp = np.empty((4096, 7), np.float32)
+    for i in range(4096):
+        R.ref_env_sample(he, float(u2[i, 0]), float(u2[i, 1]), po.fp(samp[i]))
+    dirs = rng.normal(size=(4096, 3)).astype(np.float32)
+    dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
+    ev = np.empty((4096, 4), np.float32)
+    for i in range(4096):
+        R.ref_env_eval(he, po.fp(dirs[i]), po.fp(ev[i]))
+    power = R.ref_env_power(he)
+    R.ref_env_destroy(he)
+    np.savez_compressed(OUT / "ref_env.npz", env=env, u=u2, sample=samp, dirs=dirs, eval=ev, power=np.float32(power))
+    print("reference vectors written")
+
+
+def synthetic_env(w, h):
+    v = (np.arange(h) + 0.5) / h
+    u = (np.arange(w) + 0.5) / w
+    th, ph = np.meshgrid(np.pi * v, 2 * np.pi * u, indexing="ij")
+    d = np.stack([np.cos(ph) * np.sin(th), np.cos(th), np.sin(ph) * np.sin(th)], -1)
+    sky = np.where(d[..., 1:2] > 0, np.array([0.3, 0.5, 1.0]) * (0.2 + d[..., 1:2]), np.array([0.1, 0.08, 0.05]))
+    sun = np.array([0.6, 0.7, 0.3])
+    sun /= np.linalg.norm(sun)
+    lobe = 200.0 * np.exp(-(1 - np.sum(d * sun, -1)) / 0.01)[..., None]
+    return np.ascontiguousarray((sky + lobe).astype(np.float32))
+
+
+def make_images(spp_ref=4096, k=8, spp=16):
+    """FP64 oracle image of the Cornell box and the relMSE band of equal-SPP CPU runs."""
+    from oracle import pg_oracle as po
+    import pathguidingdissertation_amd as pg
+    data = pg.generate_scene(1, 0)
+    osc = po.OracleScene(data)
+    film, st = osc.render(spp_ref, seed=9001, accum_f64=True)
+    ref = film / spp_ref
+    np.savez_compressed(OUT / "oracle_config1_4096spp.npz", image=ref.astype(np.float32), spp=spp_ref, seed=9001)
+    vals = []
+    for s in range(1, k + 1):
+        f, _ = osc.render(spp, seed=s, accum_f64=False)
+        img = (f / spp).astype(np.float64)
+        vals.append(float(np.mean((img - ref) ** 2 / (ref * ref + 1e-2))))
+    band = {"config": 1, "spp": spp, "k": k, "relmse": vals, "mean": float(np.mean(vals)),
+            "std": float(np.std(vals, ddof=1)), "reference_spp": spp_ref, "reference_seed": 9001}
+    json.dump(band, open(OUT / "relmse_band_config1.json", "w"), indent=1)
+    print("band", band)
+
+
+if __name__ == "__main__":
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--ref", action="store_true")
+    ap.add_argument("--images", action="store_true")
+    a = ap.parse_args()
+    if a.ref:
+        make_ref()
+    if a.images:
+        make_images()
