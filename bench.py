@@ -30,6 +30,19 @@ def algorithmic_bytes(nodes_per_ray: float, tris_per_ray: float, out_bytes: int)
     return 32.0 * nodes_per_ray + 40.0 * tris_per_ray + 32.0 + out_bytes
 
 
+def profiled_traffic(config: int) -> dict:
+    """HBM bytes per dispatch measured by rocprofv3 FETCH_SIZE / WRITE_SIZE runs of this
+    benchmark (scaled by a plain copy kernel), as committed under profiles/."""
+    path = ROOT / "profiles" / f"pmc_traffic_config{config}.txt"
+    out = {}
+    if path.exists():
+        for line in path.read_text().splitlines():
+            if line and not line.startswith("#"):
+                k, v = line.split()
+                out[k] = float(v)
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -95,6 +108,7 @@ def main() -> int:
 
     img = rt.image()
     finite = bool(np.isfinite(img).all())
+    traffic = profiled_traffic(a.config).get("k_extend")
 
     copy = None
     if a.copy_check:
@@ -139,7 +153,9 @@ def main() -> int:
         "kernel_ms": {"extend": round(ext_ms, 3), "shade": round(st["kernel_ms_shade"], 3),
                       "shadow": round(sh_ms, 3), "gpu_total": round(st["kernel_ms_total"], 3)},
         "roofline": {"bound": "hbm", "kernel": "k_extend", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
-                     "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
+                     "traffic": None if traffic is None else round(traffic),
+                     "traffic_source": "profiles/pmc_traffic_config%d.txt (bytes per launch)" % a.config,
                      "bytes_per_ray": round(bytes_c, 1), "nodes_per_ray": round(nodes_c, 2),
                      "tris_per_ray": round(tris_c, 2), "launches": launches,
                      "avg_launch_ms": round(ext_ms / launches, 4)},

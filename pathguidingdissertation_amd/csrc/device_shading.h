@@ -1,0 +1,489 @@
+// device_shading.h — device functions shared by the wavefront kernels: BVH
+// traversal (Geometry.h:448-567), camera rays (Scene.h:45-57), environment map
+// (Lights.h:128-228), BSDFs (Materials.h:36-639) and shadow-ray setup (Scene.h:139-146).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "device_math.h"
+#include "kernels.h"
+
+namespace pg {
+
+constexpr int kBlock = 128;
+
+__device__ __forceinline__ int f2i(float f) { return __float_as_int(f); }
+
+// AABB::rayAABB (Geometry.h:175-192)
+__device__ __forceinline__ bool slab(float4 lo, float4 hi, F3 o, F3 inv, float& t) {
+    float t0x = (lo.x - o.x) * inv.x, t0y = (lo.y - o.y) * inv.y, t0z = (lo.z - o.z) * inv.z;
+    float t1x = (hi.x - o.x) * inv.x, t1y = (hi.y - o.y) * inv.y, t1z = (hi.z - o.z) * inv.z;
+    float ex = smin(t0x, t1x), ey = smin(t0y, t1y), ez = smin(t0z, t1z);
+    float xx = smax(t0x, t1x), xy = smax(t0y, t1y), xz = smax(t0z, t1z);
+    float fe = smax(ex, smax(ey, ez));
+    float fx = smin(xx, smin(xy, xz));
+    if (fe > fx || fx < 0.f) return false;
+    t = smax(0.f, fe);
+    return true;
+}
+
+// Triangle::rayIntersect (Geometry.h:101-130) on a leaf-ordered record.
+__device__ __forceinline__ bool mt(const TriIsect* tp, F3 o, F3 d, float& t, float& u, float& v) {
+    const float4* q = reinterpret_cast<const float4*>(tp);
+    float4 a = q[0], b = q[1], c = q[2];
+    F3 v0{a.x, a.y, a.z}, e0{b.x, b.y, b.z}, e1{c.x, c.y, c.z};
+    F3 p = cross(d, e1);
+    float det = dot(e0, p);
+    if (fabsf(det) < kMtEps) return false;
+    float inv_det = 1.f / det;
+    F3 T = o - v0;
+    u = dot(T, p) * inv_det;
+    if (u < 0.f || u > 1.f) return false;
+    F3 qv = cross(T, e0);
+    v = dot(d, qv) * inv_det;
+    if (v < 0.f || v > 1.f || (u + v > 1.f)) return false;
+    t = dot(e1, qv) * inv_det;
+    return t >= 0.f;
+}
+
+__device__ __forceinline__ void node_header(const BvhNode* nodes, int idx, int& lf, int& cnt) {
+    const int* h = reinterpret_cast<const int*>(nodes + idx + 1);
+    lf = h[3];
+    cnt = h[7];
+}
+
+// Closest hit. stk: this lane's LDS stack (entries at stk[k * kBlock]).
+template <bool COUNT>
+__device__ __forceinline__ void traverse_closest(const DevScene& s, F3 o, F3 d, uint32_t& hid, float& ht,
+                                                 float& hu, float& hv, int* stk, uint32_t& cn, uint32_t& ct) {
+    F3 inv{1.f / d.x, 1.f / d.y, 1.f / d.z};
+    const float4* nf = reinterpret_cast<const float4*>(s.nodes);
+    float4 r0 = nf[2], r1 = nf[3];  // root at slot 1
+    float tb;
+    if (COUNT) cn += 1;
+    if (!slab(r0, r1, o, inv, tb)) return;
+    if (tb > ht) return;
+    int lf = f2i(r0.w), cnt = f2i(r1.w);
+    int sp = 0;
+    while (true) {
+        if (cnt > 0) {
+            for (int k = 0; k < cnt; ++k) {
+                const TriIsect* tp = s.tris + lf + k;
+                float t, u, v;
+                if (COUNT) ct += 1;
+                if (mt(tp, o, d, t, u, v)) {
+                    if (t > kEps && t < ht) {
+                        ht = t;
+                        hid = tp->id;
+                        hu = u;
+                        hv = v;
+                    }
+                }
+            }
+            if (sp == 0) break;
+            node_header(s.nodes, stk[(--sp) * kBlock], lf, cnt);
+            continue;
+        }
+        const float4* pr = nf + 2 * (lf + 1);
+        float4 a0 = pr[0], a1 = pr[1], b0 = pr[2], b1 = pr[3];
+        if (COUNT) cn += 2;
+        float dl = 0.f, dr = 0.f;
+        if (!slab(a0, a1, o, inv, dl) || dl >= ht) dl = kFltMax;
+        if (!slab(b0, b1, o, inv, dr) || dr >= ht) dr = kFltMax;
+        int nl = lf, nr = lf + 1;
+        int lfl = f2i(a0.w), cl = f2i(a1.w), lfr = f2i(b0.w), cr = f2i(b1.w);
+        if (dl > dr) {
+            float tt = dl; dl = dr; dr = tt;
+            int ti = nl; nl = nr; nr = ti;
+            ti = lfl; lfl = lfr; lfr = ti;
+            ti = cl; cl = cr; cr = ti;
+        }
+        if (dl == kFltMax) {
+            if (sp == 0) break;
+            node_header(s.nodes, stk[(--sp) * kBlock], lf, cnt);
+        } else {
+            lf = lfl;
+            cnt = cl;
+            if (dr != kFltMax) stk[(sp++) * kBlock] = nr;
+        }
+    }
+}
+
+// Any hit in (1e-4, max_t): returns true when unoccluded.
+template <bool COUNT>
+__device__ __forceinline__ bool traverse_visible(const DevScene& s, F3 o, F3 d, float max_t, int* stk,
+                                                 uint32_t& cn, uint32_t& ct) {
+    F3 inv{1.f / d.x, 1.f / d.y, 1.f / d.z};
+    const float4* nf = reinterpret_cast<const float4*>(s.nodes);
+    float4 r0 = nf[2], r1 = nf[3];
+    float tb;
+    if (COUNT) cn += 1;
+    if (!slab(r0, r1, o, inv, tb)) return true;
+    if (tb > max_t) return true;
+    int lf = f2i(r0.w), cnt = f2i(r1.w);
+    int sp = 0;
+    while (true) {
+        if (cnt > 0) {
+            for (int k = 0; k < cnt; ++k) {
+                float t, u, v;
+                if (COUNT) ct += 1;
+                if (mt(s.tris + lf + k, o, d, t, u, v)) {
+                    if (t > kEps && t < max_t) return false;
+                }
+            }
+            if (sp == 0) break;
+            node_header(s.nodes, stk[(--sp) * kBlock], lf, cnt);
+            continue;
+        }
+        const float4* pr = nf + 2 * (lf + 1);
+        float4 a0 = pr[0], a1 = pr[1], b0 = pr[2], b1 = pr[3];
+        if (COUNT) cn += 2;
+        float dl = 0.f, dr = 0.f;
+        if (!slab(a0, a1, o, inv, dl) || dl >= max_t) dl = kFltMax;
+        if (!slab(b0, b1, o, inv, dr) || dr >= max_t) dr = kFltMax;
+        int nr = lf + 1;
+        int lfl = f2i(a0.w), cl = f2i(a1.w);
+        if (dl > dr) {
+            float tt = dl; dl = dr; dr = tt;
+            nr = lf;
+            lfl = f2i(b0.w);
+            cl = f2i(b1.w);
+        }
+        if (dl == kFltMax) {
+            if (sp == 0) break;
+            node_header(s.nodes, stk[(--sp) * kBlock], lf, cnt);
+        } else {
+            lf = lfl;
+            cnt = cl;
+            if (dr != kFltMax) stk[(sp++) * kBlock] = nr;
+        }
+    }
+    return true;
+}
+
+// ------------------------------------------------------------------ camera
+// Camera::generateRay (Scene.h:45-57) with Matrix::mulPointAndPerspectiveDivide / mulVec.
+__device__ __forceinline__ void generate_ray(const CameraRec& c, float x, float y, F3& o, F3& d) {
+    float xc = (2.f * x / c.width) - 1.f;
+    float yc = (2.f * (1.f - (y / c.height))) - 1.f;
+    const float* m = c.inv_proj;
+    float zc = 0.f;
+    F3 p{((xc * m[0] + yc * m[1]) + zc * m[2]) + m[3], ((xc * m[4] + yc * m[5]) + zc * m[6]) + m[7],
+         ((xc * m[8] + yc * m[9]) + zc * m[10]) + m[11]};
+    float w = ((m[12] * xc + m[13] * yc) + m[14] * zc) + m[15];
+    w = 1.f / w;
+    F3 dc = p * w;
+    const float* k = c.cam;
+    F3 dw{(dc.x * k[0] + dc.y * k[1]) + dc.z * k[2], (dc.x * k[4] + dc.y * k[5]) + dc.z * k[6],
+          (dc.x * k[8] + dc.y * k[9]) + dc.z * k[10]};
+    d = normalize(dw);
+    o = F3{c.origin[0], c.origin[1], c.origin[2]};
+}
+
+// ------------------------------------------------------------------ environment (Lights.h:128-228)
+constexpr double kPiD = 3.14159265358979323846;
+
+__device__ __forceinline__ C3 env_lookup(const DevScene& s, float tu, float tv) {  // Imaging.h:82-108
+    float u = smax(0.f, fabsf(tu)) * (float)s.env_w;
+    float v = smax(0.f, fabsf(tv)) * (float)s.env_h;
+    int x = (int)floorf(u), y = (int)floorf(v);
+    float fu = u - (float)x, fv = v - (float)y;
+    float w0 = (1.f - fu) * (1.f - fv), w1 = fu * (1.f - fv), w2 = (1.f - fu) * fv, w3 = fu * fv;
+    x = x % s.env_w;
+    y = y % s.env_h;
+    int x1 = (x + 1) % s.env_w, y1 = (y + 1) % s.env_h;
+    float4 a = s.env_tex[y * s.env_w + x], b = s.env_tex[y * s.env_w + x1];
+    float4 c = s.env_tex[y1 * s.env_w + x], e = s.env_tex[y1 * s.env_w + x1];
+    C3 r = ((c3(a.x, a.y, a.z) * w0 + c3(b.x, b.y, b.z) * w1) + c3(c.x, c.y, c.z) * w2) + c3(e.x, e.y, e.z) * w3;
+    return r;
+}
+__device__ __forceinline__ void env_uv(F3 wi, float& u, float& v) {
+    u = atan2f(wi.z, wi.x);
+    u = (u < 0.f) ? (float)((double)u + 2.0 * kPiD) : u;
+    u = (float)((double)u / (2.0 * kPiD));
+    v = (float)((double)acosf(wi.y) / kPiD);
+}
+__device__ __forceinline__ C3 env_eval(const DevScene& s, F3 wi) {
+    float u, v;
+    env_uv(wi, u, v);
+    return env_lookup(s, u, v);
+}
+__device__ __forceinline__ float env_pdf(const DevScene& s, F3 wi) {
+    float u, v;
+    env_uv(wi, u, v);
+    float st = (float)sin(kPiD * (double)v);
+    if (st <= 0.f) return 0.f;
+    int iu = min(max((int)(u * (float)s.env_w), 0), s.env_w - 1);
+    int iv = min(max((int)(v * (float)s.env_h), 0), s.env_h - 1);
+    return (float)((double)s.env_pdf[iv * s.env_w + iu] / (2.0 * kPiD * kPiD * (double)st));
+}
+// Alias-table sample: 4 draws (texel, acceptance, in-texel u, in-texel v).
+__device__ __forceinline__ F3 env_sample(const DevScene& s, uint32_t key, uint32_t& dim, C3& col, float& pdf) {
+    int n = s.env_w * s.env_h;
+    float r1 = keyed_rand(key, dim++), r2 = keyed_rand(key, dim++);
+    float r3 = keyed_rand(key, dim++), r4 = keyed_rand(key, dim++);
+    int i = min((int)(r1 * (float)n), n - 1);
+    if (!(r2 < s.env_prob[i])) i = (int)s.env_alias[i];
+    int iu = i % s.env_w, iv = i / s.env_w;
+    float u = ((float)iu + r3) / (float)s.env_w;
+    float v = ((float)iv + r4) / (float)s.env_h;
+    float theta = (float)(kPiD * (double)v);
+    float phi = (float)(2.0 * kPiD * (double)u);
+    float ct = cosf(theta), cp = cosf(phi), st = sinf(theta), sp = sinf(phi);
+    F3 wi{cp * st, ct, sp * st};
+    pdf = (st <= 0.f) ? 0.f : (float)((double)s.env_pdf[i] / (2.0 * kPiD * kPiD * (double)st));
+    col = (pdf <= 0.f) ? c3(0.f, 0.f, 0.f) : env_eval(s, wi);
+    return wi;
+}
+
+// ------------------------------------------------------------------ BSDFs (Materials.h)
+constexpr float kInvPiF = 0.318309886183790671538f;
+
+__device__ __forceinline__ float fresnel_dielectric(float cos_theta, float iint, float iext) {  // :36-61
+    float ci = smax(smin(cos_theta, 1.f), -1.f);
+    float eta = (ci < 0.f) ? (iint / iext) : (iext / iint);
+    if (ci < 0.f) ci = fabsf(ci);
+    float eta2 = eta * eta;
+    float ci2 = smax(ci * ci, 0.f);
+    float si2 = smax(1.f - ci2, 0.f);
+    float st2 = eta2 * si2;
+    if (st2 > 1.f) return 1.f;
+    float ctt = sqrtf(smax(1.f - st2, 0.f));
+    float par = (ci - eta * ctt) / (ci + eta * ctt);
+    float per = (eta * ci - ctt) / (eta * ci + ctt);
+    return ((par * par) + (per * per)) * 0.5f;
+}
+__device__ __forceinline__ C3 fresnel_conductor(float cos_theta, C3 ior, C3 k) {  // :63-90
+    float ci = smax(smin(cos_theta, 1.f), -1.f);
+    float ci2 = smax(ci * ci, 0.f);
+    float si2 = smax(1.f - ci2, 0.f);
+    C3 cC = c3(ci, ci, ci), cC2 = c3(ci2, ci2, ci2), cS2 = c3(si2, si2, si2);
+    C3 ior2 = ior * ior, k2 = k * k;
+    C3 par = (((ior2 + k2) * cC2) - (ior * cC * 2.f) + cS2) / (((ior2 + k2) * cC2) + (ior * cC * 2.f) + cS2);
+    C3 per = ((ior2 + k2) - (ior * cC * 2.f) + cC2) / ((ior2 + k2) + (ior * cC * 2.f) + cC2);
+    return (par + per) * 0.5f;
+}
+__device__ __forceinline__ float lambda_ggx(F3 w, float alpha) {  // :93-100
+    if (w.z <= 0.f) return 0.f;
+    float a2 = alpha * alpha, c2 = w.z * w.z;
+    float s2 = smax(1.f - c2, 0.f);
+    float t2 = fabsf(s2 / c2);
+    return (sqrtf(1.f + a2 * t2) - 1.f) * 0.5f;
+}
+__device__ __forceinline__ float g_ggx(F3 wi, F3 wo, float alpha) {
+    float g1o = 1.f / (1.f + lambda_ggx(wo, alpha));
+    float g1i = 1.f / (1.f + lambda_ggx(wi, alpha));
+    return g1o * g1i;
+}
+__device__ __forceinline__ float d_ggx(F3 h, float alpha) {  // :111-118
+    if (h.z <= 0.f) return 0.f;
+    float a2 = alpha * alpha, c2 = h.z * h.z;
+    float q = c2 * (a2 - 1.f) + 1.f;
+    float den = (float)((kPiD * (double)q) * (double)q);
+    if (den <= 0.f) return 0.f;
+    return a2 / den;
+}
+__device__ __forceinline__ float cosine_pdf(F3 wl) {  // Sampling.h:98-101
+    float theta = acosf(wl.z);
+    float c = cosf(theta);
+    return (c > 0.f) ? (float)((double)c * 0.318309886183790671538) : 0.f;
+}
+
+struct SD {
+    F3 x, wo, sn, gn;
+    Frame fr;
+};
+
+__device__ inline C3 bsdf_eval(const MaterialRec& m, const SD& sd, F3 wi, float& pdf) {
+    pdf = 0.f;
+    C3 zero = c3(0.f, 0.f, 0.f);
+    C3 alb = c3(m.albedo[0], m.albedo[1], m.albedo[2]);
+    if (m.type == PG_BSDF_DIFFUSE) {
+        F3 wl = to_local(sd.fr, wi);
+        pdf = cosine_pdf(wl);
+        if (wl.z <= 0.f) return zero;
+        return alb * kInvPiF;
+    }
+    if (m.type == PG_BSDF_CONDUCTOR) {
+        F3 wl = to_local(sd.fr, wi), ol = to_local(sd.fr, sd.wo);
+        if (wl.z <= 0.f || ol.z <= 0.f) return zero;
+        if (m.alpha < kEps) return zero;
+        F3 wm = wl + ol;
+        if (length_sq(wm) < kEps) return zero;
+        wm = normalize(wm);
+        if (dot(ol, wm) <= 0.f) return zero;
+        if (wl.z * ol.z <= 0.f) return zero;
+        float D = d_ggx(wm, m.alpha);
+        pdf = (D * wm.z) / (4.f * dot(ol, wm));
+        float G = g_ggx(wl, ol, m.alpha);
+        C3 F = fresnel_conductor(dot(ol, wm), c3(m.eta[0], m.eta[1], m.eta[2]), c3(m.k[0], m.k[1], m.k[2]));
+        return alb * (((F * G) * D) / ((4.f * wl.z) * ol.z));
+    }
+    if (m.type == PG_BSDF_DIELECTRIC) {
+        F3 ol = to_local(sd.fr, sd.wo), wl = to_local(sd.fr, wi);
+        if (ol.z == 0.f || wl.z == 0.f) return zero;
+        float eta = ol.z > 0.f ? m.ext_ior / m.int_ior : m.int_ior / m.ext_ior;
+        if (eta == 1.f || m.alpha < kEps) return zero;
+        bool refl = ol.z * wl.z > 0.f;
+        float ep = 1.f;
+        if (!refl) ep = ol.z > 0.f ? eta : 1.f / eta;
+        F3 wm = wl * ep + ol;
+        if (length_sq(wm) < kEps) return zero;
+        wm = normalize(wm);
+        if (dot(wm, wl) * wl.z < 0.f || dot(wm, ol) * ol.z < 0.f) return zero;
+        float F = fresnel_dielectric(dot(ol, wm), m.int_ior, m.ext_ior);
+        if (refl) {
+            if (dot(ol, wm) <= 0.f) return zero;
+            if (wl.z * ol.z == 0.f) return zero;
+            float D = d_ggx(wm, m.alpha);
+            pdf = F * ((D * wm.z) / (4.f * dot(ol, wm)));
+            float G = g_ggx(wl, ol, m.alpha);
+            return alb * (((F * G) * D) / ((4.f * wl.z) * ol.z));
+        }
+        float sm = dot(wl, wm) + dot(ol, wm) / ep;
+        float den = sm * sm;
+        if (den <= 0.f) return zero;
+        float D = d_ggx(wm, m.alpha);
+        pdf = (1.f - F) * ((D * fabsf(dot(ol, wm))) / den);
+        float G = g_ggx(wl, ol, m.alpha);
+        return ((alb * ((((1.f - F) * G) * D) / den)) * (fabsf(dot(wl, wm)) / fabsf(wl.z))) *
+               (fabsf(dot(ol, wm)) / fabsf(ol.z));
+    }
+    return zero;  // mirror / glass: delta distributions
+}
+
+__device__ inline F3 bsdf_sample(const MaterialRec& m, const SD& sd, uint32_t key, uint32_t& dim, C3& f, float& pdf) {
+    C3 alb = c3(m.albedo[0], m.albedo[1], m.albedo[2]);
+    F3 up{0.f, 0.f, 1.f};
+    f = c3(0.f, 0.f, 0.f);
+    pdf = 0.f;
+    switch (m.type) {
+    case PG_BSDF_DIFFUSE: {  // Materials.h:146-161
+        float r1 = keyed_rand(key, dim++), r2 = keyed_rand(key, dim++);
+        float theta = acosf(sqrtf(r1));
+        float phi = (float)(2.0 * kPiD * (double)r2);
+        F3 wi = to_world(sd.fr, spherical_to_world(theta, phi));
+        f = bsdf_eval(m, sd, wi, pdf);
+        return wi;
+    }
+    case PG_BSDF_MIRROR: {  // Materials.h:201-215
+        F3 ol = to_local(sd.fr, sd.wo);
+        F3 wr{-ol.x, -ol.y, ol.z};
+        if (fabsf(wr.z) <= kEps) return up;
+        f = alb / wr.z;
+        pdf = 1.f;
+        return to_world(sd.fr, wr);
+    }
+    case PG_BSDF_CONDUCTOR: {  // Materials.h:253-285
+        F3 ol = to_local(sd.fr, sd.wo);
+        if (ol.z <= 0.f) return up;
+        if (m.alpha < kEps) {
+            F3 wr{-ol.x, -ol.y, ol.z};
+            C3 F = fresnel_conductor(wr.z, c3(m.eta[0], m.eta[1], m.eta[2]), c3(m.k[0], m.k[1], m.k[2]));
+            f = (alb / fabsf(wr.z)) * F;
+            pdf = 1.f;
+            return to_world(sd.fr, wr);
+        }
+        float a2 = m.alpha * m.alpha;
+        float r1 = keyed_rand(key, dim++), r2 = keyed_rand(key, dim++);
+        float tm = acosf(sqrtf((1.f - r1) / (r1 * (a2 - 1.f) + 1.f)));
+        float pm = (float)(2.0 * kPiD * (double)r2);
+        F3 wm = spherical_to_world(tm, pm);
+        F3 wl = -ol + (wm * 2.f) * dot(wm, ol);
+        F3 wi = to_world(sd.fr, wl);
+        f = bsdf_eval(m, sd, wi, pdf);
+        return wi;
+    }
+    case PG_BSDF_GLASS: {  // Materials.h:375-421
+        F3 ol = to_local(sd.fr, sd.wo);
+        if (fabsf(ol.z) < kEps) return up;
+        float F = fresnel_dielectric(ol.z, m.int_ior, m.ext_ior);
+        float ci = ol.z;
+        float eta = (ci < 0.f) ? (m.int_ior / m.ext_ior) : (m.ext_ior / m.int_ior);
+        if (ci < 0.f) ci = fabsf(ci);
+        if (keyed_rand(key, dim++) < F) {
+            F3 wr{-ol.x, -ol.y, ol.z};
+            pdf = F;
+            f = (alb / fabsf(wr.z)) * F;
+            return to_world(sd.fr, wr);
+        }
+        float ci2 = smax(ci * ci, 0.f), si2 = smax(1.f - ci2, 0.f);
+        float eta2 = eta * eta, st2 = eta2 * si2;
+        if (st2 > 1.f) {
+            F3 wr{-ol.x, -ol.y, ol.z};
+            pdf = F;
+            f = (alb / fabsf(wr.z)) * F;
+            return to_world(sd.fr, wr);
+        }
+        float ctt = sqrtf(smax(1.f - st2, 0.f));
+        float wz = (ol.z < 0.f) ? ctt : -ctt;
+        F3 wt{-eta * ol.x, -eta * ol.y, wz};
+        pdf = 1.f - F;
+        f = ((alb / fabsf(wt.z)) * (1.f - F)) * eta2;
+        return to_world(sd.fr, wt);
+    }
+    default: {  // PG_BSDF_DIELECTRIC, Materials.h:459-539
+        F3 ol = to_local(sd.fr, sd.wo);
+        if (m.alpha < kEps) {
+            float F = fresnel_dielectric(ol.z, m.int_ior, m.ext_ior);
+            if (keyed_rand(key, dim++) < F) {
+                F3 wl{-ol.x, -ol.y, ol.z};
+                pdf = F;
+                f = (alb / fabsf(wl.z)) * F;
+                return to_world(sd.fr, wl);
+            }
+            float eta = ol.z > 0.f ? m.ext_ior / m.int_ior : m.int_ior / m.ext_ior;
+            float eta2 = eta * eta;
+            float sg = ol.z > 0.f ? -1.f : 1.f;
+            float ci2 = smax(ol.z * ol.z, 0.f), si2 = smax(1.f - ci2, 0.f), st2 = eta2 * si2;
+            if (st2 > 1.f) {
+                F3 wr{-ol.x, -ol.y, ol.z};
+                pdf = F;
+                f = (alb / fabsf(wr.z)) * F;
+                return to_world(sd.fr, wr);
+            }
+            float ctt = sqrtf(smax(1.f - st2, 0.f));
+            F3 wl{-eta * ol.x, -eta * ol.y, ctt * sg};
+            pdf = 1.f - F;
+            f = (alb / fabsf(wl.z)) * (1.f - F);
+            return to_world(sd.fr, wl);
+        }
+        float a2 = m.alpha * m.alpha;
+        float r1 = keyed_rand(key, dim++), r2 = keyed_rand(key, dim++);
+        float tm = acosf(sqrtf((1.f - r1) / (r1 * (a2 - 1.f) + 1.f)));
+        float pm = (float)(2.0 * kPiD * (double)r2);
+        F3 wm = spherical_to_world(tm, pm);
+        float F = fresnel_dielectric(dot(ol, wm), m.int_ior, m.ext_ior);
+        if (keyed_rand(key, dim++) < F) {
+            F3 wl = -ol + (wm * 2.f) * dot(wm, ol);
+            F3 wi = to_world(sd.fr, wl);
+            f = bsdf_eval(m, sd, wi, pdf);
+            return wi;
+        }
+        Frame mf = frame_from(wm);
+        F3 orl = to_local(mf, ol);
+        float ci = orl.z;
+        float eta = ci > 0.f ? m.ext_ior / m.int_ior : m.int_ior / m.ext_ior;
+        float eta2 = eta * eta;
+        float sg = ci > 0.f ? -1.f : 1.f;
+        float ci2 = smax(ci * ci, 0.f), si2 = smax(1.f - ci2, 0.f), st2 = eta2 * si2;
+        float ctt = sqrtf(smax(1.f - st2, 0.f));
+        F3 wl{-eta * orl.x, -eta * orl.y, ctt * sg};
+        F3 wi = to_world(sd.fr, to_world(mf, wl));
+        f = bsdf_eval(m, sd, wi, pdf);
+        return wi;
+    }
+    }
+}
+
+
+// Scene::visible (Scene.h:139-146): shadow ray from p1 toward p2.
+__device__ __forceinline__ void make_visible_ray(F3 p1, F3 p2, F3& o, F3& d, float& max_t) {
+    F3 dir = p2 - p1;
+    max_t = length(dir) - (2.f * kEps);
+    dir = normalize(dir);
+    o = p1 + dir * kEps;
+    d = dir;
+}
+
+
+}  // namespace pg
