@@ -145,6 +145,37 @@ pg_status pg_set_counting(pg_ctx* ctx, int32_t enable);
  * bandwidth and for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE readings.          */
 pg_status pg_copy_benchmark(pg_ctx* ctx, uint64_t bytes, uint32_t iters, double* ms_per_copy);
 
+/* ------------------------------------------------------------------ guiding cache */
+/* Records gathered by PG_INTEGRATOR_RECORD passes with params.record != 0
+ * (RayTracer::perThreadPathVertexRecords gathered after the join, Renderer.h:720-741),
+ * in (pass, path, depth) order; 48-byte PathVertex records (Renderer.h:42-47).  */
+pg_status pg_records_count(pg_ctx* ctx, uint64_t* n);
+pg_status pg_records_read(pg_ctx* ctx, pg_path_vertex* out, uint64_t cap, uint64_t* n);
+pg_status pg_records_clear(pg_ctx* ctx);
+/* Replace the record set (tests, or records kept from an earlier run). */
+pg_status pg_records_upload(pg_ctx* ctx, const pg_path_vertex* records, uint64_t n);
+/* PointBVHNode::buildPointBVHNode (Renderer.h:189-211) on the GPU: Morton keys and a
+ * radix sort, leaves of <= 8 records under an implicit balanced tree; plus one
+ * directional quadtree per cell of `cell_size` consecutive sorted records (0: 4096),
+ * refined where a quadrant holds > 1% of the cell's flux, depth <= 10, at most
+ * `qt_nodes_per_cell` nodes (0: 1024).                                          */
+pg_status pg_cache_build(pg_ctx* ctx, uint32_t cell_size, uint32_t qt_nodes_per_cell);
+/* PointBVHNode::stats (Renderer.h:219-233): records, nodes, depth, cells, build ms. */
+pg_status pg_cache_info(pg_ctx* ctx, uint64_t* n_records, uint32_t* n_nodes, uint32_t* depth,
+                        uint32_t* n_cells, double* build_ms);
+/* Export for validation (PointBVHNode::validate, Renderer.h:144-217): node boxes
+ * (8 floats each), sorted record positions (xyz + original index bits) and the
+ * quadtree pools (8 words per node: 4 flux floats, 4 child indices).            */
+pg_status pg_cache_export(pg_ctx* ctx, float* nodes, float* sorted_pos, float* quadtrees);
+/* k nearest records (k <= 16) by squared distance, ties to the lower record index;
+ * no counterpart in the reference, which has no query on its point BVH.         */
+pg_status pg_knn(pg_ctx* ctx, const float* query_xyz, uint32_t nq, uint32_t k, uint32_t* idx,
+                 float* d2);
+/* Guide distribution at points: a direction sampled with u3 and its solid-angle
+ * pdf, and the pdf of wi_eval (the guiding design of Renderer.h:398-414).      */
+pg_status pg_guide_qt_eval(pg_ctx* ctx, const float* pos, const float* u3, const float* wi_eval,
+                           uint32_t n, float* wi_out, float* pdf_sample, float* pdf_eval);
+
 /* ------------------------------------------------------------------ host-side helpers */
 /* Procedural scenes of the benchmark configurations (no reference counterpart:
  * the reference loads scene.json + .gem files, SceneLoader.h:222-269).

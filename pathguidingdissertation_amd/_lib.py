@@ -78,6 +78,15 @@ SIGNATURES = {
     "pg_stats_reset": (C.c_int32, [_P]),
     "pg_set_counting": (C.c_int32, [_P, C.c_int32]),
     "pg_copy_benchmark": (C.c_int32, [_P, C.c_uint64, C.c_uint32, C.POINTER(C.c_double)]),
+    "pg_records_count": (C.c_int32, [_P, C.POINTER(C.c_uint64)]),
+    "pg_records_read": (C.c_int32, [_P, _P, C.c_uint64, C.POINTER(C.c_uint64)]),
+    "pg_records_clear": (C.c_int32, [_P]),
+    "pg_records_upload": (C.c_int32, [_P, _P, C.c_uint64]),
+    "pg_cache_build": (C.c_int32, [_P, C.c_uint32, C.c_uint32]),
+    "pg_cache_info": (C.c_int32, [_P, C.POINTER(C.c_uint64), _U32, _U32, _U32, C.POINTER(C.c_double)]),
+    "pg_cache_export": (C.c_int32, [_P, _F, _F, _F]),
+    "pg_knn": (C.c_int32, [_P, _F, C.c_uint32, C.c_uint32, _U32, _F]),
+    "pg_guide_qt_eval": (C.c_int32, [_P, _F, _F, _F, C.c_uint32, _F, _F, _F]),
     "pg_scene_generate": (C.c_int32, [C.c_int32, C.c_uint32, C.POINTER(_P)]),
     "pg_scene_sizes": (C.c_int32, [_P, _U32, _U32, _I32, _I32]),
     "pg_scene_copy": (C.c_int32, [_P, _F, _F, _U32, C.POINTER(PgMaterial), _F, C.POINTER(PgCameraDesc)]),

@@ -2,6 +2,7 @@
 // wavefront render loop (RayTracer::render drop-in) and test queries.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cfloat>
 #include <cstring>
 #include <string>
@@ -10,6 +11,7 @@
 #include "../../include/pg_tracer.h"
 #include "host_prep.hpp"
 #include "kernels.h"
+#include "guiding.h"
 
 using namespace pg;
 
@@ -64,12 +66,30 @@ struct pg_ctx {
     bool counting = false;
     bool timing = true;
     // host-side stats
-    double ms[4] = {0, 0, 0, 0};
+    double ms[5] = {0, 0, 0, 0, 0};
     double ms_total = 0;
     uint64_t n_extend = 0, n_shadow = 0, paths = 0;
     std::vector<hipEvent_t> pool;
     size_t pool_used = 0;
     std::vector<TimedLaunch> launches;
+    // record pass (forward vertices per path, back-propagated into records)
+    RecordState rs{};
+    uint32_t rs_depth = 0, rs_paths = 0;
+    uint32_t* rec_counts = nullptr;
+    uint32_t* rec_offsets = nullptr;
+    uint32_t* scan_tmp = nullptr;
+    uint32_t* d_total = nullptr;
+    uint32_t scan_cap = 0;
+    Record* records = nullptr;
+    uint64_t n_records = 0, rec_capacity = 0;
+    // radiance cache
+    CacheBuffers cb{};
+    uint32_t cache_cap = 0, cache_pow2_cap = 0, cache_cells_cap = 0, cache_stride = 0;
+    GuideView gv{};
+    bool cache_ready = false;
+    double cache_build_ms = 0.0;
+    uint32_t* guide_cells = nullptr;
+    uint32_t guide_cells_cap = 0;
 
     pg_status fail(pg_status code, const std::string& m) {
         err = m;
@@ -93,6 +113,19 @@ struct pg_ctx {
         dfree(d_nodes); dfree(d_tris); dfree(d_shade); dfree(d_mats); dfree(d_lights);
         dfree(d_env_tex); dfree(d_env_pdf); dfree(d_env_prob); dfree(d_env_alias);
         has_scene = false;
+    }
+    void free_records() {
+        dfree(rs.vpos); dfree(rs.vnrm); dfree(rs.vwi); dfree(rs.vw); dfree(rs.ve); dfree(rs.nv);
+        rs_depth = rs_paths = 0;
+        dfree(rec_counts); dfree(rec_offsets); dfree(scan_tmp); dfree(d_total);
+        scan_cap = 0;
+    }
+    void free_cache() {
+        dfree(cb.keys); dfree(cb.vals); dfree(cb.keys_tmp); dfree(cb.vals_tmp); dfree(cb.hist); dfree(cb.scan_tmp);
+        dfree(cb.pos); dfree(cb.dir); dfree(cb.nodes); dfree(cb.qt); dfree(cb.qt_acc);
+        cache_cap = cache_pow2_cap = cache_cells_cap = cache_stride = 0;
+        cache_ready = false;
+        gv = GuideView{};
     }
     void free_paths() {
         dfree(ps.beta); dfree(ps.radiance); dfree(ps.film_pos); dfree(ps.info);
@@ -158,6 +191,10 @@ pg_status pg_destroy(pg_ctx* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     c->free_scene();
     c->free_paths();
+    c->free_records();
+    c->free_cache();
+    dfree(c->records);
+    dfree(c->guide_cells);
     dfree(c->film);
     dfree(c->d_stats);
     dfree(c->d_counters);
@@ -406,20 +443,80 @@ static pg_status ensure_paths(pg_ctx* c, uint32_t n) {
     return PG_OK;
 }
 
+static pg_status ensure_scan(pg_ctx* c, uint32_t n) {
+    uint32_t tiles = scan_tiles(n) + 1;
+    if (c->scan_cap >= tiles) return PG_OK;
+    dfree(c->scan_tmp);
+    PG_TRY(c, dalloc(&c->scan_tmp, tiles), "alloc scan");
+    if (!c->d_total) PG_TRY(c, dalloc(&c->d_total, 1), "alloc scan total");
+    c->scan_cap = tiles;
+    return PG_OK;
+}
+
+static pg_status ensure_record_state(pg_ctx* c, uint32_t n_paths, uint32_t depth) {
+    if (c->rs_paths >= n_paths && c->rs_depth >= depth) return PG_OK;
+    dfree(c->rs.vpos); dfree(c->rs.vnrm); dfree(c->rs.vwi); dfree(c->rs.vw); dfree(c->rs.ve); dfree(c->rs.nv);
+    dfree(c->rec_counts); dfree(c->rec_offsets);
+    size_t n = (size_t)n_paths * depth;
+    PG_TRY(c, dalloc(&c->rs.vpos, n), "alloc record vertices");
+    PG_TRY(c, dalloc(&c->rs.vnrm, n), "alloc record vertices");
+    PG_TRY(c, dalloc(&c->rs.vwi, n), "alloc record vertices");
+    PG_TRY(c, dalloc(&c->rs.vw, n), "alloc record vertices");
+    PG_TRY(c, dalloc(&c->rs.ve, n), "alloc record vertices");
+    PG_TRY(c, dalloc(&c->rs.nv, n_paths), "alloc record counts");
+    PG_TRY(c, dalloc(&c->rec_counts, n_paths), "alloc record counts");
+    PG_TRY(c, dalloc(&c->rec_offsets, n_paths), "alloc record offsets");
+    c->rs.stride = n_paths;
+    c->rs_paths = n_paths;
+    c->rs_depth = depth;
+    return ensure_scan(c, n_paths);
+}
+
+static pg_status ensure_record_capacity(pg_ctx* c, uint64_t need) {
+    if (need <= c->rec_capacity) return PG_OK;
+    uint64_t cap = c->rec_capacity ? c->rec_capacity : (1u << 20);
+    while (cap < need) cap *= 2;
+    if (cap > 0xffffffffull) return c->fail(PG_ERR_OVERFLOW, "record buffer exceeds 2^32 records");
+    Record* nb = nullptr;
+    PG_TRY(c, dalloc(&nb, cap), "alloc records");
+    if (c->records && c->n_records)
+        PG_TRY(c, hipMemcpyAsync(nb, c->records, c->n_records * sizeof(Record), hipMemcpyDeviceToDevice, c->stream),
+               "grow records");
+    PG_TRY(c, hipStreamSynchronize(c->stream), "sync");
+    dfree(c->records);
+    c->records = nb;
+    c->rec_capacity = cap;
+    return PG_OK;
+}
+
 pg_status pg_render_spp(pg_ctx* c, const pg_render_params* params, uint32_t n_spp) {
     PG_CHECK_CTX(c);
     if (!c->has_scene) return c->fail(PG_ERR_STATE, "no scene uploaded");
     pg_render_params prm = params ? *params : pg_render_params_default();
-    if (prm.integrator != PG_INTEGRATOR_PATH)
-        return c->fail(PG_ERR_INVALID, "integrator not available in this build");
+    const int integ = prm.integrator;
+    if (integ != PG_INTEGRATOR_PATH && integ != PG_INTEGRATOR_RECORD && integ != PG_INTEGRATOR_GUIDED_QT)
+        return c->fail(PG_ERR_INVALID, "unknown integrator");
+    if (integ == PG_INTEGRATOR_GUIDED_QT && !c->cache_ready)
+        return c->fail(PG_ERR_STATE, "guided rendering needs pg_cache_build first");
     if (prm.max_depth > 64) return c->fail(PG_ERR_INVALID, "max_depth > 64");
+    if (!(prm.guide_prob >= 0.f && prm.guide_prob <= 1.f)) return c->fail(PG_ERR_INVALID, "guide_prob outside [0,1]");
     uint32_t W = (uint32_t)c->host.camera.width, H = (uint32_t)c->host.camera.height;
     uint32_t n = W * H;
     pg_status st = ensure_paths(c, n);
     if (st != PG_OK) return st;
+    if (integ == PG_INTEGRATOR_RECORD && (st = ensure_record_state(c, n, prm.max_depth + 1)) != PG_OK) return st;
+    if (integ == PG_INTEGRATOR_GUIDED_QT && c->guide_cells_cap < n) {
+        dfree(c->guide_cells);
+        PG_TRY(c, dalloc(&c->guide_cells, (size_t)n * kKnnMax), "alloc guide cells");
+        c->guide_cells_cap = n;
+    }
     const int stack = (int)c->host.bvh.max_depth;
     const uint32_t grid_trace = trace_grid(c->cu_count, stack);
     const uint32_t grid_shade = (uint32_t)c->cu_count * 8;
+    const uint32_t grid_guide = (uint32_t)c->cu_count * 5;
+    GuideCtx gc{c->gv, integ == PG_INTEGRATOR_GUIDED_QT ? c->guide_cells : nullptr};
+    gc.g.guide_prob = prm.guide_prob;
+    const int mode = integ == PG_INTEGRATOR_GUIDED_QT ? 1 : 0;
     c->pool_used = 0;
     c->launches.clear();
     auto timed = [&](int kind, auto&& fn) -> hipError_t {
@@ -444,10 +541,35 @@ pg_status pg_render_spp(pg_ctx* c, const pg_render_params* params, uint32_t n_sp
         for (uint32_t depth = 0; depth <= prm.max_depth; ++depth) {
             int parity = (int)(depth & 1u);
             PG_TRY(c, timed(0, [&] { return launch_extend(c->dev, c->ps, parity, stack, c->counting, grid_trace, c->stream); }), "extend");
-            PG_TRY(c, timed(1, [&] { return launch_shade(c->dev, c->ps, pp, parity, (int)depth, grid_shade, c->stream); }), "shade");
-            PG_TRY(c, timed(2, [&] { return launch_shadow(c->dev, c->ps, stack, c->counting, grid_trace, c->stream); }), "shadow");
             c->n_extend++;
+            if (integ == PG_INTEGRATOR_RECORD) {
+                PG_TRY(c, timed(1, [&] { return launch_shade_record(c->dev, c->ps, pp, c->rs, parity, (int)depth, grid_shade, c->stream); }), "shade_record");
+                continue;
+            }
+            if (mode == 1)
+                PG_TRY(c, timed(4, [&] { return launch_guide_cells(c->dev, c->ps, c->gv, c->guide_cells, parity, grid_guide, c->stream); }), "guide_cells");
+            PG_TRY(c, timed(1, [&] { return launch_shade(c->dev, c->ps, pp, gc, mode, parity, (int)depth, grid_shade, c->stream); }), "shade");
+            PG_TRY(c, timed(2, [&] { return launch_shadow(c->dev, c->ps, stack, c->counting, grid_trace, c->stream); }), "shadow");
             c->n_shadow++;
+        }
+        if (integ == PG_INTEGRATOR_RECORD) {
+            // back-propagate, then append this pass's records in (path, depth) order
+            PG_TRY(c, launch_backprop_count(c->rs, n, c->rec_counts, c->stream), "backprop count");
+            PG_TRY(c, exclusive_scan(c->rec_counts, c->rec_offsets, n, c->scan_tmp, c->d_total, c->stream), "scan");
+            uint32_t total = 0;
+            PG_TRY(c, hipMemcpyAsync(&total, c->d_total, 4, hipMemcpyDeviceToHost, c->stream), "read total");
+            PG_TRY(c, hipStreamSynchronize(c->stream), "sync");
+            uint32_t cap = 0;
+            uint64_t base = c->n_records;
+            if (prm.record) {
+                st = ensure_record_capacity(c, base + total);
+                if (st != PG_OK) return st;
+                cap = (uint32_t)std::min<uint64_t>(c->rec_capacity, 0xffffffffull);
+            }
+            PG_TRY(c, launch_backprop_write(c->rs, n, c->rec_offsets, (uint32_t)base, c->records, cap,
+                                            c->ps.radiance, c->stream), "backprop write");
+            if (prm.record) c->n_records += total;
+            c->cache_ready = false;
         }
         PG_TRY(c, timed(3, [&] { return launch_resolve(c->ps, pp, c->film, c->stream); }), "resolve");
         c->spp++;
@@ -463,6 +585,181 @@ pg_status pg_render_spp(pg_ctx* c, const pg_render_params* params, uint32_t n_sp
         if (hipEventElapsedTime(&ms, l.a, l.b) == hipSuccess) c->ms[l.kind] += ms;
     }
     return PG_OK;
+}
+
+// ------------------------------------------------------------------ records and cache
+pg_status pg_records_count(pg_ctx* c, uint64_t* n) {
+    PG_CHECK_CTX(c);
+    if (n) *n = c->n_records;
+    return PG_OK;
+}
+
+pg_status pg_records_read(pg_ctx* c, pg_path_vertex* out, uint64_t cap, uint64_t* n) {
+    PG_CHECK_CTX(c);
+    uint64_t m = std::min<uint64_t>(cap, c->n_records);
+    if (m && !out) return c->fail(PG_ERR_INVALID, "null output");
+    if (m) PG_TRY(c, hipMemcpy(out, c->records, m * sizeof(Record), hipMemcpyDeviceToHost), "read records");
+    if (n) *n = m;
+    return PG_OK;
+}
+
+pg_status pg_records_clear(pg_ctx* c) {
+    PG_CHECK_CTX(c);
+    c->n_records = 0;
+    c->cache_ready = false;
+    return PG_OK;
+}
+
+pg_status pg_records_upload(pg_ctx* c, const pg_path_vertex* in, uint64_t n) {
+    PG_CHECK_CTX(c);
+    if (n && !in) return c->fail(PG_ERR_INVALID, "null records");
+    pg_status st = ensure_record_capacity(c, n);
+    if (st != PG_OK) return st;
+    if (n) PG_TRY(c, hipMemcpy(c->records, in, n * sizeof(Record), hipMemcpyHostToDevice), "upload records");
+    c->n_records = n;
+    c->cache_ready = false;
+    return PG_OK;
+}
+
+pg_status pg_cache_build(pg_ctx* c, uint32_t cell_size, uint32_t qt_nodes_per_cell) {
+    PG_CHECK_CTX(c);
+    if (!c->has_scene) return c->fail(PG_ERR_STATE, "no scene uploaded");
+    if (cell_size == 0) cell_size = 4096;
+    if (qt_nodes_per_cell == 0) qt_nodes_per_cell = 1024;
+    if (cell_size < kLeafRecords || qt_nodes_per_cell < 1)
+        return c->fail(PG_ERR_INVALID, "cell size / quadtree budget");
+    if (c->n_records > 0x7fffffffull) return c->fail(PG_ERR_OVERFLOW, "too many records");
+    uint32_t n = (uint32_t)c->n_records;
+    uint32_t leaves = (n + kLeafRecords - 1) / kLeafRecords;
+    uint32_t P = 1, levels = 0;
+    while (P < leaves) { P <<= 1; ++levels; }
+    uint32_t cells = (n + cell_size - 1) / cell_size;
+    hipEvent_t e0 = c->ev(), e1 = c->ev();
+    if (n > c->cache_cap || P > c->cache_pow2_cap || cells * qt_nodes_per_cell > c->cache_cells_cap * c->cache_stride ||
+        !c->cb.keys) {
+        c->free_cache();
+        uint32_t cap = std::max<uint32_t>(n, 1024u);
+        uint32_t tiles = sort_tiles(cap);
+        PG_TRY(c, dalloc(&c->cb.keys, cap), "alloc cache");
+        PG_TRY(c, dalloc(&c->cb.vals, cap), "alloc cache");
+        PG_TRY(c, dalloc(&c->cb.keys_tmp, cap), "alloc cache");
+        PG_TRY(c, dalloc(&c->cb.vals_tmp, cap), "alloc cache");
+        PG_TRY(c, dalloc(&c->cb.hist, (size_t)256 * tiles), "alloc cache");
+        PG_TRY(c, dalloc(&c->cb.scan_tmp, scan_tiles(256 * tiles) + 1), "alloc cache");
+        PG_TRY(c, dalloc(&c->cb.pos, cap), "alloc cache");
+        PG_TRY(c, dalloc(&c->cb.dir, cap), "alloc cache");
+        PG_TRY(c, dalloc(&c->cb.nodes, (size_t)2 * (2 * P - 1)), "alloc cache");
+        uint32_t cells_cap = std::max<uint32_t>(cells, 1u);
+        PG_TRY(c, dalloc(&c->cb.qt, (size_t)cells_cap * qt_nodes_per_cell), "alloc quadtrees");
+        PG_TRY(c, dalloc(&c->cb.qt_acc, (size_t)cells_cap * qt_nodes_per_cell * 4), "alloc quadtrees");
+        c->cache_cap = cap;
+        c->cache_pow2_cap = P;
+        c->cache_cells_cap = cells_cap;
+        c->cache_stride = qt_nodes_per_cell;
+    }
+    c->cb.leaf_pow2 = P;
+    c->cb.levels = levels;
+    // Morton frame: the scene bounds contain every record position
+    const PreparedScene& h = c->host;
+    float ext = 2.f * h.radius;
+    float3 lo = make_float3(h.centre[0] - h.radius, h.centre[1] - h.radius, h.centre[2] - h.radius);
+    float inv = ext > 0.f ? 1.f / ext : 0.f;
+    (void)hipEventRecord(e0, c->stream);
+    if (n) PG_TRY(c, launch_cache_build(c->records, n, lo, make_float3(inv, inv, inv), c->cb, cell_size,
+                                        qt_nodes_per_cell, c->stream), "cache build");
+    (void)hipEventRecord(e1, c->stream);
+    PG_TRY(c, hipStreamSynchronize(c->stream), "cache build");
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    c->cache_build_ms = ms;
+    c->pool_used = 0;
+    GuideView& g = c->gv;
+    g.n_records = n;
+    g.n_leaf_pow2 = P;
+    g.levels = levels;
+    g.cell_size = cell_size;
+    g.n_cells = cells;
+    g.qt_stride = qt_nodes_per_cell;
+    g.rec_pos = c->cb.pos;
+    g.nodes = c->cb.nodes;
+    g.qt = c->cb.qt;
+    g.guide_prob = 0.5f;
+    c->cache_ready = true;
+    return PG_OK;
+}
+
+pg_status pg_cache_info(pg_ctx* c, uint64_t* n_records, uint32_t* n_nodes, uint32_t* depth, uint32_t* n_cells,
+                        double* build_ms) {
+    PG_CHECK_CTX(c);
+    if (!c->cache_ready) return c->fail(PG_ERR_STATE, "cache not built");
+    if (n_records) *n_records = c->gv.n_records;
+    if (n_nodes) *n_nodes = 2 * c->gv.n_leaf_pow2 - 1;
+    if (depth) *depth = c->gv.levels;
+    if (n_cells) *n_cells = c->gv.n_cells;
+    if (build_ms) *build_ms = c->cache_build_ms;
+    return PG_OK;
+}
+
+pg_status pg_cache_export(pg_ctx* c, float* nodes, float* sorted_pos, float* quadtrees) {
+    PG_CHECK_CTX(c);
+    if (!c->cache_ready) return c->fail(PG_ERR_STATE, "cache not built");
+    const GuideView& g = c->gv;
+    if (nodes) PG_TRY(c, hipMemcpy(nodes, g.nodes, (size_t)(2 * g.n_leaf_pow2 - 1) * 32, hipMemcpyDeviceToHost), "export");
+    if (sorted_pos && g.n_records)
+        PG_TRY(c, hipMemcpy(sorted_pos, g.rec_pos, (size_t)g.n_records * 16, hipMemcpyDeviceToHost), "export");
+    if (quadtrees && g.n_cells)
+        PG_TRY(c, hipMemcpy(quadtrees, g.qt, (size_t)g.n_cells * g.qt_stride * sizeof(QtNode), hipMemcpyDeviceToHost),
+               "export");
+    return PG_OK;
+}
+
+pg_status pg_knn(pg_ctx* c, const float* q, uint32_t nq, uint32_t k, uint32_t* idx, float* d2) {
+    PG_CHECK_CTX(c);
+    if (!c->cache_ready) return c->fail(PG_ERR_STATE, "cache not built");
+    if (k == 0 || k > (uint32_t)kKnnMax) return c->fail(PG_ERR_INVALID, "k must be in 1..16");
+    if (nq == 0) return PG_OK;
+    float *dq = nullptr, *dd = nullptr;
+    uint32_t* di = nullptr;
+    pg_status st = PG_OK;
+    do {
+        if ((st = c->hip(dalloc(&dq, (size_t)nq * 3), "alloc")) != PG_OK) break;
+        if ((st = c->hip(dalloc(&di, (size_t)nq * k), "alloc")) != PG_OK) break;
+        if ((st = c->hip(dalloc(&dd, (size_t)nq * k), "alloc")) != PG_OK) break;
+        if ((st = c->hip(hipMemcpy(dq, q, (size_t)nq * 12, hipMemcpyHostToDevice), "copy")) != PG_OK) break;
+        if ((st = c->hip(launch_knn(c->gv, dq, nq, k, di, dd, c->stream), "knn")) != PG_OK) break;
+        if ((st = c->hip(hipStreamSynchronize(c->stream), "sync")) != PG_OK) break;
+        hipMemcpy(idx, di, (size_t)nq * k * 4, hipMemcpyDeviceToHost);
+        st = c->hip(hipMemcpy(d2, dd, (size_t)nq * k * 4, hipMemcpyDeviceToHost), "copy");
+    } while (0);
+    dfree(dq); dfree(di); dfree(dd);
+    return st;
+}
+
+pg_status pg_guide_qt_eval(pg_ctx* c, const float* pos, const float* u3, const float* wi_eval, uint32_t n,
+                           float* wi_out, float* pdf_sample, float* pdf_eval) {
+    PG_CHECK_CTX(c);
+    if (!c->cache_ready) return c->fail(PG_ERR_STATE, "cache not built");
+    if (n == 0) return PG_OK;
+    float *dp = nullptr, *du = nullptr, *dw = nullptr, *dwo = nullptr, *dps = nullptr, *dpe = nullptr;
+    pg_status st = PG_OK;
+    do {
+        if ((st = c->hip(dalloc(&dp, (size_t)n * 3), "alloc")) != PG_OK) break;
+        if ((st = c->hip(dalloc(&du, (size_t)n * 3), "alloc")) != PG_OK) break;
+        if ((st = c->hip(dalloc(&dw, (size_t)n * 3), "alloc")) != PG_OK) break;
+        if ((st = c->hip(dalloc(&dwo, (size_t)n * 3), "alloc")) != PG_OK) break;
+        if ((st = c->hip(dalloc(&dps, n), "alloc")) != PG_OK) break;
+        if ((st = c->hip(dalloc(&dpe, n), "alloc")) != PG_OK) break;
+        hipMemcpy(dp, pos, (size_t)n * 12, hipMemcpyHostToDevice);
+        hipMemcpy(du, u3, (size_t)n * 12, hipMemcpyHostToDevice);
+        if ((st = c->hip(hipMemcpy(dw, wi_eval, (size_t)n * 12, hipMemcpyHostToDevice), "copy")) != PG_OK) break;
+        if ((st = c->hip(launch_guide_eval(c->gv, dp, du, dw, n, dwo, dps, dpe, c->stream), "guide eval")) != PG_OK) break;
+        if ((st = c->hip(hipStreamSynchronize(c->stream), "sync")) != PG_OK) break;
+        hipMemcpy(wi_out, dwo, (size_t)n * 12, hipMemcpyDeviceToHost);
+        hipMemcpy(pdf_sample, dps, (size_t)n * 4, hipMemcpyDeviceToHost);
+        st = c->hip(hipMemcpy(pdf_eval, dpe, (size_t)n * 4, hipMemcpyDeviceToHost), "copy");
+    } while (0);
+    dfree(dp); dfree(du); dfree(dw); dfree(dwo); dfree(dps); dfree(dpe);
+    return st;
 }
 
 pg_status pg_film_read(pg_ctx* c, float* rgb_sum, uint32_t* spp) {
@@ -495,6 +792,7 @@ pg_status pg_stats_read(pg_ctx* c, pg_stats* s) {
     s->kernel_ms_extend = c->ms[0];
     s->kernel_ms_shade = c->ms[1];
     s->kernel_ms_shadow = c->ms[2];
+    s->records = c->n_records;
     s->kernel_ms_total = c->ms_total;
     s->extend_launches = c->n_extend;
     s->shadow_launches = c->n_shadow;

@@ -5,6 +5,7 @@
 
 #include "../../include/pg_tracer.h"
 #include "pg_common.h"
+#include "guide.h"
 
 namespace pg {
 
@@ -46,6 +47,13 @@ struct PathState {
     uint32_t capacity;
 };
 
+// Guide inputs of the shade kernel: the cache and, per queue slot, the cells of the
+// k nearest records (kKnnMax entries, 0xffffffff when unavailable).
+struct GuideCtx {
+    GuideView g;
+    const uint32_t* cells;
+};
+
 struct PassParams {
     uint32_t seed, sample;
     uint32_t width, height;
@@ -67,13 +75,18 @@ hipError_t launch_rng(uint32_t seed, uint32_t sample, const uint32_t* pixel, con
 hipError_t launch_raygen(const DevScene& s, const PathState& ps, const PassParams& p, hipStream_t stream);
 hipError_t launch_extend(const DevScene& s, const PathState& ps, int parity, int stack, bool count,
                          uint32_t grid, hipStream_t stream);
-hipError_t launch_shade(const DevScene& s, const PathState& ps, const PassParams& p, int parity, int depth,
-                        uint32_t grid, hipStream_t stream);
+hipError_t launch_shade(const DevScene& s, const PathState& ps, const PassParams& p, const GuideCtx& gc, int mode,
+                        int parity, int depth, uint32_t grid, hipStream_t stream);
 hipError_t launch_shadow(const DevScene& s, const PathState& ps, int stack, bool count, uint32_t grid,
                          hipStream_t stream);
 hipError_t launch_resolve(const PathState& ps, const PassParams& p, float4* film, hipStream_t stream);
 hipError_t launch_clear_film(float4* film, uint32_t n, hipStream_t stream);
 hipError_t launch_copy(const float4* src, float4* dst, size_t n, int cu_count, hipStream_t stream);
+struct RecordState;
+hipError_t launch_guide_cells(const DevScene& s, const PathState& ps, const GuideView& g, uint32_t* cells, int parity,
+                              uint32_t grid, hipStream_t stream);
+hipError_t launch_shade_record(const DevScene& s, const PathState& ps, const PassParams& p, const RecordState& rs,
+                               int parity, int depth, uint32_t grid, hipStream_t stream);
 int stack_entries(int max_depth);
 uint32_t trace_grid(int cu_count, int stack);
 

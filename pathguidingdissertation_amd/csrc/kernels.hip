@@ -18,6 +18,8 @@
 #include "kernels.h"
 
 #include "device_shading.h"
+#include "guide_device.h"
+#include "guiding.h"
 
 namespace pg {
 
@@ -148,8 +150,40 @@ __global__ __launch_bounds__(kBlock) void k_shadow(DevScene s, PathState ps) {
     }
 }
 
+// Mixture of the quadtrees of the k nearest records' cells at a shading point;
+// `cells` holds one cell per neighbour (unused slots 0xffffffff).
+__device__ __forceinline__ float guide_pdf_qt(const GuideView& g, const uint32_t* cells, F3 wi) {
+    float u, v;
+    dir_to_uv(wi.x, wi.y, wi.z, u, v);
+    float sum = 0.f;
+    int k = 0;
+    for (int j = 0; j < kKnnMax; ++j) {
+        uint32_t c = cells[j];
+        if (c == 0xffffffffu) break;
+        sum += qt_pdf_uv(g.qt + (size_t)c * g.qt_stride, u, v);
+        ++k;
+    }
+    return k ? sum / (float)k * 0.07957747154594767f : 0.f;
+}
+__device__ __forceinline__ F3 guide_sample_qt(const GuideView& g, const uint32_t* cells, uint32_t key, uint32_t& dim) {
+    int k = 0;
+    while (k < kKnnMax && cells[k] != 0xffffffffu) ++k;
+    float r0 = keyed_rand(key, dim++), r1 = keyed_rand(key, dim++), r2 = keyed_rand(key, dim++);
+    int j = min((int)(r0 * (float)k), k - 1);
+    float u, v, pu;
+    qt_sample_uv(g.qt + (size_t)cells[j] * g.qt_stride, r1, r2, u, v, pu);
+    F3 w;
+    uv_to_dir(u, v, w.x, w.y, w.z);
+    return w;
+}
+
 // One bounce of RayTracer::pathTraceRecursive + computeDirect (Renderer.h:269-354, 531-613).
-__global__ __launch_bounds__(kBlock) void k_shade(DevScene s, PathState ps, PassParams p, int parity, int depth) {
+// MODE 0: BSDF sampling as the reference. MODE 1: one-sample MIS between the BSDF
+// and the quadtree guide (probability guide_prob), with the combined density used in
+// every MIS weight so that the estimator stays unbiased.
+template <int MODE>
+__global__ __launch_bounds__(kBlock) void k_shade(DevScene s, PathState ps, PassParams p, GuideCtx gc, int parity,
+                                                  int depth) {
     uint32_t n = ps.counters[parity];
     const uint32_t* qin = ps.queue[parity];
     uint32_t* qout = ps.queue[parity ^ 1];
@@ -223,6 +257,9 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene s, PathState ps, Pass
                         }
                     }
                 } else {
+                    const uint32_t* cells = gc.cells + (size_t)i * kKnnMax;
+                    const bool guided = MODE == 1 && !m.specular && gc.cells != nullptr && cells[0] != 0xffffffffu;
+                    const float ag = gc.g.guide_prob;
                     // computeDirect (Renderer.h:269-354)
                     if (!m.specular && s.n_lights > 0) {
                         float pmf = 1.f / (float)s.n_lights;
@@ -249,6 +286,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene s, PathState ps, Pass
                                     float G = (ctl * ctp) / den;
                                     float pdf_b = 0.f;
                                     C3 f = bsdf_eval(m, sd, wi, pdf_b);
+                                    if (guided) pdf_b = (1.f - ag) * pdf_b + ag * guide_pdf_qt(gc.g, cells, wi);
                                     float pal = pdf_l * pmf;
                                     float pab = (pdf_b * ctp) / den;
                                     float w = power_heuristic(pal, pab);
@@ -269,6 +307,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene s, PathState ps, Pass
                                 if (!(ctl <= 0.f)) {
                                     float pdf_b = 0.f;
                                     C3 f = bsdf_eval(m, sd, wi, pdf_b);
+                                    if (guided) pdf_b = (1.f - ag) * pdf_b + ag * guide_pdf_qt(gc.g, cells, wi);
                                     float pal = pdf_l * pmf;
                                     float w = power_heuristic(pal, pdf_b);
                                     contrib = (((le * f) * ctl) * w) / pal;
@@ -294,7 +333,19 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene s, PathState ps, Pass
                     if (cont) {
                         float pdf = 0.f;
                         C3 f;
-                        F3 wi = bsdf_sample(m, sd, key, dim, f, pdf);
+                        F3 wi;
+                        if (guided) {
+                            if (keyed_rand(key, dim++) < ag) {
+                                wi = guide_sample_qt(gc.g, cells, key, dim);
+                                f = bsdf_eval(m, sd, wi, pdf);
+                                pdf = (1.f - ag) * pdf + ag * guide_pdf_qt(gc.g, cells, wi);
+                            } else {
+                                wi = bsdf_sample(m, sd, key, dim, f, pdf);
+                                if (!(pdf <= 0.f)) pdf = (1.f - ag) * pdf + ag * guide_pdf_qt(gc.g, cells, wi);
+                            }
+                        } else {
+                            wi = bsdf_sample(m, sd, key, dim, f, pdf);
+                        }
                         if (!(pdf <= 0.f)) {
                             float sg = (dot(wi, sd.gn) >= 0.f) ? 1.f : -1.f;
                             float ctw = fabsf(dot(wi, sd.sn));
@@ -328,6 +379,134 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene s, PathState ps, Pass
             ps.sh_o[sslot] = make_float4(sh_o.x, sh_o.y, sh_o.z, sh_t);
             ps.sh_d[sslot] = make_float4(sh_d.x, sh_d.y, sh_d.z, __uint_as_float(path));
             ps.sh_c[sslot] = make_float4(sh_c.r, sh_c.g, sh_c.b, 0.f);
+        }
+        uint32_t rslot = wave_append(ps.counters + (parity ^ 1), want_ray);
+        if (want_ray) {
+            qout[rslot] = path;
+            ps.ray_o[parity ^ 1][rslot] = make_float4(nr_o.x, nr_o.y, nr_o.z, 0.f);
+            ps.ray_d[parity ^ 1][rslot] = make_float4(nr_d.x, nr_d.y, nr_d.z, 0.f);
+        }
+    }
+}
+
+// Cells of the k nearest records for every shading point of this bounce.
+constexpr int kGuideStack = 32;
+__global__ __launch_bounds__(kBlock) void k_guide_cells(DevScene s, PathState ps, GuideView g, uint32_t* cells,
+                                                        int parity) {
+    __shared__ int sn[kGuideStack * kBlock];
+    __shared__ float sd_[kGuideStack * kBlock];
+    uint32_t n = ps.counters[parity];
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+        float4 h = ps.hit[i];
+        uint32_t hid = __float_as_uint(h.x);
+        uint32_t* out = cells + (size_t)i * kKnnMax;
+        bool ok = hid != kMiss && g.n_records > 0;
+        if (ok) {
+            const MaterialRec& m = s.mats[s.shade[hid].mat];
+            ok = !m.specular && !m.is_light;
+        }
+        if (!ok) {
+            out[0] = 0xffffffffu;
+            continue;
+        }
+        float4 ro = ps.ray_o[parity][i], rdv = ps.ray_d[parity][i];
+        F3 x = F3{ro.x, ro.y, ro.z} + F3{rdv.x, rdv.y, rdv.z} * h.y;
+        float bd[kKnnMax];
+        uint32_t bs[kKnnMax];
+        knn_query<kKnnMax>(g, x, bd, bs, sn + threadIdx.x, sd_ + threadIdx.x, kBlock);
+#pragma unroll
+        for (int j = 0; j < kKnnMax; ++j) out[j] = bs[j] == 0xffffffffu ? 0xffffffffu : bs[j] / g.cell_size;
+    }
+}
+
+// One bounce of the record pass, RayTracer::guidedPathRecursive (Renderer.h:397-528):
+// BSDF-sampled continuation with a fixed Russian-roulette probability 0.7 after
+// depth rr_depth, one forward vertex {x, n, wi, f cos/(pdf rrp), emission, store}
+// per bounce. computeDirect's random numbers are drawn as the reference draws them,
+// but its value does not enter the record pass's result, so no shadow ray is traced.
+__global__ __launch_bounds__(kBlock) void k_shade_record(DevScene s, PathState ps, PassParams p, RecordState rs,
+                                                         int parity, int depth) {
+    uint32_t n = ps.counters[parity];
+    const uint32_t* qin = ps.queue[parity];
+    uint32_t* qout = ps.queue[parity ^ 1];
+    for (uint32_t base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock) {
+        uint32_t i = base + threadIdx.x;
+        bool active = i < n;
+        bool want_ray = false;
+        F3 nr_o{0.f, 0.f, 0.f}, nr_d{0.f, 0.f, 0.f};
+        uint32_t path = 0;
+        if (active) {
+            path = qin[i];
+            float4 h = ps.hit[i];
+            uint32_t hid = __float_as_uint(h.x);
+            float ht = h.y, hu = h.z, hv = h.w;
+            float4 ro = ps.ray_o[parity][i], rdv = ps.ray_d[parity][i];
+            F3 rd{rdv.x, rdv.y, rdv.z};
+            uint4 inf = ps.info[path];
+            uint32_t key = inf.x, dim = inf.y;
+            float4 vpos = make_float4(0.f, 0.f, 0.f, 0.f), vnrm = vpos, vwi = vpos, vw = vpos, ve = vpos;
+            if (hid == kMiss) {
+                C3 bg = s.has_env ? env_eval(s, rd) : c3(0.f, 0.f, 0.f);
+                ve = make_float4(bg.r, bg.g, bg.b, 0.f);
+            } else {
+                const TriShade& tsh = s.shade[hid];
+                const MaterialRec& m = s.mats[tsh.mat];
+                SD sd;
+                F3 o{ro.x, ro.y, ro.z};
+                sd.x = o + rd * ht;
+                sd.gn = ld3(tsh.gn);
+                float ha = 1.f - (hu + hv);
+                sd.sn = normalize((ld3(tsh.n0) * ha + ld3(tsh.n1) * hu) + ld3(tsh.n2) * hv);
+                sd.wo = -rd;
+                if (m.two_sided) {
+                    if (dot(sd.wo, sd.sn) < 0.f) sd.sn = -sd.sn;
+                    if (dot(sd.wo, sd.gn) < 0.f) sd.gn = -sd.gn;
+                }
+                sd.fr = frame_from(sd.sn);
+                vpos = make_float4(sd.x.x, sd.x.y, sd.x.z, 0.f);
+                vnrm = make_float4(sd.sn.x, sd.sn.y, sd.sn.z, 0.f);
+                if (m.is_light) {
+                    ve = make_float4(m.emission[0], m.emission[1], m.emission[2], 0.f);
+                } else {
+                    if (!m.specular && s.n_lights > 0) {
+                        uint32_t nl = (uint32_t)s.n_lights;
+                        float fl = floorf(keyed_rand(key, dim++) * (float)nl);
+                        uint32_t li = min(max(0u, (uint32_t)fl), nl - 1u);
+                        dim += (s.lights[li].kind == 0) ? 2u : 4u;
+                    }
+                    bool cont = true;
+                    float rr = 1.f;
+                    if (depth > (int)p.rr_depth) {
+                        if (keyed_rand(key, dim++) < 0.7f) rr = 0.7f;
+                        else cont = false;
+                    }
+                    if (cont && depth == (int)p.max_depth) cont = false;
+                    if (cont) {
+                        float pdf = 0.f;
+                        C3 f;
+                        F3 wi = bsdf_sample(m, sd, key, dim, f, pdf);
+                        float ctw = fabsf(dot(wi, sd.sn));
+                        if (!(pdf <= 0.f || ctw <= 0.f)) {
+                            C3 w = (f * ctw) / (pdf * rr);
+                            vwi = make_float4(wi.x, wi.y, wi.z, 0.f);
+                            vw = make_float4(w.r, w.g, w.b, 0.f);
+                            vpos.w = m.specular ? 0.f : 1.f;
+                            float sg = (dot(wi, sd.gn) >= 0.f) ? 1.f : -1.f;
+                            nr_o = sd.x + sd.gn * (kEps * sg);
+                            nr_d = wi;
+                            want_ray = true;
+                        }
+                    }
+                }
+            }
+            size_t o = (size_t)depth * rs.stride + path;
+            rs.vpos[o] = vpos;
+            rs.vnrm[o] = vnrm;
+            rs.vwi[o] = vwi;
+            rs.vw[o] = vw;
+            rs.ve[o] = ve;
+            if (want_ray) ps.info[path] = make_uint4(key, dim, 0u, inf.w);
+            else rs.nv[path] = (uint32_t)depth + 1u;
         }
         uint32_t rslot = wave_append(ps.counters + (parity ^ 1), want_ray);
         if (want_ray) {
@@ -440,9 +619,10 @@ hipError_t launch_extend(const DevScene& s, const PathState& ps, int parity, int
 #undef PG_CALL
     return hipGetLastError();
 }
-hipError_t launch_shade(const DevScene& s, const PathState& ps, const PassParams& p, int parity, int depth,
-                        uint32_t grid, hipStream_t stream) {
-    hipLaunchKernelGGL(k_shade, dim3(grid), dim3(kBlock), 0, stream, s, ps, p, parity, depth);
+hipError_t launch_shade(const DevScene& s, const PathState& ps, const PassParams& p, const GuideCtx& gc, int mode,
+                        int parity, int depth, uint32_t grid, hipStream_t stream) {
+    if (mode == 1) hipLaunchKernelGGL((k_shade<1>), dim3(grid), dim3(kBlock), 0, stream, s, ps, p, gc, parity, depth);
+    else hipLaunchKernelGGL((k_shade<0>), dim3(grid), dim3(kBlock), 0, stream, s, ps, p, gc, parity, depth);
     return hipGetLastError();
 }
 hipError_t launch_shadow(const DevScene& s, const PathState& ps, int stack, bool count, uint32_t grid,
@@ -452,6 +632,17 @@ hipError_t launch_shadow(const DevScene& s, const PathState& ps, int stack, bool
     else hipLaunchKernelGGL((k_shadow<false, S>), dim3(grid), dim3(kBlock), 0, stream, s, ps)
     PG_STACK_DISPATCH(stack, PG_CALL);
 #undef PG_CALL
+    return hipGetLastError();
+}
+
+hipError_t launch_guide_cells(const DevScene& s, const PathState& ps, const GuideView& g, uint32_t* cells, int parity,
+                              uint32_t grid, hipStream_t stream) {
+    hipLaunchKernelGGL(k_guide_cells, dim3(grid), dim3(kBlock), 0, stream, s, ps, g, cells, parity);
+    return hipGetLastError();
+}
+hipError_t launch_shade_record(const DevScene& s, const PathState& ps, const PassParams& p, const RecordState& rs,
+                               int parity, int depth, uint32_t grid, hipStream_t stream) {
+    hipLaunchKernelGGL(k_shade_record, dim3(grid), dim3(kBlock), 0, stream, s, ps, p, rs, parity, depth);
     return hipGetLastError();
 }
 

@@ -105,6 +105,61 @@ class Device:
     def _chk(self, code: int):
         check(code, self.ctx)
 
+    # ---------------------------------------------------------------- guiding cache
+    def records(self) -> np.ndarray:
+        """PathVertex records (n, 12) float32: pos, normal, wi, Li."""
+        n = C.c_uint64()
+        self._chk(self.lib.pg_records_count(self.ctx, C.byref(n)))
+        out = np.empty((n.value, 12), np.float32)
+        got = C.c_uint64()
+        self._chk(self.lib.pg_records_read(self.ctx, out.ctypes.data_as(C.c_void_p), n.value, C.byref(got)))
+        return out[: got.value]
+
+    def upload_records(self, rec: np.ndarray):
+        rec = np.ascontiguousarray(rec, np.float32).reshape(-1, 12)
+        self._chk(self.lib.pg_records_upload(self.ctx, rec.ctypes.data_as(C.c_void_p), rec.shape[0]))
+
+    def clear_records(self):
+        self._chk(self.lib.pg_records_clear(self.ctx))
+
+    def cache_build(self, cell_size: int = 4096, qt_nodes: int = 1024):
+        self._chk(self.lib.pg_cache_build(self.ctx, cell_size, qt_nodes))
+        return self.cache_info()
+
+    def cache_info(self) -> dict:
+        n = C.c_uint64()
+        nodes, depth, cells = C.c_uint32(), C.c_uint32(), C.c_uint32()
+        ms = C.c_double()
+        self._chk(self.lib.pg_cache_info(self.ctx, C.byref(n), C.byref(nodes), C.byref(depth), C.byref(cells),
+                                         C.byref(ms)))
+        return {"records": n.value, "nodes": nodes.value, "depth": depth.value, "cells": cells.value,
+                "build_ms": ms.value}
+
+    def cache_export(self, qt_nodes: int = 1024):
+        info = self.cache_info()
+        nodes = np.empty((info["nodes"], 8), np.float32)
+        pos = np.empty((info["records"], 4), np.float32)
+        qt = np.empty((max(info["cells"], 1) * qt_nodes, 8), np.float32)
+        self._chk(self.lib.pg_cache_export(self.ctx, _fp(nodes), _fp(pos), _fp(qt)))
+        return nodes, pos, qt
+
+    def knn(self, q: np.ndarray, k: int = 16):
+        q = np.ascontiguousarray(q, np.float32).reshape(-1, 3)
+        idx = np.empty((q.shape[0], k), np.uint32)
+        d2 = np.empty((q.shape[0], k), np.float32)
+        self._chk(self.lib.pg_knn(self.ctx, _fp(q), q.shape[0], k, _up(idx), _fp(d2)))
+        return idx, d2
+
+    def guide_eval(self, pos: np.ndarray, u3: np.ndarray, wi_eval: np.ndarray):
+        pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 3)
+        u3 = np.ascontiguousarray(u3, np.float32).reshape(-1, 3)
+        we = np.ascontiguousarray(wi_eval, np.float32).reshape(-1, 3)
+        n = pos.shape[0]
+        wi = np.empty((n, 3), np.float32)
+        ps, pe = np.empty(n, np.float32), np.empty(n, np.float32)
+        self._chk(self.lib.pg_guide_qt_eval(self.ctx, _fp(pos), _fp(u3), _fp(we), n, _fp(wi), _fp(ps), _fp(pe)))
+        return wi, ps, pe
+
     def info(self) -> dict:
         name = C.create_string_buffer(64)
         cu = C.c_int32()

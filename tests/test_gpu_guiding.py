@@ -1,0 +1,229 @@
+"""GPU record pass (RayTracer::guidedPath, Renderer.h:367-529), point-BVH radiance
+cache (Renderer.h:71-234), exact kNN and per-cell quadtree guide, and the guided
+integrator, checked against the CPU oracle, brute force and closed-form invariants."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+INVALID = 0xFFFFFFFF
+
+
+def _records_gpu(pg, device, data, spp, seed, max_depth=8):
+    sc = pg.Scene(device, data)
+    device.clear_records()
+    rt = pg.RayTracer(sc, seed=seed, max_depth=max_depth, integrator=1, record=1)
+    rt.clear()
+    rt.render(spp)
+    return sc, rt, device.records()
+
+
+def test_record_pass_matches_oracle(pg, device, cornell):
+    """Same random stream: the record-pass film (G_0, the 'incoming radiance' image)
+    and the multiset of PathVertex records agree with the oracle's."""
+    from oracle.pg_oracle import OracleScene
+    sc, rt, rec = _records_gpu(pg, device, cornell, spp=1, seed=7)
+    film, _ = rt.film()
+    osc = OracleScene(cornell)
+    ref_film, st = osc.render(1, seed=7, integrator=1, record=True, accum_f64=False, env_mode=1)
+    orec = osc.records()
+    assert abs(rec.shape[0] - orec.shape[0]) <= max(5, orec.shape[0] // 500), (rec.shape, orec.shape)
+    rel = np.abs(film - ref_film) / (np.abs(ref_film) + 1e-3)
+    assert np.mean(np.all(rel < 1e-3, axis=-1)) > 0.97
+    # multiset comparison on positions (rounded) and Li
+    def keyed(a):
+        k = np.round(a[:, :3].astype(np.float64), 4)
+        order = np.lexsort((k[:, 2], k[:, 1], k[:, 0]))
+        return a[order]
+    g, o = keyed(rec), keyed(orec[:, :12])
+    m = min(g.shape[0], o.shape[0])
+    close = np.all(np.abs(g[:m] - o[:m]) <= 1e-3 * (np.abs(o[:m]) + 1e-2), axis=1)
+    assert close.mean() > 0.95, close.mean()
+    # every stored record carries finite, positive-luminance Li (Renderer.h:388)
+    lum = 0.2126 * rec[:, 9] + 0.7152 * rec[:, 10] + 0.0722 * rec[:, 11]
+    assert np.all(np.isfinite(rec)) and np.all(lum > 0)
+
+
+def _random_records(n, seed, dup=0):
+    rng = np.random.default_rng(seed)
+    rec = np.zeros((n, 12), np.float32)
+    rec[:, :3] = rng.uniform(-5, 5, (n, 3))
+    if dup:
+        rec[-dup:, :3] = rec[:dup, :3]  # exact duplicates: distance ties resolved by index
+    nrm = rng.normal(size=(n, 3))
+    rec[:, 3:6] = nrm / np.linalg.norm(nrm, axis=1, keepdims=True)
+    w = rng.normal(size=(n, 3))
+    rec[:, 6:9] = w / np.linalg.norm(w, axis=1, keepdims=True)
+    rec[:, 9:12] = rng.exponential(1.0, (n, 3)) * (1 + 100 * (rng.random((n, 1)) < 0.01))
+    return rec
+
+
+def test_point_bvh_invariants(pg, device, cornell):
+    """PointBVHNode::validateNode / validate (Renderer.h:144-217): leaves hold at most 8
+    records, every record lies in its leaf box, children lie in their parent, and no
+    record is lost or duplicated."""
+    pg.Scene(device, cornell)
+    rec = _random_records(100_003, seed=1, dup=100)
+    device.upload_records(rec)
+    info = device.cache_build(cell_size=4096)
+    nodes, pos, _ = device.cache_export()
+    n = rec.shape[0]
+    P = (info["nodes"] + 1) // 2
+    assert P >= (n + 7) // 8 and P & (P - 1) == 0
+    order = pos[:, 3].view(np.uint32)
+    assert np.array_equal(np.sort(order), np.arange(n, dtype=np.uint32))
+    assert np.array_equal(pos[:, :3], rec[order, :3])
+    leaf = np.arange(n) // 8
+    lo, hi = nodes[P - 1 + leaf, :3], nodes[P - 1 + leaf, 4:7]
+    assert np.all(pos[:, :3] >= lo) and np.all(pos[:, :3] <= hi)
+    for k in range(P - 1):
+        for c in (2 * k + 1, 2 * k + 2):
+            if np.all(np.isfinite(nodes[c, :3])):
+                assert np.all(nodes[c, :3] >= nodes[k, :3]) and np.all(nodes[c, 4:7] <= nodes[k, 4:7])
+        if k > 4096:
+            break
+
+
+@pytest.mark.parametrize("n,dup", [(1000, 0), (100_003, 200)])
+def test_knn_matches_brute_force(pg, device, cornell, n, dup):
+    from oracle import pg_oracle as po
+    pg.Scene(device, cornell)
+    rec = _random_records(n, seed=n, dup=dup)
+    device.upload_records(rec)
+    device.cache_build()
+    rng = np.random.default_rng(3)
+    q = rng.uniform(-6, 6, (4096, 3)).astype(np.float32)
+    q[:64] = rec[:64, :3]  # queries on records
+    idx, d2 = device.knn(q, 16)
+    pts = np.ascontiguousarray(rec[:, :3])
+    bidx = np.empty((q.shape[0], 16), np.uint32)
+    bd2 = np.empty((q.shape[0], 16), np.float32)
+    po.lib().orc_knn_brute(po.fp(pts), n, po.fp(q), q.shape[0], 16, po.up(bidx), po.fp(bd2), 0)
+    assert np.array_equal(idx, bidx)
+    assert np.array_equal(d2.view(np.uint32), bd2.view(np.uint32))
+
+
+def test_knn_fewer_records_than_k(pg, device, cornell):
+    pg.Scene(device, cornell)
+    rec = _random_records(5, seed=9)
+    device.upload_records(rec)
+    device.cache_build()
+    idx, d2 = device.knn(np.zeros((3, 3), np.float32), 16)
+    assert np.all(idx[:, 5:] == INVALID)
+    assert sorted(idx[0, :5].tolist()) == list(range(5))
+
+
+def _qt_pdf(nodes, u, v):
+    """Quadtree pdf on [0,1)^2 — test-side restatement of the lookup."""
+    pdf = np.ones_like(u, dtype=np.float64)
+    node = np.zeros(u.shape, np.int64)
+    ox, oy, size = np.zeros_like(u), np.zeros_like(u), np.ones_like(u)
+    alive = np.ones(u.shape, bool)
+    e = nodes[:, :4].astype(np.float64)
+    ch = nodes[:, 4:8].view(np.int32)
+    for _ in range(11):
+        E = e[node]
+        tot = E.sum(1)
+        half = size * 0.5
+        qx = (u >= ox + half).astype(np.int64)
+        qy = (v >= oy + half).astype(np.int64)
+        q = qx + 2 * qy
+        ok = alive & (tot > 0)
+        pdf = np.where(ok, pdf * 4 * E[np.arange(len(q)), q] / np.where(tot > 0, tot, 1), pdf)
+        c = ch[node, q]
+        alive = ok & (c >= 0)
+        node = np.where(alive, c, node)
+        ox, oy = ox + qx * half, oy + qy * half
+        size = np.where(alive, half, size)
+    return pdf
+
+
+def test_quadtree_invariants(pg, device, cornell):
+    """Each cell's quadtree pdf integrates to 1 over [0,1)^2 (exact on the depth-10
+    grid), refined quadrants hold more than 1% of the cell's flux, depth <= 10."""
+    pg.Scene(device, cornell)
+    rec = _random_records(20000, seed=5)
+    # concentrate flux in a few directions so the trees refine
+    rec[::3, 6:9] = np.array([0.3, 0.4, 0.866], np.float32)
+    rec[::3, 9:12] *= 50
+    device.upload_records(rec)
+    info = device.cache_build(cell_size=4096, qt_nodes=1024)
+    _, pos, qt = device.cache_export(qt_nodes=1024)
+    qt = qt.reshape(info["cells"], 1024, 8)
+    g = (np.arange(1024) + 0.5) / 1024
+    uu, vv = np.meshgrid(g, g)
+    for c in range(info["cells"]):
+        nodes = qt[c]
+        pdf = _qt_pdf(nodes, uu.ravel(), vv.ravel())
+        assert abs(pdf.mean() - 1.0) < 1e-4, (c, pdf.mean())
+        E = nodes[:, :4].astype(np.float64)
+        ch = nodes[:, 4:8].view(np.int32)
+        total = E[0].sum()
+        used = np.unique(np.concatenate([[0], ch[ch >= 0]]))
+        for nd in used:
+            for q in range(4):
+                if ch[nd, q] >= 0:
+                    assert E[nd, q] > 0.01 * total * (1 - 1e-6)
+                    assert abs(E[ch[nd, q]].sum() - E[nd, q]) <= 1e-5 * total
+
+
+def test_guide_sampling_consistent_with_pdf(pg, device, cornell):
+    pg.Scene(device, cornell)
+    rec = _random_records(8000, seed=11)
+    rec[::2, 6:9] = np.array([0.0, 0.6, 0.8], np.float32)
+    rec[::2, 9:12] *= 20
+    device.upload_records(rec)
+    device.cache_build(cell_size=8192)
+    n = 200_000
+    rng = np.random.default_rng(2)
+    pos = np.tile(rec[:1, :3], (n, 1))
+    u3 = rng.random((n, 3)).astype(np.float32)
+    wi, ps, pe = device.guide_eval(pos, u3, np.tile(np.array([[0, 0, 1]], np.float32), (n, 1)))
+    assert np.allclose(np.linalg.norm(wi, axis=1), 1.0, atol=1e-5)
+    # sampling pdf equals the evaluated pdf of the sampled direction
+    _, ps2, pe2 = device.guide_eval(pos[:4096], u3[:4096], wi[:4096])
+    assert np.allclose(pe2, ps[:4096], rtol=1e-5)
+    # histogram in (u, v) against the pdf: chi-square over 16x16 bins
+    u = (np.arctan2(wi[:, 1], wi[:, 0]) % (2 * np.pi)) / (2 * np.pi)
+    v = (wi[:, 2] + 1) / 2
+    H, _, _ = np.histogram2d(u, v, bins=16, range=[[0, 1], [0, 1]])
+    # expected mass per bin from the pdf evaluated on a fine grid inside each bin
+    g = (np.arange(64) + 0.5) / 64
+    uu, vv = np.meshgrid(g, g, indexing="ij")
+    dirs = np.stack([np.sqrt(1 - (2 * vv - 1) ** 2) * np.cos(2 * np.pi * uu),
+                     np.sqrt(1 - (2 * vv - 1) ** 2) * np.sin(2 * np.pi * uu), 2 * vv - 1], -1).reshape(-1, 3)
+    _, _, pg_ = device.guide_eval(np.tile(rec[:1, :3], (dirs.shape[0], 1)), np.zeros_like(dirs), dirs)
+    mass = (pg_ * 4 * np.pi).reshape(64, 64).reshape(16, 4, 16, 4).mean((1, 3)) / 256
+    exp = mass * n
+    ok = exp > 20
+    chi2 = np.sum((H[ok] - exp[ok]) ** 2 / exp[ok])
+    dof = ok.sum() - 1
+    assert chi2 < dof + 6 * np.sqrt(2 * dof), (chi2, dof)
+
+
+def test_guided_render_is_unbiased_and_helps(pg, device):
+    """Closed room lit through one window (config-3 layout, small floor): the guided
+    integrator converges to the unguided one's image and is less noisy per sample."""
+    data = pg.generate_scene(101, 0)
+    sc = pg.Scene(device, data)
+    device.clear_records()
+    rec_rt = pg.RayTracer(sc, seed=1, max_depth=12, integrator=1, record=1)
+    rec_rt.clear()
+    rec_rt.render(1)
+    info = device.cache_build()
+    assert info["records"] > 100_000
+    ref_rt = pg.RayTracer(sc, seed=100, max_depth=12)
+    ref_rt.clear()
+    ref_rt.render(256)
+    ref = ref_rt.image()
+    un = pg.RayTracer(sc, seed=2, max_depth=12)
+    un.clear()
+    un.render(16)
+    gd = pg.RayTracer(sc, seed=3, max_depth=12, integrator=2, guide_prob=0.5)
+    gd.clear()
+    gd.render(16)
+    e_un = pg.relmse(un.image(), ref)
+    e_gd = pg.relmse(gd.image(), ref)
+    m_ref, m_gd = ref.mean(), gd.image().mean()
+    assert abs(m_gd - m_ref) / m_ref < 0.03, (m_gd, m_ref)
+    assert e_gd < e_un, (e_gd, e_un)
