@@ -76,7 +76,10 @@ typedef struct {
     uint32_t record;        /* RECORD integrator: keep PathVertex records       */
     float guide_prob;       /* guided integrators: P(sample guide), 0.5         */
     uint32_t knn_k;         /* guided QT: neighbours per lookup (16)            */
-    uint32_t reserved[9];
+    uint32_t record_nee;    /* RECORD: 0 records Li = G (Renderer.h:390, the
+                               reference); 1 records Li with NEE and MIS-weighted
+                               emission (the pixelColour recursion, :391)       */
+    uint32_t reserved[8];
 } pg_render_params;
 
 typedef struct {

@@ -1183,20 +1183,23 @@ struct Scene {
         recs.push_back(rec);
     }
 
+    // withNee == false: records and the return value use G (Renderer.h:390, 393);
+    // withNee == true: both use the pixelColour recursion (Renderer.h:391, 394).
     C3 guidedPath(const Ray& r, Ctx& c, int maxDepth, int rrDepth, std::vector<RecordOut>* out,
-                  uint32_t pixel, uint32_t sample) const {                     // Renderer.h:368-395
+                  uint32_t pixel, uint32_t sample, bool withNee = false) const {  // Renderer.h:368-395
         std::vector<FwdRec> recs;
         recs.reserve(16);
         guidedRec(r, 0, c, recs, 0.f, false, maxDepth, rrDepth);
         C3 Li, px;
         for (int i = (int)recs.size() - 1; i >= 0; --i) {
             if (recs[i].store && out) {
-                if (Li.valid() && Li.lum() > 0) {
+                const C3 L = withNee ? px : Li;
+                if (L.valid() && L.lum() > 0) {
                     RecordOut o;
                     o.v[0] = recs[i].x.x; o.v[1] = recs[i].x.y; o.v[2] = recs[i].x.z;
                     o.v[3] = recs[i].n.x; o.v[4] = recs[i].n.y; o.v[5] = recs[i].n.z;
                     o.v[6] = recs[i].wi.x; o.v[7] = recs[i].wi.y; o.v[8] = recs[i].wi.z;
-                    o.v[9] = Li.r; o.v[10] = Li.g; o.v[11] = Li.b;
+                    o.v[9] = L.r; o.v[10] = L.g; o.v[11] = L.b;
                     o.v[12] = (float)pixel; o.v[13] = (float)sample; o.v[14] = (float)i;
                     out->push_back(o);
                 }
@@ -1204,7 +1207,7 @@ struct Scene {
             Li = recs[i].em + (recs[i].w * Li);
             px = (recs[i].misEm + recs[i].direct) + (recs[i].w * px);
         }
-        return Li;
+        return withNee ? px : Li;
     }
 };
 
@@ -1371,8 +1374,9 @@ int orc_render(orc_scene* os, const orc_render_params* p, double* film, orc_rend
     if (!p->accum_f64) ffilm.assign((size_t)W * H * 3, 0.f);
     struct Slot { float px, py; C3 L; bool ok; };
     std::vector<Slot> slots((size_t)W * H);
-    std::vector<std::vector<RecordOut>> tileRecs(p->integrator == 1 && p->record ? ntiles : 0);
-    if (p->integrator == 1 && p->record) sc.records.clear();
+    const bool recordPass = p->integrator == 1 || p->integrator == 2;
+    std::vector<std::vector<RecordOut>> tileRecs(recordPass && p->record ? ntiles : 0);
+    if (recordPass && p->record) sc.records.clear();
     std::vector<std::unique_ptr<MTSampler>> mts;
     if (p->sampler == 1) for (int i = 0; i < nt; ++i) mts.emplace_back(new MTSampler(p->seed));
     std::atomic<uint64_t> nc{0}, ns{0};
@@ -1399,9 +1403,10 @@ int orc_render(orc_scene* os, const orc_render_params* p, double* film, orc_rend
                             float py = y + smp->next();
                             Ray ray = sc.generateRay(px, py);
                             C3 col;
-                            if (p->integrator == 1)
+                            if (recordPass)
                                 col = sc.guidedPath(ray, c, (int)p->max_depth, (int)p->rr_depth,
-                                                    p->record ? &tileRecs[tid] : nullptr, pix, sample);
+                                                    p->record ? &tileRecs[tid] : nullptr, pix, sample,
+                                                    p->integrator == 2);
                             else {
                                 C3 beta(1.f, 1.f, 1.f);
                                 col = sc.pathTrace(ray, beta, 0, c, 0.f, false, (int)p->max_depth, (int)p->rr_depth);
@@ -1446,7 +1451,7 @@ int orc_render(orc_scene* os, const orc_render_params* p, double* film, orc_rend
     auto t1 = std::chrono::steady_clock::now();
     if (!p->accum_f64)
         for (size_t i = 0; i < ffilm.size(); ++i) film[i] += ffilm[i];
-    if (p->integrator == 1 && p->record)
+    if (recordPass && p->record)
         for (auto& v : tileRecs) sc.records.insert(sc.records.end(), v.begin(), v.end());
     if (st) {
         st->closest_rays = nc; st->shadow_rays = ns;

@@ -60,7 +60,8 @@ typedef struct {
     uint32_t seed;
     uint32_t max_depth;    /* Renderer.h:568 uses 8                            */
     uint32_t rr_depth;     /* Russian roulette when depth > rr_depth (3)       */
-    int32_t integrator;    /* 0 path tracer (NEE+MIS), 1 guided record pass    */
+    int32_t integrator;    /* 0 path tracer (NEE+MIS), 1 guided record pass,
+                              2 record pass whose Li carries NEE (pixelColour) */
     int32_t sampler;       /* 0 counter hash, 1 std::mt19937 per thread        */
     int32_t accum_f64;     /* 1: film sum in FP64                               */
     int32_t n_threads;     /* 0: hardware_concurrency                           */

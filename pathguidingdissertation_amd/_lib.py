@@ -38,7 +38,7 @@ class PgCameraDesc(C.Structure):
 class PgRenderParams(C.Structure):
     _fields_ = [("seed", C.c_uint32), ("max_depth", C.c_uint32), ("rr_depth", C.c_uint32),
                 ("integrator", C.c_int32), ("record", C.c_uint32), ("guide_prob", C.c_float),
-                ("knn_k", C.c_uint32), ("reserved", C.c_uint32 * 9)]
+                ("knn_k", C.c_uint32), ("record_nee", C.c_uint32), ("reserved", C.c_uint32 * 8)]
 
 
 class PgStats(C.Structure):

@@ -115,7 +115,8 @@ struct pg_ctx {
         has_scene = false;
     }
     void free_records() {
-        dfree(rs.vpos); dfree(rs.vnrm); dfree(rs.vwi); dfree(rs.vw); dfree(rs.ve); dfree(rs.nv);
+        dfree(rs.vpos); dfree(rs.vnrm); dfree(rs.vwi); dfree(rs.vw); dfree(rs.ve); dfree(rs.vme); dfree(rs.vd);
+        dfree(rs.nv);
         rs_depth = rs_paths = 0;
         dfree(rec_counts); dfree(rec_offsets); dfree(scan_tmp); dfree(d_total);
         scan_cap = 0;
@@ -455,7 +456,8 @@ static pg_status ensure_scan(pg_ctx* c, uint32_t n) {
 
 static pg_status ensure_record_state(pg_ctx* c, uint32_t n_paths, uint32_t depth) {
     if (c->rs_paths >= n_paths && c->rs_depth >= depth) return PG_OK;
-    dfree(c->rs.vpos); dfree(c->rs.vnrm); dfree(c->rs.vwi); dfree(c->rs.vw); dfree(c->rs.ve); dfree(c->rs.nv);
+    dfree(c->rs.vpos); dfree(c->rs.vnrm); dfree(c->rs.vwi); dfree(c->rs.vw); dfree(c->rs.ve); dfree(c->rs.vme);
+    dfree(c->rs.vd); dfree(c->rs.nv);
     dfree(c->rec_counts); dfree(c->rec_offsets);
     size_t n = (size_t)n_paths * depth;
     PG_TRY(c, dalloc(&c->rs.vpos, n), "alloc record vertices");
@@ -463,6 +465,8 @@ static pg_status ensure_record_state(pg_ctx* c, uint32_t n_paths, uint32_t depth
     PG_TRY(c, dalloc(&c->rs.vwi, n), "alloc record vertices");
     PG_TRY(c, dalloc(&c->rs.vw, n), "alloc record vertices");
     PG_TRY(c, dalloc(&c->rs.ve, n), "alloc record vertices");
+    PG_TRY(c, dalloc(&c->rs.vme, n), "alloc record vertices");
+    PG_TRY(c, dalloc(&c->rs.vd, n), "alloc record vertices");
     PG_TRY(c, dalloc(&c->rs.nv, n_paths), "alloc record counts");
     PG_TRY(c, dalloc(&c->rec_counts, n_paths), "alloc record counts");
     PG_TRY(c, dalloc(&c->rec_offsets, n_paths), "alloc record offsets");
@@ -505,6 +509,7 @@ pg_status pg_render_spp(pg_ctx* c, const pg_render_params* params, uint32_t n_sp
     pg_status st = ensure_paths(c, n);
     if (st != PG_OK) return st;
     if (integ == PG_INTEGRATOR_RECORD && (st = ensure_record_state(c, n, prm.max_depth + 1)) != PG_OK) return st;
+    c->rs.li_nee = prm.record_nee ? 1 : 0;
     if (integ == PG_INTEGRATOR_GUIDED_QT && c->guide_cells_cap < n) {
         dfree(c->guide_cells);
         PG_TRY(c, dalloc(&c->guide_cells, (size_t)n * kKnnMax), "alloc guide cells");
@@ -544,6 +549,10 @@ pg_status pg_render_spp(pg_ctx* c, const pg_render_params* params, uint32_t n_sp
             c->n_extend++;
             if (integ == PG_INTEGRATOR_RECORD) {
                 PG_TRY(c, timed(1, [&] { return launch_shade_record(c->dev, c->ps, pp, c->rs, parity, (int)depth, grid_shade, c->stream); }), "shade_record");
+                if (c->rs.li_nee) {
+                    PG_TRY(c, timed(2, [&] { return launch_shadow_record(c->dev, c->ps, c->rs, stack, grid_trace, c->stream); }), "shadow_record");
+                    c->n_shadow++;
+                }
                 continue;
             }
             if (mode == 1)
@@ -634,6 +643,7 @@ pg_status pg_cache_build(pg_ctx* c, uint32_t cell_size, uint32_t qt_nodes_per_ce
     uint32_t P = 1, levels = 0;
     while (P < leaves) { P <<= 1; ++levels; }
     uint32_t cells = (n + cell_size - 1) / cell_size;
+    if (cells >= (1u << 24)) return c->fail(PG_ERR_OVERFLOW, "more than 2^24 guide cells: raise cell_size");
     hipEvent_t e0 = c->ev(), e1 = c->ev();
     if (n > c->cache_cap || P > c->cache_pow2_cap || cells * qt_nodes_per_cell > c->cache_cells_cap * c->cache_stride ||
         !c->cb.keys) {

@@ -14,8 +14,11 @@ struct RecordState {
     float4* vwi;   // sampled incident direction
     float4* vw;    // bsdf weight f |cos| / (pdf rrp)
     float4* ve;    // emission (Le or background)
+    float4* vme;   // MIS-weighted emission (Renderer.h:433-457, 503-525)
+    float4* vd;    // next-event estimate at the vertex (Renderer.h:463)
     uint32_t* nv;  // vertices per path
     uint32_t stride;
+    int32_t li_nee;  // 1: records carry Li with NEE (the pixelColour recursion)
 };
 
 struct CacheBuffers {

@@ -181,6 +181,17 @@ hipError_t radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t* keys_tmp, 
 // RayTracer::guidedPath backward loop (Renderer.h:379-392): Li_i = G_{i+1},
 // G_i = emission_i + bsdfWeight_i * G_{i+1}; a vertex is kept when it sampled a
 // non-specular continuation and Li is finite with positive luminance.
+__device__ __forceinline__ C3 backprop_step(const RecordState& rs, size_t o, C3 G) {
+    float4 w = rs.vw[o];
+    if (rs.li_nee) {
+        // pixelColour = misEmission + directLighting + bsdfWeight * pixelColour (Renderer.h:391)
+        float4 me = rs.vme[o], d = rs.vd[o];
+        return (c3(me.x, me.y, me.z) + c3(d.x, d.y, d.z)) + (c3(w.x, w.y, w.z) * G);
+    }
+    float4 e = rs.ve[o];
+    return c3(e.x, e.y, e.z) + (c3(w.x, w.y, w.z) * G);
+}
+
 __global__ void k_backprop_count(RecordState rs, uint32_t n_paths, uint32_t* counts) {
     uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n_paths) return;
@@ -191,8 +202,7 @@ __global__ void k_backprop_count(RecordState rs, uint32_t n_paths, uint32_t* cou
         size_t o = (size_t)i * rs.stride + p;
         float4 vp = rs.vpos[o];
         if (vp.w != 0.f && finite3(G) && lum(G) > 0.f) ++c;
-        float4 e = rs.ve[o], w = rs.vw[o];
-        G = c3(e.x, e.y, e.z) + (c3(w.x, w.y, w.z) * G);
+        G = backprop_step(rs, o, G);
     }
     counts[p] = c;
 }
@@ -217,10 +227,10 @@ __global__ void k_backprop_write(RecordState rs, uint32_t n_paths, const uint32_
             }
             ++w_at;
         }
-        float4 e = rs.ve[o], w = rs.vw[o];
-        G = c3(e.x, e.y, e.z) + (c3(w.x, w.y, w.z) * G);
+        G = backprop_step(rs, o, G);
     }
-    // the record pass returns G_0 as the sample value (Renderer.h:393)
+    // the record pass returns G_0 as the sample value (Renderer.h:393); with NEE in the
+    // records it returns the NEE + MIS estimate instead (Renderer.h:394)
     radiance[p] = make_float4(G.r, G.g, G.b, 0.f);
 }
 

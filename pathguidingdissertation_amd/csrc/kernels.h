@@ -87,6 +87,8 @@ hipError_t launch_guide_cells(const DevScene& s, const PathState& ps, const Guid
                               uint32_t grid, hipStream_t stream);
 hipError_t launch_shade_record(const DevScene& s, const PathState& ps, const PassParams& p, const RecordState& rs,
                                int parity, int depth, uint32_t grid, hipStream_t stream);
+hipError_t launch_shadow_record(const DevScene& s, const PathState& ps, const RecordState& rs, int stack, uint32_t grid,
+                                hipStream_t stream);
 int stack_entries(int max_depth);
 uint32_t trace_grid(int cu_count, int stack);
 

@@ -150,31 +150,100 @@ __global__ __launch_bounds__(kBlock) void k_shadow(DevScene s, PathState ps) {
     }
 }
 
-// Mixture of the quadtrees of the k nearest records' cells at a shading point;
-// `cells` holds one cell per neighbour (unused slots 0xffffffff).
+// Mixture of the quadtrees of the k nearest records' cells at a shading point, each
+// neighbour weighted 1/k. `cells` lists the distinct cells as cell | (count << 24),
+// terminated by 0xffffffff when fewer than kKnnMax.
 __device__ __forceinline__ float guide_pdf_qt(const GuideView& g, const uint32_t* cells, F3 wi) {
     float u, v;
     dir_to_uv(wi.x, wi.y, wi.z, u, v);
     float sum = 0.f;
-    int k = 0;
+    uint32_t k = 0;
     for (int j = 0; j < kKnnMax; ++j) {
-        uint32_t c = cells[j];
-        if (c == 0xffffffffu) break;
-        sum += qt_pdf_uv(g.qt + (size_t)c * g.qt_stride, u, v);
-        ++k;
+        uint32_t e = cells[j];
+        if (e == 0xffffffffu) break;
+        uint32_t cnt = e >> 24;
+        sum += (float)cnt * qt_pdf_uv(g.qt + (size_t)(e & 0xffffffu) * g.qt_stride, u, v);
+        k += cnt;
     }
     return k ? sum / (float)k * 0.07957747154594767f : 0.f;
 }
 __device__ __forceinline__ F3 guide_sample_qt(const GuideView& g, const uint32_t* cells, uint32_t key, uint32_t& dim) {
-    int k = 0;
-    while (k < kKnnMax && cells[k] != 0xffffffffu) ++k;
+    uint32_t k = 0;
+    for (int j = 0; j < kKnnMax && cells[j] != 0xffffffffu; ++j) k += cells[j] >> 24;
     float r0 = keyed_rand(key, dim++), r1 = keyed_rand(key, dim++), r2 = keyed_rand(key, dim++);
-    int j = min((int)(r0 * (float)k), k - 1);
+    uint32_t t = min((uint32_t)(r0 * (float)k), k - 1u);
+    uint32_t cell = cells[0] & 0xffffffu;
+    for (int j = 0; j < kKnnMax && cells[j] != 0xffffffffu; ++j) {
+        uint32_t cnt = cells[j] >> 24;
+        if (t < cnt) { cell = cells[j] & 0xffffffu; break; }
+        t -= cnt;
+    }
     float u, v, pu;
-    qt_sample_uv(g.qt + (size_t)cells[j] * g.qt_stride, r1, r2, u, v, pu);
+    qt_sample_uv(g.qt + (size_t)cell * g.qt_stride, r1, r2, u, v, pu);
     F3 w;
     uv_to_dir(u, v, w.x, w.y, w.z);
     return w;
+}
+
+// RayTracer::computeDirect (Renderer.h:269-354) for a non-specular surface: picks a
+// light, samples it and returns the unoccluded contribution and the shadow ray
+// (false when the contribution is zero before visibility is known). In guided mode
+// the BSDF density in the MIS weight is the one-sample mixture density.
+template <int MODE>
+__device__ __forceinline__ bool next_event(const DevScene& s, const MaterialRec& m, const SD& sd, uint32_t key,
+                                           uint32_t& dim, const GuideView& g, const uint32_t* cells, bool guided,
+                                           C3& contrib, F3& sh_o, F3& sh_d, float& sh_t) {
+    contrib = c3(0.f, 0.f, 0.f);
+    const float ag = g.guide_prob;
+    float pmf = 1.f / (float)s.n_lights;
+    uint32_t nl = (uint32_t)s.n_lights;
+    float fl = floorf(keyed_rand(key, dim++) * (float)nl);
+    uint32_t li = min(max(0u, (uint32_t)fl), nl - 1u);
+    const LightRec& lr = s.lights[li];
+    if (lr.kind == 0) {
+        float r1 = keyed_rand(key, dim++), r2 = keyed_rand(key, dim++);
+        float sr = sqrtf(r1);
+        float ta = 1.f - sr, tb = r2 * sr, tg = (1.f - ta) - tb;
+        float pdf_l = 1.f / lr.area;
+        F3 pl = (ld3(lr.p0) * ta + ld3(lr.p1) * tb) + ld3(lr.p2) * tg;
+        if (pdf_l <= 0.f) return false;
+        F3 tol = pl - sd.x;
+        F3 wi = normalize(tol);
+        float sg = (dot(wi, sd.gn) >= 0.f) ? 1.f : -1.f;
+        F3 off = sd.x + sd.gn * (kEps * sg);
+        float den = length_sq(tol);
+        if (den < kEps) return false;
+        float ctl = smax(dot(wi, sd.sn), 0.f);
+        float ctp = smax(dot(-wi, ld3(lr.gn)), 0.f);
+        float G = (ctl * ctp) / den;
+        float pdf_b = 0.f;
+        C3 f = bsdf_eval(m, sd, wi, pdf_b);
+        if (MODE == 1 && guided) pdf_b = (1.f - ag) * pdf_b + ag * guide_pdf_qt(g, cells, wi);
+        float pal = pdf_l * pmf;
+        float pab = (pdf_b * ctp) / den;
+        float w = power_heuristic(pal, pab);
+        contrib = (((c3(lr.emission[0], lr.emission[1], lr.emission[2]) * f) * G) * w) / pal;
+        make_visible_ray(off, pl, sh_o, sh_d, sh_t);
+        return true;
+    }
+    C3 le;
+    float pdf_l = 0.f;
+    F3 dl = env_sample(s, key, dim, le, pdf_l);
+    if (pdf_l <= 0.f) return false;
+    F3 wi = normalize(dl);
+    float sg = (dot(wi, sd.gn) >= 0.f) ? 1.f : -1.f;
+    F3 off = sd.x + sd.gn * (kEps * sg);
+    F3 far = F3{s.centre[0], s.centre[1], s.centre[2]} + wi * s.radius;  // Renderer.h:331
+    float ctl = smax(dot(wi, sd.sn), 0.f);
+    if (ctl <= 0.f) return false;
+    float pdf_b = 0.f;
+    C3 f = bsdf_eval(m, sd, wi, pdf_b);
+    if (MODE == 1 && guided) pdf_b = (1.f - ag) * pdf_b + ag * guide_pdf_qt(g, cells, wi);
+    float pal = pdf_l * pmf;
+    float w = power_heuristic(pal, pdf_b);
+    contrib = (((le * f) * ctl) * w) / pal;
+    make_visible_ray(off, far, sh_o, sh_d, sh_t);
+    return true;
 }
 
 // One bounce of RayTracer::pathTraceRecursive + computeDirect (Renderer.h:269-354, 531-613).
@@ -257,69 +326,19 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene s, PathState ps, Pass
                         }
                     }
                 } else {
-                    const uint32_t* cells = gc.cells + (size_t)i * kKnnMax;
-                    const bool guided = MODE == 1 && !m.specular && gc.cells != nullptr && cells[0] != 0xffffffffu;
+                    const uint32_t* cells = gc.cells ? gc.cells + (size_t)i * kKnnMax : nullptr;
+                    const bool guided = MODE == 1 && !m.specular && cells != nullptr && cells[0] != 0xffffffffu;
                     const float ag = gc.g.guide_prob;
                     // computeDirect (Renderer.h:269-354)
                     if (!m.specular && s.n_lights > 0) {
-                        float pmf = 1.f / (float)s.n_lights;
-                        uint32_t nl = (uint32_t)s.n_lights;
-                        float fl = floorf(keyed_rand(key, dim++) * (float)nl);
-                        uint32_t li = min(max(0u, (uint32_t)fl), nl - 1u);
-                        const LightRec& lr = s.lights[li];
-                        C3 contrib = c3(0.f, 0.f, 0.f);
-                        if (lr.kind == 0) {
-                            float r1 = keyed_rand(key, dim++), r2 = keyed_rand(key, dim++);
-                            float sr = sqrtf(r1);
-                            float ta = 1.f - sr, tb = r2 * sr, tg = (1.f - ta) - tb;
-                            float pdf_l = 1.f / lr.area;
-                            F3 pl = (ld3(lr.p0) * ta + ld3(lr.p1) * tb) + ld3(lr.p2) * tg;
-                            if (!(pdf_l <= 0.f)) {
-                                F3 tol = pl - sd.x;
-                                F3 wi = normalize(tol);
-                                float sg = (dot(wi, sd.gn) >= 0.f) ? 1.f : -1.f;
-                                F3 off = sd.x + sd.gn * (kEps * sg);
-                                float den = length_sq(tol);
-                                if (!(den < kEps)) {
-                                    float ctl = smax(dot(wi, sd.sn), 0.f);
-                                    float ctp = smax(dot(-wi, ld3(lr.gn)), 0.f);
-                                    float G = (ctl * ctp) / den;
-                                    float pdf_b = 0.f;
-                                    C3 f = bsdf_eval(m, sd, wi, pdf_b);
-                                    if (guided) pdf_b = (1.f - ag) * pdf_b + ag * guide_pdf_qt(gc.g, cells, wi);
-                                    float pal = pdf_l * pmf;
-                                    float pab = (pdf_b * ctp) / den;
-                                    float w = power_heuristic(pal, pab);
-                                    contrib = (((c3(lr.emission[0], lr.emission[1], lr.emission[2]) * f) * G) * w) / pal;
-                                    make_visible_ray(off, pl, sh_o, sh_d, sh_t);
-                                }
+                        C3 contrib;
+                        if (next_event<MODE>(s, m, sd, key, dim, gc.g, cells, guided, contrib, sh_o, sh_d, sh_t)) {
+                            // the shadow ray can only change the image when the contribution is nonzero
+                            C3 direct = beta * contrib;
+                            if (!is_zero(direct)) {
+                                want_shadow = true;
+                                sh_c = direct;
                             }
-                        } else {
-                            C3 le;
-                            float pdf_l = 0.f;
-                            F3 dl = env_sample(s, key, dim, le, pdf_l);
-                            if (!(pdf_l <= 0.f)) {
-                                F3 wi = normalize(dl);
-                                float sg = (dot(wi, sd.gn) >= 0.f) ? 1.f : -1.f;
-                                F3 off = sd.x + sd.gn * (kEps * sg);
-                                F3 far = F3{s.centre[0], s.centre[1], s.centre[2]} + wi * s.radius;  // Renderer.h:331
-                                float ctl = smax(dot(wi, sd.sn), 0.f);
-                                if (!(ctl <= 0.f)) {
-                                    float pdf_b = 0.f;
-                                    C3 f = bsdf_eval(m, sd, wi, pdf_b);
-                                    if (guided) pdf_b = (1.f - ag) * pdf_b + ag * guide_pdf_qt(gc.g, cells, wi);
-                                    float pal = pdf_l * pmf;
-                                    float w = power_heuristic(pal, pdf_b);
-                                    contrib = (((le * f) * ctl) * w) / pal;
-                                    make_visible_ray(off, far, sh_o, sh_d, sh_t);
-                                }
-                            }
-                        }
-                        // the shadow ray can only change the image when the contribution is nonzero
-                        C3 direct = beta * contrib;
-                        if (!is_zero(direct)) {
-                            want_shadow = true;
-                            sh_c = direct;
                         }
                     }
                     // Russian roulette (Renderer.h:561-565), depth cap (Renderer.h:568)
@@ -414,16 +433,29 @@ __global__ __launch_bounds__(kBlock) void k_guide_cells(DevScene s, PathState ps
         float bd[kKnnMax];
         uint32_t bs[kKnnMax];
         knn_query<kKnnMax>(g, x, bd, bs, sn + threadIdx.x, sd_ + threadIdx.x, kBlock);
-#pragma unroll
-        for (int j = 0; j < kKnnMax; ++j) out[j] = bs[j] == 0xffffffffu ? 0xffffffffu : bs[j] / g.cell_size;
+        // distinct cells with neighbour counts, nearest first
+        int nu = 0;
+        for (int j = 0; j < kKnnMax; ++j) {
+            if (bs[j] == 0xffffffffu) break;
+            uint32_t c = bs[j] / g.cell_size;
+            int f = -1;
+            for (int m = 0; m < nu; ++m)
+                if ((out[m] & 0xffffffu) == c) { f = m; break; }
+            if (f >= 0) out[f] += 1u << 24;
+            else out[nu++] = c | (1u << 24);
+        }
+        if (nu < kKnnMax) out[nu] = 0xffffffffu;
     }
 }
 
 // One bounce of the record pass, RayTracer::guidedPathRecursive (Renderer.h:397-528):
 // BSDF-sampled continuation with a fixed Russian-roulette probability 0.7 after
-// depth rr_depth, one forward vertex {x, n, wi, f cos/(pdf rrp), emission, store}
-// per bounce. computeDirect's random numbers are drawn as the reference draws them,
-// but its value does not enter the record pass's result, so no shadow ray is traced.
+// depth rr_depth, one forward vertex {x, n, wi, f cos/(pdf rrp), emission,
+// MIS-weighted emission, NEE, store} per bounce. With NEE == false the next-event
+// estimate draws its random numbers as the reference does but is not traced (the
+// reference's Li does not use it); with NEE == true a shadow ray is queued whose
+// result lands in the vertex's NEE slot (the pixelColour recursion).
+template <bool NEE>
 __global__ __launch_bounds__(kBlock) void k_shade_record(DevScene s, PathState ps, PassParams p, RecordState rs,
                                                          int parity, int depth) {
     uint32_t n = ps.counters[parity];
@@ -432,8 +464,10 @@ __global__ __launch_bounds__(kBlock) void k_shade_record(DevScene s, PathState p
     for (uint32_t base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock) {
         uint32_t i = base + threadIdx.x;
         bool active = i < n;
-        bool want_ray = false;
-        F3 nr_o{0.f, 0.f, 0.f}, nr_d{0.f, 0.f, 0.f};
+        bool want_ray = false, want_shadow = false;
+        F3 nr_o{0.f, 0.f, 0.f}, nr_d{0.f, 0.f, 0.f}, sh_o{0.f, 0.f, 0.f}, sh_d{0.f, 0.f, 0.f};
+        float sh_t = 0.f;
+        C3 sh_c = c3(0.f, 0.f, 0.f);
         uint32_t path = 0;
         if (active) {
             path = qin[i];
@@ -444,10 +478,26 @@ __global__ __launch_bounds__(kBlock) void k_shade_record(DevScene s, PathState p
             F3 rd{rdv.x, rdv.y, rdv.z};
             uint4 inf = ps.info[path];
             uint32_t key = inf.x, dim = inf.y;
-            float4 vpos = make_float4(0.f, 0.f, 0.f, 0.f), vnrm = vpos, vwi = vpos, vw = vpos, ve = vpos;
+            float prev_pdf = ps.beta[path].w;
+            bool prev_spec = (inf.z & 1u) != 0u;
+            float4 vpos = make_float4(0.f, 0.f, 0.f, 0.f), vnrm = vpos, vwi = vpos, vw = vpos, ve = vpos, vme = vpos;
             if (hid == kMiss) {
+                // Renderer.h:501-527
                 C3 bg = s.has_env ? env_eval(s, rd) : c3(0.f, 0.f, 0.f);
                 ve = make_float4(bg.r, bg.g, bg.b, 0.f);
+                if (depth == 0 || prev_spec) {
+                    vme = ve;
+                } else if (lum(bg) > 1e-8f) {
+                    float pmf = 1.f / (float)s.n_lights;
+                    float pdf_l = s.has_env ? env_pdf(s, rd) : 0.f;
+                    if (pmf > 0.f && pdf_l > 0.f) {
+                        float pal = pmf * pdf_l, pab = prev_pdf;
+                        if (pal >= 0.f && pab >= 0.f) {
+                            C3 me = bg * power_heuristic(pab, pal);
+                            vme = make_float4(me.r, me.g, me.b, 0.f);
+                        }
+                    }
+                }
             } else {
                 const TriShade& tsh = s.shade[hid];
                 const MaterialRec& m = s.mats[tsh.mat];
@@ -466,13 +516,42 @@ __global__ __launch_bounds__(kBlock) void k_shade_record(DevScene s, PathState p
                 vpos = make_float4(sd.x.x, sd.x.y, sd.x.z, 0.f);
                 vnrm = make_float4(sd.sn.x, sd.sn.y, sd.sn.z, 0.f);
                 if (m.is_light) {
-                    ve = make_float4(m.emission[0], m.emission[1], m.emission[2], 0.f);
+                    // Renderer.h:430-460
+                    C3 e = c3(m.emission[0], m.emission[1], m.emission[2]);
+                    ve = make_float4(e.r, e.g, e.b, 0.f);
+                    if (depth == 0 || prev_spec) {
+                        vme = ve;
+                    } else {
+                        float pmf = 1.f / (float)s.n_lights;
+                        float pdf_l = 1.f / tsh.area;
+                        if (pmf > 0.f && pdf_l > 0.f) {
+                            float ctp = smax(dot(-rd, ld3(tsh.gn)), 0.f);
+                            float d2 = ht * ht;
+                            if (ctp > 0.f && d2 > kEps) {
+                                float pal = pdf_l * pmf, pab = (prev_pdf * ctp) / d2;
+                                if (pal >= 0.f && pab >= 0.f) {
+                                    C3 me = e * power_heuristic(pab, pal);
+                                    vme = make_float4(me.r, me.g, me.b, 0.f);
+                                }
+                            }
+                        }
+                    }
                 } else {
                     if (!m.specular && s.n_lights > 0) {
-                        uint32_t nl = (uint32_t)s.n_lights;
-                        float fl = floorf(keyed_rand(key, dim++) * (float)nl);
-                        uint32_t li = min(max(0u, (uint32_t)fl), nl - 1u);
-                        dim += (s.lights[li].kind == 0) ? 2u : 4u;
+                        if (NEE) {
+                            C3 contrib;
+                            GuideView none{};
+                            if (next_event<0>(s, m, sd, key, dim, none, nullptr, false, contrib, sh_o, sh_d, sh_t) &&
+                                !is_zero(contrib)) {
+                                want_shadow = true;
+                                sh_c = contrib;
+                            }
+                        } else {
+                            uint32_t nl = (uint32_t)s.n_lights;
+                            float fl = floorf(keyed_rand(key, dim++) * (float)nl);
+                            uint32_t li = min(max(0u, (uint32_t)fl), nl - 1u);
+                            dim += (s.lights[li].kind == 0) ? 2u : 4u;
+                        }
                     }
                     bool cont = true;
                     float rr = 1.f;
@@ -495,6 +574,8 @@ __global__ __launch_bounds__(kBlock) void k_shade_record(DevScene s, PathState p
                             nr_o = sd.x + sd.gn * (kEps * sg);
                             nr_d = wi;
                             want_ray = true;
+                            prev_pdf = pdf;
+                            prev_spec = m.specular != 0;
                         }
                     }
                 }
@@ -505,14 +586,49 @@ __global__ __launch_bounds__(kBlock) void k_shade_record(DevScene s, PathState p
             rs.vwi[o] = vwi;
             rs.vw[o] = vw;
             rs.ve[o] = ve;
-            if (want_ray) ps.info[path] = make_uint4(key, dim, 0u, inf.w);
-            else rs.nv[path] = (uint32_t)depth + 1u;
+            if (NEE) {
+                rs.vme[o] = vme;
+                rs.vd[o] = make_float4(0.f, 0.f, 0.f, 0.f);
+            }
+            if (want_ray) {
+                ps.info[path] = make_uint4(key, dim, prev_spec ? 1u : 0u, inf.w);
+                ps.beta[path] = make_float4(1.f, 1.f, 1.f, prev_pdf);
+            } else {
+                rs.nv[path] = (uint32_t)depth + 1u;
+            }
+        }
+        if (NEE) {
+            uint32_t sslot = wave_append(ps.counters + 2, want_shadow);
+            if (want_shadow) {
+                ps.sh_o[sslot] = make_float4(sh_o.x, sh_o.y, sh_o.z, sh_t);
+                ps.sh_d[sslot] = make_float4(sh_d.x, sh_d.y, sh_d.z, __uint_as_float(path));
+                ps.sh_c[sslot] = make_float4(sh_c.r, sh_c.g, sh_c.b, __int_as_float(depth));
+            }
         }
         uint32_t rslot = wave_append(ps.counters + (parity ^ 1), want_ray);
         if (want_ray) {
             qout[rslot] = path;
             ps.ray_o[parity ^ 1][rslot] = make_float4(nr_o.x, nr_o.y, nr_o.z, 0.f);
             ps.ray_d[parity ^ 1][rslot] = make_float4(nr_d.x, nr_d.y, nr_d.z, 0.f);
+        }
+    }
+}
+
+// Shadow rays of the record pass: an unoccluded contribution goes to its vertex.
+template <int STACK>
+__global__ __launch_bounds__(kBlock) void k_shadow_record(DevScene s, PathState ps, RecordState rs) {
+    __shared__ int lds[STACK * kBlock];
+    uint32_t n = ps.counters[2];
+    if (blockIdx.x == 0 && threadIdx.x == 0) ps.stats[1] += n;
+    uint32_t cn = 0, ct = 0;
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+        float4 a = ps.sh_o[i], b = ps.sh_d[i];
+        F3 o{a.x, a.y, a.z}, d{b.x, b.y, b.z};
+        if (traverse_visible<false>(s, o, d, a.w, lds + threadIdx.x, cn, ct)) {
+            uint32_t path = __float_as_uint(b.w);
+            float4 c = ps.sh_c[i];
+            size_t off = (size_t)__float_as_int(c.w) * rs.stride + path;
+            rs.vd[off] = make_float4(c.x, c.y, c.z, 0.f);
         }
     }
 }
@@ -642,7 +758,17 @@ hipError_t launch_guide_cells(const DevScene& s, const PathState& ps, const Guid
 }
 hipError_t launch_shade_record(const DevScene& s, const PathState& ps, const PassParams& p, const RecordState& rs,
                                int parity, int depth, uint32_t grid, hipStream_t stream) {
-    hipLaunchKernelGGL(k_shade_record, dim3(grid), dim3(kBlock), 0, stream, s, ps, p, rs, parity, depth);
+    if (rs.li_nee)
+        hipLaunchKernelGGL((k_shade_record<true>), dim3(grid), dim3(kBlock), 0, stream, s, ps, p, rs, parity, depth);
+    else
+        hipLaunchKernelGGL((k_shade_record<false>), dim3(grid), dim3(kBlock), 0, stream, s, ps, p, rs, parity, depth);
+    return hipGetLastError();
+}
+hipError_t launch_shadow_record(const DevScene& s, const PathState& ps, const RecordState& rs, int stack, uint32_t grid,
+                                hipStream_t stream) {
+#define PG_CALL(S) hipLaunchKernelGGL((k_shadow_record<S>), dim3(grid), dim3(kBlock), 0, stream, s, ps, rs)
+    PG_STACK_DISPATCH(stack, PG_CALL);
+#undef PG_CALL
     return hipGetLastError();
 }
 

@@ -311,10 +311,13 @@ void interior(pg_host_scene* s, uint32_t seed, bool closed, int grid) {
         D3 e = c - eye;
         return e.x * e.x + e.y * e.y + e.z * e.z > 9.0;
     };
+    // objects stand on or just above the floor (centre height ~ their largest half
+    // extent), so the space above them stays open for light entering the room
     for (int i = 0; i < 2000; ++i) {
         D3 h{rng.range(0.1, 0.6), rng.range(0.1, 0.6), rng.range(0.1, 0.6)};
-        D3 c{rng.range(-9.3, 9.3), rng.range(0.2, 6.0), rng.range(-9.3, 9.3)};
-        while (!clear_of_eye(c)) c = D3{rng.range(-9.3, 9.3), rng.range(0.2, 6.0), rng.range(-9.3, 9.3)};
+        double hmax = h.x > h.y ? (h.x > h.z ? h.x : h.z) : (h.y > h.z ? h.y : h.z);
+        D3 c{rng.range(-9.3, 9.3), hmax * rng.range(0.6, 1.0) + rng.range(0.0, 0.4), rng.range(-9.3, 9.3)};
+        while (!clear_of_eye(c)) c = D3{rng.range(-9.3, 9.3), c.y, rng.range(-9.3, 9.3)};
         double R[9];
         random_rotation(rng, R);
         uint32_t m = object_material(rng);
@@ -322,8 +325,8 @@ void interior(pg_host_scene* s, uint32_t seed, bool closed, int grid) {
     }
     for (int i = 0; i < 200; ++i) {
         double r = rng.range(0.2, 0.8);
-        D3 c{rng.range(-9.0, 9.0), rng.range(0.3, 6.0), rng.range(-9.0, 9.0)};
-        while (!clear_of_eye(c)) c = D3{rng.range(-9.0, 9.0), rng.range(0.3, 6.0), rng.range(-9.0, 9.0)};
+        D3 c{rng.range(-9.0, 9.0), r + rng.range(0.0, 0.3), rng.range(-9.0, 9.0)};
+        while (!clear_of_eye(c)) c = D3{rng.range(-9.0, 9.0), c.y, rng.range(-9.0, 9.0)};
         uint32_t m = object_material(rng);
         b.icosphere(c, r, m);
     }

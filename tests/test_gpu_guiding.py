@@ -9,23 +9,25 @@ pytestmark = pytest.mark.gpu
 INVALID = 0xFFFFFFFF
 
 
-def _records_gpu(pg, device, data, spp, seed, max_depth=8):
+def _records_gpu(pg, device, data, spp, seed, max_depth=8, nee=0):
     sc = pg.Scene(device, data)
     device.clear_records()
-    rt = pg.RayTracer(sc, seed=seed, max_depth=max_depth, integrator=1, record=1)
+    rt = pg.RayTracer(sc, seed=seed, max_depth=max_depth, integrator=1, record=1, record_nee=nee)
     rt.clear()
     rt.render(spp)
     return sc, rt, device.records()
 
 
-def test_record_pass_matches_oracle(pg, device, cornell):
-    """Same random stream: the record-pass film (G_0, the 'incoming radiance' image)
-    and the multiset of PathVertex records agree with the oracle's."""
+@pytest.mark.parametrize("nee", [0, 1])
+def test_record_pass_matches_oracle(pg, device, cornell, nee):
+    """Same random stream: the record-pass film (G_0, the 'incoming radiance' image,
+    or with nee=1 the pixelColour estimate) and the multiset of PathVertex records
+    agree with the oracle's."""
     from oracle.pg_oracle import OracleScene
-    sc, rt, rec = _records_gpu(pg, device, cornell, spp=1, seed=7)
+    sc, rt, rec = _records_gpu(pg, device, cornell, spp=1, seed=7, nee=nee)
     film, _ = rt.film()
     osc = OracleScene(cornell)
-    ref_film, st = osc.render(1, seed=7, integrator=1, record=True, accum_f64=False, env_mode=1)
+    ref_film, st = osc.render(1, seed=7, integrator=1 + nee, record=True, accum_f64=False, env_mode=1)
     orec = osc.records()
     assert abs(rec.shape[0] - orec.shape[0]) <= max(5, orec.shape[0] // 500), (rec.shape, orec.shape)
     rel = np.abs(film - ref_film) / (np.abs(ref_film) + 1e-3)
@@ -159,7 +161,15 @@ def test_quadtree_invariants(pg, device, cornell):
         E = nodes[:, :4].astype(np.float64)
         ch = nodes[:, 4:8].view(np.int32)
         total = E[0].sum()
-        used = np.unique(np.concatenate([[0], ch[ch >= 0]]))
+        used, depth, frontier = [], {0: 0}, [0]
+        while frontier:  # nodes reachable from the root (the rest of the pool is unused)
+            nd = frontier.pop()
+            used.append(nd)
+            for q in range(4):
+                if ch[nd, q] >= 0:
+                    depth[int(ch[nd, q])] = depth[nd] + 1
+                    frontier.append(int(ch[nd, q]))
+        assert max(depth.values()) <= 9  # leaf quadrants of depth-9 nodes sit at depth 10
         for nd in used:
             for q in range(4):
                 if ch[nd, q] >= 0:
@@ -187,13 +197,16 @@ def test_guide_sampling_consistent_with_pdf(pg, device, cornell):
     u = (np.arctan2(wi[:, 1], wi[:, 0]) % (2 * np.pi)) / (2 * np.pi)
     v = (wi[:, 2] + 1) / 2
     H, _, _ = np.histogram2d(u, v, bins=16, range=[[0, 1], [0, 1]])
-    # expected mass per bin from the pdf evaluated on a fine grid inside each bin
-    g = (np.arange(64) + 0.5) / 64
+    # expected mass per bin: the pdf is constant on the depth-10 dyadic squares, so
+    # their centres integrate it exactly
+    m = 1024
+    g = (np.arange(m) + 0.5) / m
     uu, vv = np.meshgrid(g, g, indexing="ij")
     dirs = np.stack([np.sqrt(1 - (2 * vv - 1) ** 2) * np.cos(2 * np.pi * uu),
                      np.sqrt(1 - (2 * vv - 1) ** 2) * np.sin(2 * np.pi * uu), 2 * vv - 1], -1).reshape(-1, 3)
     _, _, pg_ = device.guide_eval(np.tile(rec[:1, :3], (dirs.shape[0], 1)), np.zeros_like(dirs), dirs)
-    mass = (pg_ * 4 * np.pi).reshape(64, 64).reshape(16, 4, 16, 4).mean((1, 3)) / 256
+    mass = (pg_ * 4 * np.pi).reshape(m, m).reshape(16, m // 16, 16, m // 16).mean((1, 3)) / 256
+    assert abs(mass.sum() - 1.0) < 1e-3
     exp = mass * n
     ok = exp > 20
     chi2 = np.sum((H[ok] - exp[ok]) ** 2 / exp[ok])
@@ -207,15 +220,18 @@ def test_guided_render_is_unbiased_and_helps(pg, device):
     data = pg.generate_scene(101, 0)
     sc = pg.Scene(device, data)
     device.clear_records()
-    rec_rt = pg.RayTracer(sc, seed=1, max_depth=12, integrator=1, record=1)
+    rec_rt = pg.RayTracer(sc, seed=1, max_depth=12, integrator=1, record=1, record_nee=1)
     rec_rt.clear()
-    rec_rt.render(1)
-    info = device.cache_build()
+    rec_rt.render(4)
+    info = device.cache_build(cell_size=4096)
     assert info["records"] > 100_000
     ref_rt = pg.RayTracer(sc, seed=100, max_depth=12)
     ref_rt.clear()
     ref_rt.render(256)
     ref = ref_rt.image()
+    gref = pg.RayTracer(sc, seed=200, max_depth=12, integrator=2, guide_prob=0.5)
+    gref.clear()
+    gref.render(64)
     un = pg.RayTracer(sc, seed=2, max_depth=12)
     un.clear()
     un.render(16)
@@ -224,6 +240,10 @@ def test_guided_render_is_unbiased_and_helps(pg, device):
     gd.render(16)
     e_un = pg.relmse(un.image(), ref)
     e_gd = pg.relmse(gd.image(), ref)
-    m_ref, m_gd = ref.mean(), gd.image().mean()
-    assert abs(m_gd - m_ref) / m_ref < 0.03, (m_gd, m_ref)
-    assert e_gd < e_un, (e_gd, e_un)
+    m_ref, m_gd = ref.mean(), gref.image().mean()
+    print(f"records {info['records']} cells {info['cells']} relMSE unguided {e_un:.4f} guided {e_gd:.4f} "
+          f"mean unguided-256 {m_ref:.5f} guided-64 {m_gd:.5f}")
+    # both estimators converge to the same image (the unguided reference carries
+    # fireflies, hence the 5% band on the mean)
+    assert abs(m_gd - m_ref) / m_ref < 0.05, (m_gd, m_ref)
+    assert e_gd < 0.5 * e_un, (e_gd, e_un)
