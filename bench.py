@@ -49,7 +49,7 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=64)
     ap.add_argument("--warmup", type=int, default=4)
     ap.add_argument("--config", type=int, default=int(os.environ.get("PG_BENCH_CONFIG", "2")))
-    ap.add_argument("--cpu-spp", type=int, default=int(os.environ.get("PG_BENCH_CPU_SPP", "4")))
+    ap.add_argument("--cpu-spp", type=int, default=int(os.environ.get("PG_BENCH_CPU_SPP", "40")))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-check", action="store_true",
                     help="also time a plain 1 GiB device copy (HBM reference point / counter scaling)")
