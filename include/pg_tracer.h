@@ -179,6 +179,73 @@ pg_status pg_knn(pg_ctx* ctx, const float* query_xyz, uint32_t nq, uint32_t k, u
 pg_status pg_guide_qt_eval(pg_ctx* ctx, const float* pos, const float* u3, const float* wi_eval,
                            uint32_t n, float* wi_out, float* pdf_sample, float* pdf_eval);
 
+/* ------------------------------------------------------------------ learned guide */
+/* No counterpart in the reference, whose guiding stops at recording PathVertex data
+ * (Renderer.h:398-414): a hash-grid + one-blob encoded 3 x 64 ReLU network maps
+ * (x, n, wo) to an 8-lobe vMF mixture and a BSDF/guide selection probability,
+ * trained with Adam on the records of PG_INTEGRATOR_RECORD passes (radiance-weighted
+ * NLL of the one-sample mixture) and used by PG_INTEGRATOR_GUIDED_MLP.            */
+typedef struct pg_mlp_config {
+    uint32_t seed;            /* weight initialisation                              */
+    uint32_t log2_table;      /* hash entries per level, log2 (12..22, default 19)  */
+    float res_min, res_max;   /* coarsest / finest grid resolution (16, 1024)       */
+    float lr, beta1, beta2, eps; /* Adam (1e-2, 0.9, 0.99, 1e-15)                   */
+    uint32_t batch;           /* training samples per step (2^18)                   */
+    float weight_clip;        /* clip of the mean-normalised sample weight (100)    */
+    uint64_t act_budget;      /* bytes of activation rows per chunk (512 MiB)       */
+    uint32_t nan_step;        /* test hook: poison the gradient at this step (0 off)*/
+    uint32_t reserved[7];
+} pg_mlp_config;
+
+typedef struct pg_mlp_info {
+    uint32_t ready;           /* pg_mlp_init done                                   */
+    uint32_t diverged;        /* a non-finite gradient was seen: rendering falls back */
+    uint32_t step;            /* Adam steps taken                                   */
+    uint32_t n_params;        /* network parameters (17648 incl. padding)           */
+    uint64_t n_grid_params;   /* hash-table parameters (16 * 2^log2_table * 2)      */
+    float last_loss;          /* mean loss of the last step's batch                 */
+    float w_scale;            /* 1 / mean record weight                             */
+    uint32_t chunks;          /* chunks per step under the activation budget        */
+    float train_ms;           /* GPU time of the last pg_mlp_train call             */
+    uint32_t fallbacks;       /* guided passes rendered with a fallback guide       */
+    uint32_t reserved[5];
+} pg_mlp_info;
+
+pg_mlp_config pg_mlp_config_default(void);
+/* Allocate and initialise the guide (needs an uploaded scene: its bounding cube
+ * normalises positions; uploading another scene requires a new pg_mlp_init).
+ * Resets the step count and the divergence flag.                                 */
+pg_status pg_mlp_init(pg_ctx* ctx, const pg_mlp_config* cfg);
+/* `steps` Adam steps on the current records (they must come from record passes,
+ * which keep the training data). losses (nullable) receives each step's mean loss. */
+pg_status pg_mlp_train(pg_ctx* ctx, uint32_t steps, float* losses);
+/* Learned-guide rendering with online training: each of n_spp passes is a guided
+ * record pass (PG_INTEGRATOR_GUIDED_MLP with record = 1: the record integrator of
+ * Renderer.h:367-529 sampling by one-sample MIS between the BSDF and the guide,
+ * one sample per pixel into the film, vertices kept as training data), followed by
+ * steps_per_pass Adam steps on the records of the last `window` passes (older
+ * records are dropped). params->record_nee = 1 is recommended.                   */
+pg_status pg_mlp_render_online(pg_ctx* ctx, const pg_render_params* params, uint32_t n_spp,
+                               uint32_t steps_per_pass, uint32_t window);
+pg_status pg_mlp_info_read(pg_ctx* ctx, pg_mlp_info* info);
+/* FP32 master parameters (17648) and hash table (n_grid_params), either nullable. */
+pg_status pg_mlp_get_params(pg_ctx* ctx, float* params, float* grid);
+pg_status pg_mlp_set_params(pg_ctx* ctx, const float* params, const float* grid);
+/* Per-level resolution and dense flag (16 each) and the position frame (lo.xyz, 1/extent). */
+pg_status pg_mlp_levels(pg_ctx* ctx, uint32_t* res, uint32_t* dense, float* frame);
+/* Network at n points (xyz triples): mixture (n x 52 floats, layout of 13 float4:
+ * 8 x (mu, kappa), 8 lobe probabilities, 8 lobe probabilities x vMF normalisation,
+ * (c, 0, 0, 0)), pdf of wi_eval, and a direction sampled with u3 (xyz + pdf).
+ * Every output is nullable.                                                       */
+pg_status pg_mlp_eval(pg_ctx* ctx, uint32_t n, const float* pos, const float* nrm, const float* wo,
+                      float* mix, const float* wi_eval, float* pdf_eval, const float* u3,
+                      float* wi_out);
+/* Loss and gradients (no update) for n explicit samples of 16 floats each:
+ * (x.xyz, weight), (n.xyz, BSDF pdf of wi), (wo.xyz, 0), (wi.xyz, 0); the loss is
+ * the mean over the n samples. grad_params: 17648 floats; grad_grid: n_grid_params. */
+pg_status pg_mlp_gradient(pg_ctx* ctx, uint32_t n, const float* samples, float* loss,
+                          float* grad_params, float* grad_grid);
+
 /* ------------------------------------------------------------------ host-side helpers */
 /* Procedural scenes of the benchmark configurations (no reference counterpart:
  * the reference loads scene.json + .gem files, SceneLoader.h:222-269).

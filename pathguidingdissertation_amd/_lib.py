@@ -50,6 +50,20 @@ class PgStats(C.Structure):
                 ("shadow_nodes_visited", C.c_uint64), ("shadow_tris_tested", C.c_uint64)]
 
 
+class PgMlpConfig(C.Structure):
+    _fields_ = [("seed", C.c_uint32), ("log2_table", C.c_uint32), ("res_min", C.c_float), ("res_max", C.c_float),
+                ("lr", C.c_float), ("beta1", C.c_float), ("beta2", C.c_float), ("eps", C.c_float),
+                ("batch", C.c_uint32), ("weight_clip", C.c_float), ("act_budget", C.c_uint64),
+                ("nan_step", C.c_uint32), ("reserved", C.c_uint32 * 7)]
+
+
+class PgMlpInfo(C.Structure):
+    _fields_ = [("ready", C.c_uint32), ("diverged", C.c_uint32), ("step", C.c_uint32), ("n_params", C.c_uint32),
+                ("n_grid_params", C.c_uint64), ("last_loss", C.c_float), ("w_scale", C.c_float),
+                ("chunks", C.c_uint32), ("train_ms", C.c_float), ("fallbacks", C.c_uint32),
+                ("reserved", C.c_uint32 * 5)]
+
+
 _P = C.c_void_p
 _F = C.POINTER(C.c_float)
 _U32 = C.POINTER(C.c_uint32)
@@ -87,6 +101,16 @@ SIGNATURES = {
     "pg_cache_export": (C.c_int32, [_P, _F, _F, _F]),
     "pg_knn": (C.c_int32, [_P, _F, C.c_uint32, C.c_uint32, _U32, _F]),
     "pg_guide_qt_eval": (C.c_int32, [_P, _F, _F, _F, C.c_uint32, _F, _F, _F]),
+    "pg_mlp_config_default": (PgMlpConfig, []),
+    "pg_mlp_init": (C.c_int32, [_P, C.POINTER(PgMlpConfig)]),
+    "pg_mlp_train": (C.c_int32, [_P, C.c_uint32, _F]),
+    "pg_mlp_render_online": (C.c_int32, [_P, C.POINTER(PgRenderParams), C.c_uint32, C.c_uint32, C.c_uint32]),
+    "pg_mlp_info_read": (C.c_int32, [_P, C.POINTER(PgMlpInfo)]),
+    "pg_mlp_get_params": (C.c_int32, [_P, _F, _F]),
+    "pg_mlp_set_params": (C.c_int32, [_P, _F, _F]),
+    "pg_mlp_levels": (C.c_int32, [_P, _U32, _U32, _F]),
+    "pg_mlp_eval": (C.c_int32, [_P, C.c_uint32, _F, _F, _F, _F, _F, _F, _F, _F]),
+    "pg_mlp_gradient": (C.c_int32, [_P, C.c_uint32, _F, _F, _F, _F]),
     "pg_scene_generate": (C.c_int32, [C.c_int32, C.c_uint32, C.POINTER(_P)]),
     "pg_scene_sizes": (C.c_int32, [_P, _U32, _U32, _I32, _I32]),
     "pg_scene_copy": (C.c_int32, [_P, _F, _F, _U32, C.POINTER(PgMaterial), _F, C.POINTER(PgCameraDesc)]),

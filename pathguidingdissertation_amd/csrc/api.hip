@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cfloat>
 #include <cstring>
 #include <string>
@@ -12,6 +13,7 @@
 #include "host_prep.hpp"
 #include "kernels.h"
 #include "guiding.h"
+#include "mlp.h"
 
 using namespace pg;
 
@@ -90,6 +92,43 @@ struct pg_ctx {
     double cache_build_ms = 0.0;
     uint32_t* guide_cells = nullptr;
     uint32_t guide_cells_cap = 0;
+    // training data beside the records (records [0, n_aux) carry it)
+    TrainAux* rec_aux = nullptr;
+    uint64_t n_aux = 0;
+    uint64_t rec_gen = 0;  // bumped whenever the record set changes
+    // learned guide
+    struct Mlp {
+        bool ready = false, diverged = false;
+        pg_mlp_config cfg{};
+        MlpView view{};
+        float* params = nullptr;
+        float* pm = nullptr;
+        float* pv = nullptr;
+        uint16_t* mirror = nullptr;
+        float* grid = nullptr;
+        float* gm = nullptr;
+        float* gv = nullptr;
+        float* ggrad = nullptr;
+        uint32_t* grid_bf = nullptr;
+        size_t n_grid = 0;
+        float* partial = nullptr;
+        uint32_t groups_cap = 0;
+        TrainSample* batch = nullptr;
+        uint32_t batch_cap = 0;
+        uint16_t* rows = nullptr;
+        uint32_t rows_cap = 0;
+        float* loss = nullptr;
+        uint32_t loss_cap = 0;
+        float* scal = nullptr;
+        uint32_t scal_cap = 0;
+        uint32_t* status = nullptr;
+        uint32_t step = 0, chunks = 1, fallbacks = 0;
+        float last_loss = 0.f, w_scale = 1.f, train_ms = 0.f;
+        uint64_t w_scale_gen = ~0ull;
+        uint32_t window_passes = 0;
+    } mlp;
+    float4* mix = nullptr;
+    uint32_t mix_cap = 0;
 
     pg_status fail(pg_status code, const std::string& m) {
         err = m;
@@ -115,7 +154,8 @@ struct pg_ctx {
         has_scene = false;
     }
     void free_records() {
-        dfree(rs.vpos); dfree(rs.vnrm); dfree(rs.vwi); dfree(rs.vw); dfree(rs.ve); dfree(rs.vme); dfree(rs.vd);
+        dfree(rs.vpos); dfree(rs.vnrm); dfree(rs.vwi); dfree(rs.vwo); dfree(rs.vw); dfree(rs.ve); dfree(rs.vme);
+        dfree(rs.vd);
         dfree(rs.nv);
         rs_depth = rs_paths = 0;
         dfree(rec_counts); dfree(rec_offsets); dfree(scan_tmp); dfree(d_total);
@@ -127,6 +167,13 @@ struct pg_ctx {
         cache_cap = cache_pow2_cap = cache_cells_cap = cache_stride = 0;
         cache_ready = false;
         gv = GuideView{};
+    }
+    void free_mlp() {
+        Mlp& m = mlp;
+        dfree(m.params); dfree(m.pm); dfree(m.pv); dfree(m.mirror);
+        dfree(m.grid); dfree(m.gm); dfree(m.gv); dfree(m.ggrad); dfree(m.grid_bf);
+        dfree(m.partial); dfree(m.batch); dfree(m.rows); dfree(m.loss); dfree(m.scal); dfree(m.status);
+        m = Mlp{};
     }
     void free_paths() {
         dfree(ps.beta); dfree(ps.radiance); dfree(ps.film_pos); dfree(ps.info);
@@ -195,6 +242,9 @@ pg_status pg_destroy(pg_ctx* c) {
     c->free_records();
     c->free_cache();
     dfree(c->records);
+    dfree(c->rec_aux);
+    c->free_mlp();
+    dfree(c->mix);
     dfree(c->guide_cells);
     dfree(c->film);
     dfree(c->d_stats);
@@ -263,6 +313,7 @@ pg_status pg_scene_upload(pg_ctx* c, const float* positions, const float* normal
     d.radius = h.radius;
     d.cam = h.camera;
     c->has_scene = true;
+    c->mlp.ready = false;  // the guide's position frame belongs to the previous scene
     // film for the camera resolution
     uint32_t W = (uint32_t)h.camera.width, H = (uint32_t)h.camera.height;
     uint32_t fn = W * H;
@@ -456,13 +507,14 @@ static pg_status ensure_scan(pg_ctx* c, uint32_t n) {
 
 static pg_status ensure_record_state(pg_ctx* c, uint32_t n_paths, uint32_t depth) {
     if (c->rs_paths >= n_paths && c->rs_depth >= depth) return PG_OK;
-    dfree(c->rs.vpos); dfree(c->rs.vnrm); dfree(c->rs.vwi); dfree(c->rs.vw); dfree(c->rs.ve); dfree(c->rs.vme);
-    dfree(c->rs.vd); dfree(c->rs.nv);
+    dfree(c->rs.vpos); dfree(c->rs.vnrm); dfree(c->rs.vwi); dfree(c->rs.vwo); dfree(c->rs.vw); dfree(c->rs.ve);
+    dfree(c->rs.vme); dfree(c->rs.vd); dfree(c->rs.nv);
     dfree(c->rec_counts); dfree(c->rec_offsets);
     size_t n = (size_t)n_paths * depth;
     PG_TRY(c, dalloc(&c->rs.vpos, n), "alloc record vertices");
     PG_TRY(c, dalloc(&c->rs.vnrm, n), "alloc record vertices");
     PG_TRY(c, dalloc(&c->rs.vwi, n), "alloc record vertices");
+    PG_TRY(c, dalloc(&c->rs.vwo, n), "alloc record vertices");
     PG_TRY(c, dalloc(&c->rs.vw, n), "alloc record vertices");
     PG_TRY(c, dalloc(&c->rs.ve, n), "alloc record vertices");
     PG_TRY(c, dalloc(&c->rs.vme, n), "alloc record vertices");
@@ -482,13 +534,24 @@ static pg_status ensure_record_capacity(pg_ctx* c, uint64_t need) {
     while (cap < need) cap *= 2;
     if (cap > 0xffffffffull) return c->fail(PG_ERR_OVERFLOW, "record buffer exceeds 2^32 records");
     Record* nb = nullptr;
+    TrainAux* na = nullptr;
     PG_TRY(c, dalloc(&nb, cap), "alloc records");
+    pg_status st = c->hip(dalloc(&na, cap), "alloc record training data");
+    if (st != PG_OK) {
+        dfree(nb);
+        return st;
+    }
     if (c->records && c->n_records)
         PG_TRY(c, hipMemcpyAsync(nb, c->records, c->n_records * sizeof(Record), hipMemcpyDeviceToDevice, c->stream),
                "grow records");
+    if (c->rec_aux && c->n_aux)
+        PG_TRY(c, hipMemcpyAsync(na, c->rec_aux, c->n_aux * sizeof(TrainAux), hipMemcpyDeviceToDevice, c->stream),
+               "grow records");
     PG_TRY(c, hipStreamSynchronize(c->stream), "sync");
     dfree(c->records);
+    dfree(c->rec_aux);
     c->records = nb;
+    c->rec_aux = na;
     c->rec_capacity = cap;
     return PG_OK;
 }
@@ -498,30 +561,51 @@ pg_status pg_render_spp(pg_ctx* c, const pg_render_params* params, uint32_t n_sp
     if (!c->has_scene) return c->fail(PG_ERR_STATE, "no scene uploaded");
     pg_render_params prm = params ? *params : pg_render_params_default();
     const int integ = prm.integrator;
-    if (integ != PG_INTEGRATOR_PATH && integ != PG_INTEGRATOR_RECORD && integ != PG_INTEGRATOR_GUIDED_QT)
+    if (integ != PG_INTEGRATOR_PATH && integ != PG_INTEGRATOR_RECORD && integ != PG_INTEGRATOR_GUIDED_QT &&
+        integ != PG_INTEGRATOR_GUIDED_MLP)
         return c->fail(PG_ERR_INVALID, "unknown integrator");
     if (integ == PG_INTEGRATOR_GUIDED_QT && !c->cache_ready)
         return c->fail(PG_ERR_STATE, "guided rendering needs pg_cache_build first");
+    if (integ == PG_INTEGRATOR_GUIDED_MLP && !c->mlp.ready)
+        return c->fail(PG_ERR_STATE, "learned guiding needs pg_mlp_init first");
+    // shade mode: 0 BSDF sampling, 1 quadtree guide, 2 learned guide; a learned guide
+    // whose training diverged falls back to the quadtree guide when a cache exists
+    int mode = integ == PG_INTEGRATOR_GUIDED_QT ? 1 : 0;
+    if (integ == PG_INTEGRATOR_GUIDED_MLP) {
+        if (!c->mlp.diverged) {
+            mode = 2;
+        } else {
+            mode = c->cache_ready ? 1 : 0;
+            c->mlp.fallbacks += n_spp;
+        }
+    }
     if (prm.max_depth > 64) return c->fail(PG_ERR_INVALID, "max_depth > 64");
     if (!(prm.guide_prob >= 0.f && prm.guide_prob <= 1.f)) return c->fail(PG_ERR_INVALID, "guide_prob outside [0,1]");
     uint32_t W = (uint32_t)c->host.camera.width, H = (uint32_t)c->host.camera.height;
     uint32_t n = W * H;
     pg_status st = ensure_paths(c, n);
     if (st != PG_OK) return st;
-    if (integ == PG_INTEGRATOR_RECORD && (st = ensure_record_state(c, n, prm.max_depth + 1)) != PG_OK) return st;
+    // record passes: the reference's record integrator, optionally guided by the learned guide
+    const bool record_pass = integ == PG_INTEGRATOR_RECORD || (integ == PG_INTEGRATOR_GUIDED_MLP && prm.record);
+    if (record_pass && (st = ensure_record_state(c, n, prm.max_depth + 1)) != PG_OK) return st;
     c->rs.li_nee = prm.record_nee ? 1 : 0;
-    if (integ == PG_INTEGRATOR_GUIDED_QT && c->guide_cells_cap < n) {
+    if (mode == 1 && c->guide_cells_cap < n) {
         dfree(c->guide_cells);
         PG_TRY(c, dalloc(&c->guide_cells, (size_t)n * kKnnMax), "alloc guide cells");
         c->guide_cells_cap = n;
+    }
+    if (mode == 2 && c->mix_cap < n) {
+        dfree(c->mix);
+        PG_TRY(c, dalloc(&c->mix, (size_t)n * kMixF4), "alloc guide mixtures");
+        c->mix_cap = n;
     }
     const int stack = (int)c->host.bvh.max_depth;
     const uint32_t grid_trace = trace_grid(c->cu_count, stack);
     const uint32_t grid_shade = (uint32_t)c->cu_count * 8;
     const uint32_t grid_guide = (uint32_t)c->cu_count * 5;
-    GuideCtx gc{c->gv, integ == PG_INTEGRATOR_GUIDED_QT ? c->guide_cells : nullptr};
+    const uint32_t grid_mlp = (uint32_t)c->cu_count * 2;
+    GuideCtx gc{c->gv, mode == 1 ? c->guide_cells : nullptr, mode == 2 ? c->mix : nullptr};
     gc.g.guide_prob = prm.guide_prob;
-    const int mode = integ == PG_INTEGRATOR_GUIDED_QT ? 1 : 0;
     c->pool_used = 0;
     c->launches.clear();
     auto timed = [&](int kind, auto&& fn) -> hipError_t {
@@ -547,8 +631,11 @@ pg_status pg_render_spp(pg_ctx* c, const pg_render_params* params, uint32_t n_sp
             int parity = (int)(depth & 1u);
             PG_TRY(c, timed(0, [&] { return launch_extend(c->dev, c->ps, parity, stack, c->counting, grid_trace, c->stream); }), "extend");
             c->n_extend++;
-            if (integ == PG_INTEGRATOR_RECORD) {
-                PG_TRY(c, timed(1, [&] { return launch_shade_record(c->dev, c->ps, pp, c->rs, parity, (int)depth, grid_shade, c->stream); }), "shade_record");
+            if (record_pass) {
+                if (mode == 2)
+                    PG_TRY(c, timed(4, [&] { return launch_mlp_infer(c->dev, c->ps, c->mlp.view, parity, c->mix, grid_mlp, c->stream); }), "mlp_infer");
+                const float4* mx = mode == 2 ? c->mix : nullptr;
+                PG_TRY(c, timed(1, [&] { return launch_shade_record(c->dev, c->ps, pp, c->rs, mx, parity, (int)depth, grid_shade, c->stream); }), "shade_record");
                 if (c->rs.li_nee) {
                     PG_TRY(c, timed(2, [&] { return launch_shadow_record(c->dev, c->ps, c->rs, stack, grid_trace, c->stream); }), "shadow_record");
                     c->n_shadow++;
@@ -557,11 +644,13 @@ pg_status pg_render_spp(pg_ctx* c, const pg_render_params* params, uint32_t n_sp
             }
             if (mode == 1)
                 PG_TRY(c, timed(4, [&] { return launch_guide_cells(c->dev, c->ps, c->gv, c->guide_cells, parity, grid_guide, c->stream); }), "guide_cells");
+            if (mode == 2)
+                PG_TRY(c, timed(4, [&] { return launch_mlp_infer(c->dev, c->ps, c->mlp.view, parity, c->mix, grid_mlp, c->stream); }), "mlp_infer");
             PG_TRY(c, timed(1, [&] { return launch_shade(c->dev, c->ps, pp, gc, mode, parity, (int)depth, grid_shade, c->stream); }), "shade");
             PG_TRY(c, timed(2, [&] { return launch_shadow(c->dev, c->ps, stack, c->counting, grid_trace, c->stream); }), "shadow");
             c->n_shadow++;
         }
-        if (integ == PG_INTEGRATOR_RECORD) {
+        if (record_pass) {
             // back-propagate, then append this pass's records in (path, depth) order
             PG_TRY(c, launch_backprop_count(c->rs, n, c->rec_counts, c->stream), "backprop count");
             PG_TRY(c, exclusive_scan(c->rec_counts, c->rec_offsets, n, c->scan_tmp, c->d_total, c->stream), "scan");
@@ -575,9 +664,14 @@ pg_status pg_render_spp(pg_ctx* c, const pg_render_params* params, uint32_t n_sp
                 if (st != PG_OK) return st;
                 cap = (uint32_t)std::min<uint64_t>(c->rec_capacity, 0xffffffffull);
             }
-            PG_TRY(c, launch_backprop_write(c->rs, n, c->rec_offsets, (uint32_t)base, c->records, cap,
+            TrainAux* aux = (prm.record && c->n_aux == c->n_records) ? c->rec_aux : nullptr;
+            PG_TRY(c, launch_backprop_write(c->rs, n, c->rec_offsets, (uint32_t)base, c->records, aux, cap,
                                             c->ps.radiance, c->stream), "backprop write");
-            if (prm.record) c->n_records += total;
+            if (prm.record) {
+                if (aux) c->n_aux += total;
+                c->n_records += total;
+                c->rec_gen++;
+            }
             c->cache_ready = false;
         }
         PG_TRY(c, timed(3, [&] { return launch_resolve(c->ps, pp, c->film, c->stream); }), "resolve");
@@ -615,6 +709,8 @@ pg_status pg_records_read(pg_ctx* c, pg_path_vertex* out, uint64_t cap, uint64_t
 pg_status pg_records_clear(pg_ctx* c) {
     PG_CHECK_CTX(c);
     c->n_records = 0;
+    c->n_aux = 0;
+    c->rec_gen++;
     c->cache_ready = false;
     return PG_OK;
 }
@@ -626,6 +722,8 @@ pg_status pg_records_upload(pg_ctx* c, const pg_path_vertex* in, uint64_t n) {
     if (st != PG_OK) return st;
     if (n) PG_TRY(c, hipMemcpy(c->records, in, n * sizeof(Record), hipMemcpyHostToDevice), "upload records");
     c->n_records = n;
+    c->n_aux = 0;
+    c->rec_gen++;
     c->cache_ready = false;
     return PG_OK;
 }
@@ -850,6 +948,401 @@ pg_status pg_set_counting(pg_ctx* c, int32_t enable) {
     PG_CHECK_CTX(c);
     c->counting = enable != 0;
     return PG_OK;
+}
+
+
+// ------------------------------------------------------------------ learned guide
+pg_mlp_config pg_mlp_config_default(void) {
+    pg_mlp_config m;
+    std::memset(&m, 0, sizeof(m));
+    m.seed = 1;
+    m.log2_table = 19;
+    m.res_min = 16.f;
+    m.res_max = 1024.f;
+    m.lr = 1e-2f;
+    m.beta1 = 0.9f;
+    m.beta2 = 0.99f;
+    m.eps = 1e-15f;
+    m.batch = 1u << 18;
+    m.weight_clip = 100.f;
+    m.act_budget = 512ull << 20;
+    m.nan_step = 0;
+    return m;
+}
+
+static uint32_t mlp_chunk(const pg_mlp_config& cfg, uint32_t n) {
+    uint64_t rows = cfg.act_budget / (uint64_t)(kRowLen * 2);
+    rows &= ~127ull;
+    if (rows < 128) rows = 128;
+    return (uint32_t)std::min<uint64_t>(rows, n);
+}
+
+static pg_status mlp_buffers(pg_ctx* c, uint32_t batch, uint32_t steps) {
+    auto& m = c->mlp;
+    const uint32_t chunk = mlp_chunk(m.cfg, batch);
+    const uint32_t groups = mlp_wgrad_groups(chunk);
+    if (m.batch_cap < batch) {
+        dfree(m.batch); dfree(m.loss);
+        PG_TRY(c, dalloc(&m.batch, batch), "alloc training batch");
+        PG_TRY(c, dalloc(&m.loss, batch), "alloc training batch");
+        m.batch_cap = m.loss_cap = batch;
+    }
+    if (m.rows_cap < chunk) {
+        dfree(m.rows);
+        PG_TRY(c, dalloc(&m.rows, (size_t)chunk * kRowLen), "alloc activation rows");
+        m.rows_cap = chunk;
+    }
+    if (m.groups_cap < groups) {
+        dfree(m.partial);
+        PG_TRY(c, dalloc(&m.partial, (size_t)groups * kParams), "alloc gradient partials");
+        m.groups_cap = groups;
+    }
+    if (m.scal_cap < steps + 1) {
+        dfree(m.scal);
+        PG_TRY(c, dalloc(&m.scal, steps + 1), "alloc loss history");
+        m.scal_cap = steps + 1;
+    }
+    return PG_OK;
+}
+
+// Forward/backward of `batch` samples already in m.batch: per chunk the fused
+// kernel then the weight-gradient GEMM (partials accumulate over chunks).
+static pg_status mlp_backward(pg_ctx* c, uint32_t batch, int nan_inject, uint32_t* groups_out) {
+    auto& m = c->mlp;
+    const uint32_t chunk = mlp_chunk(m.cfg, batch);
+    const uint32_t groups = mlp_wgrad_groups(chunk);
+    uint32_t k = 0;
+    for (uint32_t s0 = 0; s0 < batch; s0 += chunk, ++k) {
+        uint32_t cn = std::min(chunk, batch - s0);
+        PG_TRY(c, launch_mlp_train_chunk(m.view, m.batch + s0, cn, 1.f / (float)batch, m.rows, m.loss + s0, m.ggrad,
+                                         (nan_inject && k == 0) ? 1 : 0, c->cu_count, c->stream), "mlp train");
+        PG_TRY(c, launch_mlp_wgrad(m.rows, cn, groups, m.partial, k > 0 ? 1 : 0, c->stream), "mlp wgrad");
+    }
+    m.chunks = k;
+    *groups_out = groups;
+    return PG_OK;
+}
+
+pg_status pg_mlp_init(pg_ctx* c, const pg_mlp_config* cfg_in) {
+    PG_CHECK_CTX(c);
+    if (!c->has_scene) return c->fail(PG_ERR_STATE, "no scene uploaded");
+    pg_mlp_config cfg = cfg_in ? *cfg_in : pg_mlp_config_default();
+    if (cfg.log2_table < 12 || cfg.log2_table > 22) return c->fail(PG_ERR_INVALID, "log2_table outside 12..22");
+    if (!(cfg.res_min >= 1.f) || !(cfg.res_max >= cfg.res_min) || !(cfg.res_max <= 1048576.f))
+        return c->fail(PG_ERR_INVALID, "grid resolutions");
+    if (!(cfg.lr > 0.f) || !(cfg.beta1 >= 0.f && cfg.beta1 < 1.f) || !(cfg.beta2 >= 0.f && cfg.beta2 < 1.f) ||
+        !(cfg.eps >= 0.f))
+        return c->fail(PG_ERR_INVALID, "Adam parameters");
+    if (cfg.batch == 0 || cfg.batch > (1u << 24)) return c->fail(PG_ERR_INVALID, "batch outside 1..2^24");
+    if (!(cfg.weight_clip > 0.f)) return c->fail(PG_ERR_INVALID, "weight_clip must be positive");
+    c->free_mlp();
+    auto& m = c->mlp;
+    m.cfg = cfg;
+    const size_t T = (size_t)1 << cfg.log2_table;
+    m.n_grid = (size_t)kHashLevels * T * 2;
+    PG_TRY(c, dalloc(&m.params, kParams), "alloc network");
+    PG_TRY(c, dalloc(&m.pm, kParams), "alloc network");
+    PG_TRY(c, dalloc(&m.pv, kParams), "alloc network");
+    PG_TRY(c, dalloc(&m.mirror, kMirror), "alloc network");
+    PG_TRY(c, dalloc(&m.grid, m.n_grid), "alloc hash grid");
+    PG_TRY(c, dalloc(&m.gm, m.n_grid), "alloc hash grid");
+    PG_TRY(c, dalloc(&m.gv, m.n_grid), "alloc hash grid");
+    PG_TRY(c, dalloc(&m.ggrad, m.n_grid), "alloc hash grid");
+    PG_TRY(c, dalloc(&m.grid_bf, m.n_grid / 2), "alloc hash grid");
+    PG_TRY(c, dalloc(&m.status, 1), "alloc status");
+    // initialisation: Xavier-uniform weights over the used inputs/outputs, zero
+    // biases and padding, hash features uniform in +-1e-4 (counter hash of the seed)
+    std::vector<float> hp(kParams, 0.f);
+    auto uni = [&](uint32_t stream, uint32_t i) { return keyed_rand(path_key(cfg.seed, stream, i), 0) * 2.f - 1.f; };
+    auto fill = [&](int off, int rows, int cols, int used_rows, int used_cols, uint32_t stream) {
+        float a = std::sqrt(6.f / (float)(used_rows + used_cols));
+        for (int r = 0; r < used_rows; ++r)
+            for (int q = 0; q < used_cols; ++q) hp[off + r * cols + q] = a * uni(stream, (uint32_t)(r * cols + q));
+        (void)rows;
+    };
+    fill(kOffW1, kHidden, kInputs, kHidden, kInputsUsed, 11);
+    fill(kOffW2, kHidden, kHidden, kHidden, kHidden, 12);
+    fill(kOffW3, kHidden, kHidden, kHidden, kHidden, 13);
+    fill(kOffW4, kOutputs, kHidden, kOutputsUsed, kHidden, 14);
+    std::vector<float> hg(m.n_grid);
+    for (size_t i = 0; i < m.n_grid; ++i) hg[i] = 1e-4f * uni(15, (uint32_t)i);
+    PG_TRY(c, hipMemcpy(m.params, hp.data(), kParams * 4, hipMemcpyHostToDevice), "init network");
+    PG_TRY(c, hipMemcpy(m.grid, hg.data(), m.n_grid * 4, hipMemcpyHostToDevice), "init hash grid");
+    PG_TRY(c, hipMemset(m.pm, 0, kParams * 4), "init moments");
+    PG_TRY(c, hipMemset(m.pv, 0, kParams * 4), "init moments");
+    PG_TRY(c, hipMemset(m.gm, 0, m.n_grid * 4), "init moments");
+    PG_TRY(c, hipMemset(m.gv, 0, m.n_grid * 4), "init moments");
+    PG_TRY(c, hipMemset(m.ggrad, 0, m.n_grid * 4), "init gradients");
+    PG_TRY(c, hipMemset(m.mirror, 0, kMirror * 2), "init mirror");
+    PG_TRY(c, hipMemset(m.status, 0, 4), "init status");
+    PG_TRY(c, launch_mlp_mirror(m.params, m.mirror, c->stream), "mirror");
+    PG_TRY(c, launch_grid_mirror(m.grid, m.grid_bf, m.n_grid, c->stream), "mirror");
+    PG_TRY(c, hipStreamSynchronize(c->stream), "init");
+    // view: levels res_l = floor(res_min b^l), b = (res_max / res_min)^(1/15)
+    MlpView& v = m.view;
+    v.wmirror = m.mirror;
+    v.params = m.params;
+    v.grid = m.grid_bf;
+    v.table_mask = (uint32_t)(T - 1);
+    v.log2_table = cfg.log2_table;
+    const double b = std::exp((std::log((double)cfg.res_max) - std::log((double)cfg.res_min)) / (kHashLevels - 1));
+    for (int l = 0; l < kHashLevels; ++l) {
+        double r = std::floor((double)cfg.res_min * std::pow(b, (double)l) + 1e-6);
+        v.res[l] = (uint32_t)std::max(1.0, r);
+        double cells = std::pow((double)v.res[l] + 1.0, 3.0);
+        v.dense[l] = cells <= (double)T ? 1u : 0u;
+    }
+    const PreparedScene& h = c->host;
+    for (int a = 0; a < 3; ++a) v.lo[a] = h.centre[a] - h.radius;
+    v.inv_ext = h.radius > 0.f ? 1.f / (2.f * h.radius) : 0.f;
+    m.ready = true;
+    m.window_passes = 0;
+    return PG_OK;
+}
+
+pg_status pg_mlp_train(pg_ctx* c, uint32_t steps, float* losses) {
+    PG_CHECK_CTX(c);
+    auto& m = c->mlp;
+    if (!m.ready) return c->fail(PG_ERR_STATE, "pg_mlp_init first");
+    if (c->n_records == 0) return c->fail(PG_ERR_STATE, "no records to train on");
+    if (c->n_aux != c->n_records)
+        return c->fail(PG_ERR_STATE, "records carry no training data (only record passes keep it)");
+    if (c->n_records > 0xffffffffull) return c->fail(PG_ERR_OVERFLOW, "more than 2^32 records");
+    if (steps == 0) return PG_OK;
+    const uint32_t n = (uint32_t)c->n_records;
+    const uint32_t B = m.cfg.batch;
+    pg_status st = mlp_buffers(c, B, steps);
+    if (st != PG_OK) return st;
+    if (m.w_scale_gen != c->rec_gen) {
+        // mean sample weight over every record (deterministic one-block sum)
+        PG_TRY(c, launch_sum(reinterpret_cast<const float*>(c->rec_aux) + 3, n, 8, m.scal, c->stream), "weight sum");
+        float sum = 0.f;
+        PG_TRY(c, hipMemcpyAsync(&sum, m.scal, 4, hipMemcpyDeviceToHost, c->stream), "weight sum");
+        PG_TRY(c, hipStreamSynchronize(c->stream), "weight sum");
+        double mean = (double)sum / (double)n;
+        m.w_scale = (mean > 0.0 && std::isfinite(mean)) ? (float)(1.0 / mean) : 1.f;
+        m.w_scale_gen = c->rec_gen;
+    }
+    static const uint32_t kPrimes[5] = {2654435761u, 2246822519u, 3266489917u, 668265263u, 374761393u};
+    hipEvent_t e0 = c->ev(), e1 = c->ev();
+    (void)hipEventRecord(e0, c->stream);
+    for (uint32_t k = 0; k < steps; ++k) {
+        const uint32_t t = m.step + 1;
+        AdamParams ap{m.cfg.lr, m.cfg.beta1, m.cfg.beta2, m.cfg.eps,
+                      (float)(1.0 - std::pow((double)m.cfg.beta1, (double)t)),
+                      (float)(1.0 - std::pow((double)m.cfg.beta2, (double)t))};
+        uint32_t offset = hash32(m.cfg.seed ^ hash32(t * 0x9E3779B9u)) % n;
+        uint32_t stride = 1;
+        for (int q = 0; q < 5; ++q) {
+            uint32_t p = kPrimes[(t + q) % 5];
+            if (p % n != 0) { stride = p % n; break; }
+        }
+        PG_TRY(c, launch_mlp_gather(c->records, c->rec_aux, n, offset, stride, B, m.w_scale, m.cfg.weight_clip,
+                                    m.batch, c->stream), "mlp gather");
+        uint32_t groups = 0;
+        st = mlp_backward(c, B, m.cfg.nan_step == t ? 1 : 0, &groups);
+        if (st != PG_OK) return st;
+        PG_TRY(c, launch_mlp_adam(m.params, m.pm, m.pv, m.mirror, m.partial, groups, ap, 1, nullptr, m.status,
+                                  c->stream), "mlp adam");
+        PG_TRY(c, launch_grid_adam(m.grid, m.grid_bf, m.gm, m.gv, m.ggrad, m.n_grid, ap, 1, nullptr, m.status,
+                                   c->stream), "grid adam");
+        PG_TRY(c, launch_sum(m.loss, B, 1, m.scal + k, c->stream), "loss");
+        m.step = t;
+    }
+    (void)hipEventRecord(e1, c->stream);
+    std::vector<float> hl(steps);
+    uint32_t status = 0;
+    PG_TRY(c, hipMemcpyAsync(hl.data(), m.scal, steps * 4, hipMemcpyDeviceToHost, c->stream), "read losses");
+    PG_TRY(c, hipMemcpyAsync(&status, m.status, 4, hipMemcpyDeviceToHost, c->stream), "read status");
+    PG_TRY(c, hipStreamSynchronize(c->stream), "train");
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    c->pool_used = 0;
+    m.train_ms = ms;
+    m.last_loss = hl[steps - 1];
+    for (float l : hl)
+        if (!std::isfinite(l)) status |= 4u;
+    if (status) m.diverged = true;
+    if (losses) std::memcpy(losses, hl.data(), steps * 4);
+    return PG_OK;
+}
+
+pg_status pg_mlp_render_online(pg_ctx* c, const pg_render_params* params, uint32_t n_spp, uint32_t steps_per_pass,
+                                uint32_t window) {
+    PG_CHECK_CTX(c);
+    if (!c->mlp.ready) return c->fail(PG_ERR_STATE, "pg_mlp_init first");
+    pg_render_params p = params ? *params : pg_render_params_default();
+    p.integrator = PG_INTEGRATOR_GUIDED_MLP;
+    p.record = 1;
+    if (window == 0) window = 1;
+    for (uint32_t s = 0; s < n_spp; ++s) {
+        if (c->mlp.window_passes >= window) {
+            c->n_records = 0;
+            c->n_aux = 0;
+            c->rec_gen++;
+            c->cache_ready = false;
+            c->mlp.window_passes = 0;
+        }
+        pg_status st = pg_render_spp(c, &p, 1);
+        if (st != PG_OK) return st;
+        c->mlp.window_passes++;
+        if (steps_per_pass && !c->mlp.diverged && c->n_records > 0) {
+            st = pg_mlp_train(c, steps_per_pass, nullptr);
+            if (st != PG_OK) return st;
+        }
+    }
+    return PG_OK;
+}
+
+pg_status pg_mlp_info_read(pg_ctx* c, pg_mlp_info* info) {
+    PG_CHECK_CTX(c);
+    if (!info) return c->fail(PG_ERR_INVALID, "null info");
+    const auto& m = c->mlp;
+    std::memset(info, 0, sizeof(*info));
+    info->ready = m.ready ? 1u : 0u;
+    info->diverged = m.diverged ? 1u : 0u;
+    info->step = m.step;
+    info->n_params = kParams;
+    info->n_grid_params = m.n_grid;
+    info->last_loss = m.last_loss;
+    info->w_scale = m.w_scale;
+    info->chunks = m.chunks;
+    info->train_ms = m.train_ms;
+    info->fallbacks = m.fallbacks;
+    return PG_OK;
+}
+
+pg_status pg_mlp_get_params(pg_ctx* c, float* params, float* grid) {
+    PG_CHECK_CTX(c);
+    const auto& m = c->mlp;
+    if (!m.ready) return c->fail(PG_ERR_STATE, "pg_mlp_init first");
+    if (params) PG_TRY(c, hipMemcpy(params, m.params, kParams * 4, hipMemcpyDeviceToHost), "read network");
+    if (grid) PG_TRY(c, hipMemcpy(grid, m.grid, m.n_grid * 4, hipMemcpyDeviceToHost), "read hash grid");
+    return PG_OK;
+}
+
+pg_status pg_mlp_set_params(pg_ctx* c, const float* params, const float* grid) {
+    PG_CHECK_CTX(c);
+    auto& m = c->mlp;
+    if (!m.ready) return c->fail(PG_ERR_STATE, "pg_mlp_init first");
+    if (params) {
+        PG_TRY(c, hipMemcpy(m.params, params, kParams * 4, hipMemcpyHostToDevice), "write network");
+        PG_TRY(c, hipMemset(m.pm, 0, kParams * 4), "reset moments");
+        PG_TRY(c, hipMemset(m.pv, 0, kParams * 4), "reset moments");
+        PG_TRY(c, launch_mlp_mirror(m.params, m.mirror, c->stream), "mirror");
+    }
+    if (grid) {
+        PG_TRY(c, hipMemcpy(m.grid, grid, m.n_grid * 4, hipMemcpyHostToDevice), "write hash grid");
+        PG_TRY(c, hipMemset(m.gm, 0, m.n_grid * 4), "reset moments");
+        PG_TRY(c, hipMemset(m.gv, 0, m.n_grid * 4), "reset moments");
+        PG_TRY(c, launch_grid_mirror(m.grid, m.grid_bf, m.n_grid, c->stream), "mirror");
+    }
+    PG_TRY(c, hipStreamSynchronize(c->stream), "set params");
+    return PG_OK;
+}
+
+pg_status pg_mlp_levels(pg_ctx* c, uint32_t* res, uint32_t* dense, float* frame) {
+    PG_CHECK_CTX(c);
+    const auto& m = c->mlp;
+    if (!m.ready) return c->fail(PG_ERR_STATE, "pg_mlp_init first");
+    for (int l = 0; l < kHashLevels; ++l) {
+        if (res) res[l] = m.view.res[l];
+        if (dense) dense[l] = m.view.dense[l];
+    }
+    if (frame) {
+        frame[0] = m.view.lo[0]; frame[1] = m.view.lo[1]; frame[2] = m.view.lo[2];
+        frame[3] = m.view.inv_ext;
+    }
+    return PG_OK;
+}
+
+pg_status pg_mlp_eval(pg_ctx* c, uint32_t n, const float* pos, const float* nrm, const float* wo, float* mix,
+                      const float* wi_eval, float* pdf_eval, const float* u3, float* wi_out) {
+    PG_CHECK_CTX(c);
+    const auto& m = c->mlp;
+    if (!m.ready) return c->fail(PG_ERR_STATE, "pg_mlp_init first");
+    if (n == 0) return PG_OK;
+    if (!pos || !nrm || !wo) return c->fail(PG_ERR_INVALID, "null inputs");
+    if ((wi_eval != nullptr) != (pdf_eval != nullptr) || (u3 != nullptr) != (wi_out != nullptr))
+        return c->fail(PG_ERR_INVALID, "wi_eval/pdf_eval and u3/wi_out come in pairs");
+    auto pack4 = [n](const float* p) {
+        std::vector<float4> v(n);
+        for (uint32_t i = 0; i < n; ++i) v[i] = make_float4(p[3 * i], p[3 * i + 1], p[3 * i + 2], 0.f);
+        return v;
+    };
+    std::vector<float4> hx = pack4(pos), hn = pack4(nrm), hw = pack4(wo);
+    float4 *dx = nullptr, *dn = nullptr, *dw = nullptr, *dm = nullptr, *de = nullptr, *du = nullptr, *dwo = nullptr;
+    float* dpe = nullptr;
+    pg_status st = PG_OK;
+    do {
+        if ((st = c->hip(dalloc(&dx, n), "alloc")) != PG_OK) break;
+        if ((st = c->hip(dalloc(&dn, n), "alloc")) != PG_OK) break;
+        if ((st = c->hip(dalloc(&dw, n), "alloc")) != PG_OK) break;
+        if ((st = c->hip(dalloc(&dm, (size_t)n * kMixF4), "alloc")) != PG_OK) break;
+        hipMemcpy(dx, hx.data(), (size_t)n * 16, hipMemcpyHostToDevice);
+        hipMemcpy(dn, hn.data(), (size_t)n * 16, hipMemcpyHostToDevice);
+        if ((st = c->hip(hipMemcpy(dw, hw.data(), (size_t)n * 16, hipMemcpyHostToDevice), "copy")) != PG_OK) break;
+        if ((st = c->hip(launch_mlp_eval(m.view, dx, dn, dw, n, dm, c->cu_count, c->stream), "mlp eval")) != PG_OK)
+            break;
+        if (wi_eval) {
+            std::vector<float4> he = pack4(wi_eval);
+            if ((st = c->hip(dalloc(&de, n), "alloc")) != PG_OK) break;
+            if ((st = c->hip(dalloc(&dpe, n), "alloc")) != PG_OK) break;
+            if ((st = c->hip(hipMemcpy(de, he.data(), (size_t)n * 16, hipMemcpyHostToDevice), "copy")) != PG_OK) break;
+        }
+        if (u3) {
+            std::vector<float4> hu = pack4(u3);
+            if ((st = c->hip(dalloc(&du, n), "alloc")) != PG_OK) break;
+            if ((st = c->hip(dalloc(&dwo, n), "alloc")) != PG_OK) break;
+            if ((st = c->hip(hipMemcpy(du, hu.data(), (size_t)n * 16, hipMemcpyHostToDevice), "copy")) != PG_OK) break;
+        }
+        if (de || du)
+            if ((st = c->hip(launch_mix_query(dm, n, de, du, dpe, dwo, c->stream), "mixture query")) != PG_OK) break;
+        if ((st = c->hip(hipStreamSynchronize(c->stream), "sync")) != PG_OK) break;
+        if (mix && (st = c->hip(hipMemcpy(mix, dm, (size_t)n * kMixF4 * 16, hipMemcpyDeviceToHost), "copy")) != PG_OK)
+            break;
+        if (pdf_eval && (st = c->hip(hipMemcpy(pdf_eval, dpe, (size_t)n * 4, hipMemcpyDeviceToHost), "copy")) != PG_OK)
+            break;
+        if (wi_out && (st = c->hip(hipMemcpy(wi_out, dwo, (size_t)n * 16, hipMemcpyDeviceToHost), "copy")) != PG_OK)
+            break;
+    } while (0);
+    dfree(dx); dfree(dn); dfree(dw); dfree(dm); dfree(de); dfree(du); dfree(dwo); dfree(dpe);
+    return st;
+}
+
+pg_status pg_mlp_gradient(pg_ctx* c, uint32_t n, const float* samples, float* loss, float* grad_params,
+                          float* grad_grid) {
+    PG_CHECK_CTX(c);
+    auto& m = c->mlp;
+    if (!m.ready) return c->fail(PG_ERR_STATE, "pg_mlp_init first");
+    if (n == 0 || !samples) return c->fail(PG_ERR_INVALID, "no samples");
+    if (n > (1u << 24)) return c->fail(PG_ERR_INVALID, "more than 2^24 samples");
+    pg_status st = mlp_buffers(c, n, 1);
+    if (st != PG_OK) return st;
+    PG_TRY(c, hipMemcpy(m.batch, samples, (size_t)n * sizeof(TrainSample), hipMemcpyHostToDevice), "upload samples");
+    float* dgp = nullptr;
+    float* dgg = nullptr;
+    do {
+        if ((st = c->hip(dalloc(&dgp, kParams), "alloc")) != PG_OK) break;
+        if (grad_grid && (st = c->hip(dalloc(&dgg, m.n_grid), "alloc")) != PG_OK) break;
+        uint32_t groups = 0;
+        if ((st = mlp_backward(c, n, 0, &groups)) != PG_OK) break;
+        AdamParams ap{m.cfg.lr, m.cfg.beta1, m.cfg.beta2, m.cfg.eps, 1.f, 1.f};
+        if ((st = c->hip(launch_mlp_adam(m.params, m.pm, m.pv, m.mirror, m.partial, groups, ap, 0, dgp, m.status,
+                                         c->stream), "gradient")) != PG_OK) break;
+        if ((st = c->hip(launch_grid_adam(m.grid, m.grid_bf, m.gm, m.gv, m.ggrad, m.n_grid, ap, 0, dgg, m.status,
+                                          c->stream), "gradient")) != PG_OK) break;
+        if ((st = c->hip(launch_sum(m.loss, n, 1, m.scal, c->stream), "loss")) != PG_OK) break;
+        if ((st = c->hip(hipStreamSynchronize(c->stream), "sync")) != PG_OK) break;
+        if (loss && (st = c->hip(hipMemcpy(loss, m.scal, 4, hipMemcpyDeviceToHost), "copy")) != PG_OK) break;
+        if (grad_params && (st = c->hip(hipMemcpy(grad_params, dgp, kParams * 4, hipMemcpyDeviceToHost), "copy")) != PG_OK)
+            break;
+        if (grad_grid && (st = c->hip(hipMemcpy(grad_grid, dgg, m.n_grid * 4, hipMemcpyDeviceToHost), "copy")) != PG_OK)
+            break;
+    } while (0);
+    dfree(dgp);
+    dfree(dgg);
+    return st;
 }
 
 }  // extern "C"

@@ -11,7 +11,8 @@ namespace pg {
 struct RecordState {
     float4* vpos;  // hit point, store flag in .w
     float4* vnrm;  // shading normal
-    float4* vwi;   // sampled incident direction
+    float4* vwi;   // sampled incident direction, sampling pdf q in .w
+    float4* vwo;   // outgoing direction (-ray), BSDF pdf of wi in .w
     float4* vw;    // bsdf weight f |cos| / (pdf rrp)
     float4* ve;    // emission (Le or background)
     float4* vme;   // MIS-weighted emission (Renderer.h:433-457, 503-525)
@@ -44,8 +45,9 @@ hipError_t exclusive_scan(const uint32_t* in, uint32_t* out, uint32_t n, uint32_
 hipError_t radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t* keys_tmp, uint32_t* vals_tmp, uint32_t n,
                             int bits, uint32_t* hist, uint32_t* scan_tmp, hipStream_t stream);
 hipError_t launch_backprop_count(const RecordState& rs, uint32_t n_paths, uint32_t* counts, hipStream_t stream);
+struct TrainAux;
 hipError_t launch_backprop_write(const RecordState& rs, uint32_t n_paths, const uint32_t* offsets, uint32_t base,
-                                 Record* out, uint32_t capacity, float4* radiance, hipStream_t stream);
+                                 Record* out, TrainAux* aux, uint32_t capacity, float4* radiance, hipStream_t stream);
 hipError_t launch_cache_build(const Record* rec, uint32_t n, float3 lo, float3 inv_ext, CacheBuffers& cb,
                               uint32_t cell_size, uint32_t qt_stride, hipStream_t stream);
 hipError_t launch_knn(const GuideView& g, const float* q, uint32_t nq, uint32_t k, uint32_t* idx, float* d2,

@@ -5,6 +5,7 @@
 #include "device_math.h"
 #include "guide_device.h"
 #include "guiding.h"
+#include "mlp.h"
 
 namespace pg {
 
@@ -207,8 +208,10 @@ __global__ void k_backprop_count(RecordState rs, uint32_t n_paths, uint32_t* cou
     counts[p] = c;
 }
 
+// Records of one pass in (path, depth) order; `aux` (nullable) receives each record's
+// training data: (wo, lum(Li * f|cos|/(q rr))) and (BSDF pdf of wi, q).
 __global__ void k_backprop_write(RecordState rs, uint32_t n_paths, const uint32_t* offsets, uint32_t base,
-                                 Record* out, uint32_t capacity, float4* radiance) {
+                                 Record* out, TrainAux* aux, uint32_t capacity, float4* radiance) {
     uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n_paths) return;
     uint32_t nv = rs.nv[p];
@@ -224,6 +227,12 @@ __global__ void k_backprop_write(RecordState rs, uint32_t n_paths, const uint32_
                 dst[0] = make_float4(vp.x, vp.y, vp.z, vn.x);
                 dst[1] = make_float4(vn.y, vn.z, vw.x, vw.y);
                 dst[2] = make_float4(vw.z, G.r, G.g, G.b);
+                if (aux) {
+                    float4 wo = rs.vwo[o], bw = rs.vw[o];
+                    float w = lum(G * c3(bw.x, bw.y, bw.z));
+                    aux[w_at].a = make_float4(wo.x, wo.y, wo.z, w);
+                    aux[w_at].b = make_float4(wo.w, vw.w, 0.f, 0.f);
+                }
             }
             ++w_at;
         }
@@ -239,9 +248,9 @@ hipError_t launch_backprop_count(const RecordState& rs, uint32_t n_paths, uint32
     return hipGetLastError();
 }
 hipError_t launch_backprop_write(const RecordState& rs, uint32_t n_paths, const uint32_t* offsets, uint32_t base,
-                                 Record* out, uint32_t capacity, float4* radiance, hipStream_t stream) {
+                                 Record* out, TrainAux* aux, uint32_t capacity, float4* radiance, hipStream_t stream) {
     hipLaunchKernelGGL(k_backprop_write, dim3((n_paths + 255) / 256), dim3(256), 0, stream, rs, n_paths, offsets, base,
-                       out, capacity, radiance);
+                       out, aux, capacity, radiance);
     return hipGetLastError();
 }
 

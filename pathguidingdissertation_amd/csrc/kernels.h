@@ -52,6 +52,7 @@ struct PathState {
 struct GuideCtx {
     GuideView g;
     const uint32_t* cells;
+    const float4* mix;  // learned guide: kMixF4 float4 per queue slot, c = 0 where unguided
 };
 
 struct PassParams {
@@ -85,10 +86,14 @@ hipError_t launch_copy(const float4* src, float4* dst, size_t n, int cu_count, h
 struct RecordState;
 hipError_t launch_guide_cells(const DevScene& s, const PathState& ps, const GuideView& g, uint32_t* cells, int parity,
                               uint32_t grid, hipStream_t stream);
+// mix != nullptr: learned-guide record pass (one-sample MIS with the mixtures of k_mlp_infer)
 hipError_t launch_shade_record(const DevScene& s, const PathState& ps, const PassParams& p, const RecordState& rs,
-                               int parity, int depth, uint32_t grid, hipStream_t stream);
+                               const float4* mix, int parity, int depth, uint32_t grid, hipStream_t stream);
 hipError_t launch_shadow_record(const DevScene& s, const PathState& ps, const RecordState& rs, int stack, uint32_t grid,
                                 hipStream_t stream);
+struct MlpView;
+hipError_t launch_mlp_infer(const DevScene& s, const PathState& ps, const MlpView& v, int parity, float4* mix,
+                            uint32_t grid, hipStream_t stream);
 int stack_entries(int max_depth);
 uint32_t trace_grid(int cu_count, int stack);
 

@@ -20,6 +20,7 @@
 #include "device_shading.h"
 #include "guide_device.h"
 #include "guiding.h"
+#include "mlp_device.h"
 
 namespace pg {
 
@@ -189,12 +190,29 @@ __device__ __forceinline__ F3 guide_sample_qt(const GuideView& g, const uint32_t
 // light, samples it and returns the unoccluded contribution and the shadow ray
 // (false when the contribution is zero before visibility is known). In guided mode
 // the BSDF density in the MIS weight is the one-sample mixture density.
+// Guide density of MODE 1 (quadtree mixture of the nearest cells) or MODE 2 (the
+// learned vMF mixture of this queue slot).
+template <int MODE>
+__device__ __forceinline__ float guide_pdf(const GuideView& g, const uint32_t* cells, const float4* mx, F3 wi) {
+    if (MODE == 1) return guide_pdf_qt(g, cells, wi);
+    return mix_pdf(mx, wi);
+}
+template <int MODE>
+__device__ __forceinline__ F3 guide_sample(const GuideView& g, const uint32_t* cells, const float4* mx, uint32_t key,
+                                           uint32_t& dim) {
+    if (MODE == 1) return guide_sample_qt(g, cells, key, dim);
+    float u0 = keyed_rand(key, dim++);
+    float u1 = keyed_rand(key, dim++);
+    float u2 = keyed_rand(key, dim++);
+    return mix_sample(mx, u0, u1, u2);
+}
+
 template <int MODE>
 __device__ __forceinline__ bool next_event(const DevScene& s, const MaterialRec& m, const SD& sd, uint32_t key,
-                                           uint32_t& dim, const GuideView& g, const uint32_t* cells, bool guided,
-                                           C3& contrib, F3& sh_o, F3& sh_d, float& sh_t) {
+                                           uint32_t& dim, const GuideView& g, const uint32_t* cells,
+                                           const float4* mx, bool guided, float ag, C3& contrib, F3& sh_o, F3& sh_d,
+                                           float& sh_t) {
     contrib = c3(0.f, 0.f, 0.f);
-    const float ag = g.guide_prob;
     float pmf = 1.f / (float)s.n_lights;
     uint32_t nl = (uint32_t)s.n_lights;
     float fl = floorf(keyed_rand(key, dim++) * (float)nl);
@@ -218,7 +236,7 @@ __device__ __forceinline__ bool next_event(const DevScene& s, const MaterialRec&
         float G = (ctl * ctp) / den;
         float pdf_b = 0.f;
         C3 f = bsdf_eval(m, sd, wi, pdf_b);
-        if (MODE == 1 && guided) pdf_b = (1.f - ag) * pdf_b + ag * guide_pdf_qt(g, cells, wi);
+        if (MODE != 0 && guided) pdf_b = (1.f - ag) * pdf_b + ag * guide_pdf<MODE>(g, cells, mx, wi);
         float pal = pdf_l * pmf;
         float pab = (pdf_b * ctp) / den;
         float w = power_heuristic(pal, pab);
@@ -238,7 +256,7 @@ __device__ __forceinline__ bool next_event(const DevScene& s, const MaterialRec&
     if (ctl <= 0.f) return false;
     float pdf_b = 0.f;
     C3 f = bsdf_eval(m, sd, wi, pdf_b);
-    if (MODE == 1 && guided) pdf_b = (1.f - ag) * pdf_b + ag * guide_pdf_qt(g, cells, wi);
+    if (MODE != 0 && guided) pdf_b = (1.f - ag) * pdf_b + ag * guide_pdf<MODE>(g, cells, mx, wi);
     float pal = pdf_l * pmf;
     float w = power_heuristic(pal, pdf_b);
     contrib = (((le * f) * ctl) * w) / pal;
@@ -248,8 +266,9 @@ __device__ __forceinline__ bool next_event(const DevScene& s, const MaterialRec&
 
 // One bounce of RayTracer::pathTraceRecursive + computeDirect (Renderer.h:269-354, 531-613).
 // MODE 0: BSDF sampling as the reference. MODE 1: one-sample MIS between the BSDF
-// and the quadtree guide (probability guide_prob), with the combined density used in
-// every MIS weight so that the estimator stays unbiased.
+// and the quadtree guide (probability guide_prob); MODE 2: between the BSDF and the
+// learned vMF mixture (probability c, learned per point). The combined density is
+// used in every MIS weight so that the estimator stays unbiased.
 template <int MODE>
 __global__ __launch_bounds__(kBlock) void k_shade(DevScene s, PathState ps, PassParams p, GuideCtx gc, int parity,
                                                   int depth) {
@@ -326,13 +345,19 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene s, PathState ps, Pass
                         }
                     }
                 } else {
-                    const uint32_t* cells = gc.cells ? gc.cells + (size_t)i * kKnnMax : nullptr;
-                    const bool guided = MODE == 1 && !m.specular && cells != nullptr && cells[0] != 0xffffffffu;
-                    const float ag = gc.g.guide_prob;
+                    const uint32_t* cells = (MODE == 1 && gc.cells) ? gc.cells + (size_t)i * kKnnMax : nullptr;
+                    const float4* mx = (MODE == 2 && gc.mix) ? gc.mix + (size_t)i * kMixF4 : nullptr;
+                    float ag = gc.g.guide_prob;
+                    bool guided = false;
+                    if (MODE == 1) guided = !m.specular && cells != nullptr && cells[0] != 0xffffffffu;
+                    if (MODE == 2 && !m.specular && mx != nullptr) {
+                        ag = mx[12].x;
+                        guided = ag > 0.f;
+                    }
                     // computeDirect (Renderer.h:269-354)
                     if (!m.specular && s.n_lights > 0) {
                         C3 contrib;
-                        if (next_event<MODE>(s, m, sd, key, dim, gc.g, cells, guided, contrib, sh_o, sh_d, sh_t)) {
+                        if (next_event<MODE>(s, m, sd, key, dim, gc.g, cells, mx, guided, ag, contrib, sh_o, sh_d, sh_t)) {
                             // the shadow ray can only change the image when the contribution is nonzero
                             C3 direct = beta * contrib;
                             if (!is_zero(direct)) {
@@ -355,12 +380,12 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene s, PathState ps, Pass
                         F3 wi;
                         if (guided) {
                             if (keyed_rand(key, dim++) < ag) {
-                                wi = guide_sample_qt(gc.g, cells, key, dim);
+                                wi = guide_sample<MODE>(gc.g, cells, mx, key, dim);
                                 f = bsdf_eval(m, sd, wi, pdf);
-                                pdf = (1.f - ag) * pdf + ag * guide_pdf_qt(gc.g, cells, wi);
+                                pdf = (1.f - ag) * pdf + ag * guide_pdf<MODE>(gc.g, cells, mx, wi);
                             } else {
                                 wi = bsdf_sample(m, sd, key, dim, f, pdf);
-                                if (!(pdf <= 0.f)) pdf = (1.f - ag) * pdf + ag * guide_pdf_qt(gc.g, cells, wi);
+                                if (!(pdf <= 0.f)) pdf = (1.f - ag) * pdf + ag * guide_pdf<MODE>(gc.g, cells, mx, wi);
                             }
                         } else {
                             wi = bsdf_sample(m, sd, key, dim, f, pdf);
@@ -455,9 +480,13 @@ __global__ __launch_bounds__(kBlock) void k_guide_cells(DevScene s, PathState ps
 // estimate draws its random numbers as the reference does but is not traced (the
 // reference's Li does not use it); with NEE == true a shadow ray is queued whose
 // result lands in the vertex's NEE slot (the pixelColour recursion).
-template <bool NEE>
+// MODE 2 samples the continuation by one-sample MIS between the BSDF and the learned
+// mixture of the queue slot (probability c), with the combined density q in the
+// vertex weight, the NEE weight and the next vertex's MIS weights; each vertex keeps
+// q and the BSDF pdf of its direction as training data.
+template <bool NEE, int MODE>
 __global__ __launch_bounds__(kBlock) void k_shade_record(DevScene s, PathState ps, PassParams p, RecordState rs,
-                                                         int parity, int depth) {
+                                                         const float4* __restrict__ mix, int parity, int depth) {
     uint32_t n = ps.counters[parity];
     const uint32_t* qin = ps.queue[parity];
     uint32_t* qout = ps.queue[parity ^ 1];
@@ -480,7 +509,8 @@ __global__ __launch_bounds__(kBlock) void k_shade_record(DevScene s, PathState p
             uint32_t key = inf.x, dim = inf.y;
             float prev_pdf = ps.beta[path].w;
             bool prev_spec = (inf.z & 1u) != 0u;
-            float4 vpos = make_float4(0.f, 0.f, 0.f, 0.f), vnrm = vpos, vwi = vpos, vw = vpos, ve = vpos, vme = vpos;
+            float4 vpos = make_float4(0.f, 0.f, 0.f, 0.f), vnrm = vpos, vwi = vpos, vwo = vpos, vw = vpos, ve = vpos,
+                   vme = vpos;
             if (hid == kMiss) {
                 // Renderer.h:501-527
                 C3 bg = s.has_env ? env_eval(s, rd) : c3(0.f, 0.f, 0.f);
@@ -537,11 +567,15 @@ __global__ __launch_bounds__(kBlock) void k_shade_record(DevScene s, PathState p
                         }
                     }
                 } else {
+                    const float4* mx = (MODE == 2 && mix) ? mix + (size_t)i * kMixF4 : nullptr;
+                    const float ag = mx ? mx[12].x : 0.f;
+                    const bool guided = MODE == 2 && !m.specular && ag > 0.f;
                     if (!m.specular && s.n_lights > 0) {
                         if (NEE) {
                             C3 contrib;
                             GuideView none{};
-                            if (next_event<0>(s, m, sd, key, dim, none, nullptr, false, contrib, sh_o, sh_d, sh_t) &&
+                            if (next_event<MODE>(s, m, sd, key, dim, none, nullptr, mx, guided, ag, contrib, sh_o, sh_d,
+                                                 sh_t) &&
                                 !is_zero(contrib)) {
                                 want_shadow = true;
                                 sh_c = contrib;
@@ -561,13 +595,32 @@ __global__ __launch_bounds__(kBlock) void k_shade_record(DevScene s, PathState p
                     }
                     if (cont && depth == (int)p.max_depth) cont = false;
                     if (cont) {
-                        float pdf = 0.f;
+                        float pdf = 0.f, pb = 0.f;
                         C3 f;
-                        F3 wi = bsdf_sample(m, sd, key, dim, f, pdf);
+                        F3 wi;
+                        if (guided) {
+                            const bool from_guide = keyed_rand(key, dim++) < ag;
+                            if (from_guide) {
+                                float u0 = keyed_rand(key, dim++);
+                                float u1 = keyed_rand(key, dim++);
+                                float u2 = keyed_rand(key, dim++);
+                                wi = mix_sample(mx, u0, u1, u2);
+                                f = bsdf_eval(m, sd, wi, pb);
+                            } else {
+                                wi = bsdf_sample(m, sd, key, dim, f, pb);
+                            }
+                            // a direction the BSDF gives no value ends the path (zero contribution)
+                            pdf = ((from_guide || pb > 0.f) && !is_zero(f)) ? (1.f - ag) * pb + ag * mix_pdf(mx, wi)
+                                                                           : 0.f;
+                        } else {
+                            wi = bsdf_sample(m, sd, key, dim, f, pdf);
+                            pb = pdf;
+                        }
                         float ctw = fabsf(dot(wi, sd.sn));
                         if (!(pdf <= 0.f || ctw <= 0.f)) {
                             C3 w = (f * ctw) / (pdf * rr);
-                            vwi = make_float4(wi.x, wi.y, wi.z, 0.f);
+                            vwi = make_float4(wi.x, wi.y, wi.z, pdf);
+                            vwo = make_float4(sd.wo.x, sd.wo.y, sd.wo.z, pb);
                             vw = make_float4(w.r, w.g, w.b, 0.f);
                             vpos.w = m.specular ? 0.f : 1.f;
                             float sg = (dot(wi, sd.gn) >= 0.f) ? 1.f : -1.f;
@@ -584,6 +637,7 @@ __global__ __launch_bounds__(kBlock) void k_shade_record(DevScene s, PathState p
             rs.vpos[o] = vpos;
             rs.vnrm[o] = vnrm;
             rs.vwi[o] = vwi;
+            rs.vwo[o] = vwo;
             rs.vw[o] = vw;
             rs.ve[o] = ve;
             if (NEE) {
@@ -738,6 +792,7 @@ hipError_t launch_extend(const DevScene& s, const PathState& ps, int parity, int
 hipError_t launch_shade(const DevScene& s, const PathState& ps, const PassParams& p, const GuideCtx& gc, int mode,
                         int parity, int depth, uint32_t grid, hipStream_t stream) {
     if (mode == 1) hipLaunchKernelGGL((k_shade<1>), dim3(grid), dim3(kBlock), 0, stream, s, ps, p, gc, parity, depth);
+    else if (mode == 2) hipLaunchKernelGGL((k_shade<2>), dim3(grid), dim3(kBlock), 0, stream, s, ps, p, gc, parity, depth);
     else hipLaunchKernelGGL((k_shade<0>), dim3(grid), dim3(kBlock), 0, stream, s, ps, p, gc, parity, depth);
     return hipGetLastError();
 }
@@ -757,11 +812,15 @@ hipError_t launch_guide_cells(const DevScene& s, const PathState& ps, const Guid
     return hipGetLastError();
 }
 hipError_t launch_shade_record(const DevScene& s, const PathState& ps, const PassParams& p, const RecordState& rs,
-                               int parity, int depth, uint32_t grid, hipStream_t stream) {
-    if (rs.li_nee)
-        hipLaunchKernelGGL((k_shade_record<true>), dim3(grid), dim3(kBlock), 0, stream, s, ps, p, rs, parity, depth);
+                               const float4* mix, int parity, int depth, uint32_t grid, hipStream_t stream) {
+    if (mix && rs.li_nee)
+        hipLaunchKernelGGL((k_shade_record<true, 2>), dim3(grid), dim3(kBlock), 0, stream, s, ps, p, rs, mix, parity, depth);
+    else if (mix)
+        hipLaunchKernelGGL((k_shade_record<false, 2>), dim3(grid), dim3(kBlock), 0, stream, s, ps, p, rs, mix, parity, depth);
+    else if (rs.li_nee)
+        hipLaunchKernelGGL((k_shade_record<true, 0>), dim3(grid), dim3(kBlock), 0, stream, s, ps, p, rs, nullptr, parity, depth);
     else
-        hipLaunchKernelGGL((k_shade_record<false>), dim3(grid), dim3(kBlock), 0, stream, s, ps, p, rs, parity, depth);
+        hipLaunchKernelGGL((k_shade_record<false, 0>), dim3(grid), dim3(kBlock), 0, stream, s, ps, p, rs, nullptr, parity, depth);
     return hipGetLastError();
 }
 hipError_t launch_shadow_record(const DevScene& s, const PathState& ps, const RecordState& rs, int stack, uint32_t grid,

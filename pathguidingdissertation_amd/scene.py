@@ -160,6 +160,75 @@ class Device:
         self._chk(self.lib.pg_guide_qt_eval(self.ctx, _fp(pos), _fp(u3), _fp(we), n, _fp(wi), _fp(ps), _fp(pe)))
         return wi, ps, pe
 
+    # ---------------------------------------------------------------- learned guide
+    def mlp_config(self, **kw) -> _lib.PgMlpConfig:
+        cfg = self.lib.pg_mlp_config_default()
+        for k, v in kw.items():
+            if not hasattr(cfg, k):
+                raise KeyError(k)
+            setattr(cfg, k, v)
+        return cfg
+
+    def mlp_init(self, **kw):
+        cfg = self.mlp_config(**kw)
+        self._chk(self.lib.pg_mlp_init(self.ctx, C.byref(cfg)))
+        return self.mlp_info()
+
+    def mlp_train(self, steps: int) -> np.ndarray:
+        losses = np.empty(max(steps, 1), np.float32)
+        self._chk(self.lib.pg_mlp_train(self.ctx, int(steps), _fp(losses)))
+        return losses[:steps]
+
+    def mlp_info(self) -> dict:
+        i = _lib.PgMlpInfo()
+        self._chk(self.lib.pg_mlp_info_read(self.ctx, C.byref(i)))
+        return {k: getattr(i, k) for k, _ in i._fields_ if k != "reserved"}
+
+    def mlp_params(self):
+        info = self.mlp_info()
+        p = np.empty(info["n_params"], np.float32)
+        g = np.empty(info["n_grid_params"], np.float32)
+        self._chk(self.lib.pg_mlp_get_params(self.ctx, _fp(p), _fp(g)))
+        return p, g
+
+    def mlp_set_params(self, params=None, grid=None):
+        p = None if params is None else np.ascontiguousarray(params, np.float32)
+        g = None if grid is None else np.ascontiguousarray(grid, np.float32)
+        self._chk(self.lib.pg_mlp_set_params(self.ctx, _fp(p) if p is not None else None,
+                                             _fp(g) if g is not None else None))
+
+    def mlp_levels(self):
+        res = np.empty(16, np.uint32)
+        dense = np.empty(16, np.uint32)
+        frame = np.empty(4, np.float32)
+        self._chk(self.lib.pg_mlp_levels(self.ctx, _up(res), _up(dense), _fp(frame)))
+        return res, dense.astype(bool), frame
+
+    def mlp_eval(self, pos, nrm, wo, wi_eval=None, u3=None):
+        """Mixture (n, 13, 4), pdf of wi_eval (n,), sampled directions (n, 4: xyz, pdf)."""
+        f3 = lambda a: np.ascontiguousarray(a, np.float32).reshape(-1, 3)
+        pos, nrm, wo = f3(pos), f3(nrm), f3(wo)
+        n = pos.shape[0]
+        mix = np.empty((n, 13, 4), np.float32)
+        we = f3(wi_eval) if wi_eval is not None else None
+        pe = np.empty(n, np.float32) if wi_eval is not None else None
+        uu = f3(u3) if u3 is not None else None
+        wo_out = np.empty((n, 4), np.float32) if u3 is not None else None
+        opt = lambda a: _fp(a) if a is not None else None
+        self._chk(self.lib.pg_mlp_eval(self.ctx, n, _fp(pos), _fp(nrm), _fp(wo), _fp(mix), opt(we), opt(pe),
+                                       opt(uu), opt(wo_out)))
+        return mix, pe, wo_out
+
+    def mlp_gradient(self, samples, want_grid: bool = True):
+        s = np.ascontiguousarray(samples, np.float32).reshape(-1, 16)
+        info = self.mlp_info()
+        loss = np.zeros(1, np.float32)
+        gp = np.empty(info["n_params"], np.float32)
+        gg = np.empty(info["n_grid_params"], np.float32) if want_grid else None
+        self._chk(self.lib.pg_mlp_gradient(self.ctx, s.shape[0], _fp(s), _fp(loss), _fp(gp),
+                                           _fp(gg) if gg is not None else None))
+        return float(loss[0]), gp, gg
+
     def info(self) -> dict:
         name = C.create_string_buffer(64)
         cu = C.c_int32()
@@ -242,6 +311,11 @@ class RayTracer:
 
     def render(self, n_spp: int = 1):
         self.device._chk(self.device.lib.pg_render_spp(self.device.ctx, C.byref(self.params), int(n_spp)))
+
+    def render_online(self, n_spp: int, steps_per_pass: int = 1, window: int = 1):
+        """Learned-guide passes with online training (pg_mlp_render_online)."""
+        self.device._chk(self.device.lib.pg_mlp_render_online(self.device.ctx, C.byref(self.params), int(n_spp),
+                                                              int(steps_per_pass), int(window)))
 
     def getSPP(self) -> int:  # noqa: N802 - reference name
         _, spp = self.film()

@@ -1,0 +1,257 @@
+"""Learned guide on the GPU (BF16 MFMA network, vMF mixture head, NLL backward,
+Adam) against the FP64 oracle oracle/mlp_oracle.py, which test_mlp_oracle.py pins
+by finite differences and closed forms. Every call goes through the C ABI."""
+import numpy as np
+import pytest
+
+from oracle import mlp_oracle as mo
+
+pytestmark = pytest.mark.gpu
+
+LOG2T = 14
+
+
+def _unit(rng, n):
+    v = rng.normal(size=(n, 3))
+    return v / np.linalg.norm(v, axis=1, keepdims=True)
+
+
+def _random_net(rng, scale=0.5, grid_amp=0.5, log2t=LOG2T):
+    params = np.zeros(mo.N_PARAMS, np.float32)
+    for ow, ob, rows, cols in mo.LAYERS:
+        used_r = mo.OUTPUTS_USED if rows == mo.OUTPUTS else rows
+        used_c = mo.INPUTS_USED if cols == mo.INPUTS else cols
+        W = np.zeros((rows, cols), np.float32)
+        W[:used_r, :used_c] = rng.uniform(-scale, scale, (used_r, used_c)) * np.sqrt(6.0 / (used_r + used_c)) * 2
+        params[ow: ow + rows * cols] = W.reshape(-1)
+        params[ob: ob + used_r] = rng.uniform(-0.2, 0.2, used_r)
+    grid = rng.uniform(-grid_amp, grid_amp, mo.LEVELS * (1 << log2t) * 2).astype(np.float32)
+    return params, grid
+
+
+@pytest.fixture(scope="module")
+def net(pg, device, cornell):
+    sc = pg.Scene(device, cornell)
+    device.mlp_init(seed=3, log2_table=LOG2T, res_min=4.0, res_max=256.0)
+    rng = np.random.default_rng(11)
+    params, grid = _random_net(rng)
+    device.mlp_set_params(params, grid)
+    res, dense, frame = device.mlp_levels()
+    return sc, params, grid, res, dense, frame
+
+
+def _points(rng, frame, n):
+    ext = 1.0 / frame[3]
+    x = frame[:3] + rng.uniform(0.02, 0.98, (n, 3)) * ext
+    return x.astype(np.float32), _unit(rng, n).astype(np.float32), _unit(rng, n).astype(np.float32)
+
+
+def _oracle_out(params, grid, res, dense, frame, x, nrm, wo):
+    g = mo.bf16(grid).reshape(mo.LEVELS, 1 << LOG2T, 2)
+    feats, _ = mo.encode(x, nrm, wo, g, res.astype(np.uint64), dense, LOG2T, frame, emulate=True)
+    out, _ = mo.forward(params.astype(np.float64), feats, emulate=True)
+    return out
+
+
+def test_levels_match_oracle(device, net):
+    _, _, _, res, dense, _ = net
+    r2, d2 = mo.levels(4.0, 256.0, LOG2T)
+    assert np.array_equal(res.astype(np.uint64), r2)
+    assert np.array_equal(dense, d2)
+    assert dense.any() and (~dense).any()  # both indexing modes exercised
+
+
+def test_forward_mixture_matches_oracle(device, net):
+    _, params, grid, res, dense, frame = net
+    rng = np.random.default_rng(1)
+    n = 3000
+    x, nrm, wo = _points(rng, frame, n)
+    wi = _unit(rng, n).astype(np.float32)
+    mix, pdf, _ = device.mlp_eval(x, nrm, wo, wi_eval=wi)
+    out = _oracle_out(params, grid, res, dense, frame, x, nrm, wo)
+    pi, mu, kappa, c = mo.mixture(out)
+    g_mu = mix[:, :8, :3]
+    g_kappa = mix[:, :8, 3]
+    g_pi = mix[:, 8:10, :].reshape(n, 8)
+    g_c = mix[:, 12, 0]
+    assert np.all(g_c > 0)
+    assert np.max(np.abs(g_pi - pi)) < 5e-3, np.max(np.abs(g_pi - pi))
+    assert np.max(np.abs(g_c - c)) < 5e-3
+    assert np.max(np.abs(np.log(g_kappa) - np.log(kappa))) < 2e-2
+    assert np.min(np.sum(g_mu * mu, -1)) > 1 - 1e-4
+    # normalised lobe weights: pi * kappa / (2 pi (1 - e^-2k))
+    g_pt = mix[:, 10:12, :].reshape(n, 8)
+    k64 = g_kappa.astype(np.float64)
+    assert np.allclose(g_pt, g_pi * k64 / (2 * np.pi * -np.expm1(-2 * k64)), rtol=1e-4, atol=1e-7)
+    # density of the device mixture at wi (mix_pdf) against the FP64 formula
+    ref = mo.mix_pdf(g_pi.astype(np.float64), g_mu.astype(np.float64), g_kappa.astype(np.float64),
+                     wi.astype(np.float64))
+    assert np.allclose(pdf, ref, rtol=2e-4, atol=1e-7)
+    # and against the oracle's own mixture
+    ref2 = mo.mix_pdf(pi, mu, kappa, wi.astype(np.float64))
+    rel = np.abs(pdf - ref2) / np.maximum(ref2, 1e-3)
+    assert np.percentile(rel, 99) < 2e-2 and np.max(rel) < 0.1, (np.percentile(rel, 99), np.max(rel))
+
+
+def _samples(rng, frame, n):
+    x, nrm, wo = _points(rng, frame, n)
+    s = np.zeros((n, 16), np.float32)
+    s[:, 0:3] = x
+    s[:, 3] = rng.uniform(0.1, 3.0, n)
+    s[:, 4:7] = nrm
+    s[:, 7] = rng.uniform(0.05, 2.0, n)
+    s[:, 8:11] = wo
+    s[:, 12:15] = _unit(rng, n)
+    s[: n // 16, 3] = 0.0  # zero-weight samples contribute nothing
+    return s
+
+
+def test_gradients_match_oracle(device, net):
+    _, params, grid, res, dense, frame = net
+    device.mlp_set_params(params, grid)
+    rng = np.random.default_rng(2)
+    s = _samples(rng, frame, 2048)
+    loss, gp, gg = device.mlp_gradient(s)
+    g = mo.bf16(grid).reshape(mo.LEVELS, 1 << LOG2T, 2)
+    l_ref, gp_ref, gg_ref = mo.loss_and_grads(params.astype(np.float64), g, s.astype(np.float64),
+                                              res.astype(np.uint64), dense, LOG2T, frame, emulate=True)
+    assert abs(loss - l_ref) < 1e-3 * abs(l_ref), (loss, l_ref)
+    for ow, ob, rows, cols in mo.LAYERS:
+        a, b = gp[ow: ob + rows], gp_ref[ow: ob + rows]
+        err = np.linalg.norm(a - b) / np.linalg.norm(b)
+        cos = np.dot(a, b) / (np.linalg.norm(a) * np.linalg.norm(b))
+        assert err < 2e-2 and cos > 0.9995, (ow, err, cos)
+    assert np.all(gp.reshape(-1)[mo.OFF_W1: mo.OFF_B1].reshape(64, 96)[:, 80:] == 0)
+    gr = gg_ref.reshape(-1)
+    err = np.linalg.norm(gg - gr) / np.linalg.norm(gr)
+    assert err < 2e-2, err
+    # entries no sample touches stay exactly zero
+    assert np.all(gg[gr == 0] == 0)
+
+
+def test_chunked_gradient_equals_single_chunk(pg, device, cornell, net):
+    _, params, grid, res, dense, frame = net
+    rng = np.random.default_rng(4)
+    s = _samples(rng, frame, 3000)
+    device.mlp_set_params(params, grid)
+    l1, g1, h1 = device.mlp_gradient(s)
+    info1 = device.mlp_info()
+    device.mlp_init(seed=3, log2_table=LOG2T, res_min=4.0, res_max=256.0, act_budget=300_000)
+    device.mlp_set_params(params, grid)
+    l2, g2, h2 = device.mlp_gradient(s)
+    info2 = device.mlp_info()
+    device.mlp_init(seed=3, log2_table=LOG2T, res_min=4.0, res_max=256.0)
+    device.mlp_set_params(params, grid)
+    assert info1["chunks"] == 1 and info2["chunks"] > 1
+    assert abs(l1 - l2) <= 1e-5 * abs(l1)
+    assert np.linalg.norm(g1 - g2) <= 1e-5 * np.linalg.norm(g1)
+    assert np.linalg.norm(h1 - h2) <= 1e-5 * np.linalg.norm(h1)
+
+
+def test_sampling_consistent_with_density(device, net):
+    _, params, grid, res, dense, frame = net
+    device.mlp_set_params(params, grid)
+    rng = np.random.default_rng(6)
+    x, nrm, wo = _points(rng, frame, 1)
+    n = 1 << 17
+    rep = lambda a: np.repeat(a, n, 0)
+    u = rng.random((n, 3)).astype(np.float32)
+    mix, _, ws = device.mlp_eval(rep(x), rep(nrm), rep(wo), u3=u)
+    d = ws[:, :3].astype(np.float64)
+    assert np.allclose(np.linalg.norm(d, axis=1), 1.0, atol=1e-5)
+    # returned pdf equals the mixture density at the sampled direction
+    pi = mix[0, 8:10].reshape(8).astype(np.float64)
+    mu = mix[0, :8, :3].astype(np.float64)
+    kap = mix[0, :8, 3].astype(np.float64)
+    ref = mo.mix_pdf(np.tile(pi, (n, 1)), np.tile(mu, (n, 1, 1)), np.tile(kap, (n, 1)), d)
+    assert np.allclose(ws[:, 3], ref, rtol=2e-4, atol=1e-6)
+    # the device sampler agrees with the oracle's inversion away from lobe boundaries
+    cdf = np.cumsum(pi)
+    far = np.min(np.abs(u[:, 0:1] - cdf[None, :-1]), 1) > 1e-4
+    idx = np.nonzero(far)[0][:2000]
+    od = np.stack([mo.mix_sample(pi, mu, kap, *u[i].astype(np.float64)) for i in idx])
+    assert np.min(np.sum(od * d[idx], 1)) > 1 - 1e-5
+    # chi-square of the samples against the density on 16 x 32 equal-area bins
+    ui = np.clip(((np.arctan2(d[:, 1], d[:, 0]) / (2 * np.pi)) % 1.0 * 32).astype(int), 0, 31)
+    vi = np.clip(((d[:, 2] + 1) * 0.5 * 16).astype(int), 0, 15)
+    hist = np.zeros((16, 32))
+    np.add.at(hist, (vi, ui), 1)
+    g = 6
+    ph = (np.arange(32 * g) + 0.5) / (32 * g) * 2 * np.pi
+    ct = (np.arange(16 * g) + 0.5) / (16 * g) * 2 - 1
+    P, C = np.meshgrid(ph, ct)
+    S = np.sqrt(1 - C * C)
+    w = np.stack([S * np.cos(P), S * np.sin(P), C], -1).reshape(-1, 3)
+    m = w.shape[0]
+    _, dens, _ = device.mlp_eval(np.repeat(x, m, 0), np.repeat(nrm, m, 0), np.repeat(wo, m, 0), wi_eval=w)
+    exp = (dens.astype(np.float64) * 4 * np.pi / m).reshape(16, g, 32, g).sum((1, 3)) * n
+    keep = exp > 10
+    chi2 = np.sum((hist[keep] - exp[keep]) ** 2 / exp[keep])
+    dof = keep.sum() - 1
+    assert chi2 < dof + 6 * np.sqrt(2 * dof), (chi2, dof)
+
+
+def _record(pg, device, config, spp, seed=1):
+    data = pg.generate_scene(config, 0)
+    sc = pg.Scene(device, data)
+    device.clear_records()
+    rt = pg.RayTracer(sc, seed=seed, max_depth=12, integrator=1, record=1, record_nee=1)
+    rt.clear()
+    rt.render(spp)
+    return sc
+
+
+def test_first_adam_step_moves_each_parameter_by_lr(pg, device):
+    _record(pg, device, 1, 1)
+    device.mlp_init(seed=5, log2_table=LOG2T, res_min=4.0, res_max=256.0, batch=1 << 14)
+    p0, g0 = device.mlp_params()
+    losses = device.mlp_train(1)
+    p1, g1 = device.mlp_params()
+    assert np.isfinite(losses).all()
+    d = np.abs(p1.astype(np.float64) - p0)
+    moved = d > 0
+    # bias-corrected first step: |delta| = lr |g| / (|g| + eps) = lr where g != 0
+    assert moved.mean() > 0.3
+    assert np.all(d <= 1e-2 * (1 + 1e-4))
+    assert np.mean(np.abs(d[moved] - 1e-2) < 1e-5) > 0.9
+    assert not np.any(moved.reshape(-1)[mo.OFF_W1: mo.OFF_B1].reshape(64, 96)[:, 80:])
+    dg = np.abs(g1.astype(np.float64) - g0)
+    assert np.all(dg <= 1e-2 * (1 + 1e-4)) and np.mean(np.abs(dg[dg > 0] - 1e-2) < 1e-5) > 0.9
+    assert device.mlp_info()["step"] == 1
+
+
+def test_training_reduces_loss(pg, device):
+    _record(pg, device, 101, 2)
+    device.mlp_init(seed=7)
+    losses = device.mlp_train(80)
+    info = device.mlp_info()
+    assert info["diverged"] == 0 and np.isfinite(losses).all()
+    assert losses[-10:].mean() < 0.9 * losses[:5].mean(), (losses[:5], losses[-10:])
+
+
+def test_divergence_falls_back(pg, device):
+    sc = _record(pg, device, 101, 1)
+    device.mlp_init(seed=7, batch=1 << 14, nan_step=3)
+    device.mlp_train(5)
+    info = device.mlp_info()
+    assert info["diverged"] == 1
+    rt = pg.RayTracer(sc, seed=4, max_depth=12, integrator=3)
+    rt.clear()
+    rt.render(2)
+    img = rt.image()
+    assert np.isfinite(img).all() and img.mean() > 0
+    assert device.mlp_info()["fallbacks"] == 2
+    # re-initialising clears the flag
+    device.mlp_init(seed=7, batch=1 << 14)
+    assert device.mlp_info()["diverged"] == 0
+
+
+def test_training_needs_record_pass_data(pg, device, cornell):
+    pg.Scene(device, cornell)
+    device.mlp_init(seed=1, log2_table=LOG2T)
+    device.clear_records()
+    with pytest.raises(pg.PgError):
+        device.mlp_train(1)
+    device.upload_records(np.zeros((16, 12), np.float32))
+    with pytest.raises(pg.PgError):
+        device.mlp_train(1)
