@@ -5,7 +5,9 @@ A step is one sample pass over the full frame (921,600 paths, one per pixel). Th
 default run times 64 steps (the workload's 64 SPP) after 4 warmup steps and prints one
 JSON line: Mrays/s (closest-hit + shadow rays traced), the roofline of the dominant
 kernel (extend: closest-hit traversal) from HIP-event timings taken inside the timed
-region, and the CPU oracle timed on the host cores on a bounded sample.
+region, and the CPU oracle timed on the host cores on a bounded sample. A second part
+(skipped with --no-ttr) measures time-to-relMSE of the unguided, quadtree-guided and
+learned-guided renderers against a 4096-SPP reference of the same frame.
 """
 from __future__ import annotations
 
@@ -43,6 +45,117 @@ def profiled_traffic(config: int) -> dict:
     return out
 
 
+def reference_image(pg, sc, data, spp: int, max_depth: int) -> np.ndarray:
+    """Unguided GPU render of `spp` samples per pixel accumulated in FP64 on the host
+    (64-SPP film reads summed in double), seed 1000. The GPU path tracer is the same
+    estimator as the CPU oracle (pixel agreement is tested), so this stands in for the
+    4096-SPP FP64 CPU image, which would take ~20 minutes on 16 host cores."""
+    rt = pg.RayTracer(sc, seed=1000, max_depth=max_depth)
+    acc = np.zeros((data.height, data.width, 3), np.float64)
+    done = 0
+    while done < spp:
+        rt.clear()
+        rt.render(min(64, spp - done))
+        film, n = rt.film()
+        acc += film
+        done += n
+    return acc / done
+
+
+def progressive(pg, label, ref, setup, advance, cap_s, target, log):
+    """Run one method until relMSE <= target or cap_s seconds of its own wall time.
+    The clock covers setup and every render/training call; relMSE checkpoints (film
+    read-back + FP64 metric on the host) are outside it. Increments double up to 64
+    SPP, then hold about one second of work each."""
+    t_total = 0.0
+    t0 = time.perf_counter()
+    setup()
+    t_total += time.perf_counter() - t0
+    spp, k, curve, hit = 0, 1, [], None
+    while t_total < cap_s:
+        t0 = time.perf_counter()
+        img = advance(k)
+        dt = time.perf_counter() - t0
+        t_total += dt
+        spp += k
+        e = pg.relmse(img, ref)
+        curve.append((spp, round(t_total, 3), e))
+        if e <= target:
+            hit = (spp, t_total)
+            break
+        per_spp = dt / k
+        k = k * 2 if spp < 64 else max(1, min(256, int(1.0 / max(per_spp, 1e-4))))
+    log(f"ttr {label}: {len(curve)} checkpoints, {spp} SPP in {t_total:.1f}s, relMSE {curve[-1][2]:.4f}"
+        + (f", reached {target} at {hit[1]:.2f}s" if hit else ""))
+    return curve, hit
+
+
+def first_time_below(curve, e):
+    for spp, t, err in curve:
+        if err <= e:
+            return t, spp
+    return None, None
+
+
+def time_to_relmse(pg, dev, sc, data, max_depth, ref_spp, cap_s, log, target=0.01):
+    """Time-to-relMSE study: unguided vs quadtree guide vs learned guide."""
+    t0 = time.perf_counter()
+    ref = reference_image(pg, sc, data, ref_spp, max_depth)
+    log(f"ttr reference: {ref_spp} SPP in {time.perf_counter() - t0:.1f}s, mean {ref.mean():.5f}")
+    un = pg.RayTracer(sc, seed=2, max_depth=max_depth)
+    qt = pg.RayTracer(sc, seed=3, max_depth=max_depth, integrator=2, guide_prob=0.5)
+    ml = pg.RayTracer(sc, seed=5, max_depth=max_depth, record_nee=1)
+
+    def qt_setup():
+        # pass 1: 1 SPP record pass (Li with NEE), then radix sort + point BVH + quadtrees
+        dev.clear_records()
+        rec = pg.RayTracer(sc, seed=1, max_depth=max_depth, integrator=1, record=1, record_nee=1)
+        rec.clear()
+        rec.render(1)
+        dev.cache_build(cell_size=4096)
+        qt.clear()
+
+    def ml_setup():
+        dev.mlp_init(seed=7)
+        ml.clear()
+
+    def adv(rt, fn):
+        def f(k):
+            fn(k)
+            return rt.image()
+        return f
+
+    methods = {
+        "unguided": (lambda: un.clear(), adv(un, lambda k: un.render(k))),
+        "quadtree": (qt_setup, adv(qt, lambda k: qt.render(k))),
+        "learned": (ml_setup, adv(ml, lambda k: ml.render_online(k, steps=1, window=1))),
+    }
+    res = {}
+    for name, (setup, advance) in methods.items():
+        curve, hit = progressive(pg, name, ref, setup, advance, cap_s, target, log)
+        res[name] = {"seconds": None if hit is None else round(hit[1], 3),
+                     "spp": curve[-1][0], "final_relmse": round(curve[-1][2], 5), "curve": curve}
+    info = dev.mlp_info()
+    common = max(r["final_relmse"] for r in res.values())
+    times = {n: first_time_below(r["curve"], common)[0] for n, r in res.items()}
+    for r in res.values():
+        c = r["curve"]
+        step = max(1, len(c) // 16)
+        r["curve"] = [[s, t, round(e, 5)] for s, t, e in (c[::step] + ([c[-1]] if (len(c) - 1) % step else []))]
+    ratio = lambda a, b: None if (times[a] is None or times[b] is None or times[a] <= 0) else round(times[b] / times[a], 3)
+    return {"target": target, "cap_s": cap_s,
+            "reference": f"GPU unguided {ref_spp} SPP, FP64 host accumulation, seed 1000",
+            "methods": {"unguided": "path tracer with NEE + MIS (Renderer.h:531-618)",
+                        "quadtree": "1-SPP record pass, point-BVH + per-cell quadtrees, 50/50 one-sample MIS, kNN k=16",
+                        "learned": "online: 1-SPP guided record passes, 1 Adam step of 2^18 samples per pass"},
+            **res,
+            "learned_diverged": bool(info["diverged"]),
+            "common_target": round(common, 5),
+            "time_to_common_s": {n: (None if t is None else round(t, 3)) for n, t in times.items()},
+            "speedup_learned_vs_unguided": ratio("learned", "unguided"),
+            "speedup_learned_vs_quadtree": ratio("learned", "quadtree")}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -51,6 +164,10 @@ def main() -> int:
     ap.add_argument("--config", type=int, default=int(os.environ.get("PG_BENCH_CONFIG", "2")))
     ap.add_argument("--cpu-spp", type=int, default=int(os.environ.get("PG_BENCH_CPU_SPP", "40")))
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-ttr", action="store_true", help="skip the time-to-relMSE study")
+    ap.add_argument("--ttr-cap", type=float, default=float(os.environ.get("PG_BENCH_TTR_CAP", "30")),
+                    help="wall-clock seconds per method in the time-to-relMSE study")
+    ap.add_argument("--ttr-ref-spp", type=int, default=int(os.environ.get("PG_BENCH_TTR_REF", "4096")))
     ap.add_argument("--copy-check", action="store_true",
                     help="also time a plain 1 GiB device copy (HBM reference point / counter scaling)")
     a = ap.parse_args()
@@ -131,6 +248,11 @@ def main() -> int:
                          f"{cst['seconds']:.1f} s); oracle restatement with std::thread tiles (Renderer.h:648-747)",
                "samples_per_s": round(cst["paths"] / cst["seconds"], 1)}
 
+    ttr = None
+    if not a.no_ttr:
+        ttr = time_to_relmse(pg, dev, sc, data, max_depth, a.ttr_ref_spp, a.ttr_cap,
+                             lambda m: print(m, file=sys.stderr, flush=True))
+
     line = {
         "metric": "Mrays/s; time-to-relMSE 0.01: learned vs QuadTree vs unguided, 1 MI355X",
         "value": round(mrays, 2),
@@ -162,6 +284,7 @@ def main() -> int:
         "shadow_roofline": {"achieved": round(sh_achieved, 1), "bytes_per_ray": round(bytes_s, 1),
                             "nodes_per_ray": round(nodes_s, 2), "tris_per_ray": round(tris_s, 2)},
         "cpu_baseline": cpu,
+        "time_to_relmse": ttr,
         "copy_reference": copy,
         "image_finite": finite,
     }

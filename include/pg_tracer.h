@@ -219,14 +219,25 @@ pg_status pg_mlp_init(pg_ctx* ctx, const pg_mlp_config* cfg);
 /* `steps` Adam steps on the current records (they must come from record passes,
  * which keep the training data). losses (nullable) receives each step's mean loss. */
 pg_status pg_mlp_train(pg_ctx* ctx, uint32_t steps, float* losses);
-/* Learned-guide rendering with online training: each of n_spp passes is a guided
- * record pass (PG_INTEGRATOR_GUIDED_MLP with record = 1: the record integrator of
- * Renderer.h:367-529 sampling by one-sample MIS between the BSDF and the guide,
- * one sample per pixel into the film, vertices kept as training data), followed by
- * steps_per_pass Adam steps on the records of the last `window` passes (older
- * records are dropped). params->record_nee = 1 is recommended.                   */
+/* Learned-guide rendering with online training. Every `train_every`-th pass (the
+ * first included) is a guided record pass (PG_INTEGRATOR_GUIDED_MLP with record = 1:
+ * the record integrator of Renderer.h:367-529 sampling by one-sample MIS between the
+ * BSDF and the guide; its sample goes to the film and its vertices become training
+ * data) followed by `steps` Adam steps on the records of the last `window` record
+ * passes (older ones are dropped); the first `warmup_passes` record passes train
+ * `warmup_steps` steps instead. The other passes are guided path-tracing passes
+ * (record = 0). Every pass adds one sample per pixel. params->record_nee = 1 is
+ * recommended.                                                                  */
+typedef struct pg_mlp_schedule {
+    uint32_t train_every;    /* 1                                                  */
+    uint32_t steps;          /* 1                                                  */
+    uint32_t window;         /* 1                                                  */
+    uint32_t warmup_passes;  /* 0                                                  */
+    uint32_t warmup_steps;   /* 0                                                  */
+    uint32_t reserved[3];
+} pg_mlp_schedule;
 pg_status pg_mlp_render_online(pg_ctx* ctx, const pg_render_params* params, uint32_t n_spp,
-                               uint32_t steps_per_pass, uint32_t window);
+                               const pg_mlp_schedule* schedule);
 pg_status pg_mlp_info_read(pg_ctx* ctx, pg_mlp_info* info);
 /* FP32 master parameters (17648) and hash table (n_grid_params), either nullable. */
 pg_status pg_mlp_get_params(pg_ctx* ctx, float* params, float* grid);

@@ -64,6 +64,11 @@ class PgMlpInfo(C.Structure):
                 ("reserved", C.c_uint32 * 5)]
 
 
+class PgMlpSchedule(C.Structure):
+    _fields_ = [("train_every", C.c_uint32), ("steps", C.c_uint32), ("window", C.c_uint32),
+                ("warmup_passes", C.c_uint32), ("warmup_steps", C.c_uint32), ("reserved", C.c_uint32 * 3)]
+
+
 _P = C.c_void_p
 _F = C.POINTER(C.c_float)
 _U32 = C.POINTER(C.c_uint32)
@@ -104,7 +109,7 @@ SIGNATURES = {
     "pg_mlp_config_default": (PgMlpConfig, []),
     "pg_mlp_init": (C.c_int32, [_P, C.POINTER(PgMlpConfig)]),
     "pg_mlp_train": (C.c_int32, [_P, C.c_uint32, _F]),
-    "pg_mlp_render_online": (C.c_int32, [_P, C.POINTER(PgRenderParams), C.c_uint32, C.c_uint32, C.c_uint32]),
+    "pg_mlp_render_online": (C.c_int32, [_P, C.POINTER(PgRenderParams), C.c_uint32, C.POINTER(PgMlpSchedule)]),
     "pg_mlp_info_read": (C.c_int32, [_P, C.POINTER(PgMlpInfo)]),
     "pg_mlp_get_params": (C.c_int32, [_P, _F, _F]),
     "pg_mlp_set_params": (C.c_int32, [_P, _F, _F]),

@@ -111,6 +111,7 @@ struct pg_ctx {
         float* ggrad = nullptr;
         uint32_t* grid_bf = nullptr;
         size_t n_grid = 0;
+        size_t n_ggrad = 0;  // floats of the privatised gradient accumulators
         float* partial = nullptr;
         uint32_t groups_cap = 0;
         TrainSample* batch = nullptr;
@@ -122,10 +123,11 @@ struct pg_ctx {
         float* scal = nullptr;
         uint32_t scal_cap = 0;
         uint32_t* status = nullptr;
+        double* sum_part = nullptr;
         uint32_t step = 0, chunks = 1, fallbacks = 0;
         float last_loss = 0.f, w_scale = 1.f, train_ms = 0.f;
         uint64_t w_scale_gen = ~0ull;
-        uint32_t window_passes = 0;
+        uint32_t window_passes = 0, online_passes = 0, record_passes = 0;
     } mlp;
     float4* mix = nullptr;
     uint32_t mix_cap = 0;
@@ -172,7 +174,7 @@ struct pg_ctx {
         Mlp& m = mlp;
         dfree(m.params); dfree(m.pm); dfree(m.pv); dfree(m.mirror);
         dfree(m.grid); dfree(m.gm); dfree(m.gv); dfree(m.ggrad); dfree(m.grid_bf);
-        dfree(m.partial); dfree(m.batch); dfree(m.rows); dfree(m.loss); dfree(m.scal); dfree(m.status);
+        dfree(m.partial); dfree(m.batch); dfree(m.rows); dfree(m.loss); dfree(m.scal); dfree(m.status); dfree(m.sum_part);
         m = Mlp{};
     }
     void free_paths() {
@@ -1047,9 +1049,10 @@ pg_status pg_mlp_init(pg_ctx* c, const pg_mlp_config* cfg_in) {
     PG_TRY(c, dalloc(&m.grid, m.n_grid), "alloc hash grid");
     PG_TRY(c, dalloc(&m.gm, m.n_grid), "alloc hash grid");
     PG_TRY(c, dalloc(&m.gv, m.n_grid), "alloc hash grid");
-    PG_TRY(c, dalloc(&m.ggrad, m.n_grid), "alloc hash grid");
+    // view first: the gradient accumulator layout depends on the level resolutions
     PG_TRY(c, dalloc(&m.grid_bf, m.n_grid / 2), "alloc hash grid");
     PG_TRY(c, dalloc(&m.status, 1), "alloc status");
+    PG_TRY(c, dalloc(&m.sum_part, kSumParts), "alloc status");
     // initialisation: Xavier-uniform weights over the used inputs/outputs, zero
     // biases and padding, hash features uniform in +-1e-4 (counter hash of the seed)
     std::vector<float> hp(kParams, 0.f);
@@ -1072,7 +1075,6 @@ pg_status pg_mlp_init(pg_ctx* c, const pg_mlp_config* cfg_in) {
     PG_TRY(c, hipMemset(m.pv, 0, kParams * 4), "init moments");
     PG_TRY(c, hipMemset(m.gm, 0, m.n_grid * 4), "init moments");
     PG_TRY(c, hipMemset(m.gv, 0, m.n_grid * 4), "init moments");
-    PG_TRY(c, hipMemset(m.ggrad, 0, m.n_grid * 4), "init gradients");
     PG_TRY(c, hipMemset(m.mirror, 0, kMirror * 2), "init mirror");
     PG_TRY(c, hipMemset(m.status, 0, 4), "init status");
     PG_TRY(c, launch_mlp_mirror(m.params, m.mirror, c->stream), "mirror");
@@ -1095,8 +1097,21 @@ pg_status pg_mlp_init(pg_ctx* c, const pg_mlp_config* cfg_in) {
     const PreparedScene& h = c->host;
     for (int a = 0; a < 3; ++a) v.lo[a] = h.centre[a] - h.radius;
     v.inv_ext = h.radius > 0.f ? 1.f / (2.f * h.radius) : 0.f;
+    uint64_t goff = 0;
+    for (int l = 0; l < kHashLevels; ++l) {
+        uint64_t ents = v.dense[l] ? (uint64_t)(v.res[l] + 1) * (v.res[l] + 1) * (v.res[l] + 1) : (uint64_t)T;
+        v.gents[l] = (uint32_t)ents;
+        uint32_t copies = 1;
+        while (copies < kGradCopiesMax && (uint64_t)copies * 2u * ents <= kGradHitsPerCopy) copies *= 2;
+        v.gcopies[l] = copies;
+        v.goff[l] = goff;
+        goff += ents * 2u * v.gcopies[l];
+    }
+    m.n_ggrad = goff;
+    PG_TRY(c, dalloc(&m.ggrad, m.n_ggrad), "alloc hash-grid gradients");
+    PG_TRY(c, hipMemset(m.ggrad, 0, m.n_ggrad * 4), "init gradients");
     m.ready = true;
-    m.window_passes = 0;
+    m.window_passes = m.online_passes = m.record_passes = 0;
     return PG_OK;
 }
 
@@ -1115,7 +1130,7 @@ pg_status pg_mlp_train(pg_ctx* c, uint32_t steps, float* losses) {
     if (st != PG_OK) return st;
     if (m.w_scale_gen != c->rec_gen) {
         // mean sample weight over every record (deterministic one-block sum)
-        PG_TRY(c, launch_sum(reinterpret_cast<const float*>(c->rec_aux) + 3, n, 8, m.scal, c->stream), "weight sum");
+        PG_TRY(c, launch_sum(reinterpret_cast<const float*>(c->rec_aux) + 3, n, 8, m.sum_part, m.scal, c->stream), "weight sum");
         float sum = 0.f;
         PG_TRY(c, hipMemcpyAsync(&sum, m.scal, 4, hipMemcpyDeviceToHost, c->stream), "weight sum");
         PG_TRY(c, hipStreamSynchronize(c->stream), "weight sum");
@@ -1144,9 +1159,9 @@ pg_status pg_mlp_train(pg_ctx* c, uint32_t steps, float* losses) {
         if (st != PG_OK) return st;
         PG_TRY(c, launch_mlp_adam(m.params, m.pm, m.pv, m.mirror, m.partial, groups, ap, 1, nullptr, m.status,
                                   c->stream), "mlp adam");
-        PG_TRY(c, launch_grid_adam(m.grid, m.grid_bf, m.gm, m.gv, m.ggrad, m.n_grid, ap, 1, nullptr, m.status,
+        PG_TRY(c, launch_grid_adam(m.view, m.grid, m.grid_bf, m.gm, m.gv, m.ggrad, m.n_grid, ap, 1, nullptr, m.status,
                                    c->stream), "grid adam");
-        PG_TRY(c, launch_sum(m.loss, B, 1, m.scal + k, c->stream), "loss");
+        PG_TRY(c, launch_sum(m.loss, B, 1, m.sum_part, m.scal + k, c->stream), "loss");
         m.step = t;
     }
     (void)hipEventRecord(e1, c->stream);
@@ -1167,27 +1182,36 @@ pg_status pg_mlp_train(pg_ctx* c, uint32_t steps, float* losses) {
     return PG_OK;
 }
 
-pg_status pg_mlp_render_online(pg_ctx* c, const pg_render_params* params, uint32_t n_spp, uint32_t steps_per_pass,
-                                uint32_t window) {
+pg_status pg_mlp_render_online(pg_ctx* c, const pg_render_params* params, uint32_t n_spp,
+                                const pg_mlp_schedule* schedule) {
     PG_CHECK_CTX(c);
     if (!c->mlp.ready) return c->fail(PG_ERR_STATE, "pg_mlp_init first");
+    pg_mlp_schedule sch{1, 1, 1, 0, 0, {0, 0, 0}};
+    if (schedule) sch = *schedule;
+    if (sch.train_every == 0) sch.train_every = 1;
+    if (sch.window == 0) sch.window = 1;
     pg_render_params p = params ? *params : pg_render_params_default();
     p.integrator = PG_INTEGRATOR_GUIDED_MLP;
-    p.record = 1;
-    if (window == 0) window = 1;
+    auto& m = c->mlp;
     for (uint32_t s = 0; s < n_spp; ++s) {
-        if (c->mlp.window_passes >= window) {
+        const bool record = (m.online_passes % sch.train_every) == 0;
+        m.online_passes++;
+        p.record = record ? 1 : 0;
+        if (record && m.window_passes >= sch.window) {
             c->n_records = 0;
             c->n_aux = 0;
             c->rec_gen++;
             c->cache_ready = false;
-            c->mlp.window_passes = 0;
+            m.window_passes = 0;
         }
         pg_status st = pg_render_spp(c, &p, 1);
         if (st != PG_OK) return st;
-        c->mlp.window_passes++;
-        if (steps_per_pass && !c->mlp.diverged && c->n_records > 0) {
-            st = pg_mlp_train(c, steps_per_pass, nullptr);
+        if (!record) continue;
+        m.window_passes++;
+        m.record_passes++;
+        uint32_t steps = m.record_passes <= sch.warmup_passes ? sch.warmup_steps : sch.steps;
+        if (steps && !m.diverged && c->n_records > 0) {
+            st = pg_mlp_train(c, steps, nullptr);
             if (st != PG_OK) return st;
         }
     }
@@ -1330,9 +1354,9 @@ pg_status pg_mlp_gradient(pg_ctx* c, uint32_t n, const float* samples, float* lo
         AdamParams ap{m.cfg.lr, m.cfg.beta1, m.cfg.beta2, m.cfg.eps, 1.f, 1.f};
         if ((st = c->hip(launch_mlp_adam(m.params, m.pm, m.pv, m.mirror, m.partial, groups, ap, 0, dgp, m.status,
                                          c->stream), "gradient")) != PG_OK) break;
-        if ((st = c->hip(launch_grid_adam(m.grid, m.grid_bf, m.gm, m.gv, m.ggrad, m.n_grid, ap, 0, dgg, m.status,
+        if ((st = c->hip(launch_grid_adam(m.view, m.grid, m.grid_bf, m.gm, m.gv, m.ggrad, m.n_grid, ap, 0, dgg, m.status,
                                           c->stream), "gradient")) != PG_OK) break;
-        if ((st = c->hip(launch_sum(m.loss, n, 1, m.scal, c->stream), "loss")) != PG_OK) break;
+        if ((st = c->hip(launch_sum(m.loss, n, 1, m.sum_part, m.scal, c->stream), "loss")) != PG_OK) break;
         if ((st = c->hip(hipStreamSynchronize(c->stream), "sync")) != PG_OK) break;
         if (loss && (st = c->hip(hipMemcpy(loss, m.scal, 4, hipMemcpyDeviceToHost), "copy")) != PG_OK) break;
         if (grad_params && (st = c->hip(hipMemcpy(grad_params, dgp, kParams * 4, hipMemcpyDeviceToHost), "copy")) != PG_OK)

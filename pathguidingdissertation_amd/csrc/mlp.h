@@ -112,7 +112,18 @@ struct MlpView {
     uint32_t dense[kHashLevels];
     float lo[3];
     float inv_ext;            // position -> [0,1]^3 of the scene cube
+    // hash-table gradient accumulators: level l keeps gcopies[l] private copies of
+    // its gents[l] entries (two floats each) from float offset goff[l]; a block adds
+    // into copy blockIdx % gcopies[l]. Copies scale with the expected hits per entry
+    // (about 2^20 / entries, 1..64), so coarse levels, which every sample touches,
+    // are not serialised on a few thousand addresses, while the whole accumulator
+    // (~120 MB at 2^19 entries per level) stays cache-resident
+    uint32_t gcopies[kHashLevels];
+    uint32_t gents[kHashLevels];
+    uint64_t goff[kHashLevels];
 };
+constexpr uint32_t kGradCopiesMax = 64;
+constexpr uint32_t kGradHitsPerCopy = 1u << 20;
 
 // One training sample (64 bytes): position + normalised weight, normal + BSDF pdf
 // of wi, outgoing direction, incident (sampled) direction.
@@ -146,9 +157,12 @@ hipError_t launch_mlp_wgrad(const uint16_t* rows, uint32_t n, uint32_t groups, f
                             hipStream_t stream);
 hipError_t launch_mlp_adam(float* params, float* m, float* v, uint16_t* mirror, const float* partial, uint32_t groups,
                            const AdamParams& ap, int update, float* grad_out, uint32_t* status, hipStream_t stream);
-hipError_t launch_grid_adam(float* grid_master, uint32_t* grid_bf, float* m, float* v, float* grad, size_t n,
-                            const AdamParams& ap, int update, float* grad_out, uint32_t* status, hipStream_t stream);
-hipError_t launch_sum(const float* in, size_t n, size_t stride, float* out, hipStream_t stream);
+hipError_t launch_grid_adam(const MlpView& view, float* grid_master, uint32_t* grid_bf, float* m, float* v, float* grad,
+                            size_t n, const AdamParams& ap, int update, float* grad_out, uint32_t* status,
+                            hipStream_t stream);
+// Deterministic FP64 sum of in[i * stride], i < n; part: kSumParts doubles of scratch.
+constexpr int kSumParts = 256;
+hipError_t launch_sum(const float* in, size_t n, size_t stride, double* part, float* out, hipStream_t stream);
 hipError_t launch_mlp_mirror(const float* params, uint16_t* mirror, hipStream_t stream);
 hipError_t launch_grid_mirror(const float* grid_master, uint32_t* grid_bf, size_t n_entries, hipStream_t stream);
 hipError_t launch_mlp_eval(const MlpView& v, const float4* x, const float4* nrm, const float4* wo, uint32_t n,

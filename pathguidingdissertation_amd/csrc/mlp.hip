@@ -310,7 +310,7 @@ __global__ __launch_bounds__(kMlpBlock) void k_mlp_train(MlpView v, const TrainS
                     uint32_t ic[3];
                     float fr[3];
                     grid_cell(v, l, p01[nb], ic, fr);
-                    float* tab = ggrad + ((size_t)l << (v.log2_table + 1));
+                    float* tab = ggrad + v.goff[l] + (size_t)(blockIdx.x % v.gcopies[l]) * v.gents[l] * 2u;
 #pragma unroll
                     for (int cc = 0; cc < 8; ++cc) {
                         uint32_t e = grid_entry(v, l, ic[0] + (cc & 1), ic[1] + ((cc >> 1) & 1), ic[2] + (cc >> 2));
@@ -428,11 +428,23 @@ __global__ void k_mlp_adam(float* params, float* m, float* v, uint16_t* mirror, 
     if (mi >= 0) mirror[mi] = f2bf(th);
 }
 
-__global__ void k_grid_adam(float* master, uint16_t* bf, float* m, float* v, float* grad, size_t n, AdamParams ap,
-                            int update, float* grad_out, uint32_t* status) {
+// Sums (in copy order) and clears the private gradient copies of each hash entry,
+// then applies Adam; parameter p = (level, entry, feature) in the [level][table][2] layout.
+__global__ void k_grid_adam(MlpView view, float* master, uint16_t* bf, float* m, float* v, float* grad, size_t n,
+                            AdamParams ap, int update, float* grad_out, uint32_t* status) {
+    const size_t per_level = (size_t)2 << view.log2_table;
     for (size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x; p < n; p += (size_t)gridDim.x * blockDim.x) {
-        float gs = grad[p];
-        grad[p] = 0.f;
+        const int l = (int)(p / per_level);
+        const size_t q = p - (size_t)l * per_level;
+        float gs = 0.f;
+        if (q < (size_t)view.gents[l] * 2u) {
+            float* gp = grad + view.goff[l] + q;
+            const size_t cstride = (size_t)view.gents[l] * 2u;
+            for (uint32_t c = 0; c < view.gcopies[l]; ++c) {
+                gs += gp[c * cstride];
+                gp[c * cstride] = 0.f;
+            }
+        }
         if (grad_out) grad_out[p] = gs;
         if (!isfinite(gs)) {
             atomicOr(status, 2u);
@@ -463,14 +475,27 @@ __global__ void k_grid_mirror(const float* master, uint16_t* bf, size_t n) {
         bf[p] = f2bf(master[p]);
 }
 
-// Deterministic sum of in[i * stride] (one block, fixed order).
-__global__ __launch_bounds__(1024) void k_sum(const float* in, size_t n, size_t stride, float* out) {
-    __shared__ double red[1024];
+// Deterministic sum of in[i * stride]: each of kSumBlocks blocks reduces a fixed
+// interleaved share in FP64 into part[block]; k_sum_final adds the parts in order.
+constexpr int kSumBlocks = 256;
+__global__ __launch_bounds__(256) void k_sum_part(const float* in, size_t n, size_t stride, double* part) {
+    __shared__ double red[256];
     double a = 0.0;
-    for (size_t i = threadIdx.x; i < n; i += 1024) a += (double)in[i * stride];
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)kSumBlocks * 256)
+        a += (double)in[i * stride];
     red[threadIdx.x] = a;
     __syncthreads();
-    for (int w = 512; w > 0; w >>= 1) {
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+__global__ __launch_bounds__(256) void k_sum_final(const double* part, float* out) {
+    __shared__ double red[256];
+    red[threadIdx.x] = part[threadIdx.x];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
         if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
         __syncthreads();
     }
@@ -655,16 +680,18 @@ hipError_t launch_mlp_adam(float* params, float* m, float* v, uint16_t* mirror, 
     return hipGetLastError();
 }
 
-hipError_t launch_grid_adam(float* grid_master, uint32_t* grid_bf, float* m, float* v, float* grad, size_t n,
-                            const AdamParams& ap, int update, float* grad_out, uint32_t* status, hipStream_t stream) {
+hipError_t launch_grid_adam(const MlpView& view, float* grid_master, uint32_t* grid_bf, float* m, float* v, float* grad,
+                            size_t n, const AdamParams& ap, int update, float* grad_out, uint32_t* status,
+                            hipStream_t stream) {
     uint32_t blocks = (uint32_t)std::min<size_t>((n + 255) / 256, 8192);
-    hipLaunchKernelGGL(k_grid_adam, dim3(blocks), dim3(256), 0, stream, grid_master,
+    hipLaunchKernelGGL(k_grid_adam, dim3(blocks), dim3(256), 0, stream, view, grid_master,
                        reinterpret_cast<uint16_t*>(grid_bf), m, v, grad, n, ap, update, grad_out, status);
     return hipGetLastError();
 }
 
-hipError_t launch_sum(const float* in, size_t n, size_t stride, float* out, hipStream_t stream) {
-    hipLaunchKernelGGL(k_sum, dim3(1), dim3(1024), 0, stream, in, n, stride, out);
+hipError_t launch_sum(const float* in, size_t n, size_t stride, double* part, float* out, hipStream_t stream) {
+    hipLaunchKernelGGL(k_sum_part, dim3(kSumBlocks), dim3(256), 0, stream, in, n, stride, part);
+    hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(256), 0, stream, part, out);
     return hipGetLastError();
 }
 

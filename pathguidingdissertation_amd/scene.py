@@ -312,10 +312,12 @@ class RayTracer:
     def render(self, n_spp: int = 1):
         self.device._chk(self.device.lib.pg_render_spp(self.device.ctx, C.byref(self.params), int(n_spp)))
 
-    def render_online(self, n_spp: int, steps_per_pass: int = 1, window: int = 1):
+    def render_online(self, n_spp: int, steps: int = 1, window: int = 1, train_every: int = 1,
+                      warmup_passes: int = 0, warmup_steps: int = 0):
         """Learned-guide passes with online training (pg_mlp_render_online)."""
+        sch = _lib.PgMlpSchedule(train_every, steps, window, warmup_passes, warmup_steps)
         self.device._chk(self.device.lib.pg_mlp_render_online(self.device.ctx, C.byref(self.params), int(n_spp),
-                                                              int(steps_per_pass), int(window)))
+                                                              C.byref(sch)))
 
     def getSPP(self) -> int:  # noqa: N802 - reference name
         _, spp = self.film()
