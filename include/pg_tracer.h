@@ -258,6 +258,14 @@ pg_status pg_mlp_gradient(pg_ctx* ctx, uint32_t n, const float* samples, float* 
                           float* grad_params, float* grad_grid);
 
 /* ------------------------------------------------------------------ host-side helpers */
+/* Image metrics (Metrics.h:8-69): MSE, RMSE and RRMSE over width*height*3 values with
+ * the reference's FP32 sequential sums (RRMSE divides by sqrt of the summed squared
+ * truth + 1e-2, as Metrics.h:64-68 does), plus relMSE = mean (I-R)^2 / (R^2 + 1e-2)
+ * in FP64. Mismatched sizes give -1 values and PG_ERR_INVALID (Metrics.h:10-13).   */
+typedef struct { float mse, rmse, rrmse; double relmse; } pg_image_metrics;
+pg_status pg_metrics(const float* test, const float* truth, uint64_t n_test, uint64_t n_truth,
+                     int32_t width, int32_t height, pg_image_metrics* out);
+
 /* Procedural scenes of the benchmark configurations (no reference counterpart:
  * the reference loads scene.json + .gem files, SceneLoader.h:222-269).
  * config 1: Cornell box; 2: open interior (~1.09M tris, env map);

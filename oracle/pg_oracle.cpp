@@ -1591,4 +1591,22 @@ void orc_knn_brute(const float* pts, uint32_t n, const float* q, uint32_t nq, ui
     });
 }
 
+void orc_metrics(const float* test, const float* truth, uint64_t n_test, uint64_t n_truth, int width, int height,
+                 float* out) {
+    if (n_test != n_truth) {  // Metrics.h:10-13
+        out[0] = out[1] = out[2] = -1.f;
+        return;
+    }
+    const int n = width * height * 3;
+    float sse = 0.f, sq = 0.f;
+    for (int i = 0; i < n; ++i) {
+        float d = test[i] - truth[i];
+        sq += truth[i] * truth[i];  // Metrics.h:64: sum of squared truth values
+        sse += d * d;
+    }
+    out[0] = sse / (float)n;                  // Metrics.h:25
+    out[1] = sqrtf(sse / (float)n);           // Metrics.h:45
+    out[2] = out[1] / (sqrtf(sq) + 1e-2f);    // Metrics.h:67-68
+}
+
 }  // extern "C"

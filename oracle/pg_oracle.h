@@ -104,6 +104,10 @@ void orc_env_sample(const orc_scene* s, float ru, float rv, float* wi, float* rg
 void orc_env_sample_alias(const orc_scene* s, const float* u4, float* wi, float* rgb, float* pdf);
 /* Alias table export: prob, alias, pdf (w*h each). */
 void orc_env_tables(const orc_scene* s, float* prob, uint32_t* alias, float* pdf);
+/* Metrics::calculateMSE / RMSE / RRMSE (Metrics.h:8-69): FP32, sequential sums over
+ * width*height*3 values; size mismatch gives -1. out: mse, rmse, rrmse.           */
+void orc_metrics(const float* test, const float* truth, uint64_t n_test, uint64_t n_truth, int width, int height,
+                 float* out);
 void orc_env_eval(const orc_scene* s, const float* wi, float* rgb, float* pdf);
 float orc_env_marginal_integral(const orc_scene* s);
 void orc_scene_bounds(const orc_scene* s, float* centre, float* radius);

@@ -99,6 +99,7 @@ def lib():
         L.orc_env_eval.argtypes = [C.c_void_p, _F, _F, _F]
         L.orc_env_sample_alias.argtypes = [C.c_void_p, _F, _F, _F, _F]
         L.orc_env_tables.argtypes = [C.c_void_p, _F, _U32, _F]
+        L.orc_metrics.argtypes = [_F, _F, C.c_uint64, C.c_uint64, C.c_int, C.c_int, _F]
         L.orc_env_marginal_integral.restype = C.c_float
         L.orc_env_marginal_integral.argtypes = [C.c_void_p]
         L.orc_scene_bounds.argtypes = [C.c_void_p, _F, _F]
@@ -147,6 +148,7 @@ def ref():
         R.ref_env_eval.argtypes = [C.c_void_p, _F, _F]
         R.ref_env_power.restype = C.c_float
         R.ref_env_power.argtypes = [C.c_void_p]
+        R.ref_metrics.argtypes = [_F, _F, C.c_int, C.c_int, C.c_int, C.c_int, _F]
         _ref = R
     return _ref
 

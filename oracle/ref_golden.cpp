@@ -26,7 +26,9 @@ namespace std { using ::cosf; using ::sinf; using ::tanf; using ::powf; using ::
 #define private public
 #include "Geometry.h"
 #include "Scene.h"
+#include "Metrics.h"
 #undef private
+#include <vector>
 
 namespace {
 struct ListSampler : Sampler {
@@ -208,5 +210,13 @@ void ref_env_eval(void* p, const float* wi, float* out) {
     out[0] = c.r; out[1] = c.g; out[2] = c.b; out[3] = e->PDF(sd, w);
 }
 float ref_env_power(void* p) { return ((EnvironmentMap*)p)->totalIntegratedPower(); }
+
+void ref_metrics(const float* test, const float* truth, int n_test, int n_truth, int w, int h, float* out) {
+    std::vector<float> a(test, test + n_test), b(truth, truth + n_truth);
+    Metrics m;
+    out[0] = m.calculateMSE(a, b, w, h);
+    out[1] = m.calculateRMSE(a, b, w, h);
+    out[2] = m.calculateRRMSE(a, b, w, h);
+}
 
 }  // extern "C"

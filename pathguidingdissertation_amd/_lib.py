@@ -69,6 +69,10 @@ class PgMlpSchedule(C.Structure):
                 ("warmup_passes", C.c_uint32), ("warmup_steps", C.c_uint32), ("reserved", C.c_uint32 * 3)]
 
 
+class PgImageMetrics(C.Structure):
+    _fields_ = [("mse", C.c_float), ("rmse", C.c_float), ("rrmse", C.c_float), ("relmse", C.c_double)]
+
+
 _P = C.c_void_p
 _F = C.POINTER(C.c_float)
 _U32 = C.POINTER(C.c_uint32)
@@ -116,6 +120,7 @@ SIGNATURES = {
     "pg_mlp_levels": (C.c_int32, [_P, _U32, _U32, _F]),
     "pg_mlp_eval": (C.c_int32, [_P, C.c_uint32, _F, _F, _F, _F, _F, _F, _F, _F]),
     "pg_mlp_gradient": (C.c_int32, [_P, C.c_uint32, _F, _F, _F, _F]),
+    "pg_metrics": (C.c_int32, [_F, _F, C.c_uint64, C.c_uint64, C.c_int32, C.c_int32, C.POINTER(PgImageMetrics)]),
     "pg_scene_generate": (C.c_int32, [C.c_int32, C.c_uint32, C.POINTER(_P)]),
     "pg_scene_sizes": (C.c_int32, [_P, _U32, _U32, _I32, _I32]),
     "pg_scene_copy": (C.c_int32, [_P, _F, _F, _U32, C.POINTER(PgMaterial), _F, C.POINTER(PgCameraDesc)]),

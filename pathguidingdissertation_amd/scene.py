@@ -348,6 +348,20 @@ class RayTracer:
         self.device._chk(self.device.lib.pg_set_counting(self.device.ctx, 1 if on else 0))
 
 
+def metrics(img: np.ndarray, ref: np.ndarray) -> dict:
+    """Metrics::calculateMSE / RMSE / RRMSE (Metrics.h:8-69, FP32, bit-exact) and
+    relMSE (FP64) of an (H, W, 3) image against a reference, via pg_metrics."""
+    a = np.ascontiguousarray(img, np.float32)
+    b = np.ascontiguousarray(ref, np.float32)
+    h, w = (a.shape[0], a.shape[1]) if a.ndim == 3 else (1, a.size // 3)
+    out = _lib.PgImageMetrics()
+    code = load().pg_metrics(_fp(a), _fp(b), a.size, b.size, w, h, C.byref(out))
+    res = {"mse": out.mse, "rmse": out.rmse, "rrmse": out.rrmse, "relmse": out.relmse}
+    if code != _lib.PG_OK:
+        raise _lib.PgError(code, "image sizes differ")
+    return res
+
+
 def relmse(img: np.ndarray, ref: np.ndarray, eps: float = 1e-2) -> float:
     """Mean over pixels and RGB of (I - R)^2 / (R^2 + eps), in FP64."""
     i = img.astype(np.float64)

@@ -162,6 +162,17 @@ def make_ref():
     power = R.ref_env_power(he)
     R.ref_env_destroy(he)
     np.savez_compressed(OUT / "ref_env.npz", env=env, u=u2, sample=samp, dirs=dirs, eval=ev, power=np.float32(power))
+    # --- Metrics.h:8-69 on odd-sized images with a wide dynamic range
+    w, h = 37, 23
+    truth = (rng.random((h, w, 3)) ** 4 * 50).astype(np.float32)
+    truth[0, 0] = [5e4, 0.0, 1e-6]
+    test = (truth + rng.normal(0, 0.3, truth.shape) * (truth + 0.05)).astype(np.float32)
+    test[5, 7] = [1e3, 0.0, -2.0]
+    out = np.empty(3, np.float32)
+    R.ref_metrics(po.fp(test.reshape(-1)), po.fp(truth.reshape(-1)), test.size, truth.size, w, h, po.fp(out))
+    bad = np.empty(3, np.float32)
+    R.ref_metrics(po.fp(test.reshape(-1)), po.fp(truth.reshape(-1)), test.size, truth.size - 3, w, h, po.fp(bad))
+    np.savez_compressed(OUT / "ref_metrics.npz", test=test, truth=truth, out=out, mismatch=bad)
     print("reference vectors written")
 
 

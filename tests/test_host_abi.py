@@ -122,3 +122,15 @@ def test_no_cpu_fallback_without_device(pg):
     assert rc == pg._lib.PG_ERR_DEVICE
     with pytest.raises(pg.PgError):
         pg.Device(0)
+
+
+def test_metrics_bit_exact_against_reference(pg):
+    """pg_metrics keeps Metrics.h's FP32 sequential sums (bit-exact against the
+    reference's own Metrics) and adds relMSE in FP64."""
+    d = np.load(GOLDEN / "ref_metrics.npz")
+    m = pg.metrics(d["test"], d["truth"])
+    got = np.array([m["mse"], m["rmse"], m["rrmse"]], np.float32)
+    assert np.array_equal(got.view(np.uint32), d["out"].view(np.uint32)), (got, d["out"])
+    assert abs(m["relmse"] - pg.relmse(d["test"], d["truth"])) <= 1e-12 * m["relmse"]
+    with pytest.raises(pg.PgError):
+        pg.metrics(d["test"], d["truth"][:-1])

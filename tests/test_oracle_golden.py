@@ -171,3 +171,19 @@ def test_live_reference_driver(pg, po):
     assert np.array_equal(oid, ids)
     assert np.array_equal(bits(ot), bits(t))
     assert np.array_equal(bits(ob), bits(b)) and np.array_equal(bits(og), bits(g))
+
+
+def test_metrics_match_reference():
+    """Metrics::calculateMSE / RMSE / RRMSE (Metrics.h:8-69) bit for bit, and -1 on
+    mismatched sizes."""
+    import ctypes as C
+    from oracle import pg_oracle as po
+    d = np.load(GOLDEN / "ref_metrics.npz")
+    test, truth = d["test"], d["truth"]
+    h, w = truth.shape[:2]
+    out = np.empty(3, np.float32)
+    po.lib().orc_metrics(po.fp(test.reshape(-1)), po.fp(truth.reshape(-1)), test.size, truth.size, w, h, po.fp(out))
+    assert np.array_equal(out.view(np.uint32), d["out"].view(np.uint32)), (out, d["out"])
+    po.lib().orc_metrics(po.fp(test.reshape(-1)), po.fp(truth.reshape(-1)), test.size, truth.size - 3, w, h,
+                         po.fp(out))
+    assert np.array_equal(out, d["mismatch"])
