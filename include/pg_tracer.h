@@ -257,6 +257,49 @@ pg_status pg_mlp_eval(pg_ctx* ctx, uint32_t n, const float* pos, const float* nr
 pg_status pg_mlp_gradient(pg_ctx* ctx, uint32_t n, const float* samples, float* loss,
                           float* grad_params, float* grad_grid);
 
+/* ------------------------------------------------------------------ batches of views */
+/* No counterpart in the reference, which renders one camera per RayTracer::render
+ * (Renderer.h:648-747): many views of one or more scenes rendered together. Every
+ * pass runs all views' paths through the same persistent (grid-stride) wavefront
+ * kernels in view-aligned chunks of at most queue_paths paths; with use_graph the
+ * pass is captured once as a hipGraph and replayed per SPP (the sample index lives
+ * on the device). A view's samples are exactly those of rendering it alone with the
+ * same seed (RNG keyed by seed, sample and pixel within the view). Unguided only.  */
+typedef struct pg_batch pg_batch;
+typedef struct pg_batch_config {
+    uint32_t queue_paths;         /* paths per chunk (0: all views in one chunk)      */
+    uint32_t trace_blocks_per_cu; /* traversal grid (0: as many as the LDS allows)    */
+    uint32_t shade_blocks_per_cu; /* shade grid (0: 8)                                */
+    uint32_t stack;               /* traversal stack entries (0: deepest BVH, rounded
+                                     up to 16; must hold the deepest BVH)             */
+    uint32_t use_graph;           /* replay a captured hipGraph per SPP               */
+    uint32_t reserved[3];
+} pg_batch_config;
+typedef struct pg_batch_stats {
+    uint64_t paths, closest_rays, shadow_rays;
+    double gpu_ms;                /* GPU time of the rendered passes                  */
+    uint32_t views, scenes, chunks, graph_nodes;
+} pg_batch_stats;
+pg_status pg_batch_create(pg_ctx* ctx, pg_batch** out);
+pg_status pg_batch_destroy(pg_batch* batch);
+/* A scene as pg_scene_upload takes it (its BVH is built on the host here).        */
+pg_status pg_batch_add_scene(pg_batch* batch, const float* positions, const float* normals,
+                             const uint32_t* material_index, uint32_t n_tris,
+                             const pg_material* materials, uint32_t n_materials,
+                             const float* env_rgb, int32_t env_width, int32_t env_height,
+                             uint32_t* scene_id);
+pg_status pg_batch_add_view(pg_batch* batch, uint32_t scene_id, const pg_camera_desc* camera,
+                            uint32_t* view_id);
+pg_status pg_batch_configure(pg_batch* batch, const pg_batch_config* cfg);
+/* Times candidate configurations (queue size, blocks per CU, stack depth, graph)
+ * by coordinate descent on a few SPP each, keeps the fastest and clears the films. */
+pg_status pg_batch_autotune(pg_batch* batch, const pg_render_params* params, pg_batch_config* best,
+                            double* ms_per_spp);
+pg_status pg_batch_clear(pg_batch* batch);
+pg_status pg_batch_render(pg_batch* batch, const pg_render_params* params, uint32_t n_spp);
+pg_status pg_batch_film_read(pg_batch* batch, uint32_t view_id, float* rgb_sum, uint32_t* spp);
+pg_status pg_batch_stats_read(pg_batch* batch, pg_batch_stats* stats);
+
 /* ------------------------------------------------------------------ host-side helpers */
 /* Image metrics (Metrics.h:8-69): MSE, RMSE and RRMSE over width*height*3 values with
  * the reference's FP32 sequential sums (RRMSE divides by sqrt of the summed squared

@@ -73,6 +73,17 @@ class PgImageMetrics(C.Structure):
     _fields_ = [("mse", C.c_float), ("rmse", C.c_float), ("rrmse", C.c_float), ("relmse", C.c_double)]
 
 
+class PgBatchConfig(C.Structure):
+    _fields_ = [("queue_paths", C.c_uint32), ("trace_blocks_per_cu", C.c_uint32), ("shade_blocks_per_cu", C.c_uint32),
+                ("stack", C.c_uint32), ("use_graph", C.c_uint32), ("reserved", C.c_uint32 * 3)]
+
+
+class PgBatchStats(C.Structure):
+    _fields_ = [("paths", C.c_uint64), ("closest_rays", C.c_uint64), ("shadow_rays", C.c_uint64),
+                ("gpu_ms", C.c_double), ("views", C.c_uint32), ("scenes", C.c_uint32), ("chunks", C.c_uint32),
+                ("graph_nodes", C.c_uint32)]
+
+
 _P = C.c_void_p
 _F = C.POINTER(C.c_float)
 _U32 = C.POINTER(C.c_uint32)
@@ -120,6 +131,18 @@ SIGNATURES = {
     "pg_mlp_levels": (C.c_int32, [_P, _U32, _U32, _F]),
     "pg_mlp_eval": (C.c_int32, [_P, C.c_uint32, _F, _F, _F, _F, _F, _F, _F, _F]),
     "pg_mlp_gradient": (C.c_int32, [_P, C.c_uint32, _F, _F, _F, _F]),
+    "pg_batch_create": (C.c_int32, [_P, C.POINTER(_P)]),
+    "pg_batch_destroy": (C.c_int32, [_P]),
+    "pg_batch_add_scene": (C.c_int32, [_P, _F, _F, _U32, C.c_uint32, C.POINTER(PgMaterial), C.c_uint32, _F,
+                                       C.c_int32, C.c_int32, _U32]),
+    "pg_batch_add_view": (C.c_int32, [_P, C.c_uint32, C.POINTER(PgCameraDesc), _U32]),
+    "pg_batch_configure": (C.c_int32, [_P, C.POINTER(PgBatchConfig)]),
+    "pg_batch_autotune": (C.c_int32, [_P, C.POINTER(PgRenderParams), C.POINTER(PgBatchConfig),
+                                      C.POINTER(C.c_double)]),
+    "pg_batch_clear": (C.c_int32, [_P]),
+    "pg_batch_render": (C.c_int32, [_P, C.POINTER(PgRenderParams), C.c_uint32]),
+    "pg_batch_film_read": (C.c_int32, [_P, C.c_uint32, _F, _U32]),
+    "pg_batch_stats_read": (C.c_int32, [_P, C.POINTER(PgBatchStats)]),
     "pg_metrics": (C.c_int32, [_F, _F, C.c_uint64, C.c_uint64, C.c_int32, C.c_int32, C.POINTER(PgImageMetrics)]),
     "pg_scene_generate": (C.c_int32, [C.c_int32, C.c_uint32, C.POINTER(_P)]),
     "pg_scene_sizes": (C.c_int32, [_P, _U32, _U32, _I32, _I32]),

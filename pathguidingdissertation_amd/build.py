@@ -18,7 +18,7 @@ BUILD = PKG / "_build"
 LIB = PKG / "libpgtrace.so"
 
 HOST_SOURCES = ["host_prep.cpp", "scenes.cpp", "metrics.cpp"]
-HIP_SOURCES = ["kernels.hip", "guiding.hip", "mlp.hip", "api.hip"]
+HIP_SOURCES = ["kernels.hip", "guiding.hip", "mlp.hip", "batch.hip", "api.hip"]
 
 CXXFLAGS = ["-std=c++17", "-O3", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-pthread", "-Wall",
             "-Wno-unused-function"]

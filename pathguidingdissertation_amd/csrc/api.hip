@@ -14,6 +14,7 @@
 #include "kernels.h"
 #include "guiding.h"
 #include "mlp.h"
+#include "ctx_internal.h"
 
 using namespace pg;
 
@@ -184,6 +185,13 @@ struct pg_ctx {
         ps_capacity = 0;
     }
 };
+
+namespace pg_internal {
+hipStream_t stream(pg_ctx* c) { return c->stream; }
+int cu_count(pg_ctx* c) { return c->cu_count; }
+int device(pg_ctx* c) { return c->device; }
+pg_status fail(pg_ctx* c, pg_status code, const std::string& m) { return c->fail(code, m); }
+}  // namespace pg_internal
 
 #define PG_CHECK_CTX(c)                                  \
     do {                                                 \

@@ -1,0 +1,504 @@
+// batch.hip — batches of views over several scenes, replayed as a hipGraph per SPP,
+// with an autotuner for queue size, launch grids and traversal stack depth.
+//
+// The reference renders one camera per RayTracer::render (Renderer.h:648-747); many
+// small views rendered one after another are launch-bound on a GPU (a 128x128 view
+// is 16K paths: a few microseconds of work per kernel). A batch puts all views' paths
+// into one global index space (view-major), runs them through the same persistent
+// grid-stride kernels as the single-view path, with the scene chosen per path, in
+// view-aligned chunks, and captures the whole pass (every chunk, every bounce) once
+// as a hipGraph: one graph launch per SPP.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/pg_tracer.h"
+#include "ctx_internal.h"
+#include "host_prep.hpp"
+#include "kernels.h"
+
+using namespace pg;
+
+namespace {
+template <class T>
+hipError_t balloc(T** p, size_t count) {
+    *p = nullptr;
+    if (count == 0) return hipSuccess;
+    return hipMalloc((void**)p, count * sizeof(T));
+}
+template <class T>
+void bfree(T*& p) {
+    if (p) (void)hipFree((void*)p);
+    p = nullptr;
+}
+
+struct SceneBuf {
+    PreparedScene host;
+    BvhNode* nodes = nullptr;
+    TriIsect* tris = nullptr;
+    TriShade* shade = nullptr;
+    MaterialRec* mats = nullptr;
+    LightRec* lights = nullptr;
+    float4* env_tex = nullptr;
+    float* env_pdf = nullptr;
+    float* env_prob = nullptr;
+    uint32_t* env_alias = nullptr;
+    DevScene dev{};
+    ~SceneBuf() {
+        bfree(nodes); bfree(tris); bfree(shade); bfree(mats); bfree(lights);
+        bfree(env_tex); bfree(env_pdf); bfree(env_prob); bfree(env_alias);
+    }
+    // Same device layout as pg_scene_upload.
+    hipError_t upload() {
+        const PreparedScene& h = host;
+        hipError_t e;
+#define PG_B(x) if ((e = (x)) != hipSuccess) return e
+        PG_B(balloc(&nodes, h.node_slots.size()));
+        PG_B(balloc(&tris, h.tris.size()));
+        PG_B(balloc(&shade, h.shade.size()));
+        PG_B(balloc(&mats, h.mats.size()));
+        PG_B(balloc(&lights, h.lights.size() ? h.lights.size() : 1));
+        PG_B(hipMemcpy(nodes, h.node_slots.data(), h.node_slots.size() * sizeof(BvhNode), hipMemcpyHostToDevice));
+        PG_B(hipMemcpy(tris, h.tris.data(), h.tris.size() * sizeof(TriIsect), hipMemcpyHostToDevice));
+        PG_B(hipMemcpy(shade, h.shade.data(), h.shade.size() * sizeof(TriShade), hipMemcpyHostToDevice));
+        PG_B(hipMemcpy(mats, h.mats.data(), h.mats.size() * sizeof(MaterialRec), hipMemcpyHostToDevice));
+        if (!h.lights.empty())
+            PG_B(hipMemcpy(lights, h.lights.data(), h.lights.size() * sizeof(LightRec), hipMemcpyHostToDevice));
+        if (h.has_env) {
+            size_t n = (size_t)h.env.w * h.env.h;
+            PG_B(balloc(&env_tex, n));
+            PG_B(balloc(&env_pdf, n));
+            PG_B(balloc(&env_prob, n));
+            PG_B(balloc(&env_alias, n));
+            PG_B(hipMemcpy(env_tex, h.env.texels.data(), n * 16, hipMemcpyHostToDevice));
+            PG_B(hipMemcpy(env_pdf, h.env.pdf.data(), n * 4, hipMemcpyHostToDevice));
+            PG_B(hipMemcpy(env_prob, h.env.prob.data(), n * 4, hipMemcpyHostToDevice));
+            PG_B(hipMemcpy(env_alias, h.env.alias.data(), n * 4, hipMemcpyHostToDevice));
+        }
+#undef PG_B
+        DevScene& d = dev;
+        d.nodes = nodes; d.tris = tris; d.shade = shade; d.mats = mats; d.lights = lights;
+        d.env_tex = env_tex; d.env_pdf = env_pdf; d.env_prob = env_prob; d.env_alias = env_alias;
+        d.n_lights = (int32_t)h.lights.size();
+        d.env_w = h.has_env ? h.env.w : 0;
+        d.env_h = h.has_env ? h.env.h : 0;
+        d.has_env = h.has_env ? 1 : 0;
+        d.centre[0] = h.centre[0]; d.centre[1] = h.centre[1]; d.centre[2] = h.centre[2];
+        d.radius = h.radius;
+        d.cam = h.camera;
+        return hipSuccess;
+    }
+};
+}  // namespace
+
+struct pg_batch {
+    pg_ctx* ctx = nullptr;
+    std::vector<std::unique_ptr<SceneBuf>> scenes;
+    std::vector<BatchView> views;
+    uint32_t total = 0;
+    int max_depth = 0;
+    DevScene* d_scenes = nullptr;
+    BatchView* d_views = nullptr;
+    bool dirty = true;  // tables, film, chunks and path state need rebuilding
+    PathState ps{};
+    uint32_t ps_cap = 0;
+    uint32_t* d_counters = nullptr;
+    unsigned long long* d_stats = nullptr;
+    uint32_t* d_sample = nullptr;
+    float4* film = nullptr;
+    uint32_t film_cap = 0;
+    uint32_t spp = 0;
+    pg_batch_config cfg{0, 0, 0, 0, 1, {0, 0, 0}};
+    std::vector<std::pair<uint32_t, uint32_t>> chunks;  // (first global path, paths)
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    bool graph_ok = false;
+    pg_render_params gparams{};
+    size_t graph_nodes = 0;
+    uint64_t paths = 0;
+    double gpu_ms = 0.0;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+
+    pg_status fail(pg_status code, const std::string& m) { return pg_internal::fail(ctx, code, m); }
+    pg_status hip(hipError_t e, const char* what) {
+        if (e == hipSuccess) return PG_OK;
+        return fail(e == hipErrorOutOfMemory ? PG_ERR_NOMEM : PG_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+    }
+    hipStream_t stream() const { return pg_internal::stream(ctx); }
+    void drop_graph() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+        graph_ok = false;
+        graph_nodes = 0;
+    }
+    void free_paths() {
+        bfree(ps.beta); bfree(ps.radiance); bfree(ps.film_pos); bfree(ps.info);
+        for (int k = 0; k < 2; ++k) { bfree(ps.queue[k]); bfree(ps.ray_o[k]); bfree(ps.ray_d[k]); }
+        bfree(ps.hit); bfree(ps.sh_o); bfree(ps.sh_d); bfree(ps.sh_c); bfree(ps.path_view);
+        ps_cap = 0;
+    }
+    int stack() const {
+        int s = cfg.stack ? (int)cfg.stack : max_depth;
+        return stack_entries(std::max(s, 1));
+    }
+};
+
+#define PG_BTRY(b, expr, what)                        \
+    do {                                              \
+        pg_status _s = (b)->hip((expr), (what));      \
+        if (_s != PG_OK) return _s;                   \
+    } while (0)
+
+namespace {
+
+// Tables, film, view-aligned chunks and the path state for the current configuration.
+pg_status prepare(pg_batch* b) {
+    if (!b->dirty) return PG_OK;
+    if (b->views.empty()) return b->fail(PG_ERR_STATE, "batch has no views");
+    b->drop_graph();
+    PG_BTRY(b, hipStreamSynchronize(b->stream()), "sync");
+    // tables
+    bfree(b->d_scenes);
+    bfree(b->d_views);
+    std::vector<DevScene> ds;
+    for (auto& s : b->scenes) ds.push_back(s->dev);
+    PG_BTRY(b, balloc(&b->d_scenes, ds.size()), "alloc scene table");
+    PG_BTRY(b, balloc(&b->d_views, b->views.size()), "alloc view table");
+    PG_BTRY(b, hipMemcpy(b->d_scenes, ds.data(), ds.size() * sizeof(DevScene), hipMemcpyHostToDevice), "scene table");
+    PG_BTRY(b, hipMemcpy(b->d_views, b->views.data(), b->views.size() * sizeof(BatchView), hipMemcpyHostToDevice),
+            "view table");
+    // film (global pixel index), kept across reconfigurations of the same views
+    if (b->film_cap < b->total) {
+        bfree(b->film);
+        PG_BTRY(b, balloc(&b->film, b->total), "alloc batch film");
+        b->film_cap = b->total;
+        PG_BTRY(b, launch_clear_film(b->film, b->total, b->stream()), "clear film");
+        b->spp = 0;
+        PG_BTRY(b, hipMemset(b->d_sample, 0, 4), "reset sample");
+    }
+    // chunks: whole views, greedily packed up to queue_paths
+    b->chunks.clear();
+    const uint64_t q = b->cfg.queue_paths ? b->cfg.queue_paths : ~0ull;
+    uint32_t base = 0, n = 0, largest = 0;
+    for (const BatchView& v : b->views) {
+        uint32_t px = v.width * v.height;
+        if (n > 0 && (uint64_t)n + px > q) {
+            b->chunks.push_back({base, n});
+            base += n;
+            n = 0;
+        }
+        n += px;
+    }
+    if (n) b->chunks.push_back({base, n});
+    for (auto& c : b->chunks) largest = std::max(largest, c.second);
+    if (b->ps_cap < largest) {
+        b->free_paths();
+        PathState& p = b->ps;
+        uint32_t m = largest;
+        PG_BTRY(b, balloc(&p.beta, m), "alloc batch paths");
+        PG_BTRY(b, balloc(&p.radiance, m), "alloc batch paths");
+        PG_BTRY(b, balloc(&p.film_pos, m), "alloc batch paths");
+        PG_BTRY(b, balloc(&p.info, m), "alloc batch paths");
+        for (int k = 0; k < 2; ++k) {
+            PG_BTRY(b, balloc(&p.queue[k], m), "alloc batch queues");
+            PG_BTRY(b, balloc(&p.ray_o[k], m), "alloc batch queues");
+            PG_BTRY(b, balloc(&p.ray_d[k], m), "alloc batch queues");
+        }
+        PG_BTRY(b, balloc(&p.hit, m), "alloc batch hits");
+        PG_BTRY(b, balloc(&p.sh_o, m), "alloc batch shadow");
+        PG_BTRY(b, balloc(&p.sh_d, m), "alloc batch shadow");
+        PG_BTRY(b, balloc(&p.sh_c, m), "alloc batch shadow");
+        PG_BTRY(b, balloc(&p.path_view, m), "alloc batch views");
+        p.capacity = m;
+        b->ps_cap = m;
+    }
+    b->ps.counters = b->d_counters;
+    b->ps.stats = b->d_stats;
+    b->ps.bscenes = b->d_scenes;
+    b->ps.bviews = b->d_views;
+    b->dirty = false;
+    return PG_OK;
+}
+
+// One pass (one sample per pixel of every view) on the batch stream.
+hipError_t enqueue_pass(pg_batch* b, const pg_render_params& prm) {
+    const int cu = pg_internal::cu_count(b->ctx);
+    const int stack = b->stack();
+    const uint32_t tgrid = b->cfg.trace_blocks_per_cu ? (uint32_t)cu * b->cfg.trace_blocks_per_cu : trace_grid(cu, stack);
+    const uint32_t sgrid = (uint32_t)cu * (b->cfg.shade_blocks_per_cu ? b->cfg.shade_blocks_per_cu : 8u);
+    const DevScene& s0 = b->scenes[0]->dev;  // unused by the batch kernels' per-path scenes
+    const uint32_t nv = (uint32_t)b->views.size();
+    GuideCtx gc{};
+    hipStream_t st = b->stream();
+    hipError_t e;
+    for (const auto& ch : b->chunks) {
+        PassParams pp{prm.seed, 0u, 0u, 0u, prm.max_depth, prm.rr_depth, PG_INTEGRATOR_PATH};
+        if ((e = launch_raygen_batch(b->ps, nv, ch.first, ch.second, prm.seed, b->d_sample, st)) != hipSuccess) return e;
+        for (uint32_t depth = 0; depth <= prm.max_depth; ++depth) {
+            int parity = (int)(depth & 1u);
+            if ((e = launch_extend(s0, b->ps, parity, stack, false, tgrid, st, true)) != hipSuccess) return e;
+            if ((e = launch_shade(s0, b->ps, pp, gc, 0, parity, (int)depth, sgrid, st, true)) != hipSuccess) return e;
+            if ((e = launch_shadow(s0, b->ps, stack, false, tgrid, st, true)) != hipSuccess) return e;
+        }
+        if ((e = launch_resolve_batch(b->ps, nv, ch.first, ch.second, b->film, st)) != hipSuccess) return e;
+    }
+    return launch_sample_advance(b->d_sample, st);
+}
+
+bool same_params(const pg_render_params& a, const pg_render_params& b) {
+    return a.seed == b.seed && a.max_depth == b.max_depth && a.rr_depth == b.rr_depth;
+}
+
+pg_status render(pg_batch* b, const pg_render_params& prm, uint32_t n_spp) {
+    pg_status st = prepare(b);
+    if (st != PG_OK) return st;
+    hipStream_t s = b->stream();
+    if (b->cfg.use_graph && (!b->graph_ok || !same_params(prm, b->gparams))) {
+        b->drop_graph();
+        PG_BTRY(b, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin capture");
+        hipError_t e = enqueue_pass(b, prm);
+        hipGraph_t g = nullptr;
+        hipError_t e2 = hipStreamEndCapture(s, &g);
+        if (e != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            return b->hip(e, "capture pass");
+        }
+        PG_BTRY(b, e2, "end capture");
+        b->graph = g;
+        PG_BTRY(b, hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0), "instantiate graph");
+        (void)hipGraphGetNodes(g, nullptr, &b->graph_nodes);
+        b->graph_ok = true;
+        b->gparams = prm;
+    }
+    PG_BTRY(b, hipEventRecord(b->e0, s), "event");
+    for (uint32_t k = 0; k < n_spp; ++k) {
+        if (b->cfg.use_graph) PG_BTRY(b, hipGraphLaunch(b->exec, s), "graph launch");
+        else PG_BTRY(b, enqueue_pass(b, prm), "batch pass");
+    }
+    PG_BTRY(b, hipEventRecord(b->e1, s), "event");
+    PG_BTRY(b, hipStreamSynchronize(s), "batch render");
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, b->e0, b->e1);
+    b->gpu_ms += ms;
+    b->spp += n_spp;
+    b->paths += (uint64_t)b->total * n_spp;
+    return PG_OK;
+}
+
+pg_status clear(pg_batch* b) {
+    pg_status st = prepare(b);
+    if (st != PG_OK) return st;
+    PG_BTRY(b, launch_clear_film(b->film, b->total, b->stream()), "clear film");
+    PG_BTRY(b, hipMemsetAsync(b->d_sample, 0, 4, b->stream()), "reset sample");
+    PG_BTRY(b, hipStreamSynchronize(b->stream()), "sync");
+    b->spp = 0;
+    return PG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+pg_status pg_batch_create(pg_ctx* ctx, pg_batch** out) {
+    if (!ctx || !out) return PG_ERR_INVALID;
+    *out = nullptr;
+    if (hipSetDevice(pg_internal::device(ctx)) != hipSuccess) return pg_internal::fail(ctx, PG_ERR_DEVICE, "hipSetDevice");
+    std::unique_ptr<pg_batch> b(new (std::nothrow) pg_batch());
+    if (!b) return PG_ERR_NOMEM;
+    b->ctx = ctx;
+    PG_BTRY(b.get(), balloc(&b->d_counters, 4), "alloc batch counters");
+    PG_BTRY(b.get(), balloc(&b->d_stats, 8), "alloc batch stats");
+    PG_BTRY(b.get(), balloc(&b->d_sample, 1), "alloc batch sample");
+    PG_BTRY(b.get(), hipMemset(b->d_counters, 0, 16), "init counters");
+    PG_BTRY(b.get(), hipMemset(b->d_stats, 0, 64), "init stats");
+    PG_BTRY(b.get(), hipMemset(b->d_sample, 0, 4), "init sample");
+    PG_BTRY(b.get(), hipEventCreate(&b->e0), "event");
+    PG_BTRY(b.get(), hipEventCreate(&b->e1), "event");
+    *out = b.release();
+    return PG_OK;
+}
+
+pg_status pg_batch_destroy(pg_batch* b) {
+    if (!b) return PG_OK;
+    (void)hipSetDevice(pg_internal::device(b->ctx));
+    (void)hipStreamSynchronize(b->stream());
+    b->drop_graph();
+    b->free_paths();
+    bfree(b->d_scenes); bfree(b->d_views); bfree(b->film);
+    bfree(b->d_counters); bfree(b->d_stats); bfree(b->d_sample);
+    if (b->e0) (void)hipEventDestroy(b->e0);
+    if (b->e1) (void)hipEventDestroy(b->e1);
+    delete b;
+    return PG_OK;
+}
+
+pg_status pg_batch_add_scene(pg_batch* b, const float* positions, const float* normals, const uint32_t* material_index,
+                             uint32_t n_tris, const pg_material* materials, uint32_t n_materials, const float* env_rgb,
+                             int32_t env_width, int32_t env_height, uint32_t* scene_id) {
+    if (!b) return PG_ERR_INVALID;
+    if (hipSetDevice(pg_internal::device(b->ctx)) != hipSuccess) return b->fail(PG_ERR_DEVICE, "hipSetDevice");
+    std::unique_ptr<SceneBuf> s(new (std::nothrow) SceneBuf());
+    if (!s) return PG_ERR_NOMEM;
+    pg_camera_desc cam{{0.f, 0.f, 1.f}, {0.f, 0.f, 0.f}, {0.f, 1.f, 0.f}, 45.f, 1.f, 1.f};
+    std::string e = prepare_scene(positions, normals, material_index, n_tris, materials, n_materials, env_rgb,
+                                  env_width, env_height, cam, s->host);
+    if (!e.empty()) return b->fail(PG_ERR_INVALID, e);
+    if (s->host.bvh.max_depth > 64)
+        return b->fail(PG_ERR_OVERFLOW, "BVH deeper than the 64-entry traversal stack (Geometry.h:461)");
+    PG_BTRY(b, s->upload(), "upload batch scene");
+    b->max_depth = std::max(b->max_depth, (int)s->host.bvh.max_depth);
+    if (scene_id) *scene_id = (uint32_t)b->scenes.size();
+    b->scenes.push_back(std::move(s));
+    b->dirty = true;
+    return PG_OK;
+}
+
+pg_status pg_batch_add_view(pg_batch* b, uint32_t scene_id, const pg_camera_desc* camera, uint32_t* view_id) {
+    if (!b || !camera) return PG_ERR_INVALID;
+    if (scene_id >= b->scenes.size()) return b->fail(PG_ERR_INVALID, "unknown scene");
+    uint32_t w = (uint32_t)camera->width, h = (uint32_t)camera->height;
+    if (w == 0 || h == 0 || (float)w != camera->width || (float)h != camera->height)
+        return b->fail(PG_ERR_INVALID, "view resolution must be positive integers");
+    if ((uint64_t)b->total + (uint64_t)w * h > 0x7fffffffull) return b->fail(PG_ERR_OVERFLOW, "more than 2^31 pixels");
+    BatchView v;
+    v.cam = make_camera(*camera);
+    v.scene = scene_id;
+    v.width = w;
+    v.height = h;
+    v.pixel_offset = b->total;
+    if (view_id) *view_id = (uint32_t)b->views.size();
+    b->views.push_back(v);
+    b->total += w * h;
+    b->dirty = true;
+    b->film_cap = 0;  // the film layout changes
+    return PG_OK;
+}
+
+pg_status pg_batch_configure(pg_batch* b, const pg_batch_config* cfg) {
+    if (!b) return PG_ERR_INVALID;
+    pg_batch_config c = cfg ? *cfg : pg_batch_config{0, 0, 0, 0, 1, {0, 0, 0}};
+    if (c.stack && (int)c.stack < b->max_depth) return b->fail(PG_ERR_INVALID, "stack shallower than the deepest BVH");
+    if (c.stack > 64) return b->fail(PG_ERR_INVALID, "stack above 64 entries");
+    if (c.trace_blocks_per_cu > 32 || c.shade_blocks_per_cu > 32) return b->fail(PG_ERR_INVALID, "blocks per CU");
+    b->cfg = c;
+    b->dirty = true;
+    return PG_OK;
+}
+
+pg_status pg_batch_clear(pg_batch* b) {
+    if (!b) return PG_ERR_INVALID;
+    return clear(b);
+}
+
+pg_status pg_batch_render(pg_batch* b, const pg_render_params* params, uint32_t n_spp) {
+    if (!b) return PG_ERR_INVALID;
+    pg_render_params prm = params ? *params : pg_render_params_default();
+    if (prm.integrator != PG_INTEGRATOR_PATH) return b->fail(PG_ERR_INVALID, "batches render unguided");
+    if (prm.max_depth > 64) return b->fail(PG_ERR_INVALID, "max_depth > 64");
+    return render(b, prm, n_spp);
+}
+
+pg_status pg_batch_film_read(pg_batch* b, uint32_t view, float* rgb_sum, uint32_t* spp) {
+    if (!b) return PG_ERR_INVALID;
+    if (view >= b->views.size()) return b->fail(PG_ERR_INVALID, "unknown view");
+    pg_status st = prepare(b);
+    if (st != PG_OK) return st;
+    const BatchView& v = b->views[view];
+    if (rgb_sum) {
+        std::vector<float4> tmp((size_t)v.width * v.height);
+        PG_BTRY(b, hipMemcpy(tmp.data(), b->film + v.pixel_offset, tmp.size() * 16, hipMemcpyDeviceToHost), "read film");
+        for (size_t i = 0; i < tmp.size(); ++i) {
+            rgb_sum[3 * i] = tmp[i].x;
+            rgb_sum[3 * i + 1] = tmp[i].y;
+            rgb_sum[3 * i + 2] = tmp[i].z;
+        }
+    }
+    if (spp) *spp = b->spp;
+    return PG_OK;
+}
+
+pg_status pg_batch_stats_read(pg_batch* b, pg_batch_stats* s) {
+    if (!b || !s) return PG_ERR_INVALID;
+    unsigned long long h[8];
+    PG_BTRY(b, hipMemcpy(h, b->d_stats, sizeof(h), hipMemcpyDeviceToHost), "read stats");
+    std::memset(s, 0, sizeof(*s));
+    s->paths = b->paths;
+    s->closest_rays = h[0];
+    s->shadow_rays = h[1];
+    s->gpu_ms = b->gpu_ms;
+    s->views = (uint32_t)b->views.size();
+    s->scenes = (uint32_t)b->scenes.size();
+    s->chunks = (uint32_t)b->chunks.size();
+    s->graph_nodes = (uint32_t)b->graph_nodes;
+    return PG_OK;
+}
+
+pg_status pg_batch_autotune(pg_batch* b, const pg_render_params* params, pg_batch_config* best_out, double* ms_out) {
+    if (!b) return PG_ERR_INVALID;
+    pg_render_params prm = params ? *params : pg_render_params_default();
+    if (prm.integrator != PG_INTEGRATOR_PATH) return b->fail(PG_ERR_INVALID, "batches render unguided");
+    uint32_t largest = 0;
+    for (const BatchView& v : b->views) largest = std::max(largest, v.width * v.height);
+    const int base_stack = stack_entries(std::max(b->max_depth, 1));
+    // candidate values per parameter (coordinate descent in this order)
+    std::vector<std::vector<uint32_t>> cand = {
+        {1u, 0u},                                                            // use_graph
+        {0u, std::max(largest, 1u << 21), std::max(largest, 1u << 20)},     // queue_paths
+        {0u, 2u, 4u, 8u},                                                    // trace blocks per CU
+        {4u, 8u, 16u},                                                       // shade blocks per CU
+        {(uint32_t)base_stack, (uint32_t)std::min(64, base_stack + 16)},    // stack
+    };
+    auto get = [](pg_batch_config& c, int k) -> uint32_t& {
+        switch (k) {
+        case 0: return c.use_graph;
+        case 1: return c.queue_paths;
+        case 2: return c.trace_blocks_per_cu;
+        case 3: return c.shade_blocks_per_cu;
+        default: return c.stack;
+        }
+    };
+    auto measure = [&](const pg_batch_config& c, double& ms) -> pg_status {
+        pg_status st = pg_batch_configure(b, &c);
+        if (st != PG_OK) return st;
+        if ((st = clear(b)) != PG_OK) return st;
+        if ((st = render(b, prm, 1)) != PG_OK) return st;  // warm-up (and graph capture)
+        double before = b->gpu_ms;
+        if ((st = render(b, prm, 2)) != PG_OK) return st;
+        ms = (b->gpu_ms - before) / 2.0;
+        return PG_OK;
+    };
+    pg_batch_config best{0, 0, 8, (uint32_t)base_stack, 1, {0, 0, 0}};
+    double best_ms = 0.0;
+    pg_status st = measure(best, best_ms);
+    if (st != PG_OK) return st;
+    for (int k = 0; k < (int)cand.size(); ++k) {
+        for (uint32_t val : cand[k]) {
+            pg_batch_config c = best;
+            if (get(c, k) == val) continue;
+            get(c, k) = val;
+            double ms = 0.0;
+            if ((st = measure(c, ms)) != PG_OK) return st;
+            if (ms < best_ms) {
+                best_ms = ms;
+                best = c;
+            }
+        }
+    }
+    if ((st = pg_batch_configure(b, &best)) != PG_OK) return st;
+    if ((st = clear(b)) != PG_OK) return st;
+    b->gpu_ms = 0.0;
+    b->paths = 0;
+    PG_BTRY(b, hipMemset(b->d_stats, 0, 64), "reset stats");
+    if (best_out) *best_out = best;
+    if (ms_out) *ms_out = best_ms;
+    return PG_OK;
+}
+
+}  // extern "C"

@@ -27,6 +27,13 @@ struct DevScene {
     CameraRec cam;
 };
 
+// One view of a batch: camera, resolution, scene, and its first global path index
+// (= first film pixel); views are ordered by pixel_offset.
+struct BatchView {
+    CameraRec cam;
+    uint32_t scene, width, height, pixel_offset;
+};
+
 // Structure-of-arrays path state for one pass (one sample per pixel).
 struct PathState {
     float4* beta;      // throughput rgb, previous BSDF pdf
@@ -45,6 +52,11 @@ struct PathState {
     uint32_t* counters; // [0],[1] ray queue sizes, [2] shadow queue size
     unsigned long long* stats;  // [0] closest rays, [1] shadow rays, [2] nodes, [3] tris, [4] sh nodes, [5] sh tris
     uint32_t capacity;
+    // batches of views over several scenes: view of each path, the scene table and
+    // the view table (nullptr for single-view rendering)
+    uint32_t* path_view;
+    const DevScene* bscenes;
+    const BatchView* bviews;
 };
 
 // Guide inputs of the shade kernel: the cache and, per queue slot, the cells of the
@@ -75,11 +87,19 @@ hipError_t launch_rng(uint32_t seed, uint32_t sample, const uint32_t* pixel, con
 
 hipError_t launch_raygen(const DevScene& s, const PathState& ps, const PassParams& p, hipStream_t stream);
 hipError_t launch_extend(const DevScene& s, const PathState& ps, int parity, int stack, bool count,
-                         uint32_t grid, hipStream_t stream);
+                         uint32_t grid, hipStream_t stream, bool multi = false);
 hipError_t launch_shade(const DevScene& s, const PathState& ps, const PassParams& p, const GuideCtx& gc, int mode,
-                        int parity, int depth, uint32_t grid, hipStream_t stream);
+                        int parity, int depth, uint32_t grid, hipStream_t stream, bool multi = false);
 hipError_t launch_shadow(const DevScene& s, const PathState& ps, int stack, bool count, uint32_t grid,
-                         hipStream_t stream);
+                         hipStream_t stream, bool multi = false);
+// Batches: ray generation for the chunk of global paths [base, base + n) (sample index
+// read from *sample), film splat of the chunk's views into film[global pixel], and
+// the per-SPP sample advance, all graph-capturable (no host values change per SPP).
+hipError_t launch_raygen_batch(const PathState& ps, uint32_t n_views, uint32_t base, uint32_t n, uint32_t seed,
+                               const uint32_t* sample, hipStream_t stream);
+hipError_t launch_resolve_batch(const PathState& ps, uint32_t n_views, uint32_t base, uint32_t n, float4* film,
+                                hipStream_t stream);
+hipError_t launch_sample_advance(uint32_t* sample, hipStream_t stream);
 hipError_t launch_resolve(const PathState& ps, const PassParams& p, float4* film, hipStream_t stream);
 hipError_t launch_clear_film(float4* film, uint32_t n, hipStream_t stream);
 hipError_t launch_copy(const float4* src, float4* dst, size_t n, int cu_count, hipStream_t stream);

@@ -79,6 +79,13 @@ __global__ void k_rng(uint32_t seed, uint32_t sample, const uint32_t* pixel, con
 }
 
 // ------------------------------------------------------------------ wavefront kernels
+// Scene of a work item: the kernel's scene, or in a batch the scene of the path's view.
+template <bool MULTI>
+__device__ __forceinline__ const DevScene& scene_of(const DevScene& s0, const PathState& ps, uint32_t path) {
+    if constexpr (MULTI) return ps.bscenes[ps.bviews[ps.path_view[path]].scene];
+    else return s0;
+}
+
 __global__ void k_raygen(DevScene s, PathState ps, PassParams p) {
     uint32_t n = p.width * p.height;
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -103,8 +110,8 @@ __global__ void k_raygen(DevScene s, PathState ps, PassParams p) {
     ps.info[i] = make_uint4(key, 2u, 0u, i);
 }
 
-template <bool COUNT, int STACK>
-__global__ __launch_bounds__(kBlock) void k_extend(DevScene s, PathState ps, int parity) {
+template <bool COUNT, int STACK, bool MULTI>
+__global__ __launch_bounds__(kBlock) void k_extend(DevScene s0, PathState ps, int parity) {
     __shared__ int lds[STACK * kBlock];
     uint32_t n = ps.counters[parity];
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -120,6 +127,7 @@ __global__ __launch_bounds__(kBlock) void k_extend(DevScene s, PathState ps, int
         F3 o{a.x, a.y, a.z}, d{b.x, b.y, b.z};
         uint32_t hid = kMiss;
         float ht = kFltMax, hu = 0.f, hv = 0.f;
+        const DevScene& s = scene_of<MULTI>(s0, ps, MULTI ? ps.queue[parity][i] : 0u);
         traverse_closest<COUNT>(s, o, d, hid, ht, hu, hv, lds + threadIdx.x, cn, ct);
         ps.hit[i] = make_float4(__uint_as_float(hid), ht, hu, hv);
     }
@@ -129,8 +137,8 @@ __global__ __launch_bounds__(kBlock) void k_extend(DevScene s, PathState ps, int
     }
 }
 
-template <bool COUNT, int STACK>
-__global__ __launch_bounds__(kBlock) void k_shadow(DevScene s, PathState ps) {
+template <bool COUNT, int STACK, bool MULTI>
+__global__ __launch_bounds__(kBlock) void k_shadow(DevScene s0, PathState ps) {
     __shared__ int lds[STACK * kBlock];
     uint32_t n = ps.counters[2];
     if (blockIdx.x == 0 && threadIdx.x == 0) ps.stats[1] += n;
@@ -138,8 +146,9 @@ __global__ __launch_bounds__(kBlock) void k_shadow(DevScene s, PathState ps) {
     for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
         float4 a = ps.sh_o[i], b = ps.sh_d[i];
         F3 o{a.x, a.y, a.z}, d{b.x, b.y, b.z};
+        const uint32_t path = __float_as_uint(b.w);
+        const DevScene& s = scene_of<MULTI>(s0, ps, path);
         if (traverse_visible<COUNT>(s, o, d, a.w, lds + threadIdx.x, cn, ct)) {
-            uint32_t path = __float_as_uint(b.w);
             float4 c = ps.sh_c[i];
             float4 L = ps.radiance[path];
             ps.radiance[path] = make_float4(L.x + c.x, L.y + c.y, L.z + c.z, L.w);
@@ -269,8 +278,8 @@ __device__ __forceinline__ bool next_event(const DevScene& s, const MaterialRec&
 // and the quadtree guide (probability guide_prob); MODE 2: between the BSDF and the
 // learned vMF mixture (probability c, learned per point). The combined density is
 // used in every MIS weight so that the estimator stays unbiased.
-template <int MODE>
-__global__ __launch_bounds__(kBlock) void k_shade(DevScene s, PathState ps, PassParams p, GuideCtx gc, int parity,
+template <int MODE, bool MULTI>
+__global__ __launch_bounds__(kBlock) void k_shade(DevScene s0, PathState ps, PassParams p, GuideCtx gc, int parity,
                                                   int depth) {
     uint32_t n = ps.counters[parity];
     const uint32_t* qin = ps.queue[parity];
@@ -285,6 +294,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene s, PathState ps, Pass
         uint32_t path = 0;
         if (active) {
             path = qin[i];
+            const DevScene& s = scene_of<MULTI>(s0, ps, path);
             float4 h = ps.hit[i];
             uint32_t hid = __float_as_uint(h.x);
             float ht = h.y, hu = h.z, hv = h.w;
@@ -781,28 +791,129 @@ hipError_t launch_raygen(const DevScene& s, const PathState& ps, const PassParam
     return hipGetLastError();
 }
 hipError_t launch_extend(const DevScene& s, const PathState& ps, int parity, int stack, bool count, uint32_t grid,
-                         hipStream_t stream) {
-#define PG_CALL(S)                                                                                        \
-    if (count) hipLaunchKernelGGL((k_extend<true, S>), dim3(grid), dim3(kBlock), 0, stream, s, ps, parity); \
-    else hipLaunchKernelGGL((k_extend<false, S>), dim3(grid), dim3(kBlock), 0, stream, s, ps, parity)
+                         hipStream_t stream, bool multi) {
+#define PG_CALL(S)                                                                                               \
+    if (multi) hipLaunchKernelGGL((k_extend<false, S, true>), dim3(grid), dim3(kBlock), 0, stream, s, ps, parity); \
+    else if (count) hipLaunchKernelGGL((k_extend<true, S, false>), dim3(grid), dim3(kBlock), 0, stream, s, ps, parity); \
+    else hipLaunchKernelGGL((k_extend<false, S, false>), dim3(grid), dim3(kBlock), 0, stream, s, ps, parity)
     PG_STACK_DISPATCH(stack, PG_CALL);
 #undef PG_CALL
     return hipGetLastError();
 }
 hipError_t launch_shade(const DevScene& s, const PathState& ps, const PassParams& p, const GuideCtx& gc, int mode,
-                        int parity, int depth, uint32_t grid, hipStream_t stream) {
-    if (mode == 1) hipLaunchKernelGGL((k_shade<1>), dim3(grid), dim3(kBlock), 0, stream, s, ps, p, gc, parity, depth);
-    else if (mode == 2) hipLaunchKernelGGL((k_shade<2>), dim3(grid), dim3(kBlock), 0, stream, s, ps, p, gc, parity, depth);
-    else hipLaunchKernelGGL((k_shade<0>), dim3(grid), dim3(kBlock), 0, stream, s, ps, p, gc, parity, depth);
+                        int parity, int depth, uint32_t grid, hipStream_t stream, bool multi) {
+    if (multi) {
+        if (mode != 0) return hipErrorInvalidValue;  // batches render unguided
+        hipLaunchKernelGGL((k_shade<0, true>), dim3(grid), dim3(kBlock), 0, stream, s, ps, p, gc, parity, depth);
+    } else if (mode == 1) {
+        hipLaunchKernelGGL((k_shade<1, false>), dim3(grid), dim3(kBlock), 0, stream, s, ps, p, gc, parity, depth);
+    } else if (mode == 2) {
+        hipLaunchKernelGGL((k_shade<2, false>), dim3(grid), dim3(kBlock), 0, stream, s, ps, p, gc, parity, depth);
+    } else {
+        hipLaunchKernelGGL((k_shade<0, false>), dim3(grid), dim3(kBlock), 0, stream, s, ps, p, gc, parity, depth);
+    }
     return hipGetLastError();
 }
 hipError_t launch_shadow(const DevScene& s, const PathState& ps, int stack, bool count, uint32_t grid,
-                         hipStream_t stream) {
-#define PG_CALL(S)                                                                                  \
-    if (count) hipLaunchKernelGGL((k_shadow<true, S>), dim3(grid), dim3(kBlock), 0, stream, s, ps); \
-    else hipLaunchKernelGGL((k_shadow<false, S>), dim3(grid), dim3(kBlock), 0, stream, s, ps)
+                         hipStream_t stream, bool multi) {
+#define PG_CALL(S)                                                                                         \
+    if (multi) hipLaunchKernelGGL((k_shadow<false, S, true>), dim3(grid), dim3(kBlock), 0, stream, s, ps);  \
+    else if (count) hipLaunchKernelGGL((k_shadow<true, S, false>), dim3(grid), dim3(kBlock), 0, stream, s, ps); \
+    else hipLaunchKernelGGL((k_shadow<false, S, false>), dim3(grid), dim3(kBlock), 0, stream, s, ps)
     PG_STACK_DISPATCH(stack, PG_CALL);
 #undef PG_CALL
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ batches of views
+__device__ __forceinline__ uint32_t view_of(const BatchView* views, uint32_t n_views, uint32_t g) {
+    uint32_t lo = 0, hi = n_views - 1;  // last view with pixel_offset <= g
+    while (lo < hi) {
+        uint32_t mid = (lo + hi + 1) >> 1;
+        if (views[mid].pixel_offset <= g) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// k_raygen for the chunk [base, base + n) of the batch's global paths; the RNG key of
+// a path is (seed, sample, pixel within its view), so each view's samples are the
+// ones a single-view render of it draws.
+__global__ void k_raygen_batch(PathState ps, uint32_t n_views, uint32_t base, uint32_t n, uint32_t seed,
+                               const uint32_t* sample) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) {
+        ps.counters[0] = n;
+        ps.counters[1] = 0;
+        ps.counters[2] = 0;
+    }
+    if (i >= n) return;
+    const uint32_t g = base + i;
+    const uint32_t v = view_of(ps.bviews, n_views, g);
+    const BatchView& bv = ps.bviews[v];
+    const uint32_t pix = g - bv.pixel_offset;
+    uint32_t x = pix % bv.width, y = pix / bv.width;
+    uint32_t key = path_key(seed, *sample, pix);
+    float px = (float)x + keyed_rand(key, 0);
+    float py = (float)y + keyed_rand(key, 1);
+    F3 o, d;
+    generate_ray(bv.cam, px, py, o, d);
+    ps.ray_o[0][i] = make_float4(o.x, o.y, o.z, 0.f);
+    ps.ray_d[0][i] = make_float4(d.x, d.y, d.z, 0.f);
+    ps.queue[0][i] = i;
+    ps.beta[i] = make_float4(1.f, 1.f, 1.f, 0.f);
+    ps.radiance[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    ps.film_pos[i] = make_float2(px, py);
+    ps.info[i] = make_uint4(key, 2u, 0u, pix);
+    ps.path_view[i] = v;
+}
+
+// k_resolve for the chunk's views (chunks hold whole views, so every source path of a
+// pixel lies in the chunk); film is indexed by global pixel.
+__global__ void k_resolve_batch(PathState ps, uint32_t n_views, uint32_t base, uint32_t n, float4* film) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t g = base + i;
+    const BatchView& bv = ps.bviews[view_of(ps.bviews, n_views, g)];
+    const uint32_t pix = g - bv.pixel_offset;
+    const uint32_t first = bv.pixel_offset - base;  // chunk-local index of the view's pixel 0
+    int X = (int)(pix % bv.width), Y = (int)(pix / bv.width);
+    float4 acc = film[g];
+    for (int dy = -1; dy <= 0; ++dy)
+        for (int dx = -1; dx <= 0; ++dx) {
+            int sx = X + dx, sy = Y + dy;
+            if (sx < 0 || sy < 0) continue;
+            uint32_t q = first + (uint32_t)sy * bv.width + (uint32_t)sx;
+            float2 fp = ps.film_pos[q];
+            int tx = (int)floorf(fp.x + 0.5f), ty = (int)floorf(fp.y + 0.5f);
+            if (tx != X || ty != Y) continue;
+            if (!(fabsf((float)tx - fp.x) < 0.5f && fabsf((float)ty - fp.y) < 0.5f)) continue;
+            float4 L = ps.radiance[q];
+            C3 c = c3(L.x, L.y, L.z);
+            if (!finite3(c)) continue;
+            C3 add = (c * 1.f) / 1.f;
+            acc = make_float4(acc.x + add.r, acc.y + add.g, acc.z + add.b, acc.w);
+        }
+    film[g] = acc;
+}
+
+__global__ void k_sample_advance(uint32_t* sample) {
+    if (threadIdx.x == 0) sample[0] += 1u;
+}
+
+hipError_t launch_raygen_batch(const PathState& ps, uint32_t n_views, uint32_t base, uint32_t n, uint32_t seed,
+                               const uint32_t* sample, hipStream_t stream) {
+    hipLaunchKernelGGL(k_raygen_batch, dim3((n + 255) / 256), dim3(256), 0, stream, ps, n_views, base, n, seed,
+                       sample);
+    return hipGetLastError();
+}
+hipError_t launch_resolve_batch(const PathState& ps, uint32_t n_views, uint32_t base, uint32_t n, float4* film,
+                                hipStream_t stream) {
+    hipLaunchKernelGGL(k_resolve_batch, dim3((n + 255) / 256), dim3(256), 0, stream, ps, n_views, base, n, film);
+    return hipGetLastError();
+}
+hipError_t launch_sample_advance(uint32_t* sample, hipStream_t stream) {
+    hipLaunchKernelGGL(k_sample_advance, dim3(1), dim3(64), 0, stream, sample);
     return hipGetLastError();
 }
 

@@ -348,6 +348,98 @@ class RayTracer:
         self.device._chk(self.device.lib.pg_set_counting(self.device.ctx, 1 if on else 0))
 
 
+def _camera_desc(c: dict) -> PgCameraDesc:
+    return PgCameraDesc((C.c_float * 3)(*c["from"]), (C.c_float * 3)(*c["to"]), (C.c_float * 3)(*c["up"]),
+                        c["fov"], float(c["width"]), float(c["height"]))
+
+
+class Batch:
+    """Many views of one or more scenes rendered together (pg_batch_*): one hipGraph
+    launch per SPP, view-aligned chunks, autotuned launch configuration."""
+
+    def __init__(self, device: Device):
+        self.device = device
+        self.lib = device.lib
+        self.h = C.c_void_p()
+        device._chk(self.lib.pg_batch_create(device.ctx, C.byref(self.h)))
+        self.views = []  # (scene id, width, height)
+        self._keep = []
+
+    def close(self):
+        if self.h:
+            self.lib.pg_batch_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add_scene(self, data: SceneData) -> int:
+        pos = np.ascontiguousarray(data.positions, np.float32)
+        nrm = np.ascontiguousarray(data.normals, np.float32)
+        mi = np.ascontiguousarray(data.material_index, np.uint32)
+        mats = np.ascontiguousarray(data.materials)
+        env = None if data.env is None else np.ascontiguousarray(data.env, np.float32)
+        sid = C.c_uint32()
+        self.device._chk(self.lib.pg_batch_add_scene(self.h, _fp(pos), _fp(nrm), _up(mi), pos.shape[0],
+                                                     mats.ctypes.data_as(C.POINTER(PgMaterial)), mats.shape[0],
+                                                     _fp(env) if env is not None else None,
+                                                     0 if env is None else env.shape[1],
+                                                     0 if env is None else env.shape[0], C.byref(sid)))
+        return sid.value
+
+    def add_view(self, scene_id: int, camera: dict) -> int:
+        vid = C.c_uint32()
+        cam = _camera_desc(camera)
+        self.device._chk(self.lib.pg_batch_add_view(self.h, int(scene_id), C.byref(cam), C.byref(vid)))
+        self.views.append((scene_id, int(camera["width"]), int(camera["height"])))
+        return vid.value
+
+    def configure(self, **kw):
+        cfg = _lib.PgBatchConfig(0, 0, 0, 0, 1)
+        for k, v in kw.items():
+            setattr(cfg, k, v)
+        self.device._chk(self.lib.pg_batch_configure(self.h, C.byref(cfg)))
+
+    def _params(self, kw) -> PgRenderParams:
+        p = self.lib.pg_render_params_default()
+        for k, v in kw.items():
+            setattr(p, k, v)
+        return p
+
+    def autotune(self, **params):
+        p = self._params(params)
+        best = _lib.PgBatchConfig()
+        ms = C.c_double()
+        self.device._chk(self.lib.pg_batch_autotune(self.h, C.byref(p), C.byref(best), C.byref(ms)))
+        return {k: getattr(best, k) for k, _ in best._fields_ if k != "reserved"}, ms.value
+
+    def clear(self):
+        self.device._chk(self.lib.pg_batch_clear(self.h))
+
+    def render(self, n_spp: int = 1, **params):
+        p = self._params(params)
+        self.device._chk(self.lib.pg_batch_render(self.h, C.byref(p), int(n_spp)))
+
+    def film(self, view: int):
+        _, w, h = self.views[view]
+        buf = np.empty((h, w, 3), np.float32)
+        spp = C.c_uint32()
+        self.device._chk(self.lib.pg_batch_film_read(self.h, int(view), _fp(buf), C.byref(spp)))
+        return buf, spp.value
+
+    def image(self, view: int) -> np.ndarray:
+        buf, spp = self.film(view)
+        return buf * (1.0 / spp if spp > 0 else 1.0)
+
+    def stats(self) -> dict:
+        s = _lib.PgBatchStats()
+        self.device._chk(self.lib.pg_batch_stats_read(self.h, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in s._fields_}
+
+
 def metrics(img: np.ndarray, ref: np.ndarray) -> dict:
     """Metrics::calculateMSE / RMSE / RRMSE (Metrics.h:8-69, FP32, bit-exact) and
     relMSE (FP64) of an (H, W, 3) image against a reference, via pg_metrics."""
