@@ -98,7 +98,10 @@ def first_time_below(curve, e):
 
 
 def time_to_relmse(pg, dev, sc, data, max_depth, ref_spp, cap_s, log, target=0.01):
-    """Time-to-relMSE study: unguided vs quadtree guide vs learned guide."""
+    """Time-to-relMSE study: unguided vs quadtree guide vs learned guide. Besides the
+    time to relMSE <= target (null when a method does not reach it within cap_s), the
+    learned guide is compared with each baseline at that baseline's own final relMSE:
+    speedup = time the baseline needed for it / time the learned guide needed for it."""
     t0 = time.perf_counter()
     ref = reference_image(pg, sc, data, ref_spp, max_depth)
     log(f"ttr reference: {ref_spp} SPP in {time.perf_counter() - t0:.1f}s, mean {ref.mean():.5f}")
@@ -136,13 +139,25 @@ def time_to_relmse(pg, dev, sc, data, max_depth, ref_spp, cap_s, log, target=0.0
         res[name] = {"seconds": None if hit is None else round(hit[1], 3),
                      "spp": curve[-1][0], "final_relmse": round(curve[-1][2], 5), "curve": curve}
     info = dev.mlp_info()
-    common = max(r["final_relmse"] for r in res.values())
-    times = {n: first_time_below(r["curve"], common)[0] for n, r in res.items()}
+
+    def speedup(fast, slow):
+        # time `slow` needs to reach its own final relMSE over the time `fast` needs for it
+        target = res[slow]["final_relmse"]
+        t_slow = first_time_below(res[slow]["curve"], target)[0]
+        t_fast = first_time_below(res[fast]["curve"], target)[0]
+        return {"target_relmse": round(target, 5), f"{slow}_s": t_slow, f"{fast}_s": t_fast,
+                "speedup": None if not t_fast else round(t_slow / t_fast, 3)}
+
+    cmp = {"learned_vs_unguided": speedup("learned", "unguided"),
+           "learned_vs_quadtree": speedup("learned", "quadtree"),
+           "relmse_ratio_at_cap": {"unguided_over_learned": round(res["unguided"]["final_relmse"] /
+                                                                  res["learned"]["final_relmse"], 3),
+                                   "quadtree_over_learned": round(res["quadtree"]["final_relmse"] /
+                                                                  res["learned"]["final_relmse"], 3)}}
     for r in res.values():
         c = r["curve"]
         step = max(1, len(c) // 16)
         r["curve"] = [[s, t, round(e, 5)] for s, t, e in (c[::step] + ([c[-1]] if (len(c) - 1) % step else []))]
-    ratio = lambda a, b: None if (times[a] is None or times[b] is None or times[a] <= 0) else round(times[b] / times[a], 3)
     return {"target": target, "cap_s": cap_s,
             "reference": f"GPU unguided {ref_spp} SPP, FP64 host accumulation, seed 1000",
             "methods": {"unguided": "path tracer with NEE + MIS (Renderer.h:531-618)",
@@ -150,10 +165,7 @@ def time_to_relmse(pg, dev, sc, data, max_depth, ref_spp, cap_s, log, target=0.0
                         "learned": "online: 1-SPP guided record passes, 1 Adam step of 2^18 samples per pass"},
             **res,
             "learned_diverged": bool(info["diverged"]),
-            "common_target": round(common, 5),
-            "time_to_common_s": {n: (None if t is None else round(t, 3)) for n, t in times.items()},
-            "speedup_learned_vs_unguided": ratio("learned", "unguided"),
-            "speedup_learned_vs_quadtree": ratio("learned", "quadtree")}
+            "comparison": cmp}
 
 
 def main() -> int:
@@ -203,6 +215,15 @@ def main() -> int:
     tris_s = cal["shadow_tris_tested"] / max(cal["shadow_rays"], 1)
     bytes_c = algorithmic_bytes(nodes_c, tris_c, 20)
     bytes_s = algorithmic_bytes(nodes_s, tris_s, 4)
+
+    # launch shape (traversal / shade blocks per CU) picked by the library's tuner
+    t_tune = time.perf_counter()
+    tuned_ms = rt.tune()
+    tune_s = time.perf_counter() - t_tune
+    shape = {"trace_blocks_per_cu": rt.params.trace_blocks_per_cu or "lds-max",
+             "shade_blocks_per_cu": rt.params.shade_blocks_per_cu or 8, "tune_s": round(tune_s, 2),
+             "tuned_ms_per_spp": round(tuned_ms, 3)}
+    print(f"bench: launch shape {shape}", file=sys.stderr, flush=True)
 
     rt.clear()
     if a.warmup:
@@ -270,6 +291,7 @@ def main() -> int:
                                f"{W}x{H}, {a.steps} SPP, max depth {max_depth}, env map alias table",
                    "tris": data.n_tris, "width": W, "height": H, "spp": a.steps, "max_depth": max_depth},
         "samples_per_s": round(st["paths"] / wall, 1),
+        "launch_shape": shape,
         "closest_rays": st["closest_rays"],
         "shadow_rays": st["shadow_rays"],
         "kernel_ms": {"extend": round(ext_ms, 3), "shade": round(st["kernel_ms_shade"], 3),

@@ -79,7 +79,9 @@ typedef struct {
     uint32_t record_nee;    /* RECORD: 0 records Li = G (Renderer.h:390, the
                                reference); 1 records Li with NEE and MIS-weighted
                                emission (the pixelColour recursion, :391)       */
-    uint32_t reserved[8];
+    uint32_t trace_blocks_per_cu;  /* traversal grid (0: as many as LDS allows)   */
+    uint32_t shade_blocks_per_cu;  /* shade grid (0: 8)                           */
+    uint32_t reserved[6];
 } pg_render_params;
 
 typedef struct {
@@ -138,6 +140,11 @@ pg_status pg_film_clear(pg_ctx* ctx);
  * splats it with the reference's box filter (Imaging.h:186-217).               */
 pg_status pg_render_spp(pg_ctx* ctx, const pg_render_params* params, uint32_t n_spp);
 /* Film::film / Film::SPP (Imaging.h:178-272): accumulated RGB sums (W*H*3) and SPP. */
+/* Launch-shape tuner for pg_render_spp (no reference counterpart): times 2-SPP
+ * passes of `params` over candidate traversal and shade grids (coordinate descent)
+ * and returns params with the fastest shape; clears the film.                    */
+pg_status pg_render_tune(pg_ctx* ctx, const pg_render_params* params, pg_render_params* tuned,
+                         double* ms_per_spp);
 pg_status pg_film_read(pg_ctx* ctx, float* rgb_sum, uint32_t* spp);
 pg_status pg_stats_read(pg_ctx* ctx, pg_stats* stats);
 pg_status pg_stats_reset(pg_ctx* ctx);

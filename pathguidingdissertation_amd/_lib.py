@@ -38,7 +38,8 @@ class PgCameraDesc(C.Structure):
 class PgRenderParams(C.Structure):
     _fields_ = [("seed", C.c_uint32), ("max_depth", C.c_uint32), ("rr_depth", C.c_uint32),
                 ("integrator", C.c_int32), ("record", C.c_uint32), ("guide_prob", C.c_float),
-                ("knn_k", C.c_uint32), ("record_nee", C.c_uint32), ("reserved", C.c_uint32 * 8)]
+                ("knn_k", C.c_uint32), ("record_nee", C.c_uint32), ("trace_blocks_per_cu", C.c_uint32),
+                ("shade_blocks_per_cu", C.c_uint32), ("reserved", C.c_uint32 * 6)]
 
 
 class PgStats(C.Structure):
@@ -108,6 +109,8 @@ SIGNATURES = {
     "pg_film_clear": (C.c_int32, [_P]),
     "pg_render_spp": (C.c_int32, [_P, C.POINTER(PgRenderParams), C.c_uint32]),
     "pg_film_read": (C.c_int32, [_P, _F, _U32]),
+    "pg_render_tune": (C.c_int32, [_P, C.POINTER(PgRenderParams), C.POINTER(PgRenderParams),
+                                   C.POINTER(C.c_double)]),
     "pg_stats_read": (C.c_int32, [_P, C.POINTER(PgStats)]),
     "pg_stats_reset": (C.c_int32, [_P]),
     "pg_set_counting": (C.c_int32, [_P, C.c_int32]),

@@ -16,6 +16,7 @@ PKG = Path(__file__).resolve().parent
 CSRC = PKG / "csrc"
 BUILD = PKG / "_build"
 LIB = PKG / "libpgtrace.so"
+EXAMPLE = PKG / "pgd_render"  # example host program over the C++ mirror
 
 HOST_SOURCES = ["host_prep.cpp", "scenes.cpp", "metrics.cpp"]
 HIP_SOURCES = ["kernels.hip", "guiding.hip", "mlp.hip", "batch.hip", "api.hip"]
@@ -63,6 +64,11 @@ def build(verbose: bool = False) -> Path:
     tmp = LIB.with_suffix(".so.tmp")
     _run([hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", "-pthread", *map(str, objs), "-o", str(tmp)])
     os.replace(tmp, LIB)
+    # example host program over include/pgd/pgd.hpp, linked against the library
+    ex_tmp = EXAMPLE.with_suffix(".tmp")
+    _run([os.environ.get("CXX", "g++"), "-std=c++17", "-O2", "-Wall", str(PKG / "examples" / "pgd_render.cpp"),
+          "-o", str(ex_tmp), f"-L{PKG}", "-lpgtrace", f"-Wl,-rpath,$ORIGIN"])
+    os.replace(ex_tmp, EXAMPLE)
     return LIB
 
 

@@ -610,8 +610,9 @@ pg_status pg_render_spp(pg_ctx* c, const pg_render_params* params, uint32_t n_sp
         c->mix_cap = n;
     }
     const int stack = (int)c->host.bvh.max_depth;
-    const uint32_t grid_trace = trace_grid(c->cu_count, stack);
-    const uint32_t grid_shade = (uint32_t)c->cu_count * 8;
+    const uint32_t grid_trace = prm.trace_blocks_per_cu ? (uint32_t)c->cu_count * prm.trace_blocks_per_cu
+                                                        : trace_grid(c->cu_count, stack);
+    const uint32_t grid_shade = (uint32_t)c->cu_count * (prm.shade_blocks_per_cu ? prm.shade_blocks_per_cu : 8u);
     const uint32_t grid_guide = (uint32_t)c->cu_count * 5;
     const uint32_t grid_mlp = (uint32_t)c->cu_count * 2;
     GuideCtx gc{c->gv, mode == 1 ? c->guide_cells : nullptr, mode == 2 ? c->mix : nullptr};
@@ -697,6 +698,49 @@ pg_status pg_render_spp(pg_ctx* c, const pg_render_params* params, uint32_t n_sp
         float ms = 0.f;
         if (hipEventElapsedTime(&ms, l.a, l.b) == hipSuccess) c->ms[l.kind] += ms;
     }
+    return PG_OK;
+}
+
+pg_status pg_render_tune(pg_ctx* c, const pg_render_params* params, pg_render_params* tuned, double* ms_out) {
+    PG_CHECK_CTX(c);
+    if (!c->has_scene) return c->fail(PG_ERR_STATE, "no scene uploaded");
+    pg_render_params best = params ? *params : pg_render_params_default();
+    if (best.integrator == PG_INTEGRATOR_RECORD || (best.integrator == PG_INTEGRATOR_GUIDED_MLP && best.record))
+        return c->fail(PG_ERR_INVALID, "record passes are not tuned");
+    if (best.trace_blocks_per_cu > 32 || best.shade_blocks_per_cu > 32) return c->fail(PG_ERR_INVALID, "blocks per CU");
+    auto measure = [&](const pg_render_params& p, double& ms) -> pg_status {
+        pg_status st = pg_film_clear(c);
+        if (st != PG_OK) return st;
+        if ((st = pg_render_spp(c, &p, 1)) != PG_OK) return st;  // warm-up
+        double before = c->ms_total;
+        if ((st = pg_render_spp(c, &p, 2)) != PG_OK) return st;
+        ms = (c->ms_total - before) / 2.0;
+        return PG_OK;
+    };
+    double best_ms = 0.0;
+    pg_status st = measure(best, best_ms);
+    if (st != PG_OK) return st;
+    const uint32_t trace_cand[] = {0u, 4u, 6u, 8u};
+    const uint32_t shade_cand[] = {4u, 8u, 16u};
+    for (uint32_t v : trace_cand) {
+        pg_render_params p = best;
+        if (p.trace_blocks_per_cu == v) continue;
+        p.trace_blocks_per_cu = v;
+        double ms = 0.0;
+        if ((st = measure(p, ms)) != PG_OK) return st;
+        if (ms < best_ms) { best_ms = ms; best = p; }
+    }
+    for (uint32_t v : shade_cand) {
+        pg_render_params p = best;
+        if ((p.shade_blocks_per_cu ? p.shade_blocks_per_cu : 8u) == v) continue;
+        p.shade_blocks_per_cu = v;
+        double ms = 0.0;
+        if ((st = measure(p, ms)) != PG_OK) return st;
+        if (ms < best_ms) { best_ms = ms; best = p; }
+    }
+    if ((st = pg_film_clear(c)) != PG_OK) return st;
+    if (tuned) *tuned = best;
+    if (ms_out) *ms_out = best_ms;
     return PG_OK;
 }
 

@@ -1,0 +1,242 @@
+// pgd.hpp — header-only C++ mirror of the reference's hot-path classes over the C ABI
+// of include/pg_tracer.h (library libpgtrace.so).
+//
+// The reference's render loop uses Scene (Scene.h:74-176), RayTracer
+// (Renderer.h:236-751) and Metrics (Metrics.h:6-70) directly; these classes keep
+// those names and methods so that the loop can switch to the GPU path by changing
+// types. Errors from the C ABI become pgd::Error exceptions carrying pg_status.
+#pragma once
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../../include/pg_tracer.h"
+
+namespace pgd {
+
+class Error : public std::runtime_error {
+public:
+    Error(pg_status code, const std::string& what) : std::runtime_error(what), code_(code) {}
+    pg_status code() const { return code_; }
+
+private:
+    pg_status code_;
+};
+
+inline void check(pg_status s, const pg_ctx* ctx = nullptr) {
+    if (s != PG_OK) {
+        const char* m = pg_last_error(ctx);
+        throw Error(s, std::string("pg error ") + std::to_string(s) + ": " + (m ? m : ""));
+    }
+}
+
+// One GPU context (RayTracer::init's device side, Renderer.h:251-263).
+class Device {
+public:
+    explicit Device(int index = 0) { check(pg_create(&ctx_, index)); }
+    ~Device() { if (ctx_) pg_destroy(ctx_); }
+    Device(const Device&) = delete;
+    Device& operator=(const Device&) = delete;
+    pg_ctx* get() const { return ctx_; }
+    std::string name() const {
+        char buf[128] = {0};
+        int32_t cu = 0;
+        double peak = 0.0;
+        check(pg_device_info(ctx_, buf, sizeof(buf), &cu, &peak), ctx_);
+        return buf;
+    }
+
+private:
+    pg_ctx* ctx_ = nullptr;
+};
+
+// Triangle soup + materials + environment + camera, as Scene::init receives them.
+struct SceneData {
+    std::vector<float> positions;     // 9 floats per triangle (v0, v1, v2)
+    std::vector<float> normals;       // 9 floats per triangle
+    std::vector<uint32_t> material_index;
+    std::vector<pg_material> materials;
+    std::vector<float> env_rgb;       // lat-long RGB32F (empty: no environment map)
+    int32_t env_width = 0, env_height = 0;
+    pg_camera_desc camera{};
+    uint32_t triangles() const { return (uint32_t)material_index.size(); }
+
+    // Procedural scenes of the benchmark configurations (pg_scene_generate).
+    static SceneData generate(int32_t config, uint32_t seed) {
+        pg_host_scene* h = nullptr;
+        check(pg_scene_generate(config, seed, &h));
+        SceneData d;
+        uint32_t nt = 0, nm = 0;
+        int32_t ew = 0, eh = 0;
+        pg_status s = pg_scene_sizes(h, &nt, &nm, &ew, &eh);
+        if (s == PG_OK) {
+            d.positions.resize((size_t)nt * 9);
+            d.normals.resize((size_t)nt * 9);
+            d.material_index.resize(nt);
+            d.materials.resize(nm);
+            d.env_rgb.resize((size_t)ew * eh * 3);
+            d.env_width = ew;
+            d.env_height = eh;
+            s = pg_scene_copy(h, d.positions.data(), d.normals.data(), d.material_index.data(), d.materials.data(),
+                              d.env_rgb.empty() ? nullptr : d.env_rgb.data(), &d.camera);
+        }
+        pg_scene_free(h);
+        check(s);
+        return d;
+    }
+};
+
+// Scene::init + Scene::build (Scene.h:84-137); traverse / visible (Scene.h:103-146).
+class Scene {
+public:
+    Scene(Device& dev, const SceneData& d) : dev_(&dev), data_(&d) {
+        check(pg_scene_upload(dev.get(), d.positions.data(), d.normals.data(), d.material_index.data(),
+                              d.triangles(), d.materials.data(), (uint32_t)d.materials.size(),
+                              d.env_rgb.empty() ? nullptr : d.env_rgb.data(), d.env_width, d.env_height,
+                              &d.camera),
+              dev.get());
+    }
+    std::vector<pg_hit> traverse(const std::vector<pg_ray>& rays) const {
+        std::vector<pg_hit> hits(rays.size());
+        check(pg_trace_closest(dev_->get(), rays.data(), (uint32_t)rays.size(), hits.data()), dev_->get());
+        return hits;
+    }
+    std::vector<uint8_t> visible(const std::vector<pg_ray>& rays, const std::vector<float>& max_t) const {
+        std::vector<uint8_t> vis(rays.size());
+        check(pg_trace_shadow(dev_->get(), rays.data(), max_t.data(), (uint32_t)rays.size(), vis.data()),
+              dev_->get());
+        return vis;
+    }
+    Device& device() const { return *dev_; }
+    const SceneData& data() const { return *data_; }
+
+private:
+    Device* dev_;
+    const SceneData* data_;
+};
+
+// RayTracer (Renderer.h:236-751): clear / render (one sample per pixel) / getSPP /
+// the film divided by SPP (Film::save, Imaging.h:263-272).
+class RayTracer {
+public:
+    explicit RayTracer(Scene& scene, pg_render_params params = pg_render_params_default())
+        : scene_(&scene), params_(params) {}
+    pg_render_params& params() { return params_; }
+    void clear() { check(pg_film_clear(ctx()), ctx()); }
+    void render(uint32_t spp = 1) { check(pg_render_spp(ctx(), &params_, spp), ctx()); }
+    int getSPP() const {  // NOLINT(readability-identifier-naming): reference name
+        uint32_t spp = 0;
+        check(pg_film_read(ctx(), nullptr, &spp), ctx());
+        return (int)spp;
+    }
+    std::vector<float> film(uint32_t* spp = nullptr) const {
+        const pg_camera_desc& c = scene_->data().camera;
+        std::vector<float> f((size_t)c.width * (size_t)c.height * 3);
+        check(pg_film_read(ctx(), f.data(), spp), ctx());
+        return f;
+    }
+    std::vector<float> image() const {
+        uint32_t spp = 0;
+        std::vector<float> f = film(&spp);
+        if (spp)
+            for (float& v : f) v /= (float)spp;
+        return f;
+    }
+    pg_stats stats() const {
+        pg_stats s{};
+        check(pg_stats_read(ctx(), &s), ctx());
+        return s;
+    }
+
+private:
+    pg_ctx* ctx() const { return scene_->device().get(); }
+    Scene* scene_;
+    pg_render_params params_;
+};
+
+// Metrics (Metrics.h:6-70): same signatures; -1 on size mismatch as the reference.
+struct Metrics {
+    static pg_image_metrics all(const std::vector<float>& test, const std::vector<float>& truth, int width,
+                                int height) {
+        pg_image_metrics m{};
+        pg_metrics(test.data(), truth.data(), test.size(), truth.size(), width, height, &m);
+        return m;
+    }
+    float calculateMSE(std::vector<float>& t, std::vector<float>& g, int w, int h) { return all(t, g, w, h).mse; }
+    float calculateRMSE(std::vector<float>& t, std::vector<float>& g, int w, int h) { return all(t, g, w, h).rmse; }
+    float calculateRRMSE(std::vector<float>& t, std::vector<float>& g, int w, int h) { return all(t, g, w, h).rrmse; }
+};
+
+// Learned guide (no reference counterpart): init, train on recorded vertices, render
+// with online training.
+class LearnedGuide {
+public:
+    explicit LearnedGuide(Device& dev, pg_mlp_config cfg = pg_mlp_config_default()) : dev_(&dev) {
+        check(pg_mlp_init(dev.get(), &cfg), dev.get());
+    }
+    std::vector<float> train(uint32_t steps) {
+        std::vector<float> losses(steps);
+        check(pg_mlp_train(dev_->get(), steps, losses.data()), dev_->get());
+        return losses;
+    }
+    void render_online(pg_render_params params, uint32_t spp, pg_mlp_schedule schedule = {1, 1, 1, 0, 0, {0, 0, 0}}) {
+        check(pg_mlp_render_online(dev_->get(), &params, spp, &schedule), dev_->get());
+    }
+    pg_mlp_info info() const {
+        pg_mlp_info i{};
+        check(pg_mlp_info_read(dev_->get(), &i), dev_->get());
+        return i;
+    }
+
+private:
+    Device* dev_;
+};
+
+// Batches of views (no reference counterpart).
+class Batch {
+public:
+    explicit Batch(Device& dev) : dev_(&dev) { check(pg_batch_create(dev.get(), &b_), dev.get()); }
+    ~Batch() { if (b_) pg_batch_destroy(b_); }
+    Batch(const Batch&) = delete;
+    Batch& operator=(const Batch&) = delete;
+    uint32_t add_scene(const SceneData& d) {
+        uint32_t id = 0;
+        check(pg_batch_add_scene(b_, d.positions.data(), d.normals.data(), d.material_index.data(), d.triangles(),
+                                 d.materials.data(), (uint32_t)d.materials.size(),
+                                 d.env_rgb.empty() ? nullptr : d.env_rgb.data(), d.env_width, d.env_height, &id),
+              dev_->get());
+        return id;
+    }
+    uint32_t add_view(uint32_t scene, const pg_camera_desc& cam) {
+        uint32_t id = 0;
+        check(pg_batch_add_view(b_, scene, &cam, &id), dev_->get());
+        views_.push_back(cam);
+        return id;
+    }
+    std::pair<pg_batch_config, double> autotune(pg_render_params p = pg_render_params_default()) {
+        pg_batch_config c{};
+        double ms = 0.0;
+        check(pg_batch_autotune(b_, &p, &c, &ms), dev_->get());
+        return {c, ms};
+    }
+    void clear() { check(pg_batch_clear(b_), dev_->get()); }
+    void render(uint32_t spp, pg_render_params p = pg_render_params_default()) {
+        check(pg_batch_render(b_, &p, spp), dev_->get());
+    }
+    std::vector<float> film(uint32_t view, uint32_t* spp = nullptr) const {
+        const pg_camera_desc& c = views_.at(view);
+        std::vector<float> f((size_t)c.width * (size_t)c.height * 3);
+        check(pg_batch_film_read(b_, view, f.data(), spp), dev_->get());
+        return f;
+    }
+
+private:
+    Device* dev_;
+    pg_batch* b_ = nullptr;
+    std::vector<pg_camera_desc> views_;
+};
+
+}  // namespace pgd

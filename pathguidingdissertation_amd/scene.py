@@ -319,6 +319,16 @@ class RayTracer:
         self.device._chk(self.device.lib.pg_mlp_render_online(self.device.ctx, C.byref(self.params), int(n_spp),
                                                               C.byref(sch)))
 
+    def tune(self) -> float:
+        """Pick the fastest launch shape for these parameters (pg_render_tune); clears
+        the film. Returns the tuned milliseconds per SPP."""
+        tuned = PgRenderParams()
+        ms = C.c_double()
+        self.device._chk(self.device.lib.pg_render_tune(self.device.ctx, C.byref(self.params), C.byref(tuned),
+                                                        C.byref(ms)))
+        self.params = tuned
+        return ms.value
+
     def getSPP(self) -> int:  # noqa: N802 - reference name
         _, spp = self.film()
         return spp
