@@ -81,7 +81,11 @@ typedef struct {
                                emission (the pixelColour recursion, :391)       */
     uint32_t trace_blocks_per_cu;  /* traversal grid (0: as many as LDS allows)   */
     uint32_t shade_blocks_per_cu;  /* shade grid (0: 8)                           */
-    uint32_t reserved[6];
+    uint32_t sort_rays;     /* 1: trace secondary and shadow rays in (direction
+                               octant, origin Morton) order; results unchanged  */
+    uint32_t dyn_fetch;     /* 1: persistent lanes claim new rays when a wave runs
+                               below 48 busy lanes; results unchanged           */
+    uint32_t reserved[4];
 } pg_render_params;
 
 typedef struct {
@@ -141,8 +145,8 @@ pg_status pg_film_clear(pg_ctx* ctx);
 pg_status pg_render_spp(pg_ctx* ctx, const pg_render_params* params, uint32_t n_spp);
 /* Film::film / Film::SPP (Imaging.h:178-272): accumulated RGB sums (W*H*3) and SPP. */
 /* Launch-shape tuner for pg_render_spp (no reference counterpart): times 2-SPP
- * passes of `params` over candidate traversal and shade grids (coordinate descent)
- * and returns params with the fastest shape; clears the film.                    */
+ * passes of `params` over candidate traversal and shade grids and ray sorting
+ * (coordinate descent) and returns params with the fastest shape; clears the film. */
 pg_status pg_render_tune(pg_ctx* ctx, const pg_render_params* params, pg_render_params* tuned,
                          double* ms_per_spp);
 pg_status pg_film_read(pg_ctx* ctx, float* rgb_sum, uint32_t* spp);
@@ -280,7 +284,8 @@ typedef struct pg_batch_config {
     uint32_t stack;               /* traversal stack entries (0: deepest BVH, rounded
                                      up to 16; must hold the deepest BVH)             */
     uint32_t use_graph;           /* replay a captured hipGraph per SPP               */
-    uint32_t reserved[3];
+    uint32_t dyn_fetch;           /* persistent lanes with ray claims (see render params) */
+    uint32_t reserved[2];
 } pg_batch_config;
 typedef struct pg_batch_stats {
     uint64_t paths, closest_rays, shadow_rays;

@@ -39,7 +39,8 @@ class PgRenderParams(C.Structure):
     _fields_ = [("seed", C.c_uint32), ("max_depth", C.c_uint32), ("rr_depth", C.c_uint32),
                 ("integrator", C.c_int32), ("record", C.c_uint32), ("guide_prob", C.c_float),
                 ("knn_k", C.c_uint32), ("record_nee", C.c_uint32), ("trace_blocks_per_cu", C.c_uint32),
-                ("shade_blocks_per_cu", C.c_uint32), ("reserved", C.c_uint32 * 6)]
+                ("shade_blocks_per_cu", C.c_uint32), ("sort_rays", C.c_uint32),
+                ("dyn_fetch", C.c_uint32), ("reserved", C.c_uint32 * 4)]
 
 
 class PgStats(C.Structure):
@@ -76,7 +77,8 @@ class PgImageMetrics(C.Structure):
 
 class PgBatchConfig(C.Structure):
     _fields_ = [("queue_paths", C.c_uint32), ("trace_blocks_per_cu", C.c_uint32), ("shade_blocks_per_cu", C.c_uint32),
-                ("stack", C.c_uint32), ("use_graph", C.c_uint32), ("reserved", C.c_uint32 * 3)]
+                ("stack", C.c_uint32), ("use_graph", C.c_uint32), ("dyn_fetch", C.c_uint32),
+                ("reserved", C.c_uint32 * 2)]
 
 
 class PgBatchStats(C.Structure):

@@ -93,6 +93,10 @@ struct pg_ctx {
     double cache_build_ms = 0.0;
     uint32_t* guide_cells = nullptr;
     uint32_t guide_cells_cap = 0;
+    // kNN query order per bounce: Morton keys of the shading points, radix-sorted
+    uint32_t *q_keys = nullptr, *q_vals = nullptr, *q_keys_tmp = nullptr, *q_vals_tmp = nullptr;
+    uint32_t *q_hist = nullptr, *q_scan = nullptr;
+    uint32_t q_cap = 0;
     // training data beside the records (records [0, n_aux) carry it)
     TrainAux* rec_aux = nullptr;
     uint64_t n_aux = 0;
@@ -234,11 +238,11 @@ pg_status pg_create(pg_ctx** out, int device) {
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         delete c; g_last_error = "hipStreamCreate failed"; return PG_ERR_DEVICE;
     }
-    if (dalloc(&c->d_stats, 8) != hipSuccess || dalloc(&c->d_counters, 4) != hipSuccess) {
+    if (dalloc(&c->d_stats, 8) != hipSuccess || dalloc(&c->d_counters, 8) != hipSuccess) {
         pg_destroy(c); g_last_error = "device allocation failed"; return PG_ERR_NOMEM;
     }
     (void)hipMemset(c->d_stats, 0, 8 * sizeof(unsigned long long));
-    (void)hipMemset(c->d_counters, 0, 4 * sizeof(uint32_t));
+    (void)hipMemset(c->d_counters, 0, 8 * sizeof(uint32_t));
     *out = c;
     return PG_OK;
 }
@@ -256,6 +260,7 @@ pg_status pg_destroy(pg_ctx* c) {
     c->free_mlp();
     dfree(c->mix);
     dfree(c->guide_cells);
+    dfree(c->q_keys); dfree(c->q_vals); dfree(c->q_keys_tmp); dfree(c->q_vals_tmp); dfree(c->q_hist); dfree(c->q_scan);
     dfree(c->film);
     dfree(c->d_stats);
     dfree(c->d_counters);
@@ -604,6 +609,29 @@ pg_status pg_render_spp(pg_ctx* c, const pg_render_params* params, uint32_t n_sp
         PG_TRY(c, dalloc(&c->guide_cells, (size_t)n * kKnnMax), "alloc guide cells");
         c->guide_cells_cap = n;
     }
+    // sort buffers: kNN query order (quadtree guide) and ray trace order (sort_rays)
+    if ((mode == 1 || prm.sort_rays) && c->q_cap < n) {
+        dfree(c->q_keys); dfree(c->q_vals); dfree(c->q_keys_tmp); dfree(c->q_vals_tmp); dfree(c->q_hist); dfree(c->q_scan);
+        PG_TRY(c, dalloc(&c->q_keys, n), "alloc sort buffers");
+        PG_TRY(c, dalloc(&c->q_vals, n), "alloc sort buffers");
+        PG_TRY(c, dalloc(&c->q_keys_tmp, n), "alloc sort buffers");
+        PG_TRY(c, dalloc(&c->q_vals_tmp, n), "alloc sort buffers");
+        PG_TRY(c, dalloc(&c->q_hist, (size_t)256 * sort_tiles(n)), "alloc sort buffers");
+        PG_TRY(c, dalloc(&c->q_scan, scan_tiles(256 * sort_tiles(n)) + 1), "alloc sort buffers");
+        c->q_cap = n;
+    }
+    const float3 q_lo = make_float3(c->host.centre[0] - c->host.radius, c->host.centre[1] - c->host.radius,
+                                    c->host.centre[2] - c->host.radius);
+    const float q_inv = c->host.radius > 0.f ? 1.f / (2.f * c->host.radius) : 0.f;
+    // trace order of a ray queue (shadow = 1: the shadow queue) for sort_rays
+    auto ray_order = [&](int parity, int shadow) -> hipError_t {
+        hipError_t e = launch_ray_keys(c->ps, q_lo, q_inv, parity, shadow, n, c->q_keys, c->q_vals, c->stream);
+        if (e != hipSuccess) return e;
+        return radix_sort_pairs(c->q_keys, c->q_vals, c->q_keys_tmp, c->q_vals_tmp, n, 24, c->q_hist, c->q_scan,
+                                c->stream);
+    };
+    const bool sort_rays = prm.sort_rays != 0 && !record_pass;
+    const bool dyn = prm.dyn_fetch != 0 && !record_pass && !sort_rays;
     if (mode == 2 && c->mix_cap < n) {
         dfree(c->mix);
         PG_TRY(c, dalloc(&c->mix, (size_t)n * kMixF4), "alloc guide mixtures");
@@ -640,7 +668,9 @@ pg_status pg_render_spp(pg_ctx* c, const pg_render_params* params, uint32_t n_sp
         PG_TRY(c, timed(3, [&] { return launch_raygen(c->dev, c->ps, pp, c->stream); }), "raygen");
         for (uint32_t depth = 0; depth <= prm.max_depth; ++depth) {
             int parity = (int)(depth & 1u);
-            PG_TRY(c, timed(0, [&] { return launch_extend(c->dev, c->ps, parity, stack, c->counting, grid_trace, c->stream); }), "extend");
+            const bool sort_here = sort_rays && depth > 0;
+            if (sort_here) PG_TRY(c, timed(3, [&] { return ray_order(parity, 0); }), "ray order");
+            PG_TRY(c, timed(0, [&] { return launch_extend(c->dev, c->ps, parity, stack, c->counting, grid_trace, c->stream, false, sort_here ? c->q_vals : nullptr, dyn); }), "extend");
             c->n_extend++;
             if (record_pass) {
                 if (mode == 2)
@@ -653,12 +683,21 @@ pg_status pg_render_spp(pg_ctx* c, const pg_render_params* params, uint32_t n_sp
                 }
                 continue;
             }
-            if (mode == 1)
-                PG_TRY(c, timed(4, [&] { return launch_guide_cells(c->dev, c->ps, c->gv, c->guide_cells, parity, grid_guide, c->stream); }), "guide_cells");
+            if (mode == 1) {
+                PG_TRY(c, timed(4, [&] {
+                    hipError_t e = launch_query_keys(c->dev, c->ps, q_lo, q_inv, parity, n, c->q_keys, c->q_vals, c->stream);
+                    if (e == hipSuccess)
+                        e = radix_sort_pairs(c->q_keys, c->q_vals, c->q_keys_tmp, c->q_vals_tmp, n, 24, c->q_hist,
+                                             c->q_scan, c->stream);
+                    return e;
+                }), "guide query order");
+                PG_TRY(c, timed(4, [&] { return launch_guide_cells(c->dev, c->ps, c->gv, c->guide_cells, parity, grid_guide, c->stream, c->q_vals, n); }), "guide_cells");
+            }
             if (mode == 2)
                 PG_TRY(c, timed(4, [&] { return launch_mlp_infer(c->dev, c->ps, c->mlp.view, parity, c->mix, grid_mlp, c->stream); }), "mlp_infer");
             PG_TRY(c, timed(1, [&] { return launch_shade(c->dev, c->ps, pp, gc, mode, parity, (int)depth, grid_shade, c->stream); }), "shade");
-            PG_TRY(c, timed(2, [&] { return launch_shadow(c->dev, c->ps, stack, c->counting, grid_trace, c->stream); }), "shadow");
+            if (sort_rays) PG_TRY(c, timed(3, [&] { return ray_order(parity, 1); }), "shadow order");
+            PG_TRY(c, timed(2, [&] { return launch_shadow(c->dev, c->ps, stack, c->counting, grid_trace, c->stream, false, sort_rays ? c->q_vals : nullptr, dyn); }), "shadow");
             c->n_shadow++;
         }
         if (record_pass) {
@@ -734,6 +773,20 @@ pg_status pg_render_tune(pg_ctx* c, const pg_render_params* params, pg_render_pa
         pg_render_params p = best;
         if ((p.shade_blocks_per_cu ? p.shade_blocks_per_cu : 8u) == v) continue;
         p.shade_blocks_per_cu = v;
+        double ms = 0.0;
+        if ((st = measure(p, ms)) != PG_OK) return st;
+        if (ms < best_ms) { best_ms = ms; best = p; }
+    }
+    {
+        pg_render_params p = best;
+        p.dyn_fetch = best.dyn_fetch ? 0u : 1u;
+        double ms = 0.0;
+        if ((st = measure(p, ms)) != PG_OK) return st;
+        if (ms < best_ms) { best_ms = ms; best = p; }
+    }
+    {
+        pg_render_params p = best;
+        p.sort_rays = best.sort_rays ? 0u : 1u;
         double ms = 0.0;
         if ((st = measure(p, ms)) != PG_OK) return st;
         if (ms < best_ms) { best_ms = ms; best = p; }

@@ -113,7 +113,7 @@ struct pg_batch {
     float4* film = nullptr;
     uint32_t film_cap = 0;
     uint32_t spp = 0;
-    pg_batch_config cfg{0, 0, 0, 0, 1, {0, 0, 0}};
+    pg_batch_config cfg{0, 0, 0, 0, 1, 0, {0, 0}};
     std::vector<std::pair<uint32_t, uint32_t>> chunks;  // (first global path, paths)
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -243,9 +243,13 @@ hipError_t enqueue_pass(pg_batch* b, const pg_render_params& prm) {
         if ((e = launch_raygen_batch(b->ps, nv, ch.first, ch.second, prm.seed, b->d_sample, st)) != hipSuccess) return e;
         for (uint32_t depth = 0; depth <= prm.max_depth; ++depth) {
             int parity = (int)(depth & 1u);
-            if ((e = launch_extend(s0, b->ps, parity, stack, false, tgrid, st, true)) != hipSuccess) return e;
+            if ((e = launch_extend(s0, b->ps, parity, stack, false, tgrid, st, true, nullptr, b->cfg.dyn_fetch != 0)) !=
+                hipSuccess)
+                return e;
             if ((e = launch_shade(s0, b->ps, pp, gc, 0, parity, (int)depth, sgrid, st, true)) != hipSuccess) return e;
-            if ((e = launch_shadow(s0, b->ps, stack, false, tgrid, st, true)) != hipSuccess) return e;
+            if ((e = launch_shadow(s0, b->ps, stack, false, tgrid, st, true, nullptr, b->cfg.dyn_fetch != 0)) !=
+                hipSuccess)
+                return e;
         }
         if ((e = launch_resolve_batch(b->ps, nv, ch.first, ch.second, b->film, st)) != hipSuccess) return e;
     }
@@ -313,10 +317,10 @@ pg_status pg_batch_create(pg_ctx* ctx, pg_batch** out) {
     std::unique_ptr<pg_batch> b(new (std::nothrow) pg_batch());
     if (!b) return PG_ERR_NOMEM;
     b->ctx = ctx;
-    PG_BTRY(b.get(), balloc(&b->d_counters, 4), "alloc batch counters");
+    PG_BTRY(b.get(), balloc(&b->d_counters, 8), "alloc batch counters");
     PG_BTRY(b.get(), balloc(&b->d_stats, 8), "alloc batch stats");
     PG_BTRY(b.get(), balloc(&b->d_sample, 1), "alloc batch sample");
-    PG_BTRY(b.get(), hipMemset(b->d_counters, 0, 16), "init counters");
+    PG_BTRY(b.get(), hipMemset(b->d_counters, 0, 32), "init counters");
     PG_BTRY(b.get(), hipMemset(b->d_stats, 0, 64), "init stats");
     PG_BTRY(b.get(), hipMemset(b->d_sample, 0, 4), "init sample");
     PG_BTRY(b.get(), hipEventCreate(&b->e0), "event");
@@ -383,7 +387,7 @@ pg_status pg_batch_add_view(pg_batch* b, uint32_t scene_id, const pg_camera_desc
 
 pg_status pg_batch_configure(pg_batch* b, const pg_batch_config* cfg) {
     if (!b) return PG_ERR_INVALID;
-    pg_batch_config c = cfg ? *cfg : pg_batch_config{0, 0, 0, 0, 1, {0, 0, 0}};
+    pg_batch_config c = cfg ? *cfg : pg_batch_config{0, 0, 0, 0, 1, 0, {0, 0}};
     if (c.stack && (int)c.stack < b->max_depth) return b->fail(PG_ERR_INVALID, "stack shallower than the deepest BVH");
     if (c.stack > 64) return b->fail(PG_ERR_INVALID, "stack above 64 entries");
     if (c.trace_blocks_per_cu > 32 || c.shade_blocks_per_cu > 32) return b->fail(PG_ERR_INVALID, "blocks per CU");
@@ -450,6 +454,7 @@ pg_status pg_batch_autotune(pg_batch* b, const pg_render_params* params, pg_batc
     // candidate values per parameter (coordinate descent in this order)
     std::vector<std::vector<uint32_t>> cand = {
         {1u, 0u},                                                            // use_graph
+        {0u, 1u},                                                            // dyn_fetch
         {0u, std::max(largest, 1u << 21), std::max(largest, 1u << 20)},     // queue_paths
         {0u, 2u, 4u, 8u},                                                    // trace blocks per CU
         {4u, 8u, 16u},                                                       // shade blocks per CU
@@ -458,9 +463,10 @@ pg_status pg_batch_autotune(pg_batch* b, const pg_render_params* params, pg_batc
     auto get = [](pg_batch_config& c, int k) -> uint32_t& {
         switch (k) {
         case 0: return c.use_graph;
-        case 1: return c.queue_paths;
-        case 2: return c.trace_blocks_per_cu;
-        case 3: return c.shade_blocks_per_cu;
+        case 1: return c.dyn_fetch;
+        case 2: return c.queue_paths;
+        case 3: return c.trace_blocks_per_cu;
+        case 4: return c.shade_blocks_per_cu;
         default: return c.stack;
         }
     };
@@ -474,7 +480,7 @@ pg_status pg_batch_autotune(pg_batch* b, const pg_render_params* params, pg_batc
         ms = (b->gpu_ms - before) / 2.0;
         return PG_OK;
     };
-    pg_batch_config best{0, 0, 8, (uint32_t)base_stack, 1, {0, 0, 0}};
+    pg_batch_config best{0, 0, 8, (uint32_t)base_stack, 1, 0, {0, 0}};
     double best_ms = 0.0;
     pg_status st = measure(best, best_ms);
     if (st != PG_OK) return st;

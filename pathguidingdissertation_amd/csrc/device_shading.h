@@ -51,113 +51,168 @@ __device__ __forceinline__ void node_header(const BvhNode* nodes, int idx, int& 
     cnt = h[7];
 }
 
-// Closest hit. stk: this lane's LDS stack (entries at stk[k * kBlock]).
+// Resumable closest-hit traversal of one lane: ray, best hit so far, current node
+// (leftFirst, count) and stack depth. trav_begin tests the root; each trav_step is
+// one iteration of BVHNode::traverse's loop (Geometry.h:448-512): a leaf's
+// triangles then a pop, or an interior node's two children (near first, far pushed).
+struct TravState {
+    F3 o, d, inv;
+    float ht, hu, hv;
+    uint32_t hid;
+    int lf, cnt, sp;
+};
+
 template <bool COUNT>
-__device__ __forceinline__ void traverse_closest(const DevScene& s, F3 o, F3 d, uint32_t& hid, float& ht,
-                                                 float& hu, float& hv, int* stk, uint32_t& cn, uint32_t& ct) {
-    F3 inv{1.f / d.x, 1.f / d.y, 1.f / d.z};
+__device__ __forceinline__ bool trav_begin(const DevScene& s, TravState& t, uint32_t& cn) {
+    t.inv = F3{1.f / t.d.x, 1.f / t.d.y, 1.f / t.d.z};
     const float4* nf = reinterpret_cast<const float4*>(s.nodes);
     float4 r0 = nf[2], r1 = nf[3];  // root at slot 1
     float tb;
     if (COUNT) cn += 1;
-    if (!slab(r0, r1, o, inv, tb)) return;
-    if (tb > ht) return;
-    int lf = f2i(r0.w), cnt = f2i(r1.w);
-    int sp = 0;
-    while (true) {
-        if (cnt > 0) {
-            for (int k = 0; k < cnt; ++k) {
-                const TriIsect* tp = s.tris + lf + k;
-                float t, u, v;
-                if (COUNT) ct += 1;
-                if (mt(tp, o, d, t, u, v)) {
-                    if (t > kEps && t < ht) {
-                        ht = t;
-                        hid = tp->id;
-                        hu = u;
-                        hv = v;
-                    }
+    if (!slab(r0, r1, t.o, t.inv, tb)) return false;
+    if (tb > t.ht) return false;
+    t.lf = f2i(r0.w);
+    t.cnt = f2i(r1.w);
+    t.sp = 0;
+    return true;
+}
+
+// One loop iteration; false when the traversal has finished.
+template <bool COUNT>
+__device__ __forceinline__ bool trav_step(const DevScene& s, TravState& t, int* stk, uint32_t& cn, uint32_t& ct) {
+    if (t.cnt > 0) {
+        for (int k = 0; k < t.cnt; ++k) {
+            const TriIsect* tp = s.tris + t.lf + k;
+            float tt, u, v;
+            if (COUNT) ct += 1;
+            if (mt(tp, t.o, t.d, tt, u, v)) {
+                if (tt > kEps && tt < t.ht) {
+                    t.ht = tt;
+                    t.hid = tp->id;
+                    t.hu = u;
+                    t.hv = v;
                 }
             }
-            if (sp == 0) break;
-            node_header(s.nodes, stk[(--sp) * kBlock], lf, cnt);
-            continue;
         }
-        const float4* pr = nf + 2 * (lf + 1);
-        float4 a0 = pr[0], a1 = pr[1], b0 = pr[2], b1 = pr[3];
-        if (COUNT) cn += 2;
-        float dl = 0.f, dr = 0.f;
-        if (!slab(a0, a1, o, inv, dl) || dl >= ht) dl = kFltMax;
-        if (!slab(b0, b1, o, inv, dr) || dr >= ht) dr = kFltMax;
-        int nl = lf, nr = lf + 1;
-        int lfl = f2i(a0.w), cl = f2i(a1.w), lfr = f2i(b0.w), cr = f2i(b1.w);
-        if (dl > dr) {
-            float tt = dl; dl = dr; dr = tt;
-            int ti = nl; nl = nr; nr = ti;
-            ti = lfl; lfl = lfr; lfr = ti;
-            ti = cl; cl = cr; cr = ti;
-        }
-        if (dl == kFltMax) {
-            if (sp == 0) break;
-            node_header(s.nodes, stk[(--sp) * kBlock], lf, cnt);
-        } else {
-            lf = lfl;
-            cnt = cl;
-            if (dr != kFltMax) stk[(sp++) * kBlock] = nr;
-        }
+        if (t.sp == 0) return false;
+        node_header(s.nodes, stk[(--t.sp) * kBlock], t.lf, t.cnt);
+        return true;
     }
+    const float4* pr = reinterpret_cast<const float4*>(s.nodes) + 2 * (t.lf + 1);
+    float4 a0 = pr[0], a1 = pr[1], b0 = pr[2], b1 = pr[3];
+    if (COUNT) cn += 2;
+    float dl = 0.f, dr = 0.f;
+    if (!slab(a0, a1, t.o, t.inv, dl) || dl >= t.ht) dl = kFltMax;
+    if (!slab(b0, b1, t.o, t.inv, dr) || dr >= t.ht) dr = kFltMax;
+    int nl = t.lf, nr = t.lf + 1;
+    int lfl = f2i(a0.w), cl = f2i(a1.w), lfr = f2i(b0.w), cr = f2i(b1.w);
+    if (dl > dr) {
+        float tt = dl; dl = dr; dr = tt;
+        int ti = nl; nl = nr; nr = ti;
+        ti = lfl; lfl = lfr; lfr = ti;
+        ti = cl; cl = cr; cr = ti;
+    }
+    if (dl == kFltMax) {
+        if (t.sp == 0) return false;
+        node_header(s.nodes, stk[(--t.sp) * kBlock], t.lf, t.cnt);
+    } else {
+        t.lf = lfl;
+        t.cnt = cl;
+        if (dr != kFltMax) stk[(t.sp++) * kBlock] = nr;
+    }
+    return true;
+}
+
+// Closest hit. stk: this lane's LDS stack (entries at stk[k * kBlock]).
+template <bool COUNT>
+__device__ __forceinline__ void traverse_closest(const DevScene& s, F3 o, F3 d, uint32_t& hid, float& ht,
+                                                 float& hu, float& hv, int* stk, uint32_t& cn, uint32_t& ct) {
+    TravState t;
+    t.o = o;
+    t.d = d;
+    t.ht = ht;
+    t.hid = hid;
+    t.hu = hu;
+    t.hv = hv;
+    if (trav_begin<COUNT>(s, t, cn))
+        while (trav_step<COUNT>(s, t, stk, cn, ct)) {
+        }
+    hid = t.hid;
+    ht = t.ht;
+    hu = t.hu;
+    hv = t.hv;
+}
+
+// Resumable any-hit traversal (BVHNode::traverseVisible, Geometry.h:514-567) with the
+// TravState fields (ht holds maxT); vis_step returns 0 while running, 1 when the ray
+// is unoccluded, 2 when a triangle occludes it.
+template <bool COUNT>
+__device__ __forceinline__ bool vis_begin(const DevScene& s, TravState& t, uint32_t& cn) {
+    t.inv = F3{1.f / t.d.x, 1.f / t.d.y, 1.f / t.d.z};
+    const float4* nf = reinterpret_cast<const float4*>(s.nodes);
+    float4 r0 = nf[2], r1 = nf[3];
+    float tb;
+    if (COUNT) cn += 1;
+    if (!slab(r0, r1, t.o, t.inv, tb)) return false;
+    if (tb > t.ht) return false;
+    t.lf = f2i(r0.w);
+    t.cnt = f2i(r1.w);
+    t.sp = 0;
+    return true;
+}
+
+template <bool COUNT>
+__device__ __forceinline__ int vis_step(const DevScene& s, TravState& t, int* stk, uint32_t& cn, uint32_t& ct) {
+    if (t.cnt > 0) {
+        for (int k = 0; k < t.cnt; ++k) {
+            float tt, u, v;
+            if (COUNT) ct += 1;
+            if (mt(s.tris + t.lf + k, t.o, t.d, tt, u, v)) {
+                if (tt > kEps && tt < t.ht) return 2;
+            }
+        }
+        if (t.sp == 0) return 1;
+        node_header(s.nodes, stk[(--t.sp) * kBlock], t.lf, t.cnt);
+        return 0;
+    }
+    const float4* pr = reinterpret_cast<const float4*>(s.nodes) + 2 * (t.lf + 1);
+    float4 a0 = pr[0], a1 = pr[1], b0 = pr[2], b1 = pr[3];
+    if (COUNT) cn += 2;
+    float dl = 0.f, dr = 0.f;
+    if (!slab(a0, a1, t.o, t.inv, dl) || dl >= t.ht) dl = kFltMax;
+    if (!slab(b0, b1, t.o, t.inv, dr) || dr >= t.ht) dr = kFltMax;
+    int nr = t.lf + 1;
+    int lfl = f2i(a0.w), cl = f2i(a1.w);
+    if (dl > dr) {
+        float tt = dl; dl = dr; dr = tt;
+        nr = t.lf;
+        lfl = f2i(b0.w);
+        cl = f2i(b1.w);
+    }
+    if (dl == kFltMax) {
+        if (t.sp == 0) return 1;
+        node_header(s.nodes, stk[(--t.sp) * kBlock], t.lf, t.cnt);
+    } else {
+        t.lf = lfl;
+        t.cnt = cl;
+        if (dr != kFltMax) stk[(t.sp++) * kBlock] = nr;
+    }
+    return 0;
 }
 
 // Any hit in (1e-4, max_t): returns true when unoccluded.
 template <bool COUNT>
 __device__ __forceinline__ bool traverse_visible(const DevScene& s, F3 o, F3 d, float max_t, int* stk,
                                                  uint32_t& cn, uint32_t& ct) {
-    F3 inv{1.f / d.x, 1.f / d.y, 1.f / d.z};
-    const float4* nf = reinterpret_cast<const float4*>(s.nodes);
-    float4 r0 = nf[2], r1 = nf[3];
-    float tb;
-    if (COUNT) cn += 1;
-    if (!slab(r0, r1, o, inv, tb)) return true;
-    if (tb > max_t) return true;
-    int lf = f2i(r0.w), cnt = f2i(r1.w);
-    int sp = 0;
-    while (true) {
-        if (cnt > 0) {
-            for (int k = 0; k < cnt; ++k) {
-                float t, u, v;
-                if (COUNT) ct += 1;
-                if (mt(s.tris + lf + k, o, d, t, u, v)) {
-                    if (t > kEps && t < max_t) return false;
-                }
-            }
-            if (sp == 0) break;
-            node_header(s.nodes, stk[(--sp) * kBlock], lf, cnt);
-            continue;
-        }
-        const float4* pr = nf + 2 * (lf + 1);
-        float4 a0 = pr[0], a1 = pr[1], b0 = pr[2], b1 = pr[3];
-        if (COUNT) cn += 2;
-        float dl = 0.f, dr = 0.f;
-        if (!slab(a0, a1, o, inv, dl) || dl >= max_t) dl = kFltMax;
-        if (!slab(b0, b1, o, inv, dr) || dr >= max_t) dr = kFltMax;
-        int nr = lf + 1;
-        int lfl = f2i(a0.w), cl = f2i(a1.w);
-        if (dl > dr) {
-            float tt = dl; dl = dr; dr = tt;
-            nr = lf;
-            lfl = f2i(b0.w);
-            cl = f2i(b1.w);
-        }
-        if (dl == kFltMax) {
-            if (sp == 0) break;
-            node_header(s.nodes, stk[(--sp) * kBlock], lf, cnt);
-        } else {
-            lf = lfl;
-            cnt = cl;
-            if (dr != kFltMax) stk[(sp++) * kBlock] = nr;
-        }
+    TravState t;
+    t.o = o;
+    t.d = d;
+    t.ht = max_t;
+    if (!vis_begin<COUNT>(s, t, cn)) return true;
+    int r;
+    while ((r = vis_step<COUNT>(s, t, stk, cn, ct)) == 0) {
     }
-    return true;
+    return r == 1;
 }
 
 // ------------------------------------------------------------------ camera

@@ -86,12 +86,16 @@ hipError_t launch_rng(uint32_t seed, uint32_t sample, const uint32_t* pixel, con
                       float* out, hipStream_t stream);
 
 hipError_t launch_raygen(const DevScene& s, const PathState& ps, const PassParams& p, hipStream_t stream);
+// order (nullable): trace order of the queue (a sorted permutation from launch_ray_keys)
 hipError_t launch_extend(const DevScene& s, const PathState& ps, int parity, int stack, bool count,
-                         uint32_t grid, hipStream_t stream, bool multi = false);
+                         uint32_t grid, hipStream_t stream, bool multi = false, const uint32_t* order = nullptr,
+                         bool dyn = false);
 hipError_t launch_shade(const DevScene& s, const PathState& ps, const PassParams& p, const GuideCtx& gc, int mode,
                         int parity, int depth, uint32_t grid, hipStream_t stream, bool multi = false);
 hipError_t launch_shadow(const DevScene& s, const PathState& ps, int stack, bool count, uint32_t grid,
-                         hipStream_t stream, bool multi = false);
+                         hipStream_t stream, bool multi = false, const uint32_t* order = nullptr, bool dyn = false);
+hipError_t launch_ray_keys(const PathState& ps, float3 lo, float inv_ext, int parity, int shadow, uint32_t capacity,
+                           uint32_t* keys, uint32_t* vals, hipStream_t stream);
 // Batches: ray generation for the chunk of global paths [base, base + n) (sample index
 // read from *sample), film splat of the chunk's views into film[global pixel], and
 // the per-SPP sample advance, all graph-capturable (no host values change per SPP).
@@ -104,8 +108,13 @@ hipError_t launch_resolve(const PathState& ps, const PassParams& p, float4* film
 hipError_t launch_clear_film(float4* film, uint32_t n, hipStream_t stream);
 hipError_t launch_copy(const float4* src, float4* dst, size_t n, int cu_count, hipStream_t stream);
 struct RecordState;
+// kNN cells per shading point; with `order` (a sorted permutation of the queue
+// capacity from launch_query_keys + radix sort) waves visit neighbouring points.
 hipError_t launch_guide_cells(const DevScene& s, const PathState& ps, const GuideView& g, uint32_t* cells, int parity,
-                              uint32_t grid, hipStream_t stream);
+                              uint32_t grid, hipStream_t stream, const uint32_t* order = nullptr,
+                              uint32_t capacity = 0);
+hipError_t launch_query_keys(const DevScene& s, const PathState& ps, float3 lo, float inv_ext, int parity,
+                             uint32_t capacity, uint32_t* keys, uint32_t* vals, hipStream_t stream);
 // mix != nullptr: learned-guide record pass (one-sample MIS with the mixtures of k_mlp_infer)
 hipError_t launch_shade_record(const DevScene& s, const PathState& ps, const PassParams& p, const RecordState& rs,
                                const float4* mix, int parity, int depth, uint32_t grid, hipStream_t stream);

@@ -93,6 +93,7 @@ __global__ void k_raygen(DevScene s, PathState ps, PassParams p) {
         ps.counters[0] = n;
         ps.counters[1] = 0;
         ps.counters[2] = 0;
+        ps.counters[3] = 0;  // ray claims of the persistent-lane extend
     }
     if (i >= n) return;
     uint32_t x = i % p.width, y = i / p.width;
@@ -111,7 +112,8 @@ __global__ void k_raygen(DevScene s, PathState ps, PassParams p) {
 }
 
 template <bool COUNT, int STACK, bool MULTI>
-__global__ __launch_bounds__(kBlock) void k_extend(DevScene s0, PathState ps, int parity) {
+__global__ __launch_bounds__(kBlock) void k_extend(DevScene s0, PathState ps, int parity,
+                                                   const uint32_t* __restrict__ order) {
     __shared__ int lds[STACK * kBlock];
     uint32_t n = ps.counters[parity];
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -122,7 +124,8 @@ __global__ __launch_bounds__(kBlock) void k_extend(DevScene s0, PathState ps, in
     const float4* ro = ps.ray_o[parity];
     const float4* rd = ps.ray_d[parity];
     uint32_t cn = 0, ct = 0;
-    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    for (uint32_t j = blockIdx.x * kBlock + threadIdx.x; j < n; j += gridDim.x * kBlock) {
+        const uint32_t i = order ? order[j] : j;
         float4 a = ro[i], b = rd[i];
         F3 o{a.x, a.y, a.z}, d{b.x, b.y, b.z};
         uint32_t hid = kMiss;
@@ -138,12 +141,16 @@ __global__ __launch_bounds__(kBlock) void k_extend(DevScene s0, PathState ps, in
 }
 
 template <bool COUNT, int STACK, bool MULTI>
-__global__ __launch_bounds__(kBlock) void k_shadow(DevScene s0, PathState ps) {
+__global__ __launch_bounds__(kBlock) void k_shadow(DevScene s0, PathState ps, const uint32_t* __restrict__ order) {
     __shared__ int lds[STACK * kBlock];
     uint32_t n = ps.counters[2];
-    if (blockIdx.x == 0 && threadIdx.x == 0) ps.stats[1] += n;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        ps.stats[1] += n;
+        ps.counters[3] = 0;  // ray claims of the next persistent-lane extend
+    }
     uint32_t cn = 0, ct = 0;
-    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    for (uint32_t j = blockIdx.x * kBlock + threadIdx.x; j < n; j += gridDim.x * kBlock) {
+        const uint32_t i = order ? order[j] : j;
         float4 a = ps.sh_o[i], b = ps.sh_d[i];
         F3 o{a.x, a.y, a.z}, d{b.x, b.y, b.z};
         const uint32_t path = __float_as_uint(b.w);
@@ -153,6 +160,125 @@ __global__ __launch_bounds__(kBlock) void k_shadow(DevScene s0, PathState ps) {
             float4 L = ps.radiance[path];
             ps.radiance[path] = make_float4(L.x + c.x, L.y + c.y, L.z + c.z, L.w);
         }
+    }
+    if (COUNT) {
+        wave_add_u64(ps.stats + 4, cn);
+        wave_add_u64(ps.stats + 5, ct);
+    }
+}
+
+// Persistent-lane variants of extend and shadow: a lane that finishes its ray claims
+// the next one from the queue (wave-aggregated atomic on counters[3] / counters[4],
+// reset by the kernel before) whenever fewer than kRefillLanes lanes of the wave are
+// busy, so a wave is not held by its slowest ray. Per-ray arithmetic is that of
+// traverse_closest / traverse_visible; results are identical.
+constexpr int kRefillLanes = 48;
+
+template <bool COUNT, int STACK, bool MULTI>
+__global__ __launch_bounds__(kBlock) void k_extend_dyn(DevScene s0, PathState ps, int parity) {
+    __shared__ int lds[STACK * kBlock];
+    const uint32_t n = ps.counters[parity];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        ps.counters[parity ^ 1] = 0;
+        ps.counters[2] = 0;
+        ps.stats[0] += n;
+    }
+    const float4* ro = ps.ray_o[parity];
+    const float4* rd = ps.ray_d[parity];
+    int* stk = lds + threadIdx.x;
+    uint32_t cn = 0, ct = 0;
+    TravState t;
+    uint32_t slot = 0;
+    const DevScene* sc = &s0;
+    bool active = false, drained = false;
+    while (true) {
+        if (!drained) {
+            const bool want = !active;
+            const uint32_t claim = wave_append(ps.counters + 3, want);
+            if (want && claim < n) {
+                slot = claim;
+                float4 a = ro[slot], b = rd[slot];
+                t.o = F3{a.x, a.y, a.z};
+                t.d = F3{b.x, b.y, b.z};
+                t.ht = kFltMax;
+                t.hid = kMiss;
+                t.hu = 0.f;
+                t.hv = 0.f;
+                if (MULTI) sc = &scene_of<true>(s0, ps, ps.queue[parity][slot]);
+                active = trav_begin<COUNT>(*sc, t, cn);
+                if (!active) ps.hit[slot] = make_float4(__uint_as_float(kMiss), kFltMax, 0.f, 0.f);
+            }
+            drained = __ballot(want && claim >= n) != 0ull;
+        }
+        if (__ballot(active) == 0ull) {
+            if (drained) break;
+            continue;
+        }
+        const int thr = drained ? 1 : kRefillLanes;
+        do {
+            if (active && !trav_step<COUNT>(*sc, t, stk, cn, ct)) {
+                ps.hit[slot] = make_float4(__uint_as_float(t.hid), t.ht, t.hu, t.hv);
+                active = false;
+            }
+        } while (__popcll(__ballot(active)) >= thr);
+    }
+    if (COUNT) {
+        wave_add_u64(ps.stats + 2, cn);
+        wave_add_u64(ps.stats + 3, ct);
+    }
+}
+
+template <bool COUNT, int STACK, bool MULTI>
+__global__ __launch_bounds__(kBlock) void k_shadow_dyn(DevScene s0, PathState ps) {
+    __shared__ int lds[STACK * kBlock];
+    const uint32_t n = ps.counters[2];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        ps.stats[1] += n;
+        ps.counters[3] = 0;  // ray claims of the next persistent-lane extend
+    }
+    int* stk = lds + threadIdx.x;
+    uint32_t cn = 0, ct = 0;
+    TravState t;
+    uint32_t slot = 0, path = 0;
+    const DevScene* sc = &s0;
+    bool active = false, drained = false;
+    while (true) {
+        if (!drained) {
+            const bool want = !active;
+            const uint32_t claim = wave_append(ps.counters + 4, want);
+            if (want && claim < n) {
+                slot = claim;
+                float4 a = ps.sh_o[slot], b = ps.sh_d[slot];
+                t.o = F3{a.x, a.y, a.z};
+                t.d = F3{b.x, b.y, b.z};
+                t.ht = a.w;
+                path = __float_as_uint(b.w);
+                if (MULTI) sc = &scene_of<true>(s0, ps, path);
+                active = vis_begin<COUNT>(*sc, t, cn);
+                if (!active) {  // misses the root: unoccluded
+                    float4 c = ps.sh_c[slot];
+                    float4 L = ps.radiance[path];
+                    ps.radiance[path] = make_float4(L.x + c.x, L.y + c.y, L.z + c.z, L.w);
+                }
+            }
+            drained = __ballot(want && claim >= n) != 0ull;
+        }
+        if (__ballot(active) == 0ull) {
+            if (drained) break;
+            continue;
+        }
+        const int thr = drained ? 1 : kRefillLanes;
+        do {
+            if (active) {
+                int r = vis_step<COUNT>(*sc, t, stk, cn, ct);
+                if (r == 1) {
+                    float4 c = ps.sh_c[slot];
+                    float4 L = ps.radiance[path];
+                    ps.radiance[path] = make_float4(L.x + c.x, L.y + c.y, L.z + c.z, L.w);
+                }
+                if (r != 0) active = false;
+            }
+        } while (__popcll(__ballot(active)) >= thr);
     }
     if (COUNT) {
         wave_add_u64(ps.stats + 4, cn);
@@ -282,6 +408,7 @@ template <int MODE, bool MULTI>
 __global__ __launch_bounds__(kBlock) void k_shade(DevScene s0, PathState ps, PassParams p, GuideCtx gc, int parity,
                                                   int depth) {
     uint32_t n = ps.counters[parity];
+    if (blockIdx.x == 0 && threadIdx.x == 0) ps.counters[4] = 0;  // shadow-ray claims
     const uint32_t* qin = ps.queue[parity];
     uint32_t* qout = ps.queue[parity ^ 1];
     for (uint32_t base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock) {
@@ -443,14 +570,77 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene s0, PathState ps, Pas
     }
 }
 
-// Cells of the k nearest records for every shading point of this bounce.
+// Query order for the kNN of a bounce: a 24-bit Morton key of each shading point in
+// the scene cube (slots past the queue, misses, lights and specular surfaces get the
+// largest key), sorted so that the lanes of a wave search neighbouring points.
+__device__ __forceinline__ uint32_t spread8(uint32_t x) {
+    x &= 0xffu;
+    x = (x | (x << 8)) & 0x0000F00Fu;
+    x = (x | (x << 4)) & 0x000C30C3u;
+    x = (x | (x << 2)) & 0x00249249u;
+    return x;
+}
+__global__ void k_query_keys(DevScene s, PathState ps, float3 lo, float inv_ext, int parity, uint32_t capacity,
+                             uint32_t* keys, uint32_t* vals) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= capacity) return;
+    uint32_t key = 0xffffffu;
+    if (i < ps.counters[parity]) {
+        float4 h = ps.hit[i];
+        uint32_t hid = __float_as_uint(h.x);
+        if (hid != kMiss) {
+            const MaterialRec& m = s.mats[s.shade[hid].mat];
+            if (!m.specular && !m.is_light) {
+                float4 ro = ps.ray_o[parity][i], rdv = ps.ray_d[parity][i];
+                F3 x = F3{ro.x, ro.y, ro.z} + F3{rdv.x, rdv.y, rdv.z} * h.y;
+                uint32_t qx = (uint32_t)min(max((int)((x.x - lo.x) * inv_ext * 256.f), 0), 255);
+                uint32_t qy = (uint32_t)min(max((int)((x.y - lo.y) * inv_ext * 256.f), 0), 255);
+                uint32_t qz = (uint32_t)min(max((int)((x.z - lo.z) * inv_ext * 256.f), 0), 255);
+                key = (spread8(qx) << 2) | (spread8(qy) << 1) | spread8(qz);
+            }
+        }
+    }
+    keys[i] = key;
+    vals[i] = i;
+}
+
+// Trace order of a ray queue: key = direction octant (3 bits) above a 20-bit Morton
+// code of the origin in the scene cube (7/7/6 bits); slots past the queue get
+// 2^24 - 1, which sorts them last, so the first `count` entries of the sorted
+// permutation are exactly the queue. shadow = 1 keys the shadow queue.
+__global__ void k_ray_keys(PathState ps, float3 lo, float inv_ext, int parity, int shadow, uint32_t capacity,
+                           uint32_t* keys, uint32_t* vals) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= capacity) return;
+    const uint32_t n = shadow ? ps.counters[2] : ps.counters[parity];
+    uint32_t key = 0xffffffu;
+    if (i < n) {
+        float4 o = shadow ? ps.sh_o[i] : ps.ray_o[parity][i];
+        float4 d = shadow ? ps.sh_d[i] : ps.ray_d[parity][i];
+        uint32_t oct = (d.x < 0.f ? 4u : 0u) | (d.y < 0.f ? 2u : 0u) | (d.z < 0.f ? 1u : 0u);
+        uint32_t qx = (uint32_t)min(max((int)((o.x - lo.x) * inv_ext * 128.f), 0), 127);
+        uint32_t qy = (uint32_t)min(max((int)((o.y - lo.y) * inv_ext * 128.f), 0), 127);
+        uint32_t qz = (uint32_t)min(max((int)((o.z - lo.z) * inv_ext * 64.f), 0), 63);
+        uint32_t m = (spread8(qx) << 2) | (spread8(qy) << 1) | spread8(qz);
+        key = (oct << 20) | (m & 0xfffffu);
+    }
+    keys[i] = key;
+    vals[i] = i;
+}
+
+// Cells of the k nearest records for every shading point of this bounce, visited in
+// `order` (a permutation of the queue capacity; slots past the queue are skipped).
 constexpr int kGuideStack = 32;
 __global__ __launch_bounds__(kBlock) void k_guide_cells(DevScene s, PathState ps, GuideView g, uint32_t* cells,
+                                                        const uint32_t* __restrict__ order, uint32_t capacity,
                                                         int parity) {
     __shared__ int sn[kGuideStack * kBlock];
     __shared__ float sd_[kGuideStack * kBlock];
     uint32_t n = ps.counters[parity];
-    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    const uint32_t m_end = order ? capacity : n;
+    for (uint32_t j = blockIdx.x * kBlock + threadIdx.x; j < m_end; j += gridDim.x * kBlock) {
+        const uint32_t i = order ? order[j] : j;
+        if (i >= n) continue;
         float4 h = ps.hit[i];
         uint32_t hid = __float_as_uint(h.x);
         uint32_t* out = cells + (size_t)i * kKnnMax;
@@ -791,11 +981,20 @@ hipError_t launch_raygen(const DevScene& s, const PathState& ps, const PassParam
     return hipGetLastError();
 }
 hipError_t launch_extend(const DevScene& s, const PathState& ps, int parity, int stack, bool count, uint32_t grid,
-                         hipStream_t stream, bool multi) {
+                         hipStream_t stream, bool multi, const uint32_t* order, bool dyn) {
+    if (dyn) {
+#define PG_CALL(S)                                                                                                   \
+    if (multi) hipLaunchKernelGGL((k_extend_dyn<false, S, true>), dim3(grid), dim3(kBlock), 0, stream, s, ps, parity); \
+    else if (count) hipLaunchKernelGGL((k_extend_dyn<true, S, false>), dim3(grid), dim3(kBlock), 0, stream, s, ps, parity); \
+    else hipLaunchKernelGGL((k_extend_dyn<false, S, false>), dim3(grid), dim3(kBlock), 0, stream, s, ps, parity)
+        PG_STACK_DISPATCH(stack, PG_CALL);
+#undef PG_CALL
+        return hipGetLastError();
+    }
 #define PG_CALL(S)                                                                                               \
-    if (multi) hipLaunchKernelGGL((k_extend<false, S, true>), dim3(grid), dim3(kBlock), 0, stream, s, ps, parity); \
-    else if (count) hipLaunchKernelGGL((k_extend<true, S, false>), dim3(grid), dim3(kBlock), 0, stream, s, ps, parity); \
-    else hipLaunchKernelGGL((k_extend<false, S, false>), dim3(grid), dim3(kBlock), 0, stream, s, ps, parity)
+    if (multi) hipLaunchKernelGGL((k_extend<false, S, true>), dim3(grid), dim3(kBlock), 0, stream, s, ps, parity, order); \
+    else if (count) hipLaunchKernelGGL((k_extend<true, S, false>), dim3(grid), dim3(kBlock), 0, stream, s, ps, parity, order); \
+    else hipLaunchKernelGGL((k_extend<false, S, false>), dim3(grid), dim3(kBlock), 0, stream, s, ps, parity, order)
     PG_STACK_DISPATCH(stack, PG_CALL);
 #undef PG_CALL
     return hipGetLastError();
@@ -815,11 +1014,20 @@ hipError_t launch_shade(const DevScene& s, const PathState& ps, const PassParams
     return hipGetLastError();
 }
 hipError_t launch_shadow(const DevScene& s, const PathState& ps, int stack, bool count, uint32_t grid,
-                         hipStream_t stream, bool multi) {
+                         hipStream_t stream, bool multi, const uint32_t* order, bool dyn) {
+    if (dyn) {
+#define PG_CALL(S)                                                                                          \
+    if (multi) hipLaunchKernelGGL((k_shadow_dyn<false, S, true>), dim3(grid), dim3(kBlock), 0, stream, s, ps);  \
+    else if (count) hipLaunchKernelGGL((k_shadow_dyn<true, S, false>), dim3(grid), dim3(kBlock), 0, stream, s, ps); \
+    else hipLaunchKernelGGL((k_shadow_dyn<false, S, false>), dim3(grid), dim3(kBlock), 0, stream, s, ps)
+        PG_STACK_DISPATCH(stack, PG_CALL);
+#undef PG_CALL
+        return hipGetLastError();
+    }
 #define PG_CALL(S)                                                                                         \
-    if (multi) hipLaunchKernelGGL((k_shadow<false, S, true>), dim3(grid), dim3(kBlock), 0, stream, s, ps);  \
-    else if (count) hipLaunchKernelGGL((k_shadow<true, S, false>), dim3(grid), dim3(kBlock), 0, stream, s, ps); \
-    else hipLaunchKernelGGL((k_shadow<false, S, false>), dim3(grid), dim3(kBlock), 0, stream, s, ps)
+    if (multi) hipLaunchKernelGGL((k_shadow<false, S, true>), dim3(grid), dim3(kBlock), 0, stream, s, ps, order);  \
+    else if (count) hipLaunchKernelGGL((k_shadow<true, S, false>), dim3(grid), dim3(kBlock), 0, stream, s, ps, order); \
+    else hipLaunchKernelGGL((k_shadow<false, S, false>), dim3(grid), dim3(kBlock), 0, stream, s, ps, order)
     PG_STACK_DISPATCH(stack, PG_CALL);
 #undef PG_CALL
     return hipGetLastError();
@@ -846,6 +1054,7 @@ __global__ void k_raygen_batch(PathState ps, uint32_t n_views, uint32_t base, ui
         ps.counters[0] = n;
         ps.counters[1] = 0;
         ps.counters[2] = 0;
+        ps.counters[3] = 0;
     }
     if (i >= n) return;
     const uint32_t g = base + i;
@@ -918,8 +1127,20 @@ hipError_t launch_sample_advance(uint32_t* sample, hipStream_t stream) {
 }
 
 hipError_t launch_guide_cells(const DevScene& s, const PathState& ps, const GuideView& g, uint32_t* cells, int parity,
-                              uint32_t grid, hipStream_t stream) {
-    hipLaunchKernelGGL(k_guide_cells, dim3(grid), dim3(kBlock), 0, stream, s, ps, g, cells, parity);
+                              uint32_t grid, hipStream_t stream, const uint32_t* order, uint32_t capacity) {
+    hipLaunchKernelGGL(k_guide_cells, dim3(grid), dim3(kBlock), 0, stream, s, ps, g, cells, order, capacity, parity);
+    return hipGetLastError();
+}
+hipError_t launch_ray_keys(const PathState& ps, float3 lo, float inv_ext, int parity, int shadow, uint32_t capacity,
+                           uint32_t* keys, uint32_t* vals, hipStream_t stream) {
+    hipLaunchKernelGGL(k_ray_keys, dim3((capacity + 255) / 256), dim3(256), 0, stream, ps, lo, inv_ext, parity, shadow,
+                       capacity, keys, vals);
+    return hipGetLastError();
+}
+hipError_t launch_query_keys(const DevScene& s, const PathState& ps, float3 lo, float inv_ext, int parity,
+                             uint32_t capacity, uint32_t* keys, uint32_t* vals, hipStream_t stream) {
+    hipLaunchKernelGGL(k_query_keys, dim3((capacity + 255) / 256), dim3(256), 0, stream, s, ps, lo, inv_ext, parity,
+                       capacity, keys, vals);
     return hipGetLastError();
 }
 hipError_t launch_shade_record(const DevScene& s, const PathState& ps, const PassParams& p, const RecordState& rs,

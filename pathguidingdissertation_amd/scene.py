@@ -408,7 +408,7 @@ class Batch:
         return vid.value
 
     def configure(self, **kw):
-        cfg = _lib.PgBatchConfig(0, 0, 0, 0, 1)
+        cfg = _lib.PgBatchConfig(0, 0, 0, 0, 1, 0)
         for k, v in kw.items():
             setattr(cfg, k, v)
         self.device._chk(self.lib.pg_batch_configure(self.h, C.byref(cfg)))

@@ -88,6 +88,12 @@ def test_chunks_and_autotune_keep_images(pg, device, cornell, small_interior, vi
     f2 = [many.film(v)[0] for v in range(len(views))]
     for a, b in zip(f1, f2):
         assert np.array_equal(a, b)
+    # persistent lanes with ray claims
+    many.configure(use_graph=1, dyn_fetch=1)
+    many.clear()
+    many.render(2, seed=5, max_depth=12)
+    for v, a in enumerate(f1):
+        assert np.array_equal(a, many.film(v)[0])
     best, ms = many.autotune(seed=5, max_depth=12)
     assert ms > 0 and best["stack"] >= 16
     many.render(2, seed=5, max_depth=12)

@@ -116,3 +116,26 @@ def test_white_furnace(pg, device):
     outer[:, 0] = False  # row/column 0 receive half the samples (Imaging.h:199-200 rounding)
     assert abs(img[inner].mean() - 0.5) < 0.01, img[inner].mean()
     assert abs(img[outer].mean() - 1.0) < 0.01, img[outer].mean()
+
+
+@pytest.mark.gpu
+def test_ray_sorting_keeps_images(pg, device, small_interior):
+    """Tracing secondary and shadow rays in (octant, origin Morton) order, or with other
+    launch shapes, changes only the order in which rays are traced: films are identical."""
+    sc = pg.Scene(device, small_interior)
+    films, counts = [], []
+    for kw in (dict(), dict(sort_rays=1), dict(sort_rays=1, trace_blocks_per_cu=4, shade_blocks_per_cu=16),
+               dict(dyn_fetch=1), dict(dyn_fetch=1, trace_blocks_per_cu=3)):
+        rt = pg.RayTracer(sc, seed=21, max_depth=12, **kw)
+        rt.set_counting(True)
+        rt.reset_stats()
+        rt.clear()
+        rt.render(3)
+        films.append(rt.film()[0])
+        st = rt.stats()
+        counts.append(tuple(st[k] for k in ("closest_rays", "shadow_rays", "nodes_visited", "tris_tested",
+                                             "shadow_nodes_visited", "shadow_tris_tested")))
+        rt.set_counting(False)
+    for f, c in zip(films[1:], counts[1:]):
+        assert np.array_equal(films[0], f)
+        assert c == counts[0]
