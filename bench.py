@@ -180,6 +180,9 @@ def main() -> int:
     ap.add_argument("--ttr-cap", type=float, default=float(os.environ.get("PG_BENCH_TTR_CAP", "30")),
                     help="wall-clock seconds per method in the time-to-relMSE study")
     ap.add_argument("--ttr-ref-spp", type=int, default=int(os.environ.get("PG_BENCH_TTR_REF", "4096")))
+    ap.add_argument("--ttr-config", type=int, default=int(os.environ.get("PG_BENCH_TTR_CONFIG", "3")),
+                    help="scene of the time-to-relMSE study (3: the closed room lit through one window, "
+                         "where the guiding configurations are defined; 2: the open interior)")
     ap.add_argument("--copy-check", action="store_true",
                     help="also time a plain 1 GiB device copy (HBM reference point / counter scaling)")
     a = ap.parse_args()
@@ -271,8 +274,12 @@ def main() -> int:
 
     ttr = None
     if not a.no_ttr:
-        ttr = time_to_relmse(pg, dev, sc, data, max_depth, a.ttr_ref_spp, a.ttr_cap,
+        tdata = data if a.ttr_config == a.config else pg.generate_scene(a.ttr_config, 0)
+        tsc = sc if a.ttr_config == a.config else pg.Scene(dev, tdata)
+        ttr = time_to_relmse(pg, dev, tsc, tdata, max_depth, a.ttr_ref_spp, a.ttr_cap,
                              lambda m: print(m, file=sys.stderr, flush=True))
+        ttr = {"workload": f"config {a.ttr_config}: {tdata.n_tris} tris, {tdata.width}x{tdata.height}, "
+                           f"max depth {max_depth}", **ttr}
 
     line = {
         "metric": "Mrays/s; time-to-relMSE 0.01: learned vs QuadTree vs unguided, 1 MI355X",
