@@ -62,6 +62,17 @@ def reference_image(pg, sc, data, spp: int, max_depth: int) -> np.ndarray:
     return acc / done
 
 
+def trimmed_relmse(img, ref, drop=1e-4, eps=1e-2) -> float:
+    """relMSE over the pixels left after dropping the fraction `drop` with the largest
+    per-pixel error (0.01%: 92 of 921,600 pixels at 1280x720), in FP64: the part of
+    the error that is not carried by a handful of fireflies."""
+    e = ((img.astype(np.float64) - ref) ** 2 / (ref.astype(np.float64) ** 2 + eps)).reshape(-1, 3).mean(1)
+    k = int(e.size * drop)
+    if k:
+        e = np.partition(e, e.size - k)[: e.size - k]
+    return float(e.mean())
+
+
 def progressive(pg, label, ref, setup, advance, cap_s, target, log):
     """Run one method until relMSE <= target or cap_s seconds of its own wall time.
     The clock covers setup and every render/training call; relMSE checkpoints (film
@@ -79,29 +90,38 @@ def progressive(pg, label, ref, setup, advance, cap_s, target, log):
         t_total += dt
         spp += k
         e = pg.relmse(img, ref)
-        curve.append((spp, round(t_total, 3), e))
+        curve.append((spp, round(t_total, 3), e, trimmed_relmse(img, ref)))
         if e <= target:
             hit = (spp, t_total)
             break
         per_spp = dt / k
         k = k * 2 if spp < 64 else max(1, min(256, int(1.0 / max(per_spp, 1e-4))))
     log(f"ttr {label}: {len(curve)} checkpoints, {spp} SPP in {t_total:.1f}s, relMSE {curve[-1][2]:.4f}"
+        f" (trimmed {curve[-1][3]:.4f})"
         + (f", reached {target} at {hit[1]:.2f}s" if hit else ""))
     return curve, hit
 
 
-def first_time_below(curve, e):
-    for spp, t, err in curve:
-        if err <= e:
-            return t, spp
-    return None, None
+def time_staying_below(curve, e, col=2):
+    """(seconds, spp) of the first checkpoint from which every later checkpoint has
+    error <= e; (None, None) when the last one is above e. A curve that dips below e
+    and rises again (a firefly arriving later) only counts from its final crossing."""
+    hit = (None, None)
+    for c in curve:
+        if c[col] <= e:
+            if hit[0] is None:
+                hit = (c[1], c[0])
+        else:
+            hit = (None, None)
+    return hit
 
 
 def time_to_relmse(pg, dev, sc, data, max_depth, ref_spp, cap_s, log, target=0.01):
     """Time-to-relMSE study: unguided vs quadtree guide vs learned guide. Besides the
     time to relMSE <= target (null when a method does not reach it within cap_s), the
     learned guide is compared with each baseline at that baseline's own final relMSE:
-    speedup = time the baseline needed for it / time the learned guide needed for it."""
+    speedup = time the baseline needed to stay below it / time the learned guide needed
+    to stay below it, on the full relMSE and on relMSE without the worst 0.01% of pixels."""
     t0 = time.perf_counter()
     ref = reference_image(pg, sc, data, ref_spp, max_depth)
     log(f"ttr reference: {ref_spp} SPP in {time.perf_counter() - t0:.1f}s, mean {ref.mean():.5f}")
@@ -137,35 +157,85 @@ def time_to_relmse(pg, dev, sc, data, max_depth, ref_spp, cap_s, log, target=0.0
     for name, (setup, advance) in methods.items():
         curve, hit = progressive(pg, name, ref, setup, advance, cap_s, target, log)
         res[name] = {"seconds": None if hit is None else round(hit[1], 3),
-                     "spp": curve[-1][0], "final_relmse": round(curve[-1][2], 5), "curve": curve}
+                     "spp": curve[-1][0], "final_relmse": round(curve[-1][2], 5),
+                     "final_trimmed_relmse": round(curve[-1][3], 5), "curve": curve}
     info = dev.mlp_info()
 
-    def speedup(fast, slow):
-        # time `slow` needs to reach its own final relMSE over the time `fast` needs for it
-        target = res[slow]["final_relmse"]
-        t_slow = first_time_below(res[slow]["curve"], target)[0]
-        t_fast = first_time_below(res[fast]["curve"], target)[0]
+    def speedup(fast, slow, col=2):
+        # time `slow` needs to stay below its own final relMSE over the time `fast` needs
+        # for it (null: `fast` ends above it)
+        target = res[slow]["curve"][-1][col]
+        t_slow = time_staying_below(res[slow]["curve"], target, col)[0]
+        t_fast = time_staying_below(res[fast]["curve"], target, col)[0]
         return {"target_relmse": round(target, 5), f"{slow}_s": t_slow, f"{fast}_s": t_fast,
                 "speedup": None if not t_fast else round(t_slow / t_fast, 3)}
 
     cmp = {"learned_vs_unguided": speedup("learned", "unguided"),
            "learned_vs_quadtree": speedup("learned", "quadtree"),
+           "trimmed": {"learned_vs_unguided": speedup("learned", "unguided", 3),
+                       "learned_vs_quadtree": speedup("learned", "quadtree", 3)},
            "relmse_ratio_at_cap": {"unguided_over_learned": round(res["unguided"]["final_relmse"] /
                                                                   res["learned"]["final_relmse"], 3),
                                    "quadtree_over_learned": round(res["quadtree"]["final_relmse"] /
-                                                                  res["learned"]["final_relmse"], 3)}}
+                                                                  res["learned"]["final_relmse"], 3)},
+           "trimmed_relmse_ratio_at_cap": {
+               "unguided_over_learned": round(res["unguided"]["final_trimmed_relmse"] /
+                                              res["learned"]["final_trimmed_relmse"], 3),
+               "quadtree_over_learned": round(res["quadtree"]["final_trimmed_relmse"] /
+                                              res["learned"]["final_trimmed_relmse"], 3)}}
     for r in res.values():
         c = r["curve"]
         step = max(1, len(c) // 16)
-        r["curve"] = [[s, t, round(e, 5)] for s, t, e in (c[::step] + ([c[-1]] if (len(c) - 1) % step else []))]
+        r["curve"] = [[s, t, round(e, 5), round(et, 5)]
+                      for s, t, e, et in (c[::step] + ([c[-1]] if (len(c) - 1) % step else []))]
     return {"target": target, "cap_s": cap_s,
             "reference": f"GPU unguided {ref_spp} SPP, FP64 host accumulation, seed 1000",
+            "curve_columns": ["spp", "seconds", "relmse", "relmse_without_top_0.01pct_pixels"],
             "methods": {"unguided": "path tracer with NEE + MIS (Renderer.h:531-618)",
                         "quadtree": "1-SPP record pass, point-BVH + per-cell quadtrees, 50/50 one-sample MIS, kNN k=16",
                         "learned": "online: 1-SPP guided record passes, 1 Adam step of 2^18 samples per pass"},
             **res,
             "learned_diverged": bool(info["diverged"]),
             "comparison": cmp}
+
+
+def batch_study(pg, dev, n_views: int, spp: int, max_depth: int, log) -> dict:
+    """Batched views of the config-5 scene family (seed i for view i, 100k-1M triangles),
+    resolutions drawn with a seeded RNG from {128x128, 256x512, 640x360, 1280x720}: one
+    hipGraph launch per SPP after the autotuner has picked the launch configuration."""
+    rng = np.random.default_rng(12345)
+    sizes = [(128, 128), (256, 512), (640, 360), (1280, 720)]
+    b = pg.Batch(dev)
+    t0 = time.perf_counter()
+    tris = pixels = 0
+    for i in range(n_views):
+        d = pg.generate_scene(5, i)
+        tris += d.n_tris
+        sid = b.add_scene(d)
+        w, h = sizes[int(rng.integers(0, 4))]
+        cam = dict(d.camera)
+        cam.update(width=w, height=h)
+        b.add_view(sid, cam)
+        pixels += w * h
+    setup = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    best, tuned_ms = b.autotune(max_depth=max_depth)
+    tune = time.perf_counter() - t0
+    b.clear()
+    s0 = b.stats()
+    t0 = time.perf_counter()
+    b.render(spp, max_depth=max_depth)
+    wall = time.perf_counter() - t0
+    s1 = b.stats()
+    rays = (s1["closest_rays"] - s0["closest_rays"]) + (s1["shadow_rays"] - s0["shadow_rays"])
+    log(f"batch: {n_views} views, {tris} tris, {pixels} pixels, {wall * 1e3 / spp:.2f} ms/SPP, config {best}")
+    b.close()
+    return {"views": n_views, "scenes_tris": tris, "pixels": pixels, "spp": spp, "max_depth": max_depth,
+            "ms_per_spp": round(wall * 1e3 / spp, 3), "samples_per_s": round(pixels * spp / wall, 1),
+            "mrays_per_s": round(rays / wall / 1e6, 2), "gpu_ms_per_spp": round((s1["gpu_ms"] - s0["gpu_ms"]) / spp, 3),
+            "graph_nodes": s1["graph_nodes"], "chunks": s1["chunks"],
+            "tuned_config": best, "tuned_ms_per_spp": round(tuned_ms, 3), "setup_s": round(setup, 1),
+            "tune_s": round(tune, 1)}
 
 
 def main() -> int:
@@ -177,6 +247,8 @@ def main() -> int:
     ap.add_argument("--cpu-spp", type=int, default=int(os.environ.get("PG_BENCH_CPU_SPP", "40")))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-ttr", action="store_true", help="skip the time-to-relMSE study")
+    ap.add_argument("--batch-views", type=int, default=int(os.environ.get("PG_BENCH_BATCH_VIEWS", "16")),
+                    help="views of the batched-render measurement (0 skips it)")
     ap.add_argument("--ttr-cap", type=float, default=float(os.environ.get("PG_BENCH_TTR_CAP", "30")),
                     help="wall-clock seconds per method in the time-to-relMSE study")
     ap.add_argument("--ttr-ref-spp", type=int, default=int(os.environ.get("PG_BENCH_TTR_REF", "4096")))
@@ -272,6 +344,10 @@ def main() -> int:
                          f"{cst['seconds']:.1f} s); oracle restatement with std::thread tiles (Renderer.h:648-747)",
                "samples_per_s": round(cst["paths"] / cst["seconds"], 1)}
 
+    batch = None
+    if a.batch_views > 0:
+        batch = batch_study(pg, dev, a.batch_views, 8, max_depth, lambda m: print(m, file=sys.stderr, flush=True))
+
     ttr = None
     if not a.no_ttr:
         tdata = data if a.ttr_config == a.config else pg.generate_scene(a.ttr_config, 0)
@@ -314,6 +390,7 @@ def main() -> int:
                             "nodes_per_ray": round(nodes_s, 2), "tris_per_ray": round(tris_s, 2)},
         "cpu_baseline": cpu,
         "time_to_relmse": ttr,
+        "batch": batch,
         "copy_reference": copy,
         "image_finite": finite,
     }

@@ -110,7 +110,7 @@ pg_status pg_device_info(pg_ctx* ctx, char* name, size_t name_len, int32_t* cu_c
  * Lights.h:128-228; NULL => black BackgroundColour, Lights.h:75-126). The host
  * builds the binned-SAH BVH in the reference's topology (Geometry.h:293-446) and
  * uploads 32-byte nodes, leaf-ordered triangles, shading attributes, the light
- * list and the environment alias table.                                        */
+ * list and the environment alias table. At most 2^26 triangles (PG_ERR_INVALID).*/
 pg_status pg_scene_upload(pg_ctx* ctx, const float* positions, const float* normals,
                           const uint32_t* material_index, uint32_t n_tris,
                           const pg_material* materials, uint32_t n_materials,

@@ -1216,7 +1216,10 @@ pg_status pg_mlp_init(pg_ctx* c, const pg_mlp_config* cfg_in) {
     PG_TRY(c, dalloc(&m.ggrad, m.n_ggrad), "alloc hash-grid gradients");
     PG_TRY(c, hipMemset(m.ggrad, 0, m.n_ggrad * 4), "init gradients");
     m.ready = true;
-    m.window_passes = m.online_passes = m.record_passes = 0;
+    m.online_passes = m.record_passes = 0;
+    // the first online record pass starts a fresh window: records left by earlier
+    // passes (another guide's record pass, pg_records_upload) never reach training
+    m.window_passes = ~0u;
     return PG_OK;
 }
 

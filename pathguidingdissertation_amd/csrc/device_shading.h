@@ -26,10 +26,9 @@ __device__ __forceinline__ bool slab(float4 lo, float4 hi, F3 o, F3 inv, float& 
     return true;
 }
 
-// Triangle::rayIntersect (Geometry.h:101-130) on a leaf-ordered record.
-__device__ __forceinline__ bool mt(const TriIsect* tp, F3 o, F3 d, float& t, float& u, float& v) {
-    const float4* q = reinterpret_cast<const float4*>(tp);
-    float4 a = q[0], b = q[1], c = q[2];
+// Triangle::rayIntersect (Geometry.h:101-130) on a leaf-ordered record already
+// loaded as its three float4 rows (v0 + id, e0, e1).
+__device__ __forceinline__ bool mt_rows(float4 a, float4 b, float4 c, F3 o, F3 d, float& t, float& u, float& v) {
     F3 v0{a.x, a.y, a.z}, e0{b.x, b.y, b.z}, e1{c.x, c.y, c.z};
     F3 p = cross(d, e1);
     float det = dot(e0, p);
@@ -45,10 +44,34 @@ __device__ __forceinline__ bool mt(const TriIsect* tp, F3 o, F3 d, float& t, flo
     return t >= 0.f;
 }
 
+__device__ __forceinline__ bool mt(const TriIsect* tp, F3 o, F3 d, float& t, float& u, float& v) {
+    const float4* q = reinterpret_cast<const float4*>(tp);
+    return mt_rows(q[0], q[1], q[2], o, d, t, u, v);
+}
+
 __device__ __forceinline__ void node_header(const BvhNode* nodes, int idx, int& lf, int& cnt) {
     const int* h = reinterpret_cast<const int*>(nodes + idx + 1);
     lf = h[3];
     cnt = h[7];
+}
+
+// Traversal stack entry: the pushed node's (leftFirst, count) so that a pop costs no
+// memory access. Bits 31..5 hold leftFirst and bits 4..0 the count; a leaf of 31 or
+// more triangles is stored as its node index with code 31, and its pop reads the
+// header. Node and triangle indices must stay below 2^27 (checked at scene upload).
+constexpr uint32_t kStkEscape = 31u;
+__device__ __forceinline__ int stk_pack(int node, int lf, int cnt) {
+    return (int)((uint32_t)cnt < kStkEscape ? ((uint32_t)lf << 5) | (uint32_t)cnt
+                                            : ((uint32_t)node << 5) | kStkEscape);
+}
+__device__ __forceinline__ void stk_pop(const BvhNode* nodes, int e, int& lf, int& cnt) {
+    const uint32_t u = (uint32_t)e, c = u & 31u;
+    if (c == kStkEscape) {
+        node_header(nodes, (int)(u >> 5), lf, cnt);
+    } else {
+        lf = (int)(u >> 5);
+        cnt = (int)c;
+    }
 }
 
 // Resumable closest-hit traversal of one lane: ray, best hit so far, current node
@@ -81,21 +104,31 @@ __device__ __forceinline__ bool trav_begin(const DevScene& s, TravState& t, uint
 template <bool COUNT>
 __device__ __forceinline__ bool trav_step(const DevScene& s, TravState& t, int* stk, uint32_t& cn, uint32_t& ct) {
     if (t.cnt > 0) {
-        for (int k = 0; k < t.cnt; ++k) {
-            const TriIsect* tp = s.tris + t.lf + k;
+        // triangles two at a time: both records are loaded before either test, the
+        // tests run in leaf order (ties keep the first triangle, as Geometry.h:475)
+        for (int k = 0; k < t.cnt; k += 2) {
+            const float4* q0 = reinterpret_cast<const float4*>(s.tris + t.lf + k);
+            const bool two = k + 1 < t.cnt;
+            const float4* q1 = two ? q0 + 3 : q0;
+            const float4 a0 = q0[0], b0 = q0[1], c0 = q0[2];
+            const float4 a1 = q1[0], b1 = q1[1], c1 = q1[2];
             float tt, u, v;
-            if (COUNT) ct += 1;
-            if (mt(tp, t.o, t.d, tt, u, v)) {
-                if (tt > kEps && tt < t.ht) {
-                    t.ht = tt;
-                    t.hid = tp->id;
-                    t.hu = u;
-                    t.hv = v;
-                }
+            if (COUNT) ct += two ? 2 : 1;
+            if (mt_rows(a0, b0, c0, t.o, t.d, tt, u, v) && tt > kEps && tt < t.ht) {
+                t.ht = tt;
+                t.hid = __float_as_uint(a0.w);
+                t.hu = u;
+                t.hv = v;
+            }
+            if (two && mt_rows(a1, b1, c1, t.o, t.d, tt, u, v) && tt > kEps && tt < t.ht) {
+                t.ht = tt;
+                t.hid = __float_as_uint(a1.w);
+                t.hu = u;
+                t.hv = v;
             }
         }
         if (t.sp == 0) return false;
-        node_header(s.nodes, stk[(--t.sp) * kBlock], t.lf, t.cnt);
+        stk_pop(s.nodes, stk[(--t.sp) * kBlock], t.lf, t.cnt);
         return true;
     }
     const float4* pr = reinterpret_cast<const float4*>(s.nodes) + 2 * (t.lf + 1);
@@ -114,11 +147,11 @@ __device__ __forceinline__ bool trav_step(const DevScene& s, TravState& t, int* 
     }
     if (dl == kFltMax) {
         if (t.sp == 0) return false;
-        node_header(s.nodes, stk[(--t.sp) * kBlock], t.lf, t.cnt);
+        stk_pop(s.nodes, stk[(--t.sp) * kBlock], t.lf, t.cnt);
     } else {
         t.lf = lfl;
         t.cnt = cl;
-        if (dr != kFltMax) stk[(t.sp++) * kBlock] = nr;
+        if (dr != kFltMax) stk[(t.sp++) * kBlock] = stk_pack(nr, lfr, cr);
     }
     return true;
 }
@@ -164,15 +197,21 @@ __device__ __forceinline__ bool vis_begin(const DevScene& s, TravState& t, uint3
 template <bool COUNT>
 __device__ __forceinline__ int vis_step(const DevScene& s, TravState& t, int* stk, uint32_t& cn, uint32_t& ct) {
     if (t.cnt > 0) {
-        for (int k = 0; k < t.cnt; ++k) {
+        for (int k = 0; k < t.cnt; k += 2) {
+            const float4* q0 = reinterpret_cast<const float4*>(s.tris + t.lf + k);
+            const bool two = k + 1 < t.cnt;
+            const float4* q1 = two ? q0 + 3 : q0;
+            const float4 a0 = q0[0], b0 = q0[1], c0 = q0[2];
+            const float4 a1 = q1[0], b1 = q1[1], c1 = q1[2];
             float tt, u, v;
             if (COUNT) ct += 1;
-            if (mt(s.tris + t.lf + k, t.o, t.d, tt, u, v)) {
-                if (tt > kEps && tt < t.ht) return 2;
-            }
+            if (mt_rows(a0, b0, c0, t.o, t.d, tt, u, v) && tt > kEps && tt < t.ht) return 2;
+            if (!two) break;
+            if (COUNT) ct += 1;
+            if (mt_rows(a1, b1, c1, t.o, t.d, tt, u, v) && tt > kEps && tt < t.ht) return 2;
         }
         if (t.sp == 0) return 1;
-        node_header(s.nodes, stk[(--t.sp) * kBlock], t.lf, t.cnt);
+        stk_pop(s.nodes, stk[(--t.sp) * kBlock], t.lf, t.cnt);
         return 0;
     }
     const float4* pr = reinterpret_cast<const float4*>(s.nodes) + 2 * (t.lf + 1);
@@ -182,20 +221,20 @@ __device__ __forceinline__ int vis_step(const DevScene& s, TravState& t, int* st
     if (!slab(a0, a1, t.o, t.inv, dl) || dl >= t.ht) dl = kFltMax;
     if (!slab(b0, b1, t.o, t.inv, dr) || dr >= t.ht) dr = kFltMax;
     int nr = t.lf + 1;
-    int lfl = f2i(a0.w), cl = f2i(a1.w);
+    int lfl = f2i(a0.w), cl = f2i(a1.w), lfr = f2i(b0.w), cr = f2i(b1.w);
     if (dl > dr) {
         float tt = dl; dl = dr; dr = tt;
         nr = t.lf;
-        lfl = f2i(b0.w);
-        cl = f2i(b1.w);
+        int ti = lfl; lfl = lfr; lfr = ti;
+        ti = cl; cl = cr; cr = ti;
     }
     if (dl == kFltMax) {
         if (t.sp == 0) return 1;
-        node_header(s.nodes, stk[(--t.sp) * kBlock], t.lf, t.cnt);
+        stk_pop(s.nodes, stk[(--t.sp) * kBlock], t.lf, t.cnt);
     } else {
         t.lf = lfl;
         t.cnt = cl;
-        if (dr != kFltMax) stk[(t.sp++) * kBlock] = nr;
+        if (dr != kFltMax) stk[(t.sp++) * kBlock] = stk_pack(nr, lfr, cr);
     }
     return 0;
 }

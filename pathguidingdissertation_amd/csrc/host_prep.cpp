@@ -366,6 +366,8 @@ std::string prepare_scene(const float* pos, const float* nrm, const uint32_t* ma
                           const pg_material* mats, uint32_t n_mats, const float* env_rgb, int32_t env_w,
                           int32_t env_h, const pg_camera_desc& cam, PreparedScene& out) {
     if (n_tris == 0) return "scene has no triangles";
+    // traversal stack entries hold node and triangle indices in 27 bits (2N-1 nodes)
+    if (n_tris > (1u << 26)) return "scene has more than 2^26 triangles";
     if (!pos || !nrm || !mat_idx || !mats || n_mats == 0) return "null scene array";
     if (cam.width < 1.f || cam.height < 1.f) return "camera resolution must be >= 1x1";
     for (uint32_t i = 0; i < n_tris; ++i)

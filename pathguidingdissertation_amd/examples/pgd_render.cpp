@@ -1,8 +1,12 @@
 // pgd_render.cpp — example host program over the C++ mirror (pgd.hpp): the reference's
 // render loop (main.cpp:147-199) with the GPU path behind the same class names.
 //
-//   pgd_render <config> <spp> <seed> <out.bin>   render a procedural scene; writes
-//                                                 int32 width, height, spp + film sum
+//   pgd_render <config> <spp> <seed> <out.bin> [--learned]
+//                                                 render a procedural scene (unguided, or
+//                                                 with the learned guide trained online:
+//                                                 one guided record pass + one Adam step
+//                                                 per SPP); writes int32 width, height,
+//                                                 spp + film sum
 //   pgd_render --selftest                        host-only checks (no GPU needed)
 #include <cmath>
 #include <cstdio>
@@ -37,8 +41,9 @@ static int selftest() {
 
 int main(int argc, char** argv) {
     if (argc == 2 && std::strcmp(argv[1], "--selftest") == 0) return selftest();
-    if (argc != 5) {
-        std::fprintf(stderr, "usage: %s <config> <spp> <seed> <out.bin> | --selftest\n", argv[0]);
+    const bool learned = argc == 6 && std::strcmp(argv[5], "--learned") == 0;
+    if (argc != 5 && !learned) {
+        std::fprintf(stderr, "usage: %s <config> <spp> <seed> <out.bin> [--learned] | --selftest\n", argv[0]);
         return 2;
     }
     try {
@@ -50,7 +55,16 @@ int main(int argc, char** argv) {
         pgd::RayTracer rt(scene, p);
         rt.clear();
         const int spp = std::atoi(argv[2]);
-        for (int i = 0; i < spp; ++i) rt.render();  // one sample per pixel per call, as the reference loop
+        if (learned) {
+            p.record_nee = 1;
+            pgd::LearnedGuide guide(dev);
+            guide.render_online(p, (uint32_t)spp);
+            const pg_mlp_info info = guide.info();
+            std::printf("learned guide: %u Adam steps, last loss %g%s\n", info.step, info.last_loss,
+                        info.diverged ? " (diverged)" : "");
+        } else {
+            for (int i = 0; i < spp; ++i) rt.render();  // one sample per pixel per call, as the reference loop
+        }
         uint32_t n = 0;
         std::vector<float> film = rt.film(&n);
         FILE* f = std::fopen(argv[4], "wb");

@@ -38,6 +38,35 @@ def small_interior(pg):
     return pg.generate_scene(100, 0)
 
 
+def clustered_scene(pg):
+    """Cornell box plus two clusters whose triangles share one centroid each (40
+    triangles rotated about a common centre, 100 exact copies of one triangle): no
+    split separates equal centroids, so the BVH holds leaves of 40 and 100 triangles,
+    beyond the 30 that a traversal stack entry stores inline."""
+    from dataclasses import replace
+    base = pg.generate_scene(1, 0)
+    tris = []
+    c = np.array([0.2, 0.8, -0.1])
+    for k in range(40):
+        a = np.pi * k / 40
+        u = np.array([np.cos(a), np.sin(a), 0.3 * np.sin(3 * a)])
+        w = np.cross(u, [0.3, 0.4, 0.87])
+        w /= np.linalg.norm(w)
+        v = np.stack([c + 0.3 * u, c - 0.15 * u + 0.25 * w, c - 0.15 * u - 0.25 * w])
+        v -= v.mean(0) - c
+        tris.append(v)
+    one = np.array([[-0.5, 1.2, 0.3], [-0.2, 1.5, 0.1], [-0.4, 1.3, 0.6]])
+    tris += [one] * 100
+    extra = np.asarray(tris, np.float32)
+    n_extra = extra.shape[0]
+    nrm = np.cross(extra[:, 1] - extra[:, 0], extra[:, 2] - extra[:, 0])
+    nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+    nrm3 = np.repeat(nrm[:, None, :], 3, 1).astype(np.float32)
+    return replace(base, positions=np.concatenate([base.positions, extra]),
+                   normals=np.concatenate([base.normals, nrm3]),
+                   material_index=np.concatenate([base.material_index, np.zeros(n_extra, np.uint32)]))
+
+
 def seeded_rays(osc, data, n, seed):
     """Seeded parity ray set: half camera primaries with jitter, half uniform-sphere
     rays from on-surface origins offset by 1e-4 along the normal (Renderer.h:496-497),

@@ -55,6 +55,29 @@ def test_closest_hit_full_interior(pg, device):
     assert np.array_equal(gpu[1][::1024].view(np.uint32), bt.view(np.uint32))
 
 
+def test_large_leaves_bit_exact(pg, device):
+    """Leaves of 40 and 100 triangles (stored on the traversal stack by node index
+    and re-read on pop): closest hits and shadow rays against the oracle, with rays
+    aimed through both clusters from all sides."""
+    from conftest import clustered_scene
+    from oracle.pg_oracle import OracleScene
+    data = clustered_scene(pg)
+    osc = OracleScene(data)
+    rng = np.random.default_rng(11)
+    base = seeded_rays(osc, data, 1 << 15, seed=5)
+    n = 1 << 15
+    targets = np.array([[0.2, 0.8, -0.1], [-0.37, 1.33, 0.33]], np.float32)[rng.integers(0, 2, n)]
+    targets += rng.normal(0, 0.1, (n, 3)).astype(np.float32)
+    org = rng.uniform([-0.95, 0.05, -0.95], [0.95, 1.95, 0.95], (n, 3)).astype(np.float32)
+    d = targets - org
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays = np.concatenate([base, np.concatenate([org, d], 1).astype(np.float32)])
+    sc = pg.Scene(device, data)
+    _check_hits(sc.traverse(rays), osc.trace(rays))
+    max_t = rng.uniform(0.01, 3.0, rays.shape[0]).astype(np.float32)
+    assert np.array_equal(sc.visible_rays(rays, max_t), osc.shadow(rays, max_t))
+
+
 def test_shadow_rays(pg, device, small_interior):
     from oracle.pg_oracle import OracleScene
     osc = OracleScene(small_interior)

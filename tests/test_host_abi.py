@@ -64,6 +64,21 @@ def test_host_bvh_is_reference_topology(pg, cfg):
     assert np.array_equal(idx, oidx)
 
 
+def test_clustered_scene_has_large_leaves(pg):
+    """Leaves beyond the 30 triangles a traversal stack entry stores inline exist in
+    the clustered scene (its GPU parity test covers the header re-read on pop), and
+    the host build still follows the oracle's topology there."""
+    from conftest import clustered_scene
+    from oracle.pg_oracle import OracleScene
+    data = clustered_scene(pg)
+    nodes, idx = _host_bvh(pg, data)
+    onodes, oidx = OracleScene(data).bvh()
+    assert np.array_equal(nodes.view(np.uint32), onodes.view(np.uint32))
+    assert np.array_equal(idx, oidx)
+    counts = np.sort(nodes[:, 7].view(np.int32))
+    assert counts[-1] == 100 and counts[-2] == 40, counts[-4:]
+
+
 def test_host_bvh_full_interior_matches_reference_hash(pg):
     ref = json.load(open(GOLDEN / "ref_topology_config2.json"))
     data = pg.generate_scene(2, 0)
