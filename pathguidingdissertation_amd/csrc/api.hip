@@ -120,6 +120,8 @@ struct pg_ctx {
         float* partial = nullptr;
         uint32_t groups_cap = 0;
         TrainSample* batch = nullptr;
+        TrainSample* batch_raw = nullptr;  // gathered samples before the spatial sort
+        SortBuffers sort{};
         uint32_t batch_cap = 0;
         uint16_t* rows = nullptr;
         uint32_t rows_cap = 0;
@@ -180,6 +182,8 @@ struct pg_ctx {
         dfree(m.params); dfree(m.pm); dfree(m.pv); dfree(m.mirror);
         dfree(m.grid); dfree(m.gm); dfree(m.gv); dfree(m.ggrad); dfree(m.grid_bf);
         dfree(m.partial); dfree(m.batch); dfree(m.rows); dfree(m.loss); dfree(m.scal); dfree(m.status); dfree(m.sum_part);
+        dfree(m.batch_raw); dfree(m.sort.keys); dfree(m.sort.vals); dfree(m.sort.keys_tmp); dfree(m.sort.vals_tmp);
+        dfree(m.sort.hist); dfree(m.sort.scan);
         m = Mlp{};
     }
     void free_paths() {
@@ -1089,9 +1093,18 @@ static pg_status mlp_buffers(pg_ctx* c, uint32_t batch, uint32_t steps) {
     const uint32_t chunk = mlp_chunk(m.cfg, batch);
     const uint32_t groups = mlp_wgrad_groups(chunk);
     if (m.batch_cap < batch) {
-        dfree(m.batch); dfree(m.loss);
+        dfree(m.batch); dfree(m.loss); dfree(m.batch_raw);
+        dfree(m.sort.keys); dfree(m.sort.vals); dfree(m.sort.keys_tmp); dfree(m.sort.vals_tmp);
+        dfree(m.sort.hist); dfree(m.sort.scan);
         PG_TRY(c, dalloc(&m.batch, batch), "alloc training batch");
         PG_TRY(c, dalloc(&m.loss, batch), "alloc training batch");
+        PG_TRY(c, dalloc(&m.batch_raw, batch), "alloc training batch");
+        PG_TRY(c, dalloc(&m.sort.keys, batch), "alloc batch sort");
+        PG_TRY(c, dalloc(&m.sort.vals, batch), "alloc batch sort");
+        PG_TRY(c, dalloc(&m.sort.keys_tmp, batch), "alloc batch sort");
+        PG_TRY(c, dalloc(&m.sort.vals_tmp, batch), "alloc batch sort");
+        PG_TRY(c, dalloc(&m.sort.hist, (size_t)256 * sort_tiles(batch)), "alloc batch sort");
+        PG_TRY(c, dalloc(&m.sort.scan, scan_tiles(256 * sort_tiles(batch)) + 1), "alloc batch sort");
         m.batch_cap = m.loss_cap = batch;
     }
     if (m.rows_cap < chunk) {
@@ -1261,7 +1274,8 @@ pg_status pg_mlp_train(pg_ctx* c, uint32_t steps, float* losses) {
             if (p % n != 0) { stride = p % n; break; }
         }
         PG_TRY(c, launch_mlp_gather(c->records, c->rec_aux, n, offset, stride, B, m.w_scale, m.cfg.weight_clip,
-                                    m.batch, c->stream), "mlp gather");
+                                    m.batch_raw, c->stream), "mlp gather");
+        PG_TRY(c, launch_mlp_sort_batch(m.view, m.batch_raw, B, m.sort, m.batch, c->stream), "mlp batch order");
         uint32_t groups = 0;
         st = mlp_backward(c, B, m.cfg.nan_step == t ? 1 : 0, &groups);
         if (st != PG_OK) return st;

@@ -149,6 +149,14 @@ struct AdamParams {
 struct Record;
 hipError_t launch_mlp_gather(const Record* rec, const TrainAux* aux, uint32_t n_rec, uint32_t offset, uint32_t stride,
                              uint32_t batch, float w_scale, float w_clip, TrainSample* out, hipStream_t stream);
+// Key/value arrays of n entries for a radix sort, with its histogram and scan scratch
+// (sort_tiles(n) * 256 and scan_tiles(sort_tiles(n) * 256) + 1 entries).
+struct SortBuffers {
+    uint32_t *keys, *vals, *keys_tmp, *vals_tmp, *hist, *scan;
+};
+// Reorders n training samples by the Morton code of their positions (in -> out).
+hipError_t launch_mlp_sort_batch(const MlpView& v, const TrainSample* in, uint32_t n, const SortBuffers& sb,
+                                 TrainSample* out, hipStream_t stream);
 hipError_t launch_mlp_train_chunk(const MlpView& v, const TrainSample* smp, uint32_t n, float inv_batch,
                                   uint16_t* rows, float* loss, float* grid_grad, int nan_inject, int cu_count,
                                   hipStream_t stream);

@@ -18,6 +18,7 @@
 
 #include "device_math.h"
 #include "guide.h"
+#include "guiding.h"
 #include "kernels.h"
 #include "mlp_device.h"
 
@@ -198,6 +199,45 @@ __device__ __forceinline__ float head_loss_grad(float* o, F3 wi, float pb, float
     return -wgt * logP * inv_b;
 }
 
+// DPP moves inside the 16-lane rows of a wave: lane c of a row reads lane c - D
+// (row_shr) or c + D (row_shl); a lane whose source falls outside its row keeps `fill`.
+template <int D>
+__device__ __forceinline__ uint32_t row_shr(uint32_t v, uint32_t fill) {
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)fill, (int)v, 0x110 + D, 0xf, 0xf, false);
+}
+template <int D>
+__device__ __forceinline__ uint32_t row_shl(uint32_t v, uint32_t fill) {
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)fill, (int)v, 0x100 + D, 0xf, 0xf, false);
+}
+template <int D>
+__device__ __forceinline__ void run_add(int c, uint32_t start, float& a, float& b) {
+    const float au = __uint_as_float(row_shr<D>(__float_as_uint(a), 0u));
+    const float bu = __uint_as_float(row_shr<D>(__float_as_uint(b), 0u));
+    if (c - D >= (int)start) {
+        a += au;
+        b += bu;
+    }
+}
+
+// Segmented sum over runs of equal keys among consecutive lanes of a 16-lane row
+// (lane c = position in the row). Every lane must execute it. Returns true on the
+// last lane of each run, which then holds the run's totals in (a, b). Keys must not
+// equal 0xFFFFFFFE.
+__device__ __forceinline__ bool row_run_sum(uint32_t key, int c, float& a, float& b) {
+    constexpr uint32_t kOutside = 0xFFFFFFFEu;
+    const bool head = row_shr<1>(key, kOutside) != key;
+    uint32_t start = head ? (uint32_t)c : 0u;
+    start = max(start, row_shr<1>(start, 0u));
+    start = max(start, row_shr<2>(start, 0u));
+    start = max(start, row_shr<4>(start, 0u));
+    start = max(start, row_shr<8>(start, 0u));
+    run_add<1>(c, start, a, b);
+    run_add<2>(c, start, a, b);
+    run_add<4>(c, start, a, b);
+    run_add<8>(c, start, a, b);
+    return row_shl<1>(key, kOutside) != key;
+}
+
 // Apply a ReLU mask to a 64-row delta accumulator, pack it as the next backward
 // layer's K slots and store it (BF16) in the activation rows.
 __device__ __forceinline__ void mask_pack(const f32x4 (&acc)[2][4], uint32_t mask, u16x8 (&out)[2][2], uint16_t* rows,
@@ -227,7 +267,7 @@ __device__ __forceinline__ void mask_pack(const f32x4 (&acc)[2][4], uint32_t mas
     }
 }
 
-__global__ __launch_bounds__(kMlpBlock) void k_mlp_train(MlpView v, const TrainSample* __restrict__ smp, uint32_t n,
+__global__ __launch_bounds__(kMlpBlock, 2) void k_mlp_train(MlpView v, const TrainSample* __restrict__ smp, uint32_t n,
                                                          float inv_b, uint16_t* rows, float* loss, float* ggrad,
                                                          int nan_inject) {
     __shared__ __align__(16) uint16_t sw[kMirror];
@@ -295,29 +335,34 @@ __global__ __launch_bounds__(kMlpBlock) void k_mlp_train(MlpView v, const TrainS
         mask_pack(acc, f.m1, d, rows, kRowD1, base, n, lane);
         f32x4 ax[2][2];
         layer_bwd<2>(sw + kMW1, 96, 64, true, d, ax, lane);
-        // hash-grid gradients: this lane holds inputs 8g + 4mb + r = levels 4g + 2mb + r/2
+        // hash-grid gradients: this lane holds inputs 8g + 4mb + r = levels 4g + 2mb + r/2.
+        // The 16 lanes of a row hold 16 consecutive samples at one level; each run of
+        // lanes that hit the same entry adds its sum with one atomic pair from its last
+        // lane (spatially sorted batches make such runs common on the coarse levels).
 #pragma unroll
         for (int nb = 0; nb < 2; ++nb) {
-            uint32_t s = base + nb * 16 + c;
-            if (s >= n) continue;
+            const bool valid = base + nb * 16 + c < n;
 #pragma unroll
             for (int mb = 0; mb < 2; ++mb)
 #pragma unroll
                 for (int lp = 0; lp < 2; ++lp) {
                     const int l = 4 * g + 2 * mb + lp;
                     const float g0 = ax[nb][mb][2 * lp], g1 = ax[nb][mb][2 * lp + 1];
-                    if (g0 == 0.f && g1 == 0.f) continue;
+                    const bool live = valid && (g0 != 0.f || g1 != 0.f);
                     uint32_t ic[3];
                     float fr[3];
                     grid_cell(v, l, p01[nb], ic, fr);
                     float* tab = ggrad + v.goff[l] + (size_t)(blockIdx.x % v.gcopies[l]) * v.gents[l] * 2u;
 #pragma unroll
                     for (int cc = 0; cc < 8; ++cc) {
-                        uint32_t e = grid_entry(v, l, ic[0] + (cc & 1), ic[1] + ((cc >> 1) & 1), ic[2] + (cc >> 2));
-                        float w = corner_weight(cc, fr);
-                        if (w == 0.f) continue;
-                        atomicAdd(tab + 2 * e, w * g0);
-                        atomicAdd(tab + 2 * e + 1, w * g1);
+                        const uint32_t e = grid_entry(v, l, ic[0] + (cc & 1), ic[1] + ((cc >> 1) & 1), ic[2] + (cc >> 2));
+                        const float w = corner_weight(cc, fr);
+                        const bool on = live && w != 0.f;
+                        float a = on ? w * g0 : 0.f, b = on ? w * g1 : 0.f;
+                        if (row_run_sum(on ? e : 0xFFFFFFFFu, c, a, b) && on) {
+                            atomicAdd(tab + 2 * e, a);
+                            atomicAdd(tab + 2 * e + 1, b);
+                        }
                     }
                 }
         }
@@ -521,6 +566,36 @@ __global__ void k_mlp_gather(const Record* __restrict__ rec, const TrainAux* __r
     out[i] = t;
 }
 
+// Spatial order of a training batch: a 24-bit Morton key of each sample's position in
+// the scene cube (8 bits per axis), radix-sorted, then the samples permuted into it.
+__device__ __forceinline__ uint32_t spread_bits8(uint32_t x) {
+    x &= 0xffu;
+    x = (x | (x << 8)) & 0x0000f00fu;
+    x = (x | (x << 4)) & 0x000c30c3u;
+    x = (x | (x << 2)) & 0x00249249u;
+    return x;
+}
+
+__global__ void k_mlp_sample_keys(MlpView v, const TrainSample* __restrict__ smp, uint32_t n, uint32_t* keys,
+                                  uint32_t* vals) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 x = smp[i].x_w;
+    float p01[3];
+    scene_unit(v, F3{x.x, x.y, x.z}, p01);
+    uint32_t q[3];
+    for (int a = 0; a < 3; ++a) q[a] = min((uint32_t)(p01[a] * 256.f), 255u);
+    keys[i] = (spread_bits8(q[0]) << 2) | (spread_bits8(q[1]) << 1) | spread_bits8(q[2]);
+    vals[i] = i;
+}
+
+__global__ void k_mlp_permute(const TrainSample* __restrict__ in, const uint32_t* __restrict__ order, uint32_t n,
+                              TrainSample* __restrict__ out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    out[i] = in[order[i]];
+}
+
 // Forward + mixture for n explicit points.
 __global__ __launch_bounds__(kMlpBlock) void k_mlp_eval(MlpView v, const float4* __restrict__ xs,
                                                         const float4* __restrict__ ns, const float4* __restrict__ wos,
@@ -646,6 +721,19 @@ hipError_t launch_mlp_gather(const Record* rec, const TrainAux* aux, uint32_t n_
     if (!batch || !n_rec) return hipSuccess;
     hipLaunchKernelGGL(k_mlp_gather, dim3((batch + 255) / 256), dim3(256), 0, stream, rec, aux, n_rec, offset, stride,
                        batch, w_scale, w_clip, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_mlp_sort_batch(const MlpView& v, const TrainSample* in, uint32_t n, const SortBuffers& sb,
+                                 TrainSample* out, hipStream_t stream) {
+    if (!n) return hipSuccess;
+    const dim3 grid((n + 255) / 256);
+    hipLaunchKernelGGL(k_mlp_sample_keys, grid, dim3(256), 0, stream, v, in, n, sb.keys, sb.vals);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    e = radix_sort_pairs(sb.keys, sb.vals, sb.keys_tmp, sb.vals_tmp, n, 24, sb.hist, sb.scan, stream);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_mlp_permute, grid, dim3(256), 0, stream, in, sb.vals, n, out);
     return hipGetLastError();
 }
 

@@ -129,6 +129,35 @@ def test_gradients_match_oracle(device, net):
     assert np.all(gg[gr == 0] == 0)
 
 
+def test_gradients_of_clustered_samples_match_oracle(device, net):
+    """Samples in 16 tight clusters, each cluster's samples consecutive: the lanes of a
+    wave row then hit the same hash entries on most levels, so the in-row run sums that
+    replace per-lane atomics merge long runs (and runs broken by zero-weight samples);
+    the hash-grid gradient must still match the oracle's."""
+    _, params, grid, res, dense, frame = net
+    device.mlp_set_params(params, grid)
+    rng = np.random.default_rng(9)
+    s = _samples(rng, frame, 2048)
+    ext = 1.0 / frame[3]
+    centres = frame[:3] + rng.uniform(0.1, 0.9, (16, 3)) * ext
+    spread = rng.choice([1e-4, 1e-3, 2e-2], 16) * ext
+    cl = np.repeat(np.arange(16), 128)
+    s[:, 0:3] = centres[cl] + rng.normal(size=(2048, 3)) * spread[cl, None]
+    s[:, 3] = rng.uniform(0.1, 3.0, 2048)
+    s[rng.random(2048) < 0.1, 3] = 0.0
+    loss, gp, gg = device.mlp_gradient(s)
+    g = mo.bf16(grid).reshape(mo.LEVELS, 1 << LOG2T, 2)
+    l_ref, gp_ref, gg_ref = mo.loss_and_grads(params.astype(np.float64), g, s.astype(np.float64),
+                                              res.astype(np.uint64), dense, LOG2T, frame, emulate=True)
+    assert abs(loss - l_ref) < 1e-3 * abs(l_ref), (loss, l_ref)
+    gr = gg_ref.reshape(-1)
+    for lv in range(mo.LEVELS):
+        sl = slice(lv * (2 << LOG2T), (lv + 1) * (2 << LOG2T))
+        err = np.linalg.norm(gg[sl] - gr[sl]) / max(np.linalg.norm(gr[sl]), 1e-30)
+        assert err < 2e-2, (lv, err)
+    assert np.all(gg[gr == 0] == 0)
+
+
 def test_chunked_gradient_equals_single_chunk(pg, device, cornell, net):
     _, params, grid, res, dense, frame = net
     rng = np.random.default_rng(4)
