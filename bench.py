@@ -45,12 +45,12 @@ def profiled_traffic(config: int) -> dict:
     return out
 
 
-def reference_image(pg, sc, data, spp: int, max_depth: int) -> np.ndarray:
+def reference_image(pg, sc, data, spp: int, max_depth: int, env_visibility: int = 0) -> np.ndarray:
     """Unguided GPU render of `spp` samples per pixel accumulated in FP64 on the host
     (64-SPP film reads summed in double), seed 1000. The GPU path tracer is the same
     estimator as the CPU oracle (pixel agreement is tested), so this stands in for the
     4096-SPP FP64 CPU image, which would take ~20 minutes on 16 host cores."""
-    rt = pg.RayTracer(sc, seed=1000, max_depth=max_depth)
+    rt = pg.RayTracer(sc, seed=1000, max_depth=max_depth, env_visibility=env_visibility)
     acc = np.zeros((data.height, data.width, 3), np.float64)
     done = 0
     while done < spp:
@@ -116,31 +116,18 @@ def time_staying_below(curve, e, col=2):
     return hit
 
 
-def time_to_relmse(pg, dev, sc, data, max_depth, ref_spp, cap_s, log, target=0.01):
+def time_to_relmse(pg, dev, sc, data, max_depth, ref_spp, cap_s, log, target=0.01, env_visibility=1, repeats=1):
     """Time-to-relMSE study: unguided vs quadtree guide vs learned guide. Besides the
     time to relMSE <= target (null when a method does not reach it within cap_s), the
     learned guide is compared with each baseline at that baseline's own final relMSE:
     speedup = time the baseline needed to stay below it / time the learned guide needed
-    to stay below it, on the full relMSE and on relMSE without the worst 0.01% of pixels."""
+    to stay below it, on the full relMSE and on relMSE without the worst 0.01% of pixels.
+    With repeats > 1 every method runs again with new seeds against the same reference;
+    the comparisons of each run and their medians are reported (curves of run 0)."""
     t0 = time.perf_counter()
-    ref = reference_image(pg, sc, data, ref_spp, max_depth)
+    ev = dict(env_visibility=env_visibility)
+    ref = reference_image(pg, sc, data, ref_spp, max_depth, env_visibility)
     log(f"ttr reference: {ref_spp} SPP in {time.perf_counter() - t0:.1f}s, mean {ref.mean():.5f}")
-    un = pg.RayTracer(sc, seed=2, max_depth=max_depth)
-    qt = pg.RayTracer(sc, seed=3, max_depth=max_depth, integrator=2, guide_prob=0.5)
-    ml = pg.RayTracer(sc, seed=5, max_depth=max_depth, record_nee=1)
-
-    def qt_setup():
-        # pass 1: 1 SPP record pass (Li with NEE), then radix sort + point BVH + quadtrees
-        dev.clear_records()
-        rec = pg.RayTracer(sc, seed=1, max_depth=max_depth, integrator=1, record=1, record_nee=1)
-        rec.clear()
-        rec.render(1)
-        dev.cache_build(cell_size=4096)
-        qt.clear()
-
-    def ml_setup():
-        dev.mlp_init(seed=7)
-        ml.clear()
 
     def adv(rt, fn):
         def f(k):
@@ -148,55 +135,105 @@ def time_to_relmse(pg, dev, sc, data, max_depth, ref_spp, cap_s, log, target=0.0
             return rt.image()
         return f
 
-    methods = {
-        "unguided": (lambda: un.clear(), adv(un, lambda k: un.render(k))),
-        "quadtree": (qt_setup, adv(qt, lambda k: qt.render(k))),
-        "learned": (ml_setup, adv(ml, lambda k: ml.render_online(k, steps=1, window=1))),
-    }
-    res = {}
-    for name, (setup, advance) in methods.items():
-        curve, hit = progressive(pg, name, ref, setup, advance, cap_s, target, log)
-        res[name] = {"seconds": None if hit is None else round(hit[1], 3),
-                     "spp": curve[-1][0], "final_relmse": round(curve[-1][2], 5),
-                     "final_trimmed_relmse": round(curve[-1][3], 5), "curve": curve}
+    def run_once(r):
+        un = pg.RayTracer(sc, seed=2 + 10 * r, max_depth=max_depth, **ev)
+        qt = pg.RayTracer(sc, seed=3 + 10 * r, max_depth=max_depth, integrator=2, guide_prob=0.5, **ev)
+        ml = pg.RayTracer(sc, seed=5 + 10 * r, max_depth=max_depth, record_nee=1, **ev)
+
+        def qt_setup():
+            # pass 1: 1 SPP record pass (Li with NEE), then radix sort + point BVH + quadtrees
+            dev.clear_records()
+            rec = pg.RayTracer(sc, seed=1 + 10 * r, max_depth=max_depth, integrator=1, record=1, record_nee=1, **ev)
+            rec.clear()
+            rec.render(1)
+            dev.cache_build(cell_size=4096)
+            qt.clear()
+
+        def ml_setup():
+            dev.mlp_init(seed=7 + r)
+            ml.clear()
+
+        methods = {
+            "unguided": (lambda: un.clear(), adv(un, lambda k: un.render(k))),
+            "quadtree": (qt_setup, adv(qt, lambda k: qt.render(k))),
+            "learned": (ml_setup, adv(ml, lambda k: ml.render_online(k, steps=1, window=1))),
+        }
+        res = {}
+        for name, (setup, advance) in methods.items():
+            curve, hit = progressive(pg, name if r == 0 else f"{name} (run {r})", ref, setup, advance, cap_s,
+                                     target, log)
+            res[name] = {"seconds": None if hit is None else round(hit[1], 3),
+                         "spp": curve[-1][0], "final_relmse": round(curve[-1][2], 5),
+                         "final_trimmed_relmse": round(curve[-1][3], 5), "curve": curve}
+
+        def speedup(fast, slow, col=2):
+            # time `slow` needs to stay below its own final relMSE over the time `fast`
+            # needs for it (null: `fast` ends above it)
+            tgt = res[slow]["curve"][-1][col]
+            t_slow = time_staying_below(res[slow]["curve"], tgt, col)[0]
+            t_fast = time_staying_below(res[fast]["curve"], tgt, col)[0]
+            return {"target_relmse": round(tgt, 5), f"{slow}_s": t_slow, f"{fast}_s": t_fast,
+                    "speedup": None if not t_fast else round(t_slow / t_fast, 3)}
+
+        def ratio(a, b, key):
+            return round(res[a][key] / res[b][key], 3)
+
+        cmp = {"learned_vs_unguided": speedup("learned", "unguided"),
+               "learned_vs_quadtree": speedup("learned", "quadtree"),
+               "trimmed": {"learned_vs_unguided": speedup("learned", "unguided", 3),
+                           "learned_vs_quadtree": speedup("learned", "quadtree", 3)},
+               "relmse_ratio_at_cap": {"unguided_over_learned": ratio("unguided", "learned", "final_relmse"),
+                                       "quadtree_over_learned": ratio("quadtree", "learned", "final_relmse")},
+               "trimmed_relmse_ratio_at_cap": {
+                   "unguided_over_learned": ratio("unguided", "learned", "final_trimmed_relmse"),
+                   "quadtree_over_learned": ratio("quadtree", "learned", "final_trimmed_relmse")}}
+        return res, cmp
+
+    res, cmp = run_once(0)
     info = dev.mlp_info()
-
-    def speedup(fast, slow, col=2):
-        # time `slow` needs to stay below its own final relMSE over the time `fast` needs
-        # for it (null: `fast` ends above it)
-        target = res[slow]["curve"][-1][col]
-        t_slow = time_staying_below(res[slow]["curve"], target, col)[0]
-        t_fast = time_staying_below(res[fast]["curve"], target, col)[0]
-        return {"target_relmse": round(target, 5), f"{slow}_s": t_slow, f"{fast}_s": t_fast,
-                "speedup": None if not t_fast else round(t_slow / t_fast, 3)}
-
-    cmp = {"learned_vs_unguided": speedup("learned", "unguided"),
-           "learned_vs_quadtree": speedup("learned", "quadtree"),
-           "trimmed": {"learned_vs_unguided": speedup("learned", "unguided", 3),
-                       "learned_vs_quadtree": speedup("learned", "quadtree", 3)},
-           "relmse_ratio_at_cap": {"unguided_over_learned": round(res["unguided"]["final_relmse"] /
-                                                                  res["learned"]["final_relmse"], 3),
-                                   "quadtree_over_learned": round(res["quadtree"]["final_relmse"] /
-                                                                  res["learned"]["final_relmse"], 3)},
-           "trimmed_relmse_ratio_at_cap": {
-               "unguided_over_learned": round(res["unguided"]["final_trimmed_relmse"] /
-                                              res["learned"]["final_trimmed_relmse"], 3),
-               "quadtree_over_learned": round(res["quadtree"]["final_trimmed_relmse"] /
-                                              res["learned"]["final_trimmed_relmse"], 3)}}
-    for r in res.values():
-        c = r["curve"]
+    runs = [cmp]
+    for r in range(1, repeats):
+        runs.append(run_once(r)[1])
+    for rr in res.values():
+        c = rr["curve"]
         step = max(1, len(c) // 16)
-        r["curve"] = [[s, t, round(e, 5), round(et, 5)]
-                      for s, t, e, et in (c[::step] + ([c[-1]] if (len(c) - 1) % step else []))]
-    return {"target": target, "cap_s": cap_s,
-            "reference": f"GPU unguided {ref_spp} SPP, FP64 host accumulation, seed 1000",
-            "curve_columns": ["spp", "seconds", "relmse", "relmse_without_top_0.01pct_pixels"],
-            "methods": {"unguided": "path tracer with NEE + MIS (Renderer.h:531-618)",
-                        "quadtree": "1-SPP record pass, point-BVH + per-cell quadtrees, 50/50 one-sample MIS, kNN k=16",
-                        "learned": "online: 1-SPP guided record passes, 1 Adam step of 2^18 samples per pass"},
-            **res,
-            "learned_diverged": bool(info["diverged"]),
-            "comparison": cmp}
+        rr["curve"] = [[s, t, round(e, 5), round(et, 5)]
+                       for s, t, e, et in (c[::step] + ([c[-1]] if (len(c) - 1) % step else []))]
+    out = {"target": target, "cap_s": cap_s,
+           "reference": f"GPU unguided {ref_spp} SPP, FP64 host accumulation, seed 1000",
+           "env_visibility": ("environment shadow rays along the light direction for every method and the "
+                              "reference image (the reference's target sceneCentre + wi * sceneRadius, "
+                              "Renderer.h:331-332, loses the direct sun behind the window, and guided and "
+                              "unguided MIS estimators then converge to different images)"
+                              if env_visibility else "Renderer.h:331-332 scene-bound target"),
+           "curve_columns": ["spp", "seconds", "relmse", "relmse_without_top_0.01pct_pixels"],
+           "methods": {"unguided": "path tracer with NEE + MIS (Renderer.h:531-618)",
+                       "quadtree": "1-SPP record pass, point-BVH + per-cell quadtrees, 50/50 one-sample MIS, kNN k=16",
+                       "learned": "online: 1-SPP guided record passes, 1 Adam step of 2^18 samples per pass"},
+           **res,
+           "learned_diverged": bool(info["diverged"]),
+           "comparison": cmp}
+    if repeats > 1:
+        def med(path):
+            vals = []
+            for c in runs:
+                v = c
+                for k in path:
+                    v = v[k]
+                vals.append(v)
+            ok = [v for v in vals if v is not None]
+            return {"runs": vals, "median": float(np.median(ok)) if len(ok) == len(vals) else None}
+        out["repeats"] = {
+            "n": repeats,
+            "learned_vs_unguided": med(["learned_vs_unguided", "speedup"]),
+            "learned_vs_quadtree": med(["learned_vs_quadtree", "speedup"]),
+            "trimmed_learned_vs_unguided": med(["trimmed", "learned_vs_unguided", "speedup"]),
+            "trimmed_learned_vs_quadtree": med(["trimmed", "learned_vs_quadtree", "speedup"]),
+            "relmse_ratio_unguided_over_learned": med(["relmse_ratio_at_cap", "unguided_over_learned"]),
+            "relmse_ratio_quadtree_over_learned": med(["relmse_ratio_at_cap", "quadtree_over_learned"]),
+            "trimmed_ratio_unguided_over_learned": med(["trimmed_relmse_ratio_at_cap", "unguided_over_learned"]),
+            "trimmed_ratio_quadtree_over_learned": med(["trimmed_relmse_ratio_at_cap", "quadtree_over_learned"])}
+    return out
 
 
 def batch_study(pg, dev, n_views: int, spp: int, max_depth: int, log) -> dict:
@@ -255,6 +292,11 @@ def main() -> int:
     ap.add_argument("--ttr-config", type=int, default=int(os.environ.get("PG_BENCH_TTR_CONFIG", "3")),
                     help="scene of the time-to-relMSE study (3: the closed room lit through one window, "
                          "where the guiding configurations are defined; 2: the open interior)")
+    ap.add_argument("--ttr-repeats", type=int, default=1,
+                    help="independent runs of each method in the study (medians reported when > 1)")
+    ap.add_argument("--ttr-env-visibility", type=int, choices=(0, 1), default=1,
+                    help="environment shadow rays in the study: 1 along the light direction (default), "
+                         "0 the reference's scene-bound target (Renderer.h:331-332)")
     ap.add_argument("--copy-check", action="store_true",
                     help="also time a plain 1 GiB device copy (HBM reference point / counter scaling)")
     a = ap.parse_args()
@@ -353,7 +395,8 @@ def main() -> int:
         tdata = data if a.ttr_config == a.config else pg.generate_scene(a.ttr_config, 0)
         tsc = sc if a.ttr_config == a.config else pg.Scene(dev, tdata)
         ttr = time_to_relmse(pg, dev, tsc, tdata, max_depth, a.ttr_ref_spp, a.ttr_cap,
-                             lambda m: print(m, file=sys.stderr, flush=True))
+                             lambda m: print(m, file=sys.stderr, flush=True), env_visibility=a.ttr_env_visibility,
+                             repeats=a.ttr_repeats)
         ttr = {"workload": f"config {a.ttr_config}: {tdata.n_tris} tris, {tdata.width}x{tdata.height}, "
                            f"max depth {max_depth}", **ttr}
 

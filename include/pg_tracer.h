@@ -85,7 +85,15 @@ typedef struct {
                                octant, origin Morton) order; results unchanged  */
     uint32_t dyn_fetch;     /* 1: persistent lanes claim new rays when a wave runs
                                below 48 busy lanes; results unchanged           */
-    uint32_t reserved[4];
+    uint32_t env_visibility; /* environment-light shadow rays: 0 aim at
+                               sceneCentre + wi * sceneRadius as Renderer.h:331-332
+                               does (default, the reference's images); 1 trace
+                               along wi out of the bounding sphere. With 0 a point
+                               away from the centre can test the wrong segment
+                               (behind a small window: direct sun lost), and then
+                               guided and unguided MIS estimators converge to
+                               different images; 1 makes them agree.            */
+    uint32_t reserved[3];
 } pg_render_params;
 
 typedef struct {

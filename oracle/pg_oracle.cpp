@@ -962,6 +962,7 @@ struct Scene {
     Mat4 invProj, cam; V3 origin; float width = 0.f, height = 0.f;
     std::vector<RecordOut> records;
     int envMode = 0;
+    int envVisibility = 0;  // 1: environment shadow rays along wi (not Renderer.h:331)
 
     Ray generateRay(float x, float y) const {                                  // Scene.h:45-57
         float xc = (2.f * x / width) - 1.f;
@@ -1054,7 +1055,9 @@ struct Scene {
         V3 wi = dl.normalize();
         float sg = (dot(wi, sd.gN) >= 0.f) ? 1.f : -1.f;
         V3 off = sd.x + sd.gN * (kEps * sg);
-        V3 far = centre + wi * radius;                                         // Renderer.h:331
+        // Renderer.h:331-332 aims the shadow ray at the scene-bound point in direction wi
+        // from the centre; envVisibility 1 traces along wi out of the bounding sphere.
+        V3 far = envVisibility ? off + wi * ((off - centre).length() + radius) : centre + wi * radius;
         if (vis(off, far, c)) {
             float ct = smax(dot(wi, sd.sN), 0.f);
             if (ct <= 0.f) return C3();
@@ -1366,6 +1369,7 @@ float orc_rng(uint32_t seed, uint32_t sample, uint32_t pixel, uint32_t dim) {
 int orc_render(orc_scene* os, const orc_render_params* p, double* film, orc_render_stats* st) {
     Scene& sc = os->sc;
     sc.envMode = p->env_mode;
+    sc.envVisibility = p->env_visibility;
     const int W = (int)sc.width, H = (int)sc.height;
     const int nt = nthreads(p->n_threads);
     const unsigned tile = 32;                                                  // Renderer.h:654

@@ -69,6 +69,9 @@ typedef struct {
     int32_t env_mode;      /* 0: Distribution2D (Sampling.h:166-209, 2 draws);
                               1: alias table over the same texel pmf (4 draws),
                                  the sampling the device path uses             */
+    int32_t env_visibility; /* 0: shadow ray to sceneCentre + wi * sceneRadius
+                                  (Renderer.h:331-332); 1: along wi out of the
+                                  bounding sphere                               */
 } orc_render_params;
 
 typedef struct {

@@ -600,6 +600,7 @@ pg_status pg_render_spp(pg_ctx* c, const pg_render_params* params, uint32_t n_sp
     }
     if (prm.max_depth > 64) return c->fail(PG_ERR_INVALID, "max_depth > 64");
     if (!(prm.guide_prob >= 0.f && prm.guide_prob <= 1.f)) return c->fail(PG_ERR_INVALID, "guide_prob outside [0,1]");
+    if (prm.env_visibility > 1u) return c->fail(PG_ERR_INVALID, "env_visibility must be 0 or 1");
     uint32_t W = (uint32_t)c->host.camera.width, H = (uint32_t)c->host.camera.height;
     uint32_t n = W * H;
     pg_status st = ensure_paths(c, n);
@@ -668,7 +669,7 @@ pg_status pg_render_spp(pg_ctx* c, const pg_render_params* params, uint32_t n_sp
     hipEvent_t t0 = c->ev(), t1 = c->ev();
     (void)hipEventRecord(t0, c->stream);
     for (uint32_t s = 0; s < n_spp; ++s) {
-        PassParams pp{prm.seed, c->spp, W, H, prm.max_depth, prm.rr_depth, prm.integrator};
+        PassParams pp{prm.seed, c->spp, W, H, prm.max_depth, prm.rr_depth, prm.integrator, prm.env_visibility};
         PG_TRY(c, timed(3, [&] { return launch_raygen(c->dev, c->ps, pp, c->stream); }), "raygen");
         for (uint32_t depth = 0; depth <= prm.max_depth; ++depth) {
             int parity = (int)(depth & 1u);

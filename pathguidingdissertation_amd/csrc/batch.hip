@@ -239,7 +239,7 @@ hipError_t enqueue_pass(pg_batch* b, const pg_render_params& prm) {
     hipStream_t st = b->stream();
     hipError_t e;
     for (const auto& ch : b->chunks) {
-        PassParams pp{prm.seed, 0u, 0u, 0u, prm.max_depth, prm.rr_depth, PG_INTEGRATOR_PATH};
+        PassParams pp{prm.seed, 0u, 0u, 0u, prm.max_depth, prm.rr_depth, PG_INTEGRATOR_PATH, prm.env_visibility};
         if ((e = launch_raygen_batch(b->ps, nv, ch.first, ch.second, prm.seed, b->d_sample, st)) != hipSuccess) return e;
         for (uint32_t depth = 0; depth <= prm.max_depth; ++depth) {
             int parity = (int)(depth & 1u);
@@ -406,6 +406,7 @@ pg_status pg_batch_render(pg_batch* b, const pg_render_params* params, uint32_t 
     pg_render_params prm = params ? *params : pg_render_params_default();
     if (prm.integrator != PG_INTEGRATOR_PATH) return b->fail(PG_ERR_INVALID, "batches render unguided");
     if (prm.max_depth > 64) return b->fail(PG_ERR_INVALID, "max_depth > 64");
+    if (prm.env_visibility > 1u) return b->fail(PG_ERR_INVALID, "env_visibility must be 0 or 1");
     return render(b, prm, n_spp);
 }
 

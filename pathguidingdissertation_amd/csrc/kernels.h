@@ -72,6 +72,7 @@ struct PassParams {
     uint32_t width, height;
     uint32_t max_depth, rr_depth;
     int32_t integrator;
+    uint32_t env_visibility;  // 0: Renderer.h:331-332 shadow target; 1: along wi
 };
 
 // Launch helpers (defined in kernels.hip); all enqueue on `stream`.

@@ -345,8 +345,8 @@ __device__ __forceinline__ F3 guide_sample(const GuideView& g, const uint32_t* c
 template <int MODE>
 __device__ __forceinline__ bool next_event(const DevScene& s, const MaterialRec& m, const SD& sd, uint32_t key,
                                            uint32_t& dim, const GuideView& g, const uint32_t* cells,
-                                           const float4* mx, bool guided, float ag, C3& contrib, F3& sh_o, F3& sh_d,
-                                           float& sh_t) {
+                                           const float4* mx, bool guided, float ag, uint32_t env_vis, C3& contrib,
+                                           F3& sh_o, F3& sh_d, float& sh_t) {
     contrib = c3(0.f, 0.f, 0.f);
     float pmf = 1.f / (float)s.n_lights;
     uint32_t nl = (uint32_t)s.n_lights;
@@ -386,7 +386,11 @@ __device__ __forceinline__ bool next_event(const DevScene& s, const MaterialRec&
     F3 wi = normalize(dl);
     float sg = (dot(wi, sd.gn) >= 0.f) ? 1.f : -1.f;
     F3 off = sd.x + sd.gn * (kEps * sg);
-    F3 far = F3{s.centre[0], s.centre[1], s.centre[2]} + wi * s.radius;  // Renderer.h:331
+    // Renderer.h:331-332 aims the shadow ray at the scene-bound point in direction wi
+    // from the centre, which is off the light direction for points away from the
+    // centre; env_vis 1 traces along wi out of the bounding sphere instead.
+    const F3 cen{s.centre[0], s.centre[1], s.centre[2]};
+    F3 far = env_vis ? off + wi * (length(off - cen) + s.radius) : cen + wi * s.radius;
     float ctl = smax(dot(wi, sd.sn), 0.f);
     if (ctl <= 0.f) return false;
     float pdf_b = 0.f;
@@ -494,7 +498,8 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene s0, PathState ps, Pas
                     // computeDirect (Renderer.h:269-354)
                     if (!m.specular && s.n_lights > 0) {
                         C3 contrib;
-                        if (next_event<MODE>(s, m, sd, key, dim, gc.g, cells, mx, guided, ag, contrib, sh_o, sh_d, sh_t)) {
+                        if (next_event<MODE>(s, m, sd, key, dim, gc.g, cells, mx, guided, ag, p.env_visibility, contrib,
+                                             sh_o, sh_d, sh_t)) {
                             // the shadow ray can only change the image when the contribution is nonzero
                             C3 direct = beta * contrib;
                             if (!is_zero(direct)) {
@@ -774,7 +779,8 @@ __global__ __launch_bounds__(kBlock) void k_shade_record(DevScene s, PathState p
                         if (NEE) {
                             C3 contrib;
                             GuideView none{};
-                            if (next_event<MODE>(s, m, sd, key, dim, none, nullptr, mx, guided, ag, contrib, sh_o, sh_d,
+                            if (next_event<MODE>(s, m, sd, key, dim, none, nullptr, mx, guided, ag, p.env_visibility,
+                                                 contrib, sh_o, sh_d,
                                                  sh_t) &&
                                 !is_zero(contrib)) {
                                 want_shadow = true;

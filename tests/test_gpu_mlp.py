@@ -294,31 +294,43 @@ def _trimmed_relmse(img, ref, drop=1e-3):
 
 def test_learned_guide_gains_over_unguided(pg, device):
     """Closed room lit through one window (config-3 layout, small floor): online learned
-    guiding (guided record passes, one Adam step each) converges to the unguided image and
-    is less noisy at equal SPP. The noise is compared on relMSE without the worst 0.1% of
-    pixels: the full relMSE of a 32-SPP image is set by a few caustic fireflies (glass and
-    conductors reflecting the sun) and moves 10x between seeds for either method, while the
-    trimmed one is stable to ~2% (0.294-0.298 unguided, 0.214-0.222 learned over 3 seeds)."""
+    guiding (guided record passes, one Adam step each) converges to the unguided image
+    and, after 32 passes, is less noisy than unguided rendering at equal SPP.
+
+    Noise is compared on relMSE without the worst 0.1% of pixels, averaged over two
+    independent runs of each method: the full relMSE of a 32-SPP image is set by a few
+    caustic fireflies and moves 10-1000x between seeds, the trimmed one by ~2%. Measured
+    over four seeds: learned/unguided 0.82-0.91 at 32 SPP; by 128 SPP the two are equal
+    on this small scene (the gain is early and in the tail). Environment shadow rays go
+    along the light direction: with the reference's scene-bound target (Renderer.h:331)
+    the NEE estimator loses the direct sun behind the window, and a guide that learns the
+    sun direction then converges to a brighter image than the unguided reference."""
     data = pg.generate_scene(101, 0)
     sc = pg.Scene(device, data)
-    ref_rt = pg.RayTracer(sc, seed=100, max_depth=12)
+    ev = dict(env_visibility=1)
+    ref_rt = pg.RayTracer(sc, seed=100, max_depth=12, **ev)
     ref_rt.clear()
     ref_rt.render(256)
     ref = ref_rt.image()
-    un = pg.RayTracer(sc, seed=2, max_depth=12)
-    un.clear()
-    un.render(32)
-    e_un = _trimmed_relmse(un.image(), ref)
-    device.mlp_init(seed=7)
-    ml = pg.RayTracer(sc, seed=5, max_depth=12, record_nee=1)
-    ml.clear()
-    ml.render_online(32, steps=1)
-    e_ml = _trimmed_relmse(ml.image(), ref)
-    ml.render_online(96, steps=1)
-    m_ml = ml.image().mean()
-    info = device.mlp_info()
-    print(f"trimmed relMSE at 32 SPP: unguided {e_un:.4f} learned {e_ml:.4f}; mean ref {ref.mean():.5f} "
-          f"learned-128 {m_ml:.5f}; steps {info['step']} loss {info['last_loss']:.3f}")
-    assert info["diverged"] == 0 and info["step"] == 128
-    assert abs(m_ml - ref.mean()) / ref.mean() < 0.05
-    assert e_ml < 0.85 * e_un, (e_ml, e_un)
+    e_un, e_ml, means, steps = [], [], [], []
+    for run in range(2):
+        un = pg.RayTracer(sc, seed=2 + run, max_depth=12, **ev)
+        un.clear()
+        un.render(32)
+        e_un.append(_trimmed_relmse(un.image(), ref))
+        device.mlp_init(seed=7 + run)
+        ml = pg.RayTracer(sc, seed=5 + run, max_depth=12, record_nee=1, **ev)
+        ml.clear()
+        ml.render_online(32, steps=1)
+        e_ml.append(_trimmed_relmse(ml.image(), ref))
+        ml.render_online(96, steps=1)
+        means.append(float(ml.image().mean()))
+        info = device.mlp_info()
+        assert info["diverged"] == 0
+        steps.append(info["step"])
+    print(f"trimmed relMSE at 32 SPP: unguided {e_un} learned {e_ml}; mean ref {ref.mean():.5f} "
+          f"learned-128 {means}")
+    assert steps == [128, 128]
+    for m in means:
+        assert abs(m - ref.mean()) / ref.mean() < 0.05
+    assert np.mean(e_ml) < 0.93 * np.mean(e_un), (e_ml, e_un)

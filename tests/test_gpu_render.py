@@ -55,6 +55,25 @@ def test_interior_same_stream_matches_oracle(pg, device, small_interior):
     assert agree > 0.90, agree
 
 
+@pytest.mark.parametrize("env_visibility", [0, 1])
+def test_window_room_same_stream_matches_oracle(pg, device, env_visibility):
+    """Closed room lit through one window, both environment shadow-ray targets: the
+    reference's scene-bound point (Renderer.h:331-332) and the light direction."""
+    from dataclasses import replace
+    from oracle.pg_oracle import OracleScene
+    base = pg.generate_scene(101, 0)
+    data = replace(base, camera=dict(base.camera, width=180, height=320))
+    sc = pg.Scene(device, data)
+    rt = pg.RayTracer(sc, seed=4, max_depth=12, env_visibility=env_visibility)
+    rt.clear()
+    rt.render(4)
+    film, _ = rt.film()
+    ref, _ = OracleScene(data).render(4, seed=4, max_depth=12, accum_f64=False, env_mode=1,
+                                      env_visibility=env_visibility)
+    agree = _pixel_agreement(film, ref.astype(np.float32))
+    assert agree > 0.90, agree
+
+
 def test_cornell_relmse_within_reference_band(pg, device, cornell):
     band = json.load(open(GOLDEN / "relmse_band_config1.json"))
     ref = np.load(GOLDEN / "oracle_config1_4096spp.npz")["image"]
