@@ -266,8 +266,31 @@ def batch_study(pg, dev, n_views: int, spp: int, max_depth: int, log) -> dict:
     s1 = b.stats()
     rays = (s1["closest_rays"] - s0["closest_rays"]) + (s1["shadow_rays"] - s0["shadow_rays"])
     log(f"batch: {n_views} views, {tris} tris, {pixels} pixels, {wall * 1e3 / spp:.2f} ms/SPP, config {best}")
+    # the same views guided by per-scene learned networks (trained online on each scene's
+    # own camera first: 16 guided record passes with one Adam step each)
+    t0 = time.perf_counter()
+    for i in range(n_views):
+        d = pg.generate_scene(5, i)
+        sc_i = pg.Scene(dev, d)
+        dev.mlp_init(seed=7 + i)
+        pg.RayTracer(sc_i, seed=1 + i, max_depth=max_depth, record_nee=1).render_online(16, steps=1)
+        p_i, g_i = dev.mlp_params()
+        b.set_guide(i, p_i, g_i, seed=7 + i)
+    train = time.perf_counter() - t0
+    b.clear()
+    g0 = b.stats()
+    t0 = time.perf_counter()
+    b.render(spp, max_depth=max_depth, integrator=3)
+    gwall = time.perf_counter() - t0
+    g1 = b.stats()
+    grays = (g1["closest_rays"] - g0["closest_rays"]) + (g1["shadow_rays"] - g0["shadow_rays"])
+    log(f"batch guided: {gwall * 1e3 / spp:.2f} ms/SPP after {train:.1f}s of per-scene training")
     b.close()
+    guided = {"ms_per_spp": round(gwall * 1e3 / spp, 3), "samples_per_s": round(pixels * spp / gwall, 1),
+              "mrays_per_s": round(grays / gwall / 1e6, 2), "train_s": round(train, 1),
+              "guides": "one network per scene, trained online on the scene's own camera (16 SPP)"}
     return {"views": n_views, "scenes_tris": tris, "pixels": pixels, "spp": spp, "max_depth": max_depth,
+            "learned_guided": guided,
             "ms_per_spp": round(wall * 1e3 / spp, 3), "samples_per_s": round(pixels * spp / wall, 1),
             "mrays_per_s": round(rays / wall / 1e6, 2), "gpu_ms_per_spp": round((s1["gpu_ms"] - s0["gpu_ms"]) / spp, 3),
             "graph_nodes": s1["graph_nodes"], "chunks": s1["chunks"],

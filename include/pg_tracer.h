@@ -315,7 +315,18 @@ pg_status pg_batch_configure(pg_batch* batch, const pg_batch_config* cfg);
  * by coordinate descent on a few SPP each, keeps the fastest and clears the films. */
 pg_status pg_batch_autotune(pg_batch* batch, const pg_render_params* params, pg_batch_config* best,
                             double* ms_per_spp);
+/* Learned guide of one scene of the batch (no reference counterpart; configuration 5
+ * renders its views "guided by config 4's network with per-scene weights"): the
+ * configuration the network was trained with and its FP32 parameters and hash grid
+ * as pg_mlp_get_params returns them (17648 and 16 * 2^log2_table * 2 floats).
+ * pg_batch_render with params->integrator = PG_INTEGRATOR_GUIDED_MLP then evaluates
+ * each scene's network for its own paths (one-sample MIS with the BSDF, as a plain
+ * PG_INTEGRATOR_GUIDED_MLP pass of pg_render_spp); scenes without a guide render
+ * unguided. No training happens inside a batch.                                  */
+pg_status pg_batch_set_guide(pg_batch* batch, uint32_t scene_id, const pg_mlp_config* cfg,
+                             const float* params, const float* grid);
 pg_status pg_batch_clear(pg_batch* batch);
+/* params->integrator: PG_INTEGRATOR_PATH or PG_INTEGRATOR_GUIDED_MLP.             */
 pg_status pg_batch_render(pg_batch* batch, const pg_render_params* params, uint32_t n_spp);
 pg_status pg_batch_film_read(pg_batch* batch, uint32_t view_id, float* rgb_sum, uint32_t* spp);
 pg_status pg_batch_stats_read(pg_batch* batch, pg_batch_stats* stats);

@@ -146,6 +146,7 @@ SIGNATURES = {
                                       C.POINTER(C.c_double)]),
     "pg_batch_clear": (C.c_int32, [_P]),
     "pg_batch_render": (C.c_int32, [_P, C.POINTER(PgRenderParams), C.c_uint32]),
+    "pg_batch_set_guide": (C.c_int32, [_P, C.c_uint32, C.POINTER(PgMlpConfig), _F, _F]),
     "pg_batch_film_read": (C.c_int32, [_P, C.c_uint32, _F, _U32]),
     "pg_batch_stats_read": (C.c_int32, [_P, C.POINTER(PgBatchStats)]),
     "pg_metrics": (C.c_int32, [_F, _F, C.c_uint64, C.c_uint64, C.c_int32, C.c_int32, C.POINTER(PgImageMetrics)]),

@@ -1199,23 +1199,11 @@ pg_status pg_mlp_init(pg_ctx* c, const pg_mlp_config* cfg_in) {
     PG_TRY(c, launch_mlp_mirror(m.params, m.mirror, c->stream), "mirror");
     PG_TRY(c, launch_grid_mirror(m.grid, m.grid_bf, m.n_grid, c->stream), "mirror");
     PG_TRY(c, hipStreamSynchronize(c->stream), "init");
-    // view: levels res_l = floor(res_min b^l), b = (res_max / res_min)^(1/15)
     MlpView& v = m.view;
     v.wmirror = m.mirror;
     v.params = m.params;
     v.grid = m.grid_bf;
-    v.table_mask = (uint32_t)(T - 1);
-    v.log2_table = cfg.log2_table;
-    const double b = std::exp((std::log((double)cfg.res_max) - std::log((double)cfg.res_min)) / (kHashLevels - 1));
-    for (int l = 0; l < kHashLevels; ++l) {
-        double r = std::floor((double)cfg.res_min * std::pow(b, (double)l) + 1e-6);
-        v.res[l] = (uint32_t)std::max(1.0, r);
-        double cells = std::pow((double)v.res[l] + 1.0, 3.0);
-        v.dense[l] = cells <= (double)T ? 1u : 0u;
-    }
-    const PreparedScene& h = c->host;
-    for (int a = 0; a < 3; ++a) v.lo[a] = h.centre[a] - h.radius;
-    v.inv_ext = h.radius > 0.f ? 1.f / (2.f * h.radius) : 0.f;
+    mlp_view_frame(v, cfg.log2_table, cfg.res_min, cfg.res_max, c->host.centre, c->host.radius);
     uint64_t goff = 0;
     for (int l = 0; l < kHashLevels; ++l) {
         uint64_t ents = v.dense[l] ? (uint64_t)(v.res[l] + 1) * (v.res[l] + 1) * (v.res[l] + 1) : (uint64_t)T;

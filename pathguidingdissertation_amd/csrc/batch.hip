@@ -21,6 +21,7 @@
 #include "ctx_internal.h"
 #include "host_prep.hpp"
 #include "kernels.h"
+#include "mlp.h"
 
 using namespace pg;
 
@@ -37,8 +38,20 @@ void bfree(T*& p) {
     p = nullptr;
 }
 
+// A scene's learned guide in a batch: FP32 master parameters and hash grid as the
+// caller trained them (pg_mlp_get_params), their BF16 mirrors, and the view.
+struct SceneGuide {
+    float* params = nullptr;
+    uint16_t* mirror = nullptr;
+    float* grid = nullptr;
+    uint32_t* grid_bf = nullptr;
+    MlpView view{};
+    ~SceneGuide() { bfree(params); bfree(mirror); bfree(grid); bfree(grid_bf); }
+};
+
 struct SceneBuf {
     PreparedScene host;
+    std::unique_ptr<SceneGuide> guide;
     BvhNode* nodes = nullptr;
     TriIsect* tris = nullptr;
     TriShade* shade = nullptr;
@@ -115,6 +128,9 @@ struct pg_batch {
     uint32_t spp = 0;
     pg_batch_config cfg{0, 0, 0, 0, 1, 0, {0, 0}};
     std::vector<std::pair<uint32_t, uint32_t>> chunks;  // (first global path, paths)
+    std::vector<std::vector<uint32_t>> chunk_scenes;    // scenes with views in each chunk
+    float4* mix = nullptr;                              // learned-guide mixtures by queue slot
+    uint32_t mix_cap = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     bool graph_ok = false;
@@ -185,18 +201,26 @@ pg_status prepare(pg_batch* b) {
     }
     // chunks: whole views, greedily packed up to queue_paths
     b->chunks.clear();
+    b->chunk_scenes.clear();
     const uint64_t q = b->cfg.queue_paths ? b->cfg.queue_paths : ~0ull;
     uint32_t base = 0, n = 0, largest = 0;
+    std::vector<uint32_t> cs;
     for (const BatchView& v : b->views) {
         uint32_t px = v.width * v.height;
         if (n > 0 && (uint64_t)n + px > q) {
             b->chunks.push_back({base, n});
+            b->chunk_scenes.push_back(cs);
+            cs.clear();
             base += n;
             n = 0;
         }
         n += px;
+        if (std::find(cs.begin(), cs.end(), v.scene) == cs.end()) cs.push_back(v.scene);
     }
-    if (n) b->chunks.push_back({base, n});
+    if (n) {
+        b->chunks.push_back({base, n});
+        b->chunk_scenes.push_back(cs);
+    }
     for (auto& c : b->chunks) largest = std::max(largest, c.second);
     if (b->ps_cap < largest) {
         b->free_paths();
@@ -235,18 +259,34 @@ hipError_t enqueue_pass(pg_batch* b, const pg_render_params& prm) {
     const uint32_t sgrid = (uint32_t)cu * (b->cfg.shade_blocks_per_cu ? b->cfg.shade_blocks_per_cu : 8u);
     const DevScene& s0 = b->scenes[0]->dev;  // unused by the batch kernels' per-path scenes
     const uint32_t nv = (uint32_t)b->views.size();
+    const bool learned = prm.integrator == PG_INTEGRATOR_GUIDED_MLP;
     GuideCtx gc{};
+    if (learned) gc.mix = b->mix;
     hipStream_t st = b->stream();
     hipError_t e;
-    for (const auto& ch : b->chunks) {
-        PassParams pp{prm.seed, 0u, 0u, 0u, prm.max_depth, prm.rr_depth, PG_INTEGRATOR_PATH, prm.env_visibility};
+    for (size_t ci = 0; ci < b->chunks.size(); ++ci) {
+        const auto& ch = b->chunks[ci];
+        PassParams pp{prm.seed, 0u, 0u, 0u, prm.max_depth, prm.rr_depth, prm.integrator, prm.env_visibility};
         if ((e = launch_raygen_batch(b->ps, nv, ch.first, ch.second, prm.seed, b->d_sample, st)) != hipSuccess) return e;
         for (uint32_t depth = 0; depth <= prm.max_depth; ++depth) {
             int parity = (int)(depth & 1u);
             if ((e = launch_extend(s0, b->ps, parity, stack, false, tgrid, st, true, nullptr, b->cfg.dyn_fetch != 0)) !=
                 hipSuccess)
                 return e;
-            if ((e = launch_shade(s0, b->ps, pp, gc, 0, parity, (int)depth, sgrid, st, true)) != hipSuccess) return e;
+            if (learned) {
+                // every slot unguided, then each scene's network writes its own slots
+                if ((e = launch_mix_clear(b->ps, parity, b->mix, (uint32_t)cu * 4, st)) != hipSuccess) return e;
+                for (uint32_t sid : b->chunk_scenes[ci]) {
+                    const SceneGuide* g = b->scenes[sid]->guide.get();
+                    if (!g) continue;
+                    if ((e = launch_mlp_infer(s0, b->ps, g->view, parity, b->mix, (uint32_t)cu * 2, st, true, sid)) !=
+                        hipSuccess)
+                        return e;
+                }
+            }
+            if ((e = launch_shade(s0, b->ps, pp, gc, learned ? 2 : 0, parity, (int)depth, sgrid, st, true)) !=
+                hipSuccess)
+                return e;
             if ((e = launch_shadow(s0, b->ps, stack, false, tgrid, st, true, nullptr, b->cfg.dyn_fetch != 0)) !=
                 hipSuccess)
                 return e;
@@ -257,13 +297,35 @@ hipError_t enqueue_pass(pg_batch* b, const pg_render_params& prm) {
 }
 
 bool same_params(const pg_render_params& a, const pg_render_params& b) {
-    return a.seed == b.seed && a.max_depth == b.max_depth && a.rr_depth == b.rr_depth;
+    return a.seed == b.seed && a.max_depth == b.max_depth && a.rr_depth == b.rr_depth &&
+           a.integrator == b.integrator && a.env_visibility == b.env_visibility;
+}
+
+// Parameters a batch renders with: unguided, or the learned guides of its scenes.
+pg_status check_params(pg_batch* b, const pg_render_params& prm) {
+    if (prm.integrator != PG_INTEGRATOR_PATH && prm.integrator != PG_INTEGRATOR_GUIDED_MLP)
+        return b->fail(PG_ERR_INVALID, "batches render unguided or with the scenes' learned guides");
+    if (prm.integrator == PG_INTEGRATOR_GUIDED_MLP) {
+        bool any = false;
+        for (auto& s : b->scenes) any = any || s->guide != nullptr;
+        if (!any) return b->fail(PG_ERR_STATE, "no scene of the batch has a learned guide (pg_batch_set_guide)");
+    }
+    if (prm.max_depth > 64) return b->fail(PG_ERR_INVALID, "max_depth > 64");
+    if (prm.env_visibility > 1u) return b->fail(PG_ERR_INVALID, "env_visibility must be 0 or 1");
+    return PG_OK;
 }
 
 pg_status render(pg_batch* b, const pg_render_params& prm, uint32_t n_spp) {
     pg_status st = prepare(b);
     if (st != PG_OK) return st;
     hipStream_t s = b->stream();
+    if (prm.integrator == PG_INTEGRATOR_GUIDED_MLP && b->mix_cap < b->ps_cap) {
+        b->drop_graph();
+        PG_BTRY(b, hipStreamSynchronize(s), "sync");
+        bfree(b->mix);
+        PG_BTRY(b, balloc(&b->mix, (size_t)b->ps_cap * kMixF4), "alloc guide mixtures");
+        b->mix_cap = b->ps_cap;
+    }
     if (b->cfg.use_graph && (!b->graph_ok || !same_params(prm, b->gparams))) {
         b->drop_graph();
         PG_BTRY(b, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin capture");
@@ -335,7 +397,7 @@ pg_status pg_batch_destroy(pg_batch* b) {
     (void)hipStreamSynchronize(b->stream());
     b->drop_graph();
     b->free_paths();
-    bfree(b->d_scenes); bfree(b->d_views); bfree(b->film);
+    bfree(b->d_scenes); bfree(b->d_views); bfree(b->film); bfree(b->mix);
     bfree(b->d_counters); bfree(b->d_stats); bfree(b->d_sample);
     if (b->e0) (void)hipEventDestroy(b->e0);
     if (b->e1) (void)hipEventDestroy(b->e1);
@@ -404,10 +466,43 @@ pg_status pg_batch_clear(pg_batch* b) {
 pg_status pg_batch_render(pg_batch* b, const pg_render_params* params, uint32_t n_spp) {
     if (!b) return PG_ERR_INVALID;
     pg_render_params prm = params ? *params : pg_render_params_default();
-    if (prm.integrator != PG_INTEGRATOR_PATH) return b->fail(PG_ERR_INVALID, "batches render unguided");
-    if (prm.max_depth > 64) return b->fail(PG_ERR_INVALID, "max_depth > 64");
-    if (prm.env_visibility > 1u) return b->fail(PG_ERR_INVALID, "env_visibility must be 0 or 1");
+    pg_status st = check_params(b, prm);
+    if (st != PG_OK) return st;
     return render(b, prm, n_spp);
+}
+
+pg_status pg_batch_set_guide(pg_batch* b, uint32_t scene, const pg_mlp_config* cfg, const float* params,
+                             const float* grid) {
+    if (!b) return PG_ERR_INVALID;
+    if (scene >= b->scenes.size()) return b->fail(PG_ERR_INVALID, "unknown scene");
+    if (!cfg || !params || !grid) return b->fail(PG_ERR_INVALID, "null guide configuration or parameters");
+    if (cfg->log2_table < 12 || cfg->log2_table > 22) return b->fail(PG_ERR_INVALID, "log2_table outside 12..22");
+    if (!(cfg->res_min >= 1.f) || !(cfg->res_max >= cfg->res_min) || !(cfg->res_max <= 1048576.f))
+        return b->fail(PG_ERR_INVALID, "grid resolutions");
+    if (hipSetDevice(pg_internal::device(b->ctx)) != hipSuccess) return b->fail(PG_ERR_DEVICE, "hipSetDevice");
+    hipStream_t s = b->stream();
+    PG_BTRY(b, hipStreamSynchronize(s), "sync");
+    b->drop_graph();
+    std::unique_ptr<SceneGuide> g(new (std::nothrow) SceneGuide());
+    if (!g) return PG_ERR_NOMEM;
+    const size_t n_grid = (size_t)kHashLevels * ((size_t)1 << cfg->log2_table) * 2;
+    PG_BTRY(b, balloc(&g->params, kParams), "alloc guide");
+    PG_BTRY(b, balloc(&g->mirror, kMirror), "alloc guide");
+    PG_BTRY(b, balloc(&g->grid, n_grid), "alloc guide");
+    PG_BTRY(b, balloc(&g->grid_bf, n_grid / 2), "alloc guide");
+    PG_BTRY(b, hipMemcpy(g->params, params, kParams * 4, hipMemcpyHostToDevice), "upload guide");
+    PG_BTRY(b, hipMemcpy(g->grid, grid, n_grid * 4, hipMemcpyHostToDevice), "upload guide");
+    PG_BTRY(b, launch_mlp_mirror(g->params, g->mirror, s), "guide mirror");
+    PG_BTRY(b, launch_grid_mirror(g->grid, g->grid_bf, n_grid, s), "guide mirror");
+    PG_BTRY(b, hipStreamSynchronize(s), "guide upload");
+    MlpView& v = g->view;
+    v.wmirror = g->mirror;
+    v.params = g->params;
+    v.grid = g->grid_bf;
+    const PreparedScene& h = b->scenes[scene]->host;
+    mlp_view_frame(v, cfg->log2_table, cfg->res_min, cfg->res_max, h.centre, h.radius);
+    b->scenes[scene]->guide = std::move(g);
+    return PG_OK;
 }
 
 pg_status pg_batch_film_read(pg_batch* b, uint32_t view, float* rgb_sum, uint32_t* spp) {
@@ -448,7 +543,8 @@ pg_status pg_batch_stats_read(pg_batch* b, pg_batch_stats* s) {
 pg_status pg_batch_autotune(pg_batch* b, const pg_render_params* params, pg_batch_config* best_out, double* ms_out) {
     if (!b) return PG_ERR_INVALID;
     pg_render_params prm = params ? *params : pg_render_params_default();
-    if (prm.integrator != PG_INTEGRATOR_PATH) return b->fail(PG_ERR_INVALID, "batches render unguided");
+    pg_status cst = check_params(b, prm);
+    if (cst != PG_OK) return cst;
     uint32_t largest = 0;
     for (const BatchView& v : b->views) largest = std::max(largest, v.width * v.height);
     const int base_stack = stack_entries(std::max(b->max_depth, 1));

@@ -123,7 +123,8 @@ hipError_t launch_shadow_record(const DevScene& s, const PathState& ps, const Re
                                 hipStream_t stream);
 struct MlpView;
 hipError_t launch_mlp_infer(const DevScene& s, const PathState& ps, const MlpView& v, int parity, float4* mix,
-                            uint32_t grid, hipStream_t stream);
+                            uint32_t grid, hipStream_t stream, bool multi = false, uint32_t scene = 0);
+hipError_t launch_mix_clear(const PathState& ps, int parity, float4* mix, uint32_t grid, hipStream_t stream);
 int stack_entries(int max_depth);
 uint32_t trace_grid(int cu_count, int stack);
 

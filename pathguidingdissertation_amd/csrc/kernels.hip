@@ -1008,8 +1008,11 @@ hipError_t launch_extend(const DevScene& s, const PathState& ps, int parity, int
 hipError_t launch_shade(const DevScene& s, const PathState& ps, const PassParams& p, const GuideCtx& gc, int mode,
                         int parity, int depth, uint32_t grid, hipStream_t stream, bool multi) {
     if (multi) {
-        if (mode != 0) return hipErrorInvalidValue;  // batches render unguided
-        hipLaunchKernelGGL((k_shade<0, true>), dim3(grid), dim3(kBlock), 0, stream, s, ps, p, gc, parity, depth);
+        if (mode == 1) return hipErrorInvalidValue;  // batches: unguided or learned guide
+        if (mode == 2)
+            hipLaunchKernelGGL((k_shade<2, true>), dim3(grid), dim3(kBlock), 0, stream, s, ps, p, gc, parity, depth);
+        else
+            hipLaunchKernelGGL((k_shade<0, true>), dim3(grid), dim3(kBlock), 0, stream, s, ps, p, gc, parity, depth);
     } else if (mode == 1) {
         hipLaunchKernelGGL((k_shade<1, false>), dim3(grid), dim3(kBlock), 0, stream, s, ps, p, gc, parity, depth);
     } else if (mode == 2) {

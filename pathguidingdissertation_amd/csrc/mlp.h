@@ -172,6 +172,8 @@ hipError_t launch_grid_adam(const MlpView& view, float* grid_master, uint32_t* g
 constexpr int kSumParts = 256;
 hipError_t launch_sum(const float* in, size_t n, size_t stride, double* part, float* out, hipStream_t stream);
 hipError_t launch_mlp_mirror(const float* params, uint16_t* mirror, hipStream_t stream);
+// Level resolutions, dense/hashed indexing and the scene cube of a view (host).
+void mlp_view_frame(MlpView& v, uint32_t log2_table, float res_min, float res_max, const float* centre, float radius);
 hipError_t launch_grid_mirror(const float* grid_master, uint32_t* grid_bf, size_t n_entries, hipStream_t stream);
 hipError_t launch_mlp_eval(const MlpView& v, const float4* x, const float4* nrm, const float4* wo, uint32_t n,
                            float4* mix, int cu_count, hipStream_t stream);

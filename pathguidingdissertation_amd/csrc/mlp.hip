@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 
 #include "device_math.h"
 #include "guide.h"
@@ -634,8 +635,12 @@ __global__ __launch_bounds__(kMlpBlock) void k_mlp_eval(MlpView v, const float4*
 }
 
 // Mixture for every shading point of a bounce, by queue slot (c = 0 where the
-// point is not guided: misses, lights, specular materials).
-__global__ __launch_bounds__(kMlpBlock) void k_mlp_infer(DevScene s, PathState ps, MlpView v, int parity, float4* mix) {
+// point is not guided: misses, lights, specular materials). With MULTI (batches of
+// views over several scenes) only the slots whose path belongs to scene `scene` are
+// written, with that scene's network; the other slots are left as they are.
+template <bool MULTI>
+__global__ __launch_bounds__(kMlpBlock) void k_mlp_infer(DevScene s0, PathState ps, MlpView v, int parity, float4* mix,
+                                                         uint32_t scene) {
     __shared__ __align__(16) uint16_t sw[kMirror];
     __shared__ float sb[kBiasCount];
     __shared__ float stage[4 * 32 * kStagePitch];
@@ -648,12 +653,21 @@ __global__ __launch_bounds__(kMlpBlock) void k_mlp_infer(DevScene s, PathState p
     for (uint32_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
         const uint32_t base = tile * 128u + (uint32_t)wv * 32u;
         u16x8 x[3][2];
-        bool valid_nb[2];
+        bool valid_nb[2], mine_nb[2];
 #pragma unroll
         for (int nb = 0; nb < 2; ++nb) {
             uint32_t i = base + nb * 16 + c;
-            bool valid = i < n;
+            bool valid = i < n, mine = true;
             F3 xp{0.f, 0.f, 0.f}, sn{0.f, 0.f, 1.f}, wo{0.f, 0.f, 1.f};
+            const DevScene* sp = &s0;
+            if (MULTI && valid) {
+                const uint32_t path = ps.queue[parity][i];
+                const uint32_t sid = ps.bviews[ps.path_view[path]].scene;
+                mine = sid == scene;
+                sp = &ps.bscenes[sid];
+                valid = mine;
+            }
+            const DevScene& s = *sp;
             if (valid) {
                 float4 h = ps.hit[i];
                 uint32_t hid = __float_as_uint(h.x);
@@ -674,6 +688,7 @@ __global__ __launch_bounds__(kMlpBlock) void k_mlp_infer(DevScene s, PathState p
                 }
             }
             valid_nb[nb] = valid;
+            mine_nb[nb] = mine;
             float p01[3];
             scene_unit(v, xp, p01);
             u16x8 e[3];
@@ -684,16 +699,18 @@ __global__ __launch_bounds__(kMlpBlock) void k_mlp_infer(DevScene s, PathState p
         FwdOut f;
         tile_forward(sw, sb, x, f, nullptr, base, n, lane);
         stage_outputs(f, sb, st, lane);
-        // lanes 0..15 hold the validity of samples nb*16 + c: share it through the staging pad
+        // lanes 0..15 hold the validity of samples nb*16 + c: share it through the staging
+        // pad (1 guided, 0 unguided point, 2 another scene's slot)
         if (g == 0) {
-            st[c * kStagePitch + 48] = valid_nb[0] ? 1.f : 0.f;
-            st[(16 + c) * kStagePitch + 48] = valid_nb[1] ? 1.f : 0.f;
+            st[c * kStagePitch + 48] = !mine_nb[0] ? 2.f : valid_nb[0] ? 1.f : 0.f;
+            st[(16 + c) * kStagePitch + 48] = !mine_nb[1] ? 2.f : valid_nb[1] ? 1.f : 0.f;
         }
         __syncthreads();
         if (lane < 32 && base + lane < n) {
             float4* mx = mix + (size_t)(base + lane) * kMixF4;
-            if (st[lane * kStagePitch + 48] != 0.f) head_mixture(st + lane * kStagePitch, mx);
-            else mx[12] = make_float4(0.f, 0.f, 0.f, 0.f);
+            const float flag = st[lane * kStagePitch + 48];
+            if (flag == 1.f) head_mixture(st + lane * kStagePitch, mx);
+            else if (flag == 0.f) mx[12] = make_float4(0.f, 0.f, 0.f, 0.f);
         }
         __syncthreads();
     }
@@ -805,9 +822,42 @@ hipError_t launch_mlp_eval(const MlpView& v, const float4* x, const float4* nrm,
 }
 
 hipError_t launch_mlp_infer(const DevScene& s, const PathState& ps, const MlpView& v, int parity, float4* mix,
-                            uint32_t grid, hipStream_t stream) {
-    hipLaunchKernelGGL(k_mlp_infer, dim3(grid), dim3(kMlpBlock), 0, stream, s, ps, v, parity, mix);
+                            uint32_t grid, hipStream_t stream, bool multi, uint32_t scene) {
+    if (multi)
+        hipLaunchKernelGGL(k_mlp_infer<true>, dim3(grid), dim3(kMlpBlock), 0, stream, s, ps, v, parity, mix, scene);
+    else
+        hipLaunchKernelGGL(k_mlp_infer<false>, dim3(grid), dim3(kMlpBlock), 0, stream, s, ps, v, parity, mix, 0u);
     return hipGetLastError();
+}
+
+// c = 0 (unguided) for every slot of the current queue: batches clear before the
+// per-scene inference launches, so slots of scenes without a network stay unguided.
+__global__ void k_mix_clear(const uint32_t* counters, int parity, float4* mix) {
+    const uint32_t n = counters[parity];
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        mix[(size_t)i * kMixF4 + 12] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+hipError_t launch_mix_clear(const PathState& ps, int parity, float4* mix, uint32_t grid, hipStream_t stream) {
+    hipLaunchKernelGGL(k_mix_clear, dim3(grid), dim3(256), 0, stream, ps.counters, parity, mix);
+    return hipGetLastError();
+}
+
+void mlp_view_frame(MlpView& v, uint32_t log2_table, float res_min, float res_max, const float* centre, float radius) {
+    // levels res_l = floor(res_min b^l), b = (res_max / res_min)^(1/15); dense indexing
+    // while (res_l + 1)^3 fits the table
+    const size_t T = (size_t)1 << log2_table;
+    v.table_mask = (uint32_t)(T - 1);
+    v.log2_table = log2_table;
+    const double b = std::exp((std::log((double)res_max) - std::log((double)res_min)) / (kHashLevels - 1));
+    for (int l = 0; l < kHashLevels; ++l) {
+        double r = std::floor((double)res_min * std::pow(b, (double)l) + 1e-6);
+        v.res[l] = (uint32_t)std::max(1.0, r);
+        double cells = std::pow((double)v.res[l] + 1.0, 3.0);
+        v.dense[l] = cells <= (double)T ? 1u : 0u;
+    }
+    for (int a = 0; a < 3; ++a) v.lo[a] = centre[a] - radius;
+    v.inv_ext = radius > 0.f ? 1.f / (2.f * radius) : 0.f;
 }
 
 hipError_t launch_mix_query(const float4* mix, uint32_t n, const float4* wi_eval, const float4* u, float* pdf_eval,

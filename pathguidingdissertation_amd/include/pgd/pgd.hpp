@@ -190,6 +190,13 @@ public:
         check(pg_mlp_info_read(dev_->get(), &i), dev_->get());
         return i;
     }
+    // FP32 network parameters and hash grid (for pg_batch_set_guide or a checkpoint).
+    std::pair<std::vector<float>, std::vector<float>> params() const {
+        const pg_mlp_info i = info();
+        std::vector<float> p(i.n_params), g(i.n_grid_params);
+        check(pg_mlp_get_params(dev_->get(), p.data(), g.data()), dev_->get());
+        return {std::move(p), std::move(g)};
+    }
 
 private:
     Device* dev_;
@@ -221,6 +228,11 @@ public:
         double ms = 0.0;
         check(pg_batch_autotune(b_, &p, &c, &ms), dev_->get());
         return {c, ms};
+    }
+    // Per-scene learned guide; render with params.integrator = PG_INTEGRATOR_GUIDED_MLP.
+    void set_guide(uint32_t scene, const pg_mlp_config& cfg, const std::vector<float>& params,
+                   const std::vector<float>& grid) {
+        check(pg_batch_set_guide(b_, scene, &cfg, params.data(), grid.data()), dev_->get());
     }
     void clear() { check(pg_batch_clear(b_), dev_->get()); }
     void render(uint32_t spp, pg_render_params p = pg_render_params_default()) {

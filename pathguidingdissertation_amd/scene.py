@@ -407,6 +407,15 @@ class Batch:
         self.views.append((scene_id, int(camera["width"]), int(camera["height"])))
         return vid.value
 
+    def set_guide(self, scene_id: int, params: np.ndarray, grid: np.ndarray, **cfg):
+        """Learned guide of one scene (pg_batch_set_guide): the network parameters and
+        hash grid as Device.mlp_params() returns them, and the configuration keywords
+        they were trained with (Device.mlp_init's)."""
+        c = self.device.mlp_config(**cfg)
+        p = np.ascontiguousarray(params, np.float32)
+        g = np.ascontiguousarray(grid, np.float32)
+        self.device._chk(self.lib.pg_batch_set_guide(self.h, int(scene_id), C.byref(c), _fp(p), _fp(g)))
+
     def configure(self, **kw):
         cfg = _lib.PgBatchConfig(0, 0, 0, 0, 1, 0)
         for k, v in kw.items():

@@ -116,3 +116,58 @@ def test_batch_errors(pg, device, cornell):
     with pytest.raises(pg.PgError):
         b.configure(stack=65)
     b.close()
+
+
+GUIDE_CFG = dict(seed=3, log2_table=15, res_min=8.0, res_max=256.0)
+
+
+def _trained_guide(pg, device, data, seed):
+    """Online-trained network of one scene (8 guided record passes, one Adam step each)."""
+    sc = pg.Scene(device, data)
+    device.mlp_init(**dict(GUIDE_CFG, seed=seed))
+    rt = pg.RayTracer(sc, seed=seed, max_depth=12, record_nee=1)
+    rt.clear()
+    rt.render_online(8, steps=1)
+    assert device.mlp_info()["diverged"] == 0
+    return device.mlp_params()
+
+
+def _single_guided(pg, device, data, cam, guide, seed, spp):
+    sc = pg.Scene(device, dataclasses.replace(data, camera=cam))
+    device.mlp_init(**dict(GUIDE_CFG, seed=guide[2]))
+    device.mlp_set_params(guide[0], guide[1])
+    rt = pg.RayTracer(sc, seed=seed, max_depth=12, integrator=3)
+    rt.clear()
+    rt.render(spp)
+    return rt.film()[0]
+
+
+def test_batch_learned_guides_equal_single_renders(pg, device, cornell, small_interior, views):
+    """Per-scene learned guides in one batch (both scenes' views share a chunk, so each
+    bounce runs one inference launch per scene over the mixed queue): every view equals
+    the plain learned-guide pass of that scene with the same network, bit for bit, with
+    and without graph replay; a scene without a guide renders unguided."""
+    scenes = [cornell, small_interior]
+    guides = [(*_trained_guide(pg, device, d, s), s) for d, s in ((cornell, 11), (small_interior, 12))]
+    for use_graph in (1, 0):
+        b = _batch(pg, device, cornell, small_interior, views, use_graph=use_graph)
+        with pytest.raises(pg._lib.PgError):
+            b.render(1, seed=7, max_depth=12, integrator=3)  # no guide yet
+        for sid in (0, 1):
+            b.set_guide(sid, guides[sid][0], guides[sid][1], **dict(GUIDE_CFG, seed=guides[sid][2]))
+        b.render(2, seed=7, max_depth=12, integrator=3)
+        films = [b.film(v)[0] for v in range(len(views))]
+        assert b.stats()["chunks"] == 1
+        b.close()
+        for v, (sid, cam) in enumerate(views):
+            ref = _single_guided(pg, device, scenes[sid], cam, guides[sid], 7, 2)
+            assert np.array_equal(films[v], ref), f"view {v} (graph {use_graph}) differs from its single render"
+    # only the interior guided: the Cornell views equal unguided single renders
+    b = _batch(pg, device, cornell, small_interior, views, use_graph=1)
+    b.set_guide(1, guides[1][0], guides[1][1], **dict(GUIDE_CFG, seed=guides[1][2]))
+    b.render(2, seed=7, max_depth=12, integrator=3)
+    f0 = b.film(0)[0]
+    f1 = b.film(1)[0]
+    b.close()
+    assert np.array_equal(f0, _single(pg, device, cornell, views[0][1], 2, 7))
+    assert np.array_equal(f1, _single_guided(pg, device, small_interior, views[1][1], guides[1], 7, 2))
