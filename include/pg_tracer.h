@@ -35,16 +35,20 @@ typedef struct pg_ctx pg_ctx;
 /* ------------------------------------------------------------------ data formats */
 
 /* Material (Materials.h:121-897; parameters as SceneLoader.h:104-181 reads them).
- * albedo is a constant colour (procedural scenes carry no textures).            */
+ * albedo is a constant colour unless pg_scene_textures gives the material an
+ * albedo texture (the reference's Texture* albedo, Imaging.h:82-108).
+ * PG_BSDF_LAYERED added to a base type is LayeredBSDF over that base (Materials.h:
+ * 855-897: every call forwards to the base, isTwoSided is true).                */
 enum { PG_BSDF_DIFFUSE = 0, PG_BSDF_MIRROR = 1, PG_BSDF_CONDUCTOR = 2, PG_BSDF_GLASS = 3,
-       PG_BSDF_DIELECTRIC = 4 };
+       PG_BSDF_DIELECTRIC = 4, PG_BSDF_ORENNAYAR = 5, PG_BSDF_PLASTIC = 6, PG_BSDF_LAYERED = 8 };
 typedef struct {
     float type;          /* PG_BSDF_* stored as float so the record is 16 floats   */
     float albedo[3];
     float emission[3];   /* > 0 luminance makes every triangle an AreaLight        */
     float eta[3], k[3];  /* conductor                                              */
-    float int_ior, ext_ior;
-    float roughness;     /* alpha = 1.62142 * sqrt(roughness) (Materials.h:249)    */
+    float int_ior, ext_ior; /* glass, dielectric, plastic                          */
+    float roughness;     /* alpha = 1.62142 * sqrt(roughness) (Materials.h:249, 738);
+                            Oren-Nayar: sigma itself (Materials.h:649-652)         */
 } pg_material;
 
 /* Pinhole camera as the scene loader builds it (SceneLoader.h:226-244). */
@@ -124,6 +128,15 @@ pg_status pg_scene_upload(pg_ctx* ctx, const float* positions, const float* norm
                           const pg_material* materials, uint32_t n_materials,
                           const float* env_rgb, int32_t env_width, int32_t env_height,
                           const pg_camera_desc* camera);
+/* Albedo textures of the uploaded scene (Texture::sample, Imaging.h:82-108, read at
+ * the hit's interpolated uv, Geometry.h:132-137): uv holds n_tris*6 floats (u0 v0 u1
+ * v1 u2 v2 per triangle, upload order); material_texture one index per material
+ * (-1 keeps the constant albedo); texels the RGB of every texture concatenated,
+ * texture i starting at texel texel_offset[i], tex_width[i] x tex_height[i]. A new
+ * pg_scene_upload drops them. Needs an uploaded scene (PG_ERR_STATE).            */
+pg_status pg_scene_textures(pg_ctx* ctx, const float* uv, const int32_t* material_texture, uint32_t n_materials,
+                            const float* texels, const uint64_t* texel_offset, const int32_t* tex_width,
+                            const int32_t* tex_height, uint32_t n_textures);
 /* BVH statistics: node count, maximum leaf depth, build milliseconds. */
 pg_status pg_bvh_info(pg_ctx* ctx, uint32_t* n_nodes, uint32_t* max_depth, double* build_ms);
 /* Export the topology as built (8 floats per node: min.xyz, leftFirst bits,
@@ -158,6 +171,11 @@ pg_status pg_render_spp(pg_ctx* ctx, const pg_render_params* params, uint32_t n_
 pg_status pg_render_tune(pg_ctx* ctx, const pg_render_params* params, pg_render_params* tuned,
                          double* ms_per_spp);
 pg_status pg_film_read(pg_ctx* ctx, float* rgb_sum, uint32_t* spp);
+/* Film::tonemap (Imaging.h:220-244) of the whole film on the device: mean over the
+ * film's SPP, exposure, extended Reinhard with white point lmax_white, gamma 1/2.2,
+ * 8-bit RGB (width*height*3 bytes) — the canvas RayTracer::savePNG writes. The
+ * reference calls it with exposure 1 and LmaxWhite 10 (Renderer.h:705).          */
+pg_status pg_film_tonemap(pg_ctx* ctx, float exposure, float lmax_white, uint8_t* rgb8);
 pg_status pg_stats_read(pg_ctx* ctx, pg_stats* stats);
 pg_status pg_stats_reset(pg_ctx* ctx);
 /* Instrumented traversal: count node visits / triangle tests per ray (slower). */
@@ -352,6 +370,33 @@ pg_status pg_scene_copy(const pg_host_scene* s, float* positions, float* normals
                         uint32_t* material_index, pg_material* materials, float* env_rgb,
                         pg_camera_desc* camera);
 pg_status pg_scene_free(pg_host_scene* s);
+
+/* loadScene (SceneLoader.h:185-269): scene.json + .gem static meshes + PNG / Radiance
+ * HDR textures of a reference scene directory, into a host scene. Materials follow
+ * loadInstance (SceneLoader.h:104-181), one per instance; an unknown bsdf skips the
+ * instance as the reference does; every material gets its reflectance texture (a
+ * file that does not decode becomes the 1x1 white default, Imaging.h:36-42).
+ * flags: PG_LOAD_SKIP_MISSING skips instances whose mesh file is missing or not a
+ * GEM file (the reference exits instead). Errors: PG_ERR_INVALID with the reason in
+ * pg_scene_load_error(). Only 8-bit non-interlaced PNGs are decoded.             */
+enum { PG_LOAD_SKIP_MISSING = 1 };
+pg_status pg_scene_load(const char* scene_dir, uint32_t flags, pg_host_scene** out);
+const char* pg_scene_load_error(void);
+pg_status pg_scene_texture_sizes(const pg_host_scene* s, uint32_t* n_textures, uint64_t* n_texels,
+                                 uint32_t* n_warnings);
+/* uv: n_tris*6 (zeros for procedural scenes); material_texture: n_materials (-1 for
+ * procedural scenes); texels: n_texels*3; per texture: first texel, width, height. */
+pg_status pg_scene_copy_textures(const pg_host_scene* s, float* uv, int32_t* material_texture, float* texels,
+                                 uint64_t* texel_offset, int32_t* tex_width, int32_t* tex_height);
+const char* pg_scene_warning(const pg_host_scene* s, uint32_t i);
+/* Decode one image as Texture::load's decoder does (8-bit PNG samples as 0..255
+ * floats, Radiance HDR as floats), `channels` values per pixel; rgb nullable.     */
+pg_status pg_image_read(const char* path, int32_t* width, int32_t* height, int32_t* channels, float* rgb);
+/* RayTracer::savePNG (Renderer.h:751): 8-bit RGB PNG. */
+pg_status pg_image_write_png(const char* path, const uint8_t* rgb, int32_t width, int32_t height);
+/* RayTracer::saveHDR / Film::save (Renderer.h:750, Imaging.h:263-272): Radiance RGBE
+ * of an RGB float image (pass the film divided by its SPP).                       */
+pg_status pg_image_write_hdr(const char* path, const float* rgb, int32_t width, int32_t height);
 
 /* Host BVH build alone (Geometry.h:419-446) — for topology parity checks. */
 pg_status pg_bvh_build_host(const float* positions, uint32_t n_tris, uint32_t* n_nodes,

@@ -410,6 +410,7 @@ struct Ray {                                                                   /
 
 struct Tri {                                                                   // Geometry.h:61-154
     V3 p[3], nrm[3];
+    float uv[3][2] = {{0.f, 0.f}, {0.f, 0.f}, {0.f, 0.f}};                     // Vertex::u, v
     V3 n; float area = 0.f; uint32_t mat = 0;
     void init(const V3& a, const V3& b, const V3& c, const V3& na, const V3& nb, const V3& nc, uint32_t m) {
         p[0] = a; p[1] = b; p[2] = c; nrm[0] = na; nrm[1] = nb; nrm[2] = nc; mat = m;
@@ -641,18 +642,24 @@ struct BVH {                                                                   /
 };
 
 // ---------------------------------------------------------------- materials (Materials.h)
-enum { M_DIFFUSE = 0, M_MIRROR = 1, M_CONDUCTOR = 2, M_GLASS = 3, M_DIELECTRIC = 4 };
+enum { M_DIFFUSE = 0, M_MIRROR = 1, M_CONDUCTOR = 2, M_GLASS = 3, M_DIELECTRIC = 4, M_ORENNAYAR = 5,
+       M_PLASTIC = 6, M_LAYERED_FLAG = 8 };
 
 struct Material {
     int type = M_DIFFUSE;
     C3 albedo, emission, eta, k;
     float intIOR = 1.33f, extIOR = 1.f, alpha = 0.f;
+    float sigma = 0.f;     // OrenNayarBSDF::sigma (the scene's "alpha", SceneLoader.h:120)
+    bool layered = false;  // LayeredBSDF over this base (Materials.h:855-897)
+    int tex = -1;          // albedo texture (-1: constant albedo)
     void load(const float* f) {
         type = (int)f[0];
+        if (type >= M_LAYERED_FLAG) { layered = true; type -= M_LAYERED_FLAG; }
         albedo = C3(f[1], f[2], f[3]); emission = C3(f[4], f[5], f[6]);
         eta = C3(f[7], f[8], f[9]); k = C3(f[10], f[11], f[12]);
         intIOR = f[13]; extIOR = f[14];
-        alpha = 1.62142f * sqrtf(f[15]);                                       // Materials.h:249, 455
+        alpha = 1.62142f * sqrtf(f[15]);                                       // Materials.h:249, 455, 738
+        sigma = f[15];
     }
     bool isLight() const { return emission.lum() > 0.f; }                      // Materials.h:130
     bool specular() const {
@@ -660,12 +667,34 @@ struct Material {
         if (type == M_CONDUCTOR || type == M_DIELECTRIC) return alpha < kEps;
         return false;
     }
-    bool twoSided() const { return !(type == M_GLASS || type == M_DIELECTRIC); }
+    // LayeredBSDF::isTwoSided is true whatever the base (Materials.h:893)
+    bool twoSided() const { return layered || !(type == M_GLASS || type == M_DIELECTRIC); }
+};
+
+// Texture (Imaging.h:22-145): RGB texels, bilinear lookup with wrap.
+struct Texture {
+    int w = 1, h = 1;
+    std::vector<C3> t;
+    C3 sample(float tu, float tv) const {                                       // Imaging.h:82-108
+        float u = smax(0.f, fabsf(tu)) * w;
+        float v = smax(0.f, fabsf(tv)) * h;
+        int x = (int)floorf(u), y = (int)floorf(v);
+        float fu = u - x, fv = v - y;
+        float w0 = (1.f - fu) * (1.f - fv), w1 = fu * (1.f - fv), w2 = (1.f - fu) * fv, w3 = fu * fv;
+        x = x % w; y = y % h;
+        const C3& s0 = t[(size_t)y * w + x];
+        const C3& s1 = t[(size_t)y * w + ((x + 1) % w)];
+        const C3& s2 = t[(size_t)((y + 1) % h) * w + x];
+        const C3& s3 = t[(size_t)((y + 1) % h) * w + ((x + 1) % w)];
+        return ((s0 * w0 + s1 * w1) + s2 * w2) + s3 * w3;
+    }
 };
 
 struct SD {                                                                    // Materials.h:12-32
     V3 x, wo, sN, gN;
     float t = FLT_MAX;
+    float tu = 0.f, tv = 0.f;
+    C3 albedo;  // the material's albedo texture at (tu, tv), looked up once per hit
     Frame frame;
     const Material* bsdf = nullptr;
 };
@@ -715,13 +744,68 @@ static float Dggx(const V3& h, float alpha) {                                 //
     return a2 / den;
 }
 
+// Oren-Nayar (Materials.h:641-721): Lambert times the qualitative Oren-Nayar factor.
+// No hemisphere test in evaluate (the reference has none); the pdf is the cosine pdf.
+static C3 orenNayarEval(const SD& sd, const V3& wi) {
+    const Material& m = *sd.bsdf;
+    V3 ol = sd.frame.toLocal(sd.wo), wl = sd.frame.toLocal(wi);
+    float s2 = m.sigma * m.sigma;
+    float A = 1.f - (s2 / (2 * (s2 + 0.33f)));
+    float B = (0.45f * s2) / (s2 + 0.09f);
+    float phiI = sphericalPhi(wl), phiO = sphericalPhi(ol);
+    float thI = acosf(wl.z), thO = acosf(ol.z);
+    float c = cosf(phiI - phiO);
+    float sa = sinf(smax(thI, thO));
+    float tb = tanf(smin(thI, thO));
+    float on = A + (((B * smax(0.f, c)) * sa) * tb);
+    return (sd.albedo * (float)M_1_PI) * on;
+}
+// Plastic = Fresnel-weighted Lambert + normalised Blinn-Phong (Materials.h:723-853).
+static float plasticExponent(const Material& m) {                               // Materials.h:738-740
+    float a = smax(m.alpha, 0.001f);
+    return (2.f / (a * a)) - 2.f;
+}
+static C3 plasticEval(const SD& sd, const V3& wi) {                             // Materials.h:784-818
+    const Material& m = *sd.bsdf;
+    V3 wl = sd.frame.toLocal(wi), ol = sd.frame.toLocal(sd.wo);
+    if (wl.z <= 0.f || ol.z <= 0.f) return C3();
+    V3 h = wl + ol;
+    if (h.lengthSq() < kEps) return C3();
+    h = h.normalize();
+    float e = plasticExponent(m);
+    float ks = fresnelDielectric(dot(ol, h), m.intIOR, m.extIOR);
+    float kd = 1.f - ks;
+    float norm = (float)((double)(e + 2.f) / (4.f * M_PI * (double)(2.f - powf(2.f, (-e * 0.5f)))));
+    float lo = (float)((double)(e + 2.f) / (8.f * M_PI)), hi = (float)((double)(e + 4.f) / (8.f * M_PI));
+    norm = smin(smax(lo, norm), hi);
+    float g = powf(smax(h.z, 0.f), e);
+    C3 diff = sd.albedo * (float)M_1_PI;
+    C3 glossy = dot(ol, h) <= 0.f ? C3() : C3(1.f, 1.f, 1.f) * g;
+    return (diff * kd) + ((glossy * norm) * ks);
+}
+static float plasticPdf(const SD& sd, const V3& wi) {                           // Materials.h:820-848
+    const Material& m = *sd.bsdf;
+    V3 wl = sd.frame.toLocal(wi), ol = sd.frame.toLocal(sd.wo);
+    if (wl.z <= 0.f || ol.z <= 0.f) return 0.f;
+    V3 h = wl + ol;
+    if (h.lengthSq() < kEps) return 0.f;
+    h = h.normalize();
+    float e = plasticExponent(m);
+    float ks = fresnelDielectric(dot(ol, h), m.intIOR, m.extIOR);
+    float kd = 1.f - ks;
+    float ph = (float)(((double)(e + 1.f) / (2.f * M_PI)) * (double)powf(smax(h.z, 0.f), e));
+    float gp = dot(ol, h) <= 0.f ? 0.f : (ph / (4.f * dot(ol, h)));
+    float dp = cosineHemispherePDF(wl);
+    return (kd * dp) + (ks * gp);
+}
+
 static C3 bsdfEval(const SD& sd, const V3& wi) {
     const Material& m = *sd.bsdf;
     switch (m.type) {
     case M_DIFFUSE: {                                                          // Materials.h:173-178
         V3 wl = sd.frame.toLocal(wi);
         if (wl.z <= 0.f) return C3();
-        return m.albedo * kInvPiF;
+        return sd.albedo * kInvPiF;
     }
     case M_CONDUCTOR: {                                                        // Materials.h:308-331
         V3 wl = sd.frame.toLocal(wi), ol = sd.frame.toLocal(sd.wo);
@@ -734,7 +818,7 @@ static C3 bsdfEval(const SD& sd, const V3& wi) {
         if (wl.z * ol.z <= 0.f) return C3();
         float G = Gggx(wl, ol, m.alpha), D = Dggx(wm, m.alpha);
         C3 F = fresnelConductor(dot(ol, wm), m.eta, m.k);
-        return m.albedo * (((F * G) * D) / ((4.f * wl.z) * ol.z));
+        return sd.albedo * (((F * G) * D) / ((4.f * wl.z) * ol.z));
     }
     case M_DIELECTRIC: {                                                       // Materials.h:545-591
         V3 ol = sd.frame.toLocal(sd.wo), wl = sd.frame.toLocal(wi);
@@ -753,15 +837,17 @@ static C3 bsdfEval(const SD& sd, const V3& wi) {
             if (dot(ol, wm) <= 0.f) return C3();
             if (wl.z * ol.z == 0.f) return C3();
             float G = Gggx(wl, ol, m.alpha), D = Dggx(wm, m.alpha);
-            return m.albedo * (((F * G) * D) / ((4.f * wl.z) * ol.z));
+            return sd.albedo * (((F * G) * D) / ((4.f * wl.z) * ol.z));
         }
         float s = dot(wl, wm) + dot(ol, wm) / ep;
         float den = s * s;
         if (den <= 0.f) return C3();
         float G = Gggx(wl, ol, m.alpha), D = Dggx(wm, m.alpha);
-        return ((m.albedo * ((((1.f - F) * G) * D) / den)) * (fabsf(dot(wl, wm)) / fabsf(wl.z))) *
+        return ((sd.albedo * ((((1.f - F) * G) * D) / den)) * (fabsf(dot(wl, wm)) / fabsf(wl.z))) *
                (fabsf(dot(ol, wm)) / fabsf(ol.z));
     }
+    case M_ORENNAYAR: return orenNayarEval(sd, wi);
+    case M_PLASTIC: return plasticEval(sd, wi);
     default: return C3();                                                      // mirror/glass: delta
     }
 }
@@ -770,6 +856,8 @@ static float bsdfPdf(const SD& sd, const V3& wi) {
     const Material& m = *sd.bsdf;
     switch (m.type) {
     case M_DIFFUSE: return cosineHemispherePDF(sd.frame.toLocal(wi));          // Materials.h:180-184
+    case M_ORENNAYAR: return cosineHemispherePDF(sd.frame.toLocal(wi));        // Materials.h:709-714
+    case M_PLASTIC: return plasticPdf(sd, wi);
     case M_CONDUCTOR: {                                                        // Materials.h:333-353
         V3 wl = sd.frame.toLocal(wi), ol = sd.frame.toLocal(sd.wo);
         if (wl.z <= 0.f || ol.z <= 0.f) return 0.f;
@@ -820,11 +908,36 @@ static V3 bsdfSample(const SD& sd, Sampler* s, C3& f, float& pdf) {
         f = bsdfEval(sd, wi); pdf = bsdfPdf(sd, wi);
         return wi;
     }
+    case M_ORENNAYAR: {                                                        // Materials.h:655-670
+        float r1 = s->next(), r2 = s->next();
+        V3 wi = sd.frame.toWorld(cosineSampleHemisphere(r1, r2));
+        f = bsdfEval(sd, wi); pdf = bsdfPdf(sd, wi);
+        return wi;
+    }
+    case M_PLASTIC: {                                                          // Materials.h:742-778
+        V3 ol = sd.frame.toLocal(sd.wo);
+        V3 wl;
+        float r1 = s->next(), r2 = s->next();
+        float F = fresnelDielectric(fabsf(ol.z), m.intIOR, m.extIOR);
+        if (s->next() < F) {
+            float e = plasticExponent(m);
+            float base = 1.f - r1, power = 1.f / (e + 1.f);
+            float thetaH = acosf(powf(base, power));
+            float phiH = (float)(2.f * M_PI * r2);
+            V3 h = sphericalToWorld(thetaH, phiH).normalize();
+            wl = ((h * 2.f) * dot(ol, h)) - ol;
+        } else {
+            wl = cosineSampleHemisphere(r1, r2);
+        }
+        V3 wi = sd.frame.toWorld(wl);
+        f = bsdfEval(sd, wi); pdf = bsdfPdf(sd, wi);
+        return wi;
+    }
     case M_MIRROR: {                                                           // Materials.h:201-215
         V3 ol = sd.frame.toLocal(sd.wo);
         V3 wr(-ol.x, -ol.y, ol.z);
         if (fabsf(wr.z) <= kEps) { pdf = 0.f; f = C3(); return V3(0.f, 0.f, 1.f); }
-        f = m.albedo / wr.z; pdf = 1.f;
+        f = sd.albedo / wr.z; pdf = 1.f;
         return sd.frame.toWorld(wr);
     }
     case M_CONDUCTOR: {                                                        // Materials.h:253-285
@@ -833,7 +946,7 @@ static V3 bsdfSample(const SD& sd, Sampler* s, C3& f, float& pdf) {
         if (m.alpha < kEps) {
             V3 wr(-ol.x, -ol.y, ol.z);
             C3 F = fresnelConductor(wr.z, m.eta, m.k);
-            f = (m.albedo / fabsf(wr.z)) * F; pdf = 1.f;
+            f = (sd.albedo / fabsf(wr.z)) * F; pdf = 1.f;
             return sd.frame.toWorld(wr);
         }
         float a2 = m.alpha * m.alpha;
@@ -855,20 +968,20 @@ static V3 bsdfSample(const SD& sd, Sampler* s, C3& f, float& pdf) {
         if (ci < 0.f) ci = fabsf(ci);
         if (s->next() < F) {
             V3 wr(-ol.x, -ol.y, ol.z);
-            pdf = F; f = (m.albedo / fabsf(wr.z)) * F;
+            pdf = F; f = (sd.albedo / fabsf(wr.z)) * F;
             return sd.frame.toWorld(wr);
         }
         float ci2 = smax(ci * ci, 0.f), si2 = smax(1.f - ci2, 0.f);
         float eta2 = eta * eta, st2 = eta2 * si2;
         if (st2 > 1.f) {
             V3 wr(-ol.x, -ol.y, ol.z);
-            pdf = F; f = (m.albedo / fabsf(wr.z)) * F;
+            pdf = F; f = (sd.albedo / fabsf(wr.z)) * F;
             return sd.frame.toWorld(wr);
         }
         float ct = sqrtf(smax(1.f - st2, 0.f));
         float wz = (ol.z < 0.f) ? ct : -ct;
         V3 wt(-eta * ol.x, -eta * ol.y, wz);
-        pdf = 1.f - F; f = ((m.albedo / fabsf(wt.z)) * (1.f - F)) * eta2;
+        pdf = 1.f - F; f = ((sd.albedo / fabsf(wt.z)) * (1.f - F)) * eta2;
         return sd.frame.toWorld(wt);
     }
     case M_DIELECTRIC: {                                                       // Materials.h:459-539
@@ -877,7 +990,7 @@ static V3 bsdfSample(const SD& sd, Sampler* s, C3& f, float& pdf) {
             float F = fresnelDielectric(ol.z, m.intIOR, m.extIOR);
             if (s->next() < F) {
                 V3 wl(-ol.x, -ol.y, ol.z);
-                pdf = F; f = (m.albedo / fabsf(wl.z)) * F;
+                pdf = F; f = (sd.albedo / fabsf(wl.z)) * F;
                 return sd.frame.toWorld(wl);
             }
             float eta = ol.z > 0.f ? m.extIOR / m.intIOR : m.intIOR / m.extIOR;
@@ -886,12 +999,12 @@ static V3 bsdfSample(const SD& sd, Sampler* s, C3& f, float& pdf) {
             float ci2 = smax(ol.z * ol.z, 0.f), si2 = smax(1.f - ci2, 0.f), st2 = eta2 * si2;
             if (st2 > 1.f) {
                 V3 wr(-ol.x, -ol.y, ol.z);
-                pdf = F; f = (m.albedo / fabsf(wr.z)) * F;
+                pdf = F; f = (sd.albedo / fabsf(wr.z)) * F;
                 return sd.frame.toWorld(wr);
             }
             float ct = sqrtf(smax(1.f - st2, 0.f));
             V3 wl(-eta * ol.x, -eta * ol.y, ct * sg);
-            pdf = 1.f - F; f = (m.albedo / fabsf(wl.z)) * (1.f - F);
+            pdf = 1.f - F; f = (sd.albedo / fabsf(wl.z)) * (1.f - F);
             return sd.frame.toWorld(wl);
         }
         float a2 = m.alpha * m.alpha;
@@ -927,7 +1040,7 @@ static V3 bsdfSample(const SD& sd, Sampler* s, C3& f, float& pdf) {
 static void bsdfInvert(const SD& sd, const V3& wi, float& u, float& v) {
     const Material& m = *sd.bsdf;
     u = 0.f; v = 0.f;
-    if (m.type == M_DIFFUSE) {                                                 // Materials.h:163-171
+    if (m.type == M_DIFFUSE || m.type == M_ORENNAYAR) {                        // Materials.h:163-171, 672-681
         V3 wl = sd.frame.toLocal(wi);
         float theta = acosf(wl.z), phi = sphericalPhi(wl);
         float c = cosf(theta);
@@ -961,6 +1074,7 @@ struct Scene {
     // camera (Scene.h:11-72)
     Mat4 invProj, cam; V3 origin; float width = 0.f, height = 0.f;
     std::vector<RecordOut> records;
+    std::vector<Texture> texs;
     int envMode = 0;
     int envVisibility = 0;  // 1: environment shadow rays along wi (not Renderer.h:331)
 
@@ -993,7 +1107,10 @@ struct Scene {
             sd.x = r.at(h.t);
             sd.gN = t.gNormal();
             sd.sN = ((t.nrm[0] * h.a + t.nrm[1] * h.b) + t.nrm[2] * h.g).normalize();
+            sd.tu = (t.uv[0][0] * h.a + t.uv[1][0] * h.b) + t.uv[2][0] * h.g;     // Geometry.h:135-136
+            sd.tv = (t.uv[0][1] * h.a + t.uv[1][1] * h.b) + t.uv[2][1] * h.g;
             sd.bsdf = &mats[t.mat];
+            sd.albedo = sd.bsdf->tex >= 0 ? texs[sd.bsdf->tex].sample(sd.tu, sd.tv) : sd.bsdf->albedo;
             sd.wo = -r.d;
             if (sd.bsdf->twoSided()) {
                 if (dot(sd.wo, sd.sN) < 0.f) sd.sN = -sd.sN;
@@ -1287,6 +1404,60 @@ orc_scene* orc_scene_create(const float* pos, const float* nrm, const uint32_t* 
     return o;
 }
 
+void orc_scene_set_textures(orc_scene* o, const float* uv, const int32_t* mat_tex, uint32_t n_mats,
+                            const float* texels, const uint64_t* offset, const int32_t* w, const int32_t* h,
+                            uint32_t n_tex) {
+    Scene& s = o->sc;
+    s.texs.assign(n_tex, Texture());
+    for (uint32_t i = 0; i < n_tex; ++i) {
+        Texture& t = s.texs[i];
+        t.w = w[i]; t.h = h[i];
+        t.t.resize((size_t)t.w * t.h);
+        const float* src = texels + 3 * offset[i];
+        for (size_t k = 0; k < t.t.size(); ++k) t.t[k] = C3(src[3 * k], src[3 * k + 1], src[3 * k + 2]);
+    }
+    for (uint32_t i = 0; i < n_mats && i < s.mats.size(); ++i) s.mats[i].tex = mat_tex ? mat_tex[i] : -1;
+    if (uv)
+        for (size_t i = 0; i < s.tris.size(); ++i)
+            for (int k = 0; k < 3; ++k) { s.tris[i].uv[k][0] = uv[6 * i + 2 * k]; s.tris[i].uv[k][1] = uv[6 * i + 2 * k + 1]; }
+}
+
+void orc_texture_sample(const float* texels, int32_t w, int32_t h, const float* tuv, uint32_t n, float* rgb) {
+    Texture t;
+    t.w = w; t.h = h;
+    t.t.resize((size_t)w * h);
+    for (size_t k = 0; k < t.t.size(); ++k) t.t[k] = C3(texels[3 * k], texels[3 * k + 1], texels[3 * k + 2]);
+    for (uint32_t i = 0; i < n; ++i) {
+        C3 c = t.sample(tuv[2 * i], tuv[2 * i + 1]);
+        rgb[3 * i] = c.r; rgb[3 * i + 1] = c.g; rgb[3 * i + 2] = c.b;
+    }
+}
+
+void orc_bsdf_flags(const float* mat, int32_t* specular, int32_t* two_sided) {
+    Material m;
+    m.load(mat);
+    *specular = m.specular() ? 1 : 0;
+    *two_sided = m.twoSided() ? 1 : 0;
+}
+
+/* Film::tonemap (Imaging.h:220-244): exposure, luminance, extended Reinhard with
+ * white point LmaxWhite, gamma 1/2.2, clamp and truncation to 8 bits.            */
+void orc_tonemap(const float* rgb_sum, uint64_t n, int32_t spp, float exposure, float lmax_white, uint8_t* out) {
+    float inv = (spp > 0) ? 1.f / (float)spp : 1.f;
+    for (uint64_t i = 0; i < n; ++i) {
+        C3 col = (C3(rgb_sum[3 * i], rgb_sum[3 * i + 1], rgb_sum[3 * i + 2]) * inv) * exposure;
+        float lin = col.lum();
+        uint8_t* o = out + 3 * i;
+        if (lin < 1e-4f) { o[0] = o[1] = o[2] = 0; continue; }
+        float lw2 = lmax_white * lmax_white;
+        float lout = lin * (1.f + (lin / lw2)) / (1.f + lin);
+        float k = lout / lin;
+        float g = 1.f / 2.2f;
+        float c[3] = {powf(col.r * k, g), powf(col.g * k, g), powf(col.b * k, g)};
+        for (int j = 0; j < 3; ++j) o[j] = (uint8_t)(smax(0.f, smin(c[j], 1.f)) * 255.f);
+    }
+}
+
 void orc_scene_destroy(orc_scene* s) { delete s; }
 
 uint32_t orc_bvh_node_count(const orc_scene* s) { return (uint32_t)s->sc.bvh.nodes.size(); }
@@ -1477,6 +1648,7 @@ static SD makeSD(const float* mat, const float* sdv, Material& m) {
     sd.x = v3(sdv); sd.wo = v3(sdv + 3); sd.sN = v3(sdv + 6); sd.gN = v3(sdv + 9);
     sd.frame.fromVector(sd.sN);
     sd.bsdf = &m;
+    sd.albedo = m.albedo;
     sd.t = 1.f;
     return sd;
 }

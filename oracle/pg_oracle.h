@@ -20,9 +20,10 @@ extern "C" {
 typedef struct orc_scene orc_scene;
 
 /* Material record: 16 floats
- *   [0] type (0 diffuse, 1 mirror, 2 conductor, 3 glass, 4 dielectric)
+ *   [0] type (0 diffuse, 1 mirror, 2 conductor, 3 glass, 4 dielectric,
+ *       5 Oren-Nayar, 6 plastic; + 8 = LayeredBSDF over that base)
  *   [1..3] albedo  [4..6] emission  [7..9] eta  [10..12] k
- *   [13] intIOR  [14] extIOR  [15] roughness                               */
+ *   [13] intIOR  [14] extIOR  [15] roughness (Oren-Nayar: sigma)           */
 #define ORC_MAT_FLOATS 16
 
 /* cam: from[3], to[3], up[3], fov_deg, width, height (12 floats)
@@ -32,6 +33,14 @@ orc_scene* orc_scene_create(const float* pos, const float* nrm, const uint32_t* 
                             const float* env_rgb, int32_t env_w, int32_t env_h,
                             const float* cam);
 void orc_scene_destroy(orc_scene* s);
+/* Albedo textures (Imaging.h:22-145) and per-vertex uv (Vertex::u, v): uv n_tris*6
+ * (u0 v0 u1 v1 u2 v2), mat_tex per material (-1: constant albedo), texels RGB of
+ * every texture concatenated (texture i starts at texel offset[i], w[i] x h[i]). */
+void orc_scene_set_textures(orc_scene* s, const float* uv, const int32_t* mat_tex, uint32_t n_mats,
+                            const float* texels, const uint64_t* offset, const int32_t* w, const int32_t* h,
+                            uint32_t n_tex);
+/* Texture::sample (Imaging.h:82-108) at n (tu, tv) pairs of one w x h RGB texture. */
+void orc_texture_sample(const float* texels, int32_t w, int32_t h, const float* tuv, uint32_t n, float* rgb);
 
 /* BVH (binned SAH, Geometry.h:293-446). nodes: 8 floats each
  * (min.xyz, leftFirst (as int bits), max.xyz, used (as int bits)). */
@@ -96,6 +105,10 @@ void orc_bsdf_sample(const float* mat, const float* sd, const float* u, float* w
                      float* f_rgb, float* pdf);
 void orc_bsdf_eval(const float* mat, const float* sd, const float* wi, float* f_rgb, float* pdf);
 void orc_bsdf_invert(const float* mat, const float* sd, const float* wi, float* uv);
+/* BSDF::isPureSpecular and isTwoSided of a material record. */
+void orc_bsdf_flags(const float* mat, int32_t* specular, int32_t* two_sided);
+/* Film::tonemap (Imaging.h:220-244) of n accumulated pixels to 8-bit RGB. */
+void orc_tonemap(const float* rgb_sum, uint64_t n, int32_t spp, float exposure, float lmax_white, uint8_t* out);
 float orc_fresnel_dielectric(float cos_theta, float int_ior, float ext_ior);
 void orc_fresnel_conductor(float cos_theta, const float* eta, const float* k, float* out);
 float orc_ggx_d(const float* h, float alpha);

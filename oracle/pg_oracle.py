@@ -106,6 +106,11 @@ def lib():
         L.orc_point_bvh_stats.restype = C.c_int32
         L.orc_point_bvh_stats.argtypes = [_F, C.c_uint32, _I32]
         L.orc_knn_brute.argtypes = [_F, C.c_uint32, _F, C.c_uint32, C.c_uint32, _U32, _F, C.c_int32]
+        L.orc_scene_set_textures.argtypes = [C.c_void_p, _F, _I32, C.c_uint32, _F, C.POINTER(C.c_uint64), _I32,
+                                             _I32, C.c_uint32]
+        L.orc_texture_sample.argtypes = [_F, C.c_int32, C.c_int32, _F, C.c_uint32, _F]
+        L.orc_bsdf_flags.argtypes = [_F, _I32, _I32]
+        L.orc_tonemap.argtypes = [_F, C.c_uint64, C.c_int32, C.c_float, C.c_float, _U8]
         _lib = L
     return _lib
 
@@ -149,6 +154,20 @@ def ref():
         R.ref_env_power.restype = C.c_float
         R.ref_env_power.argtypes = [C.c_void_p]
         R.ref_metrics.argtypes = [_F, _F, C.c_int, C.c_int, C.c_int, C.c_int, _F]
+        R.ref_texture_sample.argtypes = [_F, C.c_int, C.c_int, _F, C.c_int, _F]
+        R.ref_texture_load.restype = C.c_int
+        R.ref_texture_load.argtypes = [C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_int), _F]
+        R.ref_load_scene.restype = C.c_void_p
+        R.ref_load_scene.argtypes = [C.c_char_p]
+        R.ref_loaded_sizes.restype = C.c_int
+        R.ref_loaded_sizes.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        R.ref_loaded_copy.argtypes = [C.c_void_p, _F, _F, _F, _U32, _F, _I32, _F]
+        R.ref_loaded_texture_name.restype = C.c_char_p
+        R.ref_loaded_texture_name.argtypes = [C.c_void_p, C.c_int]
+        R.ref_loaded_free.argtypes = [C.c_void_p]
+        R.ref_tonemap.argtypes = [_F, C.c_int, C.c_int, C.c_float, C.c_float, _U8]
+        R.ref_write_hdr.restype = C.c_int
+        R.ref_write_hdr.argtypes = [C.c_char_p, _F, C.c_int, C.c_int]
         _ref = R
     return _ref
 
@@ -168,6 +187,12 @@ class OracleScene:
         self.n_tris = pos.shape[0]
         self.h = L.orc_scene_create(fp(pos), fp(nrm), up(mi), pos.shape[0], fp(mats), mats.shape[0], fp(env),
                                     0 if env is None else env.shape[1], 0 if env is None else env.shape[0], fp(cam))
+        if getattr(data, "has_textures", False):
+            uv, mt, texels, off, tw, th = data.texture_arrays()
+            self._keep += [uv, mt, texels, off, tw, th]
+            L.orc_scene_set_textures(self.h, fp(uv), mt.ctypes.data_as(_I32), mt.shape[0], fp(texels),
+                                     off.ctypes.data_as(C.POINTER(C.c_uint64)), tw.ctypes.data_as(_I32),
+                                     th.ctypes.data_as(_I32), len(data.textures))
 
     def __del__(self):
         try:

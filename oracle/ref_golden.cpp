@@ -23,11 +23,17 @@
 #include <cstdint>
 #include <cstring>
 namespace std { using ::cosf; using ::sinf; using ::tanf; using ::powf; using ::sqrtf; }
+#include <sstream>
+#include <fstream>
+#include <iostream>
+#include "ThirdParty/GamesEngineering/GEMLoader.h"  // self-contained; before the access widening
 #define private public
 #include "Geometry.h"
 #include "Scene.h"
 #include "Metrics.h"
 #undef private
+#include <map>
+#include <string>
 #include <vector>
 
 namespace {
@@ -49,14 +55,19 @@ BSDF* makeBSDF(const float* f) {
     Texture* a = constTexture(f[1], f[2], f[3]);
     int type = (int)f[0];
     BSDF* b = nullptr;
+    bool layered = type >= 8;
+    if (layered) type -= 8;
     switch (type) {
     case 0: b = new DiffuseBSDF(a); break;
     case 1: b = new MirrorBSDF(a); break;
     case 2: b = new ConductorBSDF(a, Colour(f[7], f[8], f[9]), Colour(f[10], f[11], f[12]), f[15]); break;
     case 3: b = new GlassBSDF(a, f[13], f[14]); break;
+    case 5: b = new OrenNayarBSDF(a, f[15]); break;
+    case 6: b = new PlasticBSDF(a, f[13], f[14], f[15]); break;
     default: b = new DielectricBSDF(a, f[13], f[14], f[15]); break;
     }
     b->emission = Colour(f[4], f[5], f[6]);
+    if (layered) b = new LayeredBSDF(b, Colour(0.5f, 0.5f, 0.5f), 1.f, 1.3f, 1.f);
     return b;
 }
 
@@ -172,8 +183,9 @@ void ref_bsdf(const float* mat, const float* sd, const float* u, const float* wi
     Colour fe = b->evaluate(s, we);
     out[7] = fe.r; out[8] = fe.g; out[9] = fe.b; out[10] = b->PDF(s, we);
     float iu = 0.f, iv = 0.f;
-    if ((int)mat[0] != 4) b->invert(s, wi, iu, iv);
+    if ((int)mat[0] % 8 != 4) b->invert(s, wi, iu, iv);  // DielectricBSDF::invert is empty
     out[11] = iu; out[12] = iv; out[13] = b->isPureSpecular() ? 1.f : 0.f;
+    out[14] = b->isTwoSided() ? 1.f : 0.f;
 }
 
 float ref_fresnel_dielectric(float c, float i, float e) { return ShadingHelper::fresnelDielectric(c, i, e); }
@@ -218,5 +230,172 @@ void ref_metrics(const float* test, const float* truth, int n_test, int n_truth,
     out[1] = m.calculateRMSE(a, b, w, h);
     out[2] = m.calculateRRMSE(a, b, w, h);
 }
+
+/* Texture::sample (Imaging.h:82-108) on an RGB float texture. */
+void ref_texture_sample(const float* rgb, int w, int h, const float* tuv, int n, float* out) {
+    Texture t;
+    t.width = w; t.height = h; t.channels = 3; t.alpha = nullptr;
+    t.texels = new Colour[(size_t)w * h];
+    for (size_t i = 0; i < (size_t)w * h; ++i) t.texels[i] = Colour(rgb[3 * i], rgb[3 * i + 1], rgb[3 * i + 2]);
+    for (int i = 0; i < n; ++i) {
+        Colour c = t.sample(tuv[2 * i], tuv[2 * i + 1]);
+        out[3 * i] = c.r; out[3 * i + 1] = c.g; out[3 * i + 2] = c.b;
+    }
+}
+
+/* Texture::load (Imaging.h:47-80) of one image file (stb_image): w, h, channels and
+ * the RGB texels (w*h*3 floats, caller-sized via a first call with out == NULL). */
+int ref_texture_load(const char* path, int* w, int* h, float* out) {
+    Texture t;
+    t.load(path);
+    *w = t.width; *h = t.height;
+    if (out)
+        for (int i = 0; i < t.width * t.height; ++i) {
+            out[3 * i] = t.texels[i].r; out[3 * i + 1] = t.texels[i].g; out[3 * i + 2] = t.texels[i].b;
+        }
+    return t.channels;
+}
+
+/* The scene-file path of SceneLoader.h:104-269 (loadInstance / loadScene) written
+ * over the reference's own GEMScene / GEMModelLoader / Matrix / Triangle classes
+ * (SceneLoader.h itself includes Renderer.h, which needs a Windows-only header).
+ * Outputs per kept triangle: positions (9), normals (9), uv (6), material index;
+ * per material: the 16-float record of the oracle (type + 8 when layered) and the
+ * texture file name index; camera: from3 to3 up3 fov w h.                         */
+struct RefLoaded {
+    std::vector<float> pos, nrm, uv, mats, cam;
+    std::vector<uint32_t> mi;
+    std::vector<int32_t> mat_tex;
+    std::vector<std::string> tex_names;
+};
+
+void* ref_load_scene(const char* dir) {
+    std::string sceneName(dir);
+    RefLoaded* r = new RefLoaded();
+    GEMLoader::GEMScene gemscene;
+    gemscene.load(sceneName + "/scene.json");
+    int width = gemscene.findProperty("width").getValue(1920);
+    int height = gemscene.findProperty("height").getValue(1080);
+    float fov = gemscene.findProperty("fov").getValue(45.0f);
+    Vec4 from, to, up;
+    gemscene.findProperty("from").getValuesAsVector3(from.x, from.y, from.z);
+    gemscene.findProperty("to").getValuesAsVector3(to.x, to.y, to.z);
+    gemscene.findProperty("up").getValuesAsVector3(up.x, up.y, up.z);
+    r->cam = {from.x, from.y, from.z, to.x, to.y, to.z, up.x, up.y, up.z, fov, (float)width, (float)height};
+    std::map<std::string, int> texIndex;
+    auto tex = [&](const std::string& f) {
+        auto it = texIndex.find(f);
+        if (it != texIndex.end()) return it->second;
+        int k = (int)r->tex_names.size();
+        texIndex[f] = k;
+        r->tex_names.push_back(f);
+        return k;
+    };
+    for (size_t ii = 0; ii < gemscene.instances.size(); ++ii) {
+        GEMLoader::GEMInstance& instance = gemscene.instances[ii];
+        GEMLoader::GEMModelLoader loader;
+        std::vector<GEMLoader::GEMMesh> meshes;
+        loader.load(sceneName + "/" + instance.meshFilename, meshes);
+        std::string bsdf = instance.material.find("bsdf").getValue("");
+        float m[16] = {0};
+        bool ok = true;
+        std::string refl = sceneName + "/" + instance.material.find("reflectance").getValue("");
+        if (bsdf == "diffuse") { m[0] = 0; }
+        else if (bsdf == "orennayar") { m[0] = 5; m[15] = instance.material.find("alpha").getValue(1.0f); }
+        else if (bsdf == "glass") { m[0] = 3; m[13] = instance.material.find("intIOR").getValue(1.33f);
+                                    m[14] = instance.material.find("extIOR").getValue(1.0f); }
+        else if (bsdf == "mirror") { m[0] = 1; }
+        else if (bsdf == "plastic") { m[0] = 6; m[13] = instance.material.find("intIOR").getValue(1.33f);
+                                      m[14] = instance.material.find("extIOR").getValue(1.0f);
+                                      m[15] = instance.material.find("roughness").getValue(1.0f); }
+        else if (bsdf == "dielectric") { m[13] = instance.material.find("intIOR").getValue(1.33f);
+                                         m[14] = instance.material.find("extIOR").getValue(1.0f);
+                                         m[15] = instance.material.find("roughness").getValue(1.0f);
+                                         m[0] = m[15] < 0.001f ? 3 : 4; if (m[0] == 3) m[15] = 0.f; }
+        else if (bsdf == "conductor") { m[0] = 2;
+            instance.material.find("eta").getValuesAsVector3(m[7], m[8], m[9]);
+            instance.material.find("k").getValuesAsVector3(m[10], m[11], m[12]);
+            m[15] = instance.material.find("roughness").getValue(1.0f); }
+        else ok = false;
+        if (!ok) continue;
+        if (instance.material.find("emission").getValue("") != "")
+            instance.material.find("emission").getValuesAsVector3(m[4], m[5], m[6]);
+        if (instance.material.find("coatingThickness").getValue(0) > 0) m[0] += 8;
+        for (int k = 0; k < 16; ++k) r->mats.push_back(m[k]);
+        r->mat_tex.push_back(tex(refl));
+        uint32_t materialIndex = (uint32_t)(r->mat_tex.size() - 1);
+        std::vector<Vertex> vertices;
+        std::vector<unsigned int> indices;
+        Matrix transform;
+        memcpy(transform.m, instance.w.m, 16 * sizeof(float));
+        Matrix vecTransform = transform.invert();
+        vecTransform = vecTransform.transpose();
+        for (size_t i = 0; i < meshes.size(); i++) {
+            for (size_t n = 0; n < meshes[i].verticesStatic.size(); n++) {
+                Vertex v;
+                v.p.x = meshes[i].verticesStatic[n].position.x;
+                v.p.y = meshes[i].verticesStatic[n].position.y;
+                v.p.z = meshes[i].verticesStatic[n].position.z;
+                v.normal.x = meshes[i].verticesStatic[n].normal.x;
+                v.normal.y = meshes[i].verticesStatic[n].normal.y;
+                v.normal.z = meshes[i].verticesStatic[n].normal.z;
+                v.p = transform.mulPoint(v.p);
+                v.normal = vecTransform.mulVec(v.normal);
+                v.normal = v.normal.normalize();
+                v.u = meshes[i].verticesStatic[n].u;
+                v.v = meshes[i].verticesStatic[n].v;
+                vertices.push_back(v);
+            }
+            int offset = (int)indices.size();
+            for (size_t n = 0; n < meshes[i].indices.size(); n++) indices.push_back(offset + meshes[i].indices[n]);
+        }
+        for (size_t i = 0; i + 2 < indices.size(); i += 3) {
+            Triangle t;
+            t.init(vertices[indices[i]], vertices[indices[i + 1]], vertices[indices[i + 2]], materialIndex);
+            if (t.area > 0) {
+                for (int k = 0; k < 3; ++k) {
+                    r->pos.push_back(t.vertices[k].p.x); r->pos.push_back(t.vertices[k].p.y); r->pos.push_back(t.vertices[k].p.z);
+                    r->nrm.push_back(t.vertices[k].normal.x); r->nrm.push_back(t.vertices[k].normal.y);
+                    r->nrm.push_back(t.vertices[k].normal.z);
+                    r->uv.push_back(t.vertices[k].u); r->uv.push_back(t.vertices[k].v);
+                }
+                r->mi.push_back(materialIndex);
+            }
+        }
+    }
+    return r;
+}
+int ref_loaded_sizes(void* p, int* n_mats, int* n_tex) {
+    RefLoaded* r = (RefLoaded*)p;
+    *n_mats = (int)r->mat_tex.size(); *n_tex = (int)r->tex_names.size();
+    return (int)r->mi.size();
+}
+void ref_loaded_copy(void* p, float* pos, float* nrm, float* uv, uint32_t* mi, float* mats, int32_t* mat_tex,
+                     float* cam) {
+    RefLoaded* r = (RefLoaded*)p;
+    memcpy(pos, r->pos.data(), r->pos.size() * 4); memcpy(nrm, r->nrm.data(), r->nrm.size() * 4);
+    memcpy(uv, r->uv.data(), r->uv.size() * 4); memcpy(mi, r->mi.data(), r->mi.size() * 4);
+    memcpy(mats, r->mats.data(), r->mats.size() * 4); memcpy(mat_tex, r->mat_tex.data(), r->mat_tex.size() * 4);
+    memcpy(cam, r->cam.data(), r->cam.size() * 4);
+}
+const char* ref_loaded_texture_name(void* p, int i) { return ((RefLoaded*)p)->tex_names[i].c_str(); }
+void ref_loaded_free(void* p) { delete (RefLoaded*)p; }
+
+/* Film::tonemap (Imaging.h:220-244) of one accumulated pixel. */
+void ref_tonemap(const float* rgb_sum, int n, int spp, float exposure, float lmax, uint8_t* out) {
+    Film f;
+    f.width = (unsigned)n; f.height = 1; f.SPP = spp;
+    f.film = new Colour[n];
+    for (int i = 0; i < n; ++i) f.film[i] = Colour(rgb_sum[3 * i], rgb_sum[3 * i + 1], rgb_sum[3 * i + 2]);
+    for (int i = 0; i < n; ++i) {
+        unsigned char r, g, b;
+        f.tonemap(i, 0, r, g, b, exposure, lmax);
+        out[3 * i] = r; out[3 * i + 1] = g; out[3 * i + 2] = b;
+    }
+    delete[] f.film;
+}
+
+/* stbi_write_hdr of the mean image, as Film::save (Imaging.h:263-272). */
+int ref_write_hdr(const char* path, const float* rgb, int w, int h) { return stbi_write_hdr(path, w, h, 3, rgb); }
 
 }  // extern "C"

@@ -6,6 +6,7 @@ thin host-side mirror of the reference's Scene / RayTracer interfaces.
 """
 from ._lib import LIB_PATH, PgError, load  # noqa: F401
 from .scene import (  # noqa: F401
-    BSDF_CONDUCTOR, BSDF_DIELECTRIC, BSDF_DIFFUSE, BSDF_GLASS, BSDF_MIRROR, Batch, Device, RayTracer, Scene,
-    SceneData, generate_scene, metrics, relmse,
+    BSDF_CONDUCTOR, BSDF_DIELECTRIC, BSDF_DIFFUSE, BSDF_GLASS, BSDF_LAYERED, BSDF_MIRROR, BSDF_ORENNAYAR,
+    BSDF_PLASTIC, Batch, Device, RayTracer, Scene, SceneData, generate_scene, load_scene, metrics, read_image,
+    relmse, write_hdr, write_png,
 )

@@ -23,6 +23,8 @@ PG_INTEGRATOR_RECORD = 1
 PG_INTEGRATOR_GUIDED_QT = 2
 PG_INTEGRATOR_GUIDED_MLP = 3
 
+PG_LOAD_SKIP_MISSING = 1
+
 
 class PgMaterial(C.Structure):
     _fields_ = [("type", C.c_float), ("albedo", C.c_float * 3), ("emission", C.c_float * 3),
@@ -155,6 +157,17 @@ SIGNATURES = {
     "pg_scene_copy": (C.c_int32, [_P, _F, _F, _U32, C.POINTER(PgMaterial), _F, C.POINTER(PgCameraDesc)]),
     "pg_scene_free": (C.c_int32, [_P]),
     "pg_bvh_build_host": (C.c_int32, [_F, C.c_uint32, _U32, _F, _I32]),
+    "pg_scene_textures": (C.c_int32, [_P, _F, _I32, C.c_uint32, _F, C.POINTER(C.c_uint64), _I32, _I32,
+                                      C.c_uint32]),
+    "pg_scene_load": (C.c_int32, [C.c_char_p, C.c_uint32, C.POINTER(_P)]),
+    "pg_scene_load_error": (C.c_char_p, []),
+    "pg_scene_texture_sizes": (C.c_int32, [_P, _U32, C.POINTER(C.c_uint64), _U32]),
+    "pg_scene_copy_textures": (C.c_int32, [_P, _F, _I32, _F, C.POINTER(C.c_uint64), _I32, _I32]),
+    "pg_scene_warning": (C.c_char_p, [_P, C.c_uint32]),
+    "pg_image_read": (C.c_int32, [C.c_char_p, _I32, _I32, _I32, _F]),
+    "pg_image_write_png": (C.c_int32, [C.c_char_p, _U8, C.c_int32, C.c_int32]),
+    "pg_image_write_hdr": (C.c_int32, [C.c_char_p, _F, C.c_int32, C.c_int32]),
+    "pg_film_tonemap": (C.c_int32, [_P, C.c_float, C.c_float, _U8]),
 }
 
 _lib = None

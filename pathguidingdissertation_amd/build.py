@@ -18,7 +18,7 @@ BUILD = PKG / "_build"
 LIB = PKG / "libpgtrace.so"
 EXAMPLE = PKG / "pgd_render"  # example host program over the C++ mirror
 
-HOST_SOURCES = ["host_prep.cpp", "scenes.cpp", "metrics.cpp"]
+HOST_SOURCES = ["host_prep.cpp", "scenes.cpp", "metrics.cpp", "scene_loader.cpp"]
 HIP_SOURCES = ["kernels.hip", "guiding.hip", "mlp.hip", "batch.hip", "api.hip"]
 
 CXXFLAGS = ["-std=c++17", "-O3", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-pthread", "-Wall",
@@ -62,7 +62,7 @@ def build(verbose: bool = False) -> Path:
         _run(cmd)
         objs.append(obj)
     tmp = LIB.with_suffix(".so.tmp")
-    _run([hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", "-pthread", *map(str, objs), "-o", str(tmp)])
+    _run([hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", "-pthread", *map(str, objs), "-lz", "-o", str(tmp)])
     os.replace(tmp, LIB)
     # example host program over include/pgd/pgd.hpp, linked against the library
     ex_tmp = EXAMPLE.with_suffix(".tmp")

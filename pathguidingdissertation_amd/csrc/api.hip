@@ -57,6 +57,9 @@ struct pg_ctx {
     float* d_env_pdf = nullptr;
     float* d_env_prob = nullptr;
     uint32_t* d_env_alias = nullptr;
+    float4* d_tex_texels = nullptr;
+    int4* d_tex_info = nullptr;
+    float2* d_uv = nullptr;
     DevScene dev{};
     // path state + film
     PathState ps{};
@@ -160,6 +163,8 @@ struct pg_ctx {
     void free_scene() {
         dfree(d_nodes); dfree(d_tris); dfree(d_shade); dfree(d_mats); dfree(d_lights);
         dfree(d_env_tex); dfree(d_env_pdf); dfree(d_env_prob); dfree(d_env_alias);
+        dfree(d_tex_texels); dfree(d_tex_info); dfree(d_uv);
+        dev.tex_texels = nullptr; dev.tex_info = nullptr; dev.uv = nullptr;
         has_scene = false;
     }
     void free_records() {
@@ -344,6 +349,45 @@ pg_status pg_scene_upload(pg_ctx* c, const float* positions, const float* normal
     PG_TRY(c, launch_clear_film(c->film, fn, c->stream), "clear film");
     c->spp = 0;
     PG_TRY(c, hipStreamSynchronize(c->stream), "sync");
+    return PG_OK;
+}
+
+pg_status pg_scene_textures(pg_ctx* c, const float* uv, const int32_t* material_texture, uint32_t n_materials,
+                            const float* texels, const uint64_t* texel_offset, const int32_t* tex_width,
+                            const int32_t* tex_height, uint32_t n_textures) {
+    PG_CHECK_CTX(c);
+    if (!c->has_scene) return c->fail(PG_ERR_STATE, "pg_scene_textures before pg_scene_upload");
+    PreparedScene& h = c->host;
+    if (n_materials != h.mats.size()) return c->fail(PG_ERR_INVALID, "material count differs from the uploaded scene");
+    if (!material_texture || !uv) return c->fail(PG_ERR_INVALID, "null uv or material_texture");
+    if (n_textures > 0 && (!texels || !texel_offset || !tex_width || !tex_height))
+        return c->fail(PG_ERR_INVALID, "null texture array");
+    uint64_t total = 0;
+    for (uint32_t i = 0; i < n_textures; ++i) {
+        if (tex_width[i] < 1 || tex_height[i] < 1) return c->fail(PG_ERR_INVALID, "texture size must be >= 1x1");
+        uint64_t n = (uint64_t)tex_width[i] * (uint64_t)tex_height[i];
+        if (texel_offset[i] + n > total) total = texel_offset[i] + n;
+    }
+    if (total >= (1ull << 31)) return c->fail(PG_ERR_INVALID, "more than 2^31 texels");
+    for (uint32_t i = 0; i < n_materials; ++i)
+        if (material_texture[i] < -1 || material_texture[i] >= (int32_t)n_textures)
+            return c->fail(PG_ERR_INVALID, "material texture index out of range");
+    PG_TRY(c, hipStreamSynchronize(c->stream), "sync");
+    dfree(c->d_tex_texels); dfree(c->d_tex_info); dfree(c->d_uv);
+    size_t n_tris = h.shade.size();
+    std::vector<float4> tx((size_t)total);
+    for (uint64_t k = 0; k < total; ++k) tx[k] = make_float4(texels[3 * k], texels[3 * k + 1], texels[3 * k + 2], 0.f);
+    std::vector<int4> info(n_textures ? n_textures : 1);
+    for (uint32_t i = 0; i < n_textures; ++i) info[i] = make_int4((int)texel_offset[i], tex_width[i], tex_height[i], 0);
+    PG_TRY(c, dalloc(&c->d_tex_texels, total ? (size_t)total : 1), "alloc textures");
+    PG_TRY(c, dalloc(&c->d_tex_info, info.size()), "alloc texture table");
+    PG_TRY(c, dalloc(&c->d_uv, 3 * n_tris), "alloc uv");
+    if (total) PG_TRY(c, hipMemcpy(c->d_tex_texels, tx.data(), tx.size() * 16, hipMemcpyHostToDevice), "copy textures");
+    PG_TRY(c, hipMemcpy(c->d_tex_info, info.data(), info.size() * sizeof(int4), hipMemcpyHostToDevice), "copy texture table");
+    PG_TRY(c, hipMemcpy(c->d_uv, uv, n_tris * 6 * sizeof(float), hipMemcpyHostToDevice), "copy uv");
+    for (uint32_t i = 0; i < n_materials; ++i) h.mats[i].tex = material_texture[i];
+    PG_TRY(c, hipMemcpy(c->d_mats, h.mats.data(), h.mats.size() * sizeof(MaterialRec), hipMemcpyHostToDevice), "copy mats");
+    c->dev.tex_texels = c->d_tex_texels; c->dev.tex_info = c->d_tex_info; c->dev.uv = c->d_uv;
     return PG_OK;
 }
 
@@ -993,6 +1037,46 @@ pg_status pg_film_read(pg_ctx* c, float* rgb_sum, uint32_t* spp) {
         }
     }
     if (spp) *spp = c->spp;
+    return PG_OK;
+}
+
+// Film::tonemap (Imaging.h:220-244) of every pixel of the film.
+__global__ void k_tonemap(const float4* __restrict__ film, uint32_t n, float inv_spp, float exposure, float lw2,
+                          uint8_t* __restrict__ out) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float4 f = film[i];
+    float cr = (f.x * inv_spp) * exposure, cg = (f.y * inv_spp) * exposure, cb = (f.z * inv_spp) * exposure;
+    float lin = ((0.2126f * cr) + (0.7152f * cg)) + (0.0722f * cb);
+    uint8_t r = 0, g = 0, b = 0;
+    if (!(lin < 1e-4f)) {
+        float lout = lin * (1.f + (lin / lw2)) / (1.f + lin);
+        float k = lout / lin;
+        const float ge = 1.f / 2.2f;
+        r = (uint8_t)(pg::smax(0.f, pg::smin(powf(cr * k, ge), 1.f)) * 255.f);
+        g = (uint8_t)(pg::smax(0.f, pg::smin(powf(cg * k, ge), 1.f)) * 255.f);
+        b = (uint8_t)(pg::smax(0.f, pg::smin(powf(cb * k, ge), 1.f)) * 255.f);
+    }
+    out[3 * (size_t)i] = r;
+    out[3 * (size_t)i + 1] = g;
+    out[3 * (size_t)i + 2] = b;
+}
+
+pg_status pg_film_tonemap(pg_ctx* c, float exposure, float lmax_white, uint8_t* rgb8) {
+    PG_CHECK_CTX(c);
+    if (!c->has_scene) return c->fail(PG_ERR_STATE, "no scene uploaded");
+    if (!rgb8) return c->fail(PG_ERR_INVALID, "null output");
+    uint32_t n = c->film_n;
+    uint8_t* d = nullptr;
+    PG_TRY(c, dalloc(&d, (size_t)n * 3), "alloc tonemap output");
+    float inv = c->spp > 0 ? 1.f / (float)c->spp : 1.f;
+    hipLaunchKernelGGL(k_tonemap, dim3((n + 255) / 256), dim3(256), 0, c->stream, c->film, n, inv, exposure,
+                       lmax_white * lmax_white, d);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(rgb8, d, (size_t)n * 3, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    dfree(d);
+    if (e != hipSuccess) return c->fail(PG_ERR_DEVICE, std::string("tonemap: ") + hipGetErrorString(e));
     return PG_OK;
 }
 

@@ -385,12 +385,89 @@ __device__ __forceinline__ float cosine_pdf(F3 wl) {  // Sampling.h:98-101
 struct SD {
     F3 x, wo, sn, gn;
     Frame fr;
+    C3 alb;  // the material's albedo (its texture at the hit's uv when it has one)
 };
+
+// Albedo texture lookup (Imaging.h:82-108) at the hit's interpolated uv
+// (Triangle::interpolateAttributes, Geometry.h:135-136).
+__device__ __forceinline__ C3 tex_albedo(const DevScene& s, int tex, uint32_t tri, float a, float b, float g) {
+    const float2* uv = s.uv + 3 * (size_t)tri;
+    float2 u0 = uv[0], u1 = uv[1], u2 = uv[2];
+    float tu = (u0.x * a + u1.x * b) + u2.x * g;
+    float tv = (u0.y * a + u1.y * b) + u2.y * g;
+    int4 ti = s.tex_info[tex];
+    float u = smax(0.f, fabsf(tu)) * (float)ti.y;
+    float v = smax(0.f, fabsf(tv)) * (float)ti.z;
+    int x = (int)floorf(u), y = (int)floorf(v);
+    float fu = u - (float)x, fv = v - (float)y;
+    float w0 = (1.f - fu) * (1.f - fv), w1 = fu * (1.f - fv), w2 = (1.f - fu) * fv, w3 = fu * fv;
+    x = x % ti.y;
+    y = y % ti.z;
+    int x1 = (x + 1) % ti.y, y1 = (y + 1) % ti.z;
+    const float4* t = s.tex_texels + ti.x;
+    float4 p = t[y * ti.y + x], q = t[y * ti.y + x1], r = t[y1 * ti.y + x], e = t[y1 * ti.y + x1];
+    return ((c3(p.x, p.y, p.z) * w0 + c3(q.x, q.y, q.z) * w1) + c3(r.x, r.y, r.z) * w2) + c3(e.x, e.y, e.z) * w3;
+}
+__device__ __forceinline__ C3 material_albedo(const DevScene& s, const MaterialRec& m, uint32_t tri, float a,
+                                              float b, float g) {
+    return m.tex >= 0 ? tex_albedo(s, m.tex, tri, a, b, g) : c3(m.albedo[0], m.albedo[1], m.albedo[2]);
+}
+
+__device__ __forceinline__ float spherical_phi(F3 w) {  // MyMath.h:445-448
+    float p = atan2f(w.y, w.x);
+    return (p < 0.f) ? (float)((double)p + 2.0 * kPiD) : p;
+}
+// OrenNayarBSDF::evaluate (Materials.h:683-707): no hemisphere test, as in the reference.
+__device__ __forceinline__ C3 oren_nayar_eval(const MaterialRec& m, const SD& sd, F3 wl, F3 ol) {
+    float s2 = m.alpha * m.alpha;  // alpha holds sigma for this type
+    float A = 1.f - (s2 / (2.f * (s2 + 0.33f)));
+    float B = (0.45f * s2) / (s2 + 0.09f);
+    float phi_i = spherical_phi(wl), phi_o = spherical_phi(ol);
+    float th_i = acosf(wl.z), th_o = acosf(ol.z);
+    float c = cosf(phi_i - phi_o);
+    float sa = sinf(smax(th_i, th_o));
+    float tb = tanf(smin(th_i, th_o));
+    float on = A + (((B * smax(0.f, c)) * sa) * tb);
+    return (sd.alb * kInvPiF) * on;
+}
+__device__ __forceinline__ float plastic_exponent(float alpha) {  // Materials.h:738-740
+    float a = smax(alpha, 0.001f);
+    return (2.f / (a * a)) - 2.f;
+}
+// PlasticBSDF::evaluate / PDF (Materials.h:784-848): FP64 where the reference's M_PI
+// terms promote.
+__device__ __forceinline__ C3 plastic_eval(const MaterialRec& m, const SD& sd, F3 wl, F3 ol, float& pdf) {
+    pdf = 0.f;
+    if (wl.z <= 0.f || ol.z <= 0.f) return c3(0.f, 0.f, 0.f);
+    F3 h = wl + ol;
+    if (length_sq(h) < kEps) return c3(0.f, 0.f, 0.f);
+    h = normalize(h);
+    float e = plastic_exponent(m.alpha);
+    float oh = dot(ol, h);
+    float ks = fresnel_dielectric(oh, m.int_ior, m.ext_ior);
+    float kd = 1.f - ks;
+    float norm = (float)((double)(e + 2.f) / ((4.0 * kPiD) * (double)(2.f - powf(2.f, (-e * 0.5f)))));
+    float lo = (float)((double)(e + 2.f) / (8.0 * kPiD)), hi = (float)((double)(e + 4.f) / (8.0 * kPiD));
+    norm = smin(smax(lo, norm), hi);
+    float pw = powf(smax(h.z, 0.f), e);
+    float ph = (float)(((double)(e + 1.f) / (2.0 * kPiD)) * (double)pw);
+    float gp = oh <= 0.f ? 0.f : (ph / (4.f * oh));
+    pdf = (kd * cosine_pdf(wl)) + (ks * gp);
+    C3 diff = sd.alb * kInvPiF;
+    float gl = oh <= 0.f ? 0.f : pw;
+    return (diff * kd) + ((c3(gl, gl, gl) * norm) * ks);
+}
 
 __device__ inline C3 bsdf_eval(const MaterialRec& m, const SD& sd, F3 wi, float& pdf) {
     pdf = 0.f;
     C3 zero = c3(0.f, 0.f, 0.f);
-    C3 alb = c3(m.albedo[0], m.albedo[1], m.albedo[2]);
+    C3 alb = sd.alb;
+    if (m.type == PG_BSDF_ORENNAYAR) {  // Materials.h:683-714
+        F3 wl = to_local(sd.fr, wi);
+        pdf = cosine_pdf(wl);
+        return oren_nayar_eval(m, sd, wl, to_local(sd.fr, sd.wo));
+    }
+    if (m.type == PG_BSDF_PLASTIC) return plastic_eval(m, sd, to_local(sd.fr, wi), to_local(sd.fr, sd.wo), pdf);
     if (m.type == PG_BSDF_DIFFUSE) {
         F3 wl = to_local(sd.fr, wi);
         pdf = cosine_pdf(wl);
@@ -446,11 +523,31 @@ __device__ inline C3 bsdf_eval(const MaterialRec& m, const SD& sd, F3 wi, float&
 }
 
 __device__ inline F3 bsdf_sample(const MaterialRec& m, const SD& sd, uint32_t key, uint32_t& dim, C3& f, float& pdf) {
-    C3 alb = c3(m.albedo[0], m.albedo[1], m.albedo[2]);
+    C3 alb = sd.alb;
     F3 up{0.f, 0.f, 1.f};
     f = c3(0.f, 0.f, 0.f);
     pdf = 0.f;
     switch (m.type) {
+    case PG_BSDF_PLASTIC: {  // Materials.h:742-778
+        F3 ol = to_local(sd.fr, sd.wo);
+        float r1 = keyed_rand(key, dim++), r2 = keyed_rand(key, dim++);
+        float F = fresnel_dielectric(fabsf(ol.z), m.int_ior, m.ext_ior);
+        F3 wl;
+        if (keyed_rand(key, dim++) < F) {
+            float e = plastic_exponent(m.alpha);
+            float base = 1.f - r1, power = 1.f / (e + 1.f);
+            float th = acosf(powf(base, power));
+            float ph = (float)(2.0 * kPiD * (double)r2);
+            F3 h = normalize(spherical_to_world(th, ph));
+            wl = ((h * 2.f) * dot(ol, h)) - ol;
+        } else {
+            wl = spherical_to_world(acosf(sqrtf(r1)), (float)(2.0 * kPiD * (double)r2));
+        }
+        F3 wi = to_world(sd.fr, wl);
+        f = bsdf_eval(m, sd, wi, pdf);
+        return wi;
+    }
+    case PG_BSDF_ORENNAYAR:  // Materials.h:655-670: the diffuse cosine warp
     case PG_BSDF_DIFFUSE: {  // Materials.h:146-161
         float r1 = keyed_rand(key, dim++), r2 = keyed_rand(key, dim++);
         float theta = acosf(sqrtf(r1));

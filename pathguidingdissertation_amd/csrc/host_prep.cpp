@@ -217,55 +217,6 @@ void build_bvh(const float* positions, uint32_t n_tris, HostBvh& out) {
 }
 
 // ------------------------------------------------------------------ camera
-namespace {
-struct M4 {
-    float m[16];
-    M4() { std::memset(m, 0, sizeof(m)); m[0] = m[5] = m[10] = m[15] = 1.f; }
-    HV3 mul_point(const HV3& v) const {
-        return HV3(((v.x * m[0] + v.y * m[1]) + v.z * m[2]) + m[3],
-                   ((v.x * m[4] + v.y * m[5]) + v.z * m[6]) + m[7],
-                   ((v.x * m[8] + v.y * m[9]) + v.z * m[10]) + m[11]);
-    }
-    // Cofactor inverse in the reference's term order (MyMath.h:286-323): per entry,
-    // six signed triple products accumulated left to right.
-    M4 inverse() const {
-        static const signed char T[16][6][4] = {
-            {{1,5,10,15},{-1,5,11,14},{-1,9,6,15},{1,9,7,14},{1,13,6,11},{-1,13,7,10}},
-            {{-1,1,10,15},{1,1,11,14},{1,9,2,15},{-1,9,3,14},{-1,13,2,11},{1,13,3,10}},
-            {{1,1,6,15},{-1,1,7,14},{-1,5,2,15},{1,5,3,14},{1,13,2,7},{-1,13,3,6}},
-            {{-1,1,6,11},{1,1,7,10},{1,5,2,11},{-1,5,3,10},{-1,9,2,7},{1,9,3,6}},
-            {{-1,4,10,15},{1,4,11,14},{1,8,6,15},{-1,8,7,14},{-1,12,6,11},{1,12,7,10}},
-            {{1,0,10,15},{-1,0,11,14},{-1,8,2,15},{1,8,3,14},{1,12,2,11},{-1,12,3,10}},
-            {{-1,0,6,15},{1,0,7,14},{1,4,2,15},{-1,4,3,14},{-1,12,2,7},{1,12,3,6}},
-            {{1,0,6,11},{-1,0,7,10},{-1,4,2,11},{1,4,3,10},{1,8,2,7},{-1,8,3,6}},
-            {{1,4,9,15},{-1,4,11,13},{-1,8,5,15},{1,8,7,13},{1,12,5,11},{-1,12,7,9}},
-            {{-1,0,9,15},{1,0,11,13},{1,8,1,15},{-1,8,3,13},{-1,12,1,11},{1,12,3,9}},
-            {{1,0,5,15},{-1,0,7,13},{-1,4,1,15},{1,4,3,13},{1,12,1,7},{-1,12,3,5}},
-            {{-1,0,5,11},{1,0,7,9},{1,4,1,11},{-1,4,3,9},{-1,8,1,7},{1,8,3,5}},
-            {{-1,4,9,14},{1,4,10,13},{1,8,5,14},{-1,8,6,13},{-1,12,5,10},{1,12,6,9}},
-            {{1,0,9,14},{-1,0,10,13},{-1,8,1,14},{1,8,2,13},{1,12,1,10},{-1,12,2,9}},
-            {{-1,0,5,14},{1,0,6,13},{1,4,1,14},{-1,4,2,13},{-1,12,1,6},{1,12,2,5}},
-            {{1,0,5,10},{-1,0,6,9},{-1,4,1,10},{1,4,2,9},{1,8,1,6},{-1,8,2,5}},
-        };
-        M4 r;
-        for (int e = 0; e < 16; ++e) {
-            float acc = 0.f;
-            for (int t = 0; t < 6; ++t) {
-                const signed char* q = T[e][t];
-                float p = (m[q[1]] * m[q[2]]) * m[q[3]];
-                acc = (t == 0) ? (q[0] > 0 ? p : -p) : (q[0] > 0 ? acc + p : acc - p);
-            }
-            r.m[e] = acc;
-        }
-        float det = ((m[0] * r.m[0] + m[1] * r.m[4]) + m[2] * r.m[8]) + m[3] * r.m[12];
-        if (det == 0.f) det = 1.f;
-        det = 1.f / det;
-        for (int i = 0; i < 16; ++i) r.m[i] = r.m[i] * det;
-        return r;
-    }
-};
-}  // namespace
-
 CameraRec make_camera(const pg_camera_desc& d) {
     // Matrix::perspective (MyMath.h:350-364) with near 0.001, far 10000 (SceneLoader.h:229)
     M4 P;
@@ -378,18 +329,22 @@ std::string prepare_scene(const float* pos, const float* nrm, const uint32_t* ma
         MaterialRec& r = out.mats[i];
         std::memset(&r, 0, sizeof(r));
         r.type = (int32_t)m.type;
-        if (r.type < PG_BSDF_DIFFUSE || r.type > PG_BSDF_DIELECTRIC) return "unknown material type";
+        bool layered = r.type >= PG_BSDF_LAYERED;                             // Materials.h:855-897
+        if (layered) r.type -= PG_BSDF_LAYERED;
+        if (r.type < PG_BSDF_DIFFUSE || r.type > PG_BSDF_PLASTIC) return "unknown material type";
+        r.tex = -1;
         for (int c = 0; c < 3; ++c) {
             r.albedo[c] = m.albedo[c]; r.emission[c] = m.emission[c]; r.eta[c] = m.eta[c]; r.k[c] = m.k[c];
         }
         r.int_ior = m.int_ior; r.ext_ior = m.ext_ior;
-        r.alpha = 1.62142f * sqrtf(m.roughness);                               // Materials.h:249, 455
+        r.alpha = 1.62142f * sqrtf(m.roughness);                               // Materials.h:249, 455, 738
+        if (r.type == PG_BSDF_ORENNAYAR) r.alpha = m.roughness;               // sigma (Materials.h:649-652)
         float lum = ((0.2126f * m.emission[0]) + (0.7152f * m.emission[1])) + (0.0722f * m.emission[2]);
         r.is_light = lum > 0.f ? 1u : 0u;                                      // Materials.h:130
         bool spec = r.type == PG_BSDF_MIRROR || r.type == PG_BSDF_GLASS ||
                     ((r.type == PG_BSDF_CONDUCTOR || r.type == PG_BSDF_DIELECTRIC) && r.alpha < kEps);
         r.specular = spec ? 1u : 0u;
-        r.two_sided = (r.type == PG_BSDF_GLASS || r.type == PG_BSDF_DIELECTRIC) ? 0u : 1u;
+        r.two_sided = (!layered && (r.type == PG_BSDF_GLASS || r.type == PG_BSDF_DIELECTRIC)) ? 0u : 1u;
     }
     build_bvh(pos, n_tris, out.bvh);
     out.node_slots.assign(out.bvh.nodes.size() + 2, BvhNode{});

@@ -2,6 +2,7 @@
 // BVH build, camera matrices, environment tables and device record packing.
 #pragma once
 #include <cstdint>
+#include <cstring>
 #include <string>
 #include <vector>
 
@@ -18,6 +19,65 @@ struct HV3 {
     HV3 operator-(const HV3& v) const { return HV3(x - v.x, y - v.y, z - v.z); }
     HV3 operator*(float s) const { return HV3(x * s, y * s, z * s); }
     float operator[](int i) const { return i == 0 ? x : (i == 1 ? y : z); }
+};
+
+// 4x4 matrix in the reference's row-major layout and FP32 operation order
+// (MyMath.h:151-404).
+struct M4 {
+    float m[16];
+    M4() { std::memset(m, 0, sizeof(m)); m[0] = m[5] = m[10] = m[15] = 1.f; }
+    HV3 mul_vec(const HV3& v) const {                                           // MyMath.h:253-259
+        return HV3((v.x * m[0] + v.y * m[1]) + v.z * m[2], (v.x * m[4] + v.y * m[5]) + v.z * m[6],
+                   (v.x * m[8] + v.y * m[9]) + v.z * m[10]);
+    }
+    M4 transpose() const {                                                      // MyMath.h:187-194
+        M4 r;
+        for (int i = 0; i < 4; ++i)
+            for (int j = 0; j < 4; ++j) r.m[4 * i + j] = m[4 * j + i];
+        return r;
+    }
+    HV3 mul_point(const HV3& v) const {
+        return HV3(((v.x * m[0] + v.y * m[1]) + v.z * m[2]) + m[3],
+                   ((v.x * m[4] + v.y * m[5]) + v.z * m[6]) + m[7],
+                   ((v.x * m[8] + v.y * m[9]) + v.z * m[10]) + m[11]);
+    }
+    // Cofactor inverse in the reference's term order (MyMath.h:286-323): per entry,
+    // six signed triple products accumulated left to right.
+    M4 inverse() const {
+        static const signed char T[16][6][4] = {
+            {{1,5,10,15},{-1,5,11,14},{-1,9,6,15},{1,9,7,14},{1,13,6,11},{-1,13,7,10}},
+            {{-1,1,10,15},{1,1,11,14},{1,9,2,15},{-1,9,3,14},{-1,13,2,11},{1,13,3,10}},
+            {{1,1,6,15},{-1,1,7,14},{-1,5,2,15},{1,5,3,14},{1,13,2,7},{-1,13,3,6}},
+            {{-1,1,6,11},{1,1,7,10},{1,5,2,11},{-1,5,3,10},{-1,9,2,7},{1,9,3,6}},
+            {{-1,4,10,15},{1,4,11,14},{1,8,6,15},{-1,8,7,14},{-1,12,6,11},{1,12,7,10}},
+            {{1,0,10,15},{-1,0,11,14},{-1,8,2,15},{1,8,3,14},{1,12,2,11},{-1,12,3,10}},
+            {{-1,0,6,15},{1,0,7,14},{1,4,2,15},{-1,4,3,14},{-1,12,2,7},{1,12,3,6}},
+            {{1,0,6,11},{-1,0,7,10},{-1,4,2,11},{1,4,3,10},{1,8,2,7},{-1,8,3,6}},
+            {{1,4,9,15},{-1,4,11,13},{-1,8,5,15},{1,8,7,13},{1,12,5,11},{-1,12,7,9}},
+            {{-1,0,9,15},{1,0,11,13},{1,8,1,15},{-1,8,3,13},{-1,12,1,11},{1,12,3,9}},
+            {{1,0,5,15},{-1,0,7,13},{-1,4,1,15},{1,4,3,13},{1,12,1,7},{-1,12,3,5}},
+            {{-1,0,5,11},{1,0,7,9},{1,4,1,11},{-1,4,3,9},{-1,8,1,7},{1,8,3,5}},
+            {{-1,4,9,14},{1,4,10,13},{1,8,5,14},{-1,8,6,13},{-1,12,5,10},{1,12,6,9}},
+            {{1,0,9,14},{-1,0,10,13},{-1,8,1,14},{1,8,2,13},{1,12,1,10},{-1,12,2,9}},
+            {{-1,0,5,14},{1,0,6,13},{1,4,1,14},{-1,4,2,13},{-1,12,1,6},{1,12,2,5}},
+            {{1,0,5,10},{-1,0,6,9},{-1,4,1,10},{1,4,2,9},{1,8,1,6},{-1,8,2,5}},
+        };
+        M4 r;
+        for (int e = 0; e < 16; ++e) {
+            float acc = 0.f;
+            for (int t = 0; t < 6; ++t) {
+                const signed char* q = T[e][t];
+                float p = (m[q[1]] * m[q[2]]) * m[q[3]];
+                acc = (t == 0) ? (q[0] > 0 ? p : -p) : (q[0] > 0 ? acc + p : acc - p);
+            }
+            r.m[e] = acc;
+        }
+        float det = ((m[0] * r.m[0] + m[1] * r.m[4]) + m[2] * r.m[8]) + m[3] * r.m[12];
+        if (det == 0.f) det = 1.f;
+        det = 1.f / det;
+        for (int i = 0; i < 16; ++i) r.m[i] = r.m[i] * det;
+        return r;
+    }
 };
 
 // Binned-SAH BVH in the reference's topology and triangle order (Geometry.h:293-446).

@@ -20,6 +20,9 @@ struct DevScene {
     const float* env_pdf;       // uv-space density per texel
     const float* env_prob;      // alias acceptance
     const uint32_t* env_alias;  // alias target
+    const float4* tex_texels;   // albedo textures, RGBA32F, concatenated
+    const int4* tex_info;       // per texture: first texel, width, height
+    const float2* uv;           // per triangle ID: (u, v) of its three vertices
     int32_t n_lights;
     int32_t env_w, env_h, has_env;
     float centre[3];

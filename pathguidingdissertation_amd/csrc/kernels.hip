@@ -409,7 +409,7 @@ __device__ __forceinline__ bool next_event(const DevScene& s, const MaterialRec&
 // learned vMF mixture (probability c, learned per point). The combined density is
 // used in every MIS weight so that the estimator stays unbiased.
 template <int MODE, bool MULTI>
-__global__ __launch_bounds__(kBlock) void k_shade(DevScene s0, PathState ps, PassParams p, GuideCtx gc, int parity,
+__global__ __launch_bounds__(kBlock, 3) void k_shade(DevScene s0, PathState ps, PassParams p, GuideCtx gc, int parity,
                                                   int depth) {
     uint32_t n = ps.counters[parity];
     if (blockIdx.x == 0 && threadIdx.x == 0) ps.counters[4] = 0;  // shadow-ray claims
@@ -469,6 +469,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene s0, PathState ps, Pas
                     if (dot(sd.wo, sd.gn) < 0.f) sd.gn = -sd.gn;
                 }
                 sd.fr = frame_from(sd.sn);
+                sd.alb = material_albedo(s, m, hid, ha, hu, hv);
                 if (m.is_light) {
                     C3 e = c3(m.emission[0], m.emission[1], m.emission[2]);
                     if (depth == 0 || prev_spec) {
@@ -690,7 +691,7 @@ __global__ __launch_bounds__(kBlock) void k_guide_cells(DevScene s, PathState ps
 // vertex weight, the NEE weight and the next vertex's MIS weights; each vertex keeps
 // q and the BSDF pdf of its direction as training data.
 template <bool NEE, int MODE>
-__global__ __launch_bounds__(kBlock) void k_shade_record(DevScene s, PathState ps, PassParams p, RecordState rs,
+__global__ __launch_bounds__(kBlock, 3) void k_shade_record(DevScene s, PathState ps, PassParams p, RecordState rs,
                                                          const float4* __restrict__ mix, int parity, int depth) {
     uint32_t n = ps.counters[parity];
     const uint32_t* qin = ps.queue[parity];
@@ -748,6 +749,7 @@ __global__ __launch_bounds__(kBlock) void k_shade_record(DevScene s, PathState p
                     if (dot(sd.wo, sd.gn) < 0.f) sd.gn = -sd.gn;
                 }
                 sd.fr = frame_from(sd.sn);
+                sd.alb = material_albedo(s, m, hid, ha, hu, hv);
                 vpos = make_float4(sd.x.x, sd.x.y, sd.x.z, 0.f);
                 vnrm = make_float4(sd.sn.x, sd.sn.y, sd.sn.z, 0.f);
                 if (m.is_light) {

@@ -88,7 +88,7 @@ struct alignas(16) MaterialRec {
     uint32_t is_light;
     uint32_t specular;
     uint32_t two_sided;
-    uint32_t pad;
+    int32_t tex;  // albedo texture (-1: the constant albedo above)
 };
 static_assert(sizeof(MaterialRec) == 80, "80-byte material");
 

@@ -11,15 +11,8 @@
 #include <vector>
 
 #include "../../include/pg_tracer.h"
+#include "host_scene.hpp"
 
-struct pg_host_scene {
-    std::vector<float> pos, nrm;
-    std::vector<uint32_t> mat;
-    std::vector<pg_material> mats;
-    std::vector<float> env;
-    int32_t env_w = 0, env_h = 0;
-    pg_camera_desc cam;
-};
 
 namespace {
 

@@ -13,7 +13,8 @@ import numpy as np
 from . import _lib
 from ._lib import PgCameraDesc, PgMaterial, PgRenderParams, PgStats, check, load
 
-BSDF_DIFFUSE, BSDF_MIRROR, BSDF_CONDUCTOR, BSDF_GLASS, BSDF_DIELECTRIC = range(5)
+BSDF_DIFFUSE, BSDF_MIRROR, BSDF_CONDUCTOR, BSDF_GLASS, BSDF_DIELECTRIC, BSDF_ORENNAYAR, BSDF_PLASTIC = range(7)
+BSDF_LAYERED = 8  # added to a base type: LayeredBSDF over that base (Materials.h:855-897)
 
 MATERIAL_DTYPE = np.dtype([("type", "f4"), ("albedo", "f4", 3), ("emission", "f4", 3), ("eta", "f4", 3),
                            ("k", "f4", 3), ("int_ior", "f4"), ("ext_ior", "f4"), ("roughness", "f4")])
@@ -37,6 +38,28 @@ class SceneData:
     materials: np.ndarray          # (m,) MATERIAL_DTYPE
     env: np.ndarray | None         # (h, w, 3) float32 lat-long, y-up
     camera: dict = field(default_factory=dict)  # from, to, up, fov, width, height
+    # albedo textures (scene files; None for procedural scenes)
+    uv: np.ndarray | None = None                 # (n, 3, 2) float32 per-vertex (u, v)
+    material_texture: np.ndarray | None = None   # (m,) int32, -1 = constant albedo
+    textures: list = field(default_factory=list)  # (h, w, 3) float32 RGB each
+    warnings: list = field(default_factory=list)
+
+    @property
+    def has_textures(self) -> bool:
+        return self.material_texture is not None and len(self.textures) > 0
+
+    def texture_arrays(self):
+        """uv (n*6), material_texture, concatenated RGB texels, offsets, widths, heights."""
+        uv = np.ascontiguousarray(self.uv if self.uv is not None else np.zeros((self.n_tris, 3, 2)), np.float32)
+        mt = np.ascontiguousarray(self.material_texture if self.material_texture is not None
+                                  else np.full(len(self.materials), -1), np.int32)
+        texels = np.ascontiguousarray(np.concatenate([t.reshape(-1, 3) for t in self.textures])
+                                      if self.textures else np.zeros((0, 3)), np.float32)
+        sizes = [t.shape[0] * t.shape[1] for t in self.textures]
+        off = np.ascontiguousarray(np.concatenate([[0], np.cumsum(sizes)[:-1]]) if sizes else np.zeros(0), np.uint64)
+        tw = np.array([t.shape[1] for t in self.textures], np.int32)
+        th = np.array([t.shape[0] for t in self.textures], np.int32)
+        return uv, mt, texels, off, tw, th
 
     @property
     def n_tris(self) -> int:
@@ -81,6 +104,78 @@ def generate_scene(config: int, seed: int = 0) -> SceneData:
     camera = {"from": list(cam.from_), "to": list(cam.to), "up": list(cam.up), "fov": cam.fov_deg,
               "width": int(cam.width), "height": int(cam.height)}
     return SceneData(pos, nrm, mi, mats, env, camera)
+
+
+def load_scene(path, skip_missing: bool = False) -> SceneData:
+    """loadScene (SceneLoader.h:185-269): a reference scene directory (scene.json,
+    .gem meshes, PNG / HDR textures) as a SceneData with its albedo textures.
+    skip_missing drops instances whose mesh file is absent instead of failing."""
+    lib = load()
+    h = C.c_void_p()
+    code = lib.pg_scene_load(str(path).encode(), _lib.PG_LOAD_SKIP_MISSING if skip_missing else 0, C.byref(h))
+    if code != _lib.PG_OK:
+        raise _lib.PgError(code, (lib.pg_scene_load_error() or b"").decode())
+    try:
+        nt, nm = C.c_uint32(), C.c_uint32()
+        ew, eh = C.c_int32(), C.c_int32()
+        check(lib.pg_scene_sizes(h, C.byref(nt), C.byref(nm), C.byref(ew), C.byref(eh)))
+        ntex, nwarn = C.c_uint32(), C.c_uint32()
+        ntexel = C.c_uint64()
+        check(lib.pg_scene_texture_sizes(h, C.byref(ntex), C.byref(ntexel), C.byref(nwarn)))
+        pos = np.empty((nt.value, 3, 3), np.float32)
+        nrm = np.empty((nt.value, 3, 3), np.float32)
+        mi = np.empty(nt.value, np.uint32)
+        mats = np.zeros(nm.value, MATERIAL_DTYPE)
+        env = np.empty((eh.value, ew.value, 3), np.float32) if ew.value > 0 else None
+        cam = PgCameraDesc()
+        check(lib.pg_scene_copy(h, _fp(pos), _fp(nrm), _up(mi), mats.ctypes.data_as(C.POINTER(PgMaterial)),
+                                _fp(env) if env is not None else None, C.byref(cam)))
+        uv = np.empty((nt.value, 3, 2), np.float32)
+        mt = np.empty(nm.value, np.int32)
+        texels = np.empty((ntexel.value, 3), np.float32)
+        off = np.empty(ntex.value, np.uint64)
+        tw = np.empty(ntex.value, np.int32)
+        th = np.empty(ntex.value, np.int32)
+        check(lib.pg_scene_copy_textures(h, _fp(uv), mt.ctypes.data_as(C.POINTER(C.c_int32)), _fp(texels),
+                                         off.ctypes.data_as(C.POINTER(C.c_uint64)),
+                                         tw.ctypes.data_as(C.POINTER(C.c_int32)),
+                                         th.ctypes.data_as(C.POINTER(C.c_int32))))
+        warnings = [lib.pg_scene_warning(h, i).decode() for i in range(nwarn.value)]
+    finally:
+        lib.pg_scene_free(h)
+    textures = [texels[int(o): int(o) + int(w) * int(hh)].reshape(int(hh), int(w), 3).copy()
+                for o, w, hh in zip(off, tw, th)]
+    camera = {"from": list(cam.from_), "to": list(cam.to), "up": list(cam.up), "fov": cam.fov_deg,
+              "width": int(cam.width), "height": int(cam.height)}
+    return SceneData(pos, nrm, mi, mats, env, camera, uv=uv, material_texture=mt, textures=textures,
+                     warnings=warnings)
+
+
+def read_image(path) -> np.ndarray:
+    """Decode a PNG (8-bit samples as 0..255 floats) or Radiance HDR file as the
+    scene loader does: (h, w, channels) float32."""
+    lib = load()
+    w, h, c = C.c_int32(), C.c_int32(), C.c_int32()
+    p = str(path).encode()
+    code = lib.pg_image_read(p, C.byref(w), C.byref(h), C.byref(c), None)
+    if code != _lib.PG_OK:
+        raise _lib.PgError(code, (lib.pg_scene_load_error() or b"").decode())
+    out = np.empty((h.value, w.value, c.value), np.float32)
+    check(lib.pg_image_read(p, C.byref(w), C.byref(h), C.byref(c), _fp(out)))
+    return out
+
+
+def write_png(path, rgb8: np.ndarray) -> None:
+    """8-bit RGB PNG (RayTracer::savePNG, Renderer.h:751)."""
+    a = np.ascontiguousarray(rgb8, np.uint8)
+    check(load().pg_image_write_png(str(path).encode(), a.ctypes.data_as(C.POINTER(C.c_uint8)), a.shape[1],
+                                    a.shape[0]))
+
+
+def write_hdr(path, rgb: np.ndarray) -> None:
+    """Radiance RGBE image of an RGB float image (Film::save, Imaging.h:263-272)."""
+    a = np.ascontiguousarray(rgb, np.float32)
+    check(load().pg_image_write_hdr(str(path).encode(), _fp(a), a.shape[1], a.shape[0]))
 
 
 class Device:
@@ -258,6 +353,12 @@ class Scene:
                                         _fp(env) if env is not None else None,
                                         0 if env is None else env.shape[1], 0 if env is None else env.shape[0],
                                         C.byref(cam)))
+        if data.has_textures:
+            uv, mt, texels, off, tw, th = data.texture_arrays()
+            device._chk(lib.pg_scene_textures(device.ctx, _fp(uv), mt.ctypes.data_as(C.POINTER(C.c_int32)),
+                                              mt.shape[0], _fp(texels), off.ctypes.data_as(C.POINTER(C.c_uint64)),
+                                              tw.ctypes.data_as(C.POINTER(C.c_int32)),
+                                              th.ctypes.data_as(C.POINTER(C.c_int32)), len(data.textures)))
 
     def bvh_info(self) -> dict:
         n, d, ms = C.c_uint32(), C.c_uint32(), C.c_double()
@@ -345,6 +446,20 @@ class RayTracer:
         """Film divided by SPP (Film::save, Imaging.h:263-272)."""
         buf, spp = self.film()
         return buf * (1.0 / spp if spp > 0 else 1.0)
+
+    def tonemap(self, exposure: float = 1.0, lmax_white: float = 10.0) -> np.ndarray:
+        """Film::tonemap (Imaging.h:220-244) of every pixel on the device: (H, W, 3) uint8."""
+        W, H = self.scene.data.width, self.scene.data.height
+        out = np.empty((H, W, 3), np.uint8)
+        self.device._chk(self.device.lib.pg_film_tonemap(self.device.ctx, float(exposure), float(lmax_white),
+                                                         out.ctypes.data_as(C.POINTER(C.c_uint8))))
+        return out
+
+    def savePNG(self, filename) -> None:  # noqa: N802 - reference name (Renderer.h:751)
+        write_png(filename, self.tonemap())
+
+    def saveHDR(self, filename) -> None:  # noqa: N802 - reference name (Renderer.h:750)
+        write_hdr(filename, self.image())
 
     def stats(self) -> dict:
         s = PgStats()

@@ -98,3 +98,113 @@ def seeded_rays(osc, data, n, seed):
     org = org + nrm * (1e-4 * side)[:, None]
     sph = np.concatenate([org, d], 1).astype(np.float32)
     return np.concatenate([cam, sph], 0)
+
+
+# ---------------------------------------------------------------- scene files
+GEM_MAGIC = 4058972161
+
+
+def write_gem(path, meshes):
+    """A static .gem file (the layout GEMLoader.h:268-458 reads): magic, isAnimated 0,
+    mesh count, then per mesh: 0 material properties, vertices (position, normal,
+    tangent, u, v: 11 floats) and uint32 indices. meshes: list of (verts (n, 11), idx)."""
+    import struct
+    out = [struct.pack("<III", GEM_MAGIC, 0, len(meshes))]
+    for verts, idx in meshes:
+        verts = np.ascontiguousarray(verts, np.float32).reshape(-1, 11)
+        idx = np.ascontiguousarray(idx, np.uint32)
+        out += [struct.pack("<I", 0), struct.pack("<I", verts.shape[0]), verts.tobytes(),
+                struct.pack("<I", idx.size), idx.tobytes()]
+    Path(path).write_bytes(b"".join(out))
+
+
+def _quad_mesh(uv_scale=1.0):
+    p = np.array([[-1, 0, -1], [1, 0, -1], [1, 0, 1], [-1, 0, 1]], np.float32)
+    n = np.tile([0, 1, 0], (4, 1)).astype(np.float32)
+    t = np.tile([1, 0, 0], (4, 1)).astype(np.float32)
+    uv = np.array([[0, 0], [1, 0], [1, 1], [0, 1]], np.float32) * uv_scale
+    return np.concatenate([p, n, t, uv], 1), np.array([0, 1, 2, 0, 2, 3], np.uint32)
+
+
+def _box_mesh():
+    verts, idx = [], []
+    for axis in range(3):
+        for s in (-1.0, 1.0):
+            n = np.zeros(3); n[axis] = s
+            u = np.zeros(3); u[(axis + 1) % 3] = 1.0
+            v = np.cross(n, u)
+            base = len(verts)
+            for a, b in ((-1, -1), (1, -1), (1, 1), (-1, 1)):
+                verts.append(np.concatenate([n + a * u + b * v, n, u, [(a + 1) / 2, (b + 1) / 2]]))
+            idx += [base, base + 1, base + 2, base, base + 2, base + 3]
+    return np.asarray(verts, np.float32), np.asarray(idx, np.uint32)
+
+
+def _world(scale, rot_y_deg, translate):
+    a = np.radians(rot_y_deg)
+    c, s = np.cos(a), np.sin(a)
+    R = np.array([[c, 0, s], [0, 1, 0], [-s, 0, c]]) * np.asarray(scale)[None, :]
+    M = np.eye(4)
+    M[:3, :3] = R
+    M[:3, 3] = translate
+    return [float(x) for x in M.reshape(-1).astype(np.float32)]
+
+
+def make_scene_dir(pg, root, width=96, height=64, env=True, extra_instances=()):
+    """A reference-format scene directory written with the library's own writers:
+    textured diffuse floor (8x8 checker PNG), Oren-Nayar wall, plastic box, rough
+    conductor box, a coated (layered) glass box, an emissive quad, an instance with an
+    unknown bsdf (skipped on load), a two-mesh .gem file, and a 64x32 HDR sky."""
+    root = Path(root)
+    root.mkdir(parents=True, exist_ok=True)
+    write_gem(root / "Quad.gem", [_quad_mesh()])
+    write_gem(root / "Box.gem", [_box_mesh()])
+    # two meshes; the first has as many indices as vertices, so the reference's
+    # index offset (indices read so far) addresses the second mesh's vertices
+    tri = np.array([[-1, 0, 0, 0, 0, 1, 1, 0, 0, 0, 0], [1, 0, 0, 0, 0, 1, 1, 0, 0, 1, 0],
+                    [0, 1, 0, 0, 0, 1, 1, 0, 0, 0.5, 1]], np.float32)
+    write_gem(root / "Two.gem", [(tri, [0, 1, 2]), (tri + np.array([0, 0, 0.5] + [0] * 8, np.float32), [0, 2, 1])])
+    chk = np.zeros((8, 8, 3), np.uint8)
+    chk[(np.arange(8)[:, None] + np.arange(8)[None, :]) % 2 == 0] = [230, 200, 90]
+    chk[(np.arange(8)[:, None] + np.arange(8)[None, :]) % 2 == 1] = [40, 90, 160]
+    pg.write_png(root / "checker.png", chk)
+    pg.write_png(root / "grey.png", np.array([[[180, 180, 180]]], np.uint8))
+    pg.write_png(root / "black.png", np.array([[[0, 0, 0]]], np.uint8))
+    grad = np.zeros((4, 4, 3), np.uint8)
+    grad[..., 0] = np.arange(4)[None, :] * 60 + 20
+    grad[..., 1] = np.arange(4)[:, None] * 50 + 40
+    grad[..., 2] = 120
+    pg.write_png(root / "grad.png", grad)
+    insts = [
+        {"filename": "Quad.gem", "world": _world((3, 1, 3), 0, (0, 0, 0)), "bsdf": "diffuse",
+         "reflectance": "checker.png"},
+        {"filename": "Quad.gem", "world": [1, 0, 0, 0, 0, 0, -1, 0, 0, 1, 0, -3, 0, 0, 0, 1],
+         "bsdf": "orennayar", "reflectance": "grey.png", "alpha": "0.6"},
+        {"filename": "Box.gem", "world": _world((0.5, 0.5, 0.5), 20, (-1.2, 0.5, 0.2)), "bsdf": "plastic",
+         "reflectance": "grad.png", "roughness": "0.15", "intIOR": "1.5", "extIOR": "1"},
+        {"filename": "Box.gem", "world": _world((0.4, 0.7, 0.4), -30, (1.1, 0.7, -0.6)), "bsdf": "conductor",
+         "reflectance": "grey.png", "roughness": "0.3", "eta": "0.2 0.9 1.1", "k": "3.9 2.4 2.1"},
+        {"filename": "Box.gem", "world": _world((0.3, 0.3, 0.3), 45, (0.2, 0.3, 1.0)), "bsdf": "glass",
+         "reflectance": "grey.png", "intIOR": "1.5", "extIOR": "1", "coatingThickness": "1",
+         "coatingSigmaA": "0.5 0.5 0.5", "coatingIntIOR": "1.3", "coatingExtIOR": "1"},
+        {"filename": "Two.gem", "world": _world((0.5, 0.5, 0.5), 0, (0.0, 0.5, -1.5)), "bsdf": "plastic",
+         "reflectance": "checker.png", "roughness": "0.6"},
+        {"filename": "Quad.gem", "world": [0.4, 0, 0, 0, 0, -1, 0, 2.5, 0, 0, 0.4, 0, 0, 0, 0, 1],
+         "bsdf": "diffuse", "reflectance": "black.png", "emission": "30 25 20"},
+        {"filename": "Quad.gem", "world": _world((1, 1, 1), 0, (0, 4, 0)), "bsdf": "velvet",
+         "reflectance": "grey.png"},
+        *extra_instances,
+    ]
+    scene = {"width": str(width), "height": str(height), "fov": "50", "from": "0 1.6 4.5", "to": "0 0.7 0",
+             "up": "0 1 0", "instances": insts}
+    if env:
+        w, h = 64, 32
+        v = (np.arange(h) + 0.5) / h
+        sky = np.zeros((h, w, 3), np.float32)
+        sky[...] = (np.array([0.35, 0.5, 0.9]) * np.clip(np.cos(np.pi * v), 0.05, 1)[:, None, None]).astype(np.float32)
+        sky[3, 40] = [400.0, 380.0, 300.0]
+        pg.write_hdr(root / "sky.hdr", sky)
+        scene["envmap"] = "sky.hdr"
+    import json
+    (root / "scene.json").write_text(json.dumps(scene, indent=1))
+    return root

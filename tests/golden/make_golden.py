@@ -4,7 +4,7 @@ Reference vectors (ref_*.npz) come from the reference's own classes, compiled fr
 /root/reference by oracle/Makefile into oracle/_ref/libpgref.so (this container only).
 Oracle images (oracle_*.npz, *_band.json) come from the CPU oracle in FP64.
 
-    python tests/golden/make_golden.py [--ref] [--images]
+    python tests/golden/make_golden.py [--ref] [--images] [--scenes]
 """
 from __future__ import annotations
 
@@ -21,7 +21,12 @@ sys.path.insert(0, str(ROOT))
 sys.path.insert(0, str(ROOT / "tests"))
 OUT = ROOT / "tests" / "golden"
 
-MAT_TYPES = {"diffuse": 0, "mirror": 1, "conductor": 2, "glass": 3, "dielectric": 4}
+MAT_TYPES = {"diffuse": 0, "mirror": 1, "conductor": 2, "glass": 3, "dielectric": 4, "orennayar": 5, "plastic": 6}
+LAYERED = 8
+# reference scene directories whose every mesh file is present (no .MISSING_LARGE_BLOBS entry)
+COMPLETE_SCENES = ("cornell-box", "veach-mis", "veach-bidir", "MaterialsScene", "staircase2", "living-room",
+                   "dining-room")
+SCENES = Path("/root/reference/Scenes")
 
 
 def material(kind, albedo=(0.8, 0.6, 0.4), rough=0.3, eta=(1.0, 0.6, 1.5), k=(0.65, 0.85, 1.0), iors=(1.5, 1.0)):
@@ -176,6 +181,102 @@ def make_ref():
     print("reference vectors written")
 
 
+def make_scenes():
+    """Vectors for the scene-file path: the Oren-Nayar / Plastic / Layered BSDFs,
+    Texture::sample, Film::tonemap, and the reference's loading of its complete
+    scene directories (array hashes; the scene files themselves are not copied)."""
+    from oracle import pg_oracle as po
+    import ctypes as C
+    assert po.ref_available(), "build oracle/_ref first (make -C oracle ref)"
+    R = po.ref()
+    rng = np.random.default_rng(4242)
+    # --- BSDFs (Materials.h:641-897) with injected random numbers
+    cases = [("orennayar", dict(rough=0.0)), ("orennayar", dict(rough=0.35)), ("orennayar", dict(rough=1.0)),
+             ("plastic", dict(rough=0.05, iors=(1.5, 1.0))), ("plastic", dict(rough=0.4, iors=(1.33, 1.0))),
+             ("plastic", dict(rough=1.0, iors=(1.6, 1.1))), ("plastic", dict(rough=1e-7, iors=(1.5, 1.0)))]
+    layered = [("conductor", dict(rough=0.3)), ("glass", dict(rough=0.0)), ("diffuse", dict(rough=0.0)),
+               ("plastic", dict(rough=0.2, iors=(1.5, 1.0)))]
+    mats, sds, us, wes, outs = [], [], [], [], []
+    for kind, kw in cases + [(k, dict(kw, layered=True)) for k, kw in layered]:
+        lay = kw.pop("layered", False)
+        m = material(kind, **kw)
+        if lay:
+            m[0] += LAYERED
+        sd, u, we = bsdf_cases(rng, 384)
+        for i in range(sd.shape[0]):
+            out = np.empty(15, np.float32)
+            R.ref_bsdf(po.fp(m), po.fp(sd[i]), po.fp(u[i]), po.fp(we[i]), po.fp(out))
+            mats.append(m); sds.append(sd[i]); us.append(u[i]); wes.append(we[i]); outs.append(out)
+    np.savez_compressed(OUT / "ref_bsdf_ext.npz", mat=np.array(mats), sd=np.array(sds), u=np.array(us),
+                        wi_eval=np.array(wes), out=np.array(outs))
+    # --- Texture::sample (Imaging.h:82-108): odd sizes, uv beyond [0,1] and negative
+    tex = {}
+    for i, (w, h) in enumerate(((1, 1), (5, 3), (16, 9))):
+        t = rng.random((h, w, 3)).astype(np.float32)
+        tuv = np.concatenate([rng.uniform(-2.5, 3.5, (500, 2)), rng.random((500, 2)),
+                              [[0, 0], [1, 1], [0.999999, 0.5], [-0.0, 1.0], [2.0, -3.0]]]).astype(np.float32)
+        out = np.empty((tuv.shape[0], 3), np.float32)
+        R.ref_texture_sample(po.fp(t), w, h, po.fp(tuv), tuv.shape[0], po.fp(out))
+        tex[f"tex{i}"], tex[f"uv{i}"], tex[f"out{i}"] = t, tuv, out
+    np.savez_compressed(OUT / "ref_texture.npz", **tex)
+    # --- Film::tonemap (Imaging.h:220-244) on accumulated sums
+    n = 4096
+    rgb = (rng.random((n, 3)) ** 3 * 40.0).astype(np.float32)
+    rgb[:16] = 0.0
+    rgb[16:32] *= 1e-4
+    outs = {}
+    for spp, expo, lmax in ((1, 1.0, 10.0), (7, 1.0, 10.0), (64, 2.5, 4.0)):
+        o = np.empty((n, 3), np.uint8)
+        R.ref_tonemap(po.fp(rgb), n, spp, expo, lmax, o.ctypes.data_as(C.POINTER(C.c_uint8)))
+        outs[f"spp{spp}"] = o
+    np.savez_compressed(OUT / "ref_tonemap.npz", rgb=rgb, params=np.array([[1, 1.0, 10.0], [7, 1.0, 10.0],
+                                                                           [64, 2.5, 4.0]], np.float32), **outs)
+    # --- loadScene (SceneLoader.h:104-269) of the complete scene directories
+    summary = {}
+    for name in COMPLETE_SCENES:
+        d = SCENES / name
+        h = R.ref_load_scene(str(d).encode())
+        nm, ntx = C.c_int(), C.c_int()
+        nt = R.ref_loaded_sizes(h, C.byref(nm), C.byref(ntx))
+        pos = np.empty((nt, 3, 3), np.float32); nrm = np.empty((nt, 3, 3), np.float32)
+        uv = np.empty((nt, 3, 2), np.float32); mi = np.empty(nt, np.uint32)
+        mm = np.empty((nm.value, 16), np.float32); mt = np.empty(nm.value, np.int32); cam = np.empty(12, np.float32)
+        R.ref_loaded_copy(h, po.fp(pos), po.fp(nrm), po.fp(uv), po.up(mi), po.fp(mm),
+                          mt.ctypes.data_as(C.POINTER(C.c_int32)), po.fp(cam))
+        names = [R.ref_loaded_texture_name(h, i).decode() for i in range(ntx.value)]
+        R.ref_loaded_free(h)
+        texs = {}
+        for tn in names:
+            if not tn.lower().endswith((".png", ".hdr")):
+                continue  # JPEG / TGA / BMP are not decoded by the loader
+            w, hh = C.c_int(), C.c_int()
+            R.ref_texture_load(tn.encode(), C.byref(w), C.byref(hh), None)
+            buf = np.empty((hh.value, w.value, 3), np.float32)
+            R.ref_texture_load(tn.encode(), C.byref(w), C.byref(hh), po.fp(buf))
+            texs[Path(tn).name] = {"shape": [hh.value, w.value], "sha256": hashlib.sha256(buf.tobytes()).hexdigest()}
+
+        def sha(a):
+            return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+        summary[name] = {"n_tris": int(nt), "n_materials": int(nm.value), "camera": cam.tolist(),
+                         "positions": sha(pos), "normals": sha(nrm), "uv": sha(uv), "material_index": sha(mi),
+                         "materials": sha(mm), "material_texture": mt.tolist(),
+                         "textures": [Path(t).name for t in names], "decoded": texs}
+        if name == "cornell-box":
+            tex_rgb, tex_shape = [], []
+            for tn in names:
+                w, hh = C.c_int(), C.c_int()
+                R.ref_texture_load(tn.encode(), C.byref(w), C.byref(hh), None)
+                buf = np.empty((hh.value, w.value, 3), np.float32)
+                R.ref_texture_load(tn.encode(), C.byref(w), C.byref(hh), po.fp(buf))
+                tex_rgb.append(buf.reshape(-1, 3)); tex_shape.append((hh.value, w.value))
+            np.savez_compressed(OUT / "ref_loader_cornell.npz", positions=pos, normals=nrm, uv=uv,
+                                material_index=mi, materials=mm, material_texture=mt, camera=cam,
+                                tex_rgb=np.concatenate(tex_rgb), tex_shape=np.array(tex_shape, np.int32))
+        print("scene", name, nt, "tris")
+    json.dump(summary, open(OUT / "ref_loader.json", "w"), indent=1)
+    print("scene vectors written")
+
+
 def synthetic_env(w, h):
     v = (np.arange(h) + 0.5) / h
     u = (np.arange(w) + 0.5) / w
@@ -212,8 +313,11 @@ if __name__ == "__main__":
     ap = argparse.ArgumentParser()
     ap.add_argument("--ref", action="store_true")
     ap.add_argument("--images", action="store_true")
+    ap.add_argument("--scenes", action="store_true")
     a = ap.parse_args()
     if a.ref:
         make_ref()
     if a.images:
         make_images()
+    if a.scenes:
+        make_scenes()
