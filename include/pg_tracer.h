@@ -180,6 +180,9 @@ pg_status pg_stats_read(pg_ctx* ctx, pg_stats* stats);
 pg_status pg_stats_reset(pg_ctx* ctx);
 /* Instrumented traversal: count node visits / triangle tests per ray (slower). */
 pg_status pg_set_counting(pg_ctx* ctx, int32_t enable);
+/* Per-launch HIP-event timing of the render kernels (default on; fills the
+ * kernel_ms_extend / shade / shadow statistics). Off leaves kernel_ms_total only. */
+pg_status pg_set_timing(pg_ctx* ctx, int32_t enable);
 /* Measurement aid (no reference counterpart): a plain device-to-device streaming
  * copy of `bytes`, timed with HIP events, as the HBM reference point for achieved
  * bandwidth and for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE readings.          */
@@ -326,6 +329,11 @@ pg_status pg_batch_add_scene(pg_batch* batch, const float* positions, const floa
                              const pg_material* materials, uint32_t n_materials,
                              const float* env_rgb, int32_t env_width, int32_t env_height,
                              uint32_t* scene_id);
+/* Albedo textures of a batch scene: the arguments of pg_scene_textures. */
+pg_status pg_batch_scene_textures(pg_batch* batch, uint32_t scene_id, const float* uv,
+                                  const int32_t* material_texture, uint32_t n_materials, const float* texels,
+                                  const uint64_t* texel_offset, const int32_t* tex_width,
+                                  const int32_t* tex_height, uint32_t n_textures);
 pg_status pg_batch_add_view(pg_batch* batch, uint32_t scene_id, const pg_camera_desc* camera,
                             uint32_t* view_id);
 pg_status pg_batch_configure(pg_batch* batch, const pg_batch_config* cfg);

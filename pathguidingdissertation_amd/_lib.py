@@ -118,6 +118,7 @@ SIGNATURES = {
     "pg_stats_read": (C.c_int32, [_P, C.POINTER(PgStats)]),
     "pg_stats_reset": (C.c_int32, [_P]),
     "pg_set_counting": (C.c_int32, [_P, C.c_int32]),
+    "pg_set_timing": (C.c_int32, [_P, C.c_int32]),
     "pg_copy_benchmark": (C.c_int32, [_P, C.c_uint64, C.c_uint32, C.POINTER(C.c_double)]),
     "pg_records_count": (C.c_int32, [_P, C.POINTER(C.c_uint64)]),
     "pg_records_read": (C.c_int32, [_P, _P, C.c_uint64, C.POINTER(C.c_uint64)]),
@@ -142,6 +143,8 @@ SIGNATURES = {
     "pg_batch_destroy": (C.c_int32, [_P]),
     "pg_batch_add_scene": (C.c_int32, [_P, _F, _F, _U32, C.c_uint32, C.POINTER(PgMaterial), C.c_uint32, _F,
                                        C.c_int32, C.c_int32, _U32]),
+    "pg_batch_scene_textures": (C.c_int32, [_P, C.c_uint32, _F, _I32, C.c_uint32, _F, C.POINTER(C.c_uint64), _I32,
+                                            _I32, C.c_uint32]),
     "pg_batch_add_view": (C.c_int32, [_P, C.c_uint32, C.POINTER(PgCameraDesc), _U32]),
     "pg_batch_configure": (C.c_int32, [_P, C.POINTER(PgBatchConfig)]),
     "pg_batch_autotune": (C.c_int32, [_P, C.POINTER(PgRenderParams), C.POINTER(PgBatchConfig),

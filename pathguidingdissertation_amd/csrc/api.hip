@@ -57,9 +57,7 @@ struct pg_ctx {
     float* d_env_pdf = nullptr;
     float* d_env_prob = nullptr;
     uint32_t* d_env_alias = nullptr;
-    float4* d_tex_texels = nullptr;
-    int4* d_tex_info = nullptr;
-    float2* d_uv = nullptr;
+    pg_internal::TextureSet tex;
     DevScene dev{};
     // path state + film
     PathState ps{};
@@ -163,7 +161,7 @@ struct pg_ctx {
     void free_scene() {
         dfree(d_nodes); dfree(d_tris); dfree(d_shade); dfree(d_mats); dfree(d_lights);
         dfree(d_env_tex); dfree(d_env_pdf); dfree(d_env_prob); dfree(d_env_alias);
-        dfree(d_tex_texels); dfree(d_tex_info); dfree(d_uv);
+        tex.release();
         dev.tex_texels = nullptr; dev.tex_info = nullptr; dev.uv = nullptr;
         has_scene = false;
     }
@@ -204,6 +202,54 @@ hipStream_t stream(pg_ctx* c) { return c->stream; }
 int cu_count(pg_ctx* c) { return c->cu_count; }
 int device(pg_ctx* c) { return c->device; }
 pg_status fail(pg_ctx* c, pg_status code, const std::string& m) { return c->fail(code, m); }
+
+void TextureSet::release() {
+    if (texels) (void)hipFree(texels);
+    if (info) (void)hipFree(info);
+    if (uv) (void)hipFree(uv);
+    texels = nullptr; info = nullptr; uv = nullptr;
+}
+
+std::string upload_textures(TextureSet& set, size_t n_tris, size_t n_mats, const float* uv,
+                            const int32_t* material_texture, uint32_t n_materials, const float* texels,
+                            const uint64_t* texel_offset, const int32_t* tex_width, const int32_t* tex_height,
+                            uint32_t n_textures, pg_status* code) {
+    *code = PG_ERR_INVALID;
+    if (n_materials != n_mats) return "material count differs from the uploaded scene";
+    if (!material_texture || !uv) return "null uv or material_texture";
+    if (n_textures > 0 && (!texels || !texel_offset || !tex_width || !tex_height)) return "null texture array";
+    uint64_t total = 0;
+    for (uint32_t i = 0; i < n_textures; ++i) {
+        if (tex_width[i] < 1 || tex_height[i] < 1) return "texture size must be >= 1x1";
+        uint64_t n = (uint64_t)tex_width[i] * (uint64_t)tex_height[i];
+        if (texel_offset[i] + n > total) total = texel_offset[i] + n;
+    }
+    if (total >= (1ull << 31)) return "more than 2^31 texels";
+    for (uint32_t i = 0; i < n_materials; ++i)
+        if (material_texture[i] < -1 || material_texture[i] >= (int32_t)n_textures)
+            return "material texture index out of range";
+    set.release();
+    std::vector<float4> tx((size_t)total);
+    for (uint64_t k = 0; k < total; ++k) tx[k] = make_float4(texels[3 * k], texels[3 * k + 1], texels[3 * k + 2], 0.f);
+    std::vector<int4> info(n_textures ? n_textures : 1, make_int4(0, 1, 1, 0));
+    for (uint32_t i = 0; i < n_textures; ++i) info[i] = make_int4((int)texel_offset[i], tex_width[i], tex_height[i], 0);
+    *code = PG_ERR_NOMEM;
+    if (hipMalloc((void**)&set.texels, (total ? (size_t)total : 1) * sizeof(float4)) != hipSuccess ||
+        hipMalloc((void**)&set.info, info.size() * sizeof(int4)) != hipSuccess ||
+        hipMalloc((void**)&set.uv, (n_tris ? 3 * n_tris : 1) * sizeof(float2)) != hipSuccess) {
+        set.release();
+        return "alloc textures";
+    }
+    *code = PG_ERR_DEVICE;
+    if ((total && hipMemcpy(set.texels, tx.data(), tx.size() * sizeof(float4), hipMemcpyHostToDevice) != hipSuccess) ||
+        hipMemcpy(set.info, info.data(), info.size() * sizeof(int4), hipMemcpyHostToDevice) != hipSuccess ||
+        (n_tris && hipMemcpy(set.uv, uv, n_tris * 6 * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)) {
+        set.release();
+        return "copy textures";
+    }
+    *code = PG_OK;
+    return std::string();
+}
 }  // namespace pg_internal
 
 #define PG_CHECK_CTX(c)                                  \
@@ -358,36 +404,18 @@ pg_status pg_scene_textures(pg_ctx* c, const float* uv, const int32_t* material_
     PG_CHECK_CTX(c);
     if (!c->has_scene) return c->fail(PG_ERR_STATE, "pg_scene_textures before pg_scene_upload");
     PreparedScene& h = c->host;
-    if (n_materials != h.mats.size()) return c->fail(PG_ERR_INVALID, "material count differs from the uploaded scene");
-    if (!material_texture || !uv) return c->fail(PG_ERR_INVALID, "null uv or material_texture");
-    if (n_textures > 0 && (!texels || !texel_offset || !tex_width || !tex_height))
-        return c->fail(PG_ERR_INVALID, "null texture array");
-    uint64_t total = 0;
-    for (uint32_t i = 0; i < n_textures; ++i) {
-        if (tex_width[i] < 1 || tex_height[i] < 1) return c->fail(PG_ERR_INVALID, "texture size must be >= 1x1");
-        uint64_t n = (uint64_t)tex_width[i] * (uint64_t)tex_height[i];
-        if (texel_offset[i] + n > total) total = texel_offset[i] + n;
-    }
-    if (total >= (1ull << 31)) return c->fail(PG_ERR_INVALID, "more than 2^31 texels");
-    for (uint32_t i = 0; i < n_materials; ++i)
-        if (material_texture[i] < -1 || material_texture[i] >= (int32_t)n_textures)
-            return c->fail(PG_ERR_INVALID, "material texture index out of range");
     PG_TRY(c, hipStreamSynchronize(c->stream), "sync");
-    dfree(c->d_tex_texels); dfree(c->d_tex_info); dfree(c->d_uv);
-    size_t n_tris = h.shade.size();
-    std::vector<float4> tx((size_t)total);
-    for (uint64_t k = 0; k < total; ++k) tx[k] = make_float4(texels[3 * k], texels[3 * k + 1], texels[3 * k + 2], 0.f);
-    std::vector<int4> info(n_textures ? n_textures : 1);
-    for (uint32_t i = 0; i < n_textures; ++i) info[i] = make_int4((int)texel_offset[i], tex_width[i], tex_height[i], 0);
-    PG_TRY(c, dalloc(&c->d_tex_texels, total ? (size_t)total : 1), "alloc textures");
-    PG_TRY(c, dalloc(&c->d_tex_info, info.size()), "alloc texture table");
-    PG_TRY(c, dalloc(&c->d_uv, 3 * n_tris), "alloc uv");
-    if (total) PG_TRY(c, hipMemcpy(c->d_tex_texels, tx.data(), tx.size() * 16, hipMemcpyHostToDevice), "copy textures");
-    PG_TRY(c, hipMemcpy(c->d_tex_info, info.data(), info.size() * sizeof(int4), hipMemcpyHostToDevice), "copy texture table");
-    PG_TRY(c, hipMemcpy(c->d_uv, uv, n_tris * 6 * sizeof(float), hipMemcpyHostToDevice), "copy uv");
+    pg_status code = PG_OK;
+    std::string e = pg_internal::upload_textures(c->tex, h.shade.size(), h.mats.size(), uv, material_texture,
+                                                 n_materials, texels, texel_offset, tex_width, tex_height, n_textures,
+                                                 &code);
+    if (!e.empty()) {
+        c->dev.tex_texels = nullptr; c->dev.tex_info = nullptr; c->dev.uv = nullptr;
+        return c->fail(code, e);
+    }
     for (uint32_t i = 0; i < n_materials; ++i) h.mats[i].tex = material_texture[i];
     PG_TRY(c, hipMemcpy(c->d_mats, h.mats.data(), h.mats.size() * sizeof(MaterialRec), hipMemcpyHostToDevice), "copy mats");
-    c->dev.tex_texels = c->d_tex_texels; c->dev.tex_info = c->d_tex_info; c->dev.uv = c->d_uv;
+    c->dev.tex_texels = c->tex.texels; c->dev.tex_info = c->tex.info; c->dev.uv = c->tex.uv;
     return PG_OK;
 }
 
@@ -1138,6 +1166,12 @@ pg_status pg_copy_benchmark(pg_ctx* c, uint64_t bytes, uint32_t iters, double* m
     dfree(a);
     dfree(b);
     return st;
+}
+
+pg_status pg_set_timing(pg_ctx* c, int32_t enable) {
+    PG_CHECK_CTX(c);
+    c->timing = enable != 0;
+    return PG_OK;
 }
 
 pg_status pg_set_counting(pg_ctx* c, int32_t enable) {

@@ -61,10 +61,12 @@ struct SceneBuf {
     float* env_pdf = nullptr;
     float* env_prob = nullptr;
     uint32_t* env_alias = nullptr;
+    pg_internal::TextureSet tex;
     DevScene dev{};
     ~SceneBuf() {
         bfree(nodes); bfree(tris); bfree(shade); bfree(mats); bfree(lights);
         bfree(env_tex); bfree(env_pdf); bfree(env_prob); bfree(env_alias);
+        tex.release();
     }
     // Same device layout as pg_scene_upload.
     hipError_t upload() {
@@ -422,6 +424,31 @@ pg_status pg_batch_add_scene(pg_batch* b, const float* positions, const float* n
     b->max_depth = std::max(b->max_depth, (int)s->host.bvh.max_depth);
     if (scene_id) *scene_id = (uint32_t)b->scenes.size();
     b->scenes.push_back(std::move(s));
+    b->dirty = true;
+    return PG_OK;
+}
+
+pg_status pg_batch_scene_textures(pg_batch* b, uint32_t scene_id, const float* uv, const int32_t* material_texture,
+                                  uint32_t n_materials, const float* texels, const uint64_t* texel_offset,
+                                  const int32_t* tex_width, const int32_t* tex_height, uint32_t n_textures) {
+    if (!b) return PG_ERR_INVALID;
+    if (scene_id >= b->scenes.size()) return b->fail(PG_ERR_INVALID, "unknown scene");
+    if (hipSetDevice(pg_internal::device(b->ctx)) != hipSuccess) return b->fail(PG_ERR_DEVICE, "hipSetDevice");
+    PG_BTRY(b, hipStreamSynchronize(pg_internal::stream(b->ctx)), "sync");
+    SceneBuf& s = *b->scenes[scene_id];
+    pg_status code = PG_OK;
+    std::string e = pg_internal::upload_textures(s.tex, s.host.shade.size(), s.host.mats.size(), uv, material_texture,
+                                                 n_materials, texels, texel_offset, tex_width, tex_height, n_textures,
+                                                 &code);
+    if (!e.empty()) {
+        s.dev.tex_texels = nullptr; s.dev.tex_info = nullptr; s.dev.uv = nullptr;
+        b->dirty = true;
+        return b->fail(code, e);
+    }
+    for (uint32_t i = 0; i < n_materials; ++i) s.host.mats[i].tex = material_texture[i];
+    PG_BTRY(b, hipMemcpy(s.mats, s.host.mats.data(), s.host.mats.size() * sizeof(MaterialRec), hipMemcpyHostToDevice),
+            "copy mats");
+    s.dev.tex_texels = s.tex.texels; s.dev.tex_info = s.tex.info; s.dev.uv = s.tex.uv;
     b->dirty = true;
     return PG_OK;
 }

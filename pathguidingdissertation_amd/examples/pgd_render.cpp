@@ -7,11 +7,16 @@
 //                                                 one guided record pass + one Adam step
 //                                                 per SPP); writes int32 width, height,
 //                                                 spp + film sum
+//   pgd_render --scene <dir> <spp> <seed> <out.png> [out.hdr]
+//                                                 the reference's main.cpp flow on one of its
+//                                                 scene directories: loadScene, render, savePNG
+//                                                 (and saveHDR)
 //   pgd_render --selftest                        host-only checks (no GPU needed)
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <string>
 #include <vector>
 
 #include "../include/pgd/pgd.hpp"
@@ -39,8 +44,32 @@ static int selftest() {
     return 0;
 }
 
+static int render_scene_dir(int argc, char** argv) {
+    try {
+        pgd::SceneData data = pgd::SceneData::load(argv[2]);
+        for (const std::string& w : data.warnings) std::printf("warning: %s\n", w.c_str());
+        pgd::Device dev(0);
+        pgd::Scene scene(dev, data);
+        pg_render_params p = pg_render_params_default();
+        p.seed = (uint32_t)std::atoi(argv[4]);
+        pgd::RayTracer rt(scene, p);
+        rt.clear();
+        const int spp = std::atoi(argv[3]);
+        for (int i = 0; i < spp; ++i) rt.render();
+        rt.savePNG(argv[5]);
+        if (argc == 7) rt.saveHDR(argv[6]);
+        std::printf("rendered %s: %u triangles, %zu textures, %d spp on %s\n", argv[2], data.triangles(),
+                    data.tex_width.size(), rt.getSPP(), dev.name().c_str());
+    } catch (const pgd::Error& e) {
+        std::fprintf(stderr, "%s\n", e.what());
+        return 4;
+    }
+    return 0;
+}
+
 int main(int argc, char** argv) {
     if (argc == 2 && std::strcmp(argv[1], "--selftest") == 0) return selftest();
+    if ((argc == 6 || argc == 7) && std::strcmp(argv[1], "--scene") == 0) return render_scene_dir(argc, argv);
     const bool learned = argc == 6 && std::strcmp(argv[5], "--learned") == 0;
     if (argc != 5 && !learned) {
         std::fprintf(stderr, "usage: %s <config> <spp> <seed> <out.bin> [--learned] | --selftest\n", argv[0]);

@@ -62,12 +62,58 @@ struct SceneData {
     std::vector<float> env_rgb;       // lat-long RGB32F (empty: no environment map)
     int32_t env_width = 0, env_height = 0;
     pg_camera_desc camera{};
+    // albedo textures of scenes read from files (empty for procedural scenes)
+    std::vector<float> uv;                 // 6 floats per triangle
+    std::vector<int32_t> material_texture; // per material, -1 = constant albedo
+    std::vector<float> texels;             // RGB of every texture, concatenated
+    std::vector<uint64_t> texel_offset;
+    std::vector<int32_t> tex_width, tex_height;
+    std::vector<std::string> warnings;
     uint32_t triangles() const { return (uint32_t)material_index.size(); }
+    bool textured() const { return !tex_width.empty(); }
+
+    // loadScene (SceneLoader.h:185-269): a reference scene directory.
+    static SceneData load(const std::string& dir, uint32_t flags = 0) {
+        pg_host_scene* h = nullptr;
+        pg_status st = pg_scene_load(dir.c_str(), flags, &h);
+        if (st != PG_OK) throw Error(st, pg_scene_load_error());
+        SceneData d = copy_out(h);
+        uint32_t ntex = 0, nwarn = 0;
+        uint64_t ntexel = 0;
+        st = pg_scene_texture_sizes(h, &ntex, &ntexel, &nwarn);
+        if (st == PG_OK) {
+            d.uv.resize((size_t)d.triangles() * 6);
+            d.material_texture.resize(d.materials.size());
+            d.texels.resize((size_t)ntexel * 3);
+            d.texel_offset.resize(ntex);
+            d.tex_width.resize(ntex);
+            d.tex_height.resize(ntex);
+            st = pg_scene_copy_textures(h, d.uv.data(), d.material_texture.data(), d.texels.data(),
+                                        d.texel_offset.data(), d.tex_width.data(), d.tex_height.data());
+            for (uint32_t i = 0; i < nwarn; ++i) d.warnings.push_back(pg_scene_warning(h, i));
+        }
+        pg_scene_free(h);
+        check(st);
+        return d;
+    }
 
     // Procedural scenes of the benchmark configurations (pg_scene_generate).
     static SceneData generate(int32_t config, uint32_t seed) {
         pg_host_scene* h = nullptr;
         check(pg_scene_generate(config, seed, &h));
+        SceneData d;
+        try {
+            d = copy_out(h);
+        } catch (...) {
+            pg_scene_free(h);
+            throw;
+        }
+        pg_scene_free(h);
+        return d;
+    }
+
+private:
+    static SceneData copy_out(pg_host_scene* h) {
         SceneData d;
         uint32_t nt = 0, nm = 0;
         int32_t ew = 0, eh = 0;
@@ -83,7 +129,6 @@ struct SceneData {
             s = pg_scene_copy(h, d.positions.data(), d.normals.data(), d.material_index.data(), d.materials.data(),
                               d.env_rgb.empty() ? nullptr : d.env_rgb.data(), &d.camera);
         }
-        pg_scene_free(h);
         check(s);
         return d;
     }
@@ -98,6 +143,11 @@ public:
                               d.env_rgb.empty() ? nullptr : d.env_rgb.data(), d.env_width, d.env_height,
                               &d.camera),
               dev.get());
+        if (d.textured())
+            check(pg_scene_textures(dev.get(), d.uv.data(), d.material_texture.data(), (uint32_t)d.materials.size(),
+                                    d.texels.data(), d.texel_offset.data(), d.tex_width.data(), d.tex_height.data(),
+                                    (uint32_t)d.tex_width.size()),
+                  dev.get());
     }
     std::vector<pg_hit> traverse(const std::vector<pg_ray>& rays) const {
         std::vector<pg_hit> hits(rays.size());
@@ -149,6 +199,21 @@ public:
         pg_stats s{};
         check(pg_stats_read(ctx(), &s), ctx());
         return s;
+    }
+    // Film::tonemap of every pixel (Imaging.h:220-244), 8-bit RGB.
+    std::vector<uint8_t> tonemap(float exposure = 1.f, float lmax_white = 10.f) const {
+        const pg_camera_desc& c = scene_->data().camera;
+        std::vector<uint8_t> out((size_t)c.width * (size_t)c.height * 3);
+        check(pg_film_tonemap(ctx(), exposure, lmax_white, out.data()), ctx());
+        return out;
+    }
+    void savePNG(const std::string& filename) const {  // NOLINT: reference name (Renderer.h:751)
+        const pg_camera_desc& c = scene_->data().camera;
+        check(pg_image_write_png(filename.c_str(), tonemap().data(), (int32_t)c.width, (int32_t)c.height));
+    }
+    void saveHDR(const std::string& filename) const {  // NOLINT: reference name (Renderer.h:750)
+        const pg_camera_desc& c = scene_->data().camera;
+        check(pg_image_write_hdr(filename.c_str(), image().data(), (int32_t)c.width, (int32_t)c.height));
     }
 
 private:

@@ -469,6 +469,10 @@ class RayTracer:
     def reset_stats(self):
         self.device._chk(self.device.lib.pg_stats_reset(self.device.ctx))
 
+    def set_timing(self, on: bool):
+        """Per-launch kernel timing (HIP events around every render launch)."""
+        self.device._chk(self.device.lib.pg_set_timing(self.device.ctx, 1 if on else 0))
+
     def set_counting(self, on: bool):
         self.device._chk(self.device.lib.pg_set_counting(self.device.ctx, 1 if on else 0))
 
@@ -513,6 +517,12 @@ class Batch:
                                                      _fp(env) if env is not None else None,
                                                      0 if env is None else env.shape[1],
                                                      0 if env is None else env.shape[0], C.byref(sid)))
+        if data.has_textures:
+            uv, mt, texels, off, tw, th = data.texture_arrays()
+            self.device._chk(self.lib.pg_batch_scene_textures(
+                self.h, sid.value, _fp(uv), mt.ctypes.data_as(C.POINTER(C.c_int32)), mt.shape[0], _fp(texels),
+                off.ctypes.data_as(C.POINTER(C.c_uint64)), tw.ctypes.data_as(C.POINTER(C.c_int32)),
+                th.ctypes.data_as(C.POINTER(C.c_int32)), len(data.textures)))
         return sid.value
 
     def add_view(self, scene_id: int, camera: dict) -> int:

@@ -171,3 +171,23 @@ def test_batch_learned_guides_equal_single_renders(pg, device, cornell, small_in
     b.close()
     assert np.array_equal(f0, _single(pg, device, cornell, views[0][1], 2, 7))
     assert np.array_equal(f1, _single_guided(pg, device, small_interior, views[1][1], guides[1], 7, 2))
+
+
+def test_textured_scene_in_batch_equals_single_render(pg, device, cornell, tmp_path):
+    """A scene read from files (albedo textures, Oren-Nayar / Plastic / Layered) in a
+    batch beside a procedural one: its views equal its single-view renders."""
+    from conftest import make_scene_dir
+    tex = pg.load_scene(make_scene_dir(pg, tmp_path / "scene", width=64, height=48))
+    b = pg.Batch(device)
+    assert b.add_scene(cornell) == 0 and b.add_scene(tex) == 1
+    vs = [(1, _cam(tex, 64, 48)), (0, _cam(cornell, 40, 40)), (1, _cam(tex, 50, 30, **{"fov": 70.0}))]
+    for sid, cam in vs:
+        b.add_view(sid, cam)
+    b.configure(use_graph=1)
+    b.clear()
+    b.render(2, seed=4, max_depth=12)
+    films = [b.film(v)[0] for v in range(len(vs))]
+    b.close()
+    for v, (sid, cam) in enumerate(vs):
+        ref = _single(pg, device, [cornell, tex][sid], cam, 2, 4)
+        assert np.array_equal(films[v], ref), f"view {v}"
