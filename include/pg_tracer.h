@@ -386,7 +386,8 @@ pg_status pg_scene_free(pg_host_scene* s);
  * file that does not decode becomes the 1x1 white default, Imaging.h:36-42).
  * flags: PG_LOAD_SKIP_MISSING skips instances whose mesh file is missing or not a
  * GEM file (the reference exits instead). Errors: PG_ERR_INVALID with the reason in
- * pg_scene_load_error(). Only 8-bit non-interlaced PNGs are decoded.             */
+ * pg_scene_load_error(). Images: 8-bit non-interlaced PNG, baseline and
+ * progressive Huffman JPEG (8-bit, gray or YCbCr), Radiance HDR.                  */
 enum { PG_LOAD_SKIP_MISSING = 1 };
 pg_status pg_scene_load(const char* scene_dir, uint32_t flags, pg_host_scene** out);
 const char* pg_scene_load_error(void);
@@ -397,8 +398,8 @@ pg_status pg_scene_texture_sizes(const pg_host_scene* s, uint32_t* n_textures, u
 pg_status pg_scene_copy_textures(const pg_host_scene* s, float* uv, int32_t* material_texture, float* texels,
                                  uint64_t* texel_offset, int32_t* tex_width, int32_t* tex_height);
 const char* pg_scene_warning(const pg_host_scene* s, uint32_t i);
-/* Decode one image as Texture::load's decoder does (8-bit PNG samples as 0..255
- * floats, Radiance HDR as floats), `channels` values per pixel; rgb nullable.     */
+/* Decode one image as Texture::load's decoder does (8-bit PNG / JPEG samples as
+ * 0..255 floats, Radiance HDR as floats), `channels` values per pixel; rgb nullable. */
 pg_status pg_image_read(const char* path, int32_t* width, int32_t* height, int32_t* channels, float* rgb);
 /* RayTracer::savePNG (Renderer.h:751): 8-bit RGB PNG. */
 pg_status pg_image_write_png(const char* path, const uint8_t* rgb, int32_t width, int32_t height);

@@ -18,7 +18,7 @@ BUILD = PKG / "_build"
 LIB = PKG / "libpgtrace.so"
 EXAMPLE = PKG / "pgd_render"  # example host program over the C++ mirror
 
-HOST_SOURCES = ["host_prep.cpp", "scenes.cpp", "metrics.cpp", "scene_loader.cpp"]
+HOST_SOURCES = ["host_prep.cpp", "scenes.cpp", "metrics.cpp", "scene_loader.cpp", "jpeg_decode.cpp"]
 HIP_SOURCES = ["kernels.hip", "guiding.hip", "mlp.hip", "batch.hip", "api.hip"]
 
 CXXFLAGS = ["-std=c++17", "-O3", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-pthread", "-Wall",

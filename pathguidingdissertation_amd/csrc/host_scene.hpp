@@ -36,4 +36,8 @@ struct Image {
 // Returns an empty string on success, else the reason.
 std::string read_png(const std::string& path, Image& out);
 std::string read_hdr(const std::string& path, Image& out);
+std::string read_jpeg(const std::string& path, Image& out);
+// Texture::load's decoding: ".hdr" in the name selects Radiance HDR; otherwise the
+// file's signature picks PNG or JPEG (other formats are reported as unsupported).
+std::string read_image_file(const std::string& path, Image& out, bool& hdr);
 }  // namespace pg

@@ -253,14 +253,14 @@ struct TexCache {
     std::map<std::string, int32_t> index;
 };
 
-// Texture::load: ".hdr" in the name selects the float path; otherwise 8-bit samples
-// times 1/255. A file that does not decode becomes the 1x1 white default.
+// Texture::load: ".hdr" in the name selects the float path; otherwise 8-bit PNG or
+// JPEG samples times 1/255. A file that does not decode becomes the 1x1 white default.
 int32_t add_texture(pg_host_scene* s, TexCache& cache, const std::string& path) {
     auto it = cache.index.find(path);
     if (it != cache.index.end()) return it->second;
     Image img;
-    bool hdr = path.find(".hdr") != std::string::npos;
-    std::string e = hdr ? read_hdr(path, img) : read_png(path, img);
+    bool hdr = false;
+    std::string e = read_image_file(path, img, hdr);
     int32_t id = (int32_t)s->tex_w.size();
     s->tex_offset.push_back(s->texels.size() / 3);
     s->tex_path.push_back(path);
@@ -416,6 +416,19 @@ std::string read_png(const std::string& path, Image& out) {
         out.u8 = std::move(px);
     }
     return std::string();
+}
+
+std::string read_image_file(const std::string& path, Image& out, bool& hdr) {
+    hdr = path.find(".hdr") != std::string::npos;
+    if (hdr) return read_hdr(path, out);
+    uint8_t sig[8] = {0};
+    FILE* f = std::fopen(path.c_str(), "rb");
+    if (!f) return "cannot read";
+    size_t got = std::fread(sig, 1, 8, f);
+    std::fclose(f);
+    if (got >= 8 && sig[0] == 137 && sig[1] == 'P' && sig[2] == 'N' && sig[3] == 'G') return read_png(path, out);
+    if (got >= 3 && sig[0] == 0xFF && sig[1] == 0xD8 && sig[2] == 0xFF) return read_jpeg(path, out);
+    return "unsupported image format (PNG, JPEG and Radiance HDR are decoded)";
 }
 
 // ------------------------------------------------------------------ Radiance HDR
@@ -713,8 +726,8 @@ pg_status pg_image_read(const char* path, int32_t* width, int32_t* height, int32
     if (!path || !width || !height) return PG_ERR_INVALID;
     pg::Image img;
     std::string p(path);
-    bool hdr = p.find(".hdr") != std::string::npos;
-    std::string e = hdr ? pg::read_hdr(p, img) : pg::read_png(p, img);
+    bool hdr = false;
+    std::string e = pg::read_image_file(p, img, hdr);
     if (!e.empty()) { g_load_error = p + ": " + e; return PG_ERR_INVALID; }
     *width = img.w; *height = img.h;
     if (channels) *channels = img.channels;

@@ -233,6 +233,7 @@ def make_scenes():
                                                                            [64, 2.5, 4.0]], np.float32), **outs)
     # --- loadScene (SceneLoader.h:104-269) of the complete scene directories
     summary = {}
+    jpeg_samples = {}
     for name in COMPLETE_SCENES:
         d = SCENES / name
         h = R.ref_load_scene(str(d).encode())
@@ -247,13 +248,24 @@ def make_scenes():
         R.ref_loaded_free(h)
         texs = {}
         for tn in names:
-            if not tn.lower().endswith((".png", ".hdr")):
-                continue  # JPEG / TGA / BMP are not decoded by the loader
             w, hh = C.c_int(), C.c_int()
             R.ref_texture_load(tn.encode(), C.byref(w), C.byref(hh), None)
+            if w.value <= 0 or hh.value <= 0:
+                continue
             buf = np.empty((hh.value, w.value, 3), np.float32)
             R.ref_texture_load(tn.encode(), C.byref(w), C.byref(hh), po.fp(buf))
-            texs[Path(tn).name] = {"shape": [hh.value, w.value], "sha256": hashlib.sha256(buf.tobytes()).hexdigest()}
+            if tn.lower().endswith((".png", ".hdr")):
+                texs[Path(tn).name] = {"shape": [hh.value, w.value],
+                                       "sha256": hashlib.sha256(buf.tobytes()).hexdigest()}
+            elif tn.lower().endswith((".jpg", ".jpeg")):
+                # JPEG decoders may differ by a few levels (the IDCT and colour conversion
+                # are not bit-specified): 512 sampled pixels of the reference's decode
+                srng = np.random.default_rng(len(jpeg_samples) + 1)
+                ys, xs = srng.integers(0, hh.value, 512), srng.integers(0, w.value, 512)
+                key = f"{name}/{Path(tn).name}"
+                jpeg_samples[key + ":yx"] = np.stack([ys, xs], 1).astype(np.int32)
+                jpeg_samples[key + ":rgb8"] = np.rint(buf[ys, xs] * 255).astype(np.uint8)
+                texs[Path(tn).name] = {"shape": [hh.value, w.value], "jpeg_samples": key}
 
         def sha(a):
             return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
@@ -274,6 +286,7 @@ def make_scenes():
                                 tex_rgb=np.concatenate(tex_rgb), tex_shape=np.array(tex_shape, np.int32))
         print("scene", name, nt, "tris")
     json.dump(summary, open(OUT / "ref_loader.json", "w"), indent=1)
+    np.savez_compressed(OUT / "ref_jpeg_samples.npz", **jpeg_samples)
     print("scene vectors written")
 
 
