@@ -116,7 +116,8 @@ def time_staying_below(curve, e, col=2):
     return hit
 
 
-def time_to_relmse(pg, dev, sc, data, max_depth, ref_spp, cap_s, log, target=0.01, env_visibility=1, repeats=1):
+def time_to_relmse(pg, dev, sc, data, max_depth, ref_spp, cap_s, log, target=0.01, env_visibility=1, repeats=1,
+                   with_pss=False):
     """Time-to-relMSE study: unguided vs quadtree guide vs learned guide. Besides the
     time to relMSE <= target (null when a method does not reach it within cap_s), the
     learned guide is compared with each baseline at that baseline's own final relMSE:
@@ -138,16 +139,27 @@ def time_to_relmse(pg, dev, sc, data, max_depth, ref_spp, cap_s, log, target=0.0
     def run_once(r):
         un = pg.RayTracer(sc, seed=2 + 10 * r, max_depth=max_depth, **ev)
         qt = pg.RayTracer(sc, seed=3 + 10 * r, max_depth=max_depth, integrator=2, guide_prob=0.5, **ev)
+        ps = pg.RayTracer(sc, seed=4 + 10 * r, max_depth=max_depth, integrator=4, guide_prob=0.5, **ev)
+        pf = pg.RayTracer(sc, seed=8 + 10 * r, max_depth=max_depth, integrator=4, guide_prob=0.5, knn_k=0, **ev)
         ml = pg.RayTracer(sc, seed=5 + 10 * r, max_depth=max_depth, record_nee=1, **ev)
 
-        def qt_setup():
+        def cache_setup(rt, seed):
             # pass 1: 1 SPP record pass (Li with NEE), then radix sort + point BVH + quadtrees
             dev.clear_records()
-            rec = pg.RayTracer(sc, seed=1 + 10 * r, max_depth=max_depth, integrator=1, record=1, record_nee=1, **ev)
+            rec = pg.RayTracer(sc, seed=seed, max_depth=max_depth, integrator=1, record=1, record_nee=1, **ev)
             rec.clear()
             rec.render(1)
             dev.cache_build(cell_size=4096)
-            qt.clear()
+            rt.clear()
+
+        def qt_setup():
+            cache_setup(qt, 1 + 10 * r)
+
+        def ps_setup():
+            cache_setup(ps, 6 + 10 * r)
+
+        def pf_setup():
+            cache_setup(pf, 9 + 10 * r)
 
         def ml_setup():
             dev.mlp_init(seed=7 + r)
@@ -156,8 +168,11 @@ def time_to_relmse(pg, dev, sc, data, max_depth, ref_spp, cap_s, log, target=0.0
         methods = {
             "unguided": (lambda: un.clear(), adv(un, lambda k: un.render(k))),
             "quadtree": (qt_setup, adv(qt, lambda k: qt.render(k))),
-            "learned": (ml_setup, adv(ml, lambda k: ml.render_online(k, steps=1, window=1))),
         }
+        if with_pss:
+            methods["pss"] = (ps_setup, adv(ps, lambda k: ps.render(k)))
+            methods["pss_approx"] = (pf_setup, adv(pf, lambda k: pf.render(k)))
+        methods["learned"] = (ml_setup, adv(ml, lambda k: ml.render_online(k, steps=1, window=1)))
         res = {}
         for name, (setup, advance) in methods.items():
             curve, hit = progressive(pg, name if r == 0 else f"{name} (run {r})", ref, setup, advance, cap_s,
@@ -187,6 +202,12 @@ def time_to_relmse(pg, dev, sc, data, max_depth, ref_spp, cap_s, log, target=0.0
                "trimmed_relmse_ratio_at_cap": {
                    "unguided_over_learned": ratio("unguided", "learned", "final_trimmed_relmse"),
                    "quadtree_over_learned": ratio("quadtree", "learned", "final_trimmed_relmse")}}
+        if with_pss:
+            for v in ("pss", "pss_approx"):
+                cmp[f"{v}_vs_unguided"] = speedup(v, "unguided")
+                cmp["trimmed"][f"{v}_vs_unguided"] = speedup(v, "unguided", 3)
+                cmp["relmse_ratio_at_cap"][f"unguided_over_{v}"] = ratio("unguided", v, "final_relmse")
+                cmp["trimmed_relmse_ratio_at_cap"][f"unguided_over_{v}"] = ratio("unguided", v, "final_trimmed_relmse")
         return res, cmp
 
     res, cmp = run_once(0)
@@ -207,9 +228,14 @@ def time_to_relmse(pg, dev, sc, data, max_depth, ref_spp, cap_s, log, target=0.0
                               "unguided MIS estimators then converge to different images)"
                               if env_visibility else "Renderer.h:331-332 scene-bound target"),
            "curve_columns": ["spp", "seconds", "relmse", "relmse_without_top_0.01pct_pixels"],
-           "methods": {"unguided": "path tracer with NEE + MIS (Renderer.h:531-618)",
+           "methods": {k: v for k, v in {"unguided": "path tracer with NEE + MIS (Renderer.h:531-618)",
                        "quadtree": "1-SPP record pass, point-BVH + per-cell quadtrees, 50/50 one-sample MIS, kNN k=16",
-                       "learned": "online: 1-SPP guided record passes, 1 Adam step of 2^18 samples per pass"},
+                       "pss": "1-SPP record pass, point-BVH; per vertex the 16 nearest records inverted into the "
+                              "BSDF's primary sample space (Renderer.h:398-414), 4x4 flux histogram, 50/50 MIS",
+                       "pss_approx": "as pss with approximate neighbours (16 nearest of the 64 records under a "
+                                     "greedy point-BVH descent)",
+                       "learned": "online: 1-SPP guided record passes, 1 Adam step of 2^18 samples per pass"}.items()
+                       if k in res},
            **res,
            "learned_diverged": bool(info["diverged"]),
            "comparison": cmp}
@@ -320,6 +346,8 @@ def main() -> int:
     ap.add_argument("--ttr-env-visibility", type=int, choices=(0, 1), default=1,
                     help="environment shadow rays in the study: 1 along the light direction (default), "
                          "0 the reference's scene-bound target (Renderer.h:331-332)")
+    ap.add_argument("--ttr-pss", action="store_true",
+                    help="add the primary-sample-space guide (exact and approximate neighbours) to the study")
     ap.add_argument("--copy-check", action="store_true",
                     help="also time a plain 1 GiB device copy (HBM reference point / counter scaling)")
     a = ap.parse_args()
@@ -419,7 +447,7 @@ def main() -> int:
         tsc = sc if a.ttr_config == a.config else pg.Scene(dev, tdata)
         ttr = time_to_relmse(pg, dev, tsc, tdata, max_depth, a.ttr_ref_spp, a.ttr_cap,
                              lambda m: print(m, file=sys.stderr, flush=True), env_visibility=a.ttr_env_visibility,
-                             repeats=a.ttr_repeats)
+                             repeats=a.ttr_repeats, with_pss=a.ttr_pss)
         ttr = {"workload": f"config {a.ttr_config}: {tdata.n_tris} tris, {tdata.width}x{tdata.height}, "
                            f"max depth {max_depth}", **ttr}
 

@@ -67,8 +67,15 @@ typedef struct { uint32_t id; float t, alpha, beta, gamma; } pg_hit;
 /* Path-vertex record (Renderer.h:42-47), 48 bytes in HBM. */
 typedef struct { float pos[3]; float normal[3]; float wi[3]; float li[3]; } pg_path_vertex;
 
+/* PG_INTEGRATOR_GUIDED_PSS: the guiding the reference outlines (Renderer.h:398-414):
+ * at each vertex the k nearest records' directions are projected into the BSDF's
+ * primary sample space by BSDF::invert, binned by flux over a 4x4 (u, v) grid, and
+ * the grid is sampled and mapped back through the BSDF's warp (one-sample MIS with
+ * plain BSDF sampling). Guides Diffuse, Oren-Nayar and rough Conductor vertices;
+ * needs pg_cache_build like the quadtree guide. knn_k 16: exact 16 nearest records;
+ * 0: approximate neighbours (cheaper).                                           */
 enum { PG_INTEGRATOR_PATH = 0, PG_INTEGRATOR_RECORD = 1, PG_INTEGRATOR_GUIDED_QT = 2,
-       PG_INTEGRATOR_GUIDED_MLP = 3 };
+       PG_INTEGRATOR_GUIDED_MLP = 3, PG_INTEGRATOR_GUIDED_PSS = 4 };
 
 /* Render parameters: the reference's hard-coded constants become fields whose
  * defaults (pg_render_params_default) equal the reference.                    */
@@ -79,7 +86,9 @@ typedef struct {
     int32_t integrator;     /* PG_INTEGRATOR_*                                  */
     uint32_t record;        /* RECORD integrator: keep PathVertex records       */
     float guide_prob;       /* guided integrators: P(sample guide), 0.5         */
-    uint32_t knn_k;         /* guided QT: neighbours per lookup (16)            */
+    uint32_t knn_k;         /* guided QT / PSS: neighbours per lookup (16); PSS
+                               with 0 takes the 16 nearest of the 64 records under
+                               a greedy point-BVH descent instead of an exact search */
     uint32_t record_nee;    /* RECORD: 0 records Li = G (Renderer.h:390, the
                                reference); 1 records Li with NEE and MIS-weighted
                                emission (the pixelColour recursion, :391)       */

@@ -22,6 +22,7 @@ PG_INTEGRATOR_PATH = 0
 PG_INTEGRATOR_RECORD = 1
 PG_INTEGRATOR_GUIDED_QT = 2
 PG_INTEGRATOR_GUIDED_MLP = 3
+PG_INTEGRATOR_GUIDED_PSS = 4
 
 PG_LOAD_SKIP_MISSING = 1
 

@@ -653,15 +653,16 @@ pg_status pg_render_spp(pg_ctx* c, const pg_render_params* params, uint32_t n_sp
     pg_render_params prm = params ? *params : pg_render_params_default();
     const int integ = prm.integrator;
     if (integ != PG_INTEGRATOR_PATH && integ != PG_INTEGRATOR_RECORD && integ != PG_INTEGRATOR_GUIDED_QT &&
-        integ != PG_INTEGRATOR_GUIDED_MLP)
+        integ != PG_INTEGRATOR_GUIDED_MLP && integ != PG_INTEGRATOR_GUIDED_PSS)
         return c->fail(PG_ERR_INVALID, "unknown integrator");
-    if (integ == PG_INTEGRATOR_GUIDED_QT && !c->cache_ready)
+    if ((integ == PG_INTEGRATOR_GUIDED_QT || integ == PG_INTEGRATOR_GUIDED_PSS) && !c->cache_ready)
         return c->fail(PG_ERR_STATE, "guided rendering needs pg_cache_build first");
     if (integ == PG_INTEGRATOR_GUIDED_MLP && !c->mlp.ready)
         return c->fail(PG_ERR_STATE, "learned guiding needs pg_mlp_init first");
-    // shade mode: 0 BSDF sampling, 1 quadtree guide, 2 learned guide; a learned guide
-    // whose training diverged falls back to the quadtree guide when a cache exists
-    int mode = integ == PG_INTEGRATOR_GUIDED_QT ? 1 : 0;
+    // shade mode: 0 BSDF sampling, 1 quadtree guide, 2 learned guide, 3 primary-sample-
+    // space guide; a learned guide whose training diverged falls back to the quadtree
+    // guide when a cache exists
+    int mode = integ == PG_INTEGRATOR_GUIDED_QT ? 1 : (integ == PG_INTEGRATOR_GUIDED_PSS ? 3 : 0);
     if (integ == PG_INTEGRATOR_GUIDED_MLP) {
         if (!c->mlp.diverged) {
             mode = 2;
@@ -673,6 +674,8 @@ pg_status pg_render_spp(pg_ctx* c, const pg_render_params* params, uint32_t n_sp
     if (prm.max_depth > 64) return c->fail(PG_ERR_INVALID, "max_depth > 64");
     if (!(prm.guide_prob >= 0.f && prm.guide_prob <= 1.f)) return c->fail(PG_ERR_INVALID, "guide_prob outside [0,1]");
     if (prm.env_visibility > 1u) return c->fail(PG_ERR_INVALID, "env_visibility must be 0 or 1");
+    if (integ == PG_INTEGRATOR_GUIDED_PSS && prm.knn_k != 0 && prm.knn_k != (uint32_t)kKnnMax)
+        return c->fail(PG_ERR_INVALID, "primary-sample-space guide: knn_k must be 16 (exact) or 0 (approximate)");
     uint32_t W = (uint32_t)c->host.camera.width, H = (uint32_t)c->host.camera.height;
     uint32_t n = W * H;
     pg_status st = ensure_paths(c, n);
@@ -681,13 +684,13 @@ pg_status pg_render_spp(pg_ctx* c, const pg_render_params* params, uint32_t n_sp
     const bool record_pass = integ == PG_INTEGRATOR_RECORD || (integ == PG_INTEGRATOR_GUIDED_MLP && prm.record);
     if (record_pass && (st = ensure_record_state(c, n, prm.max_depth + 1)) != PG_OK) return st;
     c->rs.li_nee = prm.record_nee ? 1 : 0;
-    if (mode == 1 && c->guide_cells_cap < n) {
+    if ((mode == 1 || mode == 3) && c->guide_cells_cap < n) {
         dfree(c->guide_cells);
         PG_TRY(c, dalloc(&c->guide_cells, (size_t)n * kKnnMax), "alloc guide cells");
         c->guide_cells_cap = n;
     }
     // sort buffers: kNN query order (quadtree guide) and ray trace order (sort_rays)
-    if ((mode == 1 || prm.sort_rays) && c->q_cap < n) {
+    if ((mode == 1 || (mode == 3 && prm.knn_k != 0) || prm.sort_rays) && c->q_cap < n) {
         dfree(c->q_keys); dfree(c->q_vals); dfree(c->q_keys_tmp); dfree(c->q_vals_tmp); dfree(c->q_hist); dfree(c->q_scan);
         PG_TRY(c, dalloc(&c->q_keys, n), "alloc sort buffers");
         PG_TRY(c, dalloc(&c->q_vals, n), "alloc sort buffers");
@@ -720,7 +723,7 @@ pg_status pg_render_spp(pg_ctx* c, const pg_render_params* params, uint32_t n_sp
     const uint32_t grid_shade = (uint32_t)c->cu_count * (prm.shade_blocks_per_cu ? prm.shade_blocks_per_cu : 8u);
     const uint32_t grid_guide = (uint32_t)c->cu_count * 5;
     const uint32_t grid_mlp = (uint32_t)c->cu_count * 2;
-    GuideCtx gc{c->gv, mode == 1 ? c->guide_cells : nullptr, mode == 2 ? c->mix : nullptr};
+    GuideCtx gc{c->gv, (mode == 1 || mode == 3) ? c->guide_cells : nullptr, mode == 2 ? c->mix : nullptr};
     gc.g.guide_prob = prm.guide_prob;
     c->pool_used = 0;
     c->launches.clear();
@@ -760,6 +763,19 @@ pg_status pg_render_spp(pg_ctx* c, const pg_render_params* params, uint32_t n_sp
                 }
                 continue;
             }
+            if (mode == 3 && prm.knn_k == 0) {
+                // approximate neighbours: no query sort needed (no search stack)
+                PG_TRY(c, timed(4, [&] { return launch_guide_cells(c->dev, c->ps, c->gv, c->guide_cells, parity, grid_guide, c->stream, nullptr, 0, 2); }), "guide records");
+            } else if (mode == 3) {
+                PG_TRY(c, timed(4, [&] {
+                    hipError_t e = launch_query_keys(c->dev, c->ps, q_lo, q_inv, parity, n, c->q_keys, c->q_vals, c->stream);
+                    if (e == hipSuccess)
+                        e = radix_sort_pairs(c->q_keys, c->q_vals, c->q_keys_tmp, c->q_vals_tmp, n, 24, c->q_hist,
+                                             c->q_scan, c->stream);
+                    return e;
+                }), "guide query order");
+                PG_TRY(c, timed(4, [&] { return launch_guide_cells(c->dev, c->ps, c->gv, c->guide_cells, parity, grid_guide, c->stream, c->q_vals, n, 1); }), "guide records");
+            }
             if (mode == 1) {
                 PG_TRY(c, timed(4, [&] {
                     hipError_t e = launch_query_keys(c->dev, c->ps, q_lo, q_inv, parity, n, c->q_keys, c->q_vals, c->stream);
@@ -768,7 +784,7 @@ pg_status pg_render_spp(pg_ctx* c, const pg_render_params* params, uint32_t n_sp
                                              c->q_scan, c->stream);
                     return e;
                 }), "guide query order");
-                PG_TRY(c, timed(4, [&] { return launch_guide_cells(c->dev, c->ps, c->gv, c->guide_cells, parity, grid_guide, c->stream, c->q_vals, n); }), "guide_cells");
+                PG_TRY(c, timed(4, [&] { return launch_guide_cells(c->dev, c->ps, c->gv, c->guide_cells, parity, grid_guide, c->stream, c->q_vals, n, 0); }), "guide_cells");
             }
             if (mode == 2)
                 PG_TRY(c, timed(4, [&] { return launch_mlp_infer(c->dev, c->ps, c->mlp.view, parity, c->mix, grid_mlp, c->stream); }), "mlp_infer");
@@ -975,6 +991,7 @@ pg_status pg_cache_build(pg_ctx* c, uint32_t cell_size, uint32_t qt_nodes_per_ce
     g.rec_pos = c->cb.pos;
     g.nodes = c->cb.nodes;
     g.qt = c->cb.qt;
+    g.rec_dir = c->cb.dir;
     g.guide_prob = 0.5f;
     c->cache_ready = true;
     return PG_OK;

@@ -667,6 +667,50 @@ __device__ inline F3 bsdf_sample(const MaterialRec& m, const SD& sd, uint32_t ke
 }
 
 
+// BSDF::invert (Materials.h:163-171, 287-306, 672-681): the primary-sample-space
+// point (u, v) that sample() maps to wi. Defined for the BSDFs whose sampling is a
+// two-number warp: Diffuse and Oren-Nayar (cosine hemisphere) and the rough
+// Conductor (GGX half vector); false for the rest (the reference's Plastic and
+// Dielectric inverts are empty; delta BSDFs have none) and for directions the warp
+// cannot produce (below the surface, or a half vector below it). At alpha = 1 the
+// conductor's expression is 0/0; its limit u = 1 - cos^2(theta_m) is used there.
+__device__ __forceinline__ bool pss_invertible(const MaterialRec& m) {
+    return m.type == PG_BSDF_DIFFUSE || m.type == PG_BSDF_ORENNAYAR || (m.type == PG_BSDF_CONDUCTOR && !(m.alpha < kEps));
+}
+__device__ __forceinline__ bool bsdf_invert(const MaterialRec& m, const SD& sd, F3 wi, float& u, float& v) {
+    F3 wl = to_local(sd.fr, wi);
+    if (m.type == PG_BSDF_DIFFUSE || m.type == PG_BSDF_ORENNAYAR) {
+        if (!(wl.z > 0.f)) return false;
+        float c = cosf(acosf(wl.z));
+        u = c * c;
+        v = (float)((double)spherical_phi(wl) / (2.0 * kPiD));
+    } else if (m.type == PG_BSDF_CONDUCTOR && !(m.alpha < kEps)) {
+        F3 wm = normalize(wl + to_local(sd.fr, sd.wo));
+        if (!(wm.z > 0.f)) return false;
+        float c = cosf(acosf(wm.z));
+        float a2 = m.alpha * m.alpha, am1 = a2 - 1.f;
+        u = am1 == 0.f ? 1.f - c * c : (a2 / ((c * c) * (am1 * am1) + am1)) - (1.f / am1);
+        v = (float)((double)spherical_phi(wm) / (2.0 * kPiD));
+    } else {
+        return false;
+    }
+    u = fminf(fmaxf(u, 0.f), 0.99999994f);
+    v = fminf(fmaxf(v, 0.f), 0.99999994f);
+    return true;
+}
+// sample() of an invertible BSDF with its two random numbers given (the warp that
+// bsdf_invert inverts): Materials.h:146-161 / 655-670 (cosine) and 253-285 (GGX).
+__device__ __forceinline__ F3 bsdf_warp(const MaterialRec& m, const SD& sd, float r1, float r2) {
+    if (m.type == PG_BSDF_CONDUCTOR) {
+        F3 ol = to_local(sd.fr, sd.wo);
+        float a2 = m.alpha * m.alpha;
+        float tm = acosf(sqrtf((1.f - r1) / (r1 * (a2 - 1.f) + 1.f)));
+        F3 wm = spherical_to_world(tm, (float)(2.0 * kPiD * (double)r2));
+        return to_world(sd.fr, -ol + (wm * 2.f) * dot(wm, ol));
+    }
+    return to_world(sd.fr, spherical_to_world(acosf(sqrtf(r1)), (float)(2.0 * kPiD * (double)r2)));
+}
+
 // Scene::visible (Scene.h:139-146): shadow ray from p1 toward p2.
 __device__ __forceinline__ void make_visible_ray(F3 p1, F3 p2, F3& o, F3& d, float& max_t) {
     F3 dir = p2 - p1;

@@ -44,6 +44,7 @@ struct GuideView {
     const float4* rec_pos;      // sorted: xyz, original index bits
     const float4* nodes;        // 2 * (2P - 1) float4: bmin, bmax
     const QtNode* qt;           // n_cells * qt_stride
+    const float4* rec_dir;      // sorted: record direction xyz, flux (luminance of Li)
     float guide_prob;           // one-sample MIS probability of sampling the guide
 };
 

@@ -321,21 +321,89 @@ __device__ __forceinline__ F3 guide_sample_qt(const GuideView& g, const uint32_t
     return w;
 }
 
+// Guide in the BSDF's primary sample space (the design Renderer.h:398-414 outlines:
+// nearby records' directions projected into PSS by inverting the BSDF's sampling at
+// this vertex, then PSS sampled and mapped back through the BSDF's warp). The k
+// nearest records' flux is binned over a 4x4 grid of (u, v); the guide's density in
+// PSS is h = 16 w_bin / sum w, and in solid angle p_bsdf(wi) * h(invert(wi)).
+struct PssHist {
+    float w[16];
+    float total;
+};
+__device__ __forceinline__ bool pss_build(const GuideView& g, const uint32_t* recs, const MaterialRec& m,
+                                          const SD& sd, PssHist& h) {
+#pragma unroll
+    for (int b = 0; b < 16; ++b) h.w[b] = 0.f;
+    h.total = 0.f;
+    for (int j = 0; j < kKnnMax; ++j) {
+        uint32_t r = recs[j];
+        if (r == 0xffffffffu) break;
+        float4 d = g.rec_dir[r];
+        float u, v;
+        if (!(d.w > 0.f) || !bsdf_invert(m, sd, F3{d.x, d.y, d.z}, u, v)) continue;
+        int b = min((int)(u * 4.f), 3) + 4 * min((int)(v * 4.f), 3);
+#pragma unroll
+        for (int q = 0; q < 16; ++q)
+            if (q == b) h.w[q] += d.w;
+        h.total += d.w;
+    }
+    return h.total > 0.f;
+}
+__device__ __forceinline__ float pss_density(const PssHist& h, float u, float v) {
+    int b = min((int)(u * 4.f), 3) + 4 * min((int)(v * 4.f), 3);
+    float w = 0.f;
+#pragma unroll
+    for (int q = 0; q < 16; ++q)
+        if (q == b) w = h.w[q];
+    return 16.f * w / h.total;
+}
+__device__ __forceinline__ float guide_pdf_pss(const PssHist* h, const MaterialRec& m, const SD& sd, F3 wi) {
+    float u, v;
+    if (!bsdf_invert(m, sd, wi, u, v)) return 0.f;
+    float pb = 0.f;
+    (void)bsdf_eval(m, sd, wi, pb);
+    return pb * pss_density(*h, u, v);
+}
+__device__ __forceinline__ F3 guide_sample_pss(const PssHist* h, const MaterialRec& m, const SD& sd, uint32_t key,
+                                               uint32_t& dim) {
+    float r0 = keyed_rand(key, dim++) * h->total;
+    float r1 = keyed_rand(key, dim++), r2 = keyed_rand(key, dim++);
+    int b = 15;
+    float acc = 0.f;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        float na = acc + h->w[q];
+        if (b == 15 && r0 < na && h->w[q] > 0.f) b = q;
+        acc = na;
+    }
+    while (b > 0 && !(h->w[b] > 0.f)) --b;  // r0 rounded past the last bin
+    float u = ((float)(b & 3) + r1) * 0.25f, v = ((float)(b >> 2) + r2) * 0.25f;
+    return bsdf_warp(m, sd, fminf(u, 0.99999994f), fminf(v, 0.99999994f));
+}
+
 // RayTracer::computeDirect (Renderer.h:269-354) for a non-specular surface: picks a
 // light, samples it and returns the unoccluded contribution and the shadow ray
 // (false when the contribution is zero before visibility is known). In guided mode
 // the BSDF density in the MIS weight is the one-sample mixture density.
-// Guide density of MODE 1 (quadtree mixture of the nearest cells) or MODE 2 (the
-// learned vMF mixture of this queue slot).
+// Guide density of MODE 1 (quadtree mixture of the nearest cells), MODE 2 (the
+// learned vMF mixture of this queue slot) or MODE 3 (the PSS histogram).
+struct PssRef {
+    const PssHist* h;
+    const MaterialRec* m;
+    const SD* sd;
+};
 template <int MODE>
-__device__ __forceinline__ float guide_pdf(const GuideView& g, const uint32_t* cells, const float4* mx, F3 wi) {
+__device__ __forceinline__ float guide_pdf(const GuideView& g, const uint32_t* cells, const float4* mx, F3 wi,
+                                           const PssRef& ps) {
     if (MODE == 1) return guide_pdf_qt(g, cells, wi);
+    if (MODE == 3) return guide_pdf_pss(ps.h, *ps.m, *ps.sd, wi);
     return mix_pdf(mx, wi);
 }
 template <int MODE>
 __device__ __forceinline__ F3 guide_sample(const GuideView& g, const uint32_t* cells, const float4* mx, uint32_t key,
-                                           uint32_t& dim) {
+                                           uint32_t& dim, const PssRef& ps) {
     if (MODE == 1) return guide_sample_qt(g, cells, key, dim);
+    if (MODE == 3) return guide_sample_pss(ps.h, *ps.m, *ps.sd, key, dim);
     float u0 = keyed_rand(key, dim++);
     float u1 = keyed_rand(key, dim++);
     float u2 = keyed_rand(key, dim++);
@@ -346,7 +414,7 @@ template <int MODE>
 __device__ __forceinline__ bool next_event(const DevScene& s, const MaterialRec& m, const SD& sd, uint32_t key,
                                            uint32_t& dim, const GuideView& g, const uint32_t* cells,
                                            const float4* mx, bool guided, float ag, uint32_t env_vis, C3& contrib,
-                                           F3& sh_o, F3& sh_d, float& sh_t) {
+                                           F3& sh_o, F3& sh_d, float& sh_t, const PssRef& pr) {
     contrib = c3(0.f, 0.f, 0.f);
     float pmf = 1.f / (float)s.n_lights;
     uint32_t nl = (uint32_t)s.n_lights;
@@ -371,7 +439,7 @@ __device__ __forceinline__ bool next_event(const DevScene& s, const MaterialRec&
         float G = (ctl * ctp) / den;
         float pdf_b = 0.f;
         C3 f = bsdf_eval(m, sd, wi, pdf_b);
-        if (MODE != 0 && guided) pdf_b = (1.f - ag) * pdf_b + ag * guide_pdf<MODE>(g, cells, mx, wi);
+        if (MODE != 0 && guided) pdf_b = (1.f - ag) * pdf_b + ag * guide_pdf<MODE>(g, cells, mx, wi, pr);
         float pal = pdf_l * pmf;
         float pab = (pdf_b * ctp) / den;
         float w = power_heuristic(pal, pab);
@@ -395,7 +463,7 @@ __device__ __forceinline__ bool next_event(const DevScene& s, const MaterialRec&
     if (ctl <= 0.f) return false;
     float pdf_b = 0.f;
     C3 f = bsdf_eval(m, sd, wi, pdf_b);
-    if (MODE != 0 && guided) pdf_b = (1.f - ag) * pdf_b + ag * guide_pdf<MODE>(g, cells, mx, wi);
+    if (MODE != 0 && guided) pdf_b = (1.f - ag) * pdf_b + ag * guide_pdf<MODE>(g, cells, mx, wi, pr);
     float pal = pdf_l * pmf;
     float w = power_heuristic(pal, pdf_b);
     contrib = (((le * f) * ctl) * w) / pal;
@@ -487,11 +555,17 @@ __global__ __launch_bounds__(kBlock, 3) void k_shade(DevScene s0, PathState ps, 
                         }
                     }
                 } else {
-                    const uint32_t* cells = (MODE == 1 && gc.cells) ? gc.cells + (size_t)i * kKnnMax : nullptr;
+                    const uint32_t* cells = ((MODE == 1 || MODE == 3) && gc.cells) ? gc.cells + (size_t)i * kKnnMax
+                                                                                    : nullptr;
                     const float4* mx = (MODE == 2 && gc.mix) ? gc.mix + (size_t)i * kMixF4 : nullptr;
                     float ag = gc.g.guide_prob;
                     bool guided = false;
                     if (MODE == 1) guided = !m.specular && cells != nullptr && cells[0] != 0xffffffffu;
+                    PssHist ph;
+                    const PssRef pr{&ph, &m, &sd};
+                    if (MODE == 3)
+                        guided = !m.specular && cells != nullptr && cells[0] != 0xffffffffu && pss_invertible(m) &&
+                                 pss_build(gc.g, cells, m, sd, ph);
                     if (MODE == 2 && !m.specular && mx != nullptr) {
                         ag = mx[12].x;
                         guided = ag > 0.f;
@@ -500,7 +574,7 @@ __global__ __launch_bounds__(kBlock, 3) void k_shade(DevScene s0, PathState ps, 
                     if (!m.specular && s.n_lights > 0) {
                         C3 contrib;
                         if (next_event<MODE>(s, m, sd, key, dim, gc.g, cells, mx, guided, ag, p.env_visibility, contrib,
-                                             sh_o, sh_d, sh_t)) {
+                                             sh_o, sh_d, sh_t, pr)) {
                             // the shadow ray can only change the image when the contribution is nonzero
                             C3 direct = beta * contrib;
                             if (!is_zero(direct)) {
@@ -523,12 +597,13 @@ __global__ __launch_bounds__(kBlock, 3) void k_shade(DevScene s0, PathState ps, 
                         F3 wi;
                         if (guided) {
                             if (keyed_rand(key, dim++) < ag) {
-                                wi = guide_sample<MODE>(gc.g, cells, mx, key, dim);
+                                wi = guide_sample<MODE>(gc.g, cells, mx, key, dim, pr);
                                 f = bsdf_eval(m, sd, wi, pdf);
-                                pdf = (1.f - ag) * pdf + ag * guide_pdf<MODE>(gc.g, cells, mx, wi);
+                                pdf = (1.f - ag) * pdf + ag * guide_pdf<MODE>(gc.g, cells, mx, wi, pr);
                             } else {
                                 wi = bsdf_sample(m, sd, key, dim, f, pdf);
-                                if (!(pdf <= 0.f)) pdf = (1.f - ag) * pdf + ag * guide_pdf<MODE>(gc.g, cells, mx, wi);
+                                if (!(pdf <= 0.f))
+                                    pdf = (1.f - ag) * pdf + ag * guide_pdf<MODE>(gc.g, cells, mx, wi, pr);
                             }
                         } else {
                             wi = bsdf_sample(m, sd, key, dim, f, pdf);
@@ -639,7 +714,7 @@ __global__ void k_ray_keys(PathState ps, float3 lo, float inv_ext, int parity, i
 constexpr int kGuideStack = 32;
 __global__ __launch_bounds__(kBlock) void k_guide_cells(DevScene s, PathState ps, GuideView g, uint32_t* cells,
                                                         const uint32_t* __restrict__ order, uint32_t capacity,
-                                                        int parity) {
+                                                        int parity, int records_out) {
     __shared__ int sn[kGuideStack * kBlock];
     __shared__ float sd_[kGuideStack * kBlock];
     uint32_t n = ps.counters[parity];
@@ -661,9 +736,17 @@ __global__ __launch_bounds__(kBlock) void k_guide_cells(DevScene s, PathState ps
         }
         float4 ro = ps.ray_o[parity][i], rdv = ps.ray_d[parity][i];
         F3 x = F3{ro.x, ro.y, ro.z} + F3{rdv.x, rdv.y, rdv.z} * h.y;
+        if (records_out == 2) {  // approximate neighbours (greedy descent)
+            nearby_records(g, x, out);
+            continue;
+        }
         float bd[kKnnMax];
         uint32_t bs[kKnnMax];
         knn_query<kKnnMax>(g, x, bd, bs, sn + threadIdx.x, sd_ + threadIdx.x, kBlock);
+        if (records_out) {  // the k nearest records themselves (sorted index), nearest first
+            for (int j = 0; j < kKnnMax; ++j) out[j] = bs[j];
+            continue;
+        }
         // distinct cells with neighbour counts, nearest first
         int nu = 0;
         for (int j = 0; j < kKnnMax; ++j) {
@@ -782,8 +865,7 @@ __global__ __launch_bounds__(kBlock, 3) void k_shade_record(DevScene s, PathStat
                             C3 contrib;
                             GuideView none{};
                             if (next_event<MODE>(s, m, sd, key, dim, none, nullptr, mx, guided, ag, p.env_visibility,
-                                                 contrib, sh_o, sh_d,
-                                                 sh_t) &&
+                                                 contrib, sh_o, sh_d, sh_t, PssRef{nullptr, &m, &sd}) &&
                                 !is_zero(contrib)) {
                                 want_shadow = true;
                                 sh_c = contrib;
@@ -1010,13 +1092,15 @@ hipError_t launch_extend(const DevScene& s, const PathState& ps, int parity, int
 hipError_t launch_shade(const DevScene& s, const PathState& ps, const PassParams& p, const GuideCtx& gc, int mode,
                         int parity, int depth, uint32_t grid, hipStream_t stream, bool multi) {
     if (multi) {
-        if (mode == 1) return hipErrorInvalidValue;  // batches: unguided or learned guide
+        if (mode == 1 || mode == 3) return hipErrorInvalidValue;  // batches: unguided or learned guide
         if (mode == 2)
             hipLaunchKernelGGL((k_shade<2, true>), dim3(grid), dim3(kBlock), 0, stream, s, ps, p, gc, parity, depth);
         else
             hipLaunchKernelGGL((k_shade<0, true>), dim3(grid), dim3(kBlock), 0, stream, s, ps, p, gc, parity, depth);
     } else if (mode == 1) {
         hipLaunchKernelGGL((k_shade<1, false>), dim3(grid), dim3(kBlock), 0, stream, s, ps, p, gc, parity, depth);
+    } else if (mode == 3) {
+        hipLaunchKernelGGL((k_shade<3, false>), dim3(grid), dim3(kBlock), 0, stream, s, ps, p, gc, parity, depth);
     } else if (mode == 2) {
         hipLaunchKernelGGL((k_shade<2, false>), dim3(grid), dim3(kBlock), 0, stream, s, ps, p, gc, parity, depth);
     } else {
@@ -1138,8 +1222,9 @@ hipError_t launch_sample_advance(uint32_t* sample, hipStream_t stream) {
 }
 
 hipError_t launch_guide_cells(const DevScene& s, const PathState& ps, const GuideView& g, uint32_t* cells, int parity,
-                              uint32_t grid, hipStream_t stream, const uint32_t* order, uint32_t capacity) {
-    hipLaunchKernelGGL(k_guide_cells, dim3(grid), dim3(kBlock), 0, stream, s, ps, g, cells, order, capacity, parity);
+                              uint32_t grid, hipStream_t stream, const uint32_t* order, uint32_t capacity, int records_out) {
+    hipLaunchKernelGGL(k_guide_cells, dim3(grid), dim3(kBlock), 0, stream, s, ps, g, cells, order, capacity, parity,
+                       records_out);
     return hipGetLastError();
 }
 hipError_t launch_ray_keys(const PathState& ps, float3 lo, float inv_ext, int parity, int shadow, uint32_t capacity,

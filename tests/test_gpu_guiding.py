@@ -249,3 +249,79 @@ def test_guided_render_is_unbiased_and_helps(pg, device):
     # fireflies, hence the 5% band on the mean)
     assert abs(m_gd - m_ref) / m_ref < 0.05, (m_gd, m_ref)
     assert e_gd < 0.5 * e_un, (e_gd, e_un)
+
+
+def test_pss_guided_render_is_unbiased(pg, device):
+    """The primary-sample-space guide (Renderer.h:398-414's design: nearby records'
+    directions inverted into the BSDF's (u, v), binned, sampled and warped back) on
+    the window-lit test room: it converges to the unguided image; its per-sample noise
+    is reported beside the unguided and quadtree-guided integrators'."""
+    data = pg.generate_scene(101, 0)
+    sc = pg.Scene(device, data)
+    device.clear_records()
+    rec_rt = pg.RayTracer(sc, seed=1, max_depth=12, integrator=1, record=1, record_nee=1)
+    rec_rt.clear()
+    rec_rt.render(4)
+    device.cache_build(cell_size=4096)
+    ref_rt = pg.RayTracer(sc, seed=100, max_depth=12)
+    ref_rt.clear()
+    ref_rt.render(256)
+    ref = ref_rt.image()
+    ps = pg.RayTracer(sc, seed=300, max_depth=12, integrator=4, guide_prob=0.5)
+    ps.clear()
+    ps.render(64)
+    m_ps = ps.image().mean()
+    errs = {}
+    for name, integ in (("unguided", 0), ("quadtree", 2), ("pss", 4)):
+        rt = pg.RayTracer(sc, seed=7, max_depth=12, integrator=integ, guide_prob=0.5)
+        rt.clear()
+        rt.render(16)
+        errs[name] = pg.relmse(rt.image(), ref)
+    print(f"relMSE at 16 SPP {errs}; mean unguided-256 {ref.mean():.5f} pss-64 {m_ps:.5f}")
+    assert np.isfinite(m_ps)
+    assert abs(m_ps - ref.mean()) / ref.mean() < 0.05, (m_ps, ref.mean())
+    assert errs["pss"] < 2.0 * errs["unguided"], errs
+
+
+def test_pss_guided_furnace(pg, device):
+    """Diffuse (albedo 0.5) and rough-conductor spheres under a constant environment
+    Le = 1, guided in primary sample space from their own records: the diffuse sphere
+    converges to the closed form 0.5, the conductor to its unguided image."""
+    from pathguidingdissertation_amd.scene import MATERIAL_DTYPE, SceneData
+    from test_gpu_render import _icosphere
+    pos, nrm = _icosphere(pg, 1.0, (0.0, 0.0, 0.0))
+    far = np.array([[[1e4, 1e4, 1e4], [1e4 + 1e-2, 1e4, 1e4], [1e4, 1e4 + 1e-2, 1e4]],
+                    [[-1e4, -1e4, -1e4], [-1e4 + 1e-2, -1e4, -1e4], [-1e4, -1e4 + 1e-2, -1e4]]], np.float32)
+    pos = np.concatenate([pos, far])
+    nrm = np.concatenate([nrm, np.tile(np.array([0, 0, 1], np.float32), (2, 3, 1))])
+    env = np.ones((64, 128, 3), np.float32)
+    cam = {"from": [0.0, 0.0, 4.0], "to": [0.0, 0.0, 0.0], "up": [0.0, 1.0, 0.0], "fov": 40.0,
+           "width": 96, "height": 96}
+    means = {}
+    for kind in ("diffuse", "conductor"):
+        mats = np.zeros(1, MATERIAL_DTYPE)
+        mats["albedo"] = 0.5
+        if kind == "conductor":
+            mats["type"] = pg.BSDF_CONDUCTOR
+            mats["roughness"] = 0.3
+            mats["eta"] = (0.2, 0.9, 1.1)
+            mats["k"] = (3.9, 2.4, 2.1)
+        data = SceneData(pos, nrm, np.zeros(pos.shape[0], np.uint32), mats, env, cam)
+        sc = pg.Scene(device, data)
+        device.clear_records()
+        rec = pg.RayTracer(sc, seed=1, max_depth=8, integrator=1, record=1, record_nee=1)
+        rec.clear()
+        rec.render(2)
+        device.cache_build(cell_size=1024)
+        rays = sc.generate_rays(np.stack(np.meshgrid(np.arange(96) + 0.5, np.arange(96) + 0.5), -1).reshape(-1, 2))
+        ids, *_ = sc.traverse(rays)
+        hit = (ids != 0xFFFFFFFF).reshape(96, 96)
+        inner = hit & np.roll(hit, 1, 0) & np.roll(hit, -1, 0) & np.roll(hit, 1, 1) & np.roll(hit, -1, 1)
+        for integ in (0, 4):
+            rt = pg.RayTracer(sc, seed=11, max_depth=8, integrator=integ, guide_prob=0.5)
+            rt.clear()
+            rt.render(64)
+            means[(kind, integ)] = float(rt.image()[inner].mean())
+    print(means)
+    assert abs(means[("diffuse", 4)] - 0.5) < 0.01, means
+    assert abs(means[("conductor", 4)] - means[("conductor", 0)]) < 0.01 * means[("conductor", 0)] + 2e-3, means
