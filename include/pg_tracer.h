@@ -87,8 +87,9 @@ typedef struct {
     uint32_t record;        /* RECORD integrator: keep PathVertex records       */
     float guide_prob;       /* guided integrators: P(sample guide), 0.5         */
     uint32_t knn_k;         /* guided QT / PSS: neighbours per lookup (16); PSS
-                               with 0 takes the 16 nearest of the 64 records under
-                               a greedy point-BVH descent instead of an exact search */
+                               with 0 takes the 16 records nearest the centroid of
+                               the point-BVH leaf that holds the query's Morton key
+                               (lists built with the cache) instead of a search   */
     uint32_t record_nee;    /* RECORD: 0 records Li = G (Renderer.h:390, the
                                reference); 1 records Li with NEE and MIS-weighted
                                emission (the pixelColour recursion, :391)       */

@@ -58,6 +58,8 @@ struct pg_ctx {
     float* d_env_prob = nullptr;
     uint32_t* d_env_alias = nullptr;
     pg_internal::TextureSet tex;
+    uint32_t* leaf_nbrs = nullptr;
+    uint32_t leaf_nbrs_cap = 0;
     DevScene dev{};
     // path state + film
     PathState ps{};
@@ -177,6 +179,8 @@ struct pg_ctx {
         dfree(cb.keys); dfree(cb.vals); dfree(cb.keys_tmp); dfree(cb.vals_tmp); dfree(cb.hist); dfree(cb.scan_tmp);
         dfree(cb.pos); dfree(cb.dir); dfree(cb.nodes); dfree(cb.qt); dfree(cb.qt_acc);
         cache_cap = cache_pow2_cap = cache_cells_cap = cache_stride = 0;
+        dfree(leaf_nbrs);
+        leaf_nbrs_cap = 0;
         cache_ready = false;
         gv = GuideView{};
     }
@@ -992,7 +996,22 @@ pg_status pg_cache_build(pg_ctx* c, uint32_t cell_size, uint32_t qt_nodes_per_ce
     g.nodes = c->cb.nodes;
     g.qt = c->cb.qt;
     g.rec_dir = c->cb.dir;
+    g.rec_key = c->cb.keys;
+    g.m_lo[0] = lo.x; g.m_lo[1] = lo.y; g.m_lo[2] = lo.z;
+    g.m_inv = inv;
     g.guide_prob = 0.5f;
+    // neighbour lists of the leaves (approximate neighbours of the PSS guide)
+    const uint32_t n_leaves = (n + kLeafRecords - 1) / kLeafRecords;
+    if (n_leaves > c->leaf_nbrs_cap) {
+        dfree(c->leaf_nbrs);
+        PG_TRY(c, dalloc(&c->leaf_nbrs, (size_t)n_leaves * kKnnMax), "alloc leaf neighbours");
+        c->leaf_nbrs_cap = n_leaves;
+    }
+    g.leaf_nbrs = c->leaf_nbrs;
+    if (n) {
+        PG_TRY(c, launch_leaf_nbrs(g, n_leaves, c->leaf_nbrs, c->stream), "leaf neighbours");
+        PG_TRY(c, hipStreamSynchronize(c->stream), "leaf neighbours");
+    }
     c->cache_ready = true;
     return PG_OK;
 }

@@ -45,6 +45,9 @@ struct GuideView {
     const float4* nodes;        // 2 * (2P - 1) float4: bmin, bmax
     const QtNode* qt;           // n_cells * qt_stride
     const float4* rec_dir;      // sorted: record direction xyz, flux (luminance of Li)
+    const uint32_t* rec_key;    // sorted 30-bit Morton keys of the record positions
+    const uint32_t* leaf_nbrs;  // per leaf: the kKnnMax records nearest its centroid
+    float m_lo[3], m_inv;       // Morton frame of rec_key (scene bounding cube)
     float guide_prob;           // one-sample MIS probability of sampling the guide
 };
 

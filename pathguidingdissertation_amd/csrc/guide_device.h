@@ -100,56 +100,36 @@ __device__ void knn_query(const GuideView& g, F3 q, float (&bd)[K], uint32_t (&b
     }
 }
 
-// Approximate neighbours for the primary-sample-space guide: a greedy descent of the
-// point BVH (at each node the child whose box is nearer to q) down to the node that
-// holds kPssPool records (kPssPool / kLeafRecords leaves), then the kKnnMax of those
-// nearest to q by the (distance, index) order of knn_query. No stack, no pruning:
-// log2(P) pairs of box distances and kPssPool point distances per query.
-constexpr int kPssPool = 64;
+// Approximate neighbours for the primary-sample-space guide: the leaf that holds the
+// query's 30-bit Morton key in the sorted record order (a binary search over rec_key,
+// the key computed in the frame the records were sorted in) and that leaf's list of the
+// kKnnMax records nearest its centroid, computed once per cache build (k_leaf_nbrs).
+// No stack and no candidate heap per query.
+__device__ __forceinline__ uint32_t morton_expand10(uint32_t x) {
+    x &= 1023u;
+    x = (x | (x << 16)) & 0x030000FFu;
+    x = (x | (x << 8)) & 0x0300F00Fu;
+    x = (x | (x << 4)) & 0x030C30C3u;
+    x = (x | (x << 2)) & 0x09249249u;
+    return x;
+}
 __device__ __forceinline__ void nearby_records(const GuideView& g, F3 q, uint32_t* out) {
     for (int j = 0; j < kKnnMax; ++j) out[j] = 0xffffffffu;
     if (g.n_records == 0) return;
-    const uint32_t P = g.n_leaf_pow2;
-    constexpr uint32_t kPoolLevels = 3;  // log2(kPssPool / kLeafRecords)
-    uint32_t first = 0, count = g.n_records;
-    if (g.levels > kPoolLevels) {
-        int node = 0;
-        for (uint32_t level = 0; level + kPoolLevels < g.levels; ++level) {
-            int c1 = 2 * node + 1, c2 = c1 + 1;
-            float d1 = box_dist2(g.nodes[2 * c1], g.nodes[2 * c1 + 1], q);
-            float d2 = box_dist2(g.nodes[2 * c2], g.nodes[2 * c2 + 1], q);
-            node = (d2 < d1) ? c2 : c1;
-        }
-        // leftmost leaf under the node: kPoolLevels more left-child steps
-        uint32_t leaf_node = (uint32_t)node;
-        for (uint32_t k = 0; k < kPoolLevels; ++k) leaf_node = 2 * leaf_node + 1;
-        first = (leaf_node - (P - 1)) * kLeafRecords;
-        count = first < g.n_records ? min((uint32_t)kPssPool, g.n_records - first) : 0u;
+    int qx = min(max((int)((q.x - g.m_lo[0]) * g.m_inv * 1024.f), 0), 1023);
+    int qy = min(max((int)((q.y - g.m_lo[1]) * g.m_inv * 1024.f), 0), 1023);
+    int qz = min(max((int)((q.z - g.m_lo[2]) * g.m_inv * 1024.f), 0), 1023);
+    uint32_t key = (morton_expand10((uint32_t)qx) << 2) | (morton_expand10((uint32_t)qy) << 1) |
+                   morton_expand10((uint32_t)qz);
+    uint32_t lo = 0, hi = g.n_records;  // first sorted record with key >= query key
+    while (lo < hi) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (g.rec_key[mid] < key) lo = mid + 1;
+        else hi = mid;
     }
-    float bd[kKnnMax];
-    uint32_t bs[kKnnMax];
-#pragma unroll
-    for (int j = 0; j < kKnnMax; ++j) {
-        bd[j] = __int_as_float(0x7f800000);
-        bs[j] = 0xffffffffu;
-    }
-    for (uint32_t s = first; s < first + count; ++s) {
-        float4 p = g.rec_pos[s];
-        float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
-        float d = ((dx * dx) + (dy * dy)) + (dz * dz);
-        if (knn_better(g, d, s, bd[kKnnMax - 1], bs[kKnnMax - 1])) {
-            bd[kKnnMax - 1] = d;
-            bs[kKnnMax - 1] = s;
-#pragma unroll
-            for (int j = kKnnMax - 1; j > 0; --j) {
-                if (knn_better(g, bd[j], bs[j], bd[j - 1], bs[j - 1])) {
-                    float td = bd[j]; bd[j] = bd[j - 1]; bd[j - 1] = td;
-                    uint32_t ts = bs[j]; bs[j] = bs[j - 1]; bs[j - 1] = ts;
-                }
-            }
-        }
-    }
-    for (int j = 0; j < kKnnMax; ++j) out[j] = bs[j];
+    uint32_t leaf = min(lo, g.n_records - 1u) / (uint32_t)kLeafRecords;
+    const uint32_t* nb = g.leaf_nbrs + (size_t)leaf * kKnnMax;
+    for (int j = 0; j < kKnnMax; ++j) out[j] = nb[j];
 }
 
 // Quadtree pdf over [0,1)^2 at (u, v).

@@ -114,8 +114,9 @@ hipError_t launch_copy(const float4* src, float4* dst, size_t n, int cu_count, h
 struct RecordState;
 // kNN cells per shading point; with `order` (a sorted permutation of the queue
 // capacity from launch_query_keys + radix sort) waves visit neighbouring points.
+hipError_t launch_leaf_nbrs(const GuideView& g, uint32_t n_leaves, uint32_t* out, hipStream_t stream);
 // records_out: 0 distinct cells of the k nearest records with counts (quadtree guide),
-// 1 the k nearest records, 2 the records of the greedy-descent node (PSS guide).
+// 1 the k nearest records, 2 the precomputed neighbours of the query's Morton leaf (PSS guide).
 hipError_t launch_guide_cells(const DevScene& s, const PathState& ps, const GuideView& g, uint32_t* cells, int parity,
                               uint32_t grid, hipStream_t stream, const uint32_t* order = nullptr,
                               uint32_t capacity = 0, int records_out = 0);

@@ -762,6 +762,26 @@ __global__ __launch_bounds__(kBlock) void k_guide_cells(DevScene s, PathState ps
     }
 }
 
+// Per point-BVH leaf, the kKnnMax records nearest the centroid of its records (exact
+// knn_query), for the approximate neighbours of nearby_records.
+__global__ __launch_bounds__(kBlock) void k_leaf_nbrs(GuideView g, uint32_t n_leaves, uint32_t* out) {
+    __shared__ int sn[kGuideStack * kBlock];
+    __shared__ float sd_[kGuideStack * kBlock];
+    uint32_t l = blockIdx.x * kBlock + threadIdx.x;
+    if (l >= n_leaves) return;
+    uint32_t b = l * kLeafRecords, e = min(b + (uint32_t)kLeafRecords, g.n_records);
+    float cx = 0.f, cy = 0.f, cz = 0.f;
+    for (uint32_t s = b; s < e; ++s) {
+        float4 p = g.rec_pos[s];
+        cx += p.x; cy += p.y; cz += p.z;
+    }
+    float inv = 1.f / (float)(e - b);
+    float bd[kKnnMax];
+    uint32_t bs[kKnnMax];
+    knn_query<kKnnMax>(g, F3{cx * inv, cy * inv, cz * inv}, bd, bs, sn + threadIdx.x, sd_ + threadIdx.x, kBlock);
+    for (int j = 0; j < kKnnMax; ++j) out[(size_t)l * kKnnMax + j] = bs[j];
+}
+
 // One bounce of the record pass, RayTracer::guidedPathRecursive (Renderer.h:397-528):
 // BSDF-sampled continuation with a fixed Russian-roulette probability 0.7 after
 // depth rr_depth, one forward vertex {x, n, wi, f cos/(pdf rrp), emission,
@@ -1225,6 +1245,11 @@ hipError_t launch_guide_cells(const DevScene& s, const PathState& ps, const Guid
                               uint32_t grid, hipStream_t stream, const uint32_t* order, uint32_t capacity, int records_out) {
     hipLaunchKernelGGL(k_guide_cells, dim3(grid), dim3(kBlock), 0, stream, s, ps, g, cells, order, capacity, parity,
                        records_out);
+    return hipGetLastError();
+}
+hipError_t launch_leaf_nbrs(const GuideView& g, uint32_t n_leaves, uint32_t* out, hipStream_t stream) {
+    if (n_leaves == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_leaf_nbrs, dim3((n_leaves + kBlock - 1) / kBlock), dim3(kBlock), 0, stream, g, n_leaves, out);
     return hipGetLastError();
 }
 hipError_t launch_ray_keys(const PathState& ps, float3 lo, float inv_ext, int parity, int shadow, uint32_t capacity,
