@@ -164,6 +164,7 @@ def time_to_relmse(pg, dev, sc, data, max_depth, ref_spp, cap_s, log, target=0.0
         def ml_setup():
             dev.mlp_init(seed=7 + r)
             ml.clear()
+            ml.reset_stats()
 
         methods = {
             "unguided": (lambda: un.clear(), adv(un, lambda k: un.render(k))),
@@ -180,6 +181,11 @@ def time_to_relmse(pg, dev, sc, data, max_depth, ref_spp, cap_s, log, target=0.0
             res[name] = {"seconds": None if hit is None else round(hit[1], 3),
                          "spp": curve[-1][0], "final_relmse": round(curve[-1][2], 5),
                          "final_trimmed_relmse": round(curve[-1][3], 5), "curve": curve}
+            if name == "learned":
+                st = ml.stats()
+                res[name]["kernel_ms_per_spp"] = {
+                    k: round(st[f"kernel_ms_{k}"] / max(curve[-1][0], 1), 3)
+                    for k in ("extend", "shade", "shadow", "guide", "total")}
 
         def speedup(fast, slow, col=2):
             # time `slow` needs to stay below its own final relMSE over the time `fast`
@@ -220,6 +226,18 @@ def time_to_relmse(pg, dev, sc, data, max_depth, ref_spp, cap_s, log, target=0.0
         step = max(1, len(c) // 16)
         rr["curve"] = [[s, t, round(e, 5), round(et, 5)]
                        for s, t, e, et in (c[::step] + ([c[-1]] if (len(c) - 1) % step else []))]
+    # network FLOPs over the used layer sizes (80 -> 64 -> 64 -> 64 -> 41): 2 * sum(in * out)
+    # per query forward; a training sample adds the input and weight gradients (3x)
+    fwd = 2 * (80 * 64 + 64 * 64 + 64 * 64 + 64 * 41)
+    batch = 1 << 18
+    train_ms = float(info["train_ms"])
+    mlp = {"forward_flop_per_query": fwd, "train_flop_per_sample": 3 * fwd, "batch": batch,
+           "train_ms_per_step": round(train_ms, 3),
+           "train_tflops": round(3 * fwd * batch / (train_ms * 1e-3) / 1e12, 2) if train_ms > 0 else None,
+           "peak_tflops_bf16_dense": 2500.0,
+           "mfma_frac": round(3 * fwd * batch / (train_ms * 1e-3) / 2.5e15, 5) if train_ms > 0 else None,
+           "inference_ms_per_spp": res["learned"]["kernel_ms_per_spp"]["guide"],
+           "bound": "hash-grid gradient atomics (training) and gathers (inference), not MFMA"}
     out = {"target": target, "cap_s": cap_s,
            "reference": f"GPU unguided {ref_spp} SPP, FP64 host accumulation, seed 1000",
            "env_visibility": ("environment shadow rays along the light direction for every method and the "
@@ -238,6 +256,7 @@ def time_to_relmse(pg, dev, sc, data, max_depth, ref_spp, cap_s, log, target=0.0
                        if k in res},
            **res,
            "learned_diverged": bool(info["diverged"]),
+           "learned_network": mlp,
            "comparison": cmp}
     if repeats > 1:
         def med(path):

@@ -116,6 +116,8 @@ typedef struct {
     uint64_t extend_launches, shadow_launches;
     uint64_t nodes_visited, tris_tested;       /* filled when counting is enabled */
     uint64_t shadow_nodes_visited, shadow_tris_tested;
+    double kernel_ms_guide;  /* guide queries per bounce: kNN / query order (quadtree,
+                                PSS guides) or network inference (learned guide)     */
 } pg_stats;
 
 /* ------------------------------------------------------------------ context */

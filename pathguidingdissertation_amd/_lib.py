@@ -52,7 +52,8 @@ class PgStats(C.Structure):
                 ("kernel_ms_shadow", C.c_double), ("kernel_ms_total", C.c_double),
                 ("extend_launches", C.c_uint64), ("shadow_launches", C.c_uint64),
                 ("nodes_visited", C.c_uint64), ("tris_tested", C.c_uint64),
-                ("shadow_nodes_visited", C.c_uint64), ("shadow_tris_tested", C.c_uint64)]
+                ("shadow_nodes_visited", C.c_uint64), ("shadow_tris_tested", C.c_uint64),
+                ("kernel_ms_guide", C.c_double)]
 
 
 class PgMlpConfig(C.Structure):

@@ -1160,6 +1160,7 @@ pg_status pg_stats_read(pg_ctx* c, pg_stats* s) {
     s->kernel_ms_extend = c->ms[0];
     s->kernel_ms_shade = c->ms[1];
     s->kernel_ms_shadow = c->ms[2];
+    s->kernel_ms_guide = c->ms[4];
     s->records = c->n_records;
     s->kernel_ms_total = c->ms_total;
     s->extend_launches = c->n_extend;
