@@ -725,7 +725,7 @@ pg_status pg_render_spp(pg_ctx* c, const pg_render_params* params, uint32_t n_sp
     const uint32_t grid_trace = prm.trace_blocks_per_cu ? (uint32_t)c->cu_count * prm.trace_blocks_per_cu
                                                         : trace_grid(c->cu_count, stack);
     const uint32_t grid_shade = (uint32_t)c->cu_count * (prm.shade_blocks_per_cu ? prm.shade_blocks_per_cu : 8u);
-    const uint32_t grid_guide = (uint32_t)c->cu_count * 5;
+    const uint32_t grid_guide = (uint32_t)c->cu_count * (c->gv.levels <= 16 ? 10u : 5u);
     const uint32_t grid_mlp = (uint32_t)c->cu_count * 2;
     GuideCtx gc{c->gv, (mode == 1 || mode == 3) ? c->guide_cells : nullptr, mode == 2 ? c->mix : nullptr};
     gc.g.guide_prob = prm.guide_prob;

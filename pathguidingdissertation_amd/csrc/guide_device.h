@@ -33,9 +33,13 @@ __device__ __forceinline__ bool knn_better(const GuideView& g, float da, uint32_
 
 // K nearest records to q. bd/bs: squared distances and sorted record positions,
 // ascending. stk: LDS stack (node, box distance), stride `stride`.
+// cap: an upper bound on the K-th nearest squared distance (e.g. the largest distance
+// among K known records, knn_cap), used for pruning until the candidate list is
+// tighter; subtrees are pruned only when strictly farther, so the result is the same
+// exact set (ties included) as without it.
 template <int K>
 __device__ void knn_query(const GuideView& g, F3 q, float (&bd)[K], uint32_t (&bs)[K], int* stk_node,
-                          float* stk_dist, int stride) {
+                          float* stk_dist, int stride, float cap = __int_as_float(0x7f800000)) {
 #pragma unroll
     for (int j = 0; j < K; ++j) {
         bd[j] = __int_as_float(0x7f800000);  // +inf
@@ -70,7 +74,7 @@ __device__ void knn_query(const GuideView& g, F3 q, float (&bd)[K], uint32_t (&b
             int c1 = 2 * node + 1, c2 = c1 + 1;
             float d1 = box_dist2(g.nodes[2 * c1], g.nodes[2 * c1 + 1], q);
             float d2 = box_dist2(g.nodes[2 * c2], g.nodes[2 * c2 + 1], q);
-            float worst = bd[K - 1];
+            float worst = fminf(bd[K - 1], cap);
             bool v1 = !(d1 > worst), v2 = !(d2 > worst);
             if (v1 && v2) {
                 int nn = c1, ff = c2;
@@ -90,7 +94,7 @@ __device__ void knn_query(const GuideView& g, F3 q, float (&bd)[K], uint32_t (&b
         while (sp > 0) {
             --sp;
             float d = stk_dist[sp * stride];
-            if (!(d > bd[K - 1])) {
+            if (!(d > fminf(bd[K - 1], cap))) {
                 node = stk_node[sp * stride];
                 found = true;
                 break;
@@ -113,9 +117,7 @@ __device__ __forceinline__ uint32_t morton_expand10(uint32_t x) {
     x = (x | (x << 2)) & 0x09249249u;
     return x;
 }
-__device__ __forceinline__ void nearby_records(const GuideView& g, F3 q, uint32_t* out) {
-    for (int j = 0; j < kKnnMax; ++j) out[j] = 0xffffffffu;
-    if (g.n_records == 0) return;
+__device__ __forceinline__ uint32_t morton_leaf(const GuideView& g, F3 q) {
     int qx = min(max((int)((q.x - g.m_lo[0]) * g.m_inv * 1024.f), 0), 1023);
     int qy = min(max((int)((q.y - g.m_lo[1]) * g.m_inv * 1024.f), 0), 1023);
     int qz = min(max((int)((q.z - g.m_lo[2]) * g.m_inv * 1024.f), 0), 1023);
@@ -127,9 +129,32 @@ __device__ __forceinline__ void nearby_records(const GuideView& g, F3 q, uint32_
         if (g.rec_key[mid] < key) lo = mid + 1;
         else hi = mid;
     }
-    uint32_t leaf = min(lo, g.n_records - 1u) / (uint32_t)kLeafRecords;
-    const uint32_t* nb = g.leaf_nbrs + (size_t)leaf * kKnnMax;
+    return min(lo, g.n_records - 1u) / (uint32_t)kLeafRecords;
+}
+__device__ __forceinline__ void nearby_records(const GuideView& g, F3 q, uint32_t* out) {
+    for (int j = 0; j < kKnnMax; ++j) out[j] = 0xffffffffu;
+    if (g.n_records == 0) return;
+    const uint32_t* nb = g.leaf_nbrs + (size_t)morton_leaf(g, q) * kKnnMax;
     for (int j = 0; j < kKnnMax; ++j) out[j] = nb[j];
+}
+
+// Pruning bound for an exact k-nearest search (K = kKnnMax): the largest squared
+// distance from q to the records of its leaf list (nearby_records). Those are K
+// distinct records, so the K-th nearest distance is at most this value.
+__device__ __forceinline__ float knn_cap(const GuideView& g, F3 q) {
+    float cap = __int_as_float(0x7f800000);
+    if (g.n_records < (uint32_t)kKnnMax) return cap;
+    const uint32_t* nb = g.leaf_nbrs + (size_t)morton_leaf(g, q) * kKnnMax;
+    float m = 0.f;
+#pragma unroll 4
+    for (int c = 0; c < kKnnMax; ++c) {
+        uint32_t s = nb[c];
+        if (s == 0xffffffffu) return cap;
+        float4 p = g.rec_pos[s];
+        float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
+        m = fmaxf(m, ((dx * dx) + (dy * dy)) + (dz * dz));
+    }
+    return m;
 }
 
 // Quadtree pdf over [0,1)^2 at (u, v).

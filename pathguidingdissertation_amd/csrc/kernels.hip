@@ -712,11 +712,13 @@ __global__ void k_ray_keys(PathState ps, float3 lo, float inv_ext, int parity, i
 // Cells of the k nearest records for every shading point of this bounce, visited in
 // `order` (a permutation of the queue capacity; slots past the queue are skipped).
 constexpr int kGuideStack = 32;
+template <int OUT, bool SEEDED, int STACK>
 __global__ __launch_bounds__(kBlock) void k_guide_cells(DevScene s, PathState ps, GuideView g, uint32_t* cells,
                                                         const uint32_t* __restrict__ order, uint32_t capacity,
-                                                        int parity, int records_out) {
-    __shared__ int sn[kGuideStack * kBlock];
-    __shared__ float sd_[kGuideStack * kBlock];
+                                                        int parity) {
+    constexpr int records_out = OUT;
+    __shared__ int sn[(OUT == 2 ? 1 : STACK) * kBlock];
+    __shared__ float sd_[(OUT == 2 ? 1 : STACK) * kBlock];
     uint32_t n = ps.counters[parity];
     const uint32_t m_end = order ? capacity : n;
     for (uint32_t j = blockIdx.x * kBlock + threadIdx.x; j < m_end; j += gridDim.x * kBlock) {
@@ -736,13 +738,15 @@ __global__ __launch_bounds__(kBlock) void k_guide_cells(DevScene s, PathState ps
         }
         float4 ro = ps.ray_o[parity][i], rdv = ps.ray_d[parity][i];
         F3 x = F3{ro.x, ro.y, ro.z} + F3{rdv.x, rdv.y, rdv.z} * h.y;
-        if (records_out == 2) {  // approximate neighbours (greedy descent)
+        if (records_out == 2) {  // approximate neighbours (the query's leaf list)
             nearby_records(g, x, out);
             continue;
         }
         float bd[kKnnMax];
         uint32_t bs[kKnnMax];
-        knn_query<kKnnMax>(g, x, bd, bs, sn + threadIdx.x, sd_ + threadIdx.x, kBlock);
+        // exact search; SEEDED prunes from the start with the bound of the query's leaf list
+        knn_query<kKnnMax>(g, x, bd, bs, sn + threadIdx.x, sd_ + threadIdx.x, kBlock,
+                           SEEDED ? knn_cap(g, x) : __int_as_float(0x7f800000));
         if (records_out) {  // the k nearest records themselves (sorted index), nearest first
             for (int j = 0; j < kKnnMax; ++j) out[j] = bs[j];
             continue;
@@ -1243,8 +1247,22 @@ hipError_t launch_sample_advance(uint32_t* sample, hipStream_t stream) {
 
 hipError_t launch_guide_cells(const DevScene& s, const PathState& ps, const GuideView& g, uint32_t* cells, int parity,
                               uint32_t grid, hipStream_t stream, const uint32_t* order, uint32_t capacity, int records_out) {
-    hipLaunchKernelGGL(k_guide_cells, dim3(grid), dim3(kBlock), 0, stream, s, ps, g, cells, order, capacity, parity,
-                       records_out);
+    // the search stack holds at most one entry per tree level: a 16-entry stack (half
+    // the LDS) serves trees of up to 2^16 leaves
+    const bool small = g.levels <= 16;
+    const bool seeded = g.leaf_nbrs != nullptr;
+#define PG_GC(OUT, SEED, ST) \
+    hipLaunchKernelGGL((k_guide_cells<OUT, SEED, ST>), dim3(grid), dim3(kBlock), 0, stream, s, ps, g, cells, order, \
+                       capacity, parity)
+    if (records_out == 2) PG_GC(2, false, 16);
+    else if (records_out == 1) {
+        if (seeded) { if (small) PG_GC(1, true, 16); else PG_GC(1, true, 32); }
+        else { if (small) PG_GC(1, false, 16); else PG_GC(1, false, 32); }
+    } else {
+        if (seeded) { if (small) PG_GC(0, true, 16); else PG_GC(0, true, 32); }
+        else { if (small) PG_GC(0, false, 16); else PG_GC(0, false, 32); }
+    }
+#undef PG_GC
     return hipGetLastError();
 }
 hipError_t launch_leaf_nbrs(const GuideView& g, uint32_t n_leaves, uint32_t* out, hipStream_t stream) {
