@@ -43,7 +43,8 @@ class PgRenderParams(C.Structure):
                 ("integrator", C.c_int32), ("record", C.c_uint32), ("guide_prob", C.c_float),
                 ("knn_k", C.c_uint32), ("record_nee", C.c_uint32), ("trace_blocks_per_cu", C.c_uint32),
                 ("shade_blocks_per_cu", C.c_uint32), ("sort_rays", C.c_uint32),
-                ("dyn_fetch", C.c_uint32), ("env_visibility", C.c_uint32), ("reserved", C.c_uint32 * 3)]
+                ("dyn_fetch", C.c_uint32), ("env_visibility", C.c_uint32), ("samples_per_pass", C.c_uint32),
+                ("reserved", C.c_uint32 * 2)]
 
 
 class PgStats(C.Structure):

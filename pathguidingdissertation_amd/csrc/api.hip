@@ -682,10 +682,21 @@ pg_status pg_render_spp(pg_ctx* c, const pg_render_params* params, uint32_t n_sp
         return c->fail(PG_ERR_INVALID, "primary-sample-space guide: knn_k must be 16 (exact) or 0 (approximate)");
     uint32_t W = (uint32_t)c->host.camera.width, H = (uint32_t)c->host.camera.height;
     uint32_t n = W * H;
-    pg_status st = ensure_paths(c, n);
-    if (st != PG_OK) return st;
     // record passes: the reference's record integrator, optionally guided by the learned guide
     const bool record_pass = integ == PG_INTEGRATOR_RECORD || (integ == PG_INTEGRATOR_GUIDED_MLP && prm.record);
+    // samples traced together per pass (plain path tracing only): deep bounces hold few
+    // rays whose launches are latency-bound, so several samples share them
+    uint32_t spp_pass = 1;
+    if (integ == PG_INTEGRATOR_PATH && mode == 0 && !prm.sort_rays) {
+        spp_pass = prm.samples_per_pass;
+        if (spp_pass == 0) {
+            spp_pass = 4;
+            while (spp_pass > 1 && (uint64_t)spp_pass * n > (1ull << 22)) --spp_pass;
+        }
+        if ((uint64_t)spp_pass * n > 0x7fffffffull) return c->fail(PG_ERR_INVALID, "samples_per_pass x pixels >= 2^31");
+    }
+    pg_status st = ensure_paths(c, n * spp_pass);
+    if (st != PG_OK) return st;
     if (record_pass && (st = ensure_record_state(c, n, prm.max_depth + 1)) != PG_OK) return st;
     c->rs.li_nee = prm.record_nee ? 1 : 0;
     if ((mode == 1 || mode == 3) && c->guide_cells_cap < n) {
@@ -747,8 +758,10 @@ pg_status pg_render_spp(pg_ctx* c, const pg_render_params* params, uint32_t n_sp
     };
     hipEvent_t t0 = c->ev(), t1 = c->ev();
     (void)hipEventRecord(t0, c->stream);
-    for (uint32_t s = 0; s < n_spp; ++s) {
-        PassParams pp{prm.seed, c->spp, W, H, prm.max_depth, prm.rr_depth, prm.integrator, prm.env_visibility};
+    for (uint32_t s = 0; s < n_spp;) {
+        const uint32_t ns = std::min(spp_pass, n_spp - s);
+        s += ns;
+        PassParams pp{prm.seed, c->spp, W, H, prm.max_depth, prm.rr_depth, prm.integrator, prm.env_visibility, ns};
         PG_TRY(c, timed(3, [&] { return launch_raygen(c->dev, c->ps, pp, c->stream); }), "raygen");
         for (uint32_t depth = 0; depth <= prm.max_depth; ++depth) {
             int parity = (int)(depth & 1u);
@@ -822,8 +835,8 @@ pg_status pg_render_spp(pg_ctx* c, const pg_render_params* params, uint32_t n_sp
             c->cache_ready = false;
         }
         PG_TRY(c, timed(3, [&] { return launch_resolve(c->ps, pp, c->film, c->stream); }), "resolve");
-        c->spp++;
-        c->paths += n;
+        c->spp += ns;
+        c->paths += (uint64_t)n * ns;
     }
     (void)hipEventRecord(t1, c->stream);
     PG_TRY(c, hipStreamSynchronize(c->stream), "render");

@@ -76,6 +76,9 @@ struct PassParams {
     uint32_t max_depth, rr_depth;
     int32_t integrator;
     uint32_t env_visibility;  // 0: Renderer.h:331-332 shadow target; 1: along wi
+    uint32_t n_samples;       // samples per pixel traced together in this pass (0 = 1):
+                              // path i is sample (sample + i / (width*height)) of pixel
+                              // i % (width*height)
 };
 
 // Launch helpers (defined in kernels.hip); all enqueue on `stream`.

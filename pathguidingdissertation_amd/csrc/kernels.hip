@@ -87,17 +87,19 @@ __device__ __forceinline__ const DevScene& scene_of(const DevScene& s0, const Pa
 }
 
 __global__ void k_raygen(DevScene s, PathState ps, PassParams p) {
-    uint32_t n = p.width * p.height;
+    const uint32_t n = p.width * p.height;
+    const uint32_t ns = p.n_samples ? p.n_samples : 1u;
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i == 0) {
-        ps.counters[0] = n;
+        ps.counters[0] = n * ns;
         ps.counters[1] = 0;
         ps.counters[2] = 0;
         ps.counters[3] = 0;  // ray claims of the persistent-lane extend
     }
-    if (i >= n) return;
-    uint32_t x = i % p.width, y = i / p.width;
-    uint32_t key = path_key(p.seed, p.sample, i);
+    if (i >= n * ns) return;
+    const uint32_t pix = i % n, smp = i / n;
+    uint32_t x = pix % p.width, y = pix / p.width;
+    uint32_t key = path_key(p.seed, p.sample + smp, pix);
     float px = (float)x + keyed_rand(key, 0);  // Renderer.h:679-680
     float py = (float)y + keyed_rand(key, 1);
     F3 o, d;
@@ -1006,16 +1008,19 @@ __global__ __launch_bounds__(kBlock) void k_shadow_record(DevScene s, PathState 
 // pixel; non-finite samples are dropped. Each film pixel gathers its (at most
 // four) source paths in source-pixel order, so the sum is deterministic.
 __global__ void k_resolve(PathState ps, PassParams p, float4* film) {
-    uint32_t n = p.width * p.height;
+    const uint32_t n = p.width * p.height;
+    const uint32_t ns = p.n_samples ? p.n_samples : 1u;
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     int X = (int)(i % p.width), Y = (int)(i / p.width);
     float4 acc = film[i];
+    // samples in order, each as one pass of its own would add it
+    for (uint32_t smp = 0; smp < ns; ++smp)
     for (int dy = -1; dy <= 0; ++dy)
         for (int dx = -1; dx <= 0; ++dx) {
             int sx = X + dx, sy = Y + dy;
             if (sx < 0 || sy < 0) continue;
-            uint32_t q = (uint32_t)sy * p.width + (uint32_t)sx;
+            uint32_t q = smp * n + (uint32_t)sy * p.width + (uint32_t)sx;
             float2 fp = ps.film_pos[q];
             int tx = (int)floorf(fp.x + 0.5f), ty = (int)floorf(fp.y + 0.5f);
             if (tx != X || ty != Y) continue;
@@ -1090,7 +1095,7 @@ hipError_t launch_rng(uint32_t seed, uint32_t sample, const uint32_t* pixel, con
     return hipGetLastError();
 }
 hipError_t launch_raygen(const DevScene& s, const PathState& ps, const PassParams& p, hipStream_t stream) {
-    uint32_t n = p.width * p.height;
+    uint32_t n = p.width * p.height * (p.n_samples ? p.n_samples : 1u);
     hipLaunchKernelGGL(k_raygen, dim3((n + 255) / 256), dim3(256), 0, stream, s, ps, p);
     return hipGetLastError();
 }

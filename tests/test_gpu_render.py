@@ -158,3 +158,23 @@ def test_ray_sorting_keeps_images(pg, device, small_interior):
     for f, c in zip(films[1:], counts[1:]):
         assert np.array_equal(films[0], f)
         assert c == counts[0]
+
+
+@pytest.mark.gpu
+def test_samples_per_pass_keep_images(pg, device, small_interior):
+    """Several samples traced together in one wavefront pass (samples_per_pass) change
+    only which rays share a launch: films are bit-identical to one sample per pass, also
+    when the SPP count is not a multiple of it and across successive render calls."""
+    sc = pg.Scene(device, small_interior)
+    films = {}
+    for spp_pass in (1, 3, 0):
+        rt = pg.RayTracer(sc, seed=9, max_depth=12, samples_per_pass=spp_pass)
+        rt.clear()
+        rt.render(5)
+        rt.render(2)
+        film, n = rt.film()
+        st = rt.stats()
+        assert n == 7
+        films[spp_pass] = film
+    assert np.array_equal(films[1], films[3]) and np.array_equal(films[1], films[0])
+    assert films[1].sum() > 0
