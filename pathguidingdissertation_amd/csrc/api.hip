@@ -687,7 +687,7 @@ pg_status pg_render_spp(pg_ctx* c, const pg_render_params* params, uint32_t n_sp
     // samples traced together per pass (plain path tracing only): deep bounces hold few
     // rays whose launches are latency-bound, so several samples share them
     uint32_t spp_pass = 1;
-    if (integ == PG_INTEGRATOR_PATH && mode == 0 && !prm.sort_rays) {
+    if (!record_pass && !prm.sort_rays) {
         spp_pass = prm.samples_per_pass;
         if (spp_pass == 0) {
             spp_pass = 4;
@@ -695,42 +695,43 @@ pg_status pg_render_spp(pg_ctx* c, const pg_render_params* params, uint32_t n_sp
         }
         if ((uint64_t)spp_pass * n > 0x7fffffffull) return c->fail(PG_ERR_INVALID, "samples_per_pass x pixels >= 2^31");
     }
-    pg_status st = ensure_paths(c, n * spp_pass);
+    const uint32_t np = n * spp_pass;  // paths of one pass
+    pg_status st = ensure_paths(c, np);
     if (st != PG_OK) return st;
     if (record_pass && (st = ensure_record_state(c, n, prm.max_depth + 1)) != PG_OK) return st;
     c->rs.li_nee = prm.record_nee ? 1 : 0;
-    if ((mode == 1 || mode == 3) && c->guide_cells_cap < n) {
+    if ((mode == 1 || mode == 3) && c->guide_cells_cap < np) {
         dfree(c->guide_cells);
-        PG_TRY(c, dalloc(&c->guide_cells, (size_t)n * kKnnMax), "alloc guide cells");
-        c->guide_cells_cap = n;
+        PG_TRY(c, dalloc(&c->guide_cells, (size_t)np * kKnnMax), "alloc guide cells");
+        c->guide_cells_cap = np;
     }
     // sort buffers: kNN query order (quadtree guide) and ray trace order (sort_rays)
-    if ((mode == 1 || (mode == 3 && prm.knn_k != 0) || prm.sort_rays) && c->q_cap < n) {
+    if ((mode == 1 || (mode == 3 && prm.knn_k != 0) || prm.sort_rays) && c->q_cap < np) {
         dfree(c->q_keys); dfree(c->q_vals); dfree(c->q_keys_tmp); dfree(c->q_vals_tmp); dfree(c->q_hist); dfree(c->q_scan);
-        PG_TRY(c, dalloc(&c->q_keys, n), "alloc sort buffers");
-        PG_TRY(c, dalloc(&c->q_vals, n), "alloc sort buffers");
-        PG_TRY(c, dalloc(&c->q_keys_tmp, n), "alloc sort buffers");
-        PG_TRY(c, dalloc(&c->q_vals_tmp, n), "alloc sort buffers");
-        PG_TRY(c, dalloc(&c->q_hist, (size_t)256 * sort_tiles(n)), "alloc sort buffers");
-        PG_TRY(c, dalloc(&c->q_scan, scan_tiles(256 * sort_tiles(n)) + 1), "alloc sort buffers");
-        c->q_cap = n;
+        PG_TRY(c, dalloc(&c->q_keys, np), "alloc sort buffers");
+        PG_TRY(c, dalloc(&c->q_vals, np), "alloc sort buffers");
+        PG_TRY(c, dalloc(&c->q_keys_tmp, np), "alloc sort buffers");
+        PG_TRY(c, dalloc(&c->q_vals_tmp, np), "alloc sort buffers");
+        PG_TRY(c, dalloc(&c->q_hist, (size_t)256 * sort_tiles(np)), "alloc sort buffers");
+        PG_TRY(c, dalloc(&c->q_scan, scan_tiles(256 * sort_tiles(np)) + 1), "alloc sort buffers");
+        c->q_cap = np;
     }
     const float3 q_lo = make_float3(c->host.centre[0] - c->host.radius, c->host.centre[1] - c->host.radius,
                                     c->host.centre[2] - c->host.radius);
     const float q_inv = c->host.radius > 0.f ? 1.f / (2.f * c->host.radius) : 0.f;
     // trace order of a ray queue (shadow = 1: the shadow queue) for sort_rays
     auto ray_order = [&](int parity, int shadow) -> hipError_t {
-        hipError_t e = launch_ray_keys(c->ps, q_lo, q_inv, parity, shadow, n, c->q_keys, c->q_vals, c->stream);
+        hipError_t e = launch_ray_keys(c->ps, q_lo, q_inv, parity, shadow, np, c->q_keys, c->q_vals, c->stream);
         if (e != hipSuccess) return e;
-        return radix_sort_pairs(c->q_keys, c->q_vals, c->q_keys_tmp, c->q_vals_tmp, n, 24, c->q_hist, c->q_scan,
+        return radix_sort_pairs(c->q_keys, c->q_vals, c->q_keys_tmp, c->q_vals_tmp, np, 24, c->q_hist, c->q_scan,
                                 c->stream);
     };
     const bool sort_rays = prm.sort_rays != 0 && !record_pass;
     const bool dyn = prm.dyn_fetch != 0 && !record_pass && !sort_rays;
-    if (mode == 2 && c->mix_cap < n) {
+    if (mode == 2 && c->mix_cap < np) {
         dfree(c->mix);
-        PG_TRY(c, dalloc(&c->mix, (size_t)n * kMixF4), "alloc guide mixtures");
-        c->mix_cap = n;
+        PG_TRY(c, dalloc(&c->mix, (size_t)np * kMixF4), "alloc guide mixtures");
+        c->mix_cap = np;
     }
     const int stack = (int)c->host.bvh.max_depth;
     const uint32_t grid_trace = prm.trace_blocks_per_cu ? (uint32_t)c->cu_count * prm.trace_blocks_per_cu
@@ -785,23 +786,23 @@ pg_status pg_render_spp(pg_ctx* c, const pg_render_params* params, uint32_t n_sp
                 PG_TRY(c, timed(4, [&] { return launch_guide_cells(c->dev, c->ps, c->gv, c->guide_cells, parity, grid_guide, c->stream, nullptr, 0, 2); }), "guide records");
             } else if (mode == 3) {
                 PG_TRY(c, timed(4, [&] {
-                    hipError_t e = launch_query_keys(c->dev, c->ps, q_lo, q_inv, parity, n, c->q_keys, c->q_vals, c->stream);
+                    hipError_t e = launch_query_keys(c->dev, c->ps, q_lo, q_inv, parity, np, c->q_keys, c->q_vals, c->stream);
                     if (e == hipSuccess)
-                        e = radix_sort_pairs(c->q_keys, c->q_vals, c->q_keys_tmp, c->q_vals_tmp, n, 24, c->q_hist,
+                        e = radix_sort_pairs(c->q_keys, c->q_vals, c->q_keys_tmp, c->q_vals_tmp, np, 24, c->q_hist,
                                              c->q_scan, c->stream);
                     return e;
                 }), "guide query order");
-                PG_TRY(c, timed(4, [&] { return launch_guide_cells(c->dev, c->ps, c->gv, c->guide_cells, parity, grid_guide, c->stream, c->q_vals, n, 1); }), "guide records");
+                PG_TRY(c, timed(4, [&] { return launch_guide_cells(c->dev, c->ps, c->gv, c->guide_cells, parity, grid_guide, c->stream, c->q_vals, np, 1); }), "guide records");
             }
             if (mode == 1) {
                 PG_TRY(c, timed(4, [&] {
-                    hipError_t e = launch_query_keys(c->dev, c->ps, q_lo, q_inv, parity, n, c->q_keys, c->q_vals, c->stream);
+                    hipError_t e = launch_query_keys(c->dev, c->ps, q_lo, q_inv, parity, np, c->q_keys, c->q_vals, c->stream);
                     if (e == hipSuccess)
-                        e = radix_sort_pairs(c->q_keys, c->q_vals, c->q_keys_tmp, c->q_vals_tmp, n, 24, c->q_hist,
+                        e = radix_sort_pairs(c->q_keys, c->q_vals, c->q_keys_tmp, c->q_vals_tmp, np, 24, c->q_hist,
                                              c->q_scan, c->stream);
                     return e;
                 }), "guide query order");
-                PG_TRY(c, timed(4, [&] { return launch_guide_cells(c->dev, c->ps, c->gv, c->guide_cells, parity, grid_guide, c->stream, c->q_vals, n, 0); }), "guide_cells");
+                PG_TRY(c, timed(4, [&] { return launch_guide_cells(c->dev, c->ps, c->gv, c->guide_cells, parity, grid_guide, c->stream, c->q_vals, np, 0); }), "guide_cells");
             }
             if (mode == 2)
                 PG_TRY(c, timed(4, [&] { return launch_mlp_infer(c->dev, c->ps, c->mlp.view, parity, c->mix, grid_mlp, c->stream); }), "mlp_infer");

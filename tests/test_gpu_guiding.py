@@ -325,3 +325,27 @@ def test_pss_guided_furnace(pg, device):
     print(means)
     assert abs(means[("diffuse", 4)] - 0.5) < 0.01, means
     assert abs(means[("conductor", 4)] - means[("conductor", 0)]) < 0.01 * means[("conductor", 0)] + 2e-3, means
+
+
+def test_guided_samples_per_pass_keep_images(pg, device):
+    """Quadtree-, PSS- and learned-guided passes with several samples per wavefront pass
+    give the films of one sample per pass, bit for bit."""
+    data = pg.generate_scene(101, 0)
+    sc = pg.Scene(device, data)
+    device.clear_records()
+    rec = pg.RayTracer(sc, seed=1, max_depth=12, integrator=1, record=1, record_nee=1)
+    rec.clear()
+    rec.render(2)
+    device.cache_build(cell_size=4096)
+    device.mlp_init(seed=3)
+    device.mlp_train(4)
+    for integ, kw in ((2, {}), (4, {}), (4, {"knn_k": 0}), (3, {})):
+        films = []
+        for spp_pass in (1, 0):
+            rt = pg.RayTracer(sc, seed=21, max_depth=12, integrator=integ, guide_prob=0.5, samples_per_pass=spp_pass,
+                              **kw)
+            rt.clear()
+            rt.render(5)
+            films.append(rt.film()[0])
+        assert np.array_equal(films[0], films[1]), (integ, kw)
+        assert films[0].sum() > 0
