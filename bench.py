@@ -47,19 +47,37 @@ def profiled_traffic(config: int) -> dict:
 
 def reference_image(pg, sc, data, spp: int, max_depth: int, env_visibility: int = 0) -> np.ndarray:
     """Unguided GPU render of `spp` samples per pixel accumulated in FP64 on the host
-    (64-SPP film reads summed in double), seed 1000. The GPU path tracer is the same
+    (64-SPP film reads summed in double; chunk k has seed 1000 + k). The GPU path tracer is the same
     estimator as the CPU oracle (pixel agreement is tested), so this stands in for the
     4096-SPP FP64 CPU image, which would take ~20 minutes on 16 host cores."""
-    rt = pg.RayTracer(sc, seed=1000, max_depth=max_depth, env_visibility=env_visibility)
-    acc = np.zeros((data.height, data.width, 3), np.float64)
+    acc = [np.zeros((data.height, data.width, 3), np.float64) for _ in range(2)]
+    cnt = [0, 0]
     done = 0
+    chunk = 0
     while done < spp:
+        # pg_film_clear restarts the sample index, so every 64-SPP chunk takes its own
+        # seed: the chunks are independent samples of the estimator
+        rt = pg.RayTracer(sc, seed=1000 + chunk, max_depth=max_depth, env_visibility=env_visibility)
         rt.clear()
         rt.render(min(64, spp - done))
         film, n = rt.film()
-        acc += film
+        acc[chunk % 2] += film  # alternate chunks into two independent halves
+        cnt[chunk % 2] += n
         done += n
-    return acc / done
+        chunk += 1
+    ref = (acc[0] + acc[1]) / done
+    halves = (acc[0] / max(cnt[0], 1), acc[1] / max(cnt[1], 1))
+    return ref, halves
+
+
+def relmse_floor(ref, halves, eps=1e-2) -> float:
+    """The reference's own share of relMSE(I, ref): with two independent halves R1, R2 of
+    the reference, (R1 - R2)^2 / 4 estimates the variance of their mean per pixel, so
+    mean((R1 - R2)^2 / 4 / (ref^2 + eps)) estimates E[relMSE(ref, truth)], and
+    relMSE(I, ref) - floor estimates relMSE(I, truth) (the cross term vanishes for
+    independent images)."""
+    d = (halves[0] - halves[1]) ** 2 / 4.0
+    return float(np.mean(d / (ref.astype(np.float64) ** 2 + eps)))
 
 
 def trimmed_relmse(img, ref, drop=1e-4, eps=1e-2) -> float:
@@ -73,7 +91,7 @@ def trimmed_relmse(img, ref, drop=1e-4, eps=1e-2) -> float:
     return float(e.mean())
 
 
-def progressive(pg, label, ref, setup, advance, cap_s, target, log):
+def progressive(pg, label, ref, setup, advance, cap_s, target, log, floor=0.0):
     """Run one method until relMSE <= target or cap_s seconds of its own wall time.
     The clock covers setup and every render/training call; relMSE checkpoints (film
     read-back + FP64 metric on the host) are outside it. Increments double up to 64
@@ -90,14 +108,14 @@ def progressive(pg, label, ref, setup, advance, cap_s, target, log):
         t_total += dt
         spp += k
         e = pg.relmse(img, ref)
-        curve.append((spp, round(t_total, 3), e, trimmed_relmse(img, ref)))
+        curve.append((spp, round(t_total, 3), e, trimmed_relmse(img, ref), e - floor))
         if e <= target:
             hit = (spp, t_total)
             break
         per_spp = dt / k
         k = k * 2 if spp < 64 else max(1, min(256, int(1.0 / max(per_spp, 1e-4))))
     log(f"ttr {label}: {len(curve)} checkpoints, {spp} SPP in {t_total:.1f}s, relMSE {curve[-1][2]:.4f}"
-        f" (trimmed {curve[-1][3]:.4f})"
+        f" (trimmed {curve[-1][3]:.4f}, less the reference's own {curve[-1][4]:.4f})"
         + (f", reached {target} at {hit[1]:.2f}s" if hit else ""))
     return curve, hit
 
@@ -117,7 +135,7 @@ def time_staying_below(curve, e, col=2):
 
 
 def time_to_relmse(pg, dev, sc, data, max_depth, ref_spp, cap_s, log, target=0.01, env_visibility=1, repeats=1,
-                   with_pss=False):
+                   with_pss=False, train_every=1):
     """Time-to-relMSE study: unguided vs quadtree guide vs learned guide. Besides the
     time to relMSE <= target (null when a method does not reach it within cap_s), the
     learned guide is compared with each baseline at that baseline's own final relMSE:
@@ -127,8 +145,10 @@ def time_to_relmse(pg, dev, sc, data, max_depth, ref_spp, cap_s, log, target=0.0
     the comparisons of each run and their medians are reported (curves of run 0)."""
     t0 = time.perf_counter()
     ev = dict(env_visibility=env_visibility)
-    ref = reference_image(pg, sc, data, ref_spp, max_depth, env_visibility)
-    log(f"ttr reference: {ref_spp} SPP in {time.perf_counter() - t0:.1f}s, mean {ref.mean():.5f}")
+    ref, halves = reference_image(pg, sc, data, ref_spp, max_depth, env_visibility)
+    floor = relmse_floor(ref, halves)
+    log(f"ttr reference: {ref_spp} SPP in {time.perf_counter() - t0:.1f}s, mean {ref.mean():.5f}, "
+        f"own relMSE {floor:.4f}")
 
     def adv(rt, fn):
         def f(k):
@@ -173,14 +193,18 @@ def time_to_relmse(pg, dev, sc, data, max_depth, ref_spp, cap_s, log, target=0.0
         if with_pss:
             methods["pss"] = (ps_setup, adv(ps, lambda k: ps.render(k)))
             methods["pss_approx"] = (pf_setup, adv(pf, lambda k: pf.render(k)))
-        methods["learned"] = (ml_setup, adv(ml, lambda k: ml.render_online(k, steps=1, window=1)))
+        methods["learned"] = (ml_setup, adv(ml, lambda k: ml.render_online(k, steps=1, window=1,
+                                                                          train_every=train_every)))
         res = {}
         for name, (setup, advance) in methods.items():
             curve, hit = progressive(pg, name if r == 0 else f"{name} (run {r})", ref, setup, advance, cap_s,
-                                     target, log)
+                                     target, log, floor)
+            t_c = time_staying_below(curve, target, 4)
             res[name] = {"seconds": None if hit is None else round(hit[1], 3),
                          "spp": curve[-1][0], "final_relmse": round(curve[-1][2], 5),
-                         "final_trimmed_relmse": round(curve[-1][3], 5), "curve": curve}
+                         "final_trimmed_relmse": round(curve[-1][3], 5),
+                         "final_relmse_less_reference": round(curve[-1][4], 5),
+                         "seconds_to_target_less_reference": t_c[0], "curve": curve}
             if name == "learned":
                 st = ml.stats()
                 res[name]["kernel_ms_per_spp"] = {
@@ -199,8 +223,17 @@ def time_to_relmse(pg, dev, sc, data, max_depth, ref_spp, cap_s, log, target=0.0
         def ratio(a, b, key):
             return round(res[a][key] / res[b][key], 3)
 
+        def at_target(fast, slow):
+            # both reach relMSE <= target once the reference's own share is removed
+            a, b = res[fast]["seconds_to_target_less_reference"], res[slow]["seconds_to_target_less_reference"]
+            return {f"{slow}_s": b, f"{fast}_s": a, "speedup": round(b / a, 3) if a and b else None}
+
         cmp = {"learned_vs_unguided": speedup("learned", "unguided"),
                "learned_vs_quadtree": speedup("learned", "quadtree"),
+               "less_reference": {"learned_vs_unguided": speedup("learned", "unguided", 4),
+                                  "learned_vs_quadtree": speedup("learned", "quadtree", 4),
+                                  "at_target_learned_vs_unguided": at_target("learned", "unguided"),
+                                  "at_target_learned_vs_quadtree": at_target("learned", "quadtree")},
                "trimmed": {"learned_vs_unguided": speedup("learned", "unguided", 3),
                            "learned_vs_quadtree": speedup("learned", "quadtree", 3)},
                "relmse_ratio_at_cap": {"unguided_over_learned": ratio("unguided", "learned", "final_relmse"),
@@ -224,8 +257,8 @@ def time_to_relmse(pg, dev, sc, data, max_depth, ref_spp, cap_s, log, target=0.0
     for rr in res.values():
         c = rr["curve"]
         step = max(1, len(c) // 16)
-        rr["curve"] = [[s, t, round(e, 5), round(et, 5)]
-                       for s, t, e, et in (c[::step] + ([c[-1]] if (len(c) - 1) % step else []))]
+        rr["curve"] = [[s, t, round(e, 5), round(et, 5), round(ec, 5)]
+                       for s, t, e, et, ec in (c[::step] + ([c[-1]] if (len(c) - 1) % step else []))]
     # network FLOPs over the used layer sizes (80 -> 64 -> 64 -> 64 -> 41): 2 * sum(in * out)
     # per query forward; a training sample adds the input and weight gradients (3x)
     fwd = 2 * (80 * 64 + 64 * 64 + 64 * 64 + 64 * 41)
@@ -239,20 +272,25 @@ def time_to_relmse(pg, dev, sc, data, max_depth, ref_spp, cap_s, log, target=0.0
            "inference_ms_per_spp": res["learned"]["kernel_ms_per_spp"]["guide"],
            "bound": "hash-grid gradient atomics (training) and gathers (inference), not MFMA"}
     out = {"target": target, "cap_s": cap_s,
-           "reference": f"GPU unguided {ref_spp} SPP, FP64 host accumulation, seed 1000",
+           "reference": f"GPU unguided {ref_spp} SPP, FP64 host accumulation, 64-SPP chunks with seeds 1000 + k",
            "env_visibility": ("environment shadow rays along the light direction for every method and the "
                               "reference image (the reference's target sceneCentre + wi * sceneRadius, "
                               "Renderer.h:331-332, loses the direct sun behind the window, and guided and "
                               "unguided MIS estimators then converge to different images)"
                               if env_visibility else "Renderer.h:331-332 scene-bound target"),
-           "curve_columns": ["spp", "seconds", "relmse", "relmse_without_top_0.01pct_pixels"],
+           "curve_columns": ["spp", "seconds", "relmse", "relmse_without_top_0.01pct_pixels",
+                             "relmse_less_reference_own"],
+           "reference_own_relmse": round(floor, 5),
+           "reference_own_relmse_note": ("mean((R1 - R2)^2 / 4 / (R^2 + 1e-2)) over two independent halves of the "
+                                         "reference: its own noise, subtracted in the less_reference column"),
            "methods": {k: v for k, v in {"unguided": "path tracer with NEE + MIS (Renderer.h:531-618)",
                        "quadtree": "1-SPP record pass, point-BVH + per-cell quadtrees, 50/50 one-sample MIS, kNN k=16",
                        "pss": "1-SPP record pass, point-BVH; per vertex the 16 nearest records inverted into the "
                               "BSDF's primary sample space (Renderer.h:398-414), 4x4 flux histogram, 50/50 MIS",
                        "pss_approx": "as pss with approximate neighbours (16 nearest of the 64 records under a "
                                      "greedy point-BVH descent)",
-                       "learned": "online: 1-SPP guided record passes, 1 Adam step of 2^18 samples per pass"}.items()
+                       "learned": f"online: a 1-SPP guided record pass and 1 Adam step of 2^18 samples every "
+                                  f"{train_every} SPP, guided passes between"}.items()
                        if k in res},
            **res,
            "learned_diverged": bool(info["diverged"]),
@@ -365,6 +403,8 @@ def main() -> int:
     ap.add_argument("--ttr-env-visibility", type=int, choices=(0, 1), default=1,
                     help="environment shadow rays in the study: 1 along the light direction (default), "
                          "0 the reference's scene-bound target (Renderer.h:331-332)")
+    ap.add_argument("--ttr-train-every", type=int, default=int(os.environ.get("PG_BENCH_TTR_TRAIN_EVERY", "1")),
+                    help="learned guide: a record pass + training step every N passes, guided passes between")
     ap.add_argument("--ttr-pss", action="store_true",
                     help="add the primary-sample-space guide (exact and approximate neighbours) to the study")
     ap.add_argument("--copy-check", action="store_true",
@@ -466,7 +506,7 @@ def main() -> int:
         tsc = sc if a.ttr_config == a.config else pg.Scene(dev, tdata)
         ttr = time_to_relmse(pg, dev, tsc, tdata, max_depth, a.ttr_ref_spp, a.ttr_cap,
                              lambda m: print(m, file=sys.stderr, flush=True), env_visibility=a.ttr_env_visibility,
-                             repeats=a.ttr_repeats, with_pss=a.ttr_pss)
+                             repeats=a.ttr_repeats, with_pss=a.ttr_pss, train_every=a.ttr_train_every)
         ttr = {"workload": f"config {a.ttr_config}: {tdata.n_tris} tris, {tdata.width}x{tdata.height}, "
                            f"max depth {max_depth}", **ttr}
 

@@ -1473,11 +1473,24 @@ pg_status pg_mlp_render_online(pg_ctx* c, const pg_render_params* params, uint32
     pg_render_params p = params ? *params : pg_render_params_default();
     p.integrator = PG_INTEGRATOR_GUIDED_MLP;
     auto& m = c->mlp;
-    for (uint32_t s = 0; s < n_spp; ++s) {
+    for (uint32_t s = 0; s < n_spp;) {
         const bool record = (m.online_passes % sch.train_every) == 0;
+        if (!record) {
+            // the guided passes up to the next record pass go in one call, so that they
+            // share wavefront passes (samples_per_pass)
+            uint32_t k = 1;
+            while (s + k < n_spp && ((m.online_passes + k) % sch.train_every) != 0) ++k;
+            m.online_passes += k;
+            s += k;
+            p.record = 0;
+            pg_status st = pg_render_spp(c, &p, k);
+            if (st != PG_OK) return st;
+            continue;
+        }
+        ++s;
         m.online_passes++;
-        p.record = record ? 1 : 0;
-        if (record && m.window_passes >= sch.window) {
+        p.record = 1;
+        if (m.window_passes >= sch.window) {
             c->n_records = 0;
             c->n_aux = 0;
             c->rec_gen++;
@@ -1486,7 +1499,6 @@ pg_status pg_mlp_render_online(pg_ctx* c, const pg_render_params* params, uint32
         }
         pg_status st = pg_render_spp(c, &p, 1);
         if (st != PG_OK) return st;
-        if (!record) continue;
         m.window_passes++;
         m.record_passes++;
         uint32_t steps = m.record_passes <= sch.warmup_passes ? sch.warmup_steps : sch.steps;

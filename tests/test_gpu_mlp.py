@@ -338,3 +338,23 @@ def test_learned_guide_gains_over_unguided(pg, device):
     for m in means:
         assert abs(m - ref.mean()) / ref.mean() < 0.05
     assert np.mean(e_ml) < 0.93 * np.mean(e_un), (e_ml, e_un)
+
+
+def test_online_guided_passes_between_records_share_wavefronts(pg, device):
+    """With a record pass every 4th pass, the guided passes between records run as one
+    multi-sample render; the film and the training equal one sample per pass."""
+    data = pg.generate_scene(101, 0)
+    sc = pg.Scene(device, data)
+    films, steps = [], []
+    for spp_pass in (1, 0):
+        device.mlp_init(seed=9)
+        rt = pg.RayTracer(sc, seed=4, max_depth=12, record_nee=1, samples_per_pass=spp_pass)
+        rt.clear()
+        rt.render_online(11, steps=1, train_every=4)
+        film, spp = rt.film()
+        assert spp == 11
+        films.append(film)
+        steps.append(device.mlp_info()["step"])
+    assert steps == [3, 3]  # record passes 0, 4 and 8
+    assert np.array_equal(films[0], films[1])
+    assert films[0].sum() > 0
