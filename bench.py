@@ -403,7 +403,7 @@ def main() -> int:
     ap.add_argument("--ttr-env-visibility", type=int, choices=(0, 1), default=1,
                     help="environment shadow rays in the study: 1 along the light direction (default), "
                          "0 the reference's scene-bound target (Renderer.h:331-332)")
-    ap.add_argument("--ttr-train-every", type=int, default=int(os.environ.get("PG_BENCH_TTR_TRAIN_EVERY", "1")),
+    ap.add_argument("--ttr-train-every", type=int, default=int(os.environ.get("PG_BENCH_TTR_TRAIN_EVERY", "4")),
                     help="learned guide: a record pass + training step every N passes, guided passes between")
     ap.add_argument("--ttr-pss", action="store_true",
                     help="add the primary-sample-space guide (exact and approximate neighbours) to the study")

@@ -342,19 +342,28 @@ def test_learned_guide_gains_over_unguided(pg, device):
 
 def test_online_guided_passes_between_records_share_wavefronts(pg, device):
     """With a record pass every 4th pass, the guided passes between records run as one
-    multi-sample render; the film and the training equal one sample per pass."""
+    multi-sample render. Without training steps the film equals one sample per pass bit
+    for bit; with one step per record pass the Adam steps are the same and the images
+    agree (training scatters hash-grid gradients with FP32 atomics, so two trained
+    networks differ in rounding)."""
     data = pg.generate_scene(101, 0)
     sc = pg.Scene(device, data)
-    films, steps = [], []
-    for spp_pass in (1, 0):
-        device.mlp_init(seed=9)
-        rt = pg.RayTracer(sc, seed=4, max_depth=12, record_nee=1, samples_per_pass=spp_pass)
-        rt.clear()
-        rt.render_online(11, steps=1, train_every=4)
-        film, spp = rt.film()
-        assert spp == 11
-        films.append(film)
-        steps.append(device.mlp_info()["step"])
-    assert steps == [3, 3]  # record passes 0, 4 and 8
-    assert np.array_equal(films[0], films[1])
-    assert films[0].sum() > 0
+    for steps in (0, 1):
+        films, n_steps = [], []
+        for spp_pass in (1, 0):
+            device.mlp_init(seed=9)
+            rt = pg.RayTracer(sc, seed=4, max_depth=12, record_nee=1, samples_per_pass=spp_pass)
+            rt.clear()
+            rt.render_online(11, steps=steps, train_every=4)
+            film, spp = rt.film()
+            assert spp == 11
+            films.append(film)
+            n_steps.append(device.mlp_info()["step"])
+        assert films[0].sum() > 0
+        if steps == 0:
+            assert n_steps == [0, 0]
+            assert np.array_equal(films[0], films[1])
+        else:
+            assert n_steps == [3, 3]  # record passes 0, 4 and 8
+            m0, m1 = float(films[0].mean()), float(films[1].mean())
+            assert abs(m0 - m1) < 0.05 * m0, (m0, m1)
