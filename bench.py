@@ -448,7 +448,9 @@ def main() -> int:
     tuned_ms = rt.tune()
     tune_s = time.perf_counter() - t_tune
     shape = {"trace_blocks_per_cu": rt.params.trace_blocks_per_cu or "lds-max",
-             "shade_blocks_per_cu": rt.params.shade_blocks_per_cu or 8, "tune_s": round(tune_s, 2),
+             "shade_blocks_per_cu": rt.params.shade_blocks_per_cu or 8,
+             "extend_kernel": "k_extend_dyn" if rt.params.dyn_fetch else "k_extend",
+             "samples_per_pass": rt.params.samples_per_pass or "auto", "tune_s": round(tune_s, 2),
              "tuned_ms_per_spp": round(tuned_ms, 3)}
     print(f"bench: launch shape {shape}", file=sys.stderr, flush=True)
 
@@ -473,7 +475,8 @@ def main() -> int:
 
     img = rt.image()
     finite = bool(np.isfinite(img).all())
-    traffic = profiled_traffic(a.config).get("k_extend")
+    ext_name = "k_extend_dyn" if rt.params.dyn_fetch else "k_extend"
+    traffic = profiled_traffic(a.config).get(ext_name)
 
     copy = None
     if a.copy_check:
@@ -532,7 +535,7 @@ def main() -> int:
         "shadow_rays": st["shadow_rays"],
         "kernel_ms": {"extend": round(ext_ms, 3), "shade": round(st["kernel_ms_shade"], 3),
                       "shadow": round(sh_ms, 3), "gpu_total": round(st["kernel_ms_total"], 3)},
-        "roofline": {"bound": "hbm", "kernel": "k_extend", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
+        "roofline": {"bound": "hbm", "kernel": ext_name, "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
                      "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
                      "traffic": None if traffic is None else round(traffic),
                      "traffic_source": "profiles/pmc_traffic_config%d.txt (bytes per launch)" % a.config,
