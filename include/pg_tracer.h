@@ -107,11 +107,13 @@ typedef struct {
                                (behind a small window: direct sun lost), and then
                                guided and unguided MIS estimators converge to
                                different images; 1 makes them agree.            */
-    uint32_t samples_per_pass; /* path tracer: samples per pixel traced together in
-                               one wavefront pass (the deep bounces' few rays then
-                               share launches); 0 = automatic (up to 4, at most
-                               2^22 paths). Images are identical for every value.
-                               Record and guided passes trace one sample.        */
+    uint32_t samples_per_pass; /* samples per pixel traced together in one
+                               wavefront pass (the deep bounces' few rays then
+                               share launches); 0 = automatic: the call's SPP
+                               split evenly over passes of at most 2^25 paths
+                               (about 7 GB of path state, twice that with a
+                               guide). Images are identical for every value.
+                               Record passes trace one sample per pass.          */
     uint32_t reserved[2];
 } pg_render_params;
 

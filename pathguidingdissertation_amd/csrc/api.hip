@@ -684,14 +684,16 @@ pg_status pg_render_spp(pg_ctx* c, const pg_render_params* params, uint32_t n_sp
     uint32_t n = W * H;
     // record passes: the reference's record integrator, optionally guided by the learned guide
     const bool record_pass = integ == PG_INTEGRATOR_RECORD || (integ == PG_INTEGRATOR_GUIDED_MLP && prm.record);
-    // samples traced together per pass (plain path tracing only): deep bounces hold few
+    // samples traced together per pass (not in record passes): deep bounces hold few
     // rays whose launches are latency-bound, so several samples share them
     uint32_t spp_pass = 1;
     if (!record_pass && !prm.sort_rays) {
         spp_pass = prm.samples_per_pass;
         if (spp_pass == 0) {
-            spp_pass = 4;
-            while (spp_pass > 1 && (uint64_t)spp_pass * n > (1ull << 22)) --spp_pass;
+            // the call's samples split evenly over passes of at most 2^25 paths
+            const uint32_t cap = (uint32_t)std::max<uint64_t>(1, (1ull << 25) / n);
+            const uint64_t passes = std::max<uint64_t>(1, ((uint64_t)n_spp + cap - 1) / cap);
+            spp_pass = (uint32_t)std::max<uint64_t>(1, ((uint64_t)n_spp + passes - 1) / passes);
         }
         if ((uint64_t)spp_pass * n > 0x7fffffffull) return c->fail(PG_ERR_INVALID, "samples_per_pass x pixels >= 2^31");
     }

@@ -178,3 +178,18 @@ def test_samples_per_pass_keep_images(pg, device, small_interior):
         films[spp_pass] = film
     assert np.array_equal(films[1], films[3]) and np.array_equal(films[1], films[0])
     assert films[1].sum() > 0
+
+
+def test_automatic_samples_per_pass_takes_the_whole_call(pg, device, small_interior):
+    """samples_per_pass = 0 traces a call's 70 samples in one pass on a small image (well
+    under 2^25 paths); the film equals one sample per pass and 16 per pass bit for bit."""
+    sc = pg.Scene(device, small_interior)
+    films = []
+    for spp_pass in (1, 16, 0):
+        rt = pg.RayTracer(sc, seed=13, max_depth=12, samples_per_pass=spp_pass)
+        rt.clear()
+        rt.render(70)
+        film, n = rt.film()
+        assert n == 70
+        films.append(film)
+    assert np.array_equal(films[0], films[1]) and np.array_equal(films[0], films[2])
