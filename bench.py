@@ -135,7 +135,7 @@ def time_staying_below(curve, e, col=2):
 
 
 def time_to_relmse(pg, dev, sc, data, max_depth, ref_spp, cap_s, log, target=0.01, env_visibility=1, repeats=1,
-                   with_pss=False, train_every=1):
+                   with_pss=False, train_every=1, train_steps=1):
     """Time-to-relMSE study: unguided vs quadtree guide vs learned guide. Besides the
     time to relMSE <= target (null when a method does not reach it within cap_s), the
     learned guide is compared with each baseline at that baseline's own final relMSE:
@@ -193,7 +193,7 @@ def time_to_relmse(pg, dev, sc, data, max_depth, ref_spp, cap_s, log, target=0.0
         if with_pss:
             methods["pss"] = (ps_setup, adv(ps, lambda k: ps.render(k)))
             methods["pss_approx"] = (pf_setup, adv(pf, lambda k: pf.render(k)))
-        methods["learned"] = (ml_setup, adv(ml, lambda k: ml.render_online(k, steps=1, window=1,
+        methods["learned"] = (ml_setup, adv(ml, lambda k: ml.render_online(k, steps=train_steps, window=1,
                                                                           train_every=train_every)))
         res = {}
         for name, (setup, advance) in methods.items():
@@ -289,8 +289,8 @@ def time_to_relmse(pg, dev, sc, data, max_depth, ref_spp, cap_s, log, target=0.0
                               "BSDF's primary sample space (Renderer.h:398-414), 4x4 flux histogram, 50/50 MIS",
                        "pss_approx": "as pss with approximate neighbours (16 nearest of the 64 records under a "
                                      "greedy point-BVH descent)",
-                       "learned": f"online: a 1-SPP guided record pass and 1 Adam step of 2^18 samples every "
-                                  f"{train_every} SPP, guided passes between"}.items()
+                       "learned": f"online: a 1-SPP guided record pass and {train_steps} Adam step(s) of 2^18 "
+                                  f"samples every {train_every} SPP, guided passes between"}.items()
                        if k in res},
            **res,
            "learned_diverged": bool(info["diverged"]),
@@ -403,8 +403,10 @@ def main() -> int:
     ap.add_argument("--ttr-env-visibility", type=int, choices=(0, 1), default=1,
                     help="environment shadow rays in the study: 1 along the light direction (default), "
                          "0 the reference's scene-bound target (Renderer.h:331-332)")
-    ap.add_argument("--ttr-train-every", type=int, default=int(os.environ.get("PG_BENCH_TTR_TRAIN_EVERY", "4")),
-                    help="learned guide: a record pass + training step every N passes, guided passes between")
+    ap.add_argument("--ttr-train-every", type=int, default=int(os.environ.get("PG_BENCH_TTR_TRAIN_EVERY", "16")),
+                    help="learned guide: a record pass + training steps every N passes, guided passes between")
+    ap.add_argument("--ttr-train-steps", type=int, default=int(os.environ.get("PG_BENCH_TTR_TRAIN_STEPS", "4")),
+                    help="learned guide: Adam steps after each record pass")
     ap.add_argument("--ttr-pss", action="store_true",
                     help="add the primary-sample-space guide (exact and approximate neighbours) to the study")
     ap.add_argument("--copy-check", action="store_true",
@@ -509,7 +511,8 @@ def main() -> int:
         tsc = sc if a.ttr_config == a.config else pg.Scene(dev, tdata)
         ttr = time_to_relmse(pg, dev, tsc, tdata, max_depth, a.ttr_ref_spp, a.ttr_cap,
                              lambda m: print(m, file=sys.stderr, flush=True), env_visibility=a.ttr_env_visibility,
-                             repeats=a.ttr_repeats, with_pss=a.ttr_pss, train_every=a.ttr_train_every)
+                             repeats=a.ttr_repeats, with_pss=a.ttr_pss, train_every=a.ttr_train_every,
+                             train_steps=a.ttr_train_steps)
         ttr = {"workload": f"config {a.ttr_config}: {tdata.n_tris} tris, {tdata.width}x{tdata.height}, "
                            f"max depth {max_depth}", **ttr}
 
