@@ -110,8 +110,8 @@ typedef struct {
     uint32_t samples_per_pass; /* samples per pixel traced together in one
                                wavefront pass (the deep bounces' few rays then
                                share launches); 0 = automatic: the call's SPP
-                               split evenly over passes of at most 2^25 paths
-                               (about 7 GB of path state, twice that with a
+                               split evenly over passes of at most 2^26 paths
+                               (about 14 GB of path state, twice that with a
                                guide). Images are identical for every value.
                                Record passes trace one sample per pass.          */
     uint32_t reserved[2];

@@ -690,8 +690,8 @@ pg_status pg_render_spp(pg_ctx* c, const pg_render_params* params, uint32_t n_sp
     if (!record_pass && !prm.sort_rays) {
         spp_pass = prm.samples_per_pass;
         if (spp_pass == 0) {
-            // the call's samples split evenly over passes of at most 2^25 paths
-            const uint32_t cap = (uint32_t)std::max<uint64_t>(1, (1ull << 25) / n);
+            // the call's samples split evenly over passes of at most 2^26 paths
+            const uint32_t cap = (uint32_t)std::max<uint64_t>(1, (1ull << 26) / n);
             const uint64_t passes = std::max<uint64_t>(1, ((uint64_t)n_spp + cap - 1) / cap);
             spp_pass = (uint32_t)std::max<uint64_t>(1, ((uint64_t)n_spp + passes - 1) / passes);
         }

@@ -182,7 +182,7 @@ def test_samples_per_pass_keep_images(pg, device, small_interior):
 
 def test_automatic_samples_per_pass_takes_the_whole_call(pg, device, small_interior):
     """samples_per_pass = 0 traces a call's 70 samples in one pass on a small image (well
-    under 2^25 paths); the film equals one sample per pass and 16 per pass bit for bit."""
+    under 2^26 paths); the film equals one sample per pass and 16 per pass bit for bit."""
     sc = pg.Scene(device, small_interior)
     films = []
     for spp_pass in (1, 16, 0):
