@@ -1,9 +1,10 @@
 """Benchmark: unguided wavefront path tracing of the ~1.09M-triangle procedural interior
 (1280x720, max depth 12, env map through an alias table) on one MI355X.
 
-A step is one sample pass over the full frame (921,600 paths, one per pixel). The
-default run times 64 steps (the workload's 64 SPP) after 4 warmup steps and prints one
-JSON line: Mrays/s (closest-hit + shadow rays traced), the roofline of the dominant
+A step is one sample per pixel over the full frame (921,600 paths). The default run
+times 64 steps (the workload's 64 SPP, traced together as one wavefront pass of 64
+samples per pixel, the library's automatic samples_per_pass) after 4 warmup steps and
+prints one JSON line: Mrays/s (closest-hit + shadow rays traced), the roofline of the dominant
 kernel (extend: closest-hit traversal) from HIP-event timings taken inside the timed
 region, and the CPU oracle timed on the host cores on a bounded sample. A second part
 (skipped with --no-ttr) measures time-to-relMSE of the unguided, quadtree-guided and
