@@ -321,7 +321,7 @@ pg_status pg_mlp_gradient(pg_ctx* ctx, uint32_t n, const float* samples, float* 
  * (Renderer.h:648-747): many views of one or more scenes rendered together. Every
  * pass runs all views' paths through the same persistent (grid-stride) wavefront
  * kernels in view-aligned chunks of at most queue_paths paths; with use_graph the
- * pass is captured once as a hipGraph and replayed per SPP (the sample index lives
+ * pass is captured once as a hipGraph and replayed per pass (the sample index lives
  * on the device). A view's samples are exactly those of rendering it alone with the
  * same seed (RNG keyed by seed, sample and pixel within the view). Unguided only.  */
 typedef struct pg_batch pg_batch;
@@ -333,7 +333,12 @@ typedef struct pg_batch_config {
                                      up to 16; must hold the deepest BVH)             */
     uint32_t use_graph;           /* replay a captured hipGraph per SPP               */
     uint32_t dyn_fetch;           /* persistent lanes with ray claims (see render params) */
-    uint32_t reserved[2];
+    uint32_t samples_per_pass;    /* samples per pixel traced together in one pass (0: 1;
+                                     at most 64, and with more than one at most 2^26
+                                     paths per chunk); a render's samples beyond whole
+                                     passes run as one shorter pass; views are identical
+                                     for every value                                  */
+    uint32_t reserved;
 } pg_batch_config;
 typedef struct pg_batch_stats {
     uint64_t paths, closest_rays, shadow_rays;
@@ -356,8 +361,9 @@ pg_status pg_batch_scene_textures(pg_batch* batch, uint32_t scene_id, const floa
 pg_status pg_batch_add_view(pg_batch* batch, uint32_t scene_id, const pg_camera_desc* camera,
                             uint32_t* view_id);
 pg_status pg_batch_configure(pg_batch* batch, const pg_batch_config* cfg);
-/* Times candidate configurations (queue size, blocks per CU, stack depth, graph)
- * by coordinate descent on a few SPP each, keeps the fastest and clears the films. */
+/* Times candidate configurations (graph, ray claims, queue size, blocks per CU, stack
+ * depth, samples per pass) by coordinate descent on a few passes each, keeps the
+ * fastest and clears the films. */
 pg_status pg_batch_autotune(pg_batch* batch, const pg_render_params* params, pg_batch_config* best,
                             double* ms_per_spp);
 /* Learned guide of one scene of the batch (no reference counterpart; configuration 5

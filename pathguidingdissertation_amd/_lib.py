@@ -83,7 +83,7 @@ class PgImageMetrics(C.Structure):
 class PgBatchConfig(C.Structure):
     _fields_ = [("queue_paths", C.c_uint32), ("trace_blocks_per_cu", C.c_uint32), ("shade_blocks_per_cu", C.c_uint32),
                 ("stack", C.c_uint32), ("use_graph", C.c_uint32), ("dyn_fetch", C.c_uint32),
-                ("reserved", C.c_uint32 * 2)]
+                ("samples_per_pass", C.c_uint32), ("reserved", C.c_uint32)]
 
 
 class PgBatchStats(C.Structure):

@@ -128,7 +128,7 @@ struct pg_batch {
     float4* film = nullptr;
     uint32_t film_cap = 0;
     uint32_t spp = 0;
-    pg_batch_config cfg{0, 0, 0, 0, 1, 0, {0, 0}};
+    pg_batch_config cfg{0, 0, 0, 0, 1, 0, 0, 0};
     std::vector<std::pair<uint32_t, uint32_t>> chunks;  // (first global path, paths)
     std::vector<std::vector<uint32_t>> chunk_scenes;    // scenes with views in each chunk
     float4* mix = nullptr;                              // learned-guide mixtures by queue slot
@@ -166,6 +166,7 @@ struct pg_batch {
         int s = cfg.stack ? (int)cfg.stack : max_depth;
         return stack_entries(std::max(s, 1));
     }
+    uint32_t samples_per_pass() const { return cfg.samples_per_pass ? cfg.samples_per_pass : 1u; }
 };
 
 #define PG_BTRY(b, expr, what)                        \
@@ -224,10 +225,14 @@ pg_status prepare(pg_batch* b) {
         b->chunk_scenes.push_back(cs);
     }
     for (auto& c : b->chunks) largest = std::max(largest, c.second);
-    if (b->ps_cap < largest) {
+    // path state for the largest chunk times the samples traced together per pass
+    const uint64_t need = (uint64_t)largest * b->samples_per_pass();
+    if (b->samples_per_pass() > 1 && need > (1ull << 26))
+        return b->fail(PG_ERR_INVALID, "samples_per_pass x chunk paths above 2^26 (lower one of them)");
+    if (b->ps_cap < need) {
         b->free_paths();
         PathState& p = b->ps;
-        uint32_t m = largest;
+        const uint32_t m = (uint32_t)need;
         PG_BTRY(b, balloc(&p.beta, m), "alloc batch paths");
         PG_BTRY(b, balloc(&p.radiance, m), "alloc batch paths");
         PG_BTRY(b, balloc(&p.film_pos, m), "alloc batch paths");
@@ -253,8 +258,8 @@ pg_status prepare(pg_batch* b) {
     return PG_OK;
 }
 
-// One pass (one sample per pixel of every view) on the batch stream.
-hipError_t enqueue_pass(pg_batch* b, const pg_render_params& prm) {
+// One pass (ns samples per pixel of every view) on the batch stream.
+hipError_t enqueue_pass(pg_batch* b, const pg_render_params& prm, uint32_t ns) {
     const int cu = pg_internal::cu_count(b->ctx);
     const int stack = b->stack();
     const uint32_t tgrid = b->cfg.trace_blocks_per_cu ? (uint32_t)cu * b->cfg.trace_blocks_per_cu : trace_grid(cu, stack);
@@ -269,7 +274,8 @@ hipError_t enqueue_pass(pg_batch* b, const pg_render_params& prm) {
     for (size_t ci = 0; ci < b->chunks.size(); ++ci) {
         const auto& ch = b->chunks[ci];
         PassParams pp{prm.seed, 0u, 0u, 0u, prm.max_depth, prm.rr_depth, prm.integrator, prm.env_visibility, 1u};
-        if ((e = launch_raygen_batch(b->ps, nv, ch.first, ch.second, prm.seed, b->d_sample, st)) != hipSuccess) return e;
+        if ((e = launch_raygen_batch(b->ps, nv, ch.first, ch.second, ns, prm.seed, b->d_sample, st)) != hipSuccess)
+            return e;
         for (uint32_t depth = 0; depth <= prm.max_depth; ++depth) {
             int parity = (int)(depth & 1u);
             if ((e = launch_extend(s0, b->ps, parity, stack, false, tgrid, st, true, nullptr, b->cfg.dyn_fetch != 0)) !=
@@ -293,9 +299,9 @@ hipError_t enqueue_pass(pg_batch* b, const pg_render_params& prm) {
                 hipSuccess)
                 return e;
         }
-        if ((e = launch_resolve_batch(b->ps, nv, ch.first, ch.second, b->film, st)) != hipSuccess) return e;
+        if ((e = launch_resolve_batch(b->ps, nv, ch.first, ch.second, ns, b->film, st)) != hipSuccess) return e;
     }
-    return launch_sample_advance(b->d_sample, st);
+    return launch_sample_advance(b->d_sample, ns, st);
 }
 
 bool same_params(const pg_render_params& a, const pg_render_params& b) {
@@ -328,10 +334,13 @@ pg_status render(pg_batch* b, const pg_render_params& prm, uint32_t n_spp) {
         PG_BTRY(b, balloc(&b->mix, (size_t)b->ps_cap * kMixF4), "alloc guide mixtures");
         b->mix_cap = b->ps_cap;
     }
-    if (b->cfg.use_graph && (!b->graph_ok || !same_params(prm, b->gparams))) {
+    // passes of ns samples (the captured graph), then one pass of the samples left over
+    const uint32_t ns = b->samples_per_pass();
+    const uint32_t full = n_spp / ns, rest = n_spp % ns;
+    if (b->cfg.use_graph && full > 0 && (!b->graph_ok || !same_params(prm, b->gparams))) {
         b->drop_graph();
         PG_BTRY(b, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin capture");
-        hipError_t e = enqueue_pass(b, prm);
+        hipError_t e = enqueue_pass(b, prm, ns);
         hipGraph_t g = nullptr;
         hipError_t e2 = hipStreamEndCapture(s, &g);
         if (e != hipSuccess) {
@@ -346,10 +355,11 @@ pg_status render(pg_batch* b, const pg_render_params& prm, uint32_t n_spp) {
         b->gparams = prm;
     }
     PG_BTRY(b, hipEventRecord(b->e0, s), "event");
-    for (uint32_t k = 0; k < n_spp; ++k) {
+    for (uint32_t k = 0; k < full; ++k) {
         if (b->cfg.use_graph) PG_BTRY(b, hipGraphLaunch(b->exec, s), "graph launch");
-        else PG_BTRY(b, enqueue_pass(b, prm), "batch pass");
+        else PG_BTRY(b, enqueue_pass(b, prm, ns), "batch pass");
     }
+    if (rest) PG_BTRY(b, enqueue_pass(b, prm, rest), "batch pass");
     PG_BTRY(b, hipEventRecord(b->e1, s), "event");
     PG_BTRY(b, hipStreamSynchronize(s), "batch render");
     float ms = 0.f;
@@ -476,10 +486,11 @@ pg_status pg_batch_add_view(pg_batch* b, uint32_t scene_id, const pg_camera_desc
 
 pg_status pg_batch_configure(pg_batch* b, const pg_batch_config* cfg) {
     if (!b) return PG_ERR_INVALID;
-    pg_batch_config c = cfg ? *cfg : pg_batch_config{0, 0, 0, 0, 1, 0, {0, 0}};
+    pg_batch_config c = cfg ? *cfg : pg_batch_config{0, 0, 0, 0, 1, 0, 0, 0};
     if (c.stack && (int)c.stack < b->max_depth) return b->fail(PG_ERR_INVALID, "stack shallower than the deepest BVH");
     if (c.stack > 64) return b->fail(PG_ERR_INVALID, "stack above 64 entries");
     if (c.trace_blocks_per_cu > 32 || c.shade_blocks_per_cu > 32) return b->fail(PG_ERR_INVALID, "blocks per CU");
+    if (c.samples_per_pass > 64) return b->fail(PG_ERR_INVALID, "samples_per_pass above 64");
     b->cfg = c;
     b->dirty = true;
     return PG_OK;
@@ -583,6 +594,7 @@ pg_status pg_batch_autotune(pg_batch* b, const pg_render_params* params, pg_batc
         {0u, 2u, 4u, 8u},                                                    // trace blocks per CU
         {4u, 8u, 16u},                                                       // shade blocks per CU
         {(uint32_t)base_stack, (uint32_t)std::min(64, base_stack + 16)},    // stack
+        {1u, 2u, 4u, 8u},                                                    // samples per pass
     };
     auto get = [](pg_batch_config& c, int k) -> uint32_t& {
         switch (k) {
@@ -591,20 +603,36 @@ pg_status pg_batch_autotune(pg_batch* b, const pg_render_params* params, pg_batc
         case 2: return c.queue_paths;
         case 3: return c.trace_blocks_per_cu;
         case 4: return c.shade_blocks_per_cu;
-        default: return c.stack;
+        case 5: return c.stack;
+        default: return c.samples_per_pass;
         }
     };
     auto measure = [&](const pg_batch_config& c, double& ms) -> pg_status {
         pg_status st = pg_batch_configure(b, &c);
         if (st != PG_OK) return st;
         if ((st = clear(b)) != PG_OK) return st;
-        if ((st = render(b, prm, 1)) != PG_OK) return st;  // warm-up (and graph capture)
+        const uint32_t k = c.samples_per_pass ? c.samples_per_pass : 1u;  // whole passes
+        if ((st = render(b, prm, k)) != PG_OK) return st;  // warm-up (and graph capture)
         double before = b->gpu_ms;
-        if ((st = render(b, prm, 2)) != PG_OK) return st;
-        ms = (b->gpu_ms - before) / 2.0;
+        if ((st = render(b, prm, 2 * k)) != PG_OK) return st;
+        ms = (b->gpu_ms - before) / (2.0 * k);
         return PG_OK;
     };
-    pg_batch_config best{0, 0, 8, (uint32_t)base_stack, 1, 0, {0, 0}};
+    // paths of the largest chunk under c's queue size (the greedy packing of prepare)
+    auto chunk_paths = [&](const pg_batch_config& c) -> uint64_t {
+        const uint64_t q = c.queue_paths ? c.queue_paths : ~0ull;
+        uint64_t n = 0, big = 0;
+        for (const BatchView& v : b->views) {
+            const uint64_t px = (uint64_t)v.width * v.height;
+            if (n > 0 && n + px > q) {
+                big = std::max(big, n);
+                n = 0;
+            }
+            n += px;
+        }
+        return std::max(big, n);
+    };
+    pg_batch_config best{0, 0, 8, (uint32_t)base_stack, 1, 0, 1, 0};
     double best_ms = 0.0;
     pg_status st = measure(best, best_ms);
     if (st != PG_OK) return st;
@@ -613,6 +641,8 @@ pg_status pg_batch_autotune(pg_batch* b, const pg_render_params* params, pg_batc
             pg_batch_config c = best;
             if (get(c, k) == val) continue;
             get(c, k) = val;
+            // more samples per pass than the path-state limit allows: not a candidate
+            if (k == 6 && val > 1 && chunk_paths(c) * val > (1ull << 26)) continue;
             double ms = 0.0;
             if ((st = measure(c, ms)) != PG_OK) return st;
             if (ms < best_ms) {

@@ -103,14 +103,15 @@ hipError_t launch_shadow(const DevScene& s, const PathState& ps, int stack, bool
                          hipStream_t stream, bool multi = false, const uint32_t* order = nullptr, bool dyn = false);
 hipError_t launch_ray_keys(const PathState& ps, float3 lo, float inv_ext, int parity, int shadow, uint32_t capacity,
                            uint32_t* keys, uint32_t* vals, hipStream_t stream);
-// Batches: ray generation for the chunk of global paths [base, base + n) (sample index
-// read from *sample), film splat of the chunk's views into film[global pixel], and
-// the per-SPP sample advance, all graph-capturable (no host values change per SPP).
-hipError_t launch_raygen_batch(const PathState& ps, uint32_t n_views, uint32_t base, uint32_t n, uint32_t seed,
-                               const uint32_t* sample, hipStream_t stream);
-hipError_t launch_resolve_batch(const PathState& ps, uint32_t n_views, uint32_t base, uint32_t n, float4* film,
-                                hipStream_t stream);
-hipError_t launch_sample_advance(uint32_t* sample, hipStream_t stream);
+// Batches: ray generation for the chunk of global paths [base, base + n), ns samples
+// per pixel (first sample index read from *sample), film splat of the chunk's views
+// into film[global pixel], and the per-pass sample advance by ns, all graph-capturable
+// (no host values change between passes).
+hipError_t launch_raygen_batch(const PathState& ps, uint32_t n_views, uint32_t base, uint32_t n, uint32_t ns,
+                               uint32_t seed, const uint32_t* sample, hipStream_t stream);
+hipError_t launch_resolve_batch(const PathState& ps, uint32_t n_views, uint32_t base, uint32_t n, uint32_t ns,
+                                float4* film, hipStream_t stream);
+hipError_t launch_sample_advance(uint32_t* sample, uint32_t step, hipStream_t stream);
 hipError_t launch_resolve(const PathState& ps, const PassParams& p, float4* film, hipStream_t stream);
 hipError_t launch_clear_film(float4* film, uint32_t n, hipStream_t stream);
 hipError_t launch_copy(const float4* src, float4* dst, size_t n, int cu_count, hipStream_t stream);

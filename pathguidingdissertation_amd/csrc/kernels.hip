@@ -1168,25 +1168,26 @@ __device__ __forceinline__ uint32_t view_of(const BatchView* views, uint32_t n_v
     return lo;
 }
 
-// k_raygen for the chunk [base, base + n) of the batch's global paths; the RNG key of
-// a path is (seed, sample, pixel within its view), so each view's samples are the
-// ones a single-view render of it draws.
-__global__ void k_raygen_batch(PathState ps, uint32_t n_views, uint32_t base, uint32_t n, uint32_t seed,
-                               const uint32_t* sample) {
+// k_raygen for the chunk [base, base + n) of the batch's global paths, ns samples per
+// pixel: path i is sample (*sample + i / n) of chunk path i % n. The RNG key of a path
+// is (seed, sample, pixel within its view), so each view's samples are the ones a
+// single-view render of it draws.
+__global__ void k_raygen_batch(PathState ps, uint32_t n_views, uint32_t base, uint32_t n, uint32_t ns,
+                               uint32_t seed, const uint32_t* sample) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i == 0) {
-        ps.counters[0] = n;
+        ps.counters[0] = n * ns;
         ps.counters[1] = 0;
         ps.counters[2] = 0;
         ps.counters[3] = 0;
     }
-    if (i >= n) return;
-    const uint32_t g = base + i;
+    if (i >= n * ns) return;
+    const uint32_t g = base + i % n, smp = i / n;
     const uint32_t v = view_of(ps.bviews, n_views, g);
     const BatchView& bv = ps.bviews[v];
     const uint32_t pix = g - bv.pixel_offset;
     uint32_t x = pix % bv.width, y = pix / bv.width;
-    uint32_t key = path_key(seed, *sample, pix);
+    uint32_t key = path_key(seed, *sample + smp, pix);
     float px = (float)x + keyed_rand(key, 0);
     float py = (float)y + keyed_rand(key, 1);
     F3 o, d;
@@ -1202,8 +1203,10 @@ __global__ void k_raygen_batch(PathState ps, uint32_t n_views, uint32_t base, ui
 }
 
 // k_resolve for the chunk's views (chunks hold whole views, so every source path of a
-// pixel lies in the chunk); film is indexed by global pixel.
-__global__ void k_resolve_batch(PathState ps, uint32_t n_views, uint32_t base, uint32_t n, float4* film) {
+// pixel lies in the chunk); film is indexed by global pixel. The ns samples of a pass
+// are added in sample order, as ns one-sample passes would add them.
+__global__ void k_resolve_batch(PathState ps, uint32_t n_views, uint32_t base, uint32_t n, uint32_t ns,
+                                float4* film) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const uint32_t g = base + i;
@@ -1212,11 +1215,12 @@ __global__ void k_resolve_batch(PathState ps, uint32_t n_views, uint32_t base, u
     const uint32_t first = bv.pixel_offset - base;  // chunk-local index of the view's pixel 0
     int X = (int)(pix % bv.width), Y = (int)(pix / bv.width);
     float4 acc = film[g];
+    for (uint32_t smp = 0; smp < ns; ++smp)
     for (int dy = -1; dy <= 0; ++dy)
         for (int dx = -1; dx <= 0; ++dx) {
             int sx = X + dx, sy = Y + dy;
             if (sx < 0 || sy < 0) continue;
-            uint32_t q = first + (uint32_t)sy * bv.width + (uint32_t)sx;
+            uint32_t q = smp * n + first + (uint32_t)sy * bv.width + (uint32_t)sx;
             float2 fp = ps.film_pos[q];
             int tx = (int)floorf(fp.x + 0.5f), ty = (int)floorf(fp.y + 0.5f);
             if (tx != X || ty != Y) continue;
@@ -1230,23 +1234,24 @@ __global__ void k_resolve_batch(PathState ps, uint32_t n_views, uint32_t base, u
     film[g] = acc;
 }
 
-__global__ void k_sample_advance(uint32_t* sample) {
-    if (threadIdx.x == 0) sample[0] += 1u;
+__global__ void k_sample_advance(uint32_t* sample, uint32_t step) {
+    if (threadIdx.x == 0) sample[0] += step;
 }
 
-hipError_t launch_raygen_batch(const PathState& ps, uint32_t n_views, uint32_t base, uint32_t n, uint32_t seed,
-                               const uint32_t* sample, hipStream_t stream) {
-    hipLaunchKernelGGL(k_raygen_batch, dim3((n + 255) / 256), dim3(256), 0, stream, ps, n_views, base, n, seed,
-                       sample);
+hipError_t launch_raygen_batch(const PathState& ps, uint32_t n_views, uint32_t base, uint32_t n, uint32_t ns,
+                               uint32_t seed, const uint32_t* sample, hipStream_t stream) {
+    const uint32_t paths = n * ns;
+    hipLaunchKernelGGL(k_raygen_batch, dim3((paths + 255) / 256), dim3(256), 0, stream, ps, n_views, base, n, ns,
+                       seed, sample);
     return hipGetLastError();
 }
-hipError_t launch_resolve_batch(const PathState& ps, uint32_t n_views, uint32_t base, uint32_t n, float4* film,
-                                hipStream_t stream) {
-    hipLaunchKernelGGL(k_resolve_batch, dim3((n + 255) / 256), dim3(256), 0, stream, ps, n_views, base, n, film);
+hipError_t launch_resolve_batch(const PathState& ps, uint32_t n_views, uint32_t base, uint32_t n, uint32_t ns,
+                                float4* film, hipStream_t stream) {
+    hipLaunchKernelGGL(k_resolve_batch, dim3((n + 255) / 256), dim3(256), 0, stream, ps, n_views, base, n, ns, film);
     return hipGetLastError();
 }
-hipError_t launch_sample_advance(uint32_t* sample, hipStream_t stream) {
-    hipLaunchKernelGGL(k_sample_advance, dim3(1), dim3(64), 0, stream, sample);
+hipError_t launch_sample_advance(uint32_t* sample, uint32_t step, hipStream_t stream) {
+    hipLaunchKernelGGL(k_sample_advance, dim3(1), dim3(64), 0, stream, sample, step);
     return hipGetLastError();
 }
 

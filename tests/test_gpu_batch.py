@@ -103,6 +103,41 @@ def test_chunks_and_autotune_keep_images(pg, device, cornell, small_interior, vi
         assert np.array_equal(a, b)
 
 
+def test_samples_per_pass_keep_batch_views(pg, device, cornell, small_interior, views):
+    """Several samples per pixel traced together in a batch pass give the views of one
+    sample per pass bit for bit: with graph replay and eager passes, with one or several
+    chunks, with a shorter last pass (7 SPP in passes of 3 or 4), across render calls,
+    unguided and with a scene's learned guide."""
+    scenes = [cornell, small_interior]
+    guide = (*_trained_guide(pg, device, small_interior, 12), 12)
+    for integ in (0, 3):
+        def films(**cfg):
+            b = _batch(pg, device, cornell, small_interior, views, **cfg)
+            if integ == 3:
+                b.set_guide(1, guide[0], guide[1], **dict(GUIDE_CFG, seed=guide[2]))
+            b.render(5, seed=9, max_depth=12, integrator=integ)
+            b.render(2, seed=9, max_depth=12, integrator=integ)
+            out = [b.film(v) for v in range(len(views))]
+            b.close()
+            return out
+        ref = films(use_graph=1)
+        for cfg in (dict(use_graph=1, samples_per_pass=3), dict(use_graph=0, samples_per_pass=4),
+                    dict(use_graph=1, samples_per_pass=2, queue_paths=1)):
+            got = films(**cfg)
+            for v in range(len(views)):
+                assert got[v][1] == ref[v][1] == 7
+                assert np.array_equal(got[v][0], ref[v][0]), (integ, cfg, v)
+        assert ref[1][0].sum() > 0
+    # the guided interior view is the single-view learned render
+    assert np.array_equal(ref[1][0], _single_guided(pg, device, scenes[1], views[1][1], guide, 9, 7))
+    b = pg.Batch(device)
+    b.add_scene(cornell)
+    b.add_view(0, _cam(cornell, 8, 8))
+    with pytest.raises(pg.PgError):
+        b.configure(samples_per_pass=65)
+    b.close()
+
+
 def test_batch_errors(pg, device, cornell):
     b = pg.Batch(device)
     with pytest.raises(pg.PgError):
