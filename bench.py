@@ -414,8 +414,8 @@ def main() -> int:
                     help="also time a plain 1 GiB device copy (HBM reference point / counter scaling)")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU only: the path shards by pixels/samples/views with a single "
-              "film reduction (see DESIGN.md); multi-GPU runs are not part of this build.", file=sys.stderr)
+        print("bench.py measures one GPU only: the path would shard by pixels, samples or views with a "
+              "single film reduction; multi-GPU runs are not built.", file=sys.stderr)
         return 2
 
     import pathguidingdissertation_amd as pg

@@ -285,8 +285,9 @@ pg_status pg_mlp_train(pg_ctx* ctx, uint32_t steps, float* losses);
  * data) followed by `steps` Adam steps on the records of the last `window` record
  * passes (older ones are dropped); the first `warmup_passes` record passes train
  * `warmup_steps` steps instead. The other passes are guided path-tracing passes
- * (record = 0). Every pass adds one sample per pixel. params->record_nee = 1 is
- * recommended.                                                                  */
+ * (record = 0); those between two record passes are rendered by one call, so they
+ * share wavefront passes (params->samples_per_pass). Every pass adds one sample per
+ * pixel. params->record_nee = 1 is recommended.                                   */
 typedef struct pg_mlp_schedule {
     uint32_t train_every;    /* 1                                                  */
     uint32_t steps;          /* 1                                                  */

@@ -6,7 +6,7 @@
  * It is used in this container to write golden vectors (tests/golden/) that pin
  * the oracle restatement; it is never part of the product library.
  *
- * Build notes (recorded in DESIGN.md):
+ * Build notes:
  *   * libstdc++ 11 does not declare std::cosf/sinf/tanf/powf/sqrtf, which C++17
  *     <cmath> specifies and MSVC provides; the alias below names the same C
  *     functions, so no arithmetic changes.
