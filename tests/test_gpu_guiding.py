@@ -349,3 +349,35 @@ def test_guided_samples_per_pass_keep_images(pg, device):
             films.append(rt.film()[0])
         assert np.array_equal(films[0], films[1]), (integ, kw)
         assert films[0].sum() > 0
+
+
+def test_record_pass_closed_forms(pg, device):
+    """The GPU record pass on the closed forms that pin the oracle's
+    (test_oracle_record_pass.py): outside a convex diffuse polyhedron under Le = 1 every
+    record carries Li = Le and there is one record per surface hit; inside the closed
+    polyhedron no record is stored and the closest-hit rays per path follow the fixed
+    roulette and the depth-8 cap (6.7731)."""
+    from test_oracle_record_pass import _scene
+
+    def record_pass(data, seed):
+        sc = pg.Scene(device, data)
+        device.clear_records()
+        rt = pg.RayTracer(sc, seed=seed, max_depth=8, integrator=1, record=1, record_nee=0)
+        rt.clear()
+        rt.reset_stats()
+        rt.render(4)
+        return rt.stats(), device.records()
+
+    paths = 64 * 64 * 4
+    out_cam = {"from": [0.0, 0.0, 4.0], "to": [0.0, 0.0, 0.0], "up": [0.0, 1.0, 0.0], "fov": 40.0,
+               "width": 64, "height": 64}
+    st, rec = record_pass(_scene(pg, 1.0, out_cam), 3)
+    hits = st["closest_rays"] - paths
+    assert hits > 1000 and rec.shape[0] == hits, (hits, rec.shape)
+    assert np.all(np.abs(rec[:, 9:12] - 1.0) < 1e-6)
+    in_cam = {"from": [0.0, 0.0, 0.0], "to": [0.0, 0.0, 1.0], "up": [0.0, 1.0, 0.0], "fov": 60.0,
+              "width": 64, "height": 64}
+    st, rec = record_pass(_scene(pg, 2.0, in_cam), 5)
+    assert rec.shape[0] == 0
+    per_path = st["closest_rays"] / paths
+    assert abs(per_path - (5 + 0.7 + 0.7 ** 2 + 0.7 ** 3 + 0.7 ** 4)) < 0.03, per_path
