@@ -1,12 +1,13 @@
 // pgd_render.cpp — example host program over the C++ mirror (pgd.hpp): the reference's
 // render loop (main.cpp:147-199) with the GPU path behind the same class names.
 //
-//   pgd_render <config> <spp> <seed> <out.bin> [--learned]
+//   pgd_render <config> <spp> <seed> <out.bin> [--learned [train_every steps]]
 //                                                 render a procedural scene (unguided, or
 //                                                 with the learned guide trained online:
-//                                                 one guided record pass + one Adam step
-//                                                 per SPP); writes int32 width, height,
-//                                                 spp + film sum
+//                                                 a guided record pass + `steps` Adam steps
+//                                                 every `train_every` SPP, default 1 and 1,
+//                                                 guided passes between); writes int32
+//                                                 width, height, spp + film sum
 //   pgd_render --scene <dir> <spp> <seed> <out.png> [out.hdr]
 //                                                 the reference's main.cpp flow on one of its
 //                                                 scene directories: loadScene, render, savePNG
@@ -70,10 +71,16 @@ static int render_scene_dir(int argc, char** argv) {
 int main(int argc, char** argv) {
     if (argc == 2 && std::strcmp(argv[1], "--selftest") == 0) return selftest();
     if ((argc == 6 || argc == 7) && std::strcmp(argv[1], "--scene") == 0) return render_scene_dir(argc, argv);
-    const bool learned = argc == 6 && std::strcmp(argv[5], "--learned") == 0;
+    const bool learned = (argc == 6 || argc == 8) && std::strcmp(argv[5], "--learned") == 0;
     if (argc != 5 && !learned) {
-        std::fprintf(stderr, "usage: %s <config> <spp> <seed> <out.bin> [--learned] | --selftest\n", argv[0]);
+        std::fprintf(stderr, "usage: %s <config> <spp> <seed> <out.bin> [--learned [train_every steps]] | --selftest\n",
+                     argv[0]);
         return 2;
+    }
+    pg_mlp_schedule schedule{1, 1, 1, 0, 0, {0, 0, 0}};
+    if (argc == 8) {
+        schedule.train_every = (uint32_t)std::atoi(argv[6]);
+        schedule.steps = (uint32_t)std::atoi(argv[7]);
     }
     try {
         pgd::SceneData data = pgd::SceneData::generate(std::atoi(argv[1]), 0);
@@ -87,7 +94,7 @@ int main(int argc, char** argv) {
         if (learned) {
             p.record_nee = 1;
             pgd::LearnedGuide guide(dev);
-            guide.render_online(p, (uint32_t)spp);
+            guide.render_online(p, (uint32_t)spp, schedule);
             const pg_mlp_info info = guide.info();
             std::printf("learned guide: %u Adam steps, last loss %g%s\n", info.step, info.last_loss,
                         info.diverged ? " (diverged)" : "");
