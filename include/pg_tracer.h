@@ -179,9 +179,11 @@ pg_status pg_rng_values(pg_ctx* ctx, uint32_t seed, uint32_t sample, const uint3
 pg_render_params pg_render_params_default(void);
 /* RayTracer::clear (Renderer.h:265-267): zero the film and SPP count. */
 pg_status pg_film_clear(pg_ctx* ctx);
-/* RayTracer::render (Renderer.h:648-747) n_spp times: each pass traces one sample
- * per pixel through the wavefront pipeline (ray-gen, extend, shade, shadow) and
- * splats it with the reference's box filter (Imaging.h:186-217).               */
+/* RayTracer::render (Renderer.h:648-747) n_spp times: every sample of every pixel
+ * goes through the wavefront pipeline (ray-gen, extend, shade, shadow) and is
+ * splatted with the reference's box filter (Imaging.h:186-217). Samples are traced
+ * params->samples_per_pass at a time (record passes: one); the film equals n_spp
+ * calls of one sample each bit for bit.                                          */
 pg_status pg_render_spp(pg_ctx* ctx, const pg_render_params* params, uint32_t n_spp);
 /* Film::film / Film::SPP (Imaging.h:178-272): accumulated RGB sums (W*H*3) and SPP. */
 /* Launch-shape tuner for pg_render_spp (no reference counterpart): times 2-SPP
