@@ -65,3 +65,14 @@ def test_algorithmic_bytes_per_ray():
     assert bench.algorithmic_bytes(10, 0, 4) == 32 * 10 + 36
     assert bench.algorithmic_bytes(0, 3, 20) == 3 * 40 + 52
     assert abs(bench.algorithmic_bytes(86.1, 25.51, 20) - 3827.6) < 0.1
+
+
+def test_bench_refuses_more_than_one_gpu():
+    import subprocess
+    import sys
+    from pathlib import Path
+    root = Path(bench.__file__).resolve().parent
+    r = subprocess.run([sys.executable, str(root / "bench.py"), "--gpus", "2"], capture_output=True, text=True,
+                       timeout=120, cwd=root)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr and r.stdout == ""
