@@ -239,6 +239,16 @@ pg_status pg_knn(pg_ctx* ctx, const float* query_xyz, uint32_t nq, uint32_t k, u
  * pdf, and the pdf of wi_eval (the guiding design of Renderer.h:398-414).      */
 pg_status pg_guide_qt_eval(pg_ctx* ctx, const float* pos, const float* u3, const float* wi_eval,
                            uint32_t n, float* wi_out, float* pdf_sample, float* pdf_eval);
+/* The primary-sample-space guide (PG_INTEGRATOR_GUIDED_PSS with knn_k = 0) at n
+ * shading points of one scene material: the records of the point's leaf list
+ * inverted through BSDF::invert (Materials.h:163-171, 287-306, 673-681) into a 4x4
+ * flux histogram; a direction sampled with u3 (bin by flux, point in the bin, the
+ * BSDF's warp) and its solid-angle pdf, and the pdf of wi_eval. nrm is the shading
+ * normal on wo's side. Zeros where the guide does not apply (specular or
+ * non-invertible material, no record flux).                                      */
+pg_status pg_guide_pss_eval(pg_ctx* ctx, uint32_t material, const float* pos, const float* nrm,
+                            const float* wo, const float* u3, const float* wi_eval, uint32_t n,
+                            float* wi_out, float* pdf_sample, float* pdf_eval);
 
 /* ------------------------------------------------------------------ learned guide */
 /* No counterpart in the reference, whose guiding stops at recording PathVertex data

@@ -132,6 +132,7 @@ SIGNATURES = {
     "pg_cache_export": (C.c_int32, [_P, _F, _F, _F]),
     "pg_knn": (C.c_int32, [_P, _F, C.c_uint32, C.c_uint32, _U32, _F]),
     "pg_guide_qt_eval": (C.c_int32, [_P, _F, _F, _F, C.c_uint32, _F, _F, _F]),
+    "pg_guide_pss_eval": (C.c_int32, [_P, C.c_uint32, _F, _F, _F, _F, _F, C.c_uint32, _F, _F, _F]),
     "pg_mlp_config_default": (PgMlpConfig, []),
     "pg_mlp_init": (C.c_int32, [_P, C.POINTER(PgMlpConfig)]),
     "pg_mlp_train": (C.c_int32, [_P, C.c_uint32, _F]),

@@ -1106,6 +1106,40 @@ pg_status pg_guide_qt_eval(pg_ctx* c, const float* pos, const float* u3, const f
     return st;
 }
 
+pg_status pg_guide_pss_eval(pg_ctx* c, uint32_t material, const float* pos, const float* nrm, const float* wo,
+                            const float* u3, const float* wi_eval, uint32_t n, float* wi_out, float* pdf_sample,
+                            float* pdf_eval) {
+    PG_CHECK_CTX(c);
+    if (!c->cache_ready) return c->fail(PG_ERR_STATE, "cache not built");
+    if (material >= c->host.mats.size()) return c->fail(PG_ERR_INVALID, "unknown material");
+    if (n == 0) return PG_OK;
+    if (!pos || !nrm || !wo || !u3 || !wi_eval || !wi_out || !pdf_sample || !pdf_eval)
+        return c->fail(PG_ERR_INVALID, "null array");
+    float *din = nullptr, *dout = nullptr;
+    pg_status st = PG_OK;
+    const size_t v3 = (size_t)n * 3;
+    do {
+        // inputs: pos, nrm, wo, u3, wi_eval; outputs: wi_out, pdf_sample, pdf_eval
+        if ((st = c->hip(dalloc(&din, 5 * v3), "alloc")) != PG_OK) break;
+        if ((st = c->hip(dalloc(&dout, v3 + 2 * (size_t)n), "alloc")) != PG_OK) break;
+        const float* src[5] = {pos, nrm, wo, u3, wi_eval};
+        for (int k = 0; k < 5 && st == PG_OK; ++k)
+            st = c->hip(hipMemcpy(din + k * v3, src[k], v3 * 4, hipMemcpyHostToDevice), "copy");
+        if (st != PG_OK) break;
+        if ((st = c->hip(launch_pss_eval(c->dev, c->gv, material, din, din + v3, din + 2 * v3, din + 3 * v3,
+                                         din + 4 * v3, n, dout, dout + v3, dout + v3 + n, c->stream),
+                         "pss eval")) != PG_OK)
+            break;
+        if ((st = c->hip(hipStreamSynchronize(c->stream), "sync")) != PG_OK) break;
+        if ((st = c->hip(hipMemcpy(wi_out, dout, v3 * 4, hipMemcpyDeviceToHost), "copy")) != PG_OK) break;
+        if ((st = c->hip(hipMemcpy(pdf_sample, dout + v3, (size_t)n * 4, hipMemcpyDeviceToHost), "copy")) != PG_OK) break;
+        st = c->hip(hipMemcpy(pdf_eval, dout + v3 + n, (size_t)n * 4, hipMemcpyDeviceToHost), "copy");
+    } while (0);
+    dfree(din);
+    dfree(dout);
+    return st;
+}
+
 pg_status pg_film_read(pg_ctx* c, float* rgb_sum, uint32_t* spp) {
     PG_CHECK_CTX(c);
     if (!c->has_scene) return c->fail(PG_ERR_STATE, "no scene uploaded");

@@ -113,6 +113,11 @@ hipError_t launch_resolve_batch(const PathState& ps, uint32_t n_views, uint32_t 
                                 float4* film, hipStream_t stream);
 hipError_t launch_sample_advance(uint32_t* sample, uint32_t step, hipStream_t stream);
 hipError_t launch_resolve(const PathState& ps, const PassParams& p, float4* film, hipStream_t stream);
+// The primary-sample-space guide at n explicit points of material `mat` (3 floats per
+// point for pos, nrm, wo, u3, wi_eval and wi_out).
+hipError_t launch_pss_eval(const DevScene& s, const GuideView& g, uint32_t mat, const float* pos, const float* nrm,
+                           const float* wo, const float* u3, const float* wi_eval, uint32_t n, float* wi_out,
+                           float* pdf_sample, float* pdf_eval, hipStream_t stream);
 hipError_t launch_clear_film(float4* film, uint32_t n, hipStream_t stream);
 hipError_t launch_copy(const float4* src, float4* dst, size_t n, int cu_count, hipStream_t stream);
 struct RecordState;

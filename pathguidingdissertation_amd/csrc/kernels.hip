@@ -366,10 +366,11 @@ __device__ __forceinline__ float guide_pdf_pss(const PssHist* h, const MaterialR
     (void)bsdf_eval(m, sd, wi, pb);
     return pb * pss_density(*h, u, v);
 }
-__device__ __forceinline__ F3 guide_sample_pss(const PssHist* h, const MaterialRec& m, const SD& sd, uint32_t key,
-                                               uint32_t& dim) {
-    float r0 = keyed_rand(key, dim++) * h->total;
-    float r1 = keyed_rand(key, dim++), r2 = keyed_rand(key, dim++);
+// A direction from the PSS guide for uniform numbers (u0, r1, r2): u0 picks the bin
+// by flux, (r1, r2) place the point inside it, the BSDF's warp maps it back.
+__device__ __forceinline__ F3 pss_sample_u(const PssHist* h, const MaterialRec& m, const SD& sd, float u0, float r1,
+                                           float r2) {
+    float r0 = u0 * h->total;
     int b = 15;
     float acc = 0.f;
 #pragma unroll
@@ -381,6 +382,47 @@ __device__ __forceinline__ F3 guide_sample_pss(const PssHist* h, const MaterialR
     while (b > 0 && !(h->w[b] > 0.f)) --b;  // r0 rounded past the last bin
     float u = ((float)(b & 3) + r1) * 0.25f, v = ((float)(b >> 2) + r2) * 0.25f;
     return bsdf_warp(m, sd, fminf(u, 0.99999994f), fminf(v, 0.99999994f));
+}
+__device__ __forceinline__ F3 guide_sample_pss(const PssHist* h, const MaterialRec& m, const SD& sd, uint32_t key,
+                                               uint32_t& dim) {
+    float u0 = keyed_rand(key, dim++);
+    float r1 = keyed_rand(key, dim++), r2 = keyed_rand(key, dim++);
+    return pss_sample_u(h, m, sd, u0, r1, r2);
+}
+
+// The PSS guide at explicit shading points (material mat, shading normal facing wo,
+// the approximate neighbour list of each point): the density of wi_eval and a
+// direction sampled with u3 with its density. Points where the guide does not apply
+// (specular or non-invertible material, no record flux) get zeros.
+__global__ void k_pss_eval(DevScene s, GuideView g, uint32_t mat, const float* __restrict__ pos,
+                           const float* __restrict__ nrm, const float* __restrict__ wo,
+                           const float* __restrict__ u3, const float* __restrict__ wi_eval, uint32_t n,
+                           float* __restrict__ wi_out, float* __restrict__ pdf_sample, float* __restrict__ pdf_eval) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const MaterialRec& m = s.mats[mat];
+    SD sd;
+    sd.x = F3{pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]};
+    sd.sn = normalize(F3{nrm[3 * i], nrm[3 * i + 1], nrm[3 * i + 2]});
+    sd.gn = sd.sn;
+    sd.wo = normalize(F3{wo[3 * i], wo[3 * i + 1], wo[3 * i + 2]});
+    sd.fr = frame_from(sd.sn);
+    sd.alb = c3(m.albedo[0], m.albedo[1], m.albedo[2]);
+    uint32_t recs[kKnnMax];
+    nearby_records(g, sd.x, recs);
+    PssHist h;
+    float pe = 0.f, ps = 0.f;
+    F3 w{0.f, 0.f, 0.f};
+    if (!m.specular && pss_invertible(m) && pss_build(g, recs, m, sd, h)) {
+        pe = guide_pdf_pss(&h, m, sd, F3{wi_eval[3 * i], wi_eval[3 * i + 1], wi_eval[3 * i + 2]});
+        w = pss_sample_u(&h, m, sd, u3[3 * i], u3[3 * i + 1], u3[3 * i + 2]);
+        ps = guide_pdf_pss(&h, m, sd, w);
+    }
+    wi_out[3 * i] = w.x;
+    wi_out[3 * i + 1] = w.y;
+    wi_out[3 * i + 2] = w.z;
+    pdf_sample[i] = ps;
+    pdf_eval[i] = pe;
 }
 
 // RayTracer::computeDirect (Renderer.h:269-354) for a non-specular surface: picks a
@@ -1326,6 +1368,13 @@ uint32_t trace_grid(int cu_count, int stack) {
     if (per_cu > 16) per_cu = 16;
     if (per_cu < 1) per_cu = 1;
     return (uint32_t)(cu_count * per_cu);
+}
+hipError_t launch_pss_eval(const DevScene& s, const GuideView& g, uint32_t mat, const float* pos, const float* nrm,
+                           const float* wo, const float* u3, const float* wi_eval, uint32_t n, float* wi_out,
+                           float* pdf_sample, float* pdf_eval, hipStream_t stream) {
+    hipLaunchKernelGGL(k_pss_eval, dim3((n + 127) / 128), dim3(128), 0, stream, s, g, mat, pos, nrm, wo, u3, wi_eval, n,
+                       wi_out, pdf_sample, pdf_eval);
+    return hipGetLastError();
 }
 hipError_t launch_resolve(const PathState& ps, const PassParams& p, float4* film, hipStream_t stream) {
     uint32_t n = p.width * p.height;

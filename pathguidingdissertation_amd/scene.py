@@ -255,6 +255,19 @@ class Device:
         self._chk(self.lib.pg_guide_qt_eval(self.ctx, _fp(pos), _fp(u3), _fp(we), n, _fp(wi), _fp(ps), _fp(pe)))
         return wi, ps, pe
 
+    def guide_pss_eval(self, material: int, pos, nrm, wo, u3, wi_eval):
+        """Primary-sample-space guide at shading points of one material
+        (pg_guide_pss_eval): sampled directions, their pdf, and the pdf of wi_eval."""
+        arrs = [np.ascontiguousarray(a, np.float32).reshape(-1, 3) for a in (pos, nrm, wo, u3, wi_eval)]
+        n = arrs[0].shape[0]
+        if any(a.shape[0] != n for a in arrs):
+            raise ValueError("pos, nrm, wo, u3 and wi_eval need one row per point")
+        wi = np.empty((n, 3), np.float32)
+        ps, pe = np.empty(n, np.float32), np.empty(n, np.float32)
+        self._chk(self.lib.pg_guide_pss_eval(self.ctx, int(material), *(_fp(a) for a in arrs), n, _fp(wi), _fp(ps),
+                                             _fp(pe)))
+        return wi, ps, pe
+
     # ---------------------------------------------------------------- learned guide
     def mlp_config(self, **kw) -> _lib.PgMlpConfig:
         cfg = self.lib.pg_mlp_config_default()

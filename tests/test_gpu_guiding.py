@@ -381,3 +381,50 @@ def test_record_pass_closed_forms(pg, device):
     assert rec.shape[0] == 0
     per_path = st["closest_rays"] / paths
     assert abs(per_path - (5 + 0.7 + 0.7 ** 2 + 0.7 ** 3 + 0.7 ** 4)) < 0.03, per_path
+
+
+def test_pss_guide_density_integrates_to_one_and_matches_its_samples(pg, device, cornell):
+    """pg_guide_pss_eval at one floor point of the Cornell box after a record pass: the
+    guide's solid-angle density p_bsdf(wi) * 16 w_bin / sum(w) integrates to 1 (the BSDF's
+    sampling map carries [0,1]^2 onto the hemisphere with density p_bsdf, so the integral
+    is the histogram's total mass), and directions sampled with uniform u3 fall into any
+    region of the sphere with the probability the density gives it."""
+    sc = pg.Scene(device, cornell)
+    device.clear_records()
+    rec = pg.RayTracer(sc, seed=1, max_depth=8, integrator=1, record=1, record_nee=1)
+    rec.clear()
+    rec.render(4)
+    device.cache_build(cell_size=4096)
+    rng = np.random.default_rng(17)
+    n = 1_000_000
+    x = np.tile(np.array([0.0, 0.0, 0.9], np.float32), (n, 1))
+    nrm = np.tile(np.array([0.0, 1.0, 0.0], np.float32), (n, 1))
+    wo = np.tile(np.array([0.3, 1.0, 0.2], np.float32) / np.float32(np.linalg.norm([0.3, 1.0, 0.2])), (n, 1))
+    v = rng.normal(size=(n, 3))
+    wi_u = (v / np.linalg.norm(v, axis=1, keepdims=True)).astype(np.float32)  # uniform on the sphere
+    u3 = rng.random((n, 3), dtype=np.float32)
+    wi_s, ps, pe = device.guide_pss_eval(0, x, nrm, wo, u3, wi_u)
+    assert pe.max() > 0, "the guide does not apply at the test point"
+    # normalisation (Monte Carlo over the sphere; tolerance 5 standard errors + 1%)
+    f = pe.astype(np.float64) * 4 * np.pi
+    integral = float(f.mean())
+    assert abs(integral - 1.0) < 5 * f.std() / np.sqrt(n) + 0.01, integral
+    # diffuse: no density below the surface
+    assert np.all(pe[wi_u[:, 1] <= 0] == 0)
+    # samples: unit directions above the floor with positive density
+    assert np.allclose(np.linalg.norm(wi_s, axis=1), 1.0, atol=1e-4)
+    assert np.all(wi_s[:, 1] > 0) and np.all(ps > 0)
+    # region probabilities: 4 bands of cos^2(theta) x 4 azimuth sectors about +y
+    def region(w):
+        c2 = np.clip(w[:, 1], 0, 1) ** 2
+        phi = np.arctan2(w[:, 2], w[:, 0]) % (2 * np.pi)
+        return np.minimum((c2 * 4).astype(int), 3) + 4 * np.minimum((phi / (np.pi / 2)).astype(int), 3)
+    up = wi_u[:, 1] > 0
+    ru = region(wi_u)
+    terms = [f * ((ru == b) & up) for b in range(16)]
+    expect = np.array([t.mean() for t in terms])
+    se_expect = np.array([t.std() for t in terms]) / np.sqrt(n)
+    freq = np.bincount(region(wi_s), minlength=16) / n
+    se_freq = np.sqrt(expect * (1 - expect) / n)
+    tol = 5 * np.sqrt(se_expect ** 2 + se_freq ** 2) + 1e-3
+    assert np.all(np.abs(freq - expect) < tol), (freq.round(4), expect.round(4), tol.round(4))
