@@ -112,8 +112,8 @@ typedef struct {
                                share launches); 0 = automatic: the call's SPP
                                split evenly over passes of at most 2^26 paths
                                (about 14 GB of path state, twice that with a
-                               guide). Images are identical for every value.
-                               Record passes trace one sample per pass.          */
+                               guide; record passes at most 2^22 paths).
+                               Images and records are identical for every value. */
     uint32_t reserved[2];
 } pg_render_params;
 
@@ -182,8 +182,8 @@ pg_status pg_film_clear(pg_ctx* ctx);
 /* RayTracer::render (Renderer.h:648-747) n_spp times: every sample of every pixel
  * goes through the wavefront pipeline (ray-gen, extend, shade, shadow) and is
  * splatted with the reference's box filter (Imaging.h:186-217). Samples are traced
- * params->samples_per_pass at a time (record passes: one); the film equals n_spp
- * calls of one sample each bit for bit.                                          */
+ * params->samples_per_pass at a time; the film (and a record pass's records) equal
+ * n_spp calls of one sample each bit for bit.                                    */
 pg_status pg_render_spp(pg_ctx* ctx, const pg_render_params* params, uint32_t n_spp);
 /* Film::film / Film::SPP (Imaging.h:178-272): accumulated RGB sums (W*H*3) and SPP. */
 /* Launch-shape tuner for pg_render_spp (no reference counterpart): times 2-SPP

@@ -684,14 +684,18 @@ pg_status pg_render_spp(pg_ctx* c, const pg_render_params* params, uint32_t n_sp
     uint32_t n = W * H;
     // record passes: the reference's record integrator, optionally guided by the learned guide
     const bool record_pass = integ == PG_INTEGRATOR_RECORD || (integ == PG_INTEGRATOR_GUIDED_MLP && prm.record);
-    // samples traced together per pass (not in record passes): deep bounces hold few
-    // rays whose launches are latency-bound, so several samples share them
+    // samples traced together per pass: deep bounces hold few rays whose launches are
+    // latency-bound, so several samples share them. Record passes keep per-path vertex
+    // stacks (~1.7 KB per path at depth 12), so they take at most 2^22 paths.
     uint32_t spp_pass = 1;
-    if (!record_pass && !prm.sort_rays) {
+    if (!prm.sort_rays) {
         spp_pass = prm.samples_per_pass;
+        const uint64_t max_paths = record_pass ? (1ull << 22) : (1ull << 26);
+        if (record_pass && spp_pass > 1 && (uint64_t)spp_pass * n > max_paths)
+            spp_pass = (uint32_t)std::max<uint64_t>(1, max_paths / n);
         if (spp_pass == 0) {
-            // the call's samples split evenly over passes of at most 2^26 paths
-            const uint32_t cap = (uint32_t)std::max<uint64_t>(1, (1ull << 26) / n);
+            // the call's samples split evenly over passes of at most max_paths paths
+            const uint32_t cap = (uint32_t)std::max<uint64_t>(1, max_paths / n);
             const uint64_t passes = std::max<uint64_t>(1, ((uint64_t)n_spp + cap - 1) / cap);
             spp_pass = (uint32_t)std::max<uint64_t>(1, ((uint64_t)n_spp + passes - 1) / passes);
         }
@@ -700,7 +704,7 @@ pg_status pg_render_spp(pg_ctx* c, const pg_render_params* params, uint32_t n_sp
     const uint32_t np = n * spp_pass;  // paths of one pass
     pg_status st = ensure_paths(c, np);
     if (st != PG_OK) return st;
-    if (record_pass && (st = ensure_record_state(c, n, prm.max_depth + 1)) != PG_OK) return st;
+    if (record_pass && (st = ensure_record_state(c, np, prm.max_depth + 1)) != PG_OK) return st;
     c->rs.li_nee = prm.record_nee ? 1 : 0;
     if ((mode == 1 || mode == 3) && c->guide_cells_cap < np) {
         dfree(c->guide_cells);
@@ -814,9 +818,11 @@ pg_status pg_render_spp(pg_ctx* c, const pg_render_params* params, uint32_t n_sp
             c->n_shadow++;
         }
         if (record_pass) {
-            // back-propagate, then append this pass's records in (path, depth) order
-            PG_TRY(c, launch_backprop_count(c->rs, n, c->rec_counts, c->stream), "backprop count");
-            PG_TRY(c, exclusive_scan(c->rec_counts, c->rec_offsets, n, c->scan_tmp, c->d_total, c->stream), "scan");
+            // back-propagate, then append this pass's records in (path, depth) order: path
+            // i is sample i / n of pixel i % n, the order of one-sample passes
+            const uint32_t npp = n * ns;
+            PG_TRY(c, launch_backprop_count(c->rs, npp, c->rec_counts, c->stream), "backprop count");
+            PG_TRY(c, exclusive_scan(c->rec_counts, c->rec_offsets, npp, c->scan_tmp, c->d_total, c->stream), "scan");
             uint32_t total = 0;
             PG_TRY(c, hipMemcpyAsync(&total, c->d_total, 4, hipMemcpyDeviceToHost, c->stream), "read total");
             PG_TRY(c, hipStreamSynchronize(c->stream), "sync");
@@ -828,7 +834,7 @@ pg_status pg_render_spp(pg_ctx* c, const pg_render_params* params, uint32_t n_sp
                 cap = (uint32_t)std::min<uint64_t>(c->rec_capacity, 0xffffffffull);
             }
             TrainAux* aux = (prm.record && c->n_aux == c->n_records) ? c->rec_aux : nullptr;
-            PG_TRY(c, launch_backprop_write(c->rs, n, c->rec_offsets, (uint32_t)base, c->records, aux, cap,
+            PG_TRY(c, launch_backprop_write(c->rs, npp, c->rec_offsets, (uint32_t)base, c->records, aux, cap,
                                             c->ps.radiance, c->stream), "backprop write");
             if (prm.record) {
                 if (aux) c->n_aux += total;

@@ -428,3 +428,24 @@ def test_pss_guide_density_integrates_to_one_and_matches_its_samples(pg, device,
     se_freq = np.sqrt(expect * (1 - expect) / n)
     tol = 5 * np.sqrt(se_expect ** 2 + se_freq ** 2) + 1e-3
     assert np.all(np.abs(freq - expect) < tol), (freq.round(4), expect.round(4), tol.round(4))
+
+
+def test_record_pass_samples_per_pass_keep_records(pg, device, cornell):
+    """A 3-SPP record pass traced as one 3-sample wavefront pass gives the film and the
+    records of three one-sample passes bit for bit (records in (sample, pixel, depth)
+    order), for the record integrator and for learned-guided record passes."""
+    sc = pg.Scene(device, cornell)
+    for integ in (1, 3):
+        got = []
+        for spp_pass in (1, 0):
+            if integ == 3:
+                device.mlp_init(seed=4)
+            device.clear_records()
+            rt = pg.RayTracer(sc, seed=11, max_depth=8, integrator=integ, record=1, record_nee=1,
+                              samples_per_pass=spp_pass)
+            rt.clear()
+            rt.render(3)
+            got.append((rt.film()[0], device.records()))
+        assert got[0][1].shape[0] > 1000
+        assert np.array_equal(got[0][0], got[1][0]), integ
+        assert np.array_equal(got[0][1], got[1][1]), integ
