@@ -38,6 +38,17 @@ def test_null_and_bad_arguments_are_errors(pg):
     assert lib.pg_bvh_build_host(None, 0, None, None, None) == pg._lib.PG_ERR_INVALID
     p = lib.pg_render_params_default()
     assert (p.seed, p.max_depth, p.rr_depth, p.knn_k) == (1, 8, 3, 16)
+    assert p.samples_per_pass == 0  # automatic
+    # a null context is an error, never a crash, for the rendering and guide entry points
+    inv = pg._lib.PG_ERR_INVALID
+    assert lib.pg_render_spp(None, None, 1) == inv
+    assert lib.pg_mlp_render_online(None, None, 1, None) == inv
+    assert lib.pg_guide_qt_eval(None, None, None, None, 0, None, None, None) == inv
+    assert lib.pg_guide_pss_eval(None, 0, None, None, None, None, None, 0, None, None, None) == inv
+    assert lib.pg_film_tonemap(None, 1.0, 10.0, None) == inv
+    assert lib.pg_set_timing(None, 0) == inv
+    assert lib.pg_batch_configure(None, None) == inv
+    assert lib.pg_batch_render(None, None, 1) == inv
 
 
 def _host_bvh(pg, data):
