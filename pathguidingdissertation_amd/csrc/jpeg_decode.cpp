@@ -156,6 +156,7 @@ struct Decoder {
         width = u16();
         int nc = u8();
         if (width <= 0 || height <= 0) { err = "bad JPEG size (DNL not supported)"; return false; }
+        if ((int64_t)width * height > (int64_t(1) << 28)) { err = "JPEG above 2^28 pixels"; return false; }
         if (nc != 1 && nc != 3) { err = "only 1- or 3-component JPEGs are supported"; return false; }
         comps.assign(nc, Comp());
         for (int i = 0; i < nc; ++i) {

@@ -1,7 +1,10 @@
 """Malformed scene files are errors (or, for undecodable textures, warnings), never
 crashes or hangs: every truncation of a valid scene.json, seeded byte corruptions of it,
-truncated .gem meshes and a truncated PNG texture, each loaded by pg_scene_load. The
-loads run in a child process so that a crash fails this test instead of the run."""
+truncated .gem meshes and a truncated PNG texture, each loaded by pg_scene_load; and
+seeded, marker-structured random JPEG streams (quantisation and Huffman tables, frame
+headers with random sizes and sampling factors, scans with random entropy data,
+restart markers, truncations) decoded by pg_image_read. The loads run in a child
+process so that a crash fails the test instead of the run."""
 import subprocess
 import sys
 import textwrap
@@ -61,3 +64,68 @@ def test_malformed_scene_files_are_errors(pg, tmp_path):
     assert r.returncode == 0, (r.returncode, r.stdout[-2000:], r.stderr[-2000:])
     ok, err = (int(v) for v in r.stdout.split()[-2:])
     assert err > 100 and ok > 1  # truncations fail; the intact files (and harmless edits) load
+
+
+JPEG_CHILD = r'''import sys, struct
+from pathlib import Path
+import numpy as np
+sys.path.insert(0, sys.argv[1])
+import pathguidingdissertation_amd as pg
+out = Path(sys.argv[2]); out.mkdir(parents=True, exist_ok=True)
+rng = np.random.default_rng(int(sys.argv[3]))
+N = int(sys.argv[4])
+def seg(marker, payload):
+    return bytes([0xFF, marker]) + struct.pack(">H", len(payload) + 2) + payload
+def rb(k): return bytes(rng.integers(0, 256, k, dtype=np.uint8))
+ok = err = 0
+for it in range(N):
+    parts = [b"\xff\xd8"]
+    for _ in range(int(rng.integers(1, 9))):
+        kind = rng.integers(0, 8)
+        if kind == 0:   # DQT
+            t = int(rng.integers(0, 4)); parts.append(seg(0xDB, bytes([t]) + rb(64)))
+        elif kind == 1:  # SOF0 / SOF2
+            big = rng.random() < 0.05
+            h = int(rng.integers(1, 65536 if big else 300)); w = int(rng.integers(1, 65536 if big else 300))
+            nc = int(rng.choice([1, 3, 2]))
+            comp = b"".join(bytes([i + 1, int(rng.integers(1, 5)) << 4 | int(rng.integers(1, 5)), int(rng.integers(0, 4))]) for i in range(nc))
+            parts.append(seg(int(rng.choice([0xC0, 0xC2])), bytes([8]) + struct.pack(">HH", h, w) + bytes([nc]) + comp))
+        elif kind == 2:  # DHT
+            tc = int(rng.integers(0, 2)) << 4 | int(rng.integers(0, 4))
+            counts = rng.integers(0, 6, 16, dtype=np.uint8)
+            parts.append(seg(0xC4, bytes([tc]) + bytes(counts) + rb(int(counts.sum()))))
+        elif kind == 3:  # SOS + entropy data
+            ns = int(rng.integers(1, 4))
+            body = bytes([ns]) + b"".join(bytes([i + 1, int(rng.integers(0, 256))]) for i in range(ns)) + bytes([int(rng.integers(0, 64)), int(rng.integers(0, 64)), int(rng.integers(0, 256))])
+            parts.append(seg(0xDA, body) + rb(int(rng.integers(0, 400))).replace(b"\xff", b"\xff\x00"))
+        elif kind == 4:  # DRI
+            parts.append(seg(0xDD, struct.pack(">H", int(rng.integers(0, 8)))))
+        elif kind == 5:  # APP14 Adobe / APP0
+            parts.append(seg(int(rng.choice([0xE0, 0xEE])), rb(int(rng.integers(0, 16)))))
+        elif kind == 6:  # raw junk
+            parts.append(rb(int(rng.integers(0, 64))))
+        else:  # restart markers / bad lengths
+            parts.append(bytes([0xFF, int(rng.integers(0xD0, 0xD8))]) + rb(int(rng.integers(0, 8))))
+    if rng.random() < 0.7:
+        parts.append(b"\xff\xd9")
+    data = b"".join(parts)
+    if rng.random() < 0.3:
+        data = data[: int(rng.integers(2, len(data) + 1))]
+    f = out / "t.jpg"
+    f.write_bytes(data)
+    try:
+        img = pg.read_image(f)
+        assert img.ndim == 3 and np.all(np.isfinite(img))
+        ok += 1
+    except pg.PgError:
+        err += 1
+print("jpeg", ok, err, flush=True)
+'''
+
+
+def test_random_jpeg_streams_are_errors_or_images(tmp_path):
+    r = subprocess.run([sys.executable, "-c", JPEG_CHILD, str(ROOT), str(tmp_path), "5", "1500"],
+                       capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, (r.returncode, r.stdout[-2000:], r.stderr[-2000:])
+    ok, err = (int(v) for v in r.stdout.split()[-2:])
+    assert ok + err == 1500 and err > 0
