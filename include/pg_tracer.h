@@ -112,7 +112,10 @@ typedef struct {
                                share launches); 0 = automatic: the call's SPP
                                split evenly over passes of at most 2^26 paths
                                (about 14 GB of path state, twice that with a
-                               guide; record passes at most 2^22 paths).
+                               guide; record passes at most
+                               min(2^22, 2^22 x 13 / (max_depth + 1)) paths:
+                               their vertex stacks take (max_depth + 1) x 128 B
+                               per path, ~7 GB).
                                Images and records are identical for every value. */
     uint32_t reserved[2];
 } pg_render_params;

@@ -10,6 +10,7 @@ import os
 import shutil
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 from pathlib import Path
 
 PKG = Path(__file__).resolve().parent
@@ -43,24 +44,25 @@ def _run(cmd: list[str]) -> None:
 
 
 def build(verbose: bool = False) -> Path:
-    """Compile every source and link ``libpgtrace.so`` (always rebuilds)."""
+    """Compile every source (in parallel) and link ``libpgtrace.so`` (always rebuilds)."""
     BUILD.mkdir(exist_ok=True)
-    objs = []
+    hipcc = _hipcc()
+    cmds, objs = [], []
     for src in HOST_SOURCES:
         obj = BUILD / (src + ".o")
-        cmd = [os.environ.get("CXX", "g++"), *CXXFLAGS, "-c", str(CSRC / src), "-o", str(obj)]
-        if verbose:
-            print(" ".join(cmd))
-        _run(cmd)
+        cmds.append([os.environ.get("CXX", "g++"), *CXXFLAGS, "-c", str(CSRC / src), "-o", str(obj)])
         objs.append(obj)
-    hipcc = _hipcc()
     for src in HIP_SOURCES:
         obj = BUILD / (src + ".o")
-        cmd = [hipcc, *HIPFLAGS, "-c", str(CSRC / src), "-o", str(obj)]
-        if verbose:
-            print(" ".join(cmd))
-        _run(cmd)
+        cmds.append([hipcc, *HIPFLAGS, "-c", str(CSRC / src), "-o", str(obj)])
         objs.append(obj)
+    if verbose:
+        for cmd in cmds:
+            print(" ".join(cmd))
+    jobs = max(1, min(len(cmds), int(os.environ.get("MAX_JOBS", "0")) or (os.cpu_count() or 1), 16))
+    with ThreadPoolExecutor(max_workers=jobs) as pool:
+        for f in [pool.submit(_run, cmd) for cmd in cmds]:
+            f.result()
     tmp = LIB.with_suffix(".so.tmp")
     _run([hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", "-pthread", *map(str, objs), "-lz", "-o", str(tmp)])
     os.replace(tmp, LIB)

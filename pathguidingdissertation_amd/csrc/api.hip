@@ -348,8 +348,17 @@ pg_status pg_scene_upload(pg_ctx* c, const float* positions, const float* normal
     PG_TRY(c, hipStreamSynchronize(c->stream), "sync");
     c->free_scene();
     c->host = PreparedScene();
-    std::string e = prepare_scene(positions, normals, material_index, n_tris, materials, n_materials, env_rgb,
-                                  env_width, env_height, *camera, c->host);
+    std::string e;
+    try {
+        e = prepare_scene(positions, normals, material_index, n_tris, materials, n_materials, env_rgb, env_width,
+                          env_height, *camera, c->host);
+    } catch (const std::bad_alloc&) {
+        c->host = PreparedScene();
+        return c->fail(PG_ERR_NOMEM, "out of host memory preparing the scene");
+    } catch (const std::exception& x) {
+        c->host = PreparedScene();
+        return c->fail(PG_ERR_INVALID, std::string("scene preparation failed: ") + x.what());
+    }
     if (!e.empty()) return c->fail(PG_ERR_INVALID, e);
     if (c->host.bvh.max_depth > 64)
         return c->fail(PG_ERR_OVERFLOW, "BVH deeper than the 64-entry traversal stack (Geometry.h:461)");
@@ -444,7 +453,15 @@ pg_status pg_bvh_build_host(const float* positions, uint32_t n_tris, uint32_t* n
                             int32_t* tri_index_out) {
     if (!positions || n_tris == 0) { g_last_error = "empty triangle set"; return PG_ERR_INVALID; }
     HostBvh b;
-    build_bvh(positions, n_tris, b);
+    try {
+        build_bvh(positions, n_tris, b);
+    } catch (const std::bad_alloc&) {
+        g_last_error = "out of host memory building the BVH";
+        return PG_ERR_NOMEM;
+    } catch (const std::exception& x) {
+        g_last_error = std::string("BVH build failed: ") + x.what();
+        return PG_ERR_INVALID;
+    }
     if (n_nodes) *n_nodes = (uint32_t)b.nodes.size();
     if (nodes_out) std::memcpy(nodes_out, b.nodes.data(), b.nodes.size() * sizeof(BvhNode));
     if (tri_index_out) std::memcpy(tri_index_out, b.tri_index.data(), b.tri_index.size() * 4);
@@ -593,6 +610,7 @@ static pg_status ensure_paths(pg_ctx* c, uint32_t n) {
 static pg_status ensure_scan(pg_ctx* c, uint32_t n) {
     uint32_t tiles = scan_tiles(n) + 1;
     if (c->scan_cap >= tiles) return PG_OK;
+    c->scan_cap = 0;
     dfree(c->scan_tmp);
     PG_TRY(c, dalloc(&c->scan_tmp, tiles), "alloc scan");
     if (!c->d_total) PG_TRY(c, dalloc(&c->d_total, 1), "alloc scan total");
@@ -600,8 +618,16 @@ static pg_status ensure_scan(pg_ctx* c, uint32_t n) {
     return PG_OK;
 }
 
+// vertices of the per-path stacks a record pass may hold (128 B each): 2^22 paths of
+// 13 vertices (max_depth 12), ~7 GB
+static constexpr uint64_t kRecordVertexBudget = (1ull << 22) * 13;
+
 static pg_status ensure_record_state(pg_ctx* c, uint32_t n_paths, uint32_t depth) {
     if (c->rs_paths >= n_paths && c->rs_depth >= depth) return PG_OK;
+    // the sizes stay zero until every buffer is allocated, so a failed allocation
+    // makes the next call allocate again instead of writing through null pointers
+    c->rs_paths = 0;
+    c->rs_depth = 0;
     dfree(c->rs.vpos); dfree(c->rs.vnrm); dfree(c->rs.vwi); dfree(c->rs.vwo); dfree(c->rs.vw); dfree(c->rs.ve);
     dfree(c->rs.vme); dfree(c->rs.vd); dfree(c->rs.nv);
     dfree(c->rec_counts); dfree(c->rec_offsets);
@@ -686,11 +712,14 @@ pg_status pg_render_spp(pg_ctx* c, const pg_render_params* params, uint32_t n_sp
     const bool record_pass = integ == PG_INTEGRATOR_RECORD || (integ == PG_INTEGRATOR_GUIDED_MLP && prm.record);
     // samples traced together per pass: deep bounces hold few rays whose launches are
     // latency-bound, so several samples share them. Record passes keep per-path vertex
-    // stacks (~1.7 KB per path at depth 12), so they take at most 2^22 paths.
+    // stacks of (max_depth + 1) x 128 B, so a record pass takes at most
+    // kRecordVertexBudget / (max_depth + 1) paths (2^22 at depth 12, ~7 GB).
     uint32_t spp_pass = 1;
     if (!prm.sort_rays) {
         spp_pass = prm.samples_per_pass;
-        const uint64_t max_paths = record_pass ? (1ull << 22) : (1ull << 26);
+        const uint64_t record_paths =
+            std::max<uint64_t>(1, std::min<uint64_t>(1ull << 22, kRecordVertexBudget / (prm.max_depth + 1)));
+        const uint64_t max_paths = record_pass ? record_paths : (1ull << 26);
         if (record_pass && spp_pass > 1 && (uint64_t)spp_pass * n > max_paths)
             spp_pass = (uint32_t)std::max<uint64_t>(1, max_paths / n);
         if (spp_pass == 0) {
@@ -707,12 +736,14 @@ pg_status pg_render_spp(pg_ctx* c, const pg_render_params* params, uint32_t n_sp
     if (record_pass && (st = ensure_record_state(c, np, prm.max_depth + 1)) != PG_OK) return st;
     c->rs.li_nee = prm.record_nee ? 1 : 0;
     if ((mode == 1 || mode == 3) && c->guide_cells_cap < np) {
+        c->guide_cells_cap = 0;
         dfree(c->guide_cells);
         PG_TRY(c, dalloc(&c->guide_cells, (size_t)np * kKnnMax), "alloc guide cells");
         c->guide_cells_cap = np;
     }
     // sort buffers: kNN query order (quadtree guide) and ray trace order (sort_rays)
     if ((mode == 1 || (mode == 3 && prm.knn_k != 0) || prm.sort_rays) && c->q_cap < np) {
+        c->q_cap = 0;
         dfree(c->q_keys); dfree(c->q_vals); dfree(c->q_keys_tmp); dfree(c->q_vals_tmp); dfree(c->q_hist); dfree(c->q_scan);
         PG_TRY(c, dalloc(&c->q_keys, np), "alloc sort buffers");
         PG_TRY(c, dalloc(&c->q_vals, np), "alloc sort buffers");
@@ -735,6 +766,7 @@ pg_status pg_render_spp(pg_ctx* c, const pg_render_params* params, uint32_t n_sp
     const bool sort_rays = prm.sort_rays != 0 && !record_pass;
     const bool dyn = prm.dyn_fetch != 0 && !record_pass && !sort_rays;
     if (mode == 2 && c->mix_cap < np) {
+        c->mix_cap = 0;
         dfree(c->mix);
         PG_TRY(c, dalloc(&c->mix, (size_t)np * kMixF4), "alloc guide mixtures");
         c->mix_cap = np;
@@ -1025,6 +1057,7 @@ pg_status pg_cache_build(pg_ctx* c, uint32_t cell_size, uint32_t qt_nodes_per_ce
     // neighbour lists of the leaves (approximate neighbours of the PSS guide)
     const uint32_t n_leaves = (n + kLeafRecords - 1) / kLeafRecords;
     if (n_leaves > c->leaf_nbrs_cap) {
+        c->leaf_nbrs_cap = 0;
         dfree(c->leaf_nbrs);
         PG_TRY(c, dalloc(&c->leaf_nbrs, (size_t)n_leaves * kKnnMax), "alloc leaf neighbours");
         c->leaf_nbrs_cap = n_leaves;
@@ -1305,6 +1338,7 @@ static pg_status mlp_buffers(pg_ctx* c, uint32_t batch, uint32_t steps) {
     const uint32_t chunk = mlp_chunk(m.cfg, batch);
     const uint32_t groups = mlp_wgrad_groups(chunk);
     if (m.batch_cap < batch) {
+        m.batch_cap = m.loss_cap = 0;
         dfree(m.batch); dfree(m.loss); dfree(m.batch_raw);
         dfree(m.sort.keys); dfree(m.sort.vals); dfree(m.sort.keys_tmp); dfree(m.sort.vals_tmp);
         dfree(m.sort.hist); dfree(m.sort.scan);
@@ -1320,16 +1354,19 @@ static pg_status mlp_buffers(pg_ctx* c, uint32_t batch, uint32_t steps) {
         m.batch_cap = m.loss_cap = batch;
     }
     if (m.rows_cap < chunk) {
+        m.rows_cap = 0;
         dfree(m.rows);
         PG_TRY(c, dalloc(&m.rows, (size_t)chunk * kRowLen), "alloc activation rows");
         m.rows_cap = chunk;
     }
     if (m.groups_cap < groups) {
+        m.groups_cap = 0;
         dfree(m.partial);
         PG_TRY(c, dalloc(&m.partial, (size_t)groups * kParams), "alloc gradient partials");
         m.groups_cap = groups;
     }
     if (m.scal_cap < steps + 1) {
+        m.scal_cap = 0;
         dfree(m.scal);
         PG_TRY(c, dalloc(&m.scal, steps + 1), "alloc loss history");
         m.scal_cap = steps + 1;

@@ -195,6 +195,7 @@ pg_status prepare(pg_batch* b) {
             "view table");
     // film (global pixel index), kept across reconfigurations of the same views
     if (b->film_cap < b->total) {
+        b->film_cap = 0;
         bfree(b->film);
         PG_BTRY(b, balloc(&b->film, b->total), "alloc batch film");
         b->film_cap = b->total;
@@ -330,6 +331,7 @@ pg_status render(pg_batch* b, const pg_render_params& prm, uint32_t n_spp) {
     if (prm.integrator == PG_INTEGRATOR_GUIDED_MLP && b->mix_cap < b->ps_cap) {
         b->drop_graph();
         PG_BTRY(b, hipStreamSynchronize(s), "sync");
+        b->mix_cap = 0;
         bfree(b->mix);
         PG_BTRY(b, balloc(&b->mix, (size_t)b->ps_cap * kMixF4), "alloc guide mixtures");
         b->mix_cap = b->ps_cap;

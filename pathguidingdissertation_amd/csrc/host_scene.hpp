@@ -26,6 +26,8 @@ struct pg_host_scene {
 };
 
 namespace pg {
+// Largest image the decoders accept (PNG, Radiance HDR, JPEG): 2^28 pixels.
+constexpr int64_t kMaxImagePixels = int64_t(1) << 28;
 // Decoded image, `channels` values per pixel as the file stores them (stb_image's
 // req_comp = 0 convention): 8-bit PNG samples in u8, Radiance HDR floats in f32.
 struct Image {

@@ -146,6 +146,17 @@ struct Decoder {
     }
     static int extend(int v, int s) { return s == 0 ? 0 : (v < (1 << (s - 1)) ? v - (1 << s) + 1 : v); }
     int receive_extend(int s) { return extend(bits(s), s); }
+    // DC difference of a category from the stream: 8-bit samples allow categories
+    // 0..11 (T.81 F.1.2.1.1); anything larger is a corrupt table
+    int receive_dc(int t) {
+        if (t > 11) {
+            if (err.empty()) err = "bad DC category";
+            return 0;
+        }
+        return receive_extend(t);
+    }
+    // the DC predictor stays inside the range a coefficient can hold
+    static int clamp_dc(int64_t v) { return (int)std::min<int64_t>(std::max<int64_t>(v, -65536), 65535); }
 
     bool read_sof(int marker) {
         int len = u16();
@@ -156,8 +167,10 @@ struct Decoder {
         width = u16();
         int nc = u8();
         if (width <= 0 || height <= 0) { err = "bad JPEG size (DNL not supported)"; return false; }
-        if ((int64_t)width * height > (int64_t(1) << 28)) { err = "JPEG above 2^28 pixels"; return false; }
+        if ((int64_t)width * height > kMaxImagePixels) { err = "JPEG above 2^28 pixels"; return false; }
         if (nc != 1 && nc != 3) { err = "only 1- or 3-component JPEGs are supported"; return false; }
+        if (!comps.empty()) { err = "more than one frame header"; return false; }
+        hmax = vmax = 1;
         comps.assign(nc, Comp());
         for (int i = 0; i < nc; ++i) {
             comps[i].id = u8();
@@ -172,6 +185,12 @@ struct Decoder {
             hmax = std::max(hmax, comps[i].h);
             vmax = std::max(vmax, comps[i].v);
         }
+        // upsampling replicates each sample hmax / h times: factors must divide the maxima
+        for (const Comp& c : comps)
+            if (hmax % c.h != 0 || vmax % c.v != 0) {
+                err = "sampling factors that do not divide the largest ones are not supported";
+                return false;
+            }
         progressive = marker == 0xC2;
         mcux = (width + 8 * hmax - 1) / (8 * hmax);
         mcuy = (height + 8 * vmax - 1) / (8 * vmax);
@@ -219,8 +238,8 @@ struct Decoder {
     // one block of a sequential scan (T.81 F.2.2)
     void decode_seq(Comp& c, int16_t* b) {
         int t = decode(hdc[c.td]);
-        int diff = receive_extend(t);
-        c.dc_pred += diff;
+        int diff = receive_dc(t);
+        c.dc_pred = clamp_dc((int64_t)c.dc_pred + diff);
         b[0] = (int16_t)c.dc_pred;
         for (int k = 1; k < 64;) {
             int rs = decode(hac[c.ta]);
@@ -240,8 +259,8 @@ struct Decoder {
     void decode_dc_prog(Comp& c, int16_t* b, int ah, int al) {
         if (ah == 0) {
             int t = decode(hdc[c.td]);
-            c.dc_pred += receive_extend(t);
-            b[0] = (int16_t)(c.dc_pred * (1 << al));
+            c.dc_pred = clamp_dc((int64_t)c.dc_pred + receive_dc(t));
+            b[0] = (int16_t)((int64_t)c.dc_pred * (int64_t(1) << al));
         } else if (bit()) {
             b[0] = (int16_t)(b[0] | (1 << al));
         }

@@ -332,6 +332,7 @@ std::string read_png(const std::string& path, Image& out) {
         pos += 12 + (size_t)len;
     }
     if (w == 0 || h == 0 || w > (1u << 24) || h > (1u << 24)) return "bad image size";
+    if ((int64_t)w * h > kMaxImagePixels) return "PNG image above 2^28 pixels";
     if (depth != 8) return "only 8-bit PNG samples are supported";
     if (interlace) return "interlaced PNG is not supported";
     int comps;
@@ -455,6 +456,7 @@ std::string read_hdr(const std::string& path, Image& out) {
     line(l);
     int H = 0, W = 0;
     if (std::sscanf(l.c_str(), "-Y %d +X %d", &H, &W) != 2 || H <= 0 || W <= 0) return "unsupported HDR orientation";
+    if (W > (1 << 24) || H > (1 << 24) || (int64_t)W * H > kMaxImagePixels) return "HDR image above 2^28 pixels";
     out.w = W; out.h = H; out.channels = 3;
     out.f32.assign((size_t)W * H * 3, 0.f);
     const uint8_t* d = (const uint8_t*)buf.data();
@@ -659,9 +661,27 @@ std::string load_scene(const std::string& dir, uint32_t flags, pg_host_scene* s)
 }  // namespace
 }  // namespace pg
 
+static std::string g_load_error;
+
+static pg_status write_png(const char* path, const uint8_t* rgb, int32_t width, int32_t height);
+static pg_status write_hdr(const char* path, const float* rgb, int32_t width, int32_t height);
+
+// host allocation failures of the writers become status codes, never exceptions
+template <class F>
+static pg_status guarded(F&& f) {
+    try {
+        return f();
+    } catch (const std::bad_alloc&) {
+        g_load_error = "out of host memory";
+        return PG_ERR_NOMEM;
+    } catch (const std::exception& x) {
+        g_load_error = x.what();
+        return PG_ERR_INVALID;
+    }
+}
+
 extern "C" {
 
-static std::string g_load_error;
 
 pg_status pg_scene_load(const char* scene_dir, uint32_t flags, pg_host_scene** out) {
     if (!scene_dir || !out) return PG_ERR_INVALID;
@@ -674,6 +694,10 @@ pg_status pg_scene_load(const char* scene_dir, uint32_t flags, pg_host_scene** o
         delete s;
         g_load_error = "out of host memory";
         return PG_ERR_NOMEM;
+    } catch (const std::exception& x) {
+        delete s;
+        g_load_error = std::string("scene load failed: ") + x.what();
+        return PG_ERR_INVALID;
     }
     if (!e.empty()) {
         delete s;
@@ -727,7 +751,16 @@ pg_status pg_image_read(const char* path, int32_t* width, int32_t* height, int32
     pg::Image img;
     std::string p(path);
     bool hdr = false;
-    std::string e = pg::read_image_file(p, img, hdr);
+    std::string e;
+    try {
+        e = pg::read_image_file(p, img, hdr);
+    } catch (const std::bad_alloc&) {
+        g_load_error = p + ": out of host memory";
+        return PG_ERR_NOMEM;
+    } catch (const std::exception& x) {
+        g_load_error = p + ": " + x.what();
+        return PG_ERR_INVALID;
+    }
     if (!e.empty()) { g_load_error = p + ": " + e; return PG_ERR_INVALID; }
     *width = img.w; *height = img.h;
     if (channels) *channels = img.channels;
@@ -740,6 +773,16 @@ pg_status pg_image_read(const char* path, int32_t* width, int32_t* height, int32
 
 // RayTracer::savePNG (Renderer.h:751): 8-bit RGB, one zlib stream of filter-0 rows.
 pg_status pg_image_write_png(const char* path, const uint8_t* rgb, int32_t width, int32_t height) {
+    return guarded([&] { return write_png(path, rgb, width, height); });
+}
+
+pg_status pg_image_write_hdr(const char* path, const float* rgb, int32_t width, int32_t height) {
+    return guarded([&] { return write_hdr(path, rgb, width, height); });
+}
+
+}  // extern "C"
+
+static pg_status write_png(const char* path, const uint8_t* rgb, int32_t width, int32_t height) {
     if (!path || !rgb || width < 1 || height < 1) return PG_ERR_INVALID;
     size_t stride = (size_t)width * 3;
     std::vector<uint8_t> raw((stride + 1) * height);
@@ -776,7 +819,7 @@ pg_status pg_image_write_png(const char* path, const uint8_t* rgb, int32_t width
 // Film::save (Imaging.h:263-272) writes the mean image as Radiance RGBE: the shared
 // exponent of the largest component (frexp), mantissas scaled by 256 / max and
 // truncated; scanlines run-length encoded per component when 8 <= width < 32768.
-pg_status pg_image_write_hdr(const char* path, const float* rgb, int32_t width, int32_t height) {
+static pg_status write_hdr(const char* path, const float* rgb, int32_t width, int32_t height) {
     if (!path || !rgb || width < 1 || height < 1) return PG_ERR_INVALID;
     std::ofstream f(path, std::ios::binary);
     if (!f) return PG_ERR_INVALID;
@@ -815,5 +858,3 @@ pg_status pg_image_write_hdr(const char* path, const float* rgb, int32_t width, 
     }
     return f ? PG_OK : PG_ERR_INVALID;
 }
-
-}  // extern "C"

@@ -129,3 +129,58 @@ def test_random_jpeg_streams_are_errors_or_images(tmp_path):
     assert r.returncode == 0, (r.returncode, r.stdout[-2000:], r.stderr[-2000:])
     ok, err = (int(v) for v in r.stdout.split()[-2:])
     assert ok + err == 1500 and err > 0
+
+
+CRAFTED_CHILD = r'''import sys, struct, zlib
+from pathlib import Path
+sys.path.insert(0, sys.argv[1])
+import pathguidingdissertation_amd as pg
+out = Path(sys.argv[2]); out.mkdir(parents=True, exist_ok=True)
+def seg(marker, payload):
+    return bytes([0xFF, marker]) + struct.pack(">H", len(payload) + 2) + payload
+def chunk(t, body):
+    return struct.pack(">I", len(body)) + t + body + struct.pack(">I", zlib.crc32(t + body))
+cases = {}
+# Radiance header announcing 2e9 x 2e9 pixels
+cases["huge.hdr"] = b"#?RADIANCE\nFORMAT=32-bit_rle_rgbe\n\n-Y 2000000000 +X 2000000000\n" + bytes(16)
+# PNG whose IHDR announces 2^24 x 2^24 pixels
+ihdr = struct.pack(">IIBBBBB", 1 << 24, 1 << 24, 8, 2, 0, 0, 0)
+cases["huge.png"] = b"\x89PNG\r\n\x1a\n" + chunk(b"IHDR", ihdr) + chunk(b"IDAT", zlib.compress(b"\0" * 64)) + chunk(b"IEND", b"")
+# JPEG whose DC table codes category 200 (8-bit samples allow 0..11)
+dqt = seg(0xDB, bytes([0]) + bytes([1] * 64))
+sof = seg(0xC0, bytes([8]) + struct.pack(">HH", 8, 8) + bytes([1, 1, 0x11, 0]))
+dc = seg(0xC4, bytes([0x00]) + bytes([1] + [0] * 15) + bytes([200]))
+ac = seg(0xC4, bytes([0x10]) + bytes([1] + [0] * 15) + bytes([0]))
+sos = seg(0xDA, bytes([1, 1, 0x00, 0, 63, 0])) + bytes([0x00, 0x55] * 40)
+cases["dc200.jpg"] = b"\xff\xd8" + dqt + sof + dc + ac + sos + b"\xff\xd9"
+# a second frame header
+cases["twoframes.jpg"] = b"\xff\xd8" + dqt + sof + seg(0xC0, bytes([8]) + struct.pack(">HH", 8, 8) + bytes([3, 1, 0x41, 0, 2, 0x11, 0, 3, 0x11, 0])) + dc + ac + sos + b"\xff\xd9"
+# sampling factors 3 and 4 (3 does not divide 4)
+sof34 = seg(0xC0, bytes([8]) + struct.pack(">HH", 16, 16) + bytes([3, 1, 0x44, 0, 2, 0x33, 0, 3, 0x11, 0]))
+cases["factors.jpg"] = b"\xff\xd8" + dqt + sof34 + dc + ac + sos + b"\xff\xd9"
+for name, data in cases.items():
+    f = out / name
+    f.write_bytes(data)
+    try:
+        pg.read_image(f)
+        print(name, "decoded", flush=True)
+    except pg.PgError as e:
+        print(name, "error", str(e).replace(" ", "_"), flush=True)
+'''
+
+
+def test_crafted_oversized_and_corrupt_images_are_errors(tmp_path):
+    """An HDR header of 2e9 x 2e9 pixels and a PNG IHDR of 2^24 x 2^24 are rejected
+    before any allocation (no std::length_error / bad_alloc out of the C ABI); a JPEG DC
+    table coding category 200, a second frame header and sampling factors that do not
+    divide the largest ones are decode errors, not undefined shifts or wrong images."""
+    r = subprocess.run([sys.executable, "-c", CRAFTED_CHILD, str(ROOT), str(tmp_path)], capture_output=True,
+                       text=True, timeout=300)
+    assert r.returncode == 0, (r.returncode, r.stdout[-2000:], r.stderr[-2000:])
+    got = {ln.split()[0]: ln.split()[1:] for ln in r.stdout.strip().splitlines()}
+    assert set(got) == {"huge.hdr", "huge.png", "dc200.jpg", "twoframes.jpg", "factors.jpg"}
+    for name, res in got.items():
+        assert res[0] == "error", (name, res)
+    assert "2^28" in got["huge.hdr"][1] and "2^28" in got["huge.png"][1]
+    assert "DC_category" in got["dc200.jpg"][1]
+    assert "frame" in got["twoframes.jpg"][1] and "divide" in got["factors.jpg"][1]
