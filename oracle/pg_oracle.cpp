@@ -22,9 +22,13 @@
  *       (normal-incidence Fresnel, GGX normalisation, sample/pdf consistency,
  *       the reference's own sample->invert round trip, main.cpp:13-72)
  *   Env map Distribution2D               reference golden vectors + pmf sums
- *   Integrator (pathTrace/computeDirect) white-furnace closed form, 1/SPP
- *       convergence, reciprocity; Renderer.h does not build here (it needs
- *       the Windows-only GamesEngineeringBase.h), so no golden images exist
+ *   NEE constituents (light list, AreaLight + Triangle::sample, sampleLight,
+ *       calculateShadingData, visible, Film::splat)  reference vectors (ref_nee.npz)
+ *   Integrator (pathTrace/computeDirect) area-light NEE + MIS against an FP64
+ *       BSDF-sampling-only estimator (tests/independent.py), white-furnace closed
+ *       form, relMSE falling as 1/SPP, BSDF reciprocity; Renderer.h does not build
+ *       here (it needs the Windows-only GamesEngineeringBase.h), so no golden
+ *       images exist
  *   Record pass (guidedPath)             furnace closed form for Li records
  *   Point BVH / kNN                      brute force, validateNode invariants
  */
@@ -1337,6 +1341,28 @@ inline V3 v3(const float* p) { return V3(p[0], p[1], p[2]); }
 
 struct orc_scene { Scene sc; };
 
+// Film::splat with the BoxFilter (Imaging.h:147-151, 186-217): the 3x3 neighbourhood of
+// the pixel floor(x + 0.5), floor(y + 0.5); weight 1 where |px - x| < 0.5 and
+// |py - y| < 0.5; each covered pixel receives L * weight / total, in row order.
+template <class Add>
+static void splatBox(int W, int H, float x, float y, const C3& L, Add&& add) {
+    int used = 0; float total = 0.f; int ids[9]; float wts[9];
+    for (int i = -1; i <= 1; ++i)
+        for (int j = -1; j <= 1; ++j) {
+            int qx = (int)std::floor(x + 0.5f) + j;
+            int qy = (int)std::floor(y + 0.5f) + i;
+            if (qx >= 0 && qx < W && qy >= 0 && qy < H) {
+                ids[used] = qy * W + qx;
+                float dx = qx - x, dy = qy - y;
+                wts[used] = (fabsf(dx) < 0.5f && fabsf(dy) < 0.5f) ? 1.f : 0.f;
+                total += wts[used];
+                used++;
+            }
+        }
+    if (used > 0 && fabs(total) > 1e-4f)
+        for (int i = 0; i < used; ++i) add(ids[i], (L * wts[i]) / total);
+}
+
 static int nthreads(int32_t req) {
     if (req > 0) return req;
     unsigned h = std::thread::hardware_concurrency();
@@ -1600,27 +1626,11 @@ int orc_render(orc_scene* os, const orc_render_params* p, double* film, orc_rend
         for (int pix = 0; pix < W * H; ++pix) {
             const Slot& sl = slots[pix];
             if (!sl.ok) continue;
-            int used = 0; float total = 0.f; int ids[9]; float wts[9];
-            for (int i = -1; i <= 1; ++i)
-                for (int j = -1; j <= 1; ++j) {
-                    int qx = (int)std::floor(sl.px + 0.5f) + j;
-                    int qy = (int)std::floor(sl.py + 0.5f) + i;
-                    if (qx >= 0 && qx < W && qy >= 0 && qy < H) {
-                        ids[used] = qy * W + qx;
-                        float dx = qx - sl.px, dy = qy - sl.py;
-                        wts[used] = (fabsf(dx) < 0.5f && fabsf(dy) < 0.5f) ? 1.f : 0.f;
-                        total += wts[used];
-                        used++;
-                    }
-                }
-            if (used > 0 && fabs(total) > 1e-4f) {
-                for (int i = 0; i < used; ++i) {
-                    C3 add = (sl.L * wts[i]) / total;
-                    size_t o = (size_t)ids[i] * 3;
-                    if (p->accum_f64) { film[o] += add.r; film[o + 1] += add.g; film[o + 2] += add.b; }
-                    else { ffilm[o] = ffilm[o] + add.r; ffilm[o + 1] = ffilm[o + 1] + add.g; ffilm[o + 2] = ffilm[o + 2] + add.b; }
-                }
-            }
+            splatBox(W, H, sl.px, sl.py, sl.L, [&](int id, const C3& add) {
+                size_t o = (size_t)id * 3;
+                if (p->accum_f64) { film[o] += add.r; film[o + 1] += add.g; film[o + 2] += add.b; }
+                else { ffilm[o] = ffilm[o] + add.r; ffilm[o + 1] = ffilm[o + 1] + add.g; ffilm[o + 2] = ffilm[o + 2] + add.b; }
+            });
         }
     }
     auto t1 = std::chrono::steady_clock::now();
@@ -1635,6 +1645,74 @@ int orc_render(orc_scene* os, const orc_render_params* p, double* film, orc_rend
         st->n_records = sc.records.size();
     }
     return 0;
+}
+
+int32_t orc_lights(const orc_scene* o, int32_t* kind, int32_t* tri) {
+    const Scene& s = o->sc;
+    for (size_t i = 0; i < s.lights.size(); ++i) {
+        if (kind) kind[i] = s.lights[i].kind;
+        if (tri) tri[i] = s.lights[i].tri;
+    }
+    return (int32_t)s.lights.size();
+}
+
+void orc_area_light(const orc_scene* o, int32_t light, const float* u2, const float* wi, float* out) {
+    const Scene& s = o->sc;
+    const Light& L = s.lights[light];
+    const Tri& t = s.tris[L.tri];
+    FixedSampler fs;
+    fs.u[0] = u2[0]; fs.u[1] = u2[1]; fs.n = 2;
+    float pdf = 0.f;
+    V3 x = t.sample(&fs, pdf);                                                 // Lights.h:36-39
+    V3 g = t.gNormal();
+    C3 ev = dot(v3(wi), g) < 0.f ? L.emission : C3();                         // Lights.h:41-44
+    float o14[14] = {x.x, x.y, x.z, pdf, L.emission.r, L.emission.g, L.emission.b, ev.r, ev.g, ev.b,
+                     1.f / t.area, g.x, g.y, g.z};                             // Lights.h:46-56
+    std::memcpy(out, o14, sizeof(o14));
+}
+
+void orc_sample_light(const orc_scene* o, const float* u, uint32_t n, int32_t* idx, float* pmf) {
+    for (uint32_t i = 0; i < n; ++i) {
+        FixedSampler fs;
+        fs.u[0] = u[i]; fs.n = 1;
+        float pm = 0.f;
+        idx[i] = o->sc.sampleLight(&fs, pm);
+        pmf[i] = pm;
+    }
+}
+
+void orc_shading_data(const orc_scene* o, const float* rays, uint32_t n, float* out) {
+    const Scene& s = o->sc;
+    for (uint32_t i = 0; i < n; ++i) {
+        const float* r = rays + 6 * (size_t)i;
+        Ray ray(v3(r), v3(r + 3));
+        BVH::Hit h;
+        s.bvh.traverse(ray, s.tris, h, nullptr, nullptr);
+        SD sd = s.shade(h, ray);
+        float* q = out + 24 * (size_t)i;
+        std::memset(q, 0, 24 * sizeof(float));
+        q[9] = sd.wo.x; q[10] = sd.wo.y; q[11] = sd.wo.z; q[14] = sd.t;
+        if (h.t < FLT_MAX) {
+            const V3 f[7] = {sd.x, sd.gN, sd.sN, sd.wo, V3(), V3(), V3()};
+            for (int k = 0; k < 3; ++k) { q[k] = f[0][k]; q[3 + k] = f[1][k]; q[6 + k] = f[2][k]; }
+            q[12] = sd.tu; q[13] = sd.tv;
+            for (int k = 0; k < 3; ++k) {
+                q[15 + k] = sd.frame.u[k]; q[18 + k] = sd.frame.v[k]; q[21 + k] = sd.frame.w[k];
+            }
+        }
+    }
+}
+
+void orc_visible(const orc_scene* o, const float* p1, const float* p2, uint32_t n, uint8_t* vis) {
+    for (uint32_t i = 0; i < n; ++i) vis[i] = o->sc.visible(v3(p1 + 3 * (size_t)i), v3(p2 + 3 * (size_t)i)) ? 1 : 0;
+}
+
+void orc_film_splat(int32_t w, int32_t h, const float* xy, const float* L, uint32_t n, float* film) {
+    for (uint32_t i = 0; i < n; ++i)
+        splatBox(w, h, xy[2 * i], xy[2 * i + 1], C3(L[3 * i], L[3 * i + 1], L[3 * i + 2]), [&](int id, const C3& a) {
+            float* q = film + 3 * (size_t)id;
+            q[0] = q[0] + a.r; q[1] = q[1] + a.g; q[2] = q[2] + a.b;
+        });
 }
 
 uint64_t orc_records_count(const orc_scene* s) { return s->sc.records.size(); }

@@ -128,6 +128,25 @@ void orc_env_eval(const orc_scene* s, const float* wi, float* rgb, float* pdf);
 float orc_env_marginal_integral(const orc_scene* s);
 void orc_scene_bounds(const orc_scene* s, float* centre, float* radius);
 
+/* NEE constituents, one call per unit (parity aids):
+ * the light list (Scene.h:93-100, 133-136): kind 0 area (tri = its triangle), 1 env;
+ * returns the count. */
+int32_t orc_lights(const orc_scene* s, int32_t* kind, int32_t* tri);
+/* AreaLight::sample / evaluate / PDF / normal (Lights.h:28-73, Triangle::sample
+ * Geometry.h:139-150) of area light `light` with random numbers u2 and direction wi.
+ * out (14): point[3], pdf, emission[3], evaluate(wi)[3], PDF, normal[3].          */
+void orc_area_light(const orc_scene* s, int32_t light, const float* u2, const float* wi, float* out);
+/* Scene::sampleLight (Scene.h:108-119) for n random numbers: light index, pmf. */
+void orc_sample_light(const orc_scene* s, const float* u, uint32_t n, int32_t* idx, float* pmf);
+/* Scene::calculateShadingData (Scene.h:152-176) of each ray's closest hit. out per ray
+ * (24): x[3], gN[3], sN[3], wo[3], tu, tv, t, frame u[3], v[3], w[3] (misses: wo, t). */
+void orc_shading_data(const orc_scene* s, const float* rays, uint32_t n, float* out);
+/* Scene::visible (Scene.h:139-146). */
+void orc_visible(const orc_scene* s, const float* p1, const float* p2, uint32_t n, uint8_t* vis);
+/* Film::splat with the BoxFilter (Imaging.h:147-151, 186-217) of n samples (x, y, L)
+ * in order, added into a w x h x 3 FP32 film. */
+void orc_film_splat(int32_t w, int32_t h, const float* xy, const float* L, uint32_t n, float* film);
+
 /* Point BVH over records (Renderer.h:71-234): longest-axis median split,
  * leaf <= 8. Returns node count; stats: nodes, leaves, min depth, max depth. */
 int32_t orc_point_bvh_stats(const float* pos, uint32_t n, int32_t* stats);

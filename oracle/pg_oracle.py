@@ -111,6 +111,13 @@ def lib():
         L.orc_texture_sample.argtypes = [_F, C.c_int32, C.c_int32, _F, C.c_uint32, _F]
         L.orc_bsdf_flags.argtypes = [_F, _I32, _I32]
         L.orc_tonemap.argtypes = [_F, C.c_uint64, C.c_int32, C.c_float, C.c_float, _U8]
+        L.orc_lights.restype = C.c_int32
+        L.orc_lights.argtypes = [C.c_void_p, _I32, _I32]
+        L.orc_area_light.argtypes = [C.c_void_p, C.c_int32, _F, _F, _F]
+        L.orc_sample_light.argtypes = [C.c_void_p, _F, C.c_uint32, _I32, _F]
+        L.orc_shading_data.argtypes = [C.c_void_p, _F, C.c_uint32, _F]
+        L.orc_visible.argtypes = [C.c_void_p, _F, _F, C.c_uint32, _U8]
+        L.orc_film_splat.argtypes = [C.c_int32, C.c_int32, _F, _F, C.c_uint32, _F]
         _lib = L
     return _lib
 
@@ -168,6 +175,16 @@ def ref():
         R.ref_tonemap.argtypes = [_F, C.c_int, C.c_int, C.c_float, C.c_float, _U8]
         R.ref_write_hdr.restype = C.c_int
         R.ref_write_hdr.argtypes = [C.c_char_p, _F, C.c_int, C.c_int]
+        R.ref_full_create.restype = C.c_void_p
+        R.ref_full_create.argtypes = [_F, _F, _U32, C.c_uint32, _F, C.c_uint32, _F, C.c_int, C.c_int]
+        R.ref_full_destroy.argtypes = [C.c_void_p]
+        R.ref_full_lights.restype = C.c_int
+        R.ref_full_lights.argtypes = [C.c_void_p, _I32, _I32]
+        R.ref_area_light.argtypes = [C.c_void_p, C.c_int, _F, _F, _F]
+        R.ref_sample_light.argtypes = [C.c_void_p, _F, C.c_uint32, _I32, _F]
+        R.ref_shading_data.argtypes = [C.c_void_p, _F, C.c_uint32, _F]
+        R.ref_full_visible.argtypes = [C.c_void_p, _F, _F, C.c_uint32, _U8]
+        R.ref_film_splat.argtypes = [C.c_int, C.c_int, _F, _F, C.c_uint32, _F]
         _ref = R
     return _ref
 
@@ -268,3 +285,101 @@ class OracleScene:
 
 def rng(seed, sample, pixel, dim):
     return lib().orc_rng(seed, sample, pixel, dim)
+
+
+def _ip(a):
+    return a.ctypes.data_as(_I32)
+
+
+class _NeeUnits:
+    """Per-unit access to the next-event-estimation constituents; the oracle and the
+    reference driver expose the same calls (orc_* / ref_*)."""
+
+    def lights(self):
+        n = self._lights(None, None)
+        kind, tri = np.empty(n, np.int32), np.empty(n, np.int32)
+        self._lights(_ip(kind), _ip(tri))
+        return kind, tri
+
+    def area_light(self, light, u2, wi):
+        out = np.empty(14, np.float32)
+        self._area(int(light), fp(np.ascontiguousarray(u2, np.float32)), fp(np.ascontiguousarray(wi, np.float32)),
+                   fp(out))
+        return out
+
+    def sample_light(self, u):
+        u = np.ascontiguousarray(u, np.float32)
+        idx, pmf = np.empty(u.size, np.int32), np.empty(u.size, np.float32)
+        self._sample_light(fp(u), u.size, _ip(idx), fp(pmf))
+        return idx, pmf
+
+    def shading_data(self, rays):
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 6)
+        out = np.empty((rays.shape[0], 24), np.float32)
+        self._shading(fp(rays), rays.shape[0], fp(out))
+        return out
+
+    def visible(self, p1, p2):
+        p1 = np.ascontiguousarray(p1, np.float32).reshape(-1, 3)
+        p2 = np.ascontiguousarray(p2, np.float32).reshape(-1, 3)
+        vis = np.empty(p1.shape[0], np.uint8)
+        self._visible(fp(p1), fp(p2), p1.shape[0], vis.ctypes.data_as(_U8))
+        return vis
+
+
+class OracleNee(_NeeUnits):
+    """The oracle's NEE constituents on an OracleScene."""
+
+    def __init__(self, osc):
+        L, h = lib(), osc.h
+        self.osc = osc
+        self._lights = lambda k, t: L.orc_lights(h, k, t)
+        self._area = lambda l, u, w, o: L.orc_area_light(h, l, u, w, o)
+        self._sample_light = lambda u, n, i, p: L.orc_sample_light(h, u, n, i, p)
+        self._shading = lambda r, n, o: L.orc_shading_data(h, r, n, o)
+        self._visible = lambda a, b, n, v: L.orc_visible(h, a, b, n, v)
+
+
+class RefFullScene(_NeeUnits):
+    """The reference's own Scene (init + build) over a scene's triangles, materials and
+    optional environment map (built only where the reference sources exist)."""
+
+    def __init__(self, data, env=None):
+        R = ref()
+        self._keep = [np.ascontiguousarray(data.positions, np.float32), np.ascontiguousarray(data.normals, np.float32),
+                      np.ascontiguousarray(data.material_index, np.uint32),
+                      np.ascontiguousarray(data.materials_f32(), np.float32),
+                      None if env is None else np.ascontiguousarray(env, np.float32)]
+        pos, nrm, mi, mats, env = self._keep
+        h = R.ref_full_create(fp(pos), fp(nrm), up(mi), pos.shape[0], fp(mats), mats.shape[0], fp(env),
+                              0 if env is None else env.shape[1], 0 if env is None else env.shape[0])
+        self.h = h
+        self._lights = lambda k, t: R.ref_full_lights(h, k, t)
+        self._area = lambda l, u, w, o: R.ref_area_light(h, l, u, w, o)
+        self._sample_light = lambda u, n, i, p: R.ref_sample_light(h, u, n, i, p)
+        self._shading = lambda r, n, o: R.ref_shading_data(h, r, n, o)
+        self._visible = lambda a, b, n, v: R.ref_full_visible(h, a, b, n, v)
+
+    def __del__(self):
+        try:
+            ref().ref_full_destroy(self.h)
+        except Exception:
+            pass
+
+
+def film_splat(w, h, xy, L):
+    """Film::splat with the box filter (the oracle's) of samples in order: (h, w, 3) sums."""
+    xy = np.ascontiguousarray(xy, np.float32).reshape(-1, 2)
+    L = np.ascontiguousarray(L, np.float32).reshape(-1, 3)
+    film = np.zeros((h, w, 3), np.float32)
+    lib().orc_film_splat(w, h, fp(xy), fp(L), xy.shape[0], fp(film))
+    return film
+
+
+def ref_film_splat(w, h, xy, L):
+    """Film::splat of the reference's own Film and BoxFilter."""
+    xy = np.ascontiguousarray(xy, np.float32).reshape(-1, 2)
+    L = np.ascontiguousarray(L, np.float32).reshape(-1, 3)
+    film = np.zeros((h, w, 3), np.float32)
+    ref().ref_film_splat(w, h, fp(xy), fp(L), xy.shape[0], fp(film))
+    return film

@@ -395,6 +395,125 @@ void ref_tonemap(const float* rgb_sum, int n, int spp, float exposure, float lma
     delete[] f.film;
 }
 
+/* A whole Scene over the reference's classes (Scene.h:84-137): triangles with
+ * per-vertex normals (uv 0), the materials as BSDFs (16-float records), the
+ * background (EnvironmentMap over an RGB texture, or a black BackgroundColour),
+ * Scene::init + Scene::build (BVH and the light list).                          */
+struct RefFull {
+    Scene scene;
+};
+
+void* ref_full_create(const float* pos, const float* nrm, const uint32_t* mi, uint32_t n, const float* mats,
+                      uint32_t n_mats, const float* env_rgb, int env_w, int env_h) {
+    RefFull* r = new RefFull();
+    std::vector<Triangle> tris(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        Vertex v[3];
+        for (int k = 0; k < 3; ++k) {
+            v[k].p = Vec4(pos[9 * i + 3 * k], pos[9 * i + 3 * k + 1], pos[9 * i + 3 * k + 2]);
+            v[k].normal = Vec4(nrm[9 * i + 3 * k], nrm[9 * i + 3 * k + 1], nrm[9 * i + 3 * k + 2]);
+            v[k].u = 0.f; v[k].v = 0.f;
+        }
+        tris[i].init(v[0], v[1], v[2], mi[i]);
+    }
+    std::vector<BSDF*> bs;
+    for (uint32_t i = 0; i < n_mats; ++i) bs.push_back(makeBSDF(mats + 16 * (size_t)i));
+    Light* bg = nullptr;
+    if (env_rgb && env_w > 0 && env_h > 0) bg = (Light*)ref_env_create(env_rgb, env_w, env_h);
+    else bg = new BackgroundColour(Colour(0.f, 0.f, 0.f));
+    r->scene.init(tris, bs, bg);
+    r->scene.build();
+    return r;
+}
+void ref_full_destroy(void* p) { delete (RefFull*)p; }
+
+/* The light list: kind (0 area, 1 environment) and the area light's triangle. */
+int ref_full_lights(void* p, int32_t* kind, int32_t* tri) {
+    Scene& s = ((RefFull*)p)->scene;
+    for (size_t i = 0; i < s.lights.size(); ++i) {
+        bool area = s.lights[i]->isArea();
+        if (kind) kind[i] = area ? 0 : 1;
+        if (tri) tri[i] = area ? (int32_t)(((AreaLight*)s.lights[i])->triangle - s.triangles.data()) : -1;
+    }
+    return (int)s.lights.size();
+}
+
+/* AreaLight::sample / evaluate / PDF / normal (Lights.h:28-73; Triangle::sample,
+ * Geometry.h:139-150) of area light `light` with random numbers u2 and the
+ * evaluation direction wi. out: point[3], pdf, emission[3], evaluate(wi)[3],
+ * PDF(wi), normal[3] (14 floats).                                                */
+void ref_area_light(void* p, int light, const float* u2, const float* wi, float* out) {
+    Scene& s = ((RefFull*)p)->scene;
+    Light* L = s.lights[light];
+    ListSampler ls; ls.u[0] = u2[0]; ls.u[1] = u2[1]; ls.n = 2;
+    ShadingData sd;
+    Colour e; float pdf = 0.f;
+    Vec4 x = L->sample(sd, &ls, e, pdf);
+    Vec4 w(wi[0], wi[1], wi[2]);
+    Colour ev = L->evaluate(w);
+    Vec4 nn = L->normal(sd, w);
+    float o[14] = {x.x, x.y, x.z, pdf, e.r, e.g, e.b, ev.r, ev.g, ev.b, L->PDF(sd, w), nn.x, nn.y, nn.z};
+    std::memcpy(out, o, sizeof(o));
+}
+
+/* Scene::sampleLight (Scene.h:108-119) for n random numbers: index and pmf. */
+void ref_sample_light(void* p, const float* u, uint32_t n, int32_t* idx, float* pmf) {
+    Scene& s = ((RefFull*)p)->scene;
+    for (uint32_t i = 0; i < n; ++i) {
+        ListSampler ls; ls.u[0] = u[i]; ls.n = 1;
+        float pm = 0.f;
+        Light* L = s.sampleLight(&ls, pm);
+        int k = -1;
+        for (size_t j = 0; j < s.lights.size(); ++j)
+            if (s.lights[j] == L) k = (int)j;
+        idx[i] = k; pmf[i] = pm;
+    }
+}
+
+/* Scene::traverse + Scene::calculateShadingData (Scene.h:152-176) for n rays.
+ * out per ray (24 floats): x[3], gNormal[3], sNormal[3], wo[3], tu, tv, t,
+ * frame.u[3], frame.v[3], frame.w[3] (zeros where the ray misses).               */
+void ref_shading_data(void* p, const float* rays, uint32_t n, float* out) {
+    Scene& s = ((RefFull*)p)->scene;
+    for (uint32_t i = 0; i < n; ++i) {
+        const float* r = rays + 6 * (size_t)i;
+        Ray ray(Vec4(r[0], r[1], r[2]), Vec4(r[3], r[4], r[5]));
+        IntersectionData h = s.traverse(ray);
+        ShadingData sd = s.calculateShadingData(h, ray);
+        float* o = out + 24 * (size_t)i;
+        std::memset(o, 0, 24 * sizeof(float));
+        o[9] = sd.wo.x; o[10] = sd.wo.y; o[11] = sd.wo.z; o[14] = sd.t;
+        if (h.t < FLT_MAX) {
+            o[0] = sd.x.x; o[1] = sd.x.y; o[2] = sd.x.z;
+            o[3] = sd.gNormal.x; o[4] = sd.gNormal.y; o[5] = sd.gNormal.z;
+            o[6] = sd.sNormal.x; o[7] = sd.sNormal.y; o[8] = sd.sNormal.z;
+            o[12] = sd.tu; o[13] = sd.tv;
+            o[15] = sd.frame.u.x; o[16] = sd.frame.u.y; o[17] = sd.frame.u.z;
+            o[18] = sd.frame.v.x; o[19] = sd.frame.v.y; o[20] = sd.frame.v.z;
+            o[21] = sd.frame.w.x; o[22] = sd.frame.w.y; o[23] = sd.frame.w.z;
+        }
+    }
+}
+
+/* Scene::visible (Scene.h:139-146) on the whole scene. */
+void ref_full_visible(void* p, const float* p1, const float* p2, uint32_t n, uint8_t* vis) {
+    Scene& s = ((RefFull*)p)->scene;
+    for (uint32_t i = 0; i < n; ++i)
+        vis[i] = s.visible(Vec4(p1[3 * i], p1[3 * i + 1], p1[3 * i + 2]), Vec4(p2[3 * i], p2[3 * i + 1], p2[3 * i + 2]))
+                     ? 1 : 0;
+}
+
+/* Film::splat (Imaging.h:186-217) with the BoxFilter (Imaging.h:147-151) of n
+ * samples (x, y, L), in order, onto a zeroed w x h film; out: the film sums.     */
+void ref_film_splat(int w, int h, const float* xy, const float* L, uint32_t n, float* out) {
+    Film f;
+    BoxFilter box;
+    f.init(w, h, &box);
+    for (uint32_t i = 0; i < n; ++i) f.splat(xy[2 * i], xy[2 * i + 1], Colour(L[3 * i], L[3 * i + 1], L[3 * i + 2]));
+    for (int i = 0; i < w * h; ++i) { out[3 * i] = f.film[i].r; out[3 * i + 1] = f.film[i].g; out[3 * i + 2] = f.film[i].b; }
+    delete[] f.film;
+}
+
 /* stbi_write_hdr of the mean image, as Film::save (Imaging.h:263-272). */
 int ref_write_hdr(const char* path, const float* rgb, int w, int h) { return stbi_write_hdr(path, w, h, 3, rgb); }
 

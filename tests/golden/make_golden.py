@@ -4,7 +4,7 @@ Reference vectors (ref_*.npz) come from the reference's own classes, compiled fr
 /root/reference by oracle/Makefile into oracle/_ref/libpgref.so (this container only).
 Oracle images (oracle_*.npz, *_band.json) come from the CPU oracle in FP64.
 
-    python tests/golden/make_golden.py [--ref] [--images] [--scenes]
+    python tests/golden/make_golden.py [--ref] [--images] [--scenes] [--nee] [--independent]
 """
 from __future__ import annotations
 
@@ -290,6 +290,75 @@ def make_scenes():
     print("scene vectors written")
 
 
+def make_nee():
+    """Vectors for the next-event-estimation constituents over the reference's own Scene:
+    the light list (Scene.h:93-100, 133-136), AreaLight::sample / evaluate / PDF / normal
+    with Triangle::sample (Lights.h:28-73, Geometry.h:139-150), Scene::sampleLight
+    (Scene.h:108-119), Scene::calculateShadingData (Scene.h:152-176), Scene::visible
+    (Scene.h:139-146) and Film::splat with the box filter (Imaging.h:147-151, 186-217)."""
+    from oracle import pg_oracle as po
+    import pathguidingdissertation_amd as pg
+    from conftest import seeded_rays
+    assert po.ref_available(), "build oracle/_ref first (make -C oracle ref)"
+    rng = np.random.default_rng(2468)
+    out = {}
+    scenes = {"cornell": (pg.generate_scene(1, 0), None), "cornell_env": (pg.generate_scene(1, 0), synthetic_env(64, 32)),
+              "interior": (pg.generate_scene(100, 0), "own")}
+    for key, (data, env) in scenes.items():
+        if isinstance(env, str):
+            env = data.env
+        rs = po.RefFullScene(data, env)
+        kinds, tris = rs.lights()
+        out[f"{key}:light_kind"], out[f"{key}:light_tri"] = kinds, tris
+        if env is not None and key != "interior":
+            out[f"{key}:env"] = env
+        area = np.nonzero(kinds == 0)[0]
+        if area.size:
+            u2 = rng.random((256, 2)).astype(np.float32)
+            u2[:4] = [[0, 0], [1, 1], [0, 1], [1, 0]]
+            wi = rng.normal(size=(256, 3)).astype(np.float32)
+            wi /= np.linalg.norm(wi, axis=1, keepdims=True)
+            al = np.stack([rs.area_light(int(l), u2[i], wi[i]) for l in area for i in range(256)])
+            out[f"{key}:area_u2"], out[f"{key}:area_wi"], out[f"{key}:area_out"] = u2, wi, al
+        u = np.concatenate([rng.random(4093), [0.0, 0.99999994, 1.0]]).astype(np.float32)
+        idx, pmf = rs.sample_light(u)
+        out[f"{key}:sl_u"], out[f"{key}:sl_idx"], out[f"{key}:sl_pmf"] = u, idx, pmf
+        osc = po.OracleScene(data)
+        rays = seeded_rays(osc, data, 2048, seed=77)
+        out[f"{key}:rays"], out[f"{key}:shading"] = rays, rs.shading_data(rays)
+        sd = out[f"{key}:shading"]
+        hit = sd[:, 14] < np.finfo(np.float32).max
+        lo, hi = data.positions.reshape(-1, 3).min(0), data.positions.reshape(-1, 3).max(0)
+        p1 = np.concatenate([sd[hit, 0:3][:768], rng.uniform(lo, hi, (768, 3))]).astype(np.float32)
+        p2 = np.concatenate([rng.uniform(lo, hi, (p1.shape[0] - 512, 3)), sd[hit, 0:3][-512:]]).astype(np.float32)
+        out[f"{key}:vis_p1"], out[f"{key}:vis_p2"], out[f"{key}:visible"] = p1, p2, rs.visible(p1, p2)
+        del rs
+    w, h = 7, 5
+    xy = np.concatenate([rng.uniform(-1.5, [w + 1.5, h + 1.5], (3000, 2)),
+                         np.array([[0.5, 0.5], [1.5, 2.5], [-0.5, 0.0], [6.5, 4.5], [7.0, 5.0], [0.0, 0.0],
+                                   [3.4999998, 2.5000002]])]).astype(np.float32)
+    L = (rng.random((xy.shape[0], 3)) * 10).astype(np.float32)
+    out["splat:wh"], out["splat:xy"], out["splat:L"] = np.array([w, h], np.int32), xy, L
+    out["splat:film"] = po.ref_film_splat(w, h, xy, L)
+    np.savez_compressed(OUT / "ref_nee.npz", **out)
+    print("next-event vectors written")
+
+
+def make_independent(spp=4096, seed=2024):
+    """Block sums of the area-light room (tests/independent.py) from the FP64 numpy
+    BSDF-sampling estimator, which shares no code with any next-event estimation, with
+    their standard errors: the reference point of the NEE + MIS unbiasedness tests."""
+    from oracle import pg_oracle as po
+    import pathguidingdissertation_amd as pg
+    import independent as ind
+    data = ind.area_light_box(pg)
+    osc = po.OracleScene(data)
+    mean, se = ind.bsdf_only_estimate(data, osc.generate_rays, spp, seed)
+    np.savez_compressed(OUT / "independent_area_box.npz", block_sum=mean, stderr=se, spp=spp, seed=seed,
+                        block=ind.BLOCK, width=data.width, height=data.height)
+    print("independent estimate: relative standard error max", float((se / mean).max()))
+
+
 def synthetic_env(w, h):
     v = (np.arange(h) + 0.5) / h
     u = (np.arange(w) + 0.5) / w
@@ -327,6 +396,8 @@ if __name__ == "__main__":
     ap.add_argument("--ref", action="store_true")
     ap.add_argument("--images", action="store_true")
     ap.add_argument("--scenes", action="store_true")
+    ap.add_argument("--nee", action="store_true")
+    ap.add_argument("--independent", action="store_true")
     a = ap.parse_args()
     if a.ref:
         make_ref()
@@ -334,3 +405,7 @@ if __name__ == "__main__":
         make_images()
     if a.scenes:
         make_scenes()
+    if a.nee:
+        make_nee()
+    if a.independent:
+        make_independent()
