@@ -178,6 +178,16 @@ pg_status pg_generate_rays(pg_ctx* ctx, const float* pxy, uint32_t n, pg_ray* ra
 pg_status pg_rng_values(pg_ctx* ctx, uint32_t seed, uint32_t sample, const uint32_t* pixel,
                         const uint32_t* dim, uint32_t n, float* out);
 
+/* BSDF::sample / evaluate / PDF / invert / isPureSpecular / isTwoSided (Materials.h:
+ * 121-897) of one material on the device, for n shading points with given random
+ * numbers (the reference's test sampler, main.cpp:13-72) — parity aid. sd: 12 floats
+ * per query (x, wo, shading normal, geometric normal; frame from the shading normal),
+ * u: 3, wi_eval: 3. out: 15 floats per query: sampled wi[3], f[3], pdf, evaluate(
+ * wi_eval)[3], PDF(wi_eval), invert(sampled wi) u, v (0, 0 where the BSDF has no
+ * invert), isPureSpecular, isTwoSided. At most 2^24 queries.                     */
+pg_status pg_bsdf_query(pg_ctx* ctx, const pg_material* material, uint32_t n, const float* sd, const float* u,
+                        const float* wi_eval, float* out);
+
 /* ------------------------------------------------------------------ rendering */
 pg_render_params pg_render_params_default(void);
 /* RayTracer::clear (Renderer.h:265-267): zero the film and SPP count. */

@@ -112,6 +112,7 @@ SIGNATURES = {
     "pg_trace_shadow": (C.c_int32, [_P, _F, _F, C.c_uint32, _U8]),
     "pg_generate_rays": (C.c_int32, [_P, _F, C.c_uint32, _F]),
     "pg_rng_values": (C.c_int32, [_P, C.c_uint32, C.c_uint32, _U32, _U32, C.c_uint32, _F]),
+    "pg_bsdf_query": (C.c_int32, [_P, C.POINTER(PgMaterial), C.c_uint32, _F, _F, _F, _F]),
     "pg_render_params_default": (PgRenderParams, []),
     "pg_film_clear": (C.c_int32, [_P]),
     "pg_render_spp": (C.c_int32, [_P, C.POINTER(PgRenderParams), C.c_uint32]),

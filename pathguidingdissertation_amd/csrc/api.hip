@@ -561,6 +561,32 @@ pg_status pg_rng_values(pg_ctx* c, uint32_t seed, uint32_t sample, const uint32_
     return st;
 }
 
+pg_status pg_bsdf_query(pg_ctx* c, const pg_material* material, uint32_t n, const float* sd, const float* u,
+                        const float* wi_eval, float* out) {
+    PG_CHECK_CTX(c);
+    if (n == 0) return PG_OK;
+    if (!material || !sd || !u || !wi_eval || !out) return c->fail(PG_ERR_INVALID, "null array");
+    if (n > (1u << 24)) return c->fail(PG_ERR_INVALID, "more than 2^24 queries");
+    MaterialRec m;
+    if (!material_record(*material, m)) return c->fail(PG_ERR_INVALID, "unknown material type");
+    float *dsd = nullptr, *du = nullptr, *dwe = nullptr, *dout = nullptr;
+    pg_status st = PG_OK;
+    do {
+        if ((st = c->hip(dalloc(&dsd, (size_t)n * 12), "alloc")) != PG_OK) break;
+        if ((st = c->hip(dalloc(&du, (size_t)n * 3), "alloc")) != PG_OK) break;
+        if ((st = c->hip(dalloc(&dwe, (size_t)n * 3), "alloc")) != PG_OK) break;
+        if ((st = c->hip(dalloc(&dout, (size_t)n * 15), "alloc")) != PG_OK) break;
+        if ((st = c->hip(hipMemcpy(dsd, sd, (size_t)n * 48, hipMemcpyHostToDevice), "copy")) != PG_OK) break;
+        if ((st = c->hip(hipMemcpy(du, u, (size_t)n * 12, hipMemcpyHostToDevice), "copy")) != PG_OK) break;
+        if ((st = c->hip(hipMemcpy(dwe, wi_eval, (size_t)n * 12, hipMemcpyHostToDevice), "copy")) != PG_OK) break;
+        if ((st = c->hip(launch_bsdf_query(m, dsd, du, dwe, n, dout, c->stream), "bsdf query")) != PG_OK) break;
+        if ((st = c->hip(hipStreamSynchronize(c->stream), "sync")) != PG_OK) break;
+        st = c->hip(hipMemcpy(out, dout, (size_t)n * 60, hipMemcpyDeviceToHost), "copy");
+    } while (0);
+    dfree(dsd); dfree(du); dfree(dwe); dfree(dout);
+    return st;
+}
+
 pg_render_params pg_render_params_default(void) {
     pg_render_params p;
     std::memset(&p, 0, sizeof(p));

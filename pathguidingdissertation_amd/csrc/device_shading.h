@@ -522,7 +522,10 @@ __device__ inline C3 bsdf_eval(const MaterialRec& m, const SD& sd, F3 wi, float&
     return zero;  // mirror / glass: delta distributions
 }
 
-__device__ inline F3 bsdf_sample(const MaterialRec& m, const SD& sd, uint32_t key, uint32_t& dim, C3& f, float& pdf) {
+// The random numbers come from `rng.next()` in the order the reference's sample()
+// draws them: the counter RNG in the renderer, given numbers in pg_bsdf_query.
+template <class Rng>
+__device__ inline F3 bsdf_sample(const MaterialRec& m, const SD& sd, Rng& rng, C3& f, float& pdf) {
     C3 alb = sd.alb;
     F3 up{0.f, 0.f, 1.f};
     f = c3(0.f, 0.f, 0.f);
@@ -530,10 +533,10 @@ __device__ inline F3 bsdf_sample(const MaterialRec& m, const SD& sd, uint32_t ke
     switch (m.type) {
     case PG_BSDF_PLASTIC: {  // Materials.h:742-778
         F3 ol = to_local(sd.fr, sd.wo);
-        float r1 = keyed_rand(key, dim++), r2 = keyed_rand(key, dim++);
+        float r1 = rng.next(), r2 = rng.next();
         float F = fresnel_dielectric(fabsf(ol.z), m.int_ior, m.ext_ior);
         F3 wl;
-        if (keyed_rand(key, dim++) < F) {
+        if (rng.next() < F) {
             float e = plastic_exponent(m.alpha);
             float base = 1.f - r1, power = 1.f / (e + 1.f);
             float th = acosf(powf(base, power));
@@ -549,7 +552,7 @@ __device__ inline F3 bsdf_sample(const MaterialRec& m, const SD& sd, uint32_t ke
     }
     case PG_BSDF_ORENNAYAR:  // Materials.h:655-670: the diffuse cosine warp
     case PG_BSDF_DIFFUSE: {  // Materials.h:146-161
-        float r1 = keyed_rand(key, dim++), r2 = keyed_rand(key, dim++);
+        float r1 = rng.next(), r2 = rng.next();
         float theta = acosf(sqrtf(r1));
         float phi = (float)(2.0 * kPiD * (double)r2);
         F3 wi = to_world(sd.fr, spherical_to_world(theta, phi));
@@ -575,7 +578,7 @@ __device__ inline F3 bsdf_sample(const MaterialRec& m, const SD& sd, uint32_t ke
             return to_world(sd.fr, wr);
         }
         float a2 = m.alpha * m.alpha;
-        float r1 = keyed_rand(key, dim++), r2 = keyed_rand(key, dim++);
+        float r1 = rng.next(), r2 = rng.next();
         float tm = acosf(sqrtf((1.f - r1) / (r1 * (a2 - 1.f) + 1.f)));
         float pm = (float)(2.0 * kPiD * (double)r2);
         F3 wm = spherical_to_world(tm, pm);
@@ -591,7 +594,7 @@ __device__ inline F3 bsdf_sample(const MaterialRec& m, const SD& sd, uint32_t ke
         float ci = ol.z;
         float eta = (ci < 0.f) ? (m.int_ior / m.ext_ior) : (m.ext_ior / m.int_ior);
         if (ci < 0.f) ci = fabsf(ci);
-        if (keyed_rand(key, dim++) < F) {
+        if (rng.next() < F) {
             F3 wr{-ol.x, -ol.y, ol.z};
             pdf = F;
             f = (alb / fabsf(wr.z)) * F;
@@ -616,7 +619,7 @@ __device__ inline F3 bsdf_sample(const MaterialRec& m, const SD& sd, uint32_t ke
         F3 ol = to_local(sd.fr, sd.wo);
         if (m.alpha < kEps) {
             float F = fresnel_dielectric(ol.z, m.int_ior, m.ext_ior);
-            if (keyed_rand(key, dim++) < F) {
+            if (rng.next() < F) {
                 F3 wl{-ol.x, -ol.y, ol.z};
                 pdf = F;
                 f = (alb / fabsf(wl.z)) * F;
@@ -639,12 +642,12 @@ __device__ inline F3 bsdf_sample(const MaterialRec& m, const SD& sd, uint32_t ke
             return to_world(sd.fr, wl);
         }
         float a2 = m.alpha * m.alpha;
-        float r1 = keyed_rand(key, dim++), r2 = keyed_rand(key, dim++);
+        float r1 = rng.next(), r2 = rng.next();
         float tm = acosf(sqrtf((1.f - r1) / (r1 * (a2 - 1.f) + 1.f)));
         float pm = (float)(2.0 * kPiD * (double)r2);
         F3 wm = spherical_to_world(tm, pm);
         float F = fresnel_dielectric(dot(ol, wm), m.int_ior, m.ext_ior);
-        if (keyed_rand(key, dim++) < F) {
+        if (rng.next() < F) {
             F3 wl = -ol + (wm * 2.f) * dot(wm, ol);
             F3 wi = to_world(sd.fr, wl);
             f = bsdf_eval(m, sd, wi, pdf);
@@ -666,6 +669,21 @@ __device__ inline F3 bsdf_sample(const MaterialRec& m, const SD& sd, uint32_t ke
     }
 }
 
+struct KeyRng {  // the counter RNG stream of a path: keyed_rand(key, dim++)
+    uint32_t key, dim;
+    __device__ float next() { return keyed_rand(key, dim++); }
+};
+struct ListRng {  // given numbers, then 0.5 (the reference's test sampler)
+    float u[3];
+    int c;
+    __device__ float next() { return c < 3 ? u[c++] : 0.5f; }
+};
+__device__ inline F3 bsdf_sample(const MaterialRec& m, const SD& sd, uint32_t key, uint32_t& dim, C3& f, float& pdf) {
+    KeyRng r{key, dim};
+    F3 wi = bsdf_sample(m, sd, r, f, pdf);
+    dim = r.dim;
+    return wi;
+}
 
 // BSDF::invert (Materials.h:163-171, 287-306, 672-681): the primary-sample-space
 // point (u, v) that sample() maps to wi. Defined for the BSDFs whose sampling is a

@@ -19,7 +19,7 @@ namespace pg {
 constexpr int kLeafRecords = 8;
 constexpr int kKnnMax = 16;
 constexpr int kQtMaxDepth = 10;
-constexpr float kQtSplitFraction = 0.01f;
+constexpr unsigned long long kQtSplitDivisor = 100ull;  // refine above 1% of the cell's flux
 constexpr double kQtFixedScale = 16777216.0;  // 2^24: flux sums in 64-bit fixed point
 
 // 48-byte record in HBM: pos.xyz, normal.xyz, wi.xyz, Li.rgb.

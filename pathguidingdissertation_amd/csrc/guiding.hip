@@ -345,7 +345,9 @@ __global__ __launch_bounds__(256) void k_qt_build(const float4* dir, uint32_t n,
     for (int depth = 0; depth < kQtMaxDepth; ++depth) {
         // finalize energies of this level's nodes and create children
         if (threadIdx.x == 0) {
-            unsigned long long thr = (unsigned long long)((double)total * (double)kQtSplitFraction);
+            // a quadrant is refined when it holds more than 1% of the cell's flux:
+            // f > total / 100 exactly (integer flux, so f > floor(total / 100))
+            unsigned long long thr = total / kQtSplitDivisor;
             int begin = lvl_begin, end = lvl_end, nn = n_nodes;
             for (int nd = begin; nd < end; ++nd) {
                 for (int q = 0; q < 4; ++q) {

@@ -313,6 +313,28 @@ void build_env_tables(const float* rgb, int32_t w, int32_t h, EnvTables& out) {
 }
 
 // ------------------------------------------------------------------ packing
+bool material_record(const pg_material& m, MaterialRec& r) {
+    std::memset(&r, 0, sizeof(r));
+    r.type = (int32_t)m.type;
+    bool layered = r.type >= PG_BSDF_LAYERED;                                 // Materials.h:855-897
+    if (layered) r.type -= PG_BSDF_LAYERED;
+    if (r.type < PG_BSDF_DIFFUSE || r.type > PG_BSDF_PLASTIC) return false;
+    r.tex = -1;
+    for (int c = 0; c < 3; ++c) {
+        r.albedo[c] = m.albedo[c]; r.emission[c] = m.emission[c]; r.eta[c] = m.eta[c]; r.k[c] = m.k[c];
+    }
+    r.int_ior = m.int_ior; r.ext_ior = m.ext_ior;
+    r.alpha = 1.62142f * sqrtf(m.roughness);                                   // Materials.h:249, 455, 738
+    if (r.type == PG_BSDF_ORENNAYAR) r.alpha = m.roughness;                   // sigma (Materials.h:649-652)
+    float lum = ((0.2126f * m.emission[0]) + (0.7152f * m.emission[1])) + (0.0722f * m.emission[2]);
+    r.is_light = lum > 0.f ? 1u : 0u;                                          // Materials.h:130
+    bool spec = r.type == PG_BSDF_MIRROR || r.type == PG_BSDF_GLASS ||
+                ((r.type == PG_BSDF_CONDUCTOR || r.type == PG_BSDF_DIELECTRIC) && r.alpha < kEps);
+    r.specular = spec ? 1u : 0u;
+    r.two_sided = (!layered && (r.type == PG_BSDF_GLASS || r.type == PG_BSDF_DIELECTRIC)) ? 0u : 1u;
+    return true;
+}
+
 std::string prepare_scene(const float* pos, const float* nrm, const uint32_t* mat_idx, uint32_t n_tris,
                           const pg_material* mats, uint32_t n_mats, const float* env_rgb, int32_t env_w,
                           int32_t env_h, const pg_camera_desc& cam, PreparedScene& out) {
@@ -324,28 +346,8 @@ std::string prepare_scene(const float* pos, const float* nrm, const uint32_t* ma
     for (uint32_t i = 0; i < n_tris; ++i)
         if (mat_idx[i] >= n_mats) return "material index out of range";
     out.mats.resize(n_mats);
-    for (uint32_t i = 0; i < n_mats; ++i) {
-        const pg_material& m = mats[i];
-        MaterialRec& r = out.mats[i];
-        std::memset(&r, 0, sizeof(r));
-        r.type = (int32_t)m.type;
-        bool layered = r.type >= PG_BSDF_LAYERED;                             // Materials.h:855-897
-        if (layered) r.type -= PG_BSDF_LAYERED;
-        if (r.type < PG_BSDF_DIFFUSE || r.type > PG_BSDF_PLASTIC) return "unknown material type";
-        r.tex = -1;
-        for (int c = 0; c < 3; ++c) {
-            r.albedo[c] = m.albedo[c]; r.emission[c] = m.emission[c]; r.eta[c] = m.eta[c]; r.k[c] = m.k[c];
-        }
-        r.int_ior = m.int_ior; r.ext_ior = m.ext_ior;
-        r.alpha = 1.62142f * sqrtf(m.roughness);                               // Materials.h:249, 455, 738
-        if (r.type == PG_BSDF_ORENNAYAR) r.alpha = m.roughness;               // sigma (Materials.h:649-652)
-        float lum = ((0.2126f * m.emission[0]) + (0.7152f * m.emission[1])) + (0.0722f * m.emission[2]);
-        r.is_light = lum > 0.f ? 1u : 0u;                                      // Materials.h:130
-        bool spec = r.type == PG_BSDF_MIRROR || r.type == PG_BSDF_GLASS ||
-                    ((r.type == PG_BSDF_CONDUCTOR || r.type == PG_BSDF_DIELECTRIC) && r.alpha < kEps);
-        r.specular = spec ? 1u : 0u;
-        r.two_sided = (!layered && (r.type == PG_BSDF_GLASS || r.type == PG_BSDF_DIELECTRIC)) ? 0u : 1u;
-    }
+    for (uint32_t i = 0; i < n_mats; ++i)
+        if (!material_record(mats[i], out.mats[i])) return "unknown material type";
     build_bvh(pos, n_tris, out.bvh);
     out.node_slots.assign(out.bvh.nodes.size() + 2, BvhNode{});
     for (size_t i = 0; i < out.bvh.nodes.size(); ++i) out.node_slots[i + 1] = out.bvh.nodes[i];

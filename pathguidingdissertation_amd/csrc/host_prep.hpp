@@ -119,6 +119,8 @@ struct PreparedScene {
     float centre[3] = {0, 0, 0};
     float radius = 0.f;
 };
+// Device material record of a pg_material (false: unknown type).
+bool material_record(const pg_material& m, MaterialRec& r);
 std::string prepare_scene(const float* positions, const float* normals, const uint32_t* mat_idx,
                           uint32_t n_tris, const pg_material* mats, uint32_t n_mats,
                           const float* env_rgb, int32_t env_w, int32_t env_h,

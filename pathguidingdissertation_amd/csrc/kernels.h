@@ -91,6 +91,9 @@ hipError_t launch_generate_rays(const DevScene& s, const float* pxy, uint32_t n,
                                 hipStream_t stream);
 hipError_t launch_rng(uint32_t seed, uint32_t sample, const uint32_t* pixel, const uint32_t* dim, uint32_t n,
                       float* out, hipStream_t stream);
+// BSDF sample / evaluate / PDF / invert with given random numbers (15 floats out per query).
+hipError_t launch_bsdf_query(const MaterialRec& m, const float* sd, const float* u, const float* we, uint32_t n,
+                             float* out, hipStream_t stream);
 
 hipError_t launch_raygen(const DevScene& s, const PathState& ps, const PassParams& p, hipStream_t stream);
 // order (nullable): trace order of the queue (a sorted permutation from launch_ray_keys)

@@ -1125,6 +1125,42 @@ hipError_t launch_trace_shadow(const DevScene& s, const float* rays6, const floa
 #undef PG_CALL
     return hipGetLastError();
 }
+// One BSDF query per thread with given random numbers (parity aid): sample(), then
+// evaluate() and PDF() at wi_eval, invert() of the sampled direction and the flags.
+// sd: x, wo, sN, gN (12 floats; frame from sN); out: 15 floats per query.
+__global__ void k_bsdf_query(MaterialRec m, const float* sd, const float* u, const float* we, uint32_t n, float* out) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float* q = sd + 12 * (size_t)i;
+    SD s;
+    s.x = F3{q[0], q[1], q[2]};
+    s.wo = F3{q[3], q[4], q[5]};
+    s.sn = F3{q[6], q[7], q[8]};
+    s.gn = F3{q[9], q[10], q[11]};
+    s.fr = frame_from(s.sn);
+    s.alb = c3(m.albedo[0], m.albedo[1], m.albedo[2]);
+    ListRng r{{u[3 * i], u[3 * i + 1], u[3 * i + 2]}, 0};
+    C3 f;
+    float pdf = 0.f;
+    F3 wi = bsdf_sample(m, s, r, f, pdf);
+    float pe = 0.f;
+    C3 fe = bsdf_eval(m, s, F3{we[3 * i], we[3 * i + 1], we[3 * i + 2]}, pe);
+    float iu = 0.f, iv = 0.f;
+    bool inv = bsdf_invert(m, s, wi, iu, iv);
+    float* o = out + 15 * (size_t)i;
+    o[0] = wi.x; o[1] = wi.y; o[2] = wi.z;
+    o[3] = f.r; o[4] = f.g; o[5] = f.b; o[6] = pdf;
+    o[7] = fe.r; o[8] = fe.g; o[9] = fe.b; o[10] = pe;
+    o[11] = inv ? iu : 0.f; o[12] = inv ? iv : 0.f;
+    o[13] = m.specular ? 1.f : 0.f;
+    o[14] = m.two_sided ? 1.f : 0.f;
+}
+hipError_t launch_bsdf_query(const MaterialRec& m, const float* sd, const float* u, const float* we, uint32_t n,
+                             float* out, hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_bsdf_query, dim3((n + 255) / 256), dim3(256), 0, stream, m, sd, u, we, n, out);
+    return hipGetLastError();
+}
 hipError_t launch_generate_rays(const DevScene& s, const float* pxy, uint32_t n, float* rays6, hipStream_t stream) {
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(k_generate_rays, dim3((n + 255) / 256), dim3(256), 0, stream, s, pxy, n, rays6);

@@ -337,6 +337,17 @@ class Device:
                                            _fp(gg) if gg is not None else None))
         return float(loss[0]), gp, gg
 
+    def bsdf_query(self, material: np.ndarray, sd: np.ndarray, u: np.ndarray, wi_eval: np.ndarray) -> np.ndarray:
+        """BSDF sample / evaluate / PDF / invert and flags of one material (a 16-float
+        record) on the device with given random numbers: (n, 15) float32."""
+        m = PgMaterial.from_buffer_copy(np.ascontiguousarray(material, np.float32).tobytes())
+        sd = np.ascontiguousarray(sd, np.float32).reshape(-1, 12)
+        u = np.ascontiguousarray(u, np.float32).reshape(-1, 3)
+        we = np.ascontiguousarray(wi_eval, np.float32).reshape(-1, 3)
+        out = np.empty((sd.shape[0], 15), np.float32)
+        self._chk(self.lib.pg_bsdf_query(self.ctx, C.byref(m), sd.shape[0], _fp(sd), _fp(u), _fp(we), _fp(out)))
+        return out
+
     def info(self) -> dict:
         name = C.create_string_buffer(64)
         cu = C.c_int32()

@@ -449,3 +449,56 @@ def test_record_pass_samples_per_pass_keep_records(pg, device, cornell):
         assert got[0][1].shape[0] > 1000
         assert np.array_equal(got[0][0], got[1][0]), integ
         assert np.array_equal(got[0][1], got[1][1]), integ
+
+
+def test_quadtree_matches_spec_build(pg, device, cornell):
+    """Every cell's device quadtree equals, node for node, the tree built from the
+    specification (tests/quadtree_spec.py: a quadrant is refined exactly when it holds
+    more than 1% of the cell's flux and lies above depth 10; flux sums in 64-bit fixed
+    point) over the same records in the cell's Morton order: same refined quadrants in
+    both directions (an under- or over-split tree fails) and bit-equal energies.
+    Record directions keep at least 1e-6 from every depth-10 quadrant edge, so float32
+    direction round-off cannot move a record across an edge; Li = (0, g, 0) so the
+    luminance is one exactly rounded product."""
+    import quadtree_spec as qs
+    pg.Scene(device, cornell)
+    rng = np.random.default_rng(11)
+    n = 3 * 4096 + 1000
+    rec = np.zeros((n, 12), np.float32)
+    rec[:, :3] = rng.uniform(-1, 1, (n, 3))
+    rec[:, 3:6] = (0.0, 0.0, 1.0)
+    # directions: a uniform background plus three tight lobes (deep refinement)
+    uv = rng.random((n, 2))
+    lobes = np.array([[0.3, 0.7], [0.81, 0.22], [0.5004, 0.4996]])
+    which = rng.integers(-1, 3, n)
+    tight = which >= 0
+    sigma = np.array([3e-4, 1e-3, 5e-5])[which[tight]][:, None]
+    uv[tight] = lobes[which[tight]] + sigma * rng.normal(size=(tight.sum(), 2))
+    uv = np.clip(uv, 1e-4, 1 - 1e-4)
+    edge = np.abs(uv * 1024 - np.round(uv * 1024))
+    uv = np.where(edge < 1.5e-3, uv + 3e-3 / 1024, uv)  # keep off depth-10 quadrant edges
+    assert np.all(np.abs(uv * 1024 - np.round(uv * 1024)) > 1.0e-3)
+    phi, z = 2 * np.pi * uv[:, 0], 2 * uv[:, 1] - 1
+    r = np.sqrt(1 - z * z)
+    rec[:, 6:9] = np.stack([r * np.cos(phi), r * np.sin(phi), z], 1)
+    g = np.where(tight, rng.exponential(5.0, n), rng.exponential(0.2, n)).astype(np.float32)
+    rec[:, 10] = g
+    device.upload_records(rec)
+    info = device.cache_build(cell_size=4096, qt_nodes=1024)
+    _, pos, qt = device.cache_export(qt_nodes=1024)
+    qt = qt.reshape(info["cells"], 1024, 8)
+    order = pos[:, 3].view(np.uint32)
+    lum = np.float32(0.7152) * rec[order, 10]  # (0.2126 * 0 + 0.7152 * g) + 0.0722 * 0
+    u_s, v_s = uv[order, 0], uv[order, 1]
+    flux = qs.fixed_flux(lum)
+    total_nodes = 0
+    deepest = 0
+    for c in range(info["cells"]):
+        sel = slice(c * 4096, min((c + 1) * 4096, n))
+        spec = qs.build(u_s[sel], v_s[sel], flux[sel])
+        total_nodes += qs.compare(spec, qt[c])
+
+        def depth(t, d=0):
+            return max([d] + [depth(k, d + 1) for k in t[1] if k is not None])
+        deepest = max(deepest, depth(spec))
+    assert info["cells"] == 4 and total_nodes > 40 and deepest == qs.MAX_DEPTH - 1
