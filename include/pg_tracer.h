@@ -168,6 +168,14 @@ pg_status pg_bvh_export(pg_ctx* ctx, float* nodes, int32_t* tri_index);
 /* ------------------------------------------------------------------ queries (tests) */
 /* BVHNode::traverse (Geometry.h:448-512) for a batch of rays. */
 pg_status pg_trace_closest(pg_ctx* ctx, const pg_ray* rays, uint32_t n, pg_hit* hits);
+/* No counterpart in the reference. For callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc): pg_trace_closest without any allocation. Rays and hits stage through
+ * one fixed device array per GPU (64 B per ray, at most 2^18 = 262,144 rays per call,
+ * 16 MiB, held until the process ends); more rays are PG_ERR_INVALID. Every call on a
+ * GPU shares that GPU's array: calls that overlap on one GPU (several contexts or
+ * host threads) race and must be serialised by the caller.                       */
+pg_status pg_trace_closest_noalloc(pg_ctx* ctx, const pg_ray* rays, uint32_t n, pg_hit* hits);
 /* BVHNode::traverseVisible (Geometry.h:514-567): visible[i] = 1 when no hit in
  * (1e-4, max_t[i]).                                                            */
 pg_status pg_trace_shadow(pg_ctx* ctx, const pg_ray* rays, const float* max_t, uint32_t n,

@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cfloat>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -18,8 +19,33 @@
 
 using namespace pg;
 
+static __device__ __align__(256) float scratch[(size_t)(1u << 18) * 16];
+
 namespace {
 thread_local std::string g_last_error;
+
+// Staging for pg_trace_closest_noalloc: per ray the ray (6 floats), the closest-hit
+// arrays the traversal kernel writes (id, t, alpha, beta, gamma) and the packed
+// pg_hit records (5 words): 16 words, 64 B. One array per GPU, held until the
+// process ends; its device address comes from hipGetSymbolAddress (the host-side
+// name is not a device address).
+constexpr uint32_t kNoAllocMaxRays = 1u << 18;
+static_assert(sizeof(scratch) == (size_t)kNoAllocMaxRays * 16 * sizeof(float), "64 B of scratch per ray");
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+float* g_scratch[kMaxDevices] = {};
+
+// Device address of this GPU's scratch array (the device is current); nullptr on error.
+float* scratch_for_device(int device) {
+    if (device < 0 || device >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch[device]) {
+        void* ptr = nullptr;
+        if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess) return nullptr;
+        g_scratch[device] = (float*)ptr;
+    }
+    return g_scratch[device];
+}
 
 template <class T>
 hipError_t dalloc(T** p, size_t count) {
@@ -42,6 +68,7 @@ struct TimedLaunch {
 struct pg_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    float* noalloc_scratch = nullptr;  // this device's scratch array (pg_trace_closest_noalloc)
     std::string err;
     int cu_count = 0;
     std::string name;
@@ -300,6 +327,10 @@ pg_status pg_create(pg_ctx** out, int device) {
     if (dalloc(&c->d_stats, 8) != hipSuccess || dalloc(&c->d_counters, 8) != hipSuccess) {
         pg_destroy(c); g_last_error = "device allocation failed"; return PG_ERR_NOMEM;
     }
+    c->noalloc_scratch = scratch_for_device(device);
+    if (!c->noalloc_scratch) {
+        pg_destroy(c); g_last_error = "hipGetSymbolAddress of the scratch array failed"; return PG_ERR_DEVICE;
+    }
     (void)hipMemset(c->d_stats, 0, 8 * sizeof(unsigned long long));
     (void)hipMemset(c->d_counters, 0, 8 * sizeof(uint32_t));
     *out = c;
@@ -498,6 +529,32 @@ pg_status pg_trace_closest(pg_ctx* c, const pg_ray* rays, uint32_t n, pg_hit* hi
     } while (0);
     dfree(dr); dfree(did); dfree(dt); dfree(da); dfree(db); dfree(dg);
     return st;
+}
+
+pg_status pg_trace_closest_noalloc(pg_ctx* c, const pg_ray* rays, uint32_t n, pg_hit* hits) {
+    PG_CHECK_CTX(c);
+    if (!c->has_scene) return c->fail(PG_ERR_STATE, "no scene uploaded");
+    if (n == 0) return PG_OK;
+    if (!rays || !hits) return c->fail(PG_ERR_INVALID, "null array");
+    if (n > kNoAllocMaxRays)
+        return c->fail(PG_ERR_INVALID, "pg_trace_closest_noalloc: more than 2^18 rays (its scratch limit); "
+                                       "split the batch or call pg_trace_closest");
+    float* base = c->noalloc_scratch;
+    if (!base) return c->fail(PG_ERR_STATE, "no scratch array on this device");
+    float* dr = base;                                  // 6n: rays
+    uint32_t* did = (uint32_t*)(base + (size_t)6 * n); // n: ids
+    float* dt = base + (size_t)7 * n;                  // 4n: t, alpha, beta, gamma
+    float* da = dt + n;
+    float* db = da + n;
+    float* dg = db + n;
+    pg_hit* dh = (pg_hit*)(base + (size_t)11 * n);     // 5n: packed hits
+    PG_TRY(c, hipMemcpyAsync(dr, rays, (size_t)n * sizeof(pg_ray), hipMemcpyHostToDevice, c->stream), "copy rays");
+    PG_TRY(c, launch_trace_closest(c->dev, dr, n, did, dt, da, db, dg, (int)c->host.bvh.max_depth, c->counting,
+                                   c->d_stats, c->stream), "trace_closest");
+    PG_TRY(c, launch_pack_hits(did, dt, da, db, dg, n, (uint32_t*)dh, c->stream), "pack hits");
+    PG_TRY(c, hipMemcpyAsync(hits, dh, (size_t)n * sizeof(pg_hit), hipMemcpyDeviceToHost, c->stream), "copy hits");
+    PG_TRY(c, hipStreamSynchronize(c->stream), "sync");
+    return PG_OK;
 }
 
 pg_status pg_trace_shadow(pg_ctx* c, const pg_ray* rays, const float* max_t, uint32_t n, uint8_t* visible) {

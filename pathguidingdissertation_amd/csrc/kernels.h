@@ -87,6 +87,8 @@ hipError_t launch_trace_closest(const DevScene& s, const float* rays6, uint32_t 
                                 unsigned long long* stats, hipStream_t stream);
 hipError_t launch_trace_shadow(const DevScene& s, const float* rays6, const float* max_t, uint32_t n,
                                uint8_t* vis, int stack, hipStream_t stream);
+hipError_t launch_pack_hits(const uint32_t* id, const float* t, const float* a, const float* b, const float* g,
+                            uint32_t n, uint32_t* out, hipStream_t stream);
 hipError_t launch_generate_rays(const DevScene& s, const float* pxy, uint32_t n, float* rays6,
                                 hipStream_t stream);
 hipError_t launch_rng(uint32_t seed, uint32_t sample, const uint32_t* pixel, const uint32_t* dim, uint32_t n,

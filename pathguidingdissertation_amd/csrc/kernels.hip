@@ -1161,6 +1161,24 @@ hipError_t launch_bsdf_query(const MaterialRec& m, const float* sd, const float*
     hipLaunchKernelGGL(k_bsdf_query, dim3((n + 255) / 256), dim3(256), 0, stream, m, sd, u, we, n, out);
     return hipGetLastError();
 }
+// Closest-hit arrays (id, t, alpha, beta, gamma) to packed pg_hit records (5 words each).
+__global__ void k_pack_hits(const uint32_t* id, const float* t, const float* a, const float* b, const float* g,
+                            uint32_t n, uint32_t* out) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t* o = out + 5 * (size_t)i;
+    o[0] = id[i];
+    o[1] = __float_as_uint(t[i]);
+    o[2] = __float_as_uint(a[i]);
+    o[3] = __float_as_uint(b[i]);
+    o[4] = __float_as_uint(g[i]);
+}
+hipError_t launch_pack_hits(const uint32_t* id, const float* t, const float* a, const float* b, const float* g,
+                            uint32_t n, uint32_t* out, hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_pack_hits, dim3((n + 255) / 256), dim3(256), 0, stream, id, t, a, b, g, n, out);
+    return hipGetLastError();
+}
 hipError_t launch_generate_rays(const DevScene& s, const float* pxy, uint32_t n, float* rays6, hipStream_t stream) {
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(k_generate_rays, dim3((n + 255) / 256), dim3(256), 0, stream, s, pxy, n, rays6);
