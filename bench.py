@@ -1,14 +1,16 @@
 """Benchmark: unguided wavefront path tracing of the ~1.09M-triangle procedural interior
-(1280x720, max depth 12, env map through an alias table) on one MI355X.
+(1280x720, 64 SPP, max depth 12, env map through an alias table) on one MI355X.
 
-A step is one sample per pixel over the full frame (921,600 paths). The default run
-times 64 steps (the workload's 64 SPP, traced together as one wavefront pass of 64
-samples per pixel, the library's automatic samples_per_pass) after 4 warmup steps and
-prints one JSON line: Mrays/s (closest-hit + shadow rays traced), the roofline of the dominant
-kernel (extend: closest-hit traversal) from HIP-event timings taken inside the timed
-region, and the CPU oracle timed on the host cores on a bounded sample. A second part
-(skipped with --no-ttr) measures time-to-relMSE of the unguided, quadtree-guided and
-learned-guided renderers against a 4096-SPP reference of the same frame.
+A step is one full frame of the workload: 64 samples per pixel over 1280x720
+(58,982,400 camera paths), traced as one wavefront pass of 64 samples per pixel (the
+library's automatic samples_per_pass). The run times --steps frames after --warmup
+frames and prints one JSON line: Mrays/s (closest-hit + shadow rays traced), the
+roofline of the dominant kernel (extend: closest-hit traversal) from HIP-event timings
+taken inside the timed region, and the CPU oracle timed on the host cores on a bounded
+sample (all threads of the host's share and one thread). Further parts: the batched
+views of the config-5 family at its specification (48 views, 32 SPP; --batch-views 0
+skips it) and time-to-relMSE of the unguided, quadtree-guided and learned-guided
+renderers against a 4096-SPP reference (--no-ttr skips it).
 """
 from __future__ import annotations
 
@@ -264,7 +266,7 @@ def time_to_relmse(pg, dev, sc, data, max_depth, ref_spp, cap_s, log, target=0.0
     # per query forward; a training sample adds the input and weight gradients (3x)
     fwd = 2 * (80 * 64 + 64 * 64 + 64 * 64 + 64 * 41)
     batch = 1 << 18
-    train_ms = float(info["train_ms"])
+    train_ms = float(info["train_ms"])  # GPU time per Adam step
     mlp = {"forward_flop_per_query": fwd, "train_flop_per_sample": 3 * fwd, "batch": batch,
            "train_ms_per_step": round(train_ms, 3),
            "train_tflops": round(3 * fwd * batch / (train_ms * 1e-3) / 1e12, 2) if train_ms > 0 else None,
@@ -350,6 +352,21 @@ def batch_study(pg, dev, n_views: int, spp: int, max_depth: int, log) -> dict:
     s1 = b.stats()
     rays = (s1["closest_rays"] - s0["closest_rays"]) + (s1["shadow_rays"] - s0["shadow_rays"])
     log(f"batch: {n_views} views, {tris} tris, {pixels} pixels, {wall * 1e3 / spp:.2f} ms/SPP, config {best}")
+    # the same launch configuration with each pass replayed as one captured hipGraph
+    graph_cfg = dict(best, use_graph=1)
+    b.configure(**graph_cfg)
+    b.clear()
+    b.render(spp, max_depth=max_depth)  # captures the pass graph (and warms it)
+    b.clear()
+    t0 = time.perf_counter()
+    b.render(spp, max_depth=max_depth)
+    gr_wall = time.perf_counter() - t0
+    gs = b.stats()
+    graph = {"ms_per_spp": round(gr_wall * 1e3 / spp, 3), "graph_nodes": gs["graph_nodes"],
+             "config": graph_cfg,
+             "vs_tuned": round(wall / gr_wall, 3)}
+    log(f"batch graph replay: {graph['ms_per_spp']} ms/SPP, {gs['graph_nodes']} graph nodes")
+    b.configure(**best)
     # the same views guided by per-scene learned networks (trained online on each scene's
     # own camera first: 16 guided record passes with one Adam step each)
     t0 = time.perf_counter()
@@ -374,7 +391,8 @@ def batch_study(pg, dev, n_views: int, spp: int, max_depth: int, log) -> dict:
               "mrays_per_s": round(grays / gwall / 1e6, 2), "train_s": round(train, 1),
               "guides": "one network per scene, trained online on the scene's own camera (16 SPP)"}
     return {"views": n_views, "scenes_tris": tris, "pixels": pixels, "spp": spp, "max_depth": max_depth,
-            "learned_guided": guided,
+            "resolutions": "drawn from {128x128, 256x512, 640x360, 1280x720} with a seeded RNG",
+            "learned_guided": guided, "graph_replay": graph,
             "ms_per_spp": round(wall * 1e3 / spp, 3), "samples_per_s": round(pixels * spp / wall, 1),
             "mrays_per_s": round(rays / wall / 1e6, 2), "gpu_ms_per_spp": round((s1["gpu_ms"] - s0["gpu_ms"]) / spp, 3),
             "graph_nodes": s1["graph_nodes"], "chunks": s1["chunks"],
@@ -382,17 +400,39 @@ def batch_study(pg, dev, n_views: int, spp: int, max_depth: int, log) -> dict:
             "tune_s": round(tune, 1)}
 
 
+def host_cpu() -> dict:
+    """The host's CPU: model string, logical cores (std::thread::hardware_concurrency's
+    count, os.cpu_count) and the cores this process may run on (nproc)."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        nproc = len(os.sched_getaffinity(0))
+    except AttributeError:
+        nproc = os.cpu_count() or 1
+    return {"cpu_model": model, "hardware_concurrency": os.cpu_count() or 1, "nproc": nproc}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=64)
-    ap.add_argument("--warmup", type=int, default=4)
+    ap.add_argument("--steps", type=int, default=8, help="timed frames (64 SPP each)")
+    ap.add_argument("--warmup", type=int, default=1, help="untimed frames before the timed ones")
     ap.add_argument("--config", type=int, default=int(os.environ.get("PG_BENCH_CONFIG", "2")))
-    ap.add_argument("--cpu-spp", type=int, default=int(os.environ.get("PG_BENCH_CPU_SPP", "40")))
+    ap.add_argument("--cpu-spp", type=int, default=int(os.environ.get("PG_BENCH_CPU_SPP", "32")),
+                    help="SPP of the CPU oracle's multi-threaded sample")
+    ap.add_argument("--cpu-spp-1t", type=int, default=int(os.environ.get("PG_BENCH_CPU_SPP_1T", "2")),
+                    help="SPP of the CPU oracle's single-thread sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-ttr", action="store_true", help="skip the time-to-relMSE study")
-    ap.add_argument("--batch-views", type=int, default=int(os.environ.get("PG_BENCH_BATCH_VIEWS", "16")),
+    ap.add_argument("--batch-views", type=int, default=int(os.environ.get("PG_BENCH_BATCH_VIEWS", "48")),
                     help="views of the batched-render measurement (0 skips it)")
+    ap.add_argument("--batch-spp", type=int, default=int(os.environ.get("PG_BENCH_BATCH_SPP", "32")))
     ap.add_argument("--ttr-cap", type=float, default=float(os.environ.get("PG_BENCH_TTR_CAP", "30")),
                     help="wall-clock seconds per method in the time-to-relMSE study")
     ap.add_argument("--ttr-ref-spp", type=int, default=int(os.environ.get("PG_BENCH_TTR_REF", "4096")))
@@ -457,13 +497,15 @@ def main() -> int:
              "tuned_ms_per_spp": round(tuned_ms, 3)}
     print(f"bench: launch shape {shape}", file=sys.stderr, flush=True)
 
+    frame_spp = 64  # the workload's SPP: one step is one whole frame
     rt.clear()
-    if a.warmup:
-        rt.render(a.warmup)
+    for _ in range(a.warmup):
+        rt.render(frame_spp)
     rt.clear()
     rt.reset_stats()
     t0 = time.perf_counter()
-    rt.render(a.steps)  # the call returns after the stream has drained
+    for _ in range(a.steps):
+        rt.render(frame_spp)  # each call returns after the stream has drained
     t1 = time.perf_counter()
     st = rt.stats()
     wall = t1 - t0
@@ -492,19 +534,30 @@ def main() -> int:
     cpu = None
     if not a.no_cpu:
         from oracle.pg_oracle import OracleScene
-        threads = min(os.cpu_count() or 1, 16)
+        host = host_cpu()
+        # one GPU's share of the box's host cores is 16 threads, whatever the host shows
+        threads = min(host["hardware_concurrency"], host["nproc"], 16)
         osc = OracleScene(data)
         film, cst = osc.render(a.cpu_spp, seed=1, max_depth=max_depth, accum_f64=False, threads=threads, env_mode=1)
         cpu_rays = cst["closest_rays"] + cst["shadow_rays"]
+        _, c1 = osc.render(a.cpu_spp_1t, seed=1, max_depth=max_depth, accum_f64=False, threads=1, env_mode=1)
+        rays_1t = c1["closest_rays"] + c1["shadow_rays"]
         cpu = {"value": round(cpu_rays / cst["seconds"] / 1e6, 3), "unit": "Mrays/s", "cores": threads,
                "kind": "port",
                "sample": f"{a.cpu_spp} SPP of the same {W}x{H} frame ({cst['paths']} paths, {cpu_rays} rays, "
-                         f"{cst['seconds']:.1f} s); oracle restatement with std::thread tiles (Renderer.h:648-747)",
-               "samples_per_s": round(cst["paths"] / cst["seconds"], 1)}
+                         f"{cst['seconds']:.1f} s) on {threads} threads; oracle restatement with std::thread "
+                         f"tiles (Renderer.h:648-747)",
+               "samples_per_s": round(cst["paths"] / cst["seconds"], 1),
+               "single_thread": {"value": round(rays_1t / c1["seconds"] / 1e6, 3), "unit": "Mrays/s",
+                                 "sample": f"{a.cpu_spp_1t} SPP of the same frame ({c1['paths']} paths, "
+                                           f"{rays_1t} rays, {c1['seconds']:.1f} s)"},
+               **host,
+               "threads_note": "16 threads: one GPU's share of the GPU box's host cores"}
 
     batch = None
     if a.batch_views > 0:
-        batch = batch_study(pg, dev, a.batch_views, 8, max_depth, lambda m: print(m, file=sys.stderr, flush=True))
+        batch = batch_study(pg, dev, a.batch_views, a.batch_spp, max_depth,
+                            lambda m: print(m, file=sys.stderr, flush=True))
 
     ttr = None
     if not a.no_ttr:
@@ -531,8 +584,10 @@ def main() -> int:
         "dtype": "f32",
         "data": "synthetic (procedural scene, seed 0)",
         "config": {"workload": f"unguided wavefront path tracer, config {a.config}: {data.n_tris} tris, "
-                               f"{W}x{H}, {a.steps} SPP, max depth {max_depth}, env map alias table",
-                   "tris": data.n_tris, "width": W, "height": H, "spp": a.steps, "max_depth": max_depth},
+                               f"{W}x{H}, {frame_spp} SPP per step (one frame), max depth {max_depth}, "
+                               f"env map alias table",
+                   "tris": data.n_tris, "width": W, "height": H, "spp_per_step": frame_spp,
+                   "paths_per_step": W * H * frame_spp, "max_depth": max_depth},
         "samples_per_s": round(st["paths"] / wall, 1),
         "launch_shape": shape,
         "closest_rays": st["closest_rays"],

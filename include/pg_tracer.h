@@ -286,21 +286,34 @@ typedef struct pg_mlp_config {
     float weight_clip;        /* clip of the mean-normalised sample weight (100)    */
     uint64_t act_budget;      /* bytes of activation rows per chunk (512 MiB)       */
     uint32_t nan_step;        /* test hook: poison the gradient at this step (0 off)*/
-    uint32_t reserved[7];
+    float diverge_rise;       /* divergence on rising loss: the loss smoothed over
+                                 steps (newest weighted 1/4) more than this many
+                                 nats above its best value (0: 2; < 0: off) ...  */
+    uint32_t diverge_patience; /* ... for this many consecutive steps (0: 8)     */
+    uint32_t rise_test_step;  /* test hook: from this step on the detector sees the
+                                 loss raised by 10 nats per step (0 off)         */
+    uint32_t reserved[4];
 } pg_mlp_config;
+
+/* Why a learned guide was marked diverged (pg_mlp_info.diverge_reason). A diverged
+ * guide is not trained further, and PG_INTEGRATOR_GUIDED_MLP renders with the
+ * quadtree guide when pg_cache_build has built one (unguided otherwise).          */
+enum { PG_MLP_DIVERGED_NONE = 0, PG_MLP_DIVERGED_NONFINITE = 1, PG_MLP_DIVERGED_RISING_LOSS = 2 };
 
 typedef struct pg_mlp_info {
     uint32_t ready;           /* pg_mlp_init done                                   */
-    uint32_t diverged;        /* a non-finite gradient was seen: rendering falls back */
+    uint32_t diverged;        /* a non-finite loss or gradient, or a rising loss, was
+                                 seen: rendering falls back (diverge_reason)       */
     uint32_t step;            /* Adam steps taken                                   */
     uint32_t n_params;        /* network parameters (17648 incl. padding)           */
     uint64_t n_grid_params;   /* hash-table parameters (16 * 2^log2_table * 2)      */
     float last_loss;          /* mean loss of the last step's batch                 */
     float w_scale;            /* 1 / mean record weight                             */
     uint32_t chunks;          /* chunks per step under the activation budget        */
-    float train_ms;           /* GPU time of the last pg_mlp_train call             */
+    float train_ms;           /* GPU time per Adam step of the last pg_mlp_train call */
     uint32_t fallbacks;       /* guided passes rendered with a fallback guide       */
-    uint32_t reserved[5];
+    uint32_t diverge_reason;  /* PG_MLP_DIVERGED_*                                  */
+    uint32_t reserved[4];
 } pg_mlp_info;
 
 pg_mlp_config pg_mlp_config_default(void);

@@ -1570,6 +1570,8 @@ int orc_render(orc_scene* os, const orc_render_params* p, double* film, orc_rend
     const int W = (int)sc.width, H = (int)sc.height;
     const int nt = nthreads(p->n_threads);
     const unsigned tile = 32;                                                  // Renderer.h:654
+    const bool cropped = p->crop[2] > 0 && p->crop[3] > 0;
+    const int cx0 = p->crop[0], cy0 = p->crop[1], cx1 = p->crop[0] + p->crop[2], cy1 = p->crop[1] + p->crop[3];
     const unsigned tx = (W + tile - 1) / tile, ty = (H + tile - 1) / tile, ntiles = tx * ty;
     std::vector<float> ffilm;
     if (!p->accum_f64) ffilm.assign((size_t)W * H * 3, 0.f);
@@ -1596,9 +1598,14 @@ int orc_render(orc_scene* os, const orc_render_params* p, double* film, orc_rend
                     unsigned x0 = (tid % tx) * tile, y0 = (tid / tx) * tile;
                     unsigned x1 = std::min(x0 + tile - 1u, (unsigned)W - 1u);
                     unsigned y1 = std::min(y0 + tile - 1u, (unsigned)H - 1u);
+                    if (cropped && ((int)x1 < cx0 || (int)x0 >= cx1 || (int)y1 < cy0 || (int)y0 >= cy1)) continue;
                     for (unsigned y = y0; y <= y1; ++y)
                         for (unsigned x = x0; x <= x1; ++x) {
                             uint32_t pix = y * W + x;
+                            if (cropped && ((int)x < cx0 || (int)x >= cx1 || (int)y < cy0 || (int)y >= cy1)) {
+                                slots[pix].ok = false;
+                                continue;
+                            }
                             cs.key = pathKey(p->seed, sample, pix); cs.dim = 0;
                             float px = x + smp->next();
                             float py = y + smp->next();
@@ -1623,7 +1630,12 @@ int orc_render(orc_scene* os, const orc_render_params* p, double* film, orc_rend
         }
         for (auto& x : th) x.join();
         // Box-filter splat (Imaging.h:186-217) in pixel order; deterministic.
+        const int sx0 = cropped ? cx0 : 0, sy0 = cropped ? cy0 : 0;
+        const int sx1 = cropped ? std::min(cx1, W) : W, sy1 = cropped ? std::min(cy1, H) : H;
         for (int pix = 0; pix < W * H; ++pix) {
+            const int px = pix % W, py = pix / W;
+            if (py < sy0 || py >= sy1) { pix = (py < sy0 ? sy0 : H) * W - 1; continue; }
+            if (px < sx0 || px >= sx1) continue;
             const Slot& sl = slots[pix];
             if (!sl.ok) continue;
             splatBox(W, H, sl.px, sl.py, sl.L, [&](int id, const C3& add) {

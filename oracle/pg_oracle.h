@@ -81,6 +81,8 @@ typedef struct {
     int32_t env_visibility; /* 0: shadow ray to sceneCentre + wi * sceneRadius
                                   (Renderer.h:331-332); 1: along wi out of the
                                   bounding sphere                               */
+    int32_t crop[4];       /* x0, y0, width, height of the pixels traced (the
+                              rest of the film stays untouched); width 0: all  */
 } orc_render_params;
 
 typedef struct {

@@ -35,7 +35,7 @@ class RenderParams(C.Structure):
     _fields_ = [("spp", C.c_uint32), ("spp_offset", C.c_uint32), ("seed", C.c_uint32), ("max_depth", C.c_uint32),
                 ("rr_depth", C.c_uint32), ("integrator", C.c_int32), ("sampler", C.c_int32),
                 ("accum_f64", C.c_int32), ("n_threads", C.c_int32), ("record", C.c_int32),
-                ("env_mode", C.c_int32), ("env_visibility", C.c_int32)]
+                ("env_mode", C.c_int32), ("env_visibility", C.c_int32), ("crop", C.c_int32 * 4)]
 
 
 class RenderStats(C.Structure):
@@ -266,9 +266,12 @@ class OracleScene:
         return ids, t
 
     def render(self, spp, seed=1, spp_offset=0, max_depth=8, rr_depth=3, integrator=0, sampler=0, accum_f64=True,
-               threads=0, record=False, env_mode=1, env_visibility=0):
+               threads=0, record=False, env_mode=1, env_visibility=0, crop=None):
+        """crop: (x0, y0, width, height) of the pixels traced; the film is full size and
+        zero elsewhere. Each pixel keeps its own random stream, so a crop equals the same
+        pixels of a full render."""
         p = RenderParams(spp, spp_offset, seed, max_depth, rr_depth, integrator, sampler, 1 if accum_f64 else 0,
-                         threads, 1 if record else 0, env_mode, env_visibility)
+                         threads, 1 if record else 0, env_mode, env_visibility, (C.c_int32 * 4)(*(crop or (0, 0, 0, 0))))
         film = np.zeros((self.height, self.width, 3), np.float64)
         st = RenderStats()
         lib().orc_render(self.h, C.byref(p), film.ctypes.data_as(_D), C.byref(st))

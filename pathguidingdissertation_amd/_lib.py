@@ -61,14 +61,15 @@ class PgMlpConfig(C.Structure):
     _fields_ = [("seed", C.c_uint32), ("log2_table", C.c_uint32), ("res_min", C.c_float), ("res_max", C.c_float),
                 ("lr", C.c_float), ("beta1", C.c_float), ("beta2", C.c_float), ("eps", C.c_float),
                 ("batch", C.c_uint32), ("weight_clip", C.c_float), ("act_budget", C.c_uint64),
-                ("nan_step", C.c_uint32), ("reserved", C.c_uint32 * 7)]
+                ("nan_step", C.c_uint32), ("diverge_rise", C.c_float), ("diverge_patience", C.c_uint32),
+                ("rise_test_step", C.c_uint32), ("reserved", C.c_uint32 * 4)]
 
 
 class PgMlpInfo(C.Structure):
     _fields_ = [("ready", C.c_uint32), ("diverged", C.c_uint32), ("step", C.c_uint32), ("n_params", C.c_uint32),
                 ("n_grid_params", C.c_uint64), ("last_loss", C.c_float), ("w_scale", C.c_float),
                 ("chunks", C.c_uint32), ("train_ms", C.c_float), ("fallbacks", C.c_uint32),
-                ("reserved", C.c_uint32 * 5)]
+                ("diverge_reason", C.c_uint32), ("reserved", C.c_uint32 * 4)]
 
 
 class PgMlpSchedule(C.Structure):

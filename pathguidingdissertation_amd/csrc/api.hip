@@ -163,6 +163,9 @@ struct pg_ctx {
         double* sum_part = nullptr;
         uint32_t step = 0, chunks = 1, fallbacks = 0;
         float last_loss = 0.f, w_scale = 1.f, train_ms = 0.f;
+        // rising-loss detector: smoothed loss, its best value, steps above best + rise
+        float loss_ema = 0.f, loss_best = 0.f;
+        uint32_t loss_seen = 0, rise_run = 0, diverge_reason = 0;
         uint64_t w_scale_gen = ~0ull;
         uint32_t window_passes = 0, online_passes = 0, record_passes = 0;
     } mlp;
@@ -1615,11 +1618,35 @@ pg_status pg_mlp_train(pg_ctx* c, uint32_t steps, float* losses) {
     float ms = 0.f;
     (void)hipEventElapsedTime(&ms, e0, e1);
     c->pool_used = 0;
-    m.train_ms = ms;
+    m.train_ms = ms / (float)steps;  // per Adam step
     m.last_loss = hl[steps - 1];
     for (float l : hl)
         if (!std::isfinite(l)) status |= 4u;
-    if (status) m.diverged = true;
+    if (status && !m.diverged) {
+        m.diverged = true;
+        m.diverge_reason = PG_MLP_DIVERGED_NONFINITE;
+    }
+    // rising loss: the loss smoothed over steps (weight 1/4 on the newest) staying
+    // more than diverge_rise nats above its best value for diverge_patience steps
+    const float rise = m.cfg.diverge_rise == 0.f ? 2.f : m.cfg.diverge_rise;
+    const uint32_t patience = m.cfg.diverge_patience ? m.cfg.diverge_patience : 8u;
+    for (uint32_t k = 0; k < steps && !m.diverged && rise > 0.f; ++k) {
+        float l = hl[k];
+        const uint32_t t = m.step - steps + 1 + k;  // Adam step that produced hl[k]
+        if (m.cfg.rise_test_step && t >= m.cfg.rise_test_step) l += 10.f * (float)(t - m.cfg.rise_test_step + 1);
+        m.loss_ema = m.loss_seen == 0 ? l : 0.75f * m.loss_ema + 0.25f * l;
+        ++m.loss_seen;
+        if (m.loss_seen <= patience || m.loss_ema < m.loss_best) {
+            if (m.loss_seen == 1 || m.loss_ema < m.loss_best) m.loss_best = m.loss_ema;
+            m.rise_run = 0;
+            continue;
+        }
+        m.rise_run = m.loss_ema > m.loss_best + rise ? m.rise_run + 1 : 0;
+        if (m.rise_run >= patience) {
+            m.diverged = true;
+            m.diverge_reason = PG_MLP_DIVERGED_RISING_LOSS;
+        }
+    }
     if (losses) std::memcpy(losses, hl.data(), steps * 4);
     return PG_OK;
 }
@@ -1687,6 +1714,7 @@ pg_status pg_mlp_info_read(pg_ctx* c, pg_mlp_info* info) {
     info->chunks = m.chunks;
     info->train_ms = m.train_ms;
     info->fallbacks = m.fallbacks;
+    info->diverge_reason = m.diverge_reason;
     return PG_OK;
 }
 

@@ -262,21 +262,44 @@ def test_training_reduces_loss(pg, device):
     assert late < losses[:5].mean(), (losses[:5], losses[-10:])
 
 
-def test_divergence_falls_back(pg, device):
+@pytest.mark.parametrize("cause", ["nonfinite", "rising_loss"])
+def test_divergence_falls_back_to_the_quadtree_guide(pg, device, cause):
+    """A learned guide whose training diverges (a non-finite gradient, or a loss that
+    rises 2 nats above its smoothed best for 8 steps, forced from the host through
+    the test hooks) is not trained further by online rendering, and learned-guided
+    rendering runs
+    the quadtree guide of the cache: the film equals, bit for bit, a quadtree-guided
+    render with the same seed, and each such pass counts as a fallback."""
     sc = _record(pg, device, 101, 1)
-    device.mlp_init(seed=7, batch=1 << 14, nan_step=3)
-    device.mlp_train(5)
+    device.cache_build(cell_size=4096)
+    hook = dict(nan_step=3) if cause == "nonfinite" else dict(rise_test_step=6)
+    device.mlp_init(seed=7, batch=1 << 14, **hook)
+    device.mlp_train(20)
     info = device.mlp_info()
     assert info["diverged"] == 1
-    rt = pg.RayTracer(sc, seed=4, max_depth=12, integrator=3)
-    rt.clear()
-    rt.render(2)
-    img = rt.image()
-    assert np.isfinite(img).all() and img.mean() > 0
-    assert device.mlp_info()["fallbacks"] == 2
+    assert info["diverge_reason"] == (1 if cause == "nonfinite" else 2)
+    films = []
+    for integrator in (3, 2):
+        rt = pg.RayTracer(sc, seed=4, max_depth=12, integrator=integrator, guide_prob=0.5)
+        rt.clear()
+        rt.render(3)
+        films.append(rt.film()[0])
+    assert np.isfinite(films[0]).all() and films[0].mean() > 0
+    assert np.array_equal(films[0].view(np.uint32), films[1].view(np.uint32))
+    assert device.mlp_info()["fallbacks"] == 3
     # re-initialising clears the flag
     device.mlp_init(seed=7, batch=1 << 14)
-    assert device.mlp_info()["diverged"] == 0
+    assert device.mlp_info()["diverged"] == 0 and device.mlp_info()["diverge_reason"] == 0
+
+
+def test_rising_loss_detector_stays_quiet_on_normal_training(pg, device):
+    """The rising-loss detector does not fire while a guide trains normally (its loss
+    falls), with the default 2-nat rise and 8-step patience."""
+    _record(pg, device, 101, 2)
+    device.mlp_init(seed=7)
+    losses = device.mlp_train(60)
+    info = device.mlp_info()
+    assert np.isfinite(losses).all() and info["diverged"] == 0 and info["diverge_reason"] == 0
 
 
 def test_training_needs_record_pass_data(pg, device, cornell):
