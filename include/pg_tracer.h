@@ -159,6 +159,17 @@ pg_status pg_scene_upload(pg_ctx* ctx, const float* positions, const float* norm
 pg_status pg_scene_textures(pg_ctx* ctx, const float* uv, const int32_t* material_texture, uint32_t n_materials,
                             const float* texels, const uint64_t* texel_offset, const int32_t* tex_width,
                             const int32_t* tex_height, uint32_t n_textures);
+/* Where pg_scene_upload builds the BVH: PG_BVH_HOST (default; threads of the host)
+ * or PG_BVH_GPU (the device, level by level). Both give the reference's topology,
+ * node numbering and triangle order (Geometry.h:274-446) bit for bit.           */
+enum { PG_BVH_HOST = 0, PG_BVH_GPU = 1 };
+pg_status pg_set_bvh_builder(pg_ctx* ctx, int32_t builder);
+/* BVHNode::build (Geometry.h:419-446) on the GPU for n_tris triangles (9 floats of
+ * positions each): nodes_out receives the node count x 8 floats (min.xyz, leftFirst
+ * bits, max.xyz, used bits; at most 2 n_tris - 1 nodes), tri_index_out the
+ * triangleIndexes order, build_ms the GPU time. Same output as pg_bvh_build_host.  */
+pg_status pg_bvh_build_gpu(pg_ctx* ctx, const float* positions, uint32_t n_tris, uint32_t* n_nodes, float* nodes_out,
+                           int32_t* tri_index_out, double* build_ms);
 /* BVH statistics: node count, maximum leaf depth, build milliseconds. */
 pg_status pg_bvh_info(pg_ctx* ctx, uint32_t* n_nodes, uint32_t* max_depth, double* build_ms);
 /* Export the topology as built (8 floats per node: min.xyz, leftFirst bits,

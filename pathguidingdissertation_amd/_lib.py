@@ -111,6 +111,8 @@ SIGNATURES = {
     "pg_bvh_export": (C.c_int32, [_P, _F, _I32]),
     "pg_trace_closest": (C.c_int32, [_P, _F, C.c_uint32, _P]),
     "pg_trace_closest_noalloc": (C.c_int32, [_P, _F, C.c_uint32, _P]),
+    "pg_bvh_build_gpu": (C.c_int32, [_P, _F, C.c_uint32, _U32, _F, _I32, C.POINTER(C.c_double)]),
+    "pg_set_bvh_builder": (C.c_int32, [_P, C.c_int32]),
     "pg_trace_shadow": (C.c_int32, [_P, _F, _F, C.c_uint32, _U8]),
     "pg_generate_rays": (C.c_int32, [_P, _F, C.c_uint32, _F]),
     "pg_rng_values": (C.c_int32, [_P, C.c_uint32, C.c_uint32, _U32, _U32, C.c_uint32, _F]),

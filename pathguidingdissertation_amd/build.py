@@ -20,7 +20,7 @@ LIB = PKG / "libpgtrace.so"
 EXAMPLE = PKG / "pgd_render"  # example host program over the C++ mirror
 
 HOST_SOURCES = ["host_prep.cpp", "scenes.cpp", "metrics.cpp", "scene_loader.cpp", "jpeg_decode.cpp"]
-HIP_SOURCES = ["kernels.hip", "guiding.hip", "mlp.hip", "batch.hip", "api.hip"]
+HIP_SOURCES = ["kernels.hip", "guiding.hip", "mlp.hip", "batch.hip", "bvh_build.hip", "api.hip"]
 
 CXXFLAGS = ["-std=c++17", "-O3", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-pthread", "-Wall",
             "-Wno-unused-function"]

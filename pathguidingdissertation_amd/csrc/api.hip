@@ -16,6 +16,7 @@
 #include "guiding.h"
 #include "mlp.h"
 #include "ctx_internal.h"
+#include "bvh_build.h"
 
 using namespace pg;
 
@@ -68,6 +69,7 @@ struct TimedLaunch {
 struct pg_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    int32_t bvh_builder = PG_BVH_HOST;  // pg_set_bvh_builder
     float* noalloc_scratch = nullptr;  // this device's scratch array (pg_trace_closest_noalloc)
     std::string err;
     int cu_count = 0;
@@ -374,6 +376,26 @@ pg_status pg_device_info(pg_ctx* c, char* name, size_t name_len, int32_t* cu_cou
     return PG_OK;
 }
 
+// The GPU binned-SAH build of n triangles' positions (host array): the same arrays
+// as the host build, in the reference's topology; build_ms holds the GPU time.
+static pg_status build_bvh_on_device(pg_ctx* c, const float* positions, uint32_t n, HostBvh& out) {
+    float* dpos = nullptr;
+    pg_status st = c->hip(dalloc(&dpos, (size_t)n * 9), "alloc positions");
+    if (st == PG_OK)
+        st = c->hip(hipMemcpy(dpos, positions, (size_t)n * 36, hipMemcpyHostToDevice), "copy positions");
+    double ms = 0.0;
+    if (st == PG_OK) {
+        try {
+            st = c->hip(gpu_build_bvh(dpos, n, out, &ms, c->stream), "GPU BVH build");
+        } catch (const std::bad_alloc&) {
+            st = c->fail(PG_ERR_NOMEM, "out of host memory for the BVH");
+        }
+    }
+    dfree(dpos);
+    out.build_ms = ms;
+    return st;
+}
+
 pg_status pg_scene_upload(pg_ctx* c, const float* positions, const float* normals, const uint32_t* material_index,
                           uint32_t n_tris, const pg_material* materials, uint32_t n_materials, const float* env_rgb,
                           int32_t env_width, int32_t env_height, const pg_camera_desc* camera) {
@@ -383,9 +405,15 @@ pg_status pg_scene_upload(pg_ctx* c, const float* positions, const float* normal
     c->free_scene();
     c->host = PreparedScene();
     std::string e;
+    HostBvh gpu_bvh;
+    const bool on_gpu = c->bvh_builder == PG_BVH_GPU && positions && n_tris > 0 && n_tris <= (1u << 26);
+    if (on_gpu) {
+        pg_status st = build_bvh_on_device(c, positions, n_tris, gpu_bvh);
+        if (st != PG_OK) return st;
+    }
     try {
         e = prepare_scene(positions, normals, material_index, n_tris, materials, n_materials, env_rgb, env_width,
-                          env_height, *camera, c->host);
+                          env_height, *camera, c->host, on_gpu ? &gpu_bvh : nullptr);
     } catch (const std::bad_alloc&) {
         c->host = PreparedScene();
         return c->fail(PG_ERR_NOMEM, "out of host memory preparing the scene");
@@ -499,6 +527,28 @@ pg_status pg_bvh_build_host(const float* positions, uint32_t n_tris, uint32_t* n
     if (n_nodes) *n_nodes = (uint32_t)b.nodes.size();
     if (nodes_out) std::memcpy(nodes_out, b.nodes.data(), b.nodes.size() * sizeof(BvhNode));
     if (tri_index_out) std::memcpy(tri_index_out, b.tri_index.data(), b.tri_index.size() * 4);
+    return PG_OK;
+}
+
+pg_status pg_bvh_build_gpu(pg_ctx* c, const float* positions, uint32_t n_tris, uint32_t* n_nodes, float* nodes_out,
+                           int32_t* tri_index_out, double* build_ms) {
+    PG_CHECK_CTX(c);
+    if (!positions || n_tris == 0) return c->fail(PG_ERR_INVALID, "empty triangle set");
+    if (n_tris > (1u << 26)) return c->fail(PG_ERR_INVALID, "more than 2^26 triangles");
+    HostBvh b;
+    pg_status st = build_bvh_on_device(c, positions, n_tris, b);
+    if (st != PG_OK) return st;
+    if (n_nodes) *n_nodes = (uint32_t)b.nodes.size();
+    if (nodes_out) std::memcpy(nodes_out, b.nodes.data(), b.nodes.size() * sizeof(BvhNode));
+    if (tri_index_out) std::memcpy(tri_index_out, b.tri_index.data(), b.tri_index.size() * 4);
+    if (build_ms) *build_ms = b.build_ms;
+    return PG_OK;
+}
+
+pg_status pg_set_bvh_builder(pg_ctx* c, int32_t builder) {
+    PG_CHECK_CTX(c);
+    if (builder != PG_BVH_HOST && builder != PG_BVH_GPU) return c->fail(PG_ERR_INVALID, "unknown BVH builder");
+    c->bvh_builder = builder;
     return PG_OK;
 }
 

@@ -337,7 +337,7 @@ bool material_record(const pg_material& m, MaterialRec& r) {
 
 std::string prepare_scene(const float* pos, const float* nrm, const uint32_t* mat_idx, uint32_t n_tris,
                           const pg_material* mats, uint32_t n_mats, const float* env_rgb, int32_t env_w,
-                          int32_t env_h, const pg_camera_desc& cam, PreparedScene& out) {
+                          int32_t env_h, const pg_camera_desc& cam, PreparedScene& out, HostBvh* prebuilt) {
     if (n_tris == 0) return "scene has no triangles";
     // traversal stack entries hold node and triangle indices in 27 bits (2N-1 nodes)
     if (n_tris > (1u << 26)) return "scene has more than 2^26 triangles";
@@ -348,7 +348,8 @@ std::string prepare_scene(const float* pos, const float* nrm, const uint32_t* ma
     out.mats.resize(n_mats);
     for (uint32_t i = 0; i < n_mats; ++i)
         if (!material_record(mats[i], out.mats[i])) return "unknown material type";
-    build_bvh(pos, n_tris, out.bvh);
+    if (prebuilt) out.bvh = std::move(*prebuilt);  // built elsewhere (the GPU builder) in the same topology
+    else build_bvh(pos, n_tris, out.bvh);
     out.node_slots.assign(out.bvh.nodes.size() + 2, BvhNode{});
     for (size_t i = 0; i < out.bvh.nodes.size(); ++i) out.node_slots[i + 1] = out.bvh.nodes[i];
     out.tris.resize(n_tris);

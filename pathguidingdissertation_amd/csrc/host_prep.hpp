@@ -124,6 +124,6 @@ bool material_record(const pg_material& m, MaterialRec& r);
 std::string prepare_scene(const float* positions, const float* normals, const uint32_t* mat_idx,
                           uint32_t n_tris, const pg_material* mats, uint32_t n_mats,
                           const float* env_rgb, int32_t env_w, int32_t env_h,
-                          const pg_camera_desc& cam, PreparedScene& out);
+                          const pg_camera_desc& cam, PreparedScene& out, HostBvh* prebuilt = nullptr);
 
 }  // namespace pg

@@ -4,7 +4,7 @@ Reference vectors (ref_*.npz) come from the reference's own classes, compiled fr
 /root/reference by oracle/Makefile into oracle/_ref/libpgref.so (this container only).
 Oracle images (oracle_*.npz, *_band.json) come from the CPU oracle in FP64.
 
-    python tests/golden/make_golden.py [--ref] [--images] [--scenes] [--nee] [--independent]
+    python tests/golden/make_golden.py [--ref] [--images] [--scenes] [--nee] [--independent] [--config2] [--signed-zero]
 """
 from __future__ import annotations
 
@@ -359,6 +359,79 @@ def make_independent(spp=4096, seed=2024):
     print("independent estimate: relative standard error max", float((se / mean).max()))
 
 
+CONFIG2_CROPS = ((560, 420, 64, 64), (200, 150, 64, 64), (880, 330, 64, 64))
+
+
+def make_config2(spp_ref=4096, k=8, spp=64):
+    """Three 64 x 64 crops of the ~1.09M-triangle interior at 1280x720, max depth 12
+    (objects, a sunlit wall, the far corner): the FP64 oracle at 4096 SPP and the
+    relMSE of k equal-SPP oracle runs against it (the band a renderer's crops must
+    fall in). Only the crops are traced; each pixel keeps its own random stream."""
+    from oracle import pg_oracle as po
+    import pathguidingdissertation_amd as pg
+    data = pg.generate_scene(2, 0)
+    osc = po.OracleScene(data)
+
+    def crops(film):
+        return np.stack([film[y:y + h, x:x + w] for x, y, w, h in CONFIG2_CROPS])
+
+    ref = None
+    for c in CONFIG2_CROPS:
+        film, _ = osc.render(spp_ref, seed=9002, max_depth=12, accum_f64=True, crop=c)
+        ref = film if ref is None else ref + film
+    ref_c = crops(ref / spp_ref)
+    inner = np.s_[:, 1:-1, 1:-1]  # the box filter's border pixels also receive outside samples
+    vals, trimmed = [], []
+    for s in range(1, k + 1):
+        img = None
+        for c in CONFIG2_CROPS:
+            film, _ = osc.render(spp, seed=s, max_depth=12, accum_f64=True, crop=c)
+            img = film if img is None else img + film
+        img_c = crops(img / spp)
+        e = ((img_c[inner] - ref_c[inner]) ** 2 / (ref_c[inner] ** 2 + 1e-2)).mean(-1).reshape(-1)
+        vals.append(float(e.mean()))
+        trimmed.append(float(np.sort(e)[: e.size - e.size // 1000].mean()))  # without the worst 0.1%
+    np.savez_compressed(OUT / "oracle_config2_crops.npz", image=ref_c.astype(np.float32),
+                        crops=np.array(CONFIG2_CROPS, np.int32), spp=spp_ref, seed=9002,
+                        band_spp=spp, band_relmse=np.array(vals), band_trimmed=np.array(trimmed))
+    print("config 2 crops: band", np.mean(vals), np.std(vals, ddof=1), vals, "trimmed", trimmed)
+
+
+def signed_zero_positions(n=3000, seed=31):
+    """Triangles whose coordinates are often exactly -0.0 or +0.0 (both signs), so node
+    bounds meet ties between -0 and +0 (AABB::extend keeps the first one)."""
+    rng = np.random.default_rng(seed)
+    pos = rng.uniform(-1, 1, (n, 3, 3))
+    pos[:, 1:] = pos[:, :1] + rng.uniform(-0.2, 0.2, (n, 2, 3))
+    z = rng.random((n, 3, 3)) < 0.3
+    pos[z] = np.where(rng.random(z.sum()) < 0.5, -0.0, 0.0)
+    half = rng.random(n) < 0.5  # half of the triangles lie on the x >= 0 side only
+    pos[half, :, 0] = np.abs(pos[half, :, 0])
+    pos[half, :, 0] = np.where(pos[half, :, 0] == 0, np.where(rng.random((half.sum(), 3)) < 0.5, -0.0, 0.0),
+                               pos[half, :, 0])
+    return np.ascontiguousarray(pos.astype(np.float32))
+
+
+def make_signed_zero():
+    """The reference's BVH (Geometry.h:274-446) over the signed-zero triangles."""
+    from oracle import pg_oracle as po
+    import ctypes as C
+    R = po.ref()
+    pos = signed_zero_positions()
+    n = pos.shape[0]
+    nrm = np.zeros_like(pos)
+    nrm[..., 2] = 1.0
+    h = R.ref_scene_create(po.fp(pos), po.fp(nrm), n)
+    nn = R.ref_bvh_node_count(h)
+    nodes = np.empty((nn, 8), np.float32)
+    idx = np.empty(n, np.int32)
+    R.ref_bvh_export(h, po.fp(nodes), idx.ctypes.data_as(C.POINTER(C.c_int32)))
+    R.ref_scene_destroy(h)
+    neg0 = int(np.sum(np.signbit(nodes[:, [0, 1, 2, 4, 5, 6]]) & (nodes[:, [0, 1, 2, 4, 5, 6]] == 0)))
+    np.savez_compressed(OUT / "ref_bvh_signed_zero.npz", positions=pos, nodes=nodes, tri_index=idx)
+    print("signed-zero BVH:", nn, "nodes,", neg0, "bounds that are -0")
+
+
 def synthetic_env(w, h):
     v = (np.arange(h) + 0.5) / h
     u = (np.arange(w) + 0.5) / w
@@ -398,6 +471,8 @@ if __name__ == "__main__":
     ap.add_argument("--scenes", action="store_true")
     ap.add_argument("--nee", action="store_true")
     ap.add_argument("--independent", action="store_true")
+    ap.add_argument("--config2", action="store_true")
+    ap.add_argument("--signed-zero", action="store_true")
     a = ap.parse_args()
     if a.ref:
         make_ref()
@@ -409,3 +484,7 @@ if __name__ == "__main__":
         make_nee()
     if a.independent:
         make_independent()
+    if a.config2:
+        make_config2()
+    if a.signed_zero:
+        make_signed_zero()

@@ -160,3 +160,18 @@ def test_metrics_bit_exact_against_reference(pg):
     assert abs(m["relmse"] - pg.relmse(d["test"], d["truth"])) <= 1e-12 * m["relmse"]
     with pytest.raises(pg.PgError):
         pg.metrics(d["test"], d["truth"][:-1])
+
+
+def test_host_bvh_keeps_first_of_signed_zero_ties(pg):
+    """Coordinates that are often exactly -0.0 or +0.0: AABB::extend's std::min /
+    std::max keep the first of equal values, so a node bound is -0 or +0 by which
+    comes first in triangle order. The host build equals the reference's own build
+    bit for bit (tests/golden/ref_bvh_signed_zero.npz, ~2,200 bounds of -0)."""
+    from pathguidingdissertation_amd.scene import MATERIAL_DTYPE, SceneData
+    g = np.load(GOLDEN / "ref_bvh_signed_zero.npz")
+    pos = g["positions"]
+    data = SceneData(pos, np.zeros_like(pos), np.zeros(pos.shape[0], np.uint32), np.zeros(1, MATERIAL_DTYPE), None,
+                     {"from": [0, 0, 5], "to": [0, 0, 0], "up": [0, 1, 0], "fov": 40.0, "width": 8, "height": 8})
+    nodes, idx = _host_bvh(pg, data)
+    assert np.array_equal(nodes.view(np.uint32), g["nodes"].view(np.uint32))
+    assert np.array_equal(idx, g["tri_index"])
