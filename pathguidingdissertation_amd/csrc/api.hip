@@ -912,7 +912,7 @@ pg_status pg_render_spp(pg_ctx* c, const pg_render_params* params, uint32_t n_sp
                                                         : trace_grid(c->cu_count, stack);
     const uint32_t grid_shade = (uint32_t)c->cu_count * (prm.shade_blocks_per_cu ? prm.shade_blocks_per_cu : 8u);
     const uint32_t grid_guide = (uint32_t)c->cu_count * (c->gv.levels <= 16 ? 10u : 5u);
-    const uint32_t grid_mlp = (uint32_t)c->cu_count * 2;
+    const uint32_t grid_mlp = (uint32_t)c->cu_count * 3;  // 48 KB of LDS per block: 3 blocks per CU
     GuideCtx gc{c->gv, (mode == 1 || mode == 3) ? c->guide_cells : nullptr, mode == 2 ? c->mix : nullptr};
     gc.g.guide_prob = prm.guide_prob;
     c->pool_used = 0;

@@ -133,6 +133,11 @@ struct pg_batch {
     std::vector<std::vector<uint32_t>> chunk_scenes;    // scenes with views in each chunk
     float4* mix = nullptr;                              // learned-guide mixtures by queue slot
     uint32_t mix_cap = 0;
+    // learned guides of the scenes for the grouped inference (wmirror null: no guide)
+    MlpView* d_guides = nullptr;
+    uint32_t guides_cap = 0;
+    GroupBuffers grp{};
+    uint32_t grp_cap = 0, grp_scenes = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     bool graph_ok = false;
@@ -283,15 +288,12 @@ hipError_t enqueue_pass(pg_batch* b, const pg_render_params& prm, uint32_t ns) {
                 hipSuccess)
                 return e;
             if (learned) {
-                // every slot unguided, then each scene's network writes its own slots
+                // every slot unguided, then one launch runs each scene's network over
+                // that scene's slots (grouped by scene)
                 if ((e = launch_mix_clear(b->ps, parity, b->mix, (uint32_t)cu * 4, st)) != hipSuccess) return e;
-                for (uint32_t sid : b->chunk_scenes[ci]) {
-                    const SceneGuide* g = b->scenes[sid]->guide.get();
-                    if (!g) continue;
-                    if ((e = launch_mlp_infer(s0, b->ps, g->view, parity, b->mix, (uint32_t)cu * 2, st, true, sid)) !=
-                        hipSuccess)
-                        return e;
-                }
+                if ((e = launch_mlp_infer_grouped(b->ps, parity, b->d_guides, (uint32_t)b->scenes.size(), b->grp,
+                                                  b->mix, (uint32_t)cu * 3, st)) != hipSuccess)
+                    return e;
             }
             if ((e = launch_shade(s0, b->ps, pp, gc, learned ? 2 : 0, parity, (int)depth, sgrid, st, true)) !=
                 hipSuccess)
@@ -335,6 +337,36 @@ pg_status render(pg_batch* b, const pg_render_params& prm, uint32_t n_spp) {
         bfree(b->mix);
         PG_BTRY(b, balloc(&b->mix, (size_t)b->ps_cap * kMixF4), "alloc guide mixtures");
         b->mix_cap = b->ps_cap;
+    }
+    if (prm.integrator == PG_INTEGRATOR_GUIDED_MLP) {
+        // the scenes' guide table and the grouping buffers of the one-launch inference
+        const uint32_t ns_ = (uint32_t)b->scenes.size();
+        const uint32_t cap = b->ps_cap + 128u * ns_;
+        if (b->grp_cap < cap || b->grp_scenes < ns_ || b->guides_cap < ns_) {
+            b->drop_graph();
+            PG_BTRY(b, hipStreamSynchronize(s), "sync");
+            b->grp_cap = b->grp_scenes = b->guides_cap = 0;
+            bfree(b->grp.cnt); bfree(b->grp.off); bfree(b->grp.cur); bfree(b->grp.perm); bfree(b->grp.n_tiles);
+            bfree(b->d_guides);
+            PG_BTRY(b, balloc(&b->grp.cnt, ns_ + 1), "alloc grouping");
+            PG_BTRY(b, balloc(&b->grp.off, ns_ + 1), "alloc grouping");
+            PG_BTRY(b, balloc(&b->grp.cur, ns_ + 1), "alloc grouping");
+            PG_BTRY(b, balloc(&b->grp.perm, cap), "alloc grouping");
+            PG_BTRY(b, balloc(&b->grp.n_tiles, 1), "alloc grouping");
+            PG_BTRY(b, balloc(&b->d_guides, ns_), "alloc guide table");
+            PG_BTRY(b, hipMemset(b->grp.cnt, 0, (ns_ + 1) * 4), "clear grouping");
+            b->grp_cap = cap;
+            b->grp_scenes = b->guides_cap = ns_;
+        }
+        std::vector<MlpView> gv(ns_);
+        for (uint32_t i = 0; i < ns_; ++i) {
+            const SceneGuide* g = b->scenes[i]->guide.get();
+            if (g) gv[i] = g->view;
+            else std::memset(&gv[i], 0, sizeof(MlpView));
+        }
+        PG_BTRY(b, hipMemcpyAsync(b->d_guides, gv.data(), ns_ * sizeof(MlpView), hipMemcpyHostToDevice, s),
+                "guide table");
+        PG_BTRY(b, hipStreamSynchronize(s), "guide table");
     }
     // passes of ns samples (the captured graph), then one pass of the samples left over
     const uint32_t ns = b->samples_per_pass();
@@ -412,6 +444,8 @@ pg_status pg_batch_destroy(pg_batch* b) {
     b->drop_graph();
     b->free_paths();
     bfree(b->d_scenes); bfree(b->d_views); bfree(b->film); bfree(b->mix);
+    bfree(b->grp.cnt); bfree(b->grp.off); bfree(b->grp.cur); bfree(b->grp.perm); bfree(b->grp.n_tiles);
+    bfree(b->d_guides);
     bfree(b->d_counters); bfree(b->d_stats); bfree(b->d_sample);
     if (b->e0) (void)hipEventDestroy(b->e0);
     if (b->e1) (void)hipEventDestroy(b->e1);

@@ -145,6 +145,14 @@ struct MlpView;
 hipError_t launch_mlp_infer(const DevScene& s, const PathState& ps, const MlpView& v, int parity, float4* mix,
                             uint32_t grid, hipStream_t stream, bool multi = false, uint32_t scene = 0);
 hipError_t launch_mix_clear(const PathState& ps, int parity, float4* mix, uint32_t grid, hipStream_t stream);
+// Batches: the queue slots grouped by scene (cnt, off, cur: n_scenes + 1 entries each;
+// perm: queue capacity + 128 * n_scenes; n_tiles: 1) so that one launch runs every
+// scene's network (guides[s].wmirror == nullptr: scene s is unguided).
+struct GroupBuffers {
+    uint32_t *cnt, *off, *cur, *perm, *n_tiles;
+};
+hipError_t launch_mlp_infer_grouped(const PathState& ps, int parity, const MlpView* guides, uint32_t n_scenes,
+                                    const GroupBuffers& gb, float4* mix, uint32_t grid, hipStream_t stream);
 int stack_entries(int max_depth);
 uint32_t trace_grid(int cu_count, int stack);
 

@@ -95,6 +95,19 @@ __device__ __forceinline__ void stage_outputs(const FwdOut& f, const float* sb, 
             }
 }
 
+// Half nb (samples nb*16 .. nb*16+15) of the tile's output logits -> per-wave staging
+// [16][kStagePitch] (bias added): half the LDS of stage_outputs.
+__device__ __forceinline__ void stage_half(const FwdOut& f, const float* sb, float* st, int lane, int nb) {
+    const int c = lane & 15, g = lane >> 4;
+#pragma unroll
+    for (int mb = 0; mb < 3; ++mb)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            int o = mb * 16 + 4 * g + r;
+            st[c * kStagePitch + o] = (nb == 0 ? f.o[0][mb][r] : f.o[1][mb][r]) + sb[192 + o];
+        }
+}
+
 // Mixture parameters of one sample's outputs (13 float4); c = 0 marks an unusable
 // output (non-finite values).
 __device__ __forceinline__ void head_mixture(const float* o, float4* mx) {
@@ -643,12 +656,12 @@ __global__ __launch_bounds__(kMlpBlock) void k_mlp_infer(DevScene s0, PathState 
                                                          uint32_t scene) {
     __shared__ __align__(16) uint16_t sw[kMirror];
     __shared__ float sb[kBiasCount];
-    __shared__ float stage[4 * 32 * kStagePitch];
+    __shared__ float stage[4 * 16 * kStagePitch];  // half a tile at a time: 3 blocks per CU
     const uint32_t n = ps.counters[parity];
     if (blockIdx.x * 128u >= n) return;
     load_net(v, sw, sb);
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, c = lane & 15, g = lane >> 4;
-    float* st = stage + wv * 32 * kStagePitch;
+    float* st = stage + wv * 16 * kStagePitch;
     const uint32_t n_tiles = (n + 127u) / 128u;
     for (uint32_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
         const uint32_t base = tile * 128u + (uint32_t)wv * 32u;
@@ -698,21 +711,191 @@ __global__ __launch_bounds__(kMlpBlock) void k_mlp_infer(DevScene s0, PathState 
         }
         FwdOut f;
         tile_forward(sw, sb, x, f, nullptr, base, n, lane);
-        stage_outputs(f, sb, st, lane);
-        // lanes 0..15 hold the validity of samples nb*16 + c: share it through the staging
-        // pad (1 guided, 0 unguided point, 2 another scene's slot)
-        if (g == 0) {
-            st[c * kStagePitch + 48] = !mine_nb[0] ? 2.f : valid_nb[0] ? 1.f : 0.f;
-            st[(16 + c) * kStagePitch + 48] = !mine_nb[1] ? 2.f : valid_nb[1] ? 1.f : 0.f;
+#pragma unroll
+        for (int nb = 0; nb < 2; ++nb) {
+            stage_half(f, sb, st, lane, nb);
+            // lanes 0..15 hold the validity of samples nb*16 + c: share it through the
+            // staging pad (1 guided, 0 unguided point, 2 another scene's slot)
+            if (g == 0) st[c * kStagePitch + 48] = !mine_nb[nb] ? 2.f : valid_nb[nb] ? 1.f : 0.f;
+            __syncthreads();
+            const uint32_t i = base + (uint32_t)nb * 16u + (uint32_t)lane;
+            if (lane < 16 && i < n) {
+                float4* mx = mix + (size_t)i * kMixF4;
+                const float flag = st[lane * kStagePitch + 48];
+                if (flag == 1.f) head_mixture(st + lane * kStagePitch, mx);
+                else if (flag == 0.f) mx[12] = make_float4(0.f, 0.f, 0.f, 0.f);
+            }
+            __syncthreads();
         }
-        __syncthreads();
-        if (lane < 32 && base + lane < n) {
-            float4* mx = mix + (size_t)(base + lane) * kMixF4;
-            const float flag = st[lane * kStagePitch + 48];
-            if (flag == 1.f) head_mixture(st + lane * kStagePitch, mx);
-            else if (flag == 0.f) mx[12] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+}
+
+// ------------------------------------------------------------------ batches
+// Queue slots of a bounce grouped by scene, each scene's group padded to whole
+// 128-slot tiles, so that one launch evaluates every scene's network (each tile
+// belongs to one scene; a block loads a network when its tiles change scene).
+// Wave-aggregated append: the lanes that hold the same key take consecutive entries
+// of counter[key] with one atomic per distinct key of the wave (queues hold runs of
+// one scene, so most waves issue one atomic). Returns the lane's entry (active lanes).
+__device__ __forceinline__ uint32_t wave_key_append(uint32_t* counter, uint32_t key, bool active) {
+    const int lane = threadIdx.x & 63;
+    const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    unsigned long long todo = __ballot(active);
+    uint32_t mine = 0;
+    while (todo) {
+        const int leader = __ffsll(todo) - 1;
+        const uint32_t k = (uint32_t)__shfl((int)key, leader);
+        const unsigned long long grp = __ballot(active && key == k) & todo;
+        uint32_t base = 0;
+        if (lane == leader) base = atomicAdd(&counter[k], (uint32_t)__popcll(grp));
+        base = (uint32_t)__shfl((int)base, leader);
+        if (active && key == k && ((grp >> lane) & 1ull)) mine = base + (uint32_t)__popcll(grp & below);
+        todo &= ~grp;
+    }
+    return mine;
+}
+
+__global__ void k_scene_count(PathState ps, int parity, const MlpView* guides, uint32_t* cnt) {
+    const uint32_t n = ps.counters[parity];
+    const uint32_t stride = gridDim.x * blockDim.x;
+    // whole waves iterate together (wave_key_append needs every lane)
+    for (uint32_t i0 = blockIdx.x * blockDim.x; i0 < n; i0 += stride) {
+        const uint32_t i = i0 + threadIdx.x;
+        uint32_t sid = 0;
+        bool on = false;
+        if (i < n) {
+            sid = ps.bviews[ps.path_view[ps.queue[parity][i]]].scene;
+            on = guides[sid].wmirror != nullptr;
         }
-        __syncthreads();
+        (void)wave_key_append(cnt, sid, on);
+    }
+}
+// One thread: padded offsets, scatter cursors, tile count; clears the counts.
+__global__ void k_scene_offsets(uint32_t n_scenes, uint32_t* cnt, uint32_t* off, uint32_t* cur, uint32_t* n_tiles) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    uint32_t o = 0;
+    for (uint32_t s = 0; s < n_scenes; ++s) {
+        off[s] = o;
+        cur[s] = o;
+        o += (cnt[s] + 127u) & ~127u;
+        cnt[s] = 0u;
+    }
+    off[n_scenes] = o;
+    *n_tiles = o / 128u;
+}
+// Padding entries of the grouped order stay ~0 (no slot).
+__global__ void k_perm_clear(const uint32_t* n_tiles, uint32_t* perm) {
+    const uint32_t total = *n_tiles * 128u;
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < total; j += gridDim.x * blockDim.x) perm[j] = ~0u;
+}
+__global__ void k_scene_scatter(PathState ps, int parity, const MlpView* guides, uint32_t* cur, uint32_t* perm) {
+    const uint32_t n = ps.counters[parity];
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t i0 = blockIdx.x * blockDim.x; i0 < n; i0 += stride) {
+        const uint32_t i = i0 + threadIdx.x;
+        uint32_t sid = 0;
+        bool on = false;
+        if (i < n) {
+            sid = ps.bviews[ps.path_view[ps.queue[parity][i]]].scene;
+            on = guides[sid].wmirror != nullptr;
+        }
+        const uint32_t at = wave_key_append(cur, sid, on);
+        if (on) perm[at] = i;
+    }
+}
+
+// Every scene's network over its own slots in one launch (see k_scene_count): blocks
+// take contiguous tile ranges; the slots of scenes without a network stay as the
+// clear left them (unguided).
+__global__ __launch_bounds__(kMlpBlock) void k_mlp_infer_grouped(PathState ps, int parity, const MlpView* guides,
+                                                                 uint32_t n_scenes, const uint32_t* off,
+                                                                 const uint32_t* n_tiles_p, const uint32_t* perm,
+                                                                 float4* mix) {
+    __shared__ __align__(16) uint16_t sw[kMirror];
+    __shared__ float sb[kBiasCount];
+    __shared__ float stage[4 * 16 * kStagePitch];
+    __shared__ uint32_t sslot[4 * 16];
+    __shared__ MlpView sv;
+    const uint32_t n_tiles = *n_tiles_p;
+    const uint32_t per = (n_tiles + gridDim.x - 1) / gridDim.x;
+    const uint32_t t0 = blockIdx.x * per, t1 = min(t0 + per, n_tiles);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, c = lane & 15, g = lane >> 4;
+    float* st = stage + wv * 16 * kStagePitch;
+    int loaded = -1;
+    for (uint32_t tile = t0; tile < t1; ++tile) {
+        // the tile's scene: the last s with off[s] <= tile * 128
+        uint32_t lo = 0, hi = n_scenes;
+        while (hi - lo > 1) {
+            uint32_t mid = (lo + hi) >> 1;
+            if (off[mid] <= tile * 128u) lo = mid;
+            else hi = mid;
+        }
+        const int scene = (int)lo;
+        if (scene != loaded) {
+            __syncthreads();  // every wave is done with the previous network
+            if (threadIdx.x == 0) sv = guides[scene];
+            __syncthreads();
+            load_net(sv, sw, sb);
+            loaded = scene;
+        }
+        const DevScene& s = ps.bscenes[scene];
+        const uint32_t base = tile * 128u + (uint32_t)wv * 32u;
+        u16x8 x[3][2];
+        bool valid_nb[2];
+        uint32_t slot_nb[2];
+#pragma unroll
+        for (int nb = 0; nb < 2; ++nb) {
+            const uint32_t i = perm[base + nb * 16 + c];
+            slot_nb[nb] = i;
+            bool valid = i != ~0u;
+            F3 xp{0.f, 0.f, 0.f}, sn{0.f, 0.f, 1.f}, wo{0.f, 0.f, 1.f};
+            if (valid) {
+                float4 h = ps.hit[i];
+                uint32_t hid = __float_as_uint(h.x);
+                valid = hid != kMiss;
+                if (valid) {
+                    const TriShade& tsh = s.shade[hid];
+                    const MaterialRec& m = s.mats[tsh.mat];
+                    valid = !m.specular && !m.is_light;
+                    if (valid) {
+                        float4 ro = ps.ray_o[parity][i], rdv = ps.ray_d[parity][i];
+                        F3 rd{rdv.x, rdv.y, rdv.z};
+                        xp = F3{ro.x, ro.y, ro.z} + rd * h.y;
+                        float ha = 1.f - (h.z + h.w);
+                        sn = normalize((ld3(tsh.n0) * ha + ld3(tsh.n1) * h.z) + ld3(tsh.n2) * h.w);
+                        wo = -rd;
+                        if (m.two_sided && dot(wo, sn) < 0.f) sn = -sn;
+                    }
+                }
+            }
+            valid_nb[nb] = valid;
+            float p01[3];
+            scene_unit(sv, xp, p01);
+            u16x8 e[3];
+            encode_slots(sv, valid, p01, sn, wo, g, e);
+#pragma unroll
+            for (int kb = 0; kb < 3; ++kb) x[kb][nb] = e[kb];
+        }
+        FwdOut f;
+        tile_forward(sw, sb, x, f, nullptr, base, ~0u, lane);
+#pragma unroll
+        for (int nb = 0; nb < 2; ++nb) {
+            stage_half(f, sb, st, lane, nb);
+            if (g == 0) {
+                st[c * kStagePitch + 48] = valid_nb[nb] ? 1.f : 0.f;
+                sslot[wv * 16 + c] = slot_nb[nb];
+            }
+            __syncthreads();
+            if (lane < 16) {
+                const uint32_t i = sslot[wv * 16 + lane];
+                if (i != ~0u) {
+                    float4* mx = mix + (size_t)i * kMixF4;
+                    if (st[lane * kStagePitch + 48] == 1.f) head_mixture(st + lane * kStagePitch, mx);
+                    else mx[12] = make_float4(0.f, 0.f, 0.f, 0.f);
+                }
+            }
+            __syncthreads();
+        }
     }
 }
 
@@ -827,6 +1010,17 @@ hipError_t launch_mlp_infer(const DevScene& s, const PathState& ps, const MlpVie
         hipLaunchKernelGGL(k_mlp_infer<true>, dim3(grid), dim3(kMlpBlock), 0, stream, s, ps, v, parity, mix, scene);
     else
         hipLaunchKernelGGL(k_mlp_infer<false>, dim3(grid), dim3(kMlpBlock), 0, stream, s, ps, v, parity, mix, 0u);
+    return hipGetLastError();
+}
+
+hipError_t launch_mlp_infer_grouped(const PathState& ps, int parity, const MlpView* guides, uint32_t n_scenes,
+                                    const GroupBuffers& gb, float4* mix, uint32_t grid, hipStream_t stream) {
+    hipLaunchKernelGGL(k_scene_count, dim3(grid), dim3(256), 0, stream, ps, parity, guides, gb.cnt);
+    hipLaunchKernelGGL(k_scene_offsets, dim3(1), dim3(64), 0, stream, n_scenes, gb.cnt, gb.off, gb.cur, gb.n_tiles);
+    hipLaunchKernelGGL(k_perm_clear, dim3(grid), dim3(256), 0, stream, gb.n_tiles, gb.perm);
+    hipLaunchKernelGGL(k_scene_scatter, dim3(grid), dim3(256), 0, stream, ps, parity, guides, gb.cur, gb.perm);
+    hipLaunchKernelGGL(k_mlp_infer_grouped, dim3(grid), dim3(kMlpBlock), 0, stream, ps, parity, guides, n_scenes,
+                       gb.off, gb.n_tiles, gb.perm, mix);
     return hipGetLastError();
 }
 
