@@ -486,18 +486,23 @@ def main() -> int:
     bytes_c = algorithmic_bytes(nodes_c, tris_c, 20)
     bytes_s = algorithmic_bytes(nodes_s, tris_s, 4)
 
-    # launch shape (traversal / shade blocks per CU) picked by the library's tuner
+    frame_spp = 64  # the workload's SPP: one step is one whole frame
+    # launch shape (LDS stack, traversal / shade blocks per CU) picked by the library's
+    # tuner on passes of the frame's shape: all 64 samples in one pass when the paths
+    # fit one pass (2^26), as the automatic split renders a frame
+    if W * H * frame_spp <= (1 << 26):
+        rt.params.samples_per_pass = frame_spp
     t_tune = time.perf_counter()
     tuned_ms = rt.tune()
     tune_s = time.perf_counter() - t_tune
-    shape = {"trace_blocks_per_cu": rt.params.trace_blocks_per_cu or "lds-max",
+    shape = {"stack_lds": rt.params.stack_lds or 16,
+             "trace_blocks_per_cu": rt.params.trace_blocks_per_cu or "lds-max",
              "shade_blocks_per_cu": rt.params.shade_blocks_per_cu or 8,
              "extend_kernel": "k_extend_dyn" if rt.params.dyn_fetch else "k_extend",
              "samples_per_pass": rt.params.samples_per_pass or "auto", "tune_s": round(tune_s, 2),
              "tuned_ms_per_spp": round(tuned_ms, 3)}
     print(f"bench: launch shape {shape}", file=sys.stderr, flush=True)
 
-    frame_spp = 64  # the workload's SPP: one step is one whole frame
     rt.clear()
     for _ in range(a.warmup):
         rt.render(frame_spp)

@@ -117,7 +117,11 @@ typedef struct {
                                their vertex stacks take (max_depth + 1) x 128 B
                                per path, ~7 GB).
                                Images and records are identical for every value. */
-    uint32_t reserved[2];
+    uint32_t stack_lds;     /* traversal stack entries per lane kept in LDS (16, 32,
+                               48 or 64), deeper entries in global memory: fewer
+                               take less LDS per block and fit more blocks per CU.
+                               0 = automatic (16). Results unchanged.            */
+    uint32_t reserved[1];
 } pg_render_params;
 
 typedef struct {
@@ -217,12 +221,14 @@ pg_status pg_film_clear(pg_ctx* ctx);
  * params->samples_per_pass at a time; the film (and a record pass's records) equal
  * n_spp calls of one sample each bit for bit.                                    */
 pg_status pg_render_spp(pg_ctx* ctx, const pg_render_params* params, uint32_t n_spp);
-/* Film::film / Film::SPP (Imaging.h:178-272): accumulated RGB sums (W*H*3) and SPP. */
-/* Launch-shape tuner for pg_render_spp (no reference counterpart): times 2-SPP
- * passes of `params` over candidate traversal and shade grids and ray sorting
- * (coordinate descent) and returns params with the fastest shape; clears the film. */
+/* Launch-shape tuner for pg_render_spp (no reference counterpart): times passes of
+ * params->samples_per_pass samples (0: 1) — a warm-up pass, then two timed passes
+ * (one from 8 samples per pass) — over candidate LDS stack sizes, traversal and shade
+ * grids, persistent lanes and ray sorting (coordinate descent) and returns params
+ * with the fastest shape; clears the film.                                      */
 pg_status pg_render_tune(pg_ctx* ctx, const pg_render_params* params, pg_render_params* tuned,
                          double* ms_per_spp);
+/* Film::film / Film::SPP (Imaging.h:178-272): accumulated RGB sums (W*H*3) and SPP. */
 pg_status pg_film_read(pg_ctx* ctx, float* rgb_sum, uint32_t* spp);
 /* Film::tonemap (Imaging.h:220-244) of the whole film on the device: mean over the
  * film's SPP, exposure, extended Reinhard with white point lmax_white, gamma 1/2.2,
@@ -396,7 +402,8 @@ typedef struct pg_batch_config {
                                      paths per chunk); a render's samples beyond whole
                                      passes run as one shorter pass; views are identical
                                      for every value                                  */
-    uint32_t reserved;
+    uint32_t stack_lds;           /* traversal stack entries per lane in LDS (see the
+                                     render params; 0: 16)                            */
 } pg_batch_config;
 typedef struct pg_batch_stats {
     uint64_t paths, closest_rays, shadow_rays;

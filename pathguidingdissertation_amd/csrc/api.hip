@@ -228,7 +228,8 @@ struct pg_ctx {
     void free_paths() {
         dfree(ps.beta); dfree(ps.radiance); dfree(ps.film_pos); dfree(ps.info);
         for (int k = 0; k < 2; ++k) { dfree(ps.queue[k]); dfree(ps.ray_o[k]); dfree(ps.ray_d[k]); }
-        dfree(ps.hit); dfree(ps.sh_o); dfree(ps.sh_d); dfree(ps.sh_c);
+        dfree(ps.hit); dfree(ps.sh_o); dfree(ps.sh_d); dfree(ps.sh_c); dfree(ps.spill);
+        ps.spill_lanes = 0;
         ps_capacity = 0;
     }
 };
@@ -736,6 +737,10 @@ static pg_status ensure_paths(pg_ctx* c, uint32_t n) {
     PG_TRY(c, dalloc(&p.sh_o, n), "alloc shadow");
     PG_TRY(c, dalloc(&p.sh_d, n), "alloc shadow");
     PG_TRY(c, dalloc(&p.sh_c, n), "alloc shadow");
+    // traversal stack entries beyond the LDS share, for up to 32 traversal blocks of
+    // 128 lanes per CU
+    p.spill_lanes = (uint32_t)c->cu_count * 32u * 128u;
+    PG_TRY(c, dalloc(&p.spill, (size_t)p.spill_lanes * kSpillEntries), "alloc traversal stacks");
     p.counters = c->d_counters;
     p.stats = c->d_stats;
     p.capacity = n;
@@ -821,6 +826,8 @@ pg_status pg_render_spp(pg_ctx* c, const pg_render_params* params, uint32_t n_sp
     if (integ != PG_INTEGRATOR_PATH && integ != PG_INTEGRATOR_RECORD && integ != PG_INTEGRATOR_GUIDED_QT &&
         integ != PG_INTEGRATOR_GUIDED_MLP && integ != PG_INTEGRATOR_GUIDED_PSS)
         return c->fail(PG_ERR_INVALID, "unknown integrator");
+    if (prm.stack_lds % 16 != 0 || prm.stack_lds > 64)
+        return c->fail(PG_ERR_INVALID, "stack_lds must be 0, 16, 32, 48 or 64");
     if ((integ == PG_INTEGRATOR_GUIDED_QT || integ == PG_INTEGRATOR_GUIDED_PSS) && !c->cache_ready)
         return c->fail(PG_ERR_STATE, "guided rendering needs pg_cache_build first");
     if (integ == PG_INTEGRATOR_GUIDED_MLP && !c->mlp.ready)
@@ -907,7 +914,7 @@ pg_status pg_render_spp(pg_ctx* c, const pg_render_params* params, uint32_t n_sp
         PG_TRY(c, dalloc(&c->mix, (size_t)np * kMixF4), "alloc guide mixtures");
         c->mix_cap = np;
     }
-    const int stack = (int)c->host.bvh.max_depth;
+    const int stack = lds_entries((int)c->host.bvh.max_depth, prm.stack_lds, c->ps);
     const uint32_t grid_trace = prm.trace_blocks_per_cu ? (uint32_t)c->cu_count * prm.trace_blocks_per_cu
                                                         : trace_grid(c->cu_count, stack);
     const uint32_t grid_shade = (uint32_t)c->cu_count * (prm.shade_blocks_per_cu ? prm.shade_blocks_per_cu : 8u);
@@ -1034,20 +1041,33 @@ pg_status pg_render_tune(pg_ctx* c, const pg_render_params* params, pg_render_pa
     if (best.integrator == PG_INTEGRATOR_RECORD || (best.integrator == PG_INTEGRATOR_GUIDED_MLP && best.record))
         return c->fail(PG_ERR_INVALID, "record passes are not tuned");
     if (best.trace_blocks_per_cu > 32 || best.shade_blocks_per_cu > 32) return c->fail(PG_ERR_INVALID, "blocks per CU");
+    // each candidate renders whole passes of the caller's samples_per_pass (0: 1), the
+    // shape its renders will have: a warm-up pass, then two timed passes (one from 8 SPP)
+    const uint32_t k = best.samples_per_pass ? best.samples_per_pass : 1u;
+    const uint32_t timed_spp = k >= 8 ? k : 2 * k;
     auto measure = [&](const pg_render_params& p, double& ms) -> pg_status {
         pg_status st = pg_film_clear(c);
         if (st != PG_OK) return st;
-        if ((st = pg_render_spp(c, &p, 1)) != PG_OK) return st;  // warm-up
+        if ((st = pg_render_spp(c, &p, k)) != PG_OK) return st;  // warm-up
         double before = c->ms_total;
-        if ((st = pg_render_spp(c, &p, 2)) != PG_OK) return st;
-        ms = (c->ms_total - before) / 2.0;
+        if ((st = pg_render_spp(c, &p, timed_spp)) != PG_OK) return st;
+        ms = (c->ms_total - before) / timed_spp;
         return PG_OK;
     };
     double best_ms = 0.0;
     pg_status st = measure(best, best_ms);
     if (st != PG_OK) return st;
-    const uint32_t trace_cand[] = {0u, 4u, 6u, 8u};
+    const uint32_t lds_cand[] = {0u, 32u, 64u};
+    const uint32_t trace_cand[] = {0u, 4u, 6u, 8u, 12u};
     const uint32_t shade_cand[] = {4u, 8u, 16u};
+    for (uint32_t v : lds_cand) {
+        pg_render_params p = best;
+        if (p.stack_lds == v) continue;
+        p.stack_lds = v;
+        double ms = 0.0;
+        if ((st = measure(p, ms)) != PG_OK) return st;
+        if (ms < best_ms) { best_ms = ms; best = p; }
+    }
     for (uint32_t v : trace_cand) {
         pg_render_params p = best;
         if (p.trace_blocks_per_cu == v) continue;

@@ -164,7 +164,8 @@ struct pg_batch {
     void free_paths() {
         bfree(ps.beta); bfree(ps.radiance); bfree(ps.film_pos); bfree(ps.info);
         for (int k = 0; k < 2; ++k) { bfree(ps.queue[k]); bfree(ps.ray_o[k]); bfree(ps.ray_d[k]); }
-        bfree(ps.hit); bfree(ps.sh_o); bfree(ps.sh_d); bfree(ps.sh_c); bfree(ps.path_view);
+        bfree(ps.hit); bfree(ps.sh_o); bfree(ps.sh_d); bfree(ps.sh_c); bfree(ps.path_view); bfree(ps.spill);
+        ps.spill_lanes = 0;
         ps_cap = 0;
     }
     int stack() const {
@@ -253,6 +254,8 @@ pg_status prepare(pg_batch* b) {
         PG_BTRY(b, balloc(&p.sh_d, m), "alloc batch shadow");
         PG_BTRY(b, balloc(&p.sh_c, m), "alloc batch shadow");
         PG_BTRY(b, balloc(&p.path_view, m), "alloc batch views");
+        p.spill_lanes = (uint32_t)pg_internal::cu_count(b->ctx) * 32u * 128u;
+        PG_BTRY(b, balloc(&p.spill, (size_t)p.spill_lanes * kSpillEntries), "alloc traversal stacks");
         p.capacity = m;
         b->ps_cap = m;
     }
@@ -267,8 +270,9 @@ pg_status prepare(pg_batch* b) {
 // One pass (ns samples per pixel of every view) on the batch stream.
 hipError_t enqueue_pass(pg_batch* b, const pg_render_params& prm, uint32_t ns) {
     const int cu = pg_internal::cu_count(b->ctx);
-    const int stack = b->stack();
-    const uint32_t tgrid = b->cfg.trace_blocks_per_cu ? (uint32_t)cu * b->cfg.trace_blocks_per_cu : trace_grid(cu, stack);
+    const int stack = lds_entries(b->stack(), b->cfg.stack_lds, b->ps);
+    const uint32_t tgrid =
+        b->cfg.trace_blocks_per_cu ? (uint32_t)cu * b->cfg.trace_blocks_per_cu : trace_grid(cu, stack);
     const uint32_t sgrid = (uint32_t)cu * (b->cfg.shade_blocks_per_cu ? b->cfg.shade_blocks_per_cu : 8u);
     const DevScene& s0 = b->scenes[0]->dev;  // unused by the batch kernels' per-path scenes
     const uint32_t nv = (uint32_t)b->views.size();
@@ -527,6 +531,7 @@ pg_status pg_batch_configure(pg_batch* b, const pg_batch_config* cfg) {
     if (c.stack > 64) return b->fail(PG_ERR_INVALID, "stack above 64 entries");
     if (c.trace_blocks_per_cu > 32 || c.shade_blocks_per_cu > 32) return b->fail(PG_ERR_INVALID, "blocks per CU");
     if (c.samples_per_pass > 64) return b->fail(PG_ERR_INVALID, "samples_per_pass above 64");
+    if (c.stack_lds % 16 != 0 || c.stack_lds > 64) return b->fail(PG_ERR_INVALID, "stack_lds must be 0, 16, 32, 48 or 64");
     b->cfg = c;
     b->dirty = true;
     return PG_OK;
@@ -629,7 +634,7 @@ pg_status pg_batch_autotune(pg_batch* b, const pg_render_params* params, pg_batc
         {0u, std::max(largest, 1u << 21), std::max(largest, 1u << 20)},     // queue_paths
         {0u, 2u, 4u, 8u},                                                    // trace blocks per CU
         {4u, 8u, 16u},                                                       // shade blocks per CU
-        {(uint32_t)base_stack, (uint32_t)std::min(64, base_stack + 16)},    // stack
+        {0u, 32u, 64u},                                                      // stack entries in LDS
         {1u, 2u, 4u, 8u},                                                    // samples per pass
     };
     auto get = [](pg_batch_config& c, int k) -> uint32_t& {
@@ -639,7 +644,7 @@ pg_status pg_batch_autotune(pg_batch* b, const pg_render_params* params, pg_batc
         case 2: return c.queue_paths;
         case 3: return c.trace_blocks_per_cu;
         case 4: return c.shade_blocks_per_cu;
-        case 5: return c.stack;
+        case 5: return c.stack_lds;
         default: return c.samples_per_pass;
         }
     };

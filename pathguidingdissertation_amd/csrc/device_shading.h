@@ -74,6 +74,32 @@ __device__ __forceinline__ void stk_pop(const BvhNode* nodes, int e, int& lf, in
     }
 }
 
+// A lane's traversal stack with every entry in LDS: entry k at p[k * kBlock].
+struct LdsStack {
+    int* p;
+    __device__ __forceinline__ void push(int& sp, int e) const { p[(sp++) * kBlock] = e; }
+    __device__ __forceinline__ int pop(int& sp) const { return p[(--sp) * kBlock]; }
+};
+// A lane's traversal stack whose first NL entries live in LDS and deeper ones (rarely
+// reached) in global memory: entry k >= NL at spill[(k - NL) * stride]. A smaller LDS
+// share per lane lets more waves run on a CU; the entries and their order are those of
+// the all-LDS stack, so every traversal visits the same nodes.
+template <int NL>
+struct SplitStack {
+    int* lds;
+    int* spill;
+    uint32_t stride;
+    __device__ __forceinline__ void push(int& sp, int e) const {
+        if (sp < NL) lds[sp * kBlock] = e;
+        else spill[(size_t)(sp - NL) * stride] = e;
+        ++sp;
+    }
+    __device__ __forceinline__ int pop(int& sp) const {
+        --sp;
+        return sp < NL ? lds[sp * kBlock] : spill[(size_t)(sp - NL) * stride];
+    }
+};
+
 // Resumable closest-hit traversal of one lane: ray, best hit so far, current node
 // (leftFirst, count) and stack depth. trav_begin tests the root; each trav_step is
 // one iteration of BVHNode::traverse's loop (Geometry.h:448-512): a leaf's
@@ -101,8 +127,9 @@ __device__ __forceinline__ bool trav_begin(const DevScene& s, TravState& t, uint
 }
 
 // One loop iteration; false when the traversal has finished.
-template <bool COUNT>
-__device__ __forceinline__ bool trav_step(const DevScene& s, TravState& t, int* stk, uint32_t& cn, uint32_t& ct) {
+template <bool COUNT, class Stack>
+__device__ __forceinline__ bool trav_step(const DevScene& s, TravState& t, const Stack& stk, uint32_t& cn,
+                                          uint32_t& ct) {
     if (t.cnt > 0) {
         // triangles two at a time: both records are loaded before either test, the
         // tests run in leaf order (ties keep the first triangle, as Geometry.h:475)
@@ -128,7 +155,7 @@ __device__ __forceinline__ bool trav_step(const DevScene& s, TravState& t, int* 
             }
         }
         if (t.sp == 0) return false;
-        stk_pop(s.nodes, stk[(--t.sp) * kBlock], t.lf, t.cnt);
+        stk_pop(s.nodes, stk.pop(t.sp), t.lf, t.cnt);
         return true;
     }
     const float4* pr = reinterpret_cast<const float4*>(s.nodes) + 2 * (t.lf + 1);
@@ -147,19 +174,19 @@ __device__ __forceinline__ bool trav_step(const DevScene& s, TravState& t, int* 
     }
     if (dl == kFltMax) {
         if (t.sp == 0) return false;
-        stk_pop(s.nodes, stk[(--t.sp) * kBlock], t.lf, t.cnt);
+        stk_pop(s.nodes, stk.pop(t.sp), t.lf, t.cnt);
     } else {
         t.lf = lfl;
         t.cnt = cl;
-        if (dr != kFltMax) stk[(t.sp++) * kBlock] = stk_pack(nr, lfr, cr);
+        if (dr != kFltMax) stk.push(t.sp, stk_pack(nr, lfr, cr));
     }
     return true;
 }
 
-// Closest hit. stk: this lane's LDS stack (entries at stk[k * kBlock]).
-template <bool COUNT>
+// Closest hit. stk: this lane's traversal stack (LdsStack / SplitStack).
+template <bool COUNT, class Stack>
 __device__ __forceinline__ void traverse_closest(const DevScene& s, F3 o, F3 d, uint32_t& hid, float& ht,
-                                                 float& hu, float& hv, int* stk, uint32_t& cn, uint32_t& ct) {
+                                                 float& hu, float& hv, const Stack& stk, uint32_t& cn, uint32_t& ct) {
     TravState t;
     t.o = o;
     t.d = d;
@@ -194,8 +221,8 @@ __device__ __forceinline__ bool vis_begin(const DevScene& s, TravState& t, uint3
     return true;
 }
 
-template <bool COUNT>
-__device__ __forceinline__ int vis_step(const DevScene& s, TravState& t, int* stk, uint32_t& cn, uint32_t& ct) {
+template <bool COUNT, class Stack>
+__device__ __forceinline__ int vis_step(const DevScene& s, TravState& t, const Stack& stk, uint32_t& cn, uint32_t& ct) {
     if (t.cnt > 0) {
         for (int k = 0; k < t.cnt; k += 2) {
             const float4* q0 = reinterpret_cast<const float4*>(s.tris + t.lf + k);
@@ -211,7 +238,7 @@ __device__ __forceinline__ int vis_step(const DevScene& s, TravState& t, int* st
             if (mt_rows(a1, b1, c1, t.o, t.d, tt, u, v) && tt > kEps && tt < t.ht) return 2;
         }
         if (t.sp == 0) return 1;
-        stk_pop(s.nodes, stk[(--t.sp) * kBlock], t.lf, t.cnt);
+        stk_pop(s.nodes, stk.pop(t.sp), t.lf, t.cnt);
         return 0;
     }
     const float4* pr = reinterpret_cast<const float4*>(s.nodes) + 2 * (t.lf + 1);
@@ -230,18 +257,18 @@ __device__ __forceinline__ int vis_step(const DevScene& s, TravState& t, int* st
     }
     if (dl == kFltMax) {
         if (t.sp == 0) return 1;
-        stk_pop(s.nodes, stk[(--t.sp) * kBlock], t.lf, t.cnt);
+        stk_pop(s.nodes, stk.pop(t.sp), t.lf, t.cnt);
     } else {
         t.lf = lfl;
         t.cnt = cl;
-        if (dr != kFltMax) stk[(t.sp++) * kBlock] = stk_pack(nr, lfr, cr);
+        if (dr != kFltMax) stk.push(t.sp, stk_pack(nr, lfr, cr));
     }
     return 0;
 }
 
 // Any hit in (1e-4, max_t): returns true when unoccluded.
-template <bool COUNT>
-__device__ __forceinline__ bool traverse_visible(const DevScene& s, F3 o, F3 d, float max_t, int* stk,
+template <bool COUNT, class Stack>
+__device__ __forceinline__ bool traverse_visible(const DevScene& s, F3 o, F3 d, float max_t, const Stack& stk,
                                                  uint32_t& cn, uint32_t& ct) {
     TravState t;
     t.o = o;

@@ -55,6 +55,10 @@ struct PathState {
     uint32_t* counters; // [0],[1] ray queue sizes, [2] shadow queue size
     unsigned long long* stats;  // [0] closest rays, [1] shadow rays, [2] nodes, [3] tris, [4] sh nodes, [5] sh tris
     uint32_t capacity;
+    // traversal stack entries beyond the LDS share: (64 - 16) per lane of the traversal
+    // grid, lane-interleaved (spill_lanes lanes); nullptr when every entry fits in LDS
+    int* spill;
+    uint32_t spill_lanes;
     // batches of views over several scenes: view of each path, the scene table and
     // the view table (nullptr for single-view rendering)
     uint32_t* path_view;
@@ -99,12 +103,12 @@ hipError_t launch_bsdf_query(const MaterialRec& m, const float* sd, const float*
 
 hipError_t launch_raygen(const DevScene& s, const PathState& ps, const PassParams& p, hipStream_t stream);
 // order (nullable): trace order of the queue (a sorted permutation from launch_ray_keys)
-hipError_t launch_extend(const DevScene& s, const PathState& ps, int parity, int stack, bool count,
+hipError_t launch_extend(const DevScene& s, const PathState& ps, int parity, int lds_stack, bool count,
                          uint32_t grid, hipStream_t stream, bool multi = false, const uint32_t* order = nullptr,
                          bool dyn = false);
 hipError_t launch_shade(const DevScene& s, const PathState& ps, const PassParams& p, const GuideCtx& gc, int mode,
                         int parity, int depth, uint32_t grid, hipStream_t stream, bool multi = false);
-hipError_t launch_shadow(const DevScene& s, const PathState& ps, int stack, bool count, uint32_t grid,
+hipError_t launch_shadow(const DevScene& s, const PathState& ps, int lds_stack, bool count, uint32_t grid,
                          hipStream_t stream, bool multi = false, const uint32_t* order = nullptr, bool dyn = false);
 hipError_t launch_ray_keys(const PathState& ps, float3 lo, float inv_ext, int parity, int shadow, uint32_t capacity,
                            uint32_t* keys, uint32_t* vals, hipStream_t stream);
@@ -139,7 +143,7 @@ hipError_t launch_query_keys(const DevScene& s, const PathState& ps, float3 lo, 
 // mix != nullptr: learned-guide record pass (one-sample MIS with the mixtures of k_mlp_infer)
 hipError_t launch_shade_record(const DevScene& s, const PathState& ps, const PassParams& p, const RecordState& rs,
                                const float4* mix, int parity, int depth, uint32_t grid, hipStream_t stream);
-hipError_t launch_shadow_record(const DevScene& s, const PathState& ps, const RecordState& rs, int stack, uint32_t grid,
+hipError_t launch_shadow_record(const DevScene& s, const PathState& ps, const RecordState& rs, int lds_stack, uint32_t grid,
                                 hipStream_t stream);
 struct MlpView;
 hipError_t launch_mlp_infer(const DevScene& s, const PathState& ps, const MlpView& v, int parity, float4* mix,
@@ -154,6 +158,12 @@ struct GroupBuffers {
 hipError_t launch_mlp_infer_grouped(const PathState& ps, int parity, const MlpView* guides, uint32_t n_scenes,
                                     const GroupBuffers& gb, float4* mix, uint32_t grid, hipStream_t stream);
 int stack_entries(int max_depth);
-uint32_t trace_grid(int cu_count, int stack);
+// Render kernels keep the first lds_entries() traversal stack entries of a lane in LDS
+// (by default kLdsStackEntries) and the rest in PathState::spill (at most kSpillEntries
+// per lane); their launchers take that LDS count as `lds_stack`.
+constexpr int kLdsStackEntries = 16;
+constexpr int kSpillEntries = 64 - kLdsStackEntries;
+int lds_entries(int max_depth, uint32_t lds_cap, const PathState& ps);
+uint32_t trace_grid(int cu_count, int lds_stack);
 
 }  // namespace pg

@@ -14,6 +14,8 @@
 // pairs (64 contiguous bytes); triangles are stored in leaf order.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "device_math.h"
 #include "kernels.h"
 
@@ -23,6 +25,14 @@
 #include "mlp_device.h"
 
 namespace pg {
+
+// The traversal stack of this lane in a render kernel: STACK entries in the block's
+// LDS, deeper ones in ps.spill (one column per lane of the grid).
+template <int STACK>
+__device__ __forceinline__ SplitStack<STACK> lane_stack(int* lds, const PathState& ps) {
+    const uint32_t lane = blockIdx.x * kBlock + threadIdx.x;
+    return SplitStack<STACK>{lds + threadIdx.x, ps.spill ? ps.spill + lane : nullptr, ps.spill_lanes};
+}
 
 // ------------------------------------------------------------------ test-API kernels
 template <bool COUNT, int STACK>
@@ -37,7 +47,7 @@ __global__ __launch_bounds__(kBlock) void k_trace_closest(DevScene s, const floa
         F3 o{r[0], r[1], r[2]}, d{r[3], r[4], r[5]};
         uint32_t hid = kMiss;
         float ht = kFltMax, hu = 0.f, hv = 0.f;
-        traverse_closest<COUNT>(s, o, d, hid, ht, hu, hv, lds + threadIdx.x, cn, ct);
+        traverse_closest<COUNT>(s, o, d, hid, ht, hu, hv, LdsStack{lds + threadIdx.x}, cn, ct);
         id[i] = hid;
         t[i] = ht;
         a[i] = (hid == kMiss) ? 0.f : 1.f - (hu + hv);
@@ -59,7 +69,7 @@ __global__ __launch_bounds__(kBlock) void k_trace_shadow(DevScene s, const float
     const float* r = rays + 6 * (size_t)i;
     F3 o{r[0], r[1], r[2]}, d{r[3], r[4], r[5]};
     uint32_t cn = 0, ct = 0;
-    vis[i] = traverse_visible<false>(s, o, d, max_t[i], lds + threadIdx.x, cn, ct) ? 1 : 0;
+    vis[i] = traverse_visible<false>(s, o, d, max_t[i], LdsStack{lds + threadIdx.x}, cn, ct) ? 1 : 0;
 }
 
 __global__ void k_generate_rays(DevScene s, const float* pxy, uint32_t n, float* rays) {
@@ -133,7 +143,7 @@ __global__ __launch_bounds__(kBlock) void k_extend(DevScene s0, PathState ps, in
         uint32_t hid = kMiss;
         float ht = kFltMax, hu = 0.f, hv = 0.f;
         const DevScene& s = scene_of<MULTI>(s0, ps, MULTI ? ps.queue[parity][i] : 0u);
-        traverse_closest<COUNT>(s, o, d, hid, ht, hu, hv, lds + threadIdx.x, cn, ct);
+        traverse_closest<COUNT>(s, o, d, hid, ht, hu, hv, lane_stack<STACK>(lds, ps), cn, ct);
         ps.hit[i] = make_float4(__uint_as_float(hid), ht, hu, hv);
     }
     if (COUNT) {
@@ -157,7 +167,7 @@ __global__ __launch_bounds__(kBlock) void k_shadow(DevScene s0, PathState ps, co
         F3 o{a.x, a.y, a.z}, d{b.x, b.y, b.z};
         const uint32_t path = __float_as_uint(b.w);
         const DevScene& s = scene_of<MULTI>(s0, ps, path);
-        if (traverse_visible<COUNT>(s, o, d, a.w, lds + threadIdx.x, cn, ct)) {
+        if (traverse_visible<COUNT>(s, o, d, a.w, lane_stack<STACK>(lds, ps), cn, ct)) {
             float4 c = ps.sh_c[i];
             float4 L = ps.radiance[path];
             ps.radiance[path] = make_float4(L.x + c.x, L.y + c.y, L.z + c.z, L.w);
@@ -187,7 +197,7 @@ __global__ __launch_bounds__(kBlock) void k_extend_dyn(DevScene s0, PathState ps
     }
     const float4* ro = ps.ray_o[parity];
     const float4* rd = ps.ray_d[parity];
-    int* stk = lds + threadIdx.x;
+    const SplitStack<STACK> stk = lane_stack<STACK>(lds, ps);
     uint32_t cn = 0, ct = 0;
     TravState t;
     uint32_t slot = 0;
@@ -238,7 +248,7 @@ __global__ __launch_bounds__(kBlock) void k_shadow_dyn(DevScene s0, PathState ps
         ps.stats[1] += n;
         ps.counters[3] = 0;  // ray claims of the next persistent-lane extend
     }
-    int* stk = lds + threadIdx.x;
+    const SplitStack<STACK> stk = lane_stack<STACK>(lds, ps);
     uint32_t cn = 0, ct = 0;
     TravState t;
     uint32_t slot = 0, path = 0;
@@ -1036,7 +1046,7 @@ __global__ __launch_bounds__(kBlock) void k_shadow_record(DevScene s, PathState 
     for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
         float4 a = ps.sh_o[i], b = ps.sh_d[i];
         F3 o{a.x, a.y, a.z}, d{b.x, b.y, b.z};
-        if (traverse_visible<false>(s, o, d, a.w, lds + threadIdx.x, cn, ct)) {
+        if (traverse_visible<false>(s, o, d, a.w, lane_stack<STACK>(lds, ps), cn, ct)) {
             uint32_t path = __float_as_uint(b.w);
             float4 c = ps.sh_c[i];
             size_t off = (size_t)__float_as_int(c.w) * rs.stride + path;
@@ -1195,14 +1205,15 @@ hipError_t launch_raygen(const DevScene& s, const PathState& ps, const PassParam
     hipLaunchKernelGGL(k_raygen, dim3((n + 255) / 256), dim3(256), 0, stream, s, ps, p);
     return hipGetLastError();
 }
-hipError_t launch_extend(const DevScene& s, const PathState& ps, int parity, int stack, bool count, uint32_t grid,
+hipError_t launch_extend(const DevScene& s, const PathState& ps, int parity, int lds_stack, bool count, uint32_t grid,
                          hipStream_t stream, bool multi, const uint32_t* order, bool dyn) {
+    if (ps.spill && grid * (uint32_t)kBlock > ps.spill_lanes) grid = ps.spill_lanes / kBlock;
     if (dyn) {
 #define PG_CALL(S)                                                                                                   \
     if (multi) hipLaunchKernelGGL((k_extend_dyn<false, S, true>), dim3(grid), dim3(kBlock), 0, stream, s, ps, parity); \
     else if (count) hipLaunchKernelGGL((k_extend_dyn<true, S, false>), dim3(grid), dim3(kBlock), 0, stream, s, ps, parity); \
     else hipLaunchKernelGGL((k_extend_dyn<false, S, false>), dim3(grid), dim3(kBlock), 0, stream, s, ps, parity)
-        PG_STACK_DISPATCH(stack, PG_CALL);
+        PG_STACK_DISPATCH(lds_stack, PG_CALL);
 #undef PG_CALL
         return hipGetLastError();
     }
@@ -1210,7 +1221,7 @@ hipError_t launch_extend(const DevScene& s, const PathState& ps, int parity, int
     if (multi) hipLaunchKernelGGL((k_extend<false, S, true>), dim3(grid), dim3(kBlock), 0, stream, s, ps, parity, order); \
     else if (count) hipLaunchKernelGGL((k_extend<true, S, false>), dim3(grid), dim3(kBlock), 0, stream, s, ps, parity, order); \
     else hipLaunchKernelGGL((k_extend<false, S, false>), dim3(grid), dim3(kBlock), 0, stream, s, ps, parity, order)
-    PG_STACK_DISPATCH(stack, PG_CALL);
+    PG_STACK_DISPATCH(lds_stack, PG_CALL);
 #undef PG_CALL
     return hipGetLastError();
 }
@@ -1233,14 +1244,15 @@ hipError_t launch_shade(const DevScene& s, const PathState& ps, const PassParams
     }
     return hipGetLastError();
 }
-hipError_t launch_shadow(const DevScene& s, const PathState& ps, int stack, bool count, uint32_t grid,
+hipError_t launch_shadow(const DevScene& s, const PathState& ps, int lds_stack, bool count, uint32_t grid,
                          hipStream_t stream, bool multi, const uint32_t* order, bool dyn) {
+    if (ps.spill && grid * (uint32_t)kBlock > ps.spill_lanes) grid = ps.spill_lanes / kBlock;
     if (dyn) {
 #define PG_CALL(S)                                                                                          \
     if (multi) hipLaunchKernelGGL((k_shadow_dyn<false, S, true>), dim3(grid), dim3(kBlock), 0, stream, s, ps);  \
     else if (count) hipLaunchKernelGGL((k_shadow_dyn<true, S, false>), dim3(grid), dim3(kBlock), 0, stream, s, ps); \
     else hipLaunchKernelGGL((k_shadow_dyn<false, S, false>), dim3(grid), dim3(kBlock), 0, stream, s, ps)
-        PG_STACK_DISPATCH(stack, PG_CALL);
+        PG_STACK_DISPATCH(lds_stack, PG_CALL);
 #undef PG_CALL
         return hipGetLastError();
     }
@@ -1248,7 +1260,7 @@ hipError_t launch_shadow(const DevScene& s, const PathState& ps, int stack, bool
     if (multi) hipLaunchKernelGGL((k_shadow<false, S, true>), dim3(grid), dim3(kBlock), 0, stream, s, ps, order);  \
     else if (count) hipLaunchKernelGGL((k_shadow<true, S, false>), dim3(grid), dim3(kBlock), 0, stream, s, ps, order); \
     else hipLaunchKernelGGL((k_shadow<false, S, false>), dim3(grid), dim3(kBlock), 0, stream, s, ps, order)
-    PG_STACK_DISPATCH(stack, PG_CALL);
+    PG_STACK_DISPATCH(lds_stack, PG_CALL);
 #undef PG_CALL
     return hipGetLastError();
 }
@@ -1400,10 +1412,11 @@ hipError_t launch_shade_record(const DevScene& s, const PathState& ps, const Pas
         hipLaunchKernelGGL((k_shade_record<false, 0>), dim3(grid), dim3(kBlock), 0, stream, s, ps, p, rs, nullptr, parity, depth);
     return hipGetLastError();
 }
-hipError_t launch_shadow_record(const DevScene& s, const PathState& ps, const RecordState& rs, int stack, uint32_t grid,
+hipError_t launch_shadow_record(const DevScene& s, const PathState& ps, const RecordState& rs, int lds_stack, uint32_t grid,
                                 hipStream_t stream) {
+    if (ps.spill && grid * (uint32_t)kBlock > ps.spill_lanes) grid = ps.spill_lanes / kBlock;
 #define PG_CALL(S) hipLaunchKernelGGL((k_shadow_record<S>), dim3(grid), dim3(kBlock), 0, stream, s, ps, rs)
-    PG_STACK_DISPATCH(stack, PG_CALL);
+    PG_STACK_DISPATCH(lds_stack, PG_CALL);
 #undef PG_CALL
     return hipGetLastError();
 }
@@ -1415,10 +1428,19 @@ int stack_entries(int max_depth) {
     return 64;
 }
 
+// LDS entries of a render kernel's traversal stack: the whole depth without a spill
+// buffer, else min(depth, lds_cap rounded up to 16) with the rest in global memory
+// (lds_cap 0: kLdsStackEntries).
+int lds_entries(int max_depth, uint32_t lds_cap, const PathState& ps) {
+    const int whole = stack_entries(max_depth);
+    if (!ps.spill) return whole;
+    return std::min(whole, stack_entries(lds_cap ? (int)lds_cap : kLdsStackEntries));
+}
+
 // Persistent grid for the traversal kernels: as many 128-lane blocks per CU as
-// the LDS stacks (160 KB per CU) allow.
-uint32_t trace_grid(int cu_count, int stack) {
-    int per_cu = (160 * 1024) / (stack_entries(stack) * kBlock * 4);
+// the LDS stacks (160 KB per CU) allow, at most 16.
+uint32_t trace_grid(int cu_count, int lds_stack) {
+    int per_cu = (160 * 1024) / (stack_entries(lds_stack) * kBlock * 4);
     if (per_cu > 16) per_cu = 16;
     if (per_cu < 1) per_cu = 1;
     return (uint32_t)(cu_count * per_cu);

@@ -185,6 +185,41 @@ def test_ray_sorting_keeps_images(pg, device, small_interior):
 
 
 @pytest.mark.gpu
+def test_lds_stack_split_keeps_images(pg, device):
+    """Traversal stacks with 16, 32 or 64 entries per lane in LDS and the deeper ones in
+    global memory: the ~1.09M-triangle interior's BVH is 26 levels deep, so with 16
+    entries in LDS the deep pushes and pops go to global memory. Films and traversal
+    counts are identical for every split, with and without persistent lanes; other
+    values are refused."""
+    import dataclasses
+    d = pg.generate_scene(2, 0)
+    d = dataclasses.replace(d, camera=dict(d.camera, width=320, height=180))
+    sc = pg.Scene(device, d)
+    assert sc.bvh_info()["max_depth"] > 16
+    films, counts = [], []
+    for kw in (dict(stack_lds=64), dict(stack_lds=16), dict(stack_lds=32), dict(stack_lds=16, dyn_fetch=1),
+               dict(stack_lds=16, trace_blocks_per_cu=12), dict(stack_lds=0, dyn_fetch=1, trace_blocks_per_cu=3)):
+        rt = pg.RayTracer(sc, seed=5, max_depth=12, **kw)
+        rt.set_counting(True)
+        rt.reset_stats()
+        rt.clear()
+        rt.render(4)
+        films.append(rt.film()[0])
+        st = rt.stats()
+        counts.append(tuple(st[k] for k in ("closest_rays", "shadow_rays", "nodes_visited", "tris_tested",
+                                             "shadow_nodes_visited", "shadow_tris_tested")))
+        rt.set_counting(False)
+    for f, c in zip(films[1:], counts[1:]):
+        assert np.array_equal(films[0].view(np.uint32), f.view(np.uint32))
+        assert c == counts[0]
+    assert films[0].sum() > 0
+    for bad in (8, 80):
+        rt = pg.RayTracer(sc, seed=5, max_depth=12, stack_lds=bad)
+        with pytest.raises(pg.PgError):
+            rt.render(1)
+
+
+@pytest.mark.gpu
 def test_samples_per_pass_keep_images(pg, device, small_interior):
     """Several samples traced together in one wavefront pass (samples_per_pass) change
     only which rays share a launch: films are bit-identical to one sample per pass, also
