@@ -427,7 +427,9 @@ pg_status pg_scene_upload(pg_ctx* c, const float* positions, const float* normal
         return c->fail(PG_ERR_OVERFLOW, "BVH deeper than the 64-entry traversal stack (Geometry.h:461)");
     PreparedScene& h = c->host;
     PG_TRY(c, dalloc(&c->d_nodes, h.node_slots.size()), "alloc nodes");
-    PG_TRY(c, dalloc(&c->d_tris, h.tris.size()), "alloc tris");
+    // one zeroed record past the end: traversal reads 64 bytes at every triangle
+    PG_TRY(c, dalloc(&c->d_tris, h.tris.size() + 1), "alloc tris");
+    PG_TRY(c, hipMemset(c->d_tris + h.tris.size(), 0, sizeof(TriIsect)), "clear tris pad");
     PG_TRY(c, dalloc(&c->d_shade, h.shade.size()), "alloc shade");
     PG_TRY(c, dalloc(&c->d_mats, h.mats.size()), "alloc mats");
     PG_TRY(c, dalloc(&c->d_lights, h.lights.size() ? h.lights.size() : 1), "alloc lights");
@@ -455,6 +457,7 @@ pg_status pg_scene_upload(pg_ctx* c, const float* positions, const float* normal
     d.env_w = h.has_env ? h.env.w : 0;
     d.env_h = h.has_env ? h.env.h : 0;
     d.has_env = h.has_env ? 1 : 0;
+    d.finite_bounds = h.finite_bounds ? 1 : 0;
     d.centre[0] = h.centre[0]; d.centre[1] = h.centre[1]; d.centre[2] = h.centre[2];
     d.radius = h.radius;
     d.cam = h.camera;

@@ -74,7 +74,9 @@ struct SceneBuf {
         hipError_t e;
 #define PG_B(x) if ((e = (x)) != hipSuccess) return e
         PG_B(balloc(&nodes, h.node_slots.size()));
-        PG_B(balloc(&tris, h.tris.size()));
+        // one zeroed record past the end: traversal reads 64 bytes at every triangle
+        PG_B(balloc(&tris, h.tris.size() + 1));
+        PG_B(hipMemset(tris + h.tris.size(), 0, sizeof(TriIsect)));
         PG_B(balloc(&shade, h.shade.size()));
         PG_B(balloc(&mats, h.mats.size()));
         PG_B(balloc(&lights, h.lights.size() ? h.lights.size() : 1));
@@ -103,6 +105,7 @@ struct SceneBuf {
         d.env_w = h.has_env ? h.env.w : 0;
         d.env_h = h.has_env ? h.env.h : 0;
         d.has_env = h.has_env ? 1 : 0;
+        d.finite_bounds = h.finite_bounds ? 1 : 0;
         d.centre[0] = h.centre[0]; d.centre[1] = h.centre[1]; d.centre[2] = h.centre[2];
         d.radius = h.radius;
         d.cam = h.camera;

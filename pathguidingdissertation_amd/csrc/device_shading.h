@@ -26,6 +26,20 @@ __device__ __forceinline__ bool slab(float4 lo, float4 hi, F3 o, F3 inv, float& 
     return true;
 }
 
+// slab() with IEEE min/max (v_min/v_max, three-operand forms): the same hit decision
+// and entry distance whenever no product is NaN, i.e. with finite box bounds, origin
+// and 1/d; the ternary min/max of slab() differ from IEEE min/max only on NaN operands
+// and on +0/-0 ties, and no comparison made with the result tells the zeros apart.
+__device__ __forceinline__ bool slab_ieee(float4 lo, float4 hi, F3 o, F3 inv, float& t) {
+    float t0x = (lo.x - o.x) * inv.x, t0y = (lo.y - o.y) * inv.y, t0z = (lo.z - o.z) * inv.z;
+    float t1x = (hi.x - o.x) * inv.x, t1y = (hi.y - o.y) * inv.y, t1z = (hi.z - o.z) * inv.z;
+    float fe = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fminf(t0z, t1z));
+    float fx = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z));
+    if (fe > fx || fx < 0.f) return false;
+    t = fmaxf(0.f, fe);
+    return true;
+}
+
 // Triangle::rayIntersect (Geometry.h:101-130) on a leaf-ordered record already
 // loaded as its three float4 rows (v0 + id, e0, e1).
 __device__ __forceinline__ bool mt_rows(float4 a, float4 b, float4 c, F3 o, F3 d, float& t, float& u, float& v) {
@@ -84,10 +98,14 @@ struct LdsStack {
 // reached) in global memory: entry k >= NL at spill[(k - NL) * stride]. A smaller LDS
 // share per lane lets more waves run on a CU; the entries and their order are those of
 // the all-LDS stack, so every traversal visits the same nodes.
+// The two parts carry their address spaces in their pointer types (LDS: 3, global: 1)
+// so that a pop stays a ds_read plus a rare global load instead of one generic load.
+typedef __attribute__((address_space(3))) int lds_int;
+typedef __attribute__((address_space(1))) int global_int;
 template <int NL>
 struct SplitStack {
-    int* lds;
-    int* spill;
+    lds_int* lds;
+    global_int* spill;
     uint32_t stride;
     __device__ __forceinline__ void push(int& sp, int e) const {
         if (sp < NL) lds[sp * kBlock] = e;
@@ -96,24 +114,50 @@ struct SplitStack {
     }
     __device__ __forceinline__ int pop(int& sp) const {
         --sp;
-        return sp < NL ? lds[sp * kBlock] : spill[(size_t)(sp - NL) * stride];
+        int e = lds[(sp < NL ? sp : NL - 1) * kBlock];
+        if (sp >= NL) e = spill[(size_t)(sp - NL) * stride];
+        return e;
     }
 };
 
 // Resumable closest-hit traversal of one lane: ray, best hit so far, current node
-// (leftFirst, count) and stack depth. trav_begin tests the root; each trav_step is
-// one iteration of BVHNode::traverse's loop (Geometry.h:448-512): a leaf's
-// triangles then a pop, or an interior node's two children (near first, far pushed).
+// (leftFirst, count), next triangle of a leaf and stack depth. trav_begin tests the
+// root; each trav_step advances BVHNode::traverse's loop (Geometry.h:448-512) by one
+// triangle or one interior node.
 struct TravState {
     F3 o, d, inv;
     float ht, hu, hv;
     uint32_t hid;
     int lf, cnt, sp;
+    int k;      // next triangle of the current leaf (0 at an interior node)
+    bool ieee;  // box tests may use slab_ieee (no NaN can arise for this ray)
 };
+
+// Box tests of this ray may use IEEE min/max: finite scene bounds, origin and 1/d.
+__device__ __forceinline__ bool ieee_boxes(const DevScene& s, const TravState& t) {
+    return s.finite_bounds && isfinite(t.inv.x) && isfinite(t.inv.y) && isfinite(t.inv.z) && isfinite(t.o.x) &&
+           isfinite(t.o.y) && isfinite(t.o.z);
+}
+
+// Entry distances of an interior node's two children, kFltMax for a child that is
+// missed or starts beyond the current hit (Geometry.h:486-497).
+__device__ __forceinline__ void child_dists(const TravState& t, float4 a0, float4 a1, float4 b0, float4 b1, float& dl,
+                                            float& dr) {
+    dl = 0.f;
+    dr = 0.f;
+    if (t.ieee) {
+        if (!slab_ieee(a0, a1, t.o, t.inv, dl) || dl >= t.ht) dl = kFltMax;
+        if (!slab_ieee(b0, b1, t.o, t.inv, dr) || dr >= t.ht) dr = kFltMax;
+    } else {
+        if (!slab(a0, a1, t.o, t.inv, dl) || dl >= t.ht) dl = kFltMax;
+        if (!slab(b0, b1, t.o, t.inv, dr) || dr >= t.ht) dr = kFltMax;
+    }
+}
 
 template <bool COUNT>
 __device__ __forceinline__ bool trav_begin(const DevScene& s, TravState& t, uint32_t& cn) {
     t.inv = F3{1.f / t.d.x, 1.f / t.d.y, 1.f / t.d.z};
+    t.ieee = ieee_boxes(s, t);
     const float4* nf = reinterpret_cast<const float4*>(s.nodes);
     float4 r0 = nf[2], r1 = nf[3];  // root at slot 1
     float tb;
@@ -123,62 +167,60 @@ __device__ __forceinline__ bool trav_begin(const DevScene& s, TravState& t, uint
     t.lf = f2i(r0.w);
     t.cnt = f2i(r1.w);
     t.sp = 0;
+    t.k = 0;
     return true;
 }
 
-// One loop iteration; false when the traversal has finished.
+// One step of one lane: a single triangle of the current leaf, or an interior node's
+// two children (near first, far pushed); a leaf's last triangle or a node whose
+// children are both missed pops the stack. Every lane of a wave issues the same four
+// 16-byte loads per step (the node pair, or a 48-byte triangle record and 16 bytes
+// after it, hence the padded triangle array), whichever kind of step it takes; the
+// per-lane order of tests is that of BVHNode::traverse (Geometry.h:448-512): triangles
+// in leaf order (ties keep the first, as :475), children in entry order. False when
+// the traversal has finished.
 template <bool COUNT, class Stack>
 __device__ __forceinline__ bool trav_step(const DevScene& s, TravState& t, const Stack& stk, uint32_t& cn,
                                           uint32_t& ct) {
-    if (t.cnt > 0) {
-        // triangles two at a time: both records are loaded before either test, the
-        // tests run in leaf order (ties keep the first triangle, as Geometry.h:475)
-        for (int k = 0; k < t.cnt; k += 2) {
-            const float4* q0 = reinterpret_cast<const float4*>(s.tris + t.lf + k);
-            const bool two = k + 1 < t.cnt;
-            const float4* q1 = two ? q0 + 3 : q0;
-            const float4 a0 = q0[0], b0 = q0[1], c0 = q0[2];
-            const float4 a1 = q1[0], b1 = q1[1], c1 = q1[2];
-            float tt, u, v;
-            if (COUNT) ct += two ? 2 : 1;
-            if (mt_rows(a0, b0, c0, t.o, t.d, tt, u, v) && tt > kEps && tt < t.ht) {
-                t.ht = tt;
-                t.hid = __float_as_uint(a0.w);
-                t.hu = u;
-                t.hv = v;
-            }
-            if (two && mt_rows(a1, b1, c1, t.o, t.d, tt, u, v) && tt > kEps && tt < t.ht) {
-                t.ht = tt;
-                t.hid = __float_as_uint(a1.w);
-                t.hu = u;
-                t.hv = v;
-            }
+    const bool leaf = t.cnt > 0;
+    const float4* q = leaf ? reinterpret_cast<const float4*>(s.tris + t.lf + t.k)
+                           : reinterpret_cast<const float4*>(s.nodes) + 2 * (t.lf + 1);
+    const float4 q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3];
+    bool pop;
+    if (leaf) {
+        float tt, u, v;
+        if (COUNT) ct += 1;
+        if (mt_rows(q0, q1, q2, t.o, t.d, tt, u, v) && tt > kEps && tt < t.ht) {
+            t.ht = tt;
+            t.hid = __float_as_uint(q0.w);
+            t.hu = u;
+            t.hv = v;
         }
-        if (t.sp == 0) return false;
-        stk_pop(s.nodes, stk.pop(t.sp), t.lf, t.cnt);
-        return true;
-    }
-    const float4* pr = reinterpret_cast<const float4*>(s.nodes) + 2 * (t.lf + 1);
-    float4 a0 = pr[0], a1 = pr[1], b0 = pr[2], b1 = pr[3];
-    if (COUNT) cn += 2;
-    float dl = 0.f, dr = 0.f;
-    if (!slab(a0, a1, t.o, t.inv, dl) || dl >= t.ht) dl = kFltMax;
-    if (!slab(b0, b1, t.o, t.inv, dr) || dr >= t.ht) dr = kFltMax;
-    int nl = t.lf, nr = t.lf + 1;
-    int lfl = f2i(a0.w), cl = f2i(a1.w), lfr = f2i(b0.w), cr = f2i(b1.w);
-    if (dl > dr) {
-        float tt = dl; dl = dr; dr = tt;
-        int ti = nl; nl = nr; nr = ti;
-        ti = lfl; lfl = lfr; lfr = ti;
-        ti = cl; cl = cr; cr = ti;
-    }
-    if (dl == kFltMax) {
-        if (t.sp == 0) return false;
-        stk_pop(s.nodes, stk.pop(t.sp), t.lf, t.cnt);
+        pop = ++t.k == t.cnt;
     } else {
-        t.lf = lfl;
-        t.cnt = cl;
-        if (dr != kFltMax) stk.push(t.sp, stk_pack(nr, lfr, cr));
+        if (COUNT) cn += 2;
+        float dl, dr;
+        child_dists(t, q0, q1, q2, q3, dl, dr);
+        int nr = t.lf + 1;
+        int lfl = f2i(q0.w), cl = f2i(q1.w), lfr = f2i(q2.w), cr = f2i(q3.w);
+        if (dl > dr) {
+            float tt = dl; dl = dr; dr = tt;
+            nr = t.lf;
+            int ti = lfl; lfl = lfr; lfr = ti;
+            ti = cl; cl = cr; cr = ti;
+        }
+        pop = dl == kFltMax;
+        if (!pop) {
+            t.lf = lfl;
+            t.cnt = cl;
+            t.k = 0;
+            if (dr != kFltMax) stk.push(t.sp, stk_pack(nr, lfr, cr));
+        }
+    }
+    if (pop) {
+        if (t.sp == 0) return false;
+        stk_pop(s.nodes, stk.pop(t.sp), t.lf, t.cnt);
+        t.k = 0;
     }
     return true;
 }
@@ -209,6 +251,7 @@ __device__ __forceinline__ void traverse_closest(const DevScene& s, F3 o, F3 d, 
 template <bool COUNT>
 __device__ __forceinline__ bool vis_begin(const DevScene& s, TravState& t, uint32_t& cn) {
     t.inv = F3{1.f / t.d.x, 1.f / t.d.y, 1.f / t.d.z};
+    t.ieee = ieee_boxes(s, t);
     const float4* nf = reinterpret_cast<const float4*>(s.nodes);
     float4 r0 = nf[2], r1 = nf[3];
     float tb;
@@ -218,50 +261,48 @@ __device__ __forceinline__ bool vis_begin(const DevScene& s, TravState& t, uint3
     t.lf = f2i(r0.w);
     t.cnt = f2i(r1.w);
     t.sp = 0;
+    t.k = 0;
     return true;
 }
 
+// One step of an any-hit traversal, as trav_step (same loads for every lane):
+// 0 while running, 1 when the ray is unoccluded, 2 when a triangle occludes it.
 template <bool COUNT, class Stack>
 __device__ __forceinline__ int vis_step(const DevScene& s, TravState& t, const Stack& stk, uint32_t& cn, uint32_t& ct) {
-    if (t.cnt > 0) {
-        for (int k = 0; k < t.cnt; k += 2) {
-            const float4* q0 = reinterpret_cast<const float4*>(s.tris + t.lf + k);
-            const bool two = k + 1 < t.cnt;
-            const float4* q1 = two ? q0 + 3 : q0;
-            const float4 a0 = q0[0], b0 = q0[1], c0 = q0[2];
-            const float4 a1 = q1[0], b1 = q1[1], c1 = q1[2];
-            float tt, u, v;
-            if (COUNT) ct += 1;
-            if (mt_rows(a0, b0, c0, t.o, t.d, tt, u, v) && tt > kEps && tt < t.ht) return 2;
-            if (!two) break;
-            if (COUNT) ct += 1;
-            if (mt_rows(a1, b1, c1, t.o, t.d, tt, u, v) && tt > kEps && tt < t.ht) return 2;
-        }
-        if (t.sp == 0) return 1;
-        stk_pop(s.nodes, stk.pop(t.sp), t.lf, t.cnt);
-        return 0;
-    }
-    const float4* pr = reinterpret_cast<const float4*>(s.nodes) + 2 * (t.lf + 1);
-    float4 a0 = pr[0], a1 = pr[1], b0 = pr[2], b1 = pr[3];
-    if (COUNT) cn += 2;
-    float dl = 0.f, dr = 0.f;
-    if (!slab(a0, a1, t.o, t.inv, dl) || dl >= t.ht) dl = kFltMax;
-    if (!slab(b0, b1, t.o, t.inv, dr) || dr >= t.ht) dr = kFltMax;
-    int nr = t.lf + 1;
-    int lfl = f2i(a0.w), cl = f2i(a1.w), lfr = f2i(b0.w), cr = f2i(b1.w);
-    if (dl > dr) {
-        float tt = dl; dl = dr; dr = tt;
-        nr = t.lf;
-        int ti = lfl; lfl = lfr; lfr = ti;
-        ti = cl; cl = cr; cr = ti;
-    }
-    if (dl == kFltMax) {
-        if (t.sp == 0) return 1;
-        stk_pop(s.nodes, stk.pop(t.sp), t.lf, t.cnt);
+    const bool leaf = t.cnt > 0;
+    const float4* q = leaf ? reinterpret_cast<const float4*>(s.tris + t.lf + t.k)
+                           : reinterpret_cast<const float4*>(s.nodes) + 2 * (t.lf + 1);
+    const float4 q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3];
+    bool pop;
+    if (leaf) {
+        float tt, u, v;
+        if (COUNT) ct += 1;
+        if (mt_rows(q0, q1, q2, t.o, t.d, tt, u, v) && tt > kEps && tt < t.ht) return 2;
+        pop = ++t.k == t.cnt;
     } else {
-        t.lf = lfl;
-        t.cnt = cl;
-        if (dr != kFltMax) stk.push(t.sp, stk_pack(nr, lfr, cr));
+        if (COUNT) cn += 2;
+        float dl, dr;
+        child_dists(t, q0, q1, q2, q3, dl, dr);
+        int nr = t.lf + 1;
+        int lfl = f2i(q0.w), cl = f2i(q1.w), lfr = f2i(q2.w), cr = f2i(q3.w);
+        if (dl > dr) {
+            float tt = dl; dl = dr; dr = tt;
+            nr = t.lf;
+            int ti = lfl; lfl = lfr; lfr = ti;
+            ti = cl; cl = cr; cr = ti;
+        }
+        pop = dl == kFltMax;
+        if (!pop) {
+            t.lf = lfl;
+            t.cnt = cl;
+            t.k = 0;
+            if (dr != kFltMax) stk.push(t.sp, stk_pack(nr, lfr, cr));
+        }
+    }
+    if (pop) {
+        if (t.sp == 0) return 1;
+        stk_pop(s.nodes, stk.pop(t.sp), t.lf, t.cnt);
+        t.k = 0;
     }
     return 0;
 }

@@ -352,6 +352,10 @@ std::string prepare_scene(const float* pos, const float* nrm, const uint32_t* ma
     else build_bvh(pos, n_tris, out.bvh);
     out.node_slots.assign(out.bvh.nodes.size() + 2, BvhNode{});
     for (size_t i = 0; i < out.bvh.nodes.size(); ++i) out.node_slots[i + 1] = out.bvh.nodes[i];
+    out.finite_bounds = true;
+    for (const BvhNode& b : out.bvh.nodes)
+        for (int c = 0; c < 3; ++c)
+            if (!std::isfinite(b.bmin[c]) || !std::isfinite(b.bmax[c])) out.finite_bounds = false;
     out.tris.resize(n_tris);
     for (uint32_t k = 0; k < n_tris; ++k) {
         uint32_t t = (uint32_t)out.bvh.tri_index[k];

@@ -118,6 +118,7 @@ struct PreparedScene {
     CameraRec camera;
     float centre[3] = {0, 0, 0};
     float radius = 0.f;
+    bool finite_bounds = true;  // every node bound finite (traversal may use IEEE min/max)
 };
 // Device material record of a pg_material (false: unknown type).
 bool material_record(const pg_material& m, MaterialRec& r);

@@ -25,6 +25,7 @@ struct DevScene {
     const float2* uv;           // per triangle ID: (u, v) of its three vertices
     int32_t n_lights;
     int32_t env_w, env_h, has_env;
+    int32_t finite_bounds;      // 1: every BVH node bound is finite
     float centre[3];
     float radius;
     CameraRec cam;

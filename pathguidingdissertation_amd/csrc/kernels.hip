@@ -31,7 +31,8 @@ namespace pg {
 template <int STACK>
 __device__ __forceinline__ SplitStack<STACK> lane_stack(int* lds, const PathState& ps) {
     const uint32_t lane = blockIdx.x * kBlock + threadIdx.x;
-    return SplitStack<STACK>{lds + threadIdx.x, ps.spill ? ps.spill + lane : nullptr, ps.spill_lanes};
+    return SplitStack<STACK>{(lds_int*)(lds + threadIdx.x), ps.spill ? (global_int*)(ps.spill + lane) : nullptr,
+                             ps.spill_lanes};
 }
 
 // ------------------------------------------------------------------ test-API kernels
