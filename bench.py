@@ -452,7 +452,7 @@ def main() -> int:
                     help="add the primary-sample-space guide (exact and approximate neighbours) to the study")
     ap.add_argument("--shape", default="",
                     help="fixed launch shape instead of the tuner's pick, as comma-separated render "
-                         "parameters, e.g. stack_lds=16,dyn_fetch=1,shade_blocks_per_cu=4 (profiling runs)")
+                         "parameters, e.g. stack_lds=16,dyn_fetch=0,shade_blocks_per_cu=3 (profiling runs)")
     ap.add_argument("--copy-check", action="store_true",
                     help="also time a plain 1 GiB device copy (HBM reference point / counter scaling)")
     a = ap.parse_args()
@@ -506,7 +506,7 @@ def main() -> int:
     tune_s = time.perf_counter() - t_tune
     shape = {"stack_lds": rt.params.stack_lds or 16,
              "trace_blocks_per_cu": rt.params.trace_blocks_per_cu or "lds-max",
-             "shade_blocks_per_cu": rt.params.shade_blocks_per_cu or 8,
+             "shade_blocks_per_cu": rt.params.shade_blocks_per_cu or 3,
              "extend_kernel": "k_extend_dyn" if rt.params.dyn_fetch else "k_extend",
              "samples_per_pass": rt.params.samples_per_pass or "auto", "tune_s": round(tune_s, 2),
              "tuned_ms_per_spp": round(tuned_ms, 3) if tuned_ms == tuned_ms else None,

@@ -94,7 +94,7 @@ typedef struct {
                                reference); 1 records Li with NEE and MIS-weighted
                                emission (the pixelColour recursion, :391)       */
     uint32_t trace_blocks_per_cu;  /* traversal grid (0: as many as LDS allows)   */
-    uint32_t shade_blocks_per_cu;  /* shade grid (0: 8)                           */
+    uint32_t shade_blocks_per_cu;  /* shade grid, blocks of 256 lanes (0: 3)     */
     uint32_t sort_rays;     /* 1: trace secondary and shadow rays in (direction
                                octant, origin Morton) order; results unchanged  */
     uint32_t dyn_fetch;     /* 1: persistent lanes claim new rays when a wave runs
@@ -392,7 +392,7 @@ typedef struct pg_batch pg_batch;
 typedef struct pg_batch_config {
     uint32_t queue_paths;         /* paths per chunk (0: all views in one chunk)      */
     uint32_t trace_blocks_per_cu; /* traversal grid (0: as many as the LDS allows)    */
-    uint32_t shade_blocks_per_cu; /* shade grid (0: 8)                                */
+    uint32_t shade_blocks_per_cu; /* shade grid, blocks of 256 lanes (0: 3)       */
     uint32_t stack;               /* traversal stack entries (0: deepest BVH, rounded
                                      up to 16; must hold the deepest BVH)             */
     uint32_t use_graph;           /* replay a captured hipGraph per SPP               */

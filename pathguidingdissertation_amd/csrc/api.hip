@@ -577,10 +577,10 @@ pg_status pg_trace_closest(pg_ctx* c, const pg_ray* rays, uint32_t n, pg_hit* hi
         if ((st = c->hip(hipStreamSynchronize(c->stream), "sync")) != PG_OK) break;
         std::vector<uint32_t> id(n);
         std::vector<float> t(n), a(n), b(n), g(n);
-        hipMemcpy(id.data(), did, n * 4, hipMemcpyDeviceToHost);
-        hipMemcpy(t.data(), dt, n * 4, hipMemcpyDeviceToHost);
-        hipMemcpy(a.data(), da, n * 4, hipMemcpyDeviceToHost);
-        hipMemcpy(b.data(), db, n * 4, hipMemcpyDeviceToHost);
+        if ((st = c->hip(hipMemcpy(id.data(), did, n * 4, hipMemcpyDeviceToHost), "copy hits")) != PG_OK) break;
+        if ((st = c->hip(hipMemcpy(t.data(), dt, n * 4, hipMemcpyDeviceToHost), "copy hits")) != PG_OK) break;
+        if ((st = c->hip(hipMemcpy(a.data(), da, n * 4, hipMemcpyDeviceToHost), "copy hits")) != PG_OK) break;
+        if ((st = c->hip(hipMemcpy(b.data(), db, n * 4, hipMemcpyDeviceToHost), "copy hits")) != PG_OK) break;
         if ((st = c->hip(hipMemcpy(g.data(), dg, n * 4, hipMemcpyDeviceToHost), "copy hits")) != PG_OK) break;
         for (uint32_t i = 0; i < n; ++i) hits[i] = pg_hit{id[i], t[i], a[i], b[i], g[i]};
     } while (0);
@@ -920,7 +920,7 @@ pg_status pg_render_spp(pg_ctx* c, const pg_render_params* params, uint32_t n_sp
     const int stack = lds_entries((int)c->host.bvh.max_depth, prm.stack_lds, c->ps);
     const uint32_t grid_trace = prm.trace_blocks_per_cu ? (uint32_t)c->cu_count * prm.trace_blocks_per_cu
                                                         : trace_grid(c->cu_count, stack);
-    const uint32_t grid_shade = (uint32_t)c->cu_count * (prm.shade_blocks_per_cu ? prm.shade_blocks_per_cu : 8u);
+    const uint32_t grid_shade = (uint32_t)c->cu_count * (prm.shade_blocks_per_cu ? prm.shade_blocks_per_cu : kShadeBlocksPerCu);
     const uint32_t grid_guide = (uint32_t)c->cu_count * (c->gv.levels <= 16 ? 10u : 5u);
     const uint32_t grid_mlp = (uint32_t)c->cu_count * 3;  // 48 KB of LDS per block: 3 blocks per CU
     GuideCtx gc{c->gv, (mode == 1 || mode == 3) ? c->guide_cells : nullptr, mode == 2 ? c->mix : nullptr};
@@ -1062,7 +1062,7 @@ pg_status pg_render_tune(pg_ctx* c, const pg_render_params* params, pg_render_pa
     if (st != PG_OK) return st;
     const uint32_t lds_cand[] = {0u, 32u, 64u};
     const uint32_t trace_cand[] = {0u, 4u, 6u, 8u, 12u};
-    const uint32_t shade_cand[] = {4u, 8u, 16u};
+    const uint32_t shade_cand[] = {2u, 3u, 4u, 6u};
     for (uint32_t v : lds_cand) {
         pg_render_params p = best;
         if (p.stack_lds == v) continue;
@@ -1081,7 +1081,7 @@ pg_status pg_render_tune(pg_ctx* c, const pg_render_params* params, pg_render_pa
     }
     for (uint32_t v : shade_cand) {
         pg_render_params p = best;
-        if ((p.shade_blocks_per_cu ? p.shade_blocks_per_cu : 8u) == v) continue;
+        if ((p.shade_blocks_per_cu ? p.shade_blocks_per_cu : kShadeBlocksPerCu) == v) continue;
         p.shade_blocks_per_cu = v;
         double ms = 0.0;
         if ((st = measure(p, ms)) != PG_OK) return st;

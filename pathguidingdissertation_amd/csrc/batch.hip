@@ -276,7 +276,7 @@ hipError_t enqueue_pass(pg_batch* b, const pg_render_params& prm, uint32_t ns) {
     const int stack = lds_entries(b->stack(), b->cfg.stack_lds, b->ps);
     const uint32_t tgrid =
         b->cfg.trace_blocks_per_cu ? (uint32_t)cu * b->cfg.trace_blocks_per_cu : trace_grid(cu, stack);
-    const uint32_t sgrid = (uint32_t)cu * (b->cfg.shade_blocks_per_cu ? b->cfg.shade_blocks_per_cu : 8u);
+    const uint32_t sgrid = (uint32_t)cu * (b->cfg.shade_blocks_per_cu ? b->cfg.shade_blocks_per_cu : kShadeBlocksPerCu);
     const DevScene& s0 = b->scenes[0]->dev;  // unused by the batch kernels' per-path scenes
     const uint32_t nv = (uint32_t)b->views.size();
     const bool learned = prm.integrator == PG_INTEGRATOR_GUIDED_MLP;
@@ -635,8 +635,8 @@ pg_status pg_batch_autotune(pg_batch* b, const pg_render_params* params, pg_batc
         {1u, 0u},                                                            // use_graph
         {0u, 1u},                                                            // dyn_fetch
         {0u, std::max(largest, 1u << 21), std::max(largest, 1u << 20)},     // queue_paths
-        {0u, 2u, 4u, 8u},                                                    // trace blocks per CU
-        {4u, 8u, 16u},                                                       // shade blocks per CU
+        {0u, 4u, 8u, 12u},                                                   // trace blocks per CU
+        {3u, 2u, 4u},                                                        // shade blocks per CU
         {0u, 32u, 64u},                                                      // stack entries in LDS
         {1u, 2u, 4u, 8u},                                                    // samples per pass
     };
@@ -676,7 +676,7 @@ pg_status pg_batch_autotune(pg_batch* b, const pg_render_params* params, pg_batc
         }
         return std::max(big, n);
     };
-    pg_batch_config best{0, 0, 8, (uint32_t)base_stack, 1, 0, 1, 0};
+    pg_batch_config best{0, 0, kShadeBlocksPerCu, (uint32_t)base_stack, 1, 0, 1, 0};
     double best_ms = 0.0;
     pg_status st = measure(best, best_ms);
     if (st != PG_OK) return st;

@@ -92,6 +92,43 @@ __device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool want) {
     return base + (uint32_t)__popcll(below);
 }
 
+// Block-aggregated appends to two counters (both queues a shading lane may extend):
+// one atomic per counter per block of BLOCK lanes, issued together by thread 0, so
+// the device sees a quarter of the atomics of wave_append at 256 lanes. Slots follow
+// lane order within a wave and wave order within the block. Every thread of the block
+// must call it (it holds barriers); lds: 2 * (BLOCK / 64) + 2 words.
+template <int BLOCK>
+__device__ __forceinline__ void block_append2(uint32_t* ca, bool wa, uint32_t& sa, uint32_t* cb, bool wb,
+                                              uint32_t& sb, uint32_t* lds) {
+    constexpr int kWaves = BLOCK / 64;
+    const unsigned long long ma = __ballot(wa), mb = __ballot(wb);
+    const int lane = __lane_id(), w = (int)(threadIdx.x >> 6);
+    if (lane == 0) {
+        lds[w] = (uint32_t)__popcll(ma);
+        lds[kWaves + w] = (uint32_t)__popcll(mb);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t ta = 0, tb = 0;
+        for (int k = 0; k < kWaves; ++k) {
+            const uint32_t x = lds[k], y = lds[kWaves + k];
+            lds[k] = ta;
+            lds[kWaves + k] = tb;
+            ta += x;
+            tb += y;
+        }
+        const uint32_t ba = ta ? atomicAdd(ca, ta) : 0u;
+        const uint32_t bb = tb ? atomicAdd(cb, tb) : 0u;
+        lds[2 * kWaves] = ba;
+        lds[2 * kWaves + 1] = bb;
+    }
+    __syncthreads();
+    const unsigned long long below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    sa = lds[2 * kWaves] + lds[w] + (uint32_t)__popcll(ma & below);
+    sb = lds[2 * kWaves + 1] + lds[kWaves + w] + (uint32_t)__popcll(mb & below);
+    __syncthreads();  // lds is rewritten by the next call
+}
+
 __device__ __forceinline__ void wave_add_u64(unsigned long long* counter, unsigned long long v) {
     // reduce across the active lanes, one atomic per wavefront
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);

@@ -159,6 +159,10 @@ struct GroupBuffers {
 hipError_t launch_mlp_infer_grouped(const PathState& ps, int parity, const MlpView* guides, uint32_t n_scenes,
                                     const GroupBuffers& gb, float4* mix, uint32_t grid, hipStream_t stream);
 int stack_entries(int max_depth);
+// Shading kernels (k_shade, k_shade_record): 256-lane blocks, whose queue appends are
+// aggregated over the block; launched at 3 blocks per CU unless set.
+constexpr int kShadeBlock = 256;
+constexpr uint32_t kShadeBlocksPerCu = 3;
 // Render kernels keep the first lds_entries() traversal stack entries of a lane in LDS
 // (by default kLdsStackEntries) and the rest in PathState::spill (at most kSpillEntries
 // per lane); their launchers take that LDS count as `lds_stack`.

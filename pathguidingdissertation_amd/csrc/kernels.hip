@@ -532,13 +532,14 @@ __device__ __forceinline__ bool next_event(const DevScene& s, const MaterialRec&
 // learned vMF mixture (probability c, learned per point). The combined density is
 // used in every MIS weight so that the estimator stays unbiased.
 template <int MODE, bool MULTI>
-__global__ __launch_bounds__(kBlock, 3) void k_shade(DevScene s0, PathState ps, PassParams p, GuideCtx gc, int parity,
-                                                  int depth) {
+__global__ __launch_bounds__(kShadeBlock, 3) void k_shade(DevScene s0, PathState ps, PassParams p, GuideCtx gc,
+                                                       int parity, int depth) {
+    __shared__ uint32_t app[2 * (kShadeBlock / 64) + 2];
     uint32_t n = ps.counters[parity];
     if (blockIdx.x == 0 && threadIdx.x == 0) ps.counters[4] = 0;  // shadow-ray claims
     const uint32_t* qin = ps.queue[parity];
     uint32_t* qout = ps.queue[parity ^ 1];
-    for (uint32_t base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock) {
+    for (uint32_t base = blockIdx.x * kShadeBlock; base < n; base += gridDim.x * kShadeBlock) {
         uint32_t i = base + threadIdx.x;
         bool active = i < n;
         bool want_shadow = false, want_ray = false;
@@ -691,13 +692,13 @@ __global__ __launch_bounds__(kBlock, 3) void k_shade(DevScene s0, PathState ps, 
                 ps.info[path] = make_uint4(key, dim, inf.z, inf.w);
             }
         }
-        uint32_t sslot = wave_append(ps.counters + 2, want_shadow);
+        uint32_t sslot, rslot;
+        block_append2<kShadeBlock>(ps.counters + 2, want_shadow, sslot, ps.counters + (parity ^ 1), want_ray, rslot, app);
         if (want_shadow) {
             ps.sh_o[sslot] = make_float4(sh_o.x, sh_o.y, sh_o.z, sh_t);
             ps.sh_d[sslot] = make_float4(sh_d.x, sh_d.y, sh_d.z, __uint_as_float(path));
             ps.sh_c[sslot] = make_float4(sh_c.r, sh_c.g, sh_c.b, 0.f);
         }
-        uint32_t rslot = wave_append(ps.counters + (parity ^ 1), want_ray);
         if (want_ray) {
             qout[rslot] = path;
             ps.ray_o[parity ^ 1][rslot] = make_float4(nr_o.x, nr_o.y, nr_o.z, 0.f);
@@ -853,12 +854,13 @@ __global__ __launch_bounds__(kBlock) void k_leaf_nbrs(GuideView g, uint32_t n_le
 // vertex weight, the NEE weight and the next vertex's MIS weights; each vertex keeps
 // q and the BSDF pdf of its direction as training data.
 template <bool NEE, int MODE>
-__global__ __launch_bounds__(kBlock, 3) void k_shade_record(DevScene s, PathState ps, PassParams p, RecordState rs,
-                                                         const float4* __restrict__ mix, int parity, int depth) {
+__global__ __launch_bounds__(kShadeBlock, 3) void k_shade_record(DevScene s, PathState ps, PassParams p, RecordState rs,
+                                                              const float4* __restrict__ mix, int parity, int depth) {
+    __shared__ uint32_t app[2 * (kShadeBlock / 64) + 2];
     uint32_t n = ps.counters[parity];
     const uint32_t* qin = ps.queue[parity];
     uint32_t* qout = ps.queue[parity ^ 1];
-    for (uint32_t base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock) {
+    for (uint32_t base = blockIdx.x * kShadeBlock; base < n; base += gridDim.x * kShadeBlock) {
         uint32_t i = base + threadIdx.x;
         bool active = i < n;
         bool want_ray = false, want_shadow = false;
@@ -1020,15 +1022,14 @@ __global__ __launch_bounds__(kBlock, 3) void k_shade_record(DevScene s, PathStat
                 rs.nv[path] = (uint32_t)depth + 1u;
             }
         }
-        if (NEE) {
-            uint32_t sslot = wave_append(ps.counters + 2, want_shadow);
-            if (want_shadow) {
-                ps.sh_o[sslot] = make_float4(sh_o.x, sh_o.y, sh_o.z, sh_t);
-                ps.sh_d[sslot] = make_float4(sh_d.x, sh_d.y, sh_d.z, __uint_as_float(path));
-                ps.sh_c[sslot] = make_float4(sh_c.r, sh_c.g, sh_c.b, __int_as_float(depth));
-            }
+        uint32_t sslot, rslot;
+        block_append2<kShadeBlock>(ps.counters + 2, NEE && want_shadow, sslot, ps.counters + (parity ^ 1), want_ray,
+                                   rslot, app);
+        if (NEE && want_shadow) {
+            ps.sh_o[sslot] = make_float4(sh_o.x, sh_o.y, sh_o.z, sh_t);
+            ps.sh_d[sslot] = make_float4(sh_d.x, sh_d.y, sh_d.z, __uint_as_float(path));
+            ps.sh_c[sslot] = make_float4(sh_c.r, sh_c.g, sh_c.b, __int_as_float(depth));
         }
-        uint32_t rslot = wave_append(ps.counters + (parity ^ 1), want_ray);
         if (want_ray) {
             qout[rslot] = path;
             ps.ray_o[parity ^ 1][rslot] = make_float4(nr_o.x, nr_o.y, nr_o.z, 0.f);
@@ -1231,17 +1232,17 @@ hipError_t launch_shade(const DevScene& s, const PathState& ps, const PassParams
     if (multi) {
         if (mode == 1 || mode == 3) return hipErrorInvalidValue;  // batches: unguided or learned guide
         if (mode == 2)
-            hipLaunchKernelGGL((k_shade<2, true>), dim3(grid), dim3(kBlock), 0, stream, s, ps, p, gc, parity, depth);
+            hipLaunchKernelGGL((k_shade<2, true>), dim3(grid), dim3(kShadeBlock), 0, stream, s, ps, p, gc, parity, depth);
         else
-            hipLaunchKernelGGL((k_shade<0, true>), dim3(grid), dim3(kBlock), 0, stream, s, ps, p, gc, parity, depth);
+            hipLaunchKernelGGL((k_shade<0, true>), dim3(grid), dim3(kShadeBlock), 0, stream, s, ps, p, gc, parity, depth);
     } else if (mode == 1) {
-        hipLaunchKernelGGL((k_shade<1, false>), dim3(grid), dim3(kBlock), 0, stream, s, ps, p, gc, parity, depth);
+        hipLaunchKernelGGL((k_shade<1, false>), dim3(grid), dim3(kShadeBlock), 0, stream, s, ps, p, gc, parity, depth);
     } else if (mode == 3) {
-        hipLaunchKernelGGL((k_shade<3, false>), dim3(grid), dim3(kBlock), 0, stream, s, ps, p, gc, parity, depth);
+        hipLaunchKernelGGL((k_shade<3, false>), dim3(grid), dim3(kShadeBlock), 0, stream, s, ps, p, gc, parity, depth);
     } else if (mode == 2) {
-        hipLaunchKernelGGL((k_shade<2, false>), dim3(grid), dim3(kBlock), 0, stream, s, ps, p, gc, parity, depth);
+        hipLaunchKernelGGL((k_shade<2, false>), dim3(grid), dim3(kShadeBlock), 0, stream, s, ps, p, gc, parity, depth);
     } else {
-        hipLaunchKernelGGL((k_shade<0, false>), dim3(grid), dim3(kBlock), 0, stream, s, ps, p, gc, parity, depth);
+        hipLaunchKernelGGL((k_shade<0, false>), dim3(grid), dim3(kShadeBlock), 0, stream, s, ps, p, gc, parity, depth);
     }
     return hipGetLastError();
 }
@@ -1404,13 +1405,13 @@ hipError_t launch_query_keys(const DevScene& s, const PathState& ps, float3 lo, 
 hipError_t launch_shade_record(const DevScene& s, const PathState& ps, const PassParams& p, const RecordState& rs,
                                const float4* mix, int parity, int depth, uint32_t grid, hipStream_t stream) {
     if (mix && rs.li_nee)
-        hipLaunchKernelGGL((k_shade_record<true, 2>), dim3(grid), dim3(kBlock), 0, stream, s, ps, p, rs, mix, parity, depth);
+        hipLaunchKernelGGL((k_shade_record<true, 2>), dim3(grid), dim3(kShadeBlock), 0, stream, s, ps, p, rs, mix, parity, depth);
     else if (mix)
-        hipLaunchKernelGGL((k_shade_record<false, 2>), dim3(grid), dim3(kBlock), 0, stream, s, ps, p, rs, mix, parity, depth);
+        hipLaunchKernelGGL((k_shade_record<false, 2>), dim3(grid), dim3(kShadeBlock), 0, stream, s, ps, p, rs, mix, parity, depth);
     else if (rs.li_nee)
-        hipLaunchKernelGGL((k_shade_record<true, 0>), dim3(grid), dim3(kBlock), 0, stream, s, ps, p, rs, nullptr, parity, depth);
+        hipLaunchKernelGGL((k_shade_record<true, 0>), dim3(grid), dim3(kShadeBlock), 0, stream, s, ps, p, rs, nullptr, parity, depth);
     else
-        hipLaunchKernelGGL((k_shade_record<false, 0>), dim3(grid), dim3(kBlock), 0, stream, s, ps, p, rs, nullptr, parity, depth);
+        hipLaunchKernelGGL((k_shade_record<false, 0>), dim3(grid), dim3(kShadeBlock), 0, stream, s, ps, p, rs, nullptr, parity, depth);
     return hipGetLastError();
 }
 hipError_t launch_shadow_record(const DevScene& s, const PathState& ps, const RecordState& rs, int lds_stack, uint32_t grid,
