@@ -180,12 +180,31 @@ __global__ __launch_bounds__(kBlock) void k_shadow(DevScene s0, PathState ps, co
     }
 }
 
-// Persistent-lane variants of extend and shadow: a lane that finishes its ray claims
-// the next one from the queue (wave-aggregated atomic on counters[3] / counters[4],
-// reset by the kernel before) whenever fewer than kRefillLanes lanes of the wave are
+// Persistent-lane variants of extend and shadow: each wave owns an equal contiguous
+// share of the queue, and a lane that finishes its ray claims the next one of its
+// wave's share (a wave-uniform cursor, no atomics: device-scope atomics on one counter
+// are serialised at memory) whenever fewer than kRefillLanes lanes of the wave are
 // busy, so a wave is not held by its slowest ray. Per-ray arithmetic is that of
 // traverse_closest / traverse_visible; results are identical.
 constexpr int kRefillLanes = 48;
+struct WaveShare {
+    uint32_t cur, end;
+    __device__ __forceinline__ explicit WaveShare(uint32_t n) {
+        const uint32_t waves = gridDim.x * (kBlock / 64);
+        const uint32_t w = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+        cur = (uint32_t)(((uint64_t)n * w) / waves);
+        end = (uint32_t)(((uint64_t)n * (w + 1)) / waves);
+    }
+    // slot for each wanting lane (in lane order); slots at or past `end` are empty
+    __device__ __forceinline__ uint32_t claim(bool want) {
+        const unsigned long long m = __ballot(want);
+        const int lane = __lane_id();
+        const unsigned long long below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+        const uint32_t slot = cur + (uint32_t)__popcll(m & below);
+        cur += (uint32_t)__popcll(m);
+        return slot;
+    }
+};
 
 template <bool COUNT, int STACK, bool MULTI>
 __global__ __launch_bounds__(kBlock) void k_extend_dyn(DevScene s0, PathState ps, int parity) {
@@ -204,11 +223,12 @@ __global__ __launch_bounds__(kBlock) void k_extend_dyn(DevScene s0, PathState ps
     uint32_t slot = 0;
     const DevScene* sc = &s0;
     bool active = false, drained = false;
+    WaveShare share(n);
     while (true) {
         if (!drained) {
             const bool want = !active;
-            const uint32_t claim = wave_append(ps.counters + 3, want);
-            if (want && claim < n) {
+            const uint32_t claim = share.claim(want);
+            if (want && claim < share.end) {
                 slot = claim;
                 float4 a = ro[slot], b = rd[slot];
                 t.o = F3{a.x, a.y, a.z};
@@ -221,7 +241,7 @@ __global__ __launch_bounds__(kBlock) void k_extend_dyn(DevScene s0, PathState ps
                 active = trav_begin<COUNT>(*sc, t, cn);
                 if (!active) ps.hit[slot] = make_float4(__uint_as_float(kMiss), kFltMax, 0.f, 0.f);
             }
-            drained = __ballot(want && claim >= n) != 0ull;
+            drained = share.cur >= share.end;
         }
         if (__ballot(active) == 0ull) {
             if (drained) break;
@@ -255,11 +275,12 @@ __global__ __launch_bounds__(kBlock) void k_shadow_dyn(DevScene s0, PathState ps
     uint32_t slot = 0, path = 0;
     const DevScene* sc = &s0;
     bool active = false, drained = false;
+    WaveShare share(n);
     while (true) {
         if (!drained) {
             const bool want = !active;
-            const uint32_t claim = wave_append(ps.counters + 4, want);
-            if (want && claim < n) {
+            const uint32_t claim = share.claim(want);
+            if (want && claim < share.end) {
                 slot = claim;
                 float4 a = ps.sh_o[slot], b = ps.sh_d[slot];
                 t.o = F3{a.x, a.y, a.z};
@@ -274,7 +295,7 @@ __global__ __launch_bounds__(kBlock) void k_shadow_dyn(DevScene s0, PathState ps
                     ps.radiance[path] = make_float4(L.x + c.x, L.y + c.y, L.z + c.z, L.w);
                 }
             }
-            drained = __ballot(want && claim >= n) != 0ull;
+            drained = share.cur >= share.end;
         }
         if (__ballot(active) == 0ull) {
             if (drained) break;
