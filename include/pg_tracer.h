@@ -97,8 +97,10 @@ typedef struct {
     uint32_t shade_blocks_per_cu;  /* shade grid, blocks of 256 lanes (0: 3)     */
     uint32_t sort_rays;     /* 1: trace secondary and shadow rays in (direction
                                octant, origin Morton) order; results unchanged  */
-    uint32_t dyn_fetch;     /* 1: persistent lanes claim new rays when a wave runs
-                               below 48 busy lanes; results unchanged           */
+    uint32_t dyn_fetch;     /* 1 (default): persistent lanes claim new rays from
+                               their wave's share of the queue when the wave runs
+                               below 48 busy lanes; 0: one ray per lane and grid
+                               stride. Results unchanged                         */
     uint32_t env_visibility; /* environment-light shadow rays: 0 aim at
                                sceneCentre + wi * sceneRadius as Renderer.h:331-332
                                does (default, the reference's images); 1 trace

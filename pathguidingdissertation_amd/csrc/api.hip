@@ -711,6 +711,7 @@ pg_render_params pg_render_params_default(void) {
     p.record = 0;
     p.guide_prob = 0.5f;
     p.knn_k = 16;
+    p.dyn_fetch = 1;  // launch shape only: images are those of every other shape
     return p;
 }
 
@@ -919,7 +920,7 @@ pg_status pg_render_spp(pg_ctx* c, const pg_render_params* params, uint32_t n_sp
     }
     const int stack = lds_entries((int)c->host.bvh.max_depth, prm.stack_lds, c->ps);
     const uint32_t grid_trace = prm.trace_blocks_per_cu ? (uint32_t)c->cu_count * prm.trace_blocks_per_cu
-                                                        : trace_grid(c->cu_count, stack);
+                                                        : trace_grid(c->cu_count, stack, dyn);
     const uint32_t grid_shade = (uint32_t)c->cu_count * (prm.shade_blocks_per_cu ? prm.shade_blocks_per_cu : kShadeBlocksPerCu);
     const uint32_t grid_guide = (uint32_t)c->cu_count * (c->gv.levels <= 16 ? 10u : 5u);
     const uint32_t grid_mlp = (uint32_t)c->cu_count * 3;  // 48 KB of LDS per block: 3 blocks per CU
@@ -1061,8 +1062,15 @@ pg_status pg_render_tune(pg_ctx* c, const pg_render_params* params, pg_render_pa
     pg_status st = measure(best, best_ms);
     if (st != PG_OK) return st;
     const uint32_t lds_cand[] = {0u, 32u, 64u};
-    const uint32_t trace_cand[] = {0u, 4u, 6u, 8u, 12u};
+    const uint32_t trace_cand[] = {0u, 8u, 10u, 12u, 14u, 16u};
     const uint32_t shade_cand[] = {2u, 3u, 4u, 6u};
+    {
+        pg_render_params p = best;
+        p.dyn_fetch = best.dyn_fetch ? 0u : 1u;
+        double ms = 0.0;
+        if ((st = measure(p, ms)) != PG_OK) return st;
+        if (ms < best_ms) { best_ms = ms; best = p; }
+    }
     for (uint32_t v : lds_cand) {
         pg_render_params p = best;
         if (p.stack_lds == v) continue;
@@ -1083,13 +1091,6 @@ pg_status pg_render_tune(pg_ctx* c, const pg_render_params* params, pg_render_pa
         pg_render_params p = best;
         if ((p.shade_blocks_per_cu ? p.shade_blocks_per_cu : kShadeBlocksPerCu) == v) continue;
         p.shade_blocks_per_cu = v;
-        double ms = 0.0;
-        if ((st = measure(p, ms)) != PG_OK) return st;
-        if (ms < best_ms) { best_ms = ms; best = p; }
-    }
-    {
-        pg_render_params p = best;
-        p.dyn_fetch = best.dyn_fetch ? 0u : 1u;
         double ms = 0.0;
         if ((st = measure(p, ms)) != PG_OK) return st;
         if (ms < best_ms) { best_ms = ms; best = p; }

@@ -275,7 +275,7 @@ hipError_t enqueue_pass(pg_batch* b, const pg_render_params& prm, uint32_t ns) {
     const int cu = pg_internal::cu_count(b->ctx);
     const int stack = lds_entries(b->stack(), b->cfg.stack_lds, b->ps);
     const uint32_t tgrid =
-        b->cfg.trace_blocks_per_cu ? (uint32_t)cu * b->cfg.trace_blocks_per_cu : trace_grid(cu, stack);
+        b->cfg.trace_blocks_per_cu ? (uint32_t)cu * b->cfg.trace_blocks_per_cu : trace_grid(cu, stack, b->cfg.dyn_fetch != 0);
     const uint32_t sgrid = (uint32_t)cu * (b->cfg.shade_blocks_per_cu ? b->cfg.shade_blocks_per_cu : kShadeBlocksPerCu);
     const DevScene& s0 = b->scenes[0]->dev;  // unused by the batch kernels' per-path scenes
     const uint32_t nv = (uint32_t)b->views.size();

@@ -169,6 +169,6 @@ constexpr uint32_t kShadeBlocksPerCu = 3;
 constexpr int kLdsStackEntries = 16;
 constexpr int kSpillEntries = 64 - kLdsStackEntries;
 int lds_entries(int max_depth, uint32_t lds_cap, const PathState& ps);
-uint32_t trace_grid(int cu_count, int lds_stack);
+uint32_t trace_grid(int cu_count, int lds_stack, bool dyn);
 
 }  // namespace pg

@@ -1461,10 +1461,11 @@ int lds_entries(int max_depth, uint32_t lds_cap, const PathState& ps) {
 }
 
 // Persistent grid for the traversal kernels: as many 128-lane blocks per CU as
-// the LDS stacks (160 KB per CU) allow, at most 16.
-uint32_t trace_grid(int cu_count, int lds_stack) {
+// the LDS stacks (160 KB per CU) allow, at most 16 (12 for the persistent-lane
+// kernels, which keep their lanes busy with fewer waves and thrash L1 less).
+uint32_t trace_grid(int cu_count, int lds_stack, bool dyn) {
     int per_cu = (160 * 1024) / (stack_entries(lds_stack) * kBlock * 4);
-    if (per_cu > 16) per_cu = 16;
+    if (per_cu > (dyn ? 12 : 16)) per_cu = dyn ? 12 : 16;
     if (per_cu < 1) per_cu = 1;
     return (uint32_t)(cu_count * per_cu);
 }
