@@ -29,6 +29,13 @@ sys.path.insert(0, str(ROOT))
 PEAK_HBM_GBS = 8000.0  # MI355X HBM3E
 
 
+# Shade step (one closest hit): hit 20 B, shading attributes 64 B, material 32 B, path
+# state read + write 48 B, new ray 32 B; a shadow ray adds the shadow ray 48 B, the
+# alias-table entry 16 B and the bilinear environment texels 48 B.
+SHADE_BYTES_PER_HIT = 20 + 64 + 32 + 48 + 32
+SHADE_BYTES_PER_SHADOW = 48 + 16 + 48
+
+
 def algorithmic_bytes(nodes_per_ray: float, tris_per_ray: float, out_bytes: int) -> float:
     """Per ray: 32-B nodes visited, 36 B of vertices + 4 B triangle index per triangle
     tested, 32 B ray in, the hit (20 B) or visibility (4 B) out."""
@@ -536,7 +543,12 @@ def main() -> int:
     img = rt.image()
     finite = bool(np.isfinite(img).all())
     ext_name = "k_extend_dyn" if rt.params.dyn_fetch else "k_extend"
-    traffic = profiled_traffic(a.config).get(ext_name)
+    prof = profiled_traffic(a.config)
+    traffic = prof.get(ext_name)
+    avg_launch_s = ext_ms * 1e-3 / launches
+    shade_ms = st["kernel_ms_shade"]
+    shade_bytes = SHADE_BYTES_PER_HIT * st["closest_rays"] + SHADE_BYTES_PER_SHADOW * st["shadow_rays"]
+    shade_traffic = prof.get("k_shade")
 
     copy = None
     if a.copy_check:
@@ -615,7 +627,17 @@ def main() -> int:
                      "traffic_source": "profiles/pmc_traffic_config%d.txt (bytes per launch)" % a.config,
                      "bytes_per_ray": round(bytes_c, 1), "nodes_per_ray": round(nodes_c, 2),
                      "tris_per_ray": round(tris_c, 2), "launches": launches,
-                     "avg_launch_ms": round(ext_ms / launches, 4)},
+                     "avg_launch_ms": round(ext_ms / launches, 4),
+                     "traffic_frac": None if traffic is None else round(traffic / avg_launch_s / 1e9 / PEAK_HBM_GBS, 4),
+                     "note": "algorithmic node and triangle bytes come from L2 (hit rate 0.97) and the Infinity "
+                             "Cache, so frac can exceed 1; traffic_frac is the measured HBM share; the measured "
+                             "limiter is the L1 data path (profiles/k_extend_config2_1280x720.md)"},
+        "shade_roofline": {"bound": "hbm", "kernel": "k_shade",
+                           "achieved": round(shade_bytes / (shade_ms * 1e-3) / 1e9, 1) if shade_ms > 0 else None,
+                           "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                           "measured_hbm_gbs": None if shade_traffic is None or shade_ms <= 0 else
+                           round(shade_traffic * launches / (shade_ms * 1e-3) / 1e9, 1),
+                           "traffic": None if shade_traffic is None else round(shade_traffic)},
         "shadow_roofline": {"achieved": round(sh_achieved, 1), "bytes_per_ray": round(bytes_s, 1),
                             "nodes_per_ray": round(nodes_s, 2), "tris_per_ray": round(tris_s, 2)},
         "cpu_baseline": cpu,

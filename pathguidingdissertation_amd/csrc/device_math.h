@@ -79,22 +79,10 @@ __device__ __forceinline__ float power_heuristic(float pa, float pb) {  // MyMat
     return a / (a + b);
 }
 
-// Wave-aggregated append: one atomic per wavefront, slots in lane order.
-__device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool want) {
-    unsigned long long m = __ballot(want);
-    unsigned long long act = __ballot(1);
-    int leader = __ffsll((long long)act) - 1;
-    int lane = __lane_id();
-    uint32_t base = 0;
-    if (lane == leader && m != 0ull) base = atomicAdd(counter, (uint32_t)__popcll(m));
-    base = __shfl(base, leader);
-    unsigned long long below = (lane == 0) ? 0ull : (m & ((~0ull) >> (64 - lane)));
-    return base + (uint32_t)__popcll(below);
-}
-
 // Block-aggregated appends to two counters (both queues a shading lane may extend):
 // one atomic per counter per block of BLOCK lanes, issued together by thread 0, so
-// the device sees a quarter of the atomics of wave_append at 256 lanes. Slots follow
+// the device sees one atomic per counter per 256 lanes (device-scope atomics on one
+// address are serialised at memory, so their count bounds a shading kernel). Slots follow
 // lane order within a wave and wave order within the block. Every thread of the block
 // must call it (it holds barriers); lds: 2 * (BLOCK / 64) + 2 words.
 template <int BLOCK>
