@@ -10,7 +10,8 @@ taken inside the timed region, and the CPU oracle timed on the host cores on a b
 sample (all threads of the host's share and one thread). Further parts: the batched
 views of the config-5 family at its specification (48 views, 32 SPP; --batch-views 0
 skips it) and time-to-relMSE of the unguided, quadtree-guided and learned-guided
-renderers against a 4096-SPP reference (--no-ttr skips it).
+renderers against a 4096-SPP reference on the same interior (--ttr-config 3: the
+window-lit room; --no-ttr skips it).
 """
 from __future__ import annotations
 
@@ -251,6 +252,22 @@ def time_to_relmse(pg, dev, sc, data, max_depth, ref_spp, cap_s, log, target=0.0
                "trimmed_relmse_ratio_at_cap": {
                    "unguided_over_learned": ratio("unguided", "learned", "final_trimmed_relmse"),
                    "quadtree_over_learned": ratio("quadtree", "learned", "final_trimmed_relmse")}}
+        # Monte Carlo relMSE falls as 1/time: the last point of each run (reference noise
+        # removed) projects its time to the target; the ratios of those times against
+        # the north-star factors (4x unguided, 2x quadtree) give the shortfall
+        def projected(name):
+            c = res[name]["curve"][-1]
+            return round(c[4] * c[1] / target, 1) if c[4] > 0 else None
+        proj = {m: projected(m) for m in res}
+        cmp["projected_seconds_to_target"] = proj
+
+        def proj_speedup(fast, slow, factor):
+            if not proj[fast] or not proj[slow]:
+                return {"speedup": None, "target_factor": factor, "fraction_of_target": None}
+            sp = proj[slow] / proj[fast]
+            return {"speedup": round(sp, 3), "target_factor": factor, "fraction_of_target": round(sp / factor, 3)}
+        cmp["projected_speedup"] = {"learned_vs_unguided": proj_speedup("learned", "unguided", 4.0),
+                                    "learned_vs_quadtree": proj_speedup("learned", "quadtree", 2.0)}
         if with_pss:
             for v in ("pss", "pss_approx"):
                 cmp[f"{v}_vs_unguided"] = speedup(v, "unguided")
@@ -443,7 +460,7 @@ def main() -> int:
     ap.add_argument("--ttr-cap", type=float, default=float(os.environ.get("PG_BENCH_TTR_CAP", "30")),
                     help="wall-clock seconds per method in the time-to-relMSE study")
     ap.add_argument("--ttr-ref-spp", type=int, default=int(os.environ.get("PG_BENCH_TTR_REF", "4096")))
-    ap.add_argument("--ttr-config", type=int, default=int(os.environ.get("PG_BENCH_TTR_CONFIG", "3")),
+    ap.add_argument("--ttr-config", type=int, default=int(os.environ.get("PG_BENCH_TTR_CONFIG", "2")),
                     help="scene of the time-to-relMSE study (3: the closed room lit through one window, "
                          "where the guiding configurations are defined; 2: the open interior)")
     ap.add_argument("--ttr-repeats", type=int, default=1,
