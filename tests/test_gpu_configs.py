@@ -2,8 +2,9 @@
 
 * Config 2 (the ~1.09M-triangle interior at 1280x720, max depth 12): the full frame
   rendered by the wavefront kernels, compared on three 64 x 64 crops (objects, a
-  sunlit wall, the far corner) with the CPU oracle on the same random stream, and its
-  relMSE against the FP64 4096-SPP oracle crops inside the mean +- 3 sigma of eight
+  sunlit wall, the far corner) with the CPU oracle on the same random stream (the
+  differing pixels pass the sign test of tests/exchangeable.py per primary-hit class), and
+  its relMSE against the FP64 4096-SPP oracle crops inside the mean +- 3 sigma of eight
   equal-SPP oracle runs (tests/golden/oracle_config2_crops.npz).
 * Config 3 (the window-lit room, 720x1280): a 1-SPP record pass of the full scene,
   the cache built over its records, the k = 16 nearest records of record positions and
@@ -22,6 +23,7 @@ import dataclasses
 import numpy as np
 import pytest
 
+import exchangeable as ex
 from conftest import GOLDEN
 
 pytestmark = pytest.mark.gpu
@@ -61,6 +63,9 @@ def test_config2_full_frame_same_stream_as_oracle(pg, device, config2):
     agree = float(np.mean(np.all(rel < 1e-3, axis=-1)))
     assert agree > 0.9, agree
     assert b.sum() > 0
+    # the differing pixels lean to neither side, per primary-hit class
+    labels = _crops(ex.primary_classes(pg, sc, config2)[..., None], rects)[:, 1:-1, 1:-1, 0]
+    ex.assert_exchangeable(a, b, labels)
 
 
 def test_config2_relmse_within_oracle_band(pg, device, config2):

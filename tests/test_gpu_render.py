@@ -3,7 +3,9 @@
 
 * Same random stream on both sides (counter RNG keyed by seed, sample, pixel, dim):
   the films agree pixel by pixel except where a few-ulp difference in a
-  transcendental (device libm vs glibc) flips a discrete decision on a path.
+  transcendental (device libm vs glibc) flips a discrete decision on a path; the
+  pixels that differ pass a sign test per primary-hit class (tests/exchangeable.py:
+  a bias of either side in one BSDF makes them lean to one sign).
 * Statistical bar: the GPU image's relMSE against the FP64 4096-SPP oracle image
   lies within the mean +- 3 sigma of K=8 equal-SPP oracle runs.
 * Closed form: white furnace (convex diffuse object under a constant environment).
@@ -15,6 +17,7 @@ import json
 import numpy as np
 import pytest
 
+import exchangeable as ex
 from conftest import GOLDEN
 
 pytestmark = pytest.mark.gpu
@@ -45,6 +48,7 @@ def test_cornell_same_stream_matches_oracle(pg, device, cornell):
     assert agree > 0.97, agree
     # and the images are statistically indistinguishable
     assert pg.relmse(film / 8, ref / 8) < 1e-3
+    ex.assert_exchangeable(film, ref, ex.primary_classes(pg, pg.Scene(device, cornell), cornell))
 
 
 def test_interior_same_stream_matches_oracle(pg, device, small_interior):
@@ -55,6 +59,7 @@ def test_interior_same_stream_matches_oracle(pg, device, small_interior):
     ref, _ = osc.render(2, seed=3, max_depth=12, accum_f64=False, env_mode=1)
     agree = _pixel_agreement(film, ref.astype(np.float32))
     assert agree > 0.90, agree
+    ex.assert_exchangeable(film, ref, ex.primary_classes(pg, pg.Scene(device, small_interior), small_interior))
 
 
 @pytest.mark.parametrize("env_visibility", [0, 1])
@@ -74,6 +79,7 @@ def test_window_room_same_stream_matches_oracle(pg, device, env_visibility):
                                       env_visibility=env_visibility)
     agree = _pixel_agreement(film, ref.astype(np.float32))
     assert agree > 0.90, agree
+    ex.assert_exchangeable(film, ref, ex.primary_classes(pg, sc, data))
 
 
 def test_cornell_relmse_within_reference_band(pg, device, cornell):

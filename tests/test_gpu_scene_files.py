@@ -11,6 +11,7 @@ Layered BSDFs, the device tonemap and the image writers, against the CPU oracle.
 import numpy as np
 import pytest
 
+import exchangeable as ex
 from conftest import GOLDEN, make_scene_dir
 
 pytestmark = pytest.mark.gpu
@@ -40,6 +41,7 @@ def test_textured_scene_same_stream_matches_oracle(pg, device, tmp_path):
     agree = _agreement(film, ref.astype(np.float32))
     assert agree > 0.93, agree
     assert pg.relmse(film / 4, ref / 4) < 0.02
+    ex.assert_exchangeable(film, ref, ex.primary_classes(pg, pg.Scene(device, data), data))
     # the textures matter: the same scene with constant albedos renders differently
     flat = pg.SceneData(data.positions, data.normals, data.material_index, data.materials, data.env, data.camera)
     film2, _ = _render(pg, device, flat, spp=4, seed=3)
