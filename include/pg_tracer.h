@@ -402,8 +402,10 @@ typedef struct pg_batch_config {
     uint32_t samples_per_pass;    /* samples per pixel traced together in one pass (0: 1;
                                      at most 64, and with more than one at most 2^26
                                      paths per chunk); a render's samples beyond whole
-                                     passes run as one shorter pass; views are identical
-                                     for every value                                  */
+                                     passes run as one shorter pass, and a render of
+                                     fewer samples than a pass as one pass of its own
+                                     size (replayed as a graph like the others); views
+                                     are identical for every value                    */
     uint32_t stack_lds;           /* traversal stack entries per lane in LDS (see the
                                      render params; 0: 16)                            */
 } pg_batch_config;
@@ -428,9 +430,12 @@ pg_status pg_batch_scene_textures(pg_batch* batch, uint32_t scene_id, const floa
 pg_status pg_batch_add_view(pg_batch* batch, uint32_t scene_id, const pg_camera_desc* camera,
                             uint32_t* view_id);
 pg_status pg_batch_configure(pg_batch* batch, const pg_batch_config* cfg);
-/* Times candidate configurations (graph, ray claims, queue size, blocks per CU, stack
- * depth, samples per pass) by coordinate descent on a few passes each, keeps the
- * fastest and clears the films. */
+/* Times candidate configurations on one or two passes each and keeps the fastest,
+ * then clears the films. It starts from chunks of about 2^20 paths (each chunk's
+ * scenes then stay resident in L2 and the Infinity Cache while they are traced) with
+ * the most samples per pass the 2^26-path state allows, and descends coordinate by
+ * coordinate: chunk size with samples per pass, fewer samples per pass, graph, ray
+ * claims, traversal and shade blocks per CU, LDS stack entries. */
 pg_status pg_batch_autotune(pg_batch* batch, const pg_render_params* params, pg_batch_config* best,
                             double* ms_per_spp);
 /* Learned guide of one scene of the batch (no reference counterpart; configuration 5

@@ -145,6 +145,7 @@ struct pg_batch {
     hipGraphExec_t exec = nullptr;
     bool graph_ok = false;
     pg_render_params gparams{};
+    uint32_t graph_ns = 0;  // samples per pass of the captured graph
     size_t graph_nodes = 0;
     uint64_t paths = 0;
     double gpu_ms = 0.0;
@@ -375,10 +376,12 @@ pg_status render(pg_batch* b, const pg_render_params& prm, uint32_t n_spp) {
                 "guide table");
         PG_BTRY(b, hipStreamSynchronize(s), "guide table");
     }
-    // passes of ns samples (the captured graph), then one pass of the samples left over
-    const uint32_t ns = b->samples_per_pass();
+    // passes of ns samples (the captured graph), then one pass of the samples left over;
+    // a render of fewer samples than a pass is one whole pass of its own size
+    uint32_t ns = b->samples_per_pass();
+    if (n_spp > 0 && n_spp < ns) ns = n_spp;
     const uint32_t full = n_spp / ns, rest = n_spp % ns;
-    if (b->cfg.use_graph && full > 0 && (!b->graph_ok || !same_params(prm, b->gparams))) {
+    if (b->cfg.use_graph && full > 0 && (!b->graph_ok || !same_params(prm, b->gparams) || b->graph_ns != ns)) {
         b->drop_graph();
         PG_BTRY(b, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin capture");
         hipError_t e = enqueue_pass(b, prm, ns);
@@ -394,6 +397,7 @@ pg_status render(pg_batch* b, const pg_render_params& prm, uint32_t n_spp) {
         (void)hipGraphGetNodes(g, nullptr, &b->graph_nodes);
         b->graph_ok = true;
         b->gparams = prm;
+        b->graph_ns = ns;
     }
     PG_BTRY(b, hipEventRecord(b->e0, s), "event");
     for (uint32_t k = 0; k < full; ++k) {
@@ -630,41 +634,24 @@ pg_status pg_batch_autotune(pg_batch* b, const pg_render_params* params, pg_batc
     uint32_t largest = 0;
     for (const BatchView& v : b->views) largest = std::max(largest, v.width * v.height);
     const int base_stack = stack_entries(std::max(b->max_depth, 1));
-    // candidate values per parameter (coordinate descent in this order)
-    std::vector<std::vector<uint32_t>> cand = {
-        {1u, 0u},                                                            // use_graph
-        {0u, 1u},                                                            // dyn_fetch
-        {0u, std::max(largest, 1u << 21), std::max(largest, 1u << 20)},     // queue_paths
-        {0u, 4u, 8u, 12u},                                                   // trace blocks per CU
-        {3u, 2u, 4u},                                                        // shade blocks per CU
-        {0u, 32u, 64u},                                                      // stack entries in LDS
-        {1u, 2u, 4u, 8u},                                                    // samples per pass
-    };
-    auto get = [](pg_batch_config& c, int k) -> uint32_t& {
-        switch (k) {
-        case 0: return c.use_graph;
-        case 1: return c.dyn_fetch;
-        case 2: return c.queue_paths;
-        case 3: return c.trace_blocks_per_cu;
-        case 4: return c.shade_blocks_per_cu;
-        case 5: return c.stack_lds;
-        default: return c.samples_per_pass;
-        }
-    };
-    auto measure = [&](const pg_batch_config& c, double& ms) -> pg_status {
+    // GPU milliseconds per SPP of c over one pass of min(samples per pass, cap) samples
+    // (at least 8 samples in whole passes), after a one-sample warm-up pass that
+    // allocates the path state
+    auto measure = [&](const pg_batch_config& c, uint32_t cap, double& ms) -> pg_status {
         pg_status st = pg_batch_configure(b, &c);
         if (st != PG_OK) return st;
         if ((st = clear(b)) != PG_OK) return st;
-        const uint32_t k = c.samples_per_pass ? c.samples_per_pass : 1u;  // whole passes
-        if ((st = render(b, prm, k)) != PG_OK) return st;  // warm-up (and graph capture)
+        if ((st = render(b, prm, 1)) != PG_OK) return st;
+        const uint32_t k = std::min(c.samples_per_pass ? c.samples_per_pass : 1u, cap);
+        const uint32_t timed = k >= 8u ? k : 8u / k * k;
         double before = b->gpu_ms;
-        if ((st = render(b, prm, 2 * k)) != PG_OK) return st;
-        ms = (b->gpu_ms - before) / (2.0 * k);
+        if ((st = render(b, prm, timed)) != PG_OK) return st;
+        ms = (b->gpu_ms - before) / (double)timed;
         return PG_OK;
     };
-    // paths of the largest chunk under c's queue size (the greedy packing of prepare)
-    auto chunk_paths = [&](const pg_batch_config& c) -> uint64_t {
-        const uint64_t q = c.queue_paths ? c.queue_paths : ~0ull;
+    // paths of the largest chunk under queue size q (the greedy packing of prepare)
+    auto chunk_paths = [&](uint32_t q_paths) -> uint64_t {
+        const uint64_t q = q_paths ? q_paths : ~0ull;
         uint64_t n = 0, big = 0;
         for (const BatchView& v : b->views) {
             const uint64_t px = (uint64_t)v.width * v.height;
@@ -676,25 +663,69 @@ pg_status pg_batch_autotune(pg_batch* b, const pg_render_params* params, pg_batc
         }
         return std::max(big, n);
     };
-    pg_batch_config best{0, 0, kShadeBlocksPerCu, (uint32_t)base_stack, 1, 0, 1, 0};
+    // the most samples per pass (a power of two up to 64) whose chunks stay within the
+    // 2^26-path state
+    auto most_spp = [&](uint32_t q) -> uint32_t {
+        uint32_t spp = 64;
+        while (spp > 1 && chunk_paths(q) * spp > (1ull << 26)) spp >>= 1;
+        return spp;
+    };
+    // Small chunks keep one scene's BVH and triangles (at most ~1M triangles, ~75 MB)
+    // resident in L2 and the Infinity Cache while its views are traced, and many samples
+    // per pass keep each chunk's deep bounces full. Start there, with the single-view
+    // tuner's usual pick for the traversal (persistent lanes, 12 blocks per CU, 16 stack
+    // entries per lane in LDS), then descend coordinate by coordinate.
+    const uint32_t q1 = std::max(largest, 1u << 20), q2 = std::max(largest, 1u << 21);
+    pg_batch_config best{q1, 12u, 6u, (uint32_t)base_stack, 1u, 1u, most_spp(q1), 16u};
+    // the chunk size and samples per pass are timed over whole passes, the other
+    // coordinates over passes of at most 16 samples (their ranking does not depend on
+    // the pass length, and the tuner takes a fraction of the time)
+    uint32_t cap = 64u;
     double best_ms = 0.0;
-    pg_status st = measure(best, best_ms);
+    pg_status st = measure(best, cap, best_ms);
     if (st != PG_OK) return st;
-    for (int k = 0; k < (int)cand.size(); ++k) {
-        for (uint32_t val : cand[k]) {
+    auto consider = [&](pg_batch_config c) -> pg_status {
+        if (std::memcmp(&c, &best, sizeof(c)) == 0) return PG_OK;
+        double ms = 0.0;
+        pg_status s2 = measure(c, cap, ms);
+        if (s2 != PG_OK) return s2;
+        if (ms < best_ms) {
+            best_ms = ms;
+            best = c;
+        }
+        return PG_OK;
+    };
+    // chunk size with its most samples per pass, then fewer samples per pass
+    for (uint32_t q : {q2, 0u}) {
+        pg_batch_config c = best;
+        c.queue_paths = q;
+        c.samples_per_pass = most_spp(q);
+        if ((st = consider(c)) != PG_OK) return st;
+    }
+    for (uint32_t div : {2u, 4u}) {
+        pg_batch_config c = best;
+        if (most_spp(c.queue_paths) / div < 1u) continue;
+        c.samples_per_pass = most_spp(c.queue_paths) / div;
+        if ((st = consider(c)) != PG_OK) return st;
+    }
+    cap = 16u;
+    if ((st = measure(best, cap, best_ms)) != PG_OK) return st;
+    std::vector<std::pair<uint32_t pg_batch_config::*, std::vector<uint32_t>>> cand = {
+        {&pg_batch_config::use_graph, {0u, 1u}},
+        {&pg_batch_config::dyn_fetch, {0u, 1u}},
+        {&pg_batch_config::trace_blocks_per_cu, {8u, 16u}},
+        {&pg_batch_config::shade_blocks_per_cu, {3u, 4u}},
+        {&pg_batch_config::stack_lds, {32u}},
+    };
+    for (auto& kv : cand) {
+        for (uint32_t val : kv.second) {
             pg_batch_config c = best;
-            if (get(c, k) == val) continue;
-            get(c, k) = val;
-            // more samples per pass than the path-state limit allows: not a candidate
-            if (k == 6 && val > 1 && chunk_paths(c) * val > (1ull << 26)) continue;
-            double ms = 0.0;
-            if ((st = measure(c, ms)) != PG_OK) return st;
-            if (ms < best_ms) {
-                best_ms = ms;
-                best = c;
-            }
+            c.*(kv.first) = val;
+            if ((st = consider(c)) != PG_OK) return st;
         }
     }
+    // the pick's time over one whole pass
+    if ((st = measure(best, 64u, best_ms)) != PG_OK) return st;
     if ((st = pg_batch_configure(b, &best)) != PG_OK) return st;
     if ((st = clear(b)) != PG_OK) return st;
     b->gpu_ms = 0.0;

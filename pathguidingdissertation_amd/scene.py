@@ -5,7 +5,9 @@ and flatten into the C ABI of ``libpgtrace.so``; every render runs on the GPU.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
+import weakref
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -178,6 +180,21 @@ def write_hdr(path, rgb: np.ndarray) -> None:
     check(load().pg_image_write_hdr(str(path).encode(), _fp(a), a.shape[1], a.shape[0]))
 
 
+# Contexts and batches still open when the interpreter exits are closed by an atexit
+# hook, batches first: module teardown would otherwise release them in no fixed order,
+# possibly after the HIP runtime has begun to shut down.
+_open_devices: "weakref.WeakSet" = weakref.WeakSet()
+_open_batches: "weakref.WeakSet" = weakref.WeakSet()
+
+
+@atexit.register
+def _close_all():
+    for b in list(_open_batches):
+        b.close()
+    for d in list(_open_devices):
+        d.close()
+
+
 class Device:
     """A device context (pg_ctx) bound to one GPU."""
 
@@ -185,6 +202,7 @@ class Device:
         self.lib = load()
         self.ctx = C.c_void_p()
         check(self.lib.pg_create(C.byref(self.ctx), int(device)))
+        _open_devices.add(self)
 
     def close(self):
         if self.ctx:
@@ -517,6 +535,7 @@ class Batch:
         device._chk(self.lib.pg_batch_create(device.ctx, C.byref(self.h)))
         self.views = []  # (scene id, width, height)
         self._keep = []
+        _open_batches.add(self)
 
     def close(self):
         if self.h:
