@@ -10,8 +10,8 @@ taken inside the timed region, and the CPU oracle timed on the host cores on a b
 sample (all threads of the host's share and one thread). Further parts: the batched
 views of the config-5 family at its specification (48 views, 32 SPP; --batch-views 0
 skips it) and time-to-relMSE of the unguided, quadtree-guided and learned-guided
-renderers against a 4096-SPP reference on the same interior (--ttr-config 3: the
-window-lit room; --no-ttr skips it).
+renderers against a 4096-SPP reference, on the open interior and on the window-lit room
+(--ttr-config picks the scenes; --no-ttr skips it).
 """
 from __future__ import annotations
 
@@ -268,6 +268,24 @@ def time_to_relmse(pg, dev, sc, data, max_depth, ref_spp, cap_s, log, target=0.0
             return {"speedup": round(sp, 3), "target_factor": factor, "fraction_of_target": round(sp / factor, 3)}
         cmp["projected_speedup"] = {"learned_vs_unguided": proj_speedup("learned", "unguided", 4.0),
                                     "learned_vs_quadtree": proj_speedup("learned", "quadtree", 2.0)}
+
+        # time to a given error = (cost per sample) x (error x samples): a method's
+        # speedup over another is its per-sample error gain divided by its cost ratio
+        def per_sample(name):
+            spp, sec, e, et, _ = res[name]["curve"][-1]
+            return {"ms_per_spp": round(sec * 1e3 / spp, 3), "relmse_x_spp": round(e * spp, 2),
+                    "trimmed_relmse_x_spp": round(et * spp, 3)}
+        cost = {m: per_sample(m) for m in res}
+
+        def decompose(fast, slow):
+            a, b = cost[fast], cost[slow]
+            c = a["ms_per_spp"] / b["ms_per_spp"]
+            g = b["relmse_x_spp"] / a["relmse_x_spp"]
+            gt = b["trimmed_relmse_x_spp"] / a["trimmed_relmse_x_spp"]
+            return {"cost_ratio": round(c, 3), "per_sample_gain": round(g, 3), "per_sample_gain_trimmed": round(gt, 3),
+                    "equal_time_gain": round(g / c, 3), "equal_time_gain_trimmed": round(gt / c, 3)}
+        cmp["per_sample"] = {"methods": cost, "learned_vs_unguided": decompose("learned", "unguided"),
+                             "learned_vs_quadtree": decompose("learned", "quadtree")}
         if with_pss:
             for v in ("pss", "pss_approx"):
                 cmp[f"{v}_vs_unguided"] = speedup(v, "unguided")
@@ -460,9 +478,9 @@ def main() -> int:
     ap.add_argument("--ttr-cap", type=float, default=float(os.environ.get("PG_BENCH_TTR_CAP", "30")),
                     help="wall-clock seconds per method in the time-to-relMSE study")
     ap.add_argument("--ttr-ref-spp", type=int, default=int(os.environ.get("PG_BENCH_TTR_REF", "4096")))
-    ap.add_argument("--ttr-config", type=int, default=int(os.environ.get("PG_BENCH_TTR_CONFIG", "2")),
-                    help="scene of the time-to-relMSE study (3: the closed room lit through one window, "
-                         "where the guiding configurations are defined; 2: the open interior)")
+    ap.add_argument("--ttr-config", default=os.environ.get("PG_BENCH_TTR_CONFIG", "2,3"),
+                    help="scenes of the time-to-relMSE study, comma-separated (2: the open interior; 3: the "
+                         "closed room lit through one window, where the guiding configurations are defined)")
     ap.add_argument("--ttr-repeats", type=int, default=1,
                     help="independent runs of each method in the study (medians reported when > 1)")
     ap.add_argument("--ttr-env-visibility", type=int, choices=(0, 1), default=1,
@@ -605,14 +623,30 @@ def main() -> int:
 
     ttr = None
     if not a.no_ttr:
-        tdata = data if a.ttr_config == a.config else pg.generate_scene(a.ttr_config, 0)
-        tsc = sc if a.ttr_config == a.config else pg.Scene(dev, tdata)
-        ttr = time_to_relmse(pg, dev, tsc, tdata, max_depth, a.ttr_ref_spp, a.ttr_cap,
-                             lambda m: print(m, file=sys.stderr, flush=True), env_visibility=a.ttr_env_visibility,
-                             repeats=a.ttr_repeats, with_pss=a.ttr_pss, train_every=a.ttr_train_every,
-                             train_steps=a.ttr_train_steps)
-        ttr = {"workload": f"config {a.ttr_config}: {tdata.n_tris} tris, {tdata.width}x{tdata.height}, "
-                           f"max depth {max_depth}", **ttr}
+        ttr = {}
+        for tc in (int(v) for v in str(a.ttr_config).split(",") if v.strip()):
+            tdata = data if tc == a.config else pg.generate_scene(tc, 0)
+            tsc = sc if tc == a.config else pg.Scene(dev, tdata)
+            print(f"ttr: config {tc}", file=sys.stderr, flush=True)
+            r = time_to_relmse(pg, dev, tsc, tdata, max_depth, a.ttr_ref_spp, a.ttr_cap,
+                               lambda m: print(m, file=sys.stderr, flush=True), env_visibility=a.ttr_env_visibility,
+                               repeats=a.ttr_repeats, with_pss=a.ttr_pss, train_every=a.ttr_train_every,
+                               train_steps=a.ttr_train_steps)
+            ttr[f"config{tc}"] = {"workload": f"config {tc}: {tdata.n_tris} tris, {tdata.width}x{tdata.height}, "
+                                              f"max depth {max_depth}", **r}
+            ps = r["comparison"]["per_sample"]
+            print(f"ttr config {tc}: learned vs unguided {ps['learned_vs_unguided']}, "
+                  f"vs quadtree {ps['learned_vs_quadtree']}", file=sys.stderr, flush=True)
+        # one line per scene: the learned guide's time-to-relMSE factors against the targets
+        # (4x unguided, 2x quadtree), measured at each baseline's final relMSE (full and
+        # without the worst 0.01% of pixels)
+        ttr["summary"] = {k: {"learned_vs_unguided": v["comparison"]["learned_vs_unguided"]["speedup"],
+                              "learned_vs_quadtree": v["comparison"]["learned_vs_quadtree"]["speedup"],
+                              "trimmed_learned_vs_unguided": v["comparison"]["trimmed"]["learned_vs_unguided"]["speedup"],
+                              "trimmed_learned_vs_quadtree": v["comparison"]["trimmed"]["learned_vs_quadtree"]["speedup"],
+                              "targets": {"learned_vs_unguided": 4.0, "learned_vs_quadtree": 2.0},
+                              "reached_relmse_0.01_s": {m: v[m]["seconds"] for m in ("unguided", "quadtree", "learned")}}
+                          for k, v in ttr.items()}
 
     line = {
         "metric": "Mrays/s; time-to-relMSE 0.01: learned vs QuadTree vs unguided, 1 MI355X",

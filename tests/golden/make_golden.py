@@ -397,6 +397,45 @@ def make_config2(spp_ref=4096, k=8, spp=64):
     print("config 2 crops: band", np.mean(vals), np.std(vals, ddof=1), vals, "trimmed", trimmed)
 
 
+CONFIG3_CROPS = ((300, 560, 64, 64), (160, 400, 64, 64), (480, 880, 64, 64))
+
+
+def make_config3(spp_ref=4096, k=8, spp=64):
+    """Three 64 x 64 crops of the window-lit room (config 3, 720 x 1280, max depth 12,
+    environment shadow rays along the light direction, as the guided comparisons run):
+    the sun patch below the window, the indirectly lit wall above it and a dark corner
+    lit only through several bounces. The FP64 oracle at 4096 SPP and the relMSE of k
+    equal-SPP oracle runs against it, as make_config2."""
+    from oracle import pg_oracle as po
+    import pathguidingdissertation_amd as pg
+    data = pg.generate_scene(3, 0)
+    osc = po.OracleScene(data)
+
+    def crops(film):
+        return np.stack([film[y:y + h, x:x + w] for x, y, w, h in CONFIG3_CROPS])
+
+    ref = None
+    for c in CONFIG3_CROPS:
+        film, _ = osc.render(spp_ref, seed=9003, max_depth=12, accum_f64=True, env_visibility=1, crop=c)
+        ref = film if ref is None else ref + film
+    ref_c = crops(ref / spp_ref)
+    inner = np.s_[:, 1:-1, 1:-1]
+    vals, trimmed = [], []
+    for s in range(1, k + 1):
+        img = None
+        for c in CONFIG3_CROPS:
+            film, _ = osc.render(spp, seed=s, max_depth=12, accum_f64=True, env_visibility=1, crop=c)
+            img = film if img is None else img + film
+        img_c = crops(img / spp)
+        e = ((img_c[inner] - ref_c[inner]) ** 2 / (ref_c[inner] ** 2 + 1e-2)).mean(-1).reshape(-1)
+        vals.append(float(e.mean()))
+        trimmed.append(float(np.sort(e)[: e.size - e.size // 1000].mean()))  # without the worst 0.1%
+    np.savez_compressed(OUT / "oracle_config3_crops.npz", image=ref_c.astype(np.float32),
+                        crops=np.array(CONFIG3_CROPS, np.int32), spp=spp_ref, seed=9003,
+                        band_spp=spp, band_relmse=np.array(vals), band_trimmed=np.array(trimmed))
+    print("config 3 crops: band", np.mean(vals), np.std(vals, ddof=1), vals, "trimmed", trimmed)
+
+
 def signed_zero_positions(n=3000, seed=31):
     """Triangles whose coordinates are often exactly -0.0 or +0.0 (both signs), so node
     bounds meet ties between -0 and +0 (AABB::extend keeps the first one)."""
@@ -472,6 +511,7 @@ if __name__ == "__main__":
     ap.add_argument("--nee", action="store_true")
     ap.add_argument("--independent", action="store_true")
     ap.add_argument("--config2", action="store_true")
+    ap.add_argument("--config3", action="store_true")
     ap.add_argument("--signed-zero", action="store_true")
     a = ap.parse_args()
     if a.ref:
@@ -486,5 +526,7 @@ if __name__ == "__main__":
         make_independent()
     if a.config2:
         make_config2()
+    if a.config3:
+        make_config3()
     if a.signed_zero:
         make_signed_zero()

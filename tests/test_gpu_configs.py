@@ -8,7 +8,10 @@
   equal-SPP oracle runs (tests/golden/oracle_config2_crops.npz).
 * Config 3 (the window-lit room, 720x1280): a 1-SPP record pass of the full scene,
   the cache built over its records, the k = 16 nearest records of record positions and
-  of random points equal to a brute-force scan (index sets and squared distances), and
+  of random points equal to a brute-force scan (index sets and squared distances); the
+  unguided image's relMSE against FP64 4096-SPP oracle crops inside the band of eight
+  equal-SPP oracle runs, and the quadtree- and learned-guided images no worse than it
+  (tests/golden/oracle_config3_crops.npz); and
   the quadtree-guided image mean (without its brightest 0.01% of pixels) inside the
   spread of unguided runs.
 * Config 4: the learned guide with its 2^19-entry hash table trained on the config-3
@@ -149,6 +152,59 @@ def test_config3_quadtree_guided_mean_inside_unguided_spread(pg, device, config3
     u, q = res["unguided"], res["quadtree"]
     se = np.sqrt(u.var(ddof=1) / u.size + q.var(ddof=1) / q.size)
     assert abs(q.mean() - u.mean()) <= 4 * se + 1e-3 * u.mean(), (u, q)
+
+
+def _crop_relmse(img, g):
+    """relMSE of the crops (box-filter border pixels dropped) against the oracle crops,
+    and the same without the worst 0.1% of pixels."""
+    rects = [tuple(int(v) for v in r) for r in g["crops"]]
+    a = _crops(img, rects)[:, 1:-1, 1:-1].astype(np.float64)
+    ref = g["image"][:, 1:-1, 1:-1].astype(np.float64)
+    e = ((a - ref) ** 2 / (ref ** 2 + 1e-2)).mean(-1).reshape(-1)
+    return float(e.mean()), float(np.sort(e)[: e.size - e.size // 1000].mean())
+
+
+def test_config3_relmse_within_oracle_band(pg, device, config3, config3_cache):
+    """The window-lit room at 64 SPP against the FP64 4096-SPP oracle crops
+    (tests/golden/oracle_config3_crops.npz: the sun patch, the indirectly lit wall, a
+    corner lit through several bounces; environment shadow rays along the light
+    direction): the unguided image's relMSE, with and without the worst 0.1% of pixels,
+    inside the mean +- 3 sigma of eight equal-SPP oracle runs; the quadtree-guided and
+    the learned-guided images (network trained online first) at the same SPP no worse
+    than the band's upper end without the worst 0.1% (a guide changes the variance, not
+    the image; the full relMSE of a guided image is left out, since a single caustic
+    firefly sets it)."""
+    g = np.load(GOLDEN / "oracle_config3_crops.npz")
+    spp = int(g["band_spp"])
+    band, tb = g["band_relmse"], g["band_trimmed"]
+    lo, hi = band.mean() - 3 * band.std(ddof=1), band.mean() + 3 * band.std(ddof=1)
+    tlo, thi = tb.mean() - 3 * tb.std(ddof=1), tb.mean() + 3 * tb.std(ddof=1)
+    sc, _, _ = config3_cache
+    rt = pg.RayTracer(sc, seed=77, max_depth=12, env_visibility=1)
+    rt.clear()
+    rt.render(spp)
+    r, rt_ = _crop_relmse(rt.image(), g)
+    assert lo <= r <= hi, (r, lo, hi, band)
+    assert tlo <= rt_ <= thi, (rt_, tlo, thi, tb)
+    qt = pg.RayTracer(sc, seed=78, max_depth=12, env_visibility=1, integrator=2, guide_prob=0.5)
+    qt.clear()
+    qt.render(spp)
+    q_r, q_t = _crop_relmse(qt.image(), g)
+    assert q_t <= thi, (q_t, thi)
+    # the network trained online for 256 SPP (16 record passes, 64 Adam steps), then a
+    # fresh 64-SPP learned-guided image
+    device.mlp_init(seed=11)
+    tr = pg.RayTracer(sc, seed=79, max_depth=12, env_visibility=1, record_nee=1)
+    tr.clear()
+    tr.render_online(256, steps=4, train_every=16)
+    assert device.mlp_info()["diverged"] == 0
+    ml = pg.RayTracer(sc, seed=80, max_depth=12, env_visibility=1, integrator=3)
+    ml.clear()
+    ml.render(spp)
+    m_r, m_t = _crop_relmse(ml.image(), g)
+    assert m_t <= thi, (m_t, thi)
+    print(f"config 3 crops at {spp} SPP: unguided {r:.4f} / {rt_:.4f}, quadtree {q_r:.4f} / {q_t:.4f}, "
+          f"learned {m_r:.4f} / {m_t:.4f}; band {band.mean():.4f} +- {band.std(ddof=1):.4f}", flush=True)
 
 
 def test_config4_full_table_trains_and_matches_oracle_gradient(pg, device, config3, config3_cache):
