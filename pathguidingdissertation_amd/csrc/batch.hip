@@ -241,6 +241,7 @@ pg_status prepare(pg_batch* b) {
     if (b->samples_per_pass() > 1 && need > (1ull << 26))
         return b->fail(PG_ERR_INVALID, "samples_per_pass x chunk paths above 2^26 (lower one of them)");
     if (b->ps_cap < need) {
+        b->ps_cap = 0;  // stays 0 until every buffer below is allocated
         b->free_paths();
         PathState& p = b->ps;
         const uint32_t m = (uint32_t)need;
