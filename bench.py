@@ -146,7 +146,7 @@ def time_staying_below(curve, e, col=2):
 
 
 def time_to_relmse(pg, dev, sc, data, max_depth, ref_spp, cap_s, log, target=0.01, env_visibility=1, repeats=1,
-                   with_pss=False, train_every=1, train_steps=1):
+                   with_pss=False, train_every=1, train_steps=1, record_samples=1):
     """Time-to-relMSE study: unguided vs quadtree guide vs learned guide. Besides the
     time to relMSE <= target (null when a method does not reach it within cap_s), the
     learned guide is compared with each baseline at that baseline's own final relMSE:
@@ -205,7 +205,8 @@ def time_to_relmse(pg, dev, sc, data, max_depth, ref_spp, cap_s, log, target=0.0
             methods["pss"] = (ps_setup, adv(ps, lambda k: ps.render(k)))
             methods["pss_approx"] = (pf_setup, adv(pf, lambda k: pf.render(k)))
         methods["learned"] = (ml_setup, adv(ml, lambda k: ml.render_online(k, steps=train_steps, window=1,
-                                                                          train_every=train_every)))
+                                                                          train_every=train_every,
+                                                                          record_samples=record_samples)))
         res = {}
         for name, (setup, advance) in methods.items():
             curve, hit = progressive(pg, name if r == 0 else f"{name} (run {r})", ref, setup, advance, cap_s,
@@ -334,8 +335,8 @@ def time_to_relmse(pg, dev, sc, data, max_depth, ref_spp, cap_s, log, target=0.0
                               "BSDF's primary sample space (Renderer.h:398-414), 4x4 flux histogram, 50/50 MIS",
                        "pss_approx": "as pss with approximate neighbours (16 nearest of the 64 records under a "
                                      "greedy point-BVH descent)",
-                       "learned": f"online: a 1-SPP guided record pass and {train_steps} Adam step(s) of 2^18 "
-                                  f"samples every {train_every} SPP, guided passes between"}.items()
+                       "learned": f"online: a {record_samples}-SPP guided record pass and {train_steps} Adam step(s) "
+                                  f"of 2^18 samples every {train_every} SPP, guided passes between"}.items()
                        if k in res},
            **res,
            "learned_diverged": bool(info["diverged"]),
@@ -486,10 +487,12 @@ def main() -> int:
     ap.add_argument("--ttr-env-visibility", type=int, choices=(0, 1), default=1,
                     help="environment shadow rays in the study: 1 along the light direction (default), "
                          "0 the reference's scene-bound target (Renderer.h:331-332)")
-    ap.add_argument("--ttr-train-every", type=int, default=int(os.environ.get("PG_BENCH_TTR_TRAIN_EVERY", "16")),
-                    help="learned guide: a record pass + training steps every N passes, guided passes between")
-    ap.add_argument("--ttr-train-steps", type=int, default=int(os.environ.get("PG_BENCH_TTR_TRAIN_STEPS", "4")),
+    ap.add_argument("--ttr-train-every", type=int, default=int(os.environ.get("PG_BENCH_TTR_TRAIN_EVERY", "32")),
+                    help="learned guide: a record pass + training steps every N SPP, guided passes between")
+    ap.add_argument("--ttr-train-steps", type=int, default=int(os.environ.get("PG_BENCH_TTR_TRAIN_STEPS", "8")),
                     help="learned guide: Adam steps after each record pass")
+    ap.add_argument("--ttr-record-samples", type=int, default=int(os.environ.get("PG_BENCH_TTR_RECORD_SAMPLES", "2")),
+                    help="learned guide: samples per pixel of each record pass (traced as one wavefront pass)")
     ap.add_argument("--ttr-pss", action="store_true",
                     help="add the primary-sample-space guide (exact and approximate neighbours) to the study")
     ap.add_argument("--shape", default="",
@@ -631,7 +634,7 @@ def main() -> int:
             r = time_to_relmse(pg, dev, tsc, tdata, max_depth, a.ttr_ref_spp, a.ttr_cap,
                                lambda m: print(m, file=sys.stderr, flush=True), env_visibility=a.ttr_env_visibility,
                                repeats=a.ttr_repeats, with_pss=a.ttr_pss, train_every=a.ttr_train_every,
-                               train_steps=a.ttr_train_steps)
+                               train_steps=a.ttr_train_steps, record_samples=a.ttr_record_samples)
             ttr[f"config{tc}"] = {"workload": f"config {tc}: {tdata.n_tris} tris, {tdata.width}x{tdata.height}, "
                                               f"max depth {max_depth}", **r}
             ps = r["comparison"]["per_sample"]

@@ -85,7 +85,9 @@ typedef struct {
     uint32_t rr_depth;      /* Renderer.h:561 -> Russian roulette when depth > 3 */
     int32_t integrator;     /* PG_INTEGRATOR_*                                  */
     uint32_t record;        /* RECORD integrator: keep PathVertex records       */
-    float guide_prob;       /* guided integrators: P(sample guide), 0.5         */
+    float guide_prob;       /* guided integrators: P(sample guide), 0.5; the
+                               learned guide samples with its own per-point
+                               probability, capped at guide_prob (0 disables it) */
     uint32_t knn_k;         /* guided QT / PSS: neighbours per lookup (16); PSS
                                with 0 takes the 16 records nearest the centroid of
                                the point-BVH leaf that holds the query's Morton key
@@ -308,7 +310,7 @@ typedef struct pg_mlp_config {
     float diverge_rise;       /* divergence on rising loss: the loss smoothed over
                                  steps (newest weighted 1/4) more than this many
                                  nats above its best value (0: 2; < 0: off) ...  */
-    uint32_t diverge_patience; /* ... for this many consecutive steps (0: 8)     */
+    uint32_t diverge_patience; /* ... for this many consecutive steps (0: 16)    */
     uint32_t rise_test_step;  /* test hook: from this step on the detector sees the
                                  loss raised by 10 nats per step (0 off)         */
     uint32_t reserved[4];
@@ -343,23 +345,25 @@ pg_status pg_mlp_init(pg_ctx* ctx, const pg_mlp_config* cfg);
 /* `steps` Adam steps on the current records (they must come from record passes,
  * which keep the training data). losses (nullable) receives each step's mean loss. */
 pg_status pg_mlp_train(pg_ctx* ctx, uint32_t steps, float* losses);
-/* Learned-guide rendering with online training. Every `train_every`-th pass (the
- * first included) is a guided record pass (PG_INTEGRATOR_GUIDED_MLP with record = 1:
- * the record integrator of Renderer.h:367-529 sampling by one-sample MIS between the
- * BSDF and the guide; its sample goes to the film and its vertices become training
- * data) followed by `steps` Adam steps on the records of the last `window` record
- * passes (older ones are dropped); the first `warmup_passes` record passes train
- * `warmup_steps` steps instead. The other passes are guided path-tracing passes
- * (record = 0); those between two record passes are rendered by one call, so they
- * share wavefront passes (params->samples_per_pass). Every pass adds one sample per
- * pixel. params->record_nee = 1 is recommended.                                   */
+/* Learned-guide rendering with online training. Every `train_every` SPP (the first
+ * included) begin with a guided record pass of `record_samples` samples per pixel
+ * (PG_INTEGRATOR_GUIDED_MLP with record = 1: the record integrator of
+ * Renderer.h:367-529 sampling by one-sample MIS between the BSDF and the guide; its
+ * samples go to the film and their vertices become training data, traced together as
+ * one wavefront pass) followed by `steps` Adam steps on the records of the last
+ * `window` record passes (older ones are dropped); the first `warmup_passes` record
+ * passes train `warmup_steps` steps instead. The other SPP are guided path-tracing
+ * passes (record = 0); those between two record passes are rendered by one call, so
+ * they share wavefront passes (params->samples_per_pass). params->record_nee = 1 is
+ * recommended. record_samples must be below train_every (PG_ERR_INVALID).          */
 typedef struct pg_mlp_schedule {
     uint32_t train_every;    /* 1                                                  */
     uint32_t steps;          /* 1                                                  */
     uint32_t window;         /* 1                                                  */
     uint32_t warmup_passes;  /* 0                                                  */
     uint32_t warmup_steps;   /* 0                                                  */
-    uint32_t reserved[3];
+    uint32_t record_samples; /* samples per pixel of a record pass; 0 = 1          */
+    uint32_t reserved[2];
 } pg_mlp_schedule;
 pg_status pg_mlp_render_online(pg_ctx* ctx, const pg_render_params* params, uint32_t n_spp,
                                const pg_mlp_schedule* schedule);

@@ -74,7 +74,8 @@ class PgMlpInfo(C.Structure):
 
 class PgMlpSchedule(C.Structure):
     _fields_ = [("train_every", C.c_uint32), ("steps", C.c_uint32), ("window", C.c_uint32),
-                ("warmup_passes", C.c_uint32), ("warmup_steps", C.c_uint32), ("reserved", C.c_uint32 * 3)]
+                ("warmup_passes", C.c_uint32), ("warmup_steps", C.c_uint32), ("record_samples", C.c_uint32),
+                ("reserved", C.c_uint32 * 2)]
 
 
 class PgImageMetrics(C.Structure):

@@ -947,7 +947,8 @@ pg_status pg_render_spp(pg_ctx* c, const pg_render_params* params, uint32_t n_sp
     for (uint32_t s = 0; s < n_spp;) {
         const uint32_t ns = std::min(spp_pass, n_spp - s);
         s += ns;
-        PassParams pp{prm.seed, c->spp, W, H, prm.max_depth, prm.rr_depth, prm.integrator, prm.env_visibility, ns};
+        PassParams pp{prm.seed,           c->spp, W, H, prm.max_depth, prm.rr_depth, prm.integrator, prm.env_visibility,
+                      ns,                 prm.guide_prob};
         PG_TRY(c, timed(3, [&] { return launch_raygen(c->dev, c->ps, pp, c->stream); }), "raygen");
         for (uint32_t depth = 0; depth <= prm.max_depth; ++depth) {
             int parity = (int)(depth & 1u);
@@ -1703,7 +1704,9 @@ pg_status pg_mlp_train(pg_ctx* c, uint32_t steps, float* losses) {
     // rising loss: the loss smoothed over steps (weight 1/4 on the newest) staying
     // more than diverge_rise nats above its best value for diverge_patience steps
     const float rise = m.cfg.diverge_rise == 0.f ? 2.f : m.cfg.diverge_rise;
-    const uint32_t patience = m.cfg.diverge_patience ? m.cfg.diverge_patience : 8u;
+    // (16 steps by default: online training takes several steps on each record pass,
+    // and one pass's records can raise the loss for all of its steps)
+    const uint32_t patience = m.cfg.diverge_patience ? m.cfg.diverge_patience : 16u;
     for (uint32_t k = 0; k < steps && !m.diverged && rise > 0.f; ++k) {
         float l = hl[k];
         const uint32_t t = m.step - steps + 1 + k;  // Adam step that produced hl[k]
@@ -1729,10 +1732,13 @@ pg_status pg_mlp_render_online(pg_ctx* c, const pg_render_params* params, uint32
                                 const pg_mlp_schedule* schedule) {
     PG_CHECK_CTX(c);
     if (!c->mlp.ready) return c->fail(PG_ERR_STATE, "pg_mlp_init first");
-    pg_mlp_schedule sch{1, 1, 1, 0, 0, {0, 0, 0}};
+    pg_mlp_schedule sch{1, 1, 1, 0, 0, 1, {0, 0}};
     if (schedule) sch = *schedule;
     if (sch.train_every == 0) sch.train_every = 1;
     if (sch.window == 0) sch.window = 1;
+    if (sch.record_samples == 0) sch.record_samples = 1;
+    if (sch.record_samples > 1 && sch.record_samples >= sch.train_every)
+        return c->fail(PG_ERR_INVALID, "record_samples must be below train_every");
     pg_render_params p = params ? *params : pg_render_params_default();
     p.integrator = PG_INTEGRATOR_GUIDED_MLP;
     auto& m = c->mlp;
@@ -1750,8 +1756,10 @@ pg_status pg_mlp_render_online(pg_ctx* c, const pg_render_params* params, uint32
             if (st != PG_OK) return st;
             continue;
         }
-        ++s;
-        m.online_passes++;
+        // a record pass of record_samples samples (fewer when the call ends first)
+        const uint32_t rs = std::min(sch.record_samples, n_spp - s);
+        s += rs;
+        m.online_passes += rs;
         p.record = 1;
         if (m.window_passes >= sch.window) {
             c->n_records = 0;
@@ -1760,7 +1768,7 @@ pg_status pg_mlp_render_online(pg_ctx* c, const pg_render_params* params, uint32
             c->cache_ready = false;
             m.window_passes = 0;
         }
-        pg_status st = pg_render_spp(c, &p, 1);
+        pg_status st = pg_render_spp(c, &p, rs);
         if (st != PG_OK) return st;
         m.window_passes++;
         m.record_passes++;

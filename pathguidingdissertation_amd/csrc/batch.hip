@@ -288,7 +288,8 @@ hipError_t enqueue_pass(pg_batch* b, const pg_render_params& prm, uint32_t ns) {
     hipError_t e;
     for (size_t ci = 0; ci < b->chunks.size(); ++ci) {
         const auto& ch = b->chunks[ci];
-        PassParams pp{prm.seed, 0u, 0u, 0u, prm.max_depth, prm.rr_depth, prm.integrator, prm.env_visibility, 1u};
+        PassParams pp{prm.seed,         0u, 0u, 0u, prm.max_depth, prm.rr_depth, prm.integrator, prm.env_visibility,
+                      1u,               prm.guide_prob};
         if ((e = launch_raygen_batch(b->ps, nv, ch.first, ch.second, ns, prm.seed, b->d_sample, st)) != hipSuccess)
             return e;
         for (uint32_t depth = 0; depth <= prm.max_depth; ++depth) {

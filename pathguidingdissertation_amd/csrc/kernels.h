@@ -84,6 +84,7 @@ struct PassParams {
     uint32_t n_samples;       // samples per pixel traced together in this pass (0 = 1):
                               // path i is sample (sample + i / (width*height)) of pixel
                               // i % (width*height)
+    float guide_cap;          // learned guide: its selection probability is capped here
 };
 
 // Launch helpers (defined in kernels.hip); all enqueue on `stream`.

@@ -550,8 +550,10 @@ __device__ __forceinline__ bool next_event(const DevScene& s, const MaterialRec&
 // One bounce of RayTracer::pathTraceRecursive + computeDirect (Renderer.h:269-354, 531-613).
 // MODE 0: BSDF sampling as the reference. MODE 1: one-sample MIS between the BSDF
 // and the quadtree guide (probability guide_prob); MODE 2: between the BSDF and the
-// learned vMF mixture (probability c, learned per point). The combined density is
-// used in every MIS weight so that the estimator stays unbiased.
+// learned vMF mixture (probability c, learned per point, at most guide_prob: the
+// weight of a BSDF-sampled direction is then at most 1 / (1 - guide_prob) times the
+// plain BSDF-sampling weight where the guide misses the radiance). The combined
+// density is used in every MIS weight so that the estimator stays unbiased.
 template <int MODE, bool MULTI>
 __global__ __launch_bounds__(kShadeBlock, 3) void k_shade(DevScene s0, PathState ps, PassParams p, GuideCtx gc,
                                                        int parity, int depth) {
@@ -644,7 +646,7 @@ __global__ __launch_bounds__(kShadeBlock, 3) void k_shade(DevScene s0, PathState
                         guided = !m.specular && cells != nullptr && cells[0] != 0xffffffffu && pss_invertible(m) &&
                                  pss_build(gc.g, cells, m, sd, ph);
                     if (MODE == 2 && !m.specular && mx != nullptr) {
-                        ag = mx[12].x;
+                        ag = fminf(mx[12].x, p.guide_cap);
                         guided = ag > 0.f;
                     }
                     // computeDirect (Renderer.h:269-354)
@@ -960,7 +962,7 @@ __global__ __launch_bounds__(kShadeBlock, 3) void k_shade_record(DevScene s, Pat
                     }
                 } else {
                     const float4* mx = (MODE == 2 && mix) ? mix + (size_t)i * kMixF4 : nullptr;
-                    const float ag = mx ? mx[12].x : 0.f;
+                    const float ag = mx ? fminf(mx[12].x, p.guide_cap) : 0.f;
                     const bool guided = MODE == 2 && !m.specular && ag > 0.f;
                     if (!m.specular && s.n_lights > 0) {
                         if (NEE) {

@@ -82,7 +82,7 @@ int main(int argc, char** argv) {
                      argv[0]);
         return 2;
     }
-    pg_mlp_schedule schedule{1, 1, 1, 0, 0, {0, 0, 0}};
+    pg_mlp_schedule schedule{1, 1, 1, 0, 0, 1, {0, 0}};
     if (argc >= 8) {
         schedule.train_every = (uint32_t)std::atoi(argv[6]);
         schedule.steps = (uint32_t)std::atoi(argv[7]);

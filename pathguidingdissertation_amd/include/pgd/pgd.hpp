@@ -247,7 +247,7 @@ public:
         check(pg_mlp_train(dev_->get(), steps, losses.data()), dev_->get());
         return losses;
     }
-    void render_online(pg_render_params params, uint32_t spp, pg_mlp_schedule schedule = {1, 1, 1, 0, 0, {0, 0, 0}}) {
+    void render_online(pg_render_params params, uint32_t spp, pg_mlp_schedule schedule = {1, 1, 1, 0, 0, 1, {0, 0}}) {
         check(pg_mlp_render_online(dev_->get(), &params, spp, &schedule), dev_->get());
     }
     pg_mlp_info info() const {

@@ -456,9 +456,9 @@ class RayTracer:
         self.device._chk(self.device.lib.pg_render_spp(self.device.ctx, C.byref(self.params), int(n_spp)))
 
     def render_online(self, n_spp: int, steps: int = 1, window: int = 1, train_every: int = 1,
-                      warmup_passes: int = 0, warmup_steps: int = 0):
+                      warmup_passes: int = 0, warmup_steps: int = 0, record_samples: int = 1):
         """Learned-guide passes with online training (pg_mlp_render_online)."""
-        sch = _lib.PgMlpSchedule(train_every, steps, window, warmup_passes, warmup_steps)
+        sch = _lib.PgMlpSchedule(train_every, steps, window, warmup_passes, warmup_steps, record_samples)
         self.device._chk(self.device.lib.pg_mlp_render_online(self.device.ctx, C.byref(self.params), int(n_spp),
                                                               C.byref(sch)))
 

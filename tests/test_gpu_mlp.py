@@ -265,7 +265,7 @@ def test_training_reduces_loss(pg, device):
 @pytest.mark.parametrize("cause", ["nonfinite", "rising_loss"])
 def test_divergence_falls_back_to_the_quadtree_guide(pg, device, cause):
     """A learned guide whose training diverges (a non-finite gradient, or a loss that
-    rises 2 nats above its smoothed best for 8 steps, forced from the host through
+    rises 2 nats above its smoothed best for 16 steps, forced from the host through
     the test hooks) is not trained further by online rendering, and learned-guided
     rendering runs
     the quadtree guide of the cache: the film equals, bit for bit, a quadtree-guided
@@ -274,7 +274,7 @@ def test_divergence_falls_back_to_the_quadtree_guide(pg, device, cause):
     device.cache_build(cell_size=4096)
     hook = dict(nan_step=3) if cause == "nonfinite" else dict(rise_test_step=6)
     device.mlp_init(seed=7, batch=1 << 14, **hook)
-    device.mlp_train(20)
+    device.mlp_train(40)
     info = device.mlp_info()
     assert info["diverged"] == 1
     assert info["diverge_reason"] == (1 if cause == "nonfinite" else 2)
@@ -294,7 +294,7 @@ def test_divergence_falls_back_to_the_quadtree_guide(pg, device, cause):
 
 def test_rising_loss_detector_stays_quiet_on_normal_training(pg, device):
     """The rising-loss detector does not fire while a guide trains normally (its loss
-    falls), with the default 2-nat rise and 8-step patience."""
+    falls), with the default 2-nat rise and 16-step patience."""
     _record(pg, device, 101, 2)
     device.mlp_init(seed=7)
     losses = device.mlp_train(60)
