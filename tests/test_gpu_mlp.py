@@ -292,6 +292,25 @@ def test_divergence_falls_back_to_the_quadtree_guide(pg, device, cause):
     assert device.mlp_info()["diverged"] == 0 and device.mlp_info()["diverge_reason"] == 0
 
 
+def test_guide_prob_caps_the_learned_selection_probability(pg, device):
+    """The learned guide samples its mixture with its per-point probability capped at
+    render params guide_prob: with 0 no point is guided, and the film equals the
+    unguided path tracer's bit for bit (same random numbers, BSDF-only MIS); with the
+    default 0.5 the guide is used and the image changes."""
+    sc = _record(pg, device, 101, 1)
+    device.mlp_init(seed=7, batch=1 << 14)
+    device.mlp_train(8)
+    films = {}
+    for name, kw in (("capped_off", dict(integrator=3, guide_prob=0.0)), ("unguided", dict(integrator=0)),
+                     ("guided", dict(integrator=3, guide_prob=0.5))):
+        rt = pg.RayTracer(sc, seed=6, max_depth=12, **kw)
+        rt.clear()
+        rt.render(2)
+        films[name] = rt.film()[0]
+    assert np.array_equal(films["capped_off"].view(np.uint32), films["unguided"].view(np.uint32))
+    assert not np.array_equal(films["guided"].view(np.uint32), films["unguided"].view(np.uint32))
+
+
 def test_rising_loss_detector_stays_quiet_on_normal_training(pg, device):
     """The rising-loss detector does not fire while a guide trains normally (its loss
     falls), with the default 2-nat rise and 16-step patience."""
