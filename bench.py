@@ -365,10 +365,13 @@ def time_to_relmse(pg, dev, sc, data, max_depth, ref_spp, cap_s, log, target=0.0
     return out
 
 
-def batch_study(pg, dev, n_views: int, spp: int, max_depth: int, log) -> dict:
+def batch_study(pg, dev, n_views: int, spp: int, max_depth: int, log, shape: dict | None = None,
+                guided: bool = True) -> dict:
     """Batched views of the config-5 scene family (seed i for view i, 100k-1M triangles),
     resolutions drawn with a seeded RNG from {128x128, 256x512, 640x360, 1280x720}: one
-    hipGraph launch per SPP after the autotuner has picked the launch configuration."""
+    hipGraph launch per SPP after the autotuner has picked the launch configuration
+    (`shape`: a fixed configuration instead, for profiling runs; `guided`: also the
+    learned-guided batch)."""
     rng = np.random.default_rng(12345)
     sizes = [(128, 128), (256, 512), (640, 360), (1280, 720)]
     b = pg.Batch(dev)
@@ -385,7 +388,11 @@ def batch_study(pg, dev, n_views: int, spp: int, max_depth: int, log) -> dict:
         pixels += w * h
     setup = time.perf_counter() - t0
     t0 = time.perf_counter()
-    best, tuned_ms = b.autotune(max_depth=max_depth)
+    if shape:
+        best, tuned_ms = dict(shape), None
+        b.configure(**best)
+    else:
+        best, tuned_ms = b.autotune(max_depth=max_depth)
     tune = time.perf_counter() - t0
     b.clear()
     s0 = b.stats()
@@ -410,8 +417,24 @@ def batch_study(pg, dev, n_views: int, spp: int, max_depth: int, log) -> dict:
              "vs_tuned": round(wall / gr_wall, 3)}
     log(f"batch graph replay: {graph['ms_per_spp']} ms/SPP, {gs['graph_nodes']} graph nodes")
     b.configure(**best)
-    # the same views guided by per-scene learned networks (trained online on each scene's
-    # own camera first: 16 guided record passes with one Adam step each)
+    guided_res = None
+    if guided:
+        guided_res = _batch_guided(pg, dev, b, n_views, spp, max_depth, pixels, log)
+    b.close()
+    return {"views": n_views, "scenes_tris": tris, "pixels": pixels, "spp": spp, "max_depth": max_depth,
+            "resolutions": "drawn from {128x128, 256x512, 640x360, 1280x720} with a seeded RNG",
+            "learned_guided": guided_res, "graph_replay": graph,
+            "ms_per_spp": round(wall * 1e3 / spp, 3), "samples_per_s": round(pixels * spp / wall, 1),
+            "mrays_per_s": round(rays / wall / 1e6, 2), "gpu_ms_per_spp": round((s1["gpu_ms"] - s0["gpu_ms"]) / spp, 3),
+            "graph_nodes": s1["graph_nodes"], "chunks": s1["chunks"],
+            "tuned_config": best, "tuned_ms_per_spp": round(tuned_ms, 3) if tuned_ms is not None else None,
+            "config_source": "--batch-shape" if shape else "pg_batch_autotune",
+            "setup_s": round(setup, 1), "tune_s": round(tune, 1)}
+
+
+def _batch_guided(pg, dev, b, n_views: int, spp: int, max_depth: int, pixels: int, log) -> dict:
+    """The batch's views guided by per-scene learned networks (each trained online on its
+    scene's own camera first: 16 guided record passes with one Adam step each)."""
     t0 = time.perf_counter()
     for i in range(n_views):
         d = pg.generate_scene(5, i)
@@ -429,18 +452,9 @@ def batch_study(pg, dev, n_views: int, spp: int, max_depth: int, log) -> dict:
     g1 = b.stats()
     grays = (g1["closest_rays"] - g0["closest_rays"]) + (g1["shadow_rays"] - g0["shadow_rays"])
     log(f"batch guided: {gwall * 1e3 / spp:.2f} ms/SPP after {train:.1f}s of per-scene training")
-    b.close()
-    guided = {"ms_per_spp": round(gwall * 1e3 / spp, 3), "samples_per_s": round(pixels * spp / gwall, 1),
-              "mrays_per_s": round(grays / gwall / 1e6, 2), "train_s": round(train, 1),
-              "guides": "one network per scene, trained online on the scene's own camera (16 SPP)"}
-    return {"views": n_views, "scenes_tris": tris, "pixels": pixels, "spp": spp, "max_depth": max_depth,
-            "resolutions": "drawn from {128x128, 256x512, 640x360, 1280x720} with a seeded RNG",
-            "learned_guided": guided, "graph_replay": graph,
-            "ms_per_spp": round(wall * 1e3 / spp, 3), "samples_per_s": round(pixels * spp / wall, 1),
-            "mrays_per_s": round(rays / wall / 1e6, 2), "gpu_ms_per_spp": round((s1["gpu_ms"] - s0["gpu_ms"]) / spp, 3),
-            "graph_nodes": s1["graph_nodes"], "chunks": s1["chunks"],
-            "tuned_config": best, "tuned_ms_per_spp": round(tuned_ms, 3), "setup_s": round(setup, 1),
-            "tune_s": round(tune, 1)}
+    return {"ms_per_spp": round(gwall * 1e3 / spp, 3), "samples_per_s": round(pixels * spp / gwall, 1),
+            "mrays_per_s": round(grays / gwall / 1e6, 2), "train_s": round(train, 1),
+            "guides": "one network per scene, trained online on the scene's own camera (16 SPP)"}
 
 
 def host_cpu() -> dict:
@@ -476,6 +490,10 @@ def main() -> int:
     ap.add_argument("--batch-views", type=int, default=int(os.environ.get("PG_BENCH_BATCH_VIEWS", "48")),
                     help="views of the batched-render measurement (0 skips it)")
     ap.add_argument("--batch-spp", type=int, default=int(os.environ.get("PG_BENCH_BATCH_SPP", "32")))
+    ap.add_argument("--batch-shape", default="",
+                    help="fixed batch configuration instead of the batch tuner's pick, as comma-separated "
+                         "pg_batch_config fields, e.g. queue_paths=1048576,samples_per_pass=64 (profiling runs)")
+    ap.add_argument("--batch-no-guided", action="store_true", help="skip the learned-guided batch")
     ap.add_argument("--ttr-cap", type=float, default=float(os.environ.get("PG_BENCH_TTR_CAP", "30")),
                     help="wall-clock seconds per method in the time-to-relMSE study")
     ap.add_argument("--ttr-ref-spp", type=int, default=int(os.environ.get("PG_BENCH_TTR_REF", "4096")))
@@ -621,8 +639,12 @@ def main() -> int:
 
     batch = None
     if a.batch_views > 0:
+        bshape = None
+        if a.batch_shape:
+            bshape = {k.strip(): int(v) for k, v in (kv.split("=") for kv in a.batch_shape.split(","))}
         batch = batch_study(pg, dev, a.batch_views, a.batch_spp, max_depth,
-                            lambda m: print(m, file=sys.stderr, flush=True))
+                            lambda m: print(m, file=sys.stderr, flush=True), shape=bshape,
+                            guided=not a.batch_no_guided)
 
     ttr = None
     if not a.no_ttr:
