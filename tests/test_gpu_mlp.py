@@ -409,3 +409,34 @@ def test_online_guided_passes_between_records_share_wavefronts(pg, device):
             assert n_steps == [3, 3]  # record passes 0, 4 and 8
             m0, m1 = float(films[0].mean()), float(films[1].mean())
             assert abs(m0 - m1) < 0.05 * m0, (m0, m1)
+
+
+def test_multi_sample_record_passes_keep_the_schedule(pg, device):
+    """Record passes of several samples (pg_mlp_schedule.record_samples): every
+    train_every SPP begin with one record pass of record_samples samples, the film holds
+    every requested sample and agrees with the one made with one-sample record passes
+    (the extra record samples are samples of the same image by the record integrator),
+    the Adam steps follow the record passes, and record_samples at or above train_every
+    is an error."""
+    data = pg.generate_scene(101, 0)
+    sc = pg.Scene(device, data)
+    films = []
+    for rs in (1, 2):
+        device.mlp_init(seed=9)
+        rt = pg.RayTracer(sc, seed=4, max_depth=12, record_nee=1)
+        rt.clear()
+        rt.render_online(21, steps=0, train_every=8, record_samples=rs)
+        film, spp = rt.film()
+        assert spp == 21
+        films.append(film)
+    assert films[0].sum() > 0
+    m0, m1 = float(films[0].mean()), float(films[1].mean())
+    assert abs(m0 - m1) < 0.05 * m0, (m0, m1)
+    device.mlp_init(seed=9)
+    rt = pg.RayTracer(sc, seed=4, max_depth=12, record_nee=1)
+    rt.clear()
+    rt.render_online(40, steps=2, train_every=8, record_samples=2)
+    assert rt.film()[1] == 40
+    assert device.mlp_info()["step"] == 10  # record passes at SPP 0, 8, 16, 24 and 32
+    with pytest.raises(pg.PgError):
+        rt.render_online(8, steps=1, train_every=4, record_samples=4)
