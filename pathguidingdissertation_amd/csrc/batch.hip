@@ -333,6 +333,7 @@ pg_status check_params(pg_batch* b, const pg_render_params& prm) {
     }
     if (prm.max_depth > 64) return b->fail(PG_ERR_INVALID, "max_depth > 64");
     if (prm.env_visibility > 1u) return b->fail(PG_ERR_INVALID, "env_visibility must be 0 or 1");
+    if (!(prm.guide_prob >= 0.f && prm.guide_prob <= 1.f)) return b->fail(PG_ERR_INVALID, "guide_prob outside [0,1]");
     return PG_OK;
 }
 

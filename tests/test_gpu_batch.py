@@ -150,6 +150,8 @@ def test_batch_errors(pg, device, cornell):
         b.render(1, integrator=2)
     with pytest.raises(pg.PgError):
         b.configure(stack=65)
+    with pytest.raises(pg.PgError):
+        b.render(1, guide_prob=1.5)
     b.close()
 
 
