@@ -692,7 +692,10 @@ pg_status pg_batch_autotune(pg_batch* b, const pg_render_params* params, pg_batc
         double ms = 0.0;
         pg_status s2 = measure(c, cap, ms);
         if (s2 != PG_OK) return s2;
-        if (ms < best_ms) {
+        // graph replay stays on unless eager launches are clearly faster: the graph takes
+        // the per-launch host work off the render (hundreds of launches per pass)
+        const double margin = (best.use_graph && !c.use_graph) ? 0.98 : 1.0;
+        if (ms < best_ms * margin) {
             best_ms = ms;
             best = c;
         }
