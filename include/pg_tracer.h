@@ -439,7 +439,8 @@ pg_status pg_batch_configure(pg_batch* batch, const pg_batch_config* cfg);
  * scenes then stay resident in L2 and the Infinity Cache while they are traced) with
  * the most samples per pass the 2^26-path state allows, and descends coordinate by
  * coordinate: chunk size with samples per pass, fewer samples per pass, graph, ray
- * claims, traversal and shade blocks per CU, LDS stack entries. */
+ * claims, traversal and shade blocks per CU, LDS stack entries. Graph replay is kept
+ * unless eager launches are more than 2% faster. */
 pg_status pg_batch_autotune(pg_batch* batch, const pg_render_params* params, pg_batch_config* best,
                             double* ms_per_spp);
 /* Learned guide of one scene of the batch (no reference counterpart; configuration 5
