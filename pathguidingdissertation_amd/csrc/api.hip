@@ -911,7 +911,7 @@ pg_status pg_render_spp(pg_ctx* c, const pg_render_params* params, uint32_t n_sp
                                 c->stream);
     };
     const bool sort_rays = prm.sort_rays != 0 && !record_pass;
-    const bool dyn = prm.dyn_fetch != 0 && !record_pass && !sort_rays;
+    const bool dyn = prm.dyn_fetch != 0 && !sort_rays;
     if (mode == 2 && c->mix_cap < np) {
         c->mix_cap = 0;
         dfree(c->mix);

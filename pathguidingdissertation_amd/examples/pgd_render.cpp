@@ -2,14 +2,15 @@
 // render loop (main.cpp:147-199) with the GPU path behind the same class names.
 //
 //   pgd_render <config> <spp> <seed> <out.bin> [--learned [train_every steps
-//              [max_depth env_visibility]]]
+//              [max_depth env_visibility [record_samples]]]]
 //                                                 render a procedural scene (unguided, or
 //                                                 with the learned guide trained online:
 //                                                 a guided record pass + `steps` Adam steps
 //                                                 every `train_every` SPP, default 1 and 1,
 //                                                 guided passes between; max depth default
 //                                                 8, environment shadow rays along the light
-//                                                 direction when env_visibility is 1);
+//                                                 direction when env_visibility is 1, record
+//                                                 passes of record_samples samples, default 1);
 //                                                 writes int32 width, height, spp + film sum
 //   pgd_render --scene <dir> <spp> <seed> <out.png> [out.hdr]
 //                                                 the reference's main.cpp flow on one of its
@@ -74,11 +75,12 @@ static int render_scene_dir(int argc, char** argv) {
 int main(int argc, char** argv) {
     if (argc == 2 && std::strcmp(argv[1], "--selftest") == 0) return selftest();
     if ((argc == 6 || argc == 7) && std::strcmp(argv[1], "--scene") == 0) return render_scene_dir(argc, argv);
-    const bool learned = (argc == 6 || argc == 8 || argc == 10) && std::strcmp(argv[5], "--learned") == 0;
+    const bool learned =
+        (argc == 6 || argc == 8 || argc == 10 || argc == 11) && std::strcmp(argv[5], "--learned") == 0;
     if (argc != 5 && !learned) {
         std::fprintf(stderr,
                      "usage: %s <config> <spp> <seed> <out.bin> [--learned [train_every steps [max_depth "
-                     "env_visibility]]] | --selftest\n",
+                     "env_visibility [record_samples]]]] | --selftest\n",
                      argv[0]);
         return 2;
     }
@@ -87,13 +89,14 @@ int main(int argc, char** argv) {
         schedule.train_every = (uint32_t)std::atoi(argv[6]);
         schedule.steps = (uint32_t)std::atoi(argv[7]);
     }
+    if (argc == 11) schedule.record_samples = (uint32_t)std::atoi(argv[10]);
     try {
         pgd::SceneData data = pgd::SceneData::generate(std::atoi(argv[1]), 0);
         pgd::Device dev(0);
         pgd::Scene scene(dev, data);
         pg_render_params p = pg_render_params_default();
         p.seed = (uint32_t)std::atoi(argv[3]);
-        if (argc == 10) {
+        if (argc >= 10) {
             p.max_depth = (uint32_t)std::atoi(argv[8]);
             p.env_visibility = (uint32_t)std::atoi(argv[9]);
         }
