@@ -766,6 +766,9 @@ static pg_status ensure_scan(pg_ctx* c, uint32_t n) {
 // vertices of the per-path stacks a record pass may hold (128 B each): 2^22 paths of
 // 13 vertices (max_depth 12), ~7 GB
 static constexpr uint64_t kRecordVertexBudget = (1ull << 22) * 13;
+// bounces (from 0) whose learned-guide queries are evaluated in Morton order, split by XCD
+static constexpr uint32_t kInferSortDepth = 3;
+static constexpr int kInferSortBits = 16;
 
 static pg_status ensure_record_state(pg_ctx* c, uint32_t n_paths, uint32_t depth) {
     if (c->rs_paths >= n_paths && c->rs_depth >= depth) return PG_OK;
@@ -889,7 +892,7 @@ pg_status pg_render_spp(pg_ctx* c, const pg_render_params* params, uint32_t n_sp
         c->guide_cells_cap = np;
     }
     // sort buffers: kNN query order (quadtree guide) and ray trace order (sort_rays)
-    if ((mode == 1 || (mode == 3 && prm.knn_k != 0) || prm.sort_rays) && c->q_cap < np) {
+    if ((mode == 1 || mode == 2 || (mode == 3 && prm.knn_k != 0) || prm.sort_rays) && c->q_cap < np) {
         c->q_cap = 0;
         dfree(c->q_keys); dfree(c->q_vals); dfree(c->q_keys_tmp); dfree(c->q_vals_tmp); dfree(c->q_hist); dfree(c->q_scan);
         PG_TRY(c, dalloc(&c->q_keys, np), "alloc sort buffers");
@@ -903,6 +906,20 @@ pg_status pg_render_spp(pg_ctx* c, const pg_render_params* params, uint32_t n_sp
     const float3 q_lo = make_float3(c->host.centre[0] - c->host.radius, c->host.centre[1] - c->host.radius,
                                     c->host.centre[2] - c->host.radius);
     const float q_inv = c->host.radius > 0.f ? 1.f / (2.f * c->host.radius) : 0.f;
+    // learned guide: the bounce's queries in Morton order of their shading points, for
+    // the first bounces (whose queues fill most of the pass; deeper queues are short and
+    // would pay for sorting the whole capacity)
+    auto infer_order = [&](int parity, uint32_t depth, const uint32_t*& order) -> hipError_t {
+        order = nullptr;
+        if (depth > kInferSortDepth) return hipSuccess;
+        hipError_t e = launch_query_keys(c->dev, c->ps, q_lo, q_inv, parity, np, c->q_keys, c->q_vals, c->stream,
+                                         kInferSortBits);
+        if (e == hipSuccess)
+            e = radix_sort_pairs(c->q_keys, c->q_vals, c->q_keys_tmp, c->q_vals_tmp, np, kInferSortBits, c->q_hist,
+                                 c->q_scan, c->stream);
+        order = c->q_vals;
+        return e;
+    };
     // trace order of a ray queue (shadow = 1: the shadow queue) for sort_rays
     auto ray_order = [&](int parity, int shadow) -> hipError_t {
         hipError_t e = launch_ray_keys(c->ps, q_lo, q_inv, parity, shadow, np, c->q_keys, c->q_vals, c->stream);
@@ -957,8 +974,11 @@ pg_status pg_render_spp(pg_ctx* c, const pg_render_params* params, uint32_t n_sp
             PG_TRY(c, timed(0, [&] { return launch_extend(c->dev, c->ps, parity, stack, c->counting, grid_trace, c->stream, false, sort_here ? c->q_vals : nullptr, dyn); }), "extend");
             c->n_extend++;
             if (record_pass) {
-                if (mode == 2)
-                    PG_TRY(c, timed(4, [&] { return launch_mlp_infer(c->dev, c->ps, c->mlp.view, parity, c->mix, grid_mlp, c->stream); }), "mlp_infer");
+                if (mode == 2) {
+                    const uint32_t* ord = nullptr;
+                    PG_TRY(c, timed(4, [&] { return infer_order(parity, depth, ord); }), "query order");
+                    PG_TRY(c, timed(4, [&] { return launch_mlp_infer(c->dev, c->ps, c->mlp.view, parity, c->mix, grid_mlp, c->stream, false, 0u, ord); }), "mlp_infer");
+                }
                 const float4* mx = mode == 2 ? c->mix : nullptr;
                 PG_TRY(c, timed(1, [&] { return launch_shade_record(c->dev, c->ps, pp, c->rs, mx, parity, (int)depth, grid_shade, c->stream); }), "shade_record");
                 if (c->rs.li_nee) {
@@ -990,8 +1010,11 @@ pg_status pg_render_spp(pg_ctx* c, const pg_render_params* params, uint32_t n_sp
                 }), "guide query order");
                 PG_TRY(c, timed(4, [&] { return launch_guide_cells(c->dev, c->ps, c->gv, c->guide_cells, parity, grid_guide, c->stream, c->q_vals, np, 0); }), "guide_cells");
             }
-            if (mode == 2)
-                PG_TRY(c, timed(4, [&] { return launch_mlp_infer(c->dev, c->ps, c->mlp.view, parity, c->mix, grid_mlp, c->stream); }), "mlp_infer");
+            if (mode == 2) {
+                const uint32_t* ord = nullptr;
+                PG_TRY(c, timed(4, [&] { return infer_order(parity, depth, ord); }), "query order");
+                PG_TRY(c, timed(4, [&] { return launch_mlp_infer(c->dev, c->ps, c->mlp.view, parity, c->mix, grid_mlp, c->stream, false, 0u, ord); }), "mlp_infer");
+            }
             PG_TRY(c, timed(1, [&] { return launch_shade(c->dev, c->ps, pp, gc, mode, parity, (int)depth, grid_shade, c->stream); }), "shade");
             if (sort_rays) PG_TRY(c, timed(3, [&] { return ray_order(parity, 1); }), "shadow order");
             PG_TRY(c, timed(2, [&] { return launch_shadow(c->dev, c->ps, stack, c->counting, grid_trace, c->stream, false, sort_rays ? c->q_vals : nullptr, dyn); }), "shadow");

@@ -140,16 +140,22 @@ hipError_t launch_leaf_nbrs(const GuideView& g, uint32_t n_leaves, uint32_t* out
 hipError_t launch_guide_cells(const DevScene& s, const PathState& ps, const GuideView& g, uint32_t* cells, int parity,
                               uint32_t grid, hipStream_t stream, const uint32_t* order = nullptr,
                               uint32_t capacity = 0, int records_out = 0);
+// key_bits < 24 keeps the top key_bits of the 24-bit Morton code (a coarser order that
+// sorts in fewer radix passes)
 hipError_t launch_query_keys(const DevScene& s, const PathState& ps, float3 lo, float inv_ext, int parity,
-                             uint32_t capacity, uint32_t* keys, uint32_t* vals, hipStream_t stream);
+                             uint32_t capacity, uint32_t* keys, uint32_t* vals, hipStream_t stream, int key_bits = 24);
 // mix != nullptr: learned-guide record pass (one-sample MIS with the mixtures of k_mlp_infer)
 hipError_t launch_shade_record(const DevScene& s, const PathState& ps, const PassParams& p, const RecordState& rs,
                                const float4* mix, int parity, int depth, uint32_t grid, hipStream_t stream);
 hipError_t launch_shadow_record(const DevScene& s, const PathState& ps, const RecordState& rs, int lds_stack, uint32_t grid,
                                 hipStream_t stream);
 struct MlpView;
+// order (nullable): the queue slots in Morton order of their shading points (a sorted
+// permutation whose first counters[parity] entries are the queue); the eighth of them
+// that XCD x evaluates stays in that XCD's L2 (see k_mlp_infer)
 hipError_t launch_mlp_infer(const DevScene& s, const PathState& ps, const MlpView& v, int parity, float4* mix,
-                            uint32_t grid, hipStream_t stream, bool multi = false, uint32_t scene = 0);
+                            uint32_t grid, hipStream_t stream, bool multi = false, uint32_t scene = 0,
+                            const uint32_t* order = nullptr);
 hipError_t launch_mix_clear(const PathState& ps, int parity, float4* mix, uint32_t grid, hipStream_t stream);
 // Batches: the queue slots grouped by scene (cnt, off, cur: n_scenes + 1 entries each;
 // perm: queue capacity + 128 * n_scenes; n_tiles: 1) so that one launch runs every

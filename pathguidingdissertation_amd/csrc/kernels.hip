@@ -741,7 +741,7 @@ __device__ __forceinline__ uint32_t spread8(uint32_t x) {
     return x;
 }
 __global__ void k_query_keys(DevScene s, PathState ps, float3 lo, float inv_ext, int parity, uint32_t capacity,
-                             uint32_t* keys, uint32_t* vals) {
+                             uint32_t* keys, uint32_t* vals, int shift) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= capacity) return;
     uint32_t key = 0xffffffu;
@@ -760,7 +760,7 @@ __global__ void k_query_keys(DevScene s, PathState ps, float3 lo, float inv_ext,
             }
         }
     }
-    keys[i] = key;
+    keys[i] = key >> shift;
     vals[i] = i;
 }
 
@@ -1420,9 +1420,9 @@ hipError_t launch_ray_keys(const PathState& ps, float3 lo, float inv_ext, int pa
     return hipGetLastError();
 }
 hipError_t launch_query_keys(const DevScene& s, const PathState& ps, float3 lo, float inv_ext, int parity,
-                             uint32_t capacity, uint32_t* keys, uint32_t* vals, hipStream_t stream) {
+                             uint32_t capacity, uint32_t* keys, uint32_t* vals, hipStream_t stream, int key_bits) {
     hipLaunchKernelGGL(k_query_keys, dim3((capacity + 255) / 256), dim3(256), 0, stream, s, ps, lo, inv_ext, parity,
-                       capacity, keys, vals);
+                       capacity, keys, vals, 24 - key_bits);
     return hipGetLastError();
 }
 hipError_t launch_shade_record(const DevScene& s, const PathState& ps, const PassParams& p, const RecordState& rs,

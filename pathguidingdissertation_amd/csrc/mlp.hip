@@ -653,24 +653,36 @@ __global__ __launch_bounds__(kMlpBlock) void k_mlp_eval(MlpView v, const float4*
 // written, with that scene's network; the other slots are left as they are.
 template <bool MULTI>
 __global__ __launch_bounds__(kMlpBlock) void k_mlp_infer(DevScene s0, PathState ps, MlpView v, int parity, float4* mix,
-                                                         uint32_t scene) {
+                                                         uint32_t scene, const uint32_t* __restrict__ order) {
     __shared__ __align__(16) uint16_t sw[kMirror];
     __shared__ float sb[kBiasCount];
     __shared__ float stage[4 * 16 * kStagePitch];  // half a tile at a time: 3 blocks per CU
     const uint32_t n = ps.counters[parity];
-    if (blockIdx.x * 128u >= n) return;
+    const uint32_t n_tiles = (n + 127u) / 128u;
+    uint32_t t_begin = blockIdx.x, t_end = n_tiles, t_step = gridDim.x;
+    if (order && (gridDim.x & 7u) == 0u) {
+        // Workgroups are dispatched to the 8 XCDs round robin (block b runs on XCD
+        // b % 8): the blocks of XCD x take the x-th eighth of the Morton-ordered
+        // queries, so the hash-grid lines of one eighth of the scene's surface stay in
+        // that XCD's L2 instead of every XCD caching lines of the whole scene.
+        const uint32_t xcd = blockIdx.x & 7u;
+        t_begin = (uint32_t)(((uint64_t)n_tiles * xcd) / 8u) + (blockIdx.x >> 3);
+        t_end = (uint32_t)(((uint64_t)n_tiles * (xcd + 1u)) / 8u);
+        t_step = gridDim.x >> 3;
+    }
+    if (t_begin >= t_end) return;
     load_net(v, sw, sb);
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, c = lane & 15, g = lane >> 4;
     float* st = stage + wv * 16 * kStagePitch;
-    const uint32_t n_tiles = (n + 127u) / 128u;
-    for (uint32_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
+    for (uint32_t tile = t_begin; tile < t_end; tile += t_step) {
         const uint32_t base = tile * 128u + (uint32_t)wv * 32u;
         u16x8 x[3][2];
         bool valid_nb[2], mine_nb[2];
 #pragma unroll
         for (int nb = 0; nb < 2; ++nb) {
-            uint32_t i = base + nb * 16 + c;
-            bool valid = i < n, mine = true;
+            const uint32_t k = base + nb * 16 + c;
+            bool valid = k < n, mine = true;
+            const uint32_t i = (valid && order) ? order[k] : k;
             F3 xp{0.f, 0.f, 0.f}, sn{0.f, 0.f, 1.f}, wo{0.f, 0.f, 1.f};
             const DevScene* sp = &s0;
             if (MULTI && valid) {
@@ -718,8 +730,9 @@ __global__ __launch_bounds__(kMlpBlock) void k_mlp_infer(DevScene s0, PathState 
             // staging pad (1 guided, 0 unguided point, 2 another scene's slot)
             if (g == 0) st[c * kStagePitch + 48] = !mine_nb[nb] ? 2.f : valid_nb[nb] ? 1.f : 0.f;
             __syncthreads();
-            const uint32_t i = base + (uint32_t)nb * 16u + (uint32_t)lane;
-            if (lane < 16 && i < n) {
+            const uint32_t k = base + (uint32_t)nb * 16u + (uint32_t)lane;
+            if (lane < 16 && k < n) {
+                const uint32_t i = order ? order[k] : k;
                 float4* mx = mix + (size_t)i * kMixF4;
                 const float flag = st[lane * kStagePitch + 48];
                 if (flag == 1.f) head_mixture(st + lane * kStagePitch, mx);
@@ -1005,11 +1018,13 @@ hipError_t launch_mlp_eval(const MlpView& v, const float4* x, const float4* nrm,
 }
 
 hipError_t launch_mlp_infer(const DevScene& s, const PathState& ps, const MlpView& v, int parity, float4* mix,
-                            uint32_t grid, hipStream_t stream, bool multi, uint32_t scene) {
+                            uint32_t grid, hipStream_t stream, bool multi, uint32_t scene, const uint32_t* order) {
     if (multi)
-        hipLaunchKernelGGL(k_mlp_infer<true>, dim3(grid), dim3(kMlpBlock), 0, stream, s, ps, v, parity, mix, scene);
+        hipLaunchKernelGGL(k_mlp_infer<true>, dim3(grid), dim3(kMlpBlock), 0, stream, s, ps, v, parity, mix, scene,
+                           order);
     else
-        hipLaunchKernelGGL(k_mlp_infer<false>, dim3(grid), dim3(kMlpBlock), 0, stream, s, ps, v, parity, mix, 0u);
+        hipLaunchKernelGGL(k_mlp_infer<false>, dim3(grid), dim3(kMlpBlock), 0, stream, s, ps, v, parity, mix, 0u,
+                           order);
     return hipGetLastError();
 }
 
