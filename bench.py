@@ -255,7 +255,7 @@ def time_to_relmse(pg, dev, sc, data, max_depth, ref_spp, cap_s, log, target=0.0
                    "quadtree_over_learned": ratio("quadtree", "learned", "final_trimmed_relmse")}}
         # Monte Carlo relMSE falls as 1/time: the last point of each run (reference noise
         # removed) projects its time to the target; the ratios of those times against
-        # the north-star factors (4x unguided, 2x quadtree) give the shortfall
+        # the target factors (4x unguided, 2x quadtree) give the shortfall
         def projected(name):
             c = res[name]["curve"][-1]
             return round(c[4] * c[1] / target, 1) if c[4] > 0 else None
@@ -635,7 +635,7 @@ def main() -> int:
                                  "sample": f"{a.cpu_spp_1t} SPP of the same frame ({c1['paths']} paths, "
                                            f"{rays_1t} rays, {c1['seconds']:.1f} s)"},
                **host,
-               "threads_note": "16 threads: one GPU's share of the GPU box's host cores"}
+               "threads_note": "16 threads: one GPU's share of the host's cores"}
 
     batch = None
     if a.batch_views > 0:
