@@ -146,7 +146,7 @@ def time_staying_below(curve, e, col=2):
 
 
 def time_to_relmse(pg, dev, sc, data, max_depth, ref_spp, cap_s, log, target=0.01, env_visibility=1, repeats=1,
-                   with_pss=False, train_every=1, train_steps=1, record_samples=1):
+                   with_pss=False, train_every=1, train_steps=1, record_samples=1, guide_depth=0):
     """Time-to-relMSE study: unguided vs quadtree guide vs learned guide. Besides the
     time to relMSE <= target (null when a method does not reach it within cap_s), the
     learned guide is compared with each baseline at that baseline's own final relMSE:
@@ -156,6 +156,7 @@ def time_to_relmse(pg, dev, sc, data, max_depth, ref_spp, cap_s, log, target=0.0
     the comparisons of each run and their medians are reported (curves of run 0)."""
     t0 = time.perf_counter()
     ev = dict(env_visibility=env_visibility)
+    gd = dict(guide_depth=guide_depth)  # the guided methods guide the first bounces only
     ref, halves = reference_image(pg, sc, data, ref_spp, max_depth, env_visibility)
     floor = relmse_floor(ref, halves)
     log(f"ttr reference: {ref_spp} SPP in {time.perf_counter() - t0:.1f}s, mean {ref.mean():.5f}, "
@@ -169,10 +170,11 @@ def time_to_relmse(pg, dev, sc, data, max_depth, ref_spp, cap_s, log, target=0.0
 
     def run_once(r):
         un = pg.RayTracer(sc, seed=2 + 10 * r, max_depth=max_depth, **ev)
-        qt = pg.RayTracer(sc, seed=3 + 10 * r, max_depth=max_depth, integrator=2, guide_prob=0.5, **ev)
-        ps = pg.RayTracer(sc, seed=4 + 10 * r, max_depth=max_depth, integrator=4, guide_prob=0.5, **ev)
-        pf = pg.RayTracer(sc, seed=8 + 10 * r, max_depth=max_depth, integrator=4, guide_prob=0.5, knn_k=0, **ev)
-        ml = pg.RayTracer(sc, seed=5 + 10 * r, max_depth=max_depth, record_nee=1, **ev)
+        qt = pg.RayTracer(sc, seed=3 + 10 * r, max_depth=max_depth, integrator=2, guide_prob=0.5, **ev, **gd)
+        ps = pg.RayTracer(sc, seed=4 + 10 * r, max_depth=max_depth, integrator=4, guide_prob=0.5, **ev, **gd)
+        pf = pg.RayTracer(sc, seed=8 + 10 * r, max_depth=max_depth, integrator=4, guide_prob=0.5, knn_k=0, **ev,
+                          **gd)
+        ml = pg.RayTracer(sc, seed=5 + 10 * r, max_depth=max_depth, record_nee=1, **ev, **gd)
 
         def cache_setup(rt, seed):
             # pass 1: 1 SPP record pass (Li with NEE), then radix sort + point BVH + quadtrees
@@ -338,6 +340,8 @@ def time_to_relmse(pg, dev, sc, data, max_depth, ref_spp, cap_s, log, target=0.0
                        "learned": f"online: a {record_samples}-SPP guided record pass and {train_steps} Adam step(s) "
                                   f"of 2^18 samples every {train_every} SPP, guided passes between"}.items()
                        if k in res},
+           "guide_depth": (f"the guided methods guide the first {guide_depth} bounces and sample the BSDF alone "
+                           "deeper" if guide_depth else "the guided methods guide every bounce"),
            **res,
            "learned_diverged": bool(info["diverged"]),
            "learned_network": mlp,
@@ -511,6 +515,9 @@ def main() -> int:
                     help="learned guide: Adam steps after each record pass")
     ap.add_argument("--ttr-record-samples", type=int, default=int(os.environ.get("PG_BENCH_TTR_RECORD_SAMPLES", "2")),
                     help="learned guide: samples per pixel of each record pass (traced as one wavefront pass)")
+    ap.add_argument("--ttr-guide-depth", type=int, default=int(os.environ.get("PG_BENCH_TTR_GUIDE_DEPTH", "2")),
+                    help="guided methods guide the bounces below this depth and sample the BSDF alone deeper "
+                         "(0: every bounce)")
     ap.add_argument("--ttr-pss", action="store_true",
                     help="add the primary-sample-space guide (exact and approximate neighbours) to the study")
     ap.add_argument("--shape", default="",
@@ -651,12 +658,14 @@ def main() -> int:
         ttr = {}
         for tc in (int(v) for v in str(a.ttr_config).split(",") if v.strip()):
             tdata = data if tc == a.config else pg.generate_scene(tc, 0)
-            tsc = sc if tc == a.config else pg.Scene(dev, tdata)
+            # a fresh upload: the batch study above leaves the device holding its last scene
+            tsc = pg.Scene(dev, tdata)
             print(f"ttr: config {tc}", file=sys.stderr, flush=True)
             r = time_to_relmse(pg, dev, tsc, tdata, max_depth, a.ttr_ref_spp, a.ttr_cap,
                                lambda m: print(m, file=sys.stderr, flush=True), env_visibility=a.ttr_env_visibility,
                                repeats=a.ttr_repeats, with_pss=a.ttr_pss, train_every=a.ttr_train_every,
-                               train_steps=a.ttr_train_steps, record_samples=a.ttr_record_samples)
+                               train_steps=a.ttr_train_steps, record_samples=a.ttr_record_samples,
+                               guide_depth=a.ttr_guide_depth)
             ttr[f"config{tc}"] = {"workload": f"config {tc}: {tdata.n_tris} tris, {tdata.width}x{tdata.height}, "
                                               f"max depth {max_depth}", **r}
             ps = r["comparison"]["per_sample"]

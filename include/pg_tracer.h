@@ -125,7 +125,9 @@ typedef struct {
                                48 or 64), deeper entries in global memory: fewer
                                take less LDS per block and fit more blocks per CU.
                                0 = automatic (16). Results unchanged.            */
-    uint32_t reserved[1];
+    uint32_t guide_depth;   /* guided integrators: guide the bounces below this
+                               depth and sample the BSDF alone deeper (0 = every
+                               bounce); single-view renders                      */
 } pg_render_params;
 
 typedef struct {

@@ -44,7 +44,7 @@ class PgRenderParams(C.Structure):
                 ("knn_k", C.c_uint32), ("record_nee", C.c_uint32), ("trace_blocks_per_cu", C.c_uint32),
                 ("shade_blocks_per_cu", C.c_uint32), ("sort_rays", C.c_uint32),
                 ("dyn_fetch", C.c_uint32), ("env_visibility", C.c_uint32), ("samples_per_pass", C.c_uint32),
-                ("stack_lds", C.c_uint32), ("reserved", C.c_uint32 * 1)]
+                ("stack_lds", C.c_uint32), ("guide_depth", C.c_uint32)]
 
 
 class PgStats(C.Structure):

@@ -969,17 +969,21 @@ pg_status pg_render_spp(pg_ctx* c, const pg_render_params* params, uint32_t n_sp
         PG_TRY(c, timed(3, [&] { return launch_raygen(c->dev, c->ps, pp, c->stream); }), "raygen");
         for (uint32_t depth = 0; depth <= prm.max_depth; ++depth) {
             int parity = (int)(depth & 1u);
+            // guided bounces; deeper ones sample the BSDF alone (the MIS at every vertex
+            // uses that vertex's own density, so mixing the two keeps the estimate unbiased)
+            const bool guide_here = prm.guide_depth == 0 || depth < prm.guide_depth;
+            const int mode_here = guide_here ? mode : 0;
             const bool sort_here = sort_rays && depth > 0;
             if (sort_here) PG_TRY(c, timed(3, [&] { return ray_order(parity, 0); }), "ray order");
             PG_TRY(c, timed(0, [&] { return launch_extend(c->dev, c->ps, parity, stack, c->counting, grid_trace, c->stream, false, sort_here ? c->q_vals : nullptr, dyn); }), "extend");
             c->n_extend++;
             if (record_pass) {
-                if (mode == 2) {
+                if (mode_here == 2) {
                     const uint32_t* ord = nullptr;
                     PG_TRY(c, timed(4, [&] { return infer_order(parity, depth, ord); }), "query order");
                     PG_TRY(c, timed(4, [&] { return launch_mlp_infer(c->dev, c->ps, c->mlp.view, parity, c->mix, grid_mlp, c->stream, false, 0u, ord); }), "mlp_infer");
                 }
-                const float4* mx = mode == 2 ? c->mix : nullptr;
+                const float4* mx = mode_here == 2 ? c->mix : nullptr;
                 PG_TRY(c, timed(1, [&] { return launch_shade_record(c->dev, c->ps, pp, c->rs, mx, parity, (int)depth, grid_shade, c->stream); }), "shade_record");
                 if (c->rs.li_nee) {
                     PG_TRY(c, timed(2, [&] { return launch_shadow_record(c->dev, c->ps, c->rs, stack, grid_trace, c->stream); }), "shadow_record");
@@ -987,10 +991,10 @@ pg_status pg_render_spp(pg_ctx* c, const pg_render_params* params, uint32_t n_sp
                 }
                 continue;
             }
-            if (mode == 3 && prm.knn_k == 0) {
+            if (mode_here == 3 && prm.knn_k == 0) {
                 // approximate neighbours: no query sort needed (no search stack)
                 PG_TRY(c, timed(4, [&] { return launch_guide_cells(c->dev, c->ps, c->gv, c->guide_cells, parity, grid_guide, c->stream, nullptr, 0, 2); }), "guide records");
-            } else if (mode == 3) {
+            } else if (mode_here == 3) {
                 PG_TRY(c, timed(4, [&] {
                     hipError_t e = launch_query_keys(c->dev, c->ps, q_lo, q_inv, parity, np, c->q_keys, c->q_vals, c->stream);
                     if (e == hipSuccess)
@@ -1000,7 +1004,7 @@ pg_status pg_render_spp(pg_ctx* c, const pg_render_params* params, uint32_t n_sp
                 }), "guide query order");
                 PG_TRY(c, timed(4, [&] { return launch_guide_cells(c->dev, c->ps, c->gv, c->guide_cells, parity, grid_guide, c->stream, c->q_vals, np, 1); }), "guide records");
             }
-            if (mode == 1) {
+            if (mode_here == 1) {
                 PG_TRY(c, timed(4, [&] {
                     hipError_t e = launch_query_keys(c->dev, c->ps, q_lo, q_inv, parity, np, c->q_keys, c->q_vals, c->stream);
                     if (e == hipSuccess)
@@ -1010,12 +1014,12 @@ pg_status pg_render_spp(pg_ctx* c, const pg_render_params* params, uint32_t n_sp
                 }), "guide query order");
                 PG_TRY(c, timed(4, [&] { return launch_guide_cells(c->dev, c->ps, c->gv, c->guide_cells, parity, grid_guide, c->stream, c->q_vals, np, 0); }), "guide_cells");
             }
-            if (mode == 2) {
+            if (mode_here == 2) {
                 const uint32_t* ord = nullptr;
                 PG_TRY(c, timed(4, [&] { return infer_order(parity, depth, ord); }), "query order");
                 PG_TRY(c, timed(4, [&] { return launch_mlp_infer(c->dev, c->ps, c->mlp.view, parity, c->mix, grid_mlp, c->stream, false, 0u, ord); }), "mlp_infer");
             }
-            PG_TRY(c, timed(1, [&] { return launch_shade(c->dev, c->ps, pp, gc, mode, parity, (int)depth, grid_shade, c->stream); }), "shade");
+            PG_TRY(c, timed(1, [&] { return launch_shade(c->dev, c->ps, pp, gc, mode_here, parity, (int)depth, grid_shade, c->stream); }), "shade");
             if (sort_rays) PG_TRY(c, timed(3, [&] { return ray_order(parity, 1); }), "shadow order");
             PG_TRY(c, timed(2, [&] { return launch_shadow(c->dev, c->ps, stack, c->counting, grid_trace, c->stream, false, sort_rays ? c->q_vals : nullptr, dyn); }), "shadow");
             c->n_shadow++;

@@ -202,6 +202,7 @@ class Device:
         self.lib = load()
         self.ctx = C.c_void_p()
         check(self.lib.pg_create(C.byref(self.ctx), int(device)))
+        self.current_scene = None  # the Scene whose upload the context holds
         _open_devices.add(self)
 
     def close(self):
@@ -376,12 +377,15 @@ class Device:
 
 class Scene:
     """Scene::init + Scene::build (Scene.h:84-137) on the device; traverse / visible as
-    batched queries (Scene.h:103-146)."""
+    batched queries (Scene.h:103-146). A device context holds one uploaded scene: once
+    another Scene is created on the same Device, this one and its RayTracers raise
+    instead of rendering the other scene."""
 
     def __init__(self, device: Device, data: SceneData):
         self.device = device
         self.data = data
         lib = device.lib
+        device.current_scene = None
         pos = np.ascontiguousarray(data.positions, np.float32)
         nrm = np.ascontiguousarray(data.normals, np.float32)
         mi = np.ascontiguousarray(data.material_index, np.uint32)
@@ -401,13 +405,22 @@ class Scene:
                                               mt.shape[0], _fp(texels), off.ctypes.data_as(C.POINTER(C.c_uint64)),
                                               tw.ctypes.data_as(C.POINTER(C.c_int32)),
                                               th.ctypes.data_as(C.POINTER(C.c_int32)), len(data.textures)))
+        device.current_scene = self
+
+    def check_current(self):
+        """Raise unless the device context holds this scene's upload."""
+        if self.device.current_scene is not self:
+            raise RuntimeError("another Scene was uploaded to this Device since: the context renders and "
+                               "traces that one (create this Scene again to use it)")
 
     def bvh_info(self) -> dict:
+        self.check_current()
         n, d, ms = C.c_uint32(), C.c_uint32(), C.c_double()
         self.device._chk(self.device.lib.pg_bvh_info(self.device.ctx, C.byref(n), C.byref(d), C.byref(ms)))
         return {"nodes": n.value, "max_depth": d.value, "build_ms": ms.value}
 
     def bvh_export(self):
+        self.check_current()
         info = self.bvh_info()
         nodes = np.empty((info["nodes"], 8), np.float32)
         idx = np.empty(self.data.n_tris, np.int32)
@@ -417,6 +430,7 @@ class Scene:
 
     def traverse(self, rays: np.ndarray):
         """Closest hits for rays (n, 6) = origin, direction. Returns id, t, alpha, beta, gamma."""
+        self.check_current()
         rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 6)
         n = rays.shape[0]
         hits = np.empty(n, np.dtype([("id", "u4"), ("t", "f4"), ("alpha", "f4"), ("beta", "f4"), ("gamma", "f4")]))
@@ -425,6 +439,7 @@ class Scene:
         return hits["id"].copy(), hits["t"].copy(), hits["alpha"].copy(), hits["beta"].copy(), hits["gamma"].copy()
 
     def visible_rays(self, rays: np.ndarray, max_t: np.ndarray) -> np.ndarray:
+        self.check_current()
         rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 6)
         max_t = np.ascontiguousarray(max_t, np.float32)
         out = np.empty(rays.shape[0], np.uint8)
@@ -433,6 +448,7 @@ class Scene:
         return out
 
     def generate_rays(self, pxy: np.ndarray) -> np.ndarray:
+        self.check_current()
         pxy = np.ascontiguousarray(pxy, np.float32).reshape(-1, 2)
         out = np.empty((pxy.shape[0], 6), np.float32)
         self.device._chk(self.device.lib.pg_generate_rays(self.device.ctx, _fp(pxy), pxy.shape[0], _fp(out)))
@@ -450,14 +466,17 @@ class RayTracer:
             setattr(self.params, k, v)
 
     def clear(self):
+        self.scene.check_current()
         self.device._chk(self.device.lib.pg_film_clear(self.device.ctx))
 
     def render(self, n_spp: int = 1):
+        self.scene.check_current()
         self.device._chk(self.device.lib.pg_render_spp(self.device.ctx, C.byref(self.params), int(n_spp)))
 
     def render_online(self, n_spp: int, steps: int = 1, window: int = 1, train_every: int = 1,
                       warmup_passes: int = 0, warmup_steps: int = 0, record_samples: int = 1):
         """Learned-guide passes with online training (pg_mlp_render_online)."""
+        self.scene.check_current()
         sch = _lib.PgMlpSchedule(train_every, steps, window, warmup_passes, warmup_steps, record_samples)
         self.device._chk(self.device.lib.pg_mlp_render_online(self.device.ctx, C.byref(self.params), int(n_spp),
                                                               C.byref(sch)))
@@ -465,6 +484,7 @@ class RayTracer:
     def tune(self) -> float:
         """Pick the fastest launch shape for these parameters (pg_render_tune); clears
         the film. Returns the tuned milliseconds per SPP."""
+        self.scene.check_current()
         tuned = PgRenderParams()
         ms = C.c_double()
         self.device._chk(self.device.lib.pg_render_tune(self.device.ctx, C.byref(self.params), C.byref(tuned),
@@ -478,6 +498,7 @@ class RayTracer:
 
     def film(self):
         """(H, W, 3) float32 sum over samples and the SPP count."""
+        self.scene.check_current()
         W, H = self.scene.data.width, self.scene.data.height
         buf = np.empty((H, W, 3), np.float32)
         spp = C.c_uint32()

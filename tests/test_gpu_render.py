@@ -258,3 +258,23 @@ def test_automatic_samples_per_pass_takes_the_whole_call(pg, device, small_inter
         assert n == 70
         films.append(film)
     assert np.array_equal(films[0], films[1]) and np.array_equal(films[0], films[2])
+
+
+def test_stale_scene_raises_instead_of_rendering_another(pg, device, cornell):
+    """A device context holds one uploaded scene: a Scene (and its RayTracers) created
+    before another Scene on the same Device raise instead of silently rendering or
+    tracing the other scene; creating it again makes it current."""
+    a = pg.Scene(device, cornell)
+    rt = pg.RayTracer(a, seed=1)
+    rt.clear()
+    rt.render(1)
+    pg.Scene(device, pg.generate_scene(100, 0))
+    with pytest.raises(RuntimeError):
+        rt.render(1)
+    with pytest.raises(RuntimeError):
+        a.traverse(np.zeros((1, 6), np.float32))
+    b = pg.Scene(device, cornell)
+    rt2 = pg.RayTracer(b, seed=1)
+    rt2.clear()
+    rt2.render(1)
+    assert rt2.getSPP() == 1
