@@ -370,7 +370,7 @@ def time_to_relmse(pg, dev, sc, data, max_depth, ref_spp, cap_s, log, target=0.0
 
 
 def batch_study(pg, dev, n_views: int, spp: int, max_depth: int, log, shape: dict | None = None,
-                guided: bool = True) -> dict:
+                guided: bool = True, guide_depth: int = 0) -> dict:
     """Batched views of the config-5 scene family (seed i for view i, 100k-1M triangles),
     resolutions drawn with a seeded RNG from {128x128, 256x512, 640x360, 1280x720}: one
     hipGraph launch per SPP after the autotuner has picked the launch configuration
@@ -423,7 +423,7 @@ def batch_study(pg, dev, n_views: int, spp: int, max_depth: int, log, shape: dic
     b.configure(**best)
     guided_res = None
     if guided:
-        guided_res = _batch_guided(pg, dev, b, n_views, spp, max_depth, pixels, log)
+        guided_res = _batch_guided(pg, dev, b, n_views, spp, max_depth, pixels, log, guide_depth)
     b.close()
     return {"views": n_views, "scenes_tris": tris, "pixels": pixels, "spp": spp, "max_depth": max_depth,
             "resolutions": "drawn from {128x128, 256x512, 640x360, 1280x720} with a seeded RNG",
@@ -436,9 +436,10 @@ def batch_study(pg, dev, n_views: int, spp: int, max_depth: int, log, shape: dic
             "setup_s": round(setup, 1), "tune_s": round(tune, 1)}
 
 
-def _batch_guided(pg, dev, b, n_views: int, spp: int, max_depth: int, pixels: int, log) -> dict:
+def _batch_guided(pg, dev, b, n_views: int, spp: int, max_depth: int, pixels: int, log, guide_depth: int = 0) -> dict:
     """The batch's views guided by per-scene learned networks (each trained online on its
-    scene's own camera first: 16 guided record passes with one Adam step each)."""
+    scene's own camera first: 16 guided record passes with one Adam step each), guiding
+    every bounce and, with guide_depth > 0, the first guide_depth bounces."""
     t0 = time.perf_counter()
     for i in range(n_views):
         d = pg.generate_scene(5, i)
@@ -448,17 +449,27 @@ def _batch_guided(pg, dev, b, n_views: int, spp: int, max_depth: int, pixels: in
         p_i, g_i = dev.mlp_params()
         b.set_guide(i, p_i, g_i, seed=7 + i)
     train = time.perf_counter() - t0
-    b.clear()
-    g0 = b.stats()
-    t0 = time.perf_counter()
-    b.render(spp, max_depth=max_depth, integrator=3)
-    gwall = time.perf_counter() - t0
-    g1 = b.stats()
-    grays = (g1["closest_rays"] - g0["closest_rays"]) + (g1["shadow_rays"] - g0["shadow_rays"])
-    log(f"batch guided: {gwall * 1e3 / spp:.2f} ms/SPP after {train:.1f}s of per-scene training")
-    return {"ms_per_spp": round(gwall * 1e3 / spp, 3), "samples_per_s": round(pixels * spp / gwall, 1),
-            "mrays_per_s": round(grays / gwall / 1e6, 2), "train_s": round(train, 1),
-            "guides": "one network per scene, trained online on the scene's own camera (16 SPP)"}
+    out = {"train_s": round(train, 1),
+           "guides": "one network per scene, trained online on the scene's own camera (16 SPP)"}
+    for depth in (0, guide_depth) if guide_depth else (0,):
+        # an untimed render first (it captures the pass graph when the tuner chose one)
+        b.clear()
+        b.render(spp, max_depth=max_depth, integrator=3, guide_depth=depth)
+        b.clear()
+        g0 = b.stats()
+        t0 = time.perf_counter()
+        b.render(spp, max_depth=max_depth, integrator=3, guide_depth=depth)
+        gwall = time.perf_counter() - t0
+        g1 = b.stats()
+        grays = (g1["closest_rays"] - g0["closest_rays"]) + (g1["shadow_rays"] - g0["shadow_rays"])
+        r = {"ms_per_spp": round(gwall * 1e3 / spp, 3), "samples_per_s": round(pixels * spp / gwall, 1),
+             "mrays_per_s": round(grays / gwall / 1e6, 2)}
+        log(f"batch guided (guide_depth {depth}): {r['ms_per_spp']:.2f} ms/SPP after {train:.1f}s of per-scene training")
+        if depth == 0:
+            out.update(r)
+        else:
+            out[f"guide_depth_{depth}"] = r
+    return out
 
 
 def host_cpu() -> dict:
@@ -517,7 +528,7 @@ def main() -> int:
                     help="learned guide: samples per pixel of each record pass (traced as one wavefront pass)")
     ap.add_argument("--ttr-guide-depth", type=int, default=int(os.environ.get("PG_BENCH_TTR_GUIDE_DEPTH", "2")),
                     help="guided methods guide the bounces below this depth and sample the BSDF alone deeper "
-                         "(0: every bounce)")
+                         "(0: every bounce); the learned-guided batch is timed at 0 and at this depth")
     ap.add_argument("--ttr-pss", action="store_true",
                     help="add the primary-sample-space guide (exact and approximate neighbours) to the study")
     ap.add_argument("--shape", default="",
@@ -651,7 +662,7 @@ def main() -> int:
             bshape = {k.strip(): int(v) for k, v in (kv.split("=") for kv in a.batch_shape.split(","))}
         batch = batch_study(pg, dev, a.batch_views, a.batch_spp, max_depth,
                             lambda m: print(m, file=sys.stderr, flush=True), shape=bshape,
-                            guided=not a.batch_no_guided)
+                            guided=not a.batch_no_guided, guide_depth=a.ttr_guide_depth)
 
     ttr = None
     if not a.no_ttr:

@@ -127,7 +127,7 @@ typedef struct {
                                0 = automatic (16). Results unchanged.            */
     uint32_t guide_depth;   /* guided integrators: guide the bounces below this
                                depth and sample the BSDF alone deeper (0 = every
-                               bounce); single-view renders                      */
+                               bounce); single-view and batch renders            */
 } pg_render_params;
 
 typedef struct {

@@ -297,7 +297,9 @@ hipError_t enqueue_pass(pg_batch* b, const pg_render_params& prm, uint32_t ns) {
             if ((e = launch_extend(s0, b->ps, parity, stack, false, tgrid, st, true, nullptr, b->cfg.dyn_fetch != 0)) !=
                 hipSuccess)
                 return e;
-            if (learned) {
+            // guide_depth d: the network guides the bounces below depth d only
+            const bool guide_here = learned && (prm.guide_depth == 0 || depth < prm.guide_depth);
+            if (guide_here) {
                 // every slot unguided, then one launch runs each scene's network over
                 // that scene's slots (grouped by scene)
                 if ((e = launch_mix_clear(b->ps, parity, b->mix, (uint32_t)cu * 4, st)) != hipSuccess) return e;
@@ -305,7 +307,7 @@ hipError_t enqueue_pass(pg_batch* b, const pg_render_params& prm, uint32_t ns) {
                                                   b->mix, (uint32_t)cu * 3, st)) != hipSuccess)
                     return e;
             }
-            if ((e = launch_shade(s0, b->ps, pp, gc, learned ? 2 : 0, parity, (int)depth, sgrid, st, true)) !=
+            if ((e = launch_shade(s0, b->ps, pp, gc, guide_here ? 2 : 0, parity, (int)depth, sgrid, st, true)) !=
                 hipSuccess)
                 return e;
             if ((e = launch_shadow(s0, b->ps, stack, false, tgrid, st, true, nullptr, b->cfg.dyn_fetch != 0)) !=
@@ -318,8 +320,10 @@ hipError_t enqueue_pass(pg_batch* b, const pg_render_params& prm, uint32_t ns) {
 }
 
 bool same_params(const pg_render_params& a, const pg_render_params& b) {
+    // every field enqueue_pass bakes into the captured launches
     return a.seed == b.seed && a.max_depth == b.max_depth && a.rr_depth == b.rr_depth &&
-           a.integrator == b.integrator && a.env_visibility == b.env_visibility;
+           a.integrator == b.integrator && a.env_visibility == b.env_visibility && a.guide_prob == b.guide_prob &&
+           a.guide_depth == b.guide_depth;
 }
 
 // Parameters a batch renders with: unguided, or the learned guides of its scenes.

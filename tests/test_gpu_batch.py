@@ -169,11 +169,11 @@ def _trained_guide(pg, device, data, seed):
     return device.mlp_params()
 
 
-def _single_guided(pg, device, data, cam, guide, seed, spp):
+def _single_guided(pg, device, data, cam, guide, seed, spp, **params):
     sc = pg.Scene(device, dataclasses.replace(data, camera=cam))
     device.mlp_init(**dict(GUIDE_CFG, seed=guide[2]))
     device.mlp_set_params(guide[0], guide[1])
-    rt = pg.RayTracer(sc, seed=seed, max_depth=12, integrator=3)
+    rt = pg.RayTracer(sc, seed=seed, max_depth=12, integrator=3, **params)
     rt.clear()
     rt.render(spp)
     return rt.film()[0]
@@ -208,6 +208,30 @@ def test_batch_learned_guides_equal_single_renders(pg, device, cornell, small_in
     b.close()
     assert np.array_equal(f0, _single(pg, device, cornell, views[0][1], 2, 7))
     assert np.array_equal(f1, _single_guided(pg, device, small_interior, views[1][1], guides[1], 7, 2))
+
+
+def test_batch_guide_depth_and_cap_follow_single_renders(pg, device, cornell, small_interior, views):
+    """`guide_depth` and `guide_prob` reach the batch's guided bounces, and a render with
+    other values captures its graph afresh: on one batch with graph replay, renders at
+    (depth 0, cap 0.5), (depth 2, cap 0.5) and (depth 2, cap 0.25) each equal the single
+    learned-guide renders with the same values, bit for bit."""
+    guides = [(*_trained_guide(pg, device, d, s), s) for d, s in ((cornell, 11), (small_interior, 12))]
+    scenes = [cornell, small_interior]
+    b = _batch(pg, device, cornell, small_interior, views, use_graph=1)
+    for sid in (0, 1):
+        b.set_guide(sid, guides[sid][0], guides[sid][1], **dict(GUIDE_CFG, seed=guides[sid][2]))
+    runs = []
+    for depth, cap in ((0, 0.5), (2, 0.5), (2, 0.25)):
+        b.clear()
+        b.render(2, seed=7, max_depth=12, integrator=3, guide_depth=depth, guide_prob=cap)
+        assert b.stats()["graph_nodes"] > 0
+        runs.append(((depth, cap), [b.film(v)[0] for v in range(len(views))]))
+    b.close()
+    for (depth, cap), films in runs:
+        for v, (sid, cam) in enumerate(views):
+            ref = _single_guided(pg, device, scenes[sid], cam, guides[sid], 7, 2, guide_depth=depth, guide_prob=cap)
+            assert np.array_equal(films[v], ref), f"view {v} (guide_depth {depth}, guide_prob {cap}) differs"
+    assert not np.array_equal(runs[0][1][1], runs[1][1][1])  # the depth limit changes the interior view
 
 
 def test_textured_scene_in_batch_equals_single_render(pg, device, cornell, tmp_path):
