@@ -768,20 +768,38 @@ __device__ __forceinline__ uint32_t wave_key_append(uint32_t* counter, uint32_t 
     return mine;
 }
 
-__global__ void k_scene_count(PathState ps, int parity, const MlpView* guides, uint32_t* cnt) {
+// The scene of queue slot i, and whether that scene has a network.
+__device__ __forceinline__ bool slot_scene(const PathState& ps, int parity, const MlpView* guides, uint32_t i,
+                                           uint32_t n, uint32_t& sid) {
+    sid = 0;
+    if (i >= n) return false;
+    sid = ps.bviews[ps.path_view[ps.queue[parity][i]]].scene;
+    return guides[sid].wmirror != nullptr;
+}
+
+// Scenes below kLdsScenes are counted per block in LDS and reach the global counters
+// with one atomic per block and scene: a chunk's queue is mostly one scene, and one
+// global atomic per wave on that one counter serialised the whole launch (3.9 ms for
+// 22M slots). Scenes from kLdsScenes up take the per-wave global path.
+constexpr uint32_t kLdsScenes = 256;
+
+__global__ __launch_bounds__(256) void k_scene_count(PathState ps, int parity, const MlpView* guides,
+                                                     uint32_t n_scenes, uint32_t* cnt) {
+    __shared__ uint32_t lcnt[kLdsScenes];
+    for (uint32_t t = threadIdx.x; t < kLdsScenes; t += blockDim.x) lcnt[t] = 0u;
+    __syncthreads();
     const uint32_t n = ps.counters[parity];
     const uint32_t stride = gridDim.x * blockDim.x;
     // whole waves iterate together (wave_key_append needs every lane)
     for (uint32_t i0 = blockIdx.x * blockDim.x; i0 < n; i0 += stride) {
-        const uint32_t i = i0 + threadIdx.x;
-        uint32_t sid = 0;
-        bool on = false;
-        if (i < n) {
-            sid = ps.bviews[ps.path_view[ps.queue[parity][i]]].scene;
-            on = guides[sid].wmirror != nullptr;
-        }
-        (void)wave_key_append(cnt, sid, on);
+        uint32_t sid;
+        const bool on = slot_scene(ps, parity, guides, i0 + threadIdx.x, n, sid);
+        (void)wave_key_append(lcnt, sid, on && sid < kLdsScenes);
+        (void)wave_key_append(cnt, sid, on && sid >= kLdsScenes);
     }
+    __syncthreads();
+    for (uint32_t t = threadIdx.x; t < kLdsScenes && t < n_scenes; t += blockDim.x)
+        if (lcnt[t]) atomicAdd(&cnt[t], lcnt[t]);
 }
 // One thread: padded offsets, scatter cursors, tile count; clears the counts.
 __global__ void k_scene_offsets(uint32_t n_scenes, uint32_t* cnt, uint32_t* off, uint32_t* cur, uint32_t* n_tiles) {
@@ -801,25 +819,44 @@ __global__ void k_perm_clear(const uint32_t* n_tiles, uint32_t* perm) {
     const uint32_t total = *n_tiles * 128u;
     for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < total; j += gridDim.x * blockDim.x) perm[j] = ~0u;
 }
-__global__ void k_scene_scatter(PathState ps, int parity, const MlpView* guides, uint32_t* cur, uint32_t* perm) {
+// Each block owns a contiguous run of whole waves of the queue: it counts its slots
+// per scene in LDS, reserves each scene's entries with one atomic on that scene's
+// cursor, then hands them out from LDS cursors in a second sweep.
+__global__ __launch_bounds__(256) void k_scene_scatter(PathState ps, int parity, const MlpView* guides,
+                                                       uint32_t n_scenes, uint32_t* cur, uint32_t* perm) {
+    __shared__ uint32_t lcur[kLdsScenes];
+    for (uint32_t t = threadIdx.x; t < kLdsScenes; t += blockDim.x) lcur[t] = 0u;
+    __syncthreads();
     const uint32_t n = ps.counters[parity];
-    const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t i0 = blockIdx.x * blockDim.x; i0 < n; i0 += stride) {
+    const uint32_t per = ((n + gridDim.x - 1u) / gridDim.x + 63u) & ~63u;
+    const uint32_t lo = min(blockIdx.x * per, n), hi = min(lo + per, n);
+    for (uint32_t i0 = lo; i0 < hi; i0 += blockDim.x) {
         const uint32_t i = i0 + threadIdx.x;
-        uint32_t sid = 0;
-        bool on = false;
-        if (i < n) {
-            sid = ps.bviews[ps.path_view[ps.queue[parity][i]]].scene;
-            on = guides[sid].wmirror != nullptr;
-        }
-        const uint32_t at = wave_key_append(cur, sid, on);
+        uint32_t sid;
+        const bool on = slot_scene(ps, parity, guides, i, hi, sid);
+        (void)wave_key_append(lcur, sid, on && sid < kLdsScenes);
+        const bool wide = on && sid >= kLdsScenes;
+        const uint32_t at = wave_key_append(cur, sid, wide);
+        if (wide) perm[at] = i;
+    }
+    __syncthreads();
+    for (uint32_t t = threadIdx.x; t < kLdsScenes && t < n_scenes; t += blockDim.x)
+        if (lcur[t]) lcur[t] = atomicAdd(&cur[t], lcur[t]);
+    __syncthreads();
+    for (uint32_t i0 = lo; i0 < hi; i0 += blockDim.x) {
+        const uint32_t i = i0 + threadIdx.x;
+        uint32_t sid;
+        const bool on = slot_scene(ps, parity, guides, i, hi, sid) && sid < kLdsScenes;
+        const uint32_t at = wave_key_append(lcur, sid, on);
         if (on) perm[at] = i;
     }
 }
 
 // Every scene's network over its own slots in one launch (see k_scene_count): blocks
-// take contiguous tile ranges; the slots of scenes without a network stay as the
-// clear left them (unguided).
+// stride over the tiles, so the blocks in flight work on neighbouring tiles (slots of
+// neighbouring pixels, whose hash-grid lines the L2 then shares); a block's tiles only
+// move forward, so it loads each scene's network at most once. The slots of scenes
+// without a network stay as the clear left them (unguided).
 __global__ __launch_bounds__(kMlpBlock) void k_mlp_infer_grouped(PathState ps, int parity, const MlpView* guides,
                                                                  uint32_t n_scenes, const uint32_t* off,
                                                                  const uint32_t* n_tiles_p, const uint32_t* perm,
@@ -830,12 +867,10 @@ __global__ __launch_bounds__(kMlpBlock) void k_mlp_infer_grouped(PathState ps, i
     __shared__ uint32_t sslot[4 * 16];
     __shared__ MlpView sv;
     const uint32_t n_tiles = *n_tiles_p;
-    const uint32_t per = (n_tiles + gridDim.x - 1) / gridDim.x;
-    const uint32_t t0 = blockIdx.x * per, t1 = min(t0 + per, n_tiles);
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, c = lane & 15, g = lane >> 4;
     float* st = stage + wv * 16 * kStagePitch;
     int loaded = -1;
-    for (uint32_t tile = t0; tile < t1; ++tile) {
+    for (uint32_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
         // the tile's scene: the last s with off[s] <= tile * 128
         uint32_t lo = 0, hi = n_scenes;
         while (hi - lo > 1) {
@@ -1030,10 +1065,11 @@ hipError_t launch_mlp_infer(const DevScene& s, const PathState& ps, const MlpVie
 
 hipError_t launch_mlp_infer_grouped(const PathState& ps, int parity, const MlpView* guides, uint32_t n_scenes,
                                     const GroupBuffers& gb, float4* mix, uint32_t grid, hipStream_t stream) {
-    hipLaunchKernelGGL(k_scene_count, dim3(grid), dim3(256), 0, stream, ps, parity, guides, gb.cnt);
+    hipLaunchKernelGGL(k_scene_count, dim3(grid), dim3(256), 0, stream, ps, parity, guides, n_scenes, gb.cnt);
     hipLaunchKernelGGL(k_scene_offsets, dim3(1), dim3(64), 0, stream, n_scenes, gb.cnt, gb.off, gb.cur, gb.n_tiles);
     hipLaunchKernelGGL(k_perm_clear, dim3(grid), dim3(256), 0, stream, gb.n_tiles, gb.perm);
-    hipLaunchKernelGGL(k_scene_scatter, dim3(grid), dim3(256), 0, stream, ps, parity, guides, gb.cur, gb.perm);
+    hipLaunchKernelGGL(k_scene_scatter, dim3(grid), dim3(256), 0, stream, ps, parity, guides, n_scenes, gb.cur,
+                       gb.perm);
     hipLaunchKernelGGL(k_mlp_infer_grouped, dim3(grid), dim3(kMlpBlock), 0, stream, ps, parity, guides, n_scenes,
                        gb.off, gb.n_tiles, gb.perm, mix);
     return hipGetLastError();

@@ -234,6 +234,31 @@ def test_batch_guide_depth_and_cap_follow_single_renders(pg, device, cornell, sm
     assert not np.array_equal(runs[0][1][1], runs[1][1][1])  # the depth limit changes the interior view
 
 
+def test_batch_of_more_scenes_than_the_lds_table_groups_every_scene(pg, device, cornell):
+    """260 copies of the Cornell box in one batch (one 16x16 view each) with networks on
+    scenes 0, 1, 257 and 259 only: the inference grouping counts scenes below 256 per block
+    in LDS and the others with global atomics, in the same launches; every guided view
+    equals the single learned-guide render and every other view the unguided one."""
+    guide = (*_trained_guide(pg, device, cornell, 11), 11)
+    cam = _cam(cornell, 16, 16)
+    b = pg.Batch(device)
+    for i in range(260):
+        assert b.add_scene(cornell) == i
+        b.add_view(i, cam)
+    guided = (0, 1, 257, 259)
+    for sid in guided:
+        b.set_guide(sid, guide[0], guide[1], **dict(GUIDE_CFG, seed=guide[2]))
+    b.clear()
+    b.render(2, seed=7, max_depth=12, integrator=3)
+    films = [b.film(v)[0] for v in range(260)]
+    b.close()
+    ref_g = _single_guided(pg, device, cornell, cam, guide, 7, 2)
+    ref_u = _single(pg, device, cornell, cam, 2, 7)
+    assert not np.array_equal(ref_g, ref_u)
+    for v in range(260):
+        assert np.array_equal(films[v], ref_g if v in guided else ref_u), f"view {v}"
+
+
 def test_textured_scene_in_batch_equals_single_render(pg, device, cornell, tmp_path):
     """A scene read from files (albedo textures, Oren-Nayar / Plastic / Layered) in a
     batch beside a procedural one: its views equal its single-view renders."""
