@@ -300,9 +300,8 @@ hipError_t enqueue_pass(pg_batch* b, const pg_render_params& prm, uint32_t ns) {
             // guide_depth d: the network guides the bounces below depth d only
             const bool guide_here = learned && (prm.guide_depth == 0 || depth < prm.guide_depth);
             if (guide_here) {
-                // every slot unguided, then one launch runs each scene's network over
-                // that scene's slots (grouped by scene)
-                if ((e = launch_mix_clear(b->ps, parity, b->mix, (uint32_t)cu * 4, st)) != hipSuccess) return e;
+                // one launch runs each scene's network over that scene's slots (grouped
+                // by scene); the slots of scenes without a network become unguided
                 if ((e = launch_mlp_infer_grouped(b->ps, parity, b->d_guides, (uint32_t)b->scenes.size(), b->grp,
                                                   b->mix, (uint32_t)cu * 3, st)) != hipSuccess)
                     return e;

@@ -156,7 +156,6 @@ struct MlpView;
 hipError_t launch_mlp_infer(const DevScene& s, const PathState& ps, const MlpView& v, int parity, float4* mix,
                             uint32_t grid, hipStream_t stream, bool multi = false, uint32_t scene = 0,
                             const uint32_t* order = nullptr);
-hipError_t launch_mix_clear(const PathState& ps, int parity, float4* mix, uint32_t grid, hipStream_t stream);
 // Batches: the queue slots grouped by scene (cnt, off, cur: n_scenes + 1 entries each;
 // perm: queue capacity + 128 * n_scenes; n_tiles: 1) so that one launch runs every
 // scene's network (guides[s].wmirror == nullptr: scene s is unguided).

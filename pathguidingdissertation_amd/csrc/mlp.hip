@@ -821,9 +821,12 @@ __global__ void k_perm_clear(const uint32_t* n_tiles, uint32_t* perm) {
 }
 // Each block owns a contiguous run of whole waves of the queue: it counts its slots
 // per scene in LDS, reserves each scene's entries with one atomic on that scene's
-// cursor, then hands them out from LDS cursors in a second sweep.
+// cursor, then hands them out from LDS cursors in a second sweep. Slots of scenes
+// without a network get c = 0 (unguided) here; k_mlp_infer_grouped writes every other
+// slot of the queue.
 __global__ __launch_bounds__(256) void k_scene_scatter(PathState ps, int parity, const MlpView* guides,
-                                                       uint32_t n_scenes, uint32_t* cur, uint32_t* perm) {
+                                                       uint32_t n_scenes, uint32_t* cur, uint32_t* perm,
+                                                       float4* mix) {
     __shared__ uint32_t lcur[kLdsScenes];
     for (uint32_t t = threadIdx.x; t < kLdsScenes; t += blockDim.x) lcur[t] = 0u;
     __syncthreads();
@@ -834,6 +837,7 @@ __global__ __launch_bounds__(256) void k_scene_scatter(PathState ps, int parity,
         const uint32_t i = i0 + threadIdx.x;
         uint32_t sid;
         const bool on = slot_scene(ps, parity, guides, i, hi, sid);
+        if (!on && i < hi) mix[(size_t)i * kMixF4 + 12] = make_float4(0.f, 0.f, 0.f, 0.f);
         (void)wave_key_append(lcur, sid, on && sid < kLdsScenes);
         const bool wide = on && sid >= kLdsScenes;
         const uint32_t at = wave_key_append(cur, sid, wide);
@@ -856,7 +860,7 @@ __global__ __launch_bounds__(256) void k_scene_scatter(PathState ps, int parity,
 // stride over the tiles, so the blocks in flight work on neighbouring tiles (slots of
 // neighbouring pixels, whose hash-grid lines the L2 then shares); a block's tiles only
 // move forward, so it loads each scene's network at most once. The slots of scenes
-// without a network stay as the clear left them (unguided).
+// without a network were set unguided by k_scene_scatter.
 __global__ __launch_bounds__(kMlpBlock) void k_mlp_infer_grouped(PathState ps, int parity,
                                                                  const MlpView* __restrict__ guides, uint32_t n_scenes,
                                                                  const uint32_t* __restrict__ off,
@@ -1072,22 +1076,9 @@ hipError_t launch_mlp_infer_grouped(const PathState& ps, int parity, const MlpVi
     hipLaunchKernelGGL(k_scene_offsets, dim3(1), dim3(64), 0, stream, n_scenes, gb.cnt, gb.off, gb.cur, gb.n_tiles);
     hipLaunchKernelGGL(k_perm_clear, dim3(grid), dim3(256), 0, stream, gb.n_tiles, gb.perm);
     hipLaunchKernelGGL(k_scene_scatter, dim3(grid), dim3(256), 0, stream, ps, parity, guides, n_scenes, gb.cur,
-                       gb.perm);
+                       gb.perm, mix);
     hipLaunchKernelGGL(k_mlp_infer_grouped, dim3(grid), dim3(kMlpBlock), 0, stream, ps, parity, guides, n_scenes,
                        gb.off, gb.n_tiles, gb.perm, mix);
-    return hipGetLastError();
-}
-
-// c = 0 (unguided) for every slot of the current queue: batches clear before the
-// per-scene inference launches, so slots of scenes without a network stay unguided.
-__global__ void k_mix_clear(const uint32_t* counters, int parity, float4* mix) {
-    const uint32_t n = counters[parity];
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-        mix[(size_t)i * kMixF4 + 12] = make_float4(0.f, 0.f, 0.f, 0.f);
-}
-
-hipError_t launch_mix_clear(const PathState& ps, int parity, float4* mix, uint32_t grid, hipStream_t stream) {
-    hipLaunchKernelGGL(k_mix_clear, dim3(grid), dim3(256), 0, stream, ps.counters, parity, mix);
     return hipGetLastError();
 }
 
