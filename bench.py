@@ -146,7 +146,8 @@ def time_staying_below(curve, e, col=2):
 
 
 def time_to_relmse(pg, dev, sc, data, max_depth, ref_spp, cap_s, log, target=0.01, env_visibility=1, repeats=1,
-                   with_pss=False, train_every=1, train_steps=1, record_samples=1, guide_depth=0):
+                   with_pss=False, train_every=1, train_steps=1, record_samples=1, guide_depth=0,
+                   train_batch=1 << 18):
     """Time-to-relMSE study: unguided vs quadtree guide vs learned guide. Besides the
     time to relMSE <= target (null when a method does not reach it within cap_s), the
     learned guide is compared with each baseline at that baseline's own final relMSE:
@@ -195,7 +196,7 @@ def time_to_relmse(pg, dev, sc, data, max_depth, ref_spp, cap_s, log, target=0.0
             cache_setup(pf, 9 + 10 * r)
 
         def ml_setup():
-            dev.mlp_init(seed=7 + r)
+            dev.mlp_init(seed=7 + r, batch=train_batch)
             ml.clear()
             ml.reset_stats()
 
@@ -310,7 +311,7 @@ def time_to_relmse(pg, dev, sc, data, max_depth, ref_spp, cap_s, log, target=0.0
     # network FLOPs over the used layer sizes (80 -> 64 -> 64 -> 64 -> 41): 2 * sum(in * out)
     # per query forward; a training sample adds the input and weight gradients (3x)
     fwd = 2 * (80 * 64 + 64 * 64 + 64 * 64 + 64 * 41)
-    batch = 1 << 18
+    batch = train_batch
     train_ms = float(info["train_ms"])  # GPU time per Adam step
     mlp = {"forward_flop_per_query": fwd, "train_flop_per_sample": 3 * fwd, "batch": batch,
            "train_ms_per_step": round(train_ms, 3),
@@ -338,7 +339,7 @@ def time_to_relmse(pg, dev, sc, data, max_depth, ref_spp, cap_s, log, target=0.0
                        "pss_approx": "as pss with approximate neighbours (16 nearest of the 64 records under a "
                                      "greedy point-BVH descent)",
                        "learned": f"online: a {record_samples}-SPP guided record pass and {train_steps} Adam step(s) "
-                                  f"of 2^18 samples every {train_every} SPP, guided passes between"}.items()
+                                  f"of {train_batch} samples every {train_every} SPP, guided passes between"}.items()
                        if k in res},
            "guide_depth": (f"the guided methods guide the first {guide_depth} bounces and sample the BSDF alone "
                            "deeper" if guide_depth else "the guided methods guide every bounce"),
@@ -524,6 +525,8 @@ def main() -> int:
                     help="learned guide: a record pass + training steps every N SPP, guided passes between")
     ap.add_argument("--ttr-train-steps", type=int, default=int(os.environ.get("PG_BENCH_TTR_TRAIN_STEPS", "8")),
                     help="learned guide: Adam steps after each record pass")
+    ap.add_argument("--ttr-train-batch", type=int, default=int(os.environ.get("PG_BENCH_TTR_TRAIN_BATCH", "131072")),
+                    help="learned guide: training samples per Adam step")
     ap.add_argument("--ttr-record-samples", type=int, default=int(os.environ.get("PG_BENCH_TTR_RECORD_SAMPLES", "2")),
                     help="learned guide: samples per pixel of each record pass (traced as one wavefront pass)")
     ap.add_argument("--ttr-guide-depth", type=int, default=int(os.environ.get("PG_BENCH_TTR_GUIDE_DEPTH", "2")),
@@ -676,7 +679,7 @@ def main() -> int:
                                lambda m: print(m, file=sys.stderr, flush=True), env_visibility=a.ttr_env_visibility,
                                repeats=a.ttr_repeats, with_pss=a.ttr_pss, train_every=a.ttr_train_every,
                                train_steps=a.ttr_train_steps, record_samples=a.ttr_record_samples,
-                               guide_depth=a.ttr_guide_depth)
+                               guide_depth=a.ttr_guide_depth, train_batch=a.ttr_train_batch)
             ttr[f"config{tc}"] = {"workload": f"config {tc}: {tdata.n_tris} tris, {tdata.width}x{tdata.height}, "
                                               f"max depth {max_depth}", **r}
             ps = r["comparison"]["per_sample"]
